@@ -1,0 +1,225 @@
+"""ctypes bindings of the two native libraries.
+
+* ``libgauxc_amd.so``: the GauXC-compatible C API (include/gauxc/c/gauxc.h).
+* ``libgauxc_amd_device.so``: the device C ABI (include/gauxc_amd/device.h),
+  used directly by kernel-level tests and by the benchmark for timings.
+
+Loading fails loudly when the libraries have not been built: there is no
+Python or CPU fallback for any product entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HOST_LIB = os.path.join(LIB_DIR, "libgauxc_amd.so")
+DEVICE_LIB = os.path.join(LIB_DIR, "libgauxc_amd_device.so")
+
+c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+P_DBL = ctypes.POINTER(ctypes.c_double)
+P_I32 = ctypes.POINTER(ctypes.c_int32)
+P_I64 = ctypes.POINTER(ctypes.c_int64)
+
+
+class GauXCStatus(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int), ("message", ctypes.c_void_p)]
+
+
+class GauXCHandle(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("ptr", ctypes.c_void_p)]
+
+
+class GauXCAtom(ctypes.Structure):
+    _fields_ = [("Z", c_i64), ("x", c_dbl), ("y", c_dbl), ("z", c_dbl)]
+
+
+class GauXCShell(ctypes.Structure):
+    _fields_ = [("l", c_i32), ("pure", ctypes.c_bool), ("nprim", c_i32),
+                ("exponents", c_dbl * 32), ("coefficients", c_dbl * 32),
+                ("origin", c_dbl * 3), ("shell_tolerance", c_dbl)]
+
+
+class GauXCMolecularWeightsSettings(ctypes.Structure):
+    _fields_ = [("weight_alg", ctypes.c_int), ("becke_size_adjustment", ctypes.c_bool)]
+
+
+class XcqShell(ctypes.Structure):
+    _fields_ = [("nprim", c_i32), ("l", c_i32), ("pure", c_i32), ("pad_", c_i32),
+                ("alpha", c_dbl * 32), ("coeff", c_dbl * 32), ("origin", c_dbl * 3),
+                ("cutoff_radius", c_dbl)]
+
+
+class XcqTask(ctypes.Structure):
+    _fields_ = [("iparent", c_i32), ("npts", c_i32), ("nshells", c_i32), ("nbe", c_i32),
+                ("point_offset", c_i64), ("shell_offset", c_i64), ("dist_nearest", c_dbl)]
+
+
+# Enumerations (include/gauxc/c/gauxc.h, GauXC order)
+RADIAL_MURA_KNOWLES = 1
+GRID = {"FineGrid": 0, "UltraFineGrid": 1, "SuperFineGrid": 2, "GM3": 3, "GM5": 4}
+PRUNING_UNPRUNED = 0
+WEIGHT_SSF = 2
+EXEC_HOST, EXEC_DEVICE = 0, 1
+XCQ_FUNC = {"SVWN5": 1, "PBE": 2, "PBE0": 3}
+XCQ_PREC = {"fp64": 0, "mixed": 1}
+
+# Every symbol each header declares (checked by the CPU test-suite).
+HOST_SYMBOLS = [
+    "gauxc_status_delete", "gauxc_molecule_new", "gauxc_molecule_new_from_atoms",
+    "gauxc_molecule_delete", "gauxc_molecule_natoms", "gauxc_basisset_new",
+    "gauxc_basisset_new_from_shells", "gauxc_basisset_delete", "gauxc_basisset_nbf",
+    "gauxc_basisset_get_shells", "gauxc_molgrid_new_default", "gauxc_molgrid_delete",
+    "gauxc_runtime_environment_new", "gauxc_device_runtime_environment_new",
+    "gauxc_runtime_environment_delete", "gauxc_load_balancer_factory_new",
+    "gauxc_load_balancer_factory_delete", "gauxc_load_balancer_factory_get_instance",
+    "gauxc_load_balancer_delete", "gauxc_load_balancer_task_counts",
+    "gauxc_load_balancer_export_tasks", "gauxc_load_balancer_weights_modified",
+    "gauxc_molecular_weights_factory_new", "gauxc_molecular_weights_factory_delete",
+    "gauxc_molecular_weights_factory_get_instance", "gauxc_molecular_weights_modify_weights",
+    "gauxc_molecular_weights_delete", "gauxc_functional_from_string", "gauxc_functional_delete",
+    "gauxc_integrator_new", "gauxc_integrator_delete", "gauxc_integrator_integrate_den",
+    "gauxc_integrator_eval_exc_rks", "gauxc_integrator_eval_exc_vxc_rks",
+    "gauxc_integrator_device_context",
+]
+DEVICE_SYMBOLS = [
+    "xcq_device_count", "xcq_last_error", "xcq_context_create", "xcq_context_destroy",
+    "xcq_synchronize", "xcq_upload_basis", "xcq_upload_tasks", "xcq_upload_density",
+    "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_download_weights", "xcq_exc_vxc",
+    "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
+    "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
+]
+
+_host = None
+_dev = None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{os.path.basename(path)} is not built; run `python -m gauxc_amd.build` "
+            "(the HIP path has no fallback)")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+def device_lib():
+    global _dev
+    if _dev is None:
+        lib = _load(DEVICE_LIB)
+        H = ctypes.c_void_p
+        sig = {
+            "xcq_device_count": (ctypes.c_int, [P_I32]),
+            "xcq_last_error": (ctypes.c_char_p, []),
+            "xcq_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, c_dbl,
+                                                  ctypes.POINTER(H)]),
+            "xcq_context_destroy": (ctypes.c_int, [H]),
+            "xcq_synchronize": (ctypes.c_int, [H]),
+            "xcq_upload_basis": (ctypes.c_int, [H, c_i32, c_vp]),
+            "xcq_upload_tasks": (ctypes.c_int, [H, c_i32, c_vp, c_i64, c_vp, c_vp, c_i64,
+                                                c_vp]),
+            "xcq_upload_density": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
+            "xcq_ssf_weights": (ctypes.c_int, [ctypes.c_int, c_i32, c_vp, c_i64, c_vp, c_vp,
+                                               c_vp]),
+            "xcq_ssf_weights_tasks": (ctypes.c_int, [H, c_i32, c_vp]),
+            "xcq_download_weights": (ctypes.c_int, [H, c_vp]),
+            "xcq_exc_vxc": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "xcq_integrate_den": (ctypes.c_int, [H, c_vp]),
+            "xcq_download_results": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_i64]),
+            "xcq_task_collocation": (ctypes.c_int, [H, c_i32, ctypes.c_int, ctypes.c_int,
+                                                    c_vp]),
+            "xcq_eval_functional": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_i64, c_vp,
+                                                   c_vp, c_vp, c_vp, c_vp]),
+            "xcq_stage_times": (ctypes.c_int, [H, c_vp, ctypes.c_int]),
+            "xcq_work_counters": (ctypes.c_int, [H, c_vp, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
+        _dev = lib
+    return _dev
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        device_lib()
+        lib = _load(HOST_LIB)
+        S = ctypes.POINTER(GauXCStatus)
+        Hd = GauXCHandle
+        PH = ctypes.POINTER(GauXCHandle)
+        sig = {
+            "gauxc_status_delete": (None, [S]),
+            "gauxc_molecule_new_from_atoms": (Hd, [S, c_vp, ctypes.c_size_t]),
+            "gauxc_molecule_delete": (None, [S, PH]),
+            "gauxc_molecule_natoms": (ctypes.c_size_t, [S, Hd]),
+            "gauxc_basisset_new_from_shells": (Hd, [S, c_vp, ctypes.c_size_t, ctypes.c_bool]),
+            "gauxc_basisset_delete": (None, [S, PH]),
+            "gauxc_basisset_nbf": (c_i32, [S, Hd]),
+            "gauxc_basisset_get_shells": (None, [S, Hd, c_vp, c_vp]),
+            "gauxc_molgrid_new_default": (Hd, [S, Hd, ctypes.c_int, c_i64, ctypes.c_int,
+                                               ctypes.c_int]),
+            "gauxc_molgrid_delete": (None, [S, PH]),
+            "gauxc_runtime_environment_new": (Hd, [S]),
+            "gauxc_device_runtime_environment_new": (Hd, [S, c_dbl]),
+            "gauxc_runtime_environment_delete": (None, [S, PH]),
+            "gauxc_load_balancer_factory_new": (Hd, [S, ctypes.c_int, ctypes.c_char_p]),
+            "gauxc_load_balancer_factory_delete": (None, [S, PH]),
+            "gauxc_load_balancer_factory_get_instance": (Hd, [S, Hd, Hd, Hd, Hd, Hd]),
+            "gauxc_load_balancer_delete": (None, [S, PH]),
+            "gauxc_load_balancer_task_counts": (None, [S, Hd, c_vp]),
+            "gauxc_load_balancer_export_tasks": (None, [S, Hd, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                        c_vp, c_vp, c_vp]),
+            "gauxc_load_balancer_weights_modified": (ctypes.c_bool, [S, Hd]),
+            "gauxc_molecular_weights_factory_new": (Hd, [S, ctypes.c_int, ctypes.c_char_p,
+                                                         GauXCMolecularWeightsSettings]),
+            "gauxc_molecular_weights_factory_delete": (None, [S, PH]),
+            "gauxc_molecular_weights_factory_get_instance": (Hd, [S, Hd]),
+            "gauxc_molecular_weights_modify_weights": (None, [S, Hd, Hd]),
+            "gauxc_molecular_weights_delete": (None, [S, PH]),
+            "gauxc_functional_from_string": (Hd, [S, ctypes.c_char_p, ctypes.c_bool]),
+            "gauxc_functional_delete": (None, [S, PH]),
+            "gauxc_integrator_new": (Hd, [S, Hd, Hd, ctypes.c_int, ctypes.c_char_p,
+                                          ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+            "gauxc_integrator_delete": (None, [S, PH]),
+            "gauxc_integrator_integrate_den": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp]),
+            "gauxc_integrator_eval_exc_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp]),
+            "gauxc_integrator_eval_exc_vxc_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp,
+                                                         c_vp, c_i64]),
+            "gauxc_integrator_device_context": (c_vp, [S, Hd]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
+        _host = lib
+    return _host
+
+
+class GauXCError(RuntimeError):
+    pass
+
+
+class Status:
+    """Wraps a GauXCStatus; raise_if() turns a non-zero code into GauXCError."""
+
+    def __init__(self):
+        self.s = GauXCStatus(0, None)
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.s)
+
+    def raise_if(self):
+        if self.s.code != 0:
+            msg = ctypes.string_at(self.s.message).decode() if self.s.message else "error"
+            host_lib().gauxc_status_delete(self.ptr)
+            raise GauXCError(msg)
+
+
+def xcq_check(rc, what="device call"):
+    if rc != 0:
+        raise GauXCError(f"{what} failed ({rc}): {device_lib().xcq_last_error().decode()}")
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    rc = device_lib().xcq_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0

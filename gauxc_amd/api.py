@@ -1,0 +1,435 @@
+"""Python mirror of GauXC's eval_exc_vxc API over the C API.
+
+Usage mirrors GauXC's C++ test driver (tests/xc_integrator.cxx:163-199):
+
+    mol   = Molecule(Z, xyz)
+    basis = BasisSet.from_arrays(...)            # normalises like Shell<double>
+    mg    = MolGridFactory.create_default_molgrid(mol, "Unpruned", 512,
+                                                   "MuraKnowles", "UltraFineGrid")
+    lb    = LoadBalancerFactory("Host", "Default").get_instance(rt, mol, mg, basis)
+    MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    integ = XCIntegratorFactory("Device", "Replicated", "Default", "Default",
+                                "Default").get_instance(Functional("PBE0"), lb)
+    exc, vxc = integ.eval_exc_vxc(P)
+
+Every call goes through the native libraries; nothing here computes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+def _enum_exec(ex):
+    if isinstance(ex, int):
+        return ex
+    return {"host": N.EXEC_HOST, "device": N.EXEC_DEVICE}[ex.lower()]
+
+
+class _Handle:
+    _delete = None
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.ptr:
+                st = N.Status()
+                getattr(N.host_lib(), self._delete)(st.ptr, ctypes.byref(self.h))
+        except Exception:
+            pass
+
+
+class Molecule(_Handle):
+    _delete = "gauxc_molecule_delete"
+
+    def __init__(self, Z, xyz):
+        Z = np.asarray(Z, dtype=np.int64)
+        xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+        atoms = (N.GauXCAtom * len(Z))()
+        for i, (z, r) in enumerate(zip(Z, xyz)):
+            atoms[i] = N.GauXCAtom(int(z), float(r[0]), float(r[1]), float(r[2]))
+        st = N.Status()
+        h = N.host_lib().gauxc_molecule_new_from_atoms(st.ptr, atoms, len(Z))
+        st.raise_if()
+        super().__init__(h)
+        self.Z = Z
+        self.xyz = xyz
+
+    def natoms(self):
+        return len(self.Z)
+
+
+class BasisSet(_Handle):
+    _delete = "gauxc_basisset_delete"
+
+    def __init__(self, l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=None):
+        n = len(l)
+        shells = (N.GauXCShell * n)()
+        for i in range(n):
+            a = np.asarray(alpha[i], dtype=np.float64)
+            c = np.asarray(coeff[i], dtype=np.float64)
+            s = shells[i]
+            s.l = int(l[i])
+            s.pure = bool(pure[i])
+            s.nprim = len(a)
+            for k in range(len(a)):
+                s.exponents[k] = a[k]
+                s.coefficients[k] = c[k]
+            for k in range(3):
+                s.origin[k] = float(origin[i][k])
+            s.shell_tolerance = 0.0 if shell_tolerance is None else float(shell_tolerance)
+        st = N.Status()
+        h = N.host_lib().gauxc_basisset_new_from_shells(st.ptr, shells, n, bool(normalize))
+        st.raise_if()
+        super().__init__(h)
+        self.nshells = n
+
+    @classmethod
+    def from_arrays(cls, nprim, l, pure, alpha, coeff, origin, normalize=True,
+                    shell_tolerance=None):
+        al = [np.asarray(alpha[i][:nprim[i]]) for i in range(len(l))]
+        cl = [np.asarray(coeff[i][:nprim[i]]) for i in range(len(l))]
+        return cls(l, pure, al, cl, origin, normalize, shell_tolerance)
+
+    def nbf(self):
+        st = N.Status()
+        n = N.host_lib().gauxc_basisset_nbf(st.ptr, self.h)
+        st.raise_if()
+        return int(n)
+
+    def shells(self):
+        """Normalised shells as (struct array of GauXCShell, cutoff radii)."""
+        arr = (N.GauXCShell * self.nshells)()
+        rad = np.zeros(self.nshells)
+        st = N.Status()
+        N.host_lib().gauxc_basisset_get_shells(st.ptr, self.h, arr, rad.ctypes.data)
+        st.raise_if()
+        return arr, rad
+
+    def xcq_shells(self):
+        arr, rad = self.shells()
+        out = (N.XcqShell * self.nshells)()
+        for i, s in enumerate(arr):
+            d = out[i]
+            d.nprim, d.l, d.pure = s.nprim, s.l, int(s.pure)
+            for k in range(s.nprim):
+                d.alpha[k] = s.exponents[k]
+                d.coeff[k] = s.coefficients[k]
+            for k in range(3):
+                d.origin[k] = s.origin[k]
+            d.cutoff_radius = rad[i]
+        return out
+
+
+class MolGrid(_Handle):
+    _delete = "gauxc_molgrid_delete"
+
+
+class MolGridFactory:
+    @staticmethod
+    def create_default_molgrid(mol, pruning="Unpruned", batch_size=512, radial="MuraKnowles",
+                               grid="UltraFineGrid"):
+        if pruning != "Unpruned":
+            raise N.GauXCError("Only unpruned atomic grids are provided")
+        rq = {"Becke": 0, "MuraKnowles": 1, "MurrayHandyLaming": 2, "TreutlerAhlrichs": 3}[radial]
+        st = N.Status()
+        h = N.host_lib().gauxc_molgrid_new_default(st.ptr, mol.h, N.PRUNING_UNPRUNED,
+                                                   int(batch_size), rq, N.GRID[grid])
+        st.raise_if()
+        return MolGrid(h)
+
+
+class RuntimeEnvironment(_Handle):
+    _delete = "gauxc_runtime_environment_delete"
+
+    def __init__(self, fill_fraction=0.9):
+        st = N.Status()
+        h = N.host_lib().gauxc_device_runtime_environment_new(st.ptr, float(fill_fraction))
+        st.raise_if()
+        super().__init__(h)
+
+
+DeviceRuntimeEnvironment = RuntimeEnvironment
+
+
+class LoadBalancer(_Handle):
+    _delete = "gauxc_load_balancer_delete"
+
+    def task_counts(self):
+        c = np.zeros(3, dtype=np.int64)
+        st = N.Status()
+        N.host_lib().gauxc_load_balancer_task_counts(st.ptr, self.h, c.ctypes.data)
+        st.raise_if()
+        return int(c[0]), int(c[1]), int(c[2])
+
+    def get_tasks(self):
+        """Tasks as flat numpy arrays (GauXC LoadBalancer::get_tasks())."""
+        nt, npt, nsh = self.task_counts()
+        t = dict(iparent=np.zeros(nt, np.int32), npts=np.zeros(nt, np.int32),
+                 nshells=np.zeros(nt, np.int32), nbe=np.zeros(nt, np.int32),
+                 dist_nearest=np.zeros(nt), points=np.zeros((npt, 3)), weights=np.zeros(npt),
+                 shell_list=np.zeros(nsh, np.int32))
+        st = N.Status()
+        N.host_lib().gauxc_load_balancer_export_tasks(
+            st.ptr, self.h, *[t[k].ctypes.data for k in
+                              ("iparent", "npts", "nshells", "nbe", "dist_nearest", "points",
+                               "weights", "shell_list")])
+        st.raise_if()
+        t["point_offset"] = np.concatenate([[0], np.cumsum(t["npts"])[:-1]]).astype(np.int64)
+        t["shell_offset"] = np.concatenate([[0], np.cumsum(t["nshells"])[:-1]]).astype(np.int64)
+        return t
+
+    def weights_modified(self):
+        st = N.Status()
+        r = N.host_lib().gauxc_load_balancer_weights_modified(st.ptr, self.h)
+        st.raise_if()
+        return bool(r)
+
+
+class LoadBalancerFactory(_Handle):
+    _delete = "gauxc_load_balancer_factory_delete"
+
+    def __init__(self, ex="Host", kernel="Default"):
+        st = N.Status()
+        h = N.host_lib().gauxc_load_balancer_factory_new(st.ptr, _enum_exec(ex), kernel.encode())
+        st.raise_if()
+        super().__init__(h)
+
+    def get_instance(self, rt, mol, mg, basis):
+        st = N.Status()
+        h = N.host_lib().gauxc_load_balancer_factory_get_instance(st.ptr, self.h, rt.h, mol.h,
+                                                                  mg.h, basis.h)
+        st.raise_if()
+        lb = LoadBalancer(h)
+        lb._keep = (mol, mg, basis)
+        return lb
+
+
+class MolecularWeights(_Handle):
+    _delete = "gauxc_molecular_weights_delete"
+
+    def modify_weights(self, lb):
+        st = N.Status()
+        N.host_lib().gauxc_molecular_weights_modify_weights(st.ptr, self.h, lb.h)
+        st.raise_if()
+
+
+class MolecularWeightsFactory(_Handle):
+    _delete = "gauxc_molecular_weights_factory_delete"
+
+    def __init__(self, ex="Device", lwd="Default", weight_alg="SSF"):
+        alg = {"NOTPARTITIONED": 0, "Becke": 1, "SSF": 2, "LKO": 3}[weight_alg]
+        st = N.Status()
+        h = N.host_lib().gauxc_molecular_weights_factory_new(
+            st.ptr, _enum_exec(ex), lwd.encode(), N.GauXCMolecularWeightsSettings(alg, False))
+        st.raise_if()
+        super().__init__(h)
+
+    def get_instance(self):
+        st = N.Status()
+        h = N.host_lib().gauxc_molecular_weights_factory_get_instance(st.ptr, self.h)
+        st.raise_if()
+        return MolecularWeights(h)
+
+
+class Functional(_Handle):
+    _delete = "gauxc_functional_delete"
+
+    def __init__(self, name, polarized=False):
+        st = N.Status()
+        h = N.host_lib().gauxc_functional_from_string(st.ptr, name.encode(), bool(polarized))
+        st.raise_if()
+        super().__init__(h)
+        self.name = name
+
+
+class XCIntegrator(_Handle):
+    _delete = "gauxc_integrator_delete"
+
+    def _mat(self, P):
+        P = np.asfortranarray(np.asarray(P, dtype=np.float64))
+        if P.ndim != 2:
+            raise N.GauXCError("P must be a matrix")
+        return P
+
+    def eval_exc_vxc(self, P):
+        P = self._mat(P)
+        m, n = P.shape
+        vxc = np.zeros((m, n), order="F")
+        exc = ctypes.c_double(0.)
+        st = N.Status()
+        N.host_lib().gauxc_integrator_eval_exc_vxc_rks(st.ptr, self.h, m, n, P.ctypes.data, m,
+                                                       ctypes.addressof(exc), vxc.ctypes.data,
+                                                       m)
+        st.raise_if()
+        return exc.value, vxc
+
+    def eval_exc(self, P):
+        P = self._mat(P)
+        m, n = P.shape
+        exc = ctypes.c_double(0.)
+        st = N.Status()
+        N.host_lib().gauxc_integrator_eval_exc_rks(st.ptr, self.h, m, n, P.ctypes.data, m,
+                                                   ctypes.addressof(exc))
+        st.raise_if()
+        return exc.value
+
+    def integrate_den(self, P):
+        P = self._mat(P)
+        m, n = P.shape
+        den = ctypes.c_double(0.)
+        st = N.Status()
+        N.host_lib().gauxc_integrator_integrate_den(st.ptr, self.h, m, n, P.ctypes.data, m,
+                                                    ctypes.addressof(den))
+        st.raise_if()
+        return den.value
+
+    def device_context(self):
+        st = N.Status()
+        p = N.host_lib().gauxc_integrator_device_context(st.ptr, self.h)
+        st.raise_if()
+        return p
+
+
+class XCIntegratorFactory:
+    def __init__(self, ex="Device", input_type="Replicated", integrator_kernel="Default",
+                 lwd_kernel="Default", reduction_kernel="Default"):
+        self.args = (_enum_exec(ex), input_type, integrator_kernel, lwd_kernel, reduction_kernel)
+
+    def get_instance(self, func, lb):
+        ex, it, ik, lk, rk = self.args
+        st = N.Status()
+        h = N.host_lib().gauxc_integrator_new(st.ptr, func.h, lb.h, ex, it.encode(), ik.encode(),
+                                              lk.encode(), rk.encode())
+        st.raise_if()
+        integ = XCIntegrator(h)
+        integ._keep = (func, lb)
+        return integ
+
+
+class DeviceContext:
+    """Direct handle on the device C ABI (include/gauxc_amd/device.h)."""
+
+    def __init__(self, device=0, arena_bytes=0, fill_fraction=0.9):
+        self.lib = N.device_lib()
+        h = ctypes.c_void_p()
+        N.xcq_check(self.lib.xcq_context_create(device, arena_bytes, fill_fraction,
+                                                ctypes.byref(h)), "xcq_context_create")
+        self.h = h
+        self.nbf = None
+        self.tasks = None
+
+    def close(self):
+        if self.h:
+            self.lib.xcq_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_basis(self, xcq_shells):
+        N.xcq_check(self.lib.xcq_upload_basis(self.h, len(xcq_shells), xcq_shells),
+                    "xcq_upload_basis")
+
+    def upload_tasks(self, t):
+        nt = len(t["npts"])
+        desc = (N.XcqTask * nt)()
+        for i in range(nt):
+            desc[i] = N.XcqTask(int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
+                                int(t["nbe"][i]), int(t["point_offset"][i]),
+                                int(t["shell_offset"][i]), float(t["dist_nearest"][i]))
+        pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+        w = np.ascontiguousarray(t["weights"], dtype=np.float64)
+        sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+        N.xcq_check(self.lib.xcq_upload_tasks(self.h, nt, desc, len(w), pts.ctypes.data,
+                                              w.ctypes.data, len(sl), sl.ctypes.data),
+                    "xcq_upload_tasks")
+        self.tasks = t
+
+    def upload_density(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        self.nbf = P.shape[0]
+        N.xcq_check(self.lib.xcq_upload_density(self.h, P.shape[0], P.ctypes.data, P.shape[0]),
+                    "xcq_upload_density")
+
+    def ssf_weights(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        N.xcq_check(self.lib.xcq_ssf_weights_tasks(self.h, len(xyz), xyz.ctypes.data),
+                    "xcq_ssf_weights_tasks")
+
+    def weights(self):
+        w = np.zeros(len(self.tasks["weights"]))
+        N.xcq_check(self.lib.xcq_download_weights(self.h, w.ctypes.data), "xcq_download_weights")
+        return w
+
+    def exc_vxc(self, func="PBE", precision="fp64", want_vxc=True):
+        N.xcq_check(self.lib.xcq_exc_vxc(self.h, N.XCQ_FUNC[func], N.XCQ_PREC[precision],
+                                         int(bool(want_vxc))), "xcq_exc_vxc")
+
+    def synchronize(self):
+        N.xcq_check(self.lib.xcq_synchronize(self.h), "xcq_synchronize")
+
+    def results(self, want_vxc=True):
+        exc = ctypes.c_double(0.)
+        nel = ctypes.c_double(0.)
+        vxc = np.zeros((self.nbf, self.nbf), order="F") if want_vxc else None
+        N.xcq_check(self.lib.xcq_download_results(
+            self.h, ctypes.addressof(exc), ctypes.addressof(nel),
+            vxc.ctypes.data if want_vxc else None, self.nbf), "xcq_download_results")
+        return exc.value, nel.value, vxc
+
+    def integrate_den(self):
+        nel = ctypes.c_double(0.)
+        N.xcq_check(self.lib.xcq_integrate_den(self.h, ctypes.addressof(nel)),
+                    "xcq_integrate_den")
+        return nel.value
+
+    def collocation(self, task, deriv=True, precision="fp64"):
+        nbe = int(self.tasks["nbe"][task])
+        npts = int(self.tasks["npts"][task])
+        ncomp = 4 if deriv else 1
+        out = np.zeros((ncomp, nbe, npts))
+        N.xcq_check(self.lib.xcq_task_collocation(self.h, task, int(deriv), N.XCQ_PREC[precision],
+                                                  out.ctypes.data), "xcq_task_collocation")
+        return out
+
+    def stage_times(self):
+        ms = np.zeros(7)
+        self.lib.xcq_stage_times(self.h, ms.ctypes.data, 7)
+        return ms
+
+    def work_counters(self):
+        c = np.zeros(4, dtype=np.int64)
+        self.lib.xcq_work_counters(self.h, c.ctypes.data, 4)
+        return c
+
+
+def eval_functional_device(func, rho, sigma=None, device=0):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    n = len(rho)
+    eps = np.zeros(n)
+    vr = np.zeros(n)
+    gga = func != "SVWN5"
+    vs = np.zeros(n) if gga else None
+    sig = np.ascontiguousarray(sigma, dtype=np.float64) if gga else None
+    N.xcq_check(N.device_lib().xcq_eval_functional(
+        device, N.XCQ_FUNC[func], n, rho.ctypes.data, sig.ctypes.data if gga else None,
+        eps.ctypes.data, vr.ctypes.data, vs.ctypes.data if gga else None), "xcq_eval_functional")
+    return eps, vr, vs
+
+
+def ssf_weights_device(xyz, points, parent, weights, device=0):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    par = np.ascontiguousarray(parent, dtype=np.int32)
+    w = np.array(weights, dtype=np.float64, copy=True)
+    N.xcq_check(N.device_lib().xcq_ssf_weights(device, len(xyz), xyz.ctypes.data, len(w),
+                                               pts.ctypes.data, par.ctypes.data, w.ctypes.data),
+                "xcq_ssf_weights")
+    return w

@@ -1,0 +1,244 @@
+// Point-wise kernels of the XC path on gfx950: SSF partition weights, the
+// functional with weight folding and EXC/NEL block sums, the deterministic
+// final reduction, and VXC symmetrisation.
+#include "internal.hpp"
+#include "functional.hpp"
+
+
+namespace xcq {
+
+// ---------------------------------------------------------------------------
+// SSF weights, one thread per point.
+// GauXC semantics: host reference weights.cxx:117-237 (Frisch polynomial,
+// a = 0.64, points closer to the parent than 0.5(1-a) R_nearest keep their
+// weight).  The partition function of atom A is the product
+// P_A = prod_{B != A} s(mu_AB), s(mu) = (1 - f(mu/a))/2 inside |mu| < a,
+// 0 above, 1 below -- the same quantity the reference accumulates pairwise.
+// Only atoms with d_A < (1+a)/(1-a) d_min can have P_A > 0 (their
+// mu against the nearest atom is >= a otherwise), so the candidate set is
+// filtered by that bound, slightly widened; exact zeros are still decided
+// by the mu test itself.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ssf_s(double mu) {
+  constexpr double a = 0.64;
+  if (mu <= -a) return 1.0;
+  if (mu >= a) return 0.0;
+  const double s = mu / a, s2 = s * s, s3 = s * s2, s5 = s3 * s2, s7 = s5 * s2;
+  const double f = (35. * (s - s3) + 21. * s5 - 5. * s7) / 16.;
+  return 0.5 * (1.0 - f);
+}
+
+__global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __restrict__ xyz,
+                                             const double* __restrict__ rab,
+                                             const double* __restrict__ dnear, int64_t npts,
+                                             const double* __restrict__ px,
+                                             const double* __restrict__ py,
+                                             const double* __restrict__ pz,
+                                             const int32_t* __restrict__ parent,
+                                             double* __restrict__ w) {
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= npts) return;
+  constexpr double a = 0.64;
+  const double x = px[ip], y = py[ip], z = pz[ip];
+  const int par = parent[ip];
+  auto dist = [&](int A) {
+    const double dx = x - xyz[3 * A], dy = y - xyz[3 * A + 1], dz = z - xyz[3 * A + 2];
+    return sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  const double dpar = dist(par);
+  if (dpar < 0.5 * (1.0 - a) * dnear[par]) return;
+  double dmin = dpar;
+  for (int A = 0; A < natoms; ++A) dmin = fmin(dmin, dist(A));
+  const double bound = (1.0 + a) / (1.0 - a) * dmin * (1.0 + 1e-10) + 1e-12;
+  double sum = 0.0, ppar = 0.0;
+  for (int A = 0; A < natoms; ++A) {
+    const double dA = dist(A);
+    if (dA > bound) continue;
+    double PA = 1.0;
+    for (int B = 0; B < natoms && PA > 0.0; ++B) {
+      if (B == A) continue;
+      const double mu = (dA - dist(B)) / rab[(size_t)A + (size_t)B * natoms];
+      PA *= ssf_s(mu);
+    }
+    sum += PA;
+    if (A == par) ppar = PA;
+  }
+  w[ip] *= ppar / sum;
+}
+
+int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* rab,
+               const double* dnear, int64_t npts, const double* px, const double* py,
+               const double* pz, const int32_t* parent, double* w) {
+  if (npts == 0) return XCQ_SUCCESS;
+  const int64_t nb = (npts + 255) / 256;
+  hipLaunchKernelGGL(k_ssf, dim3((unsigned)nb), dim3(256), 0, s, natoms, xyz, rab, dnear, npts,
+                     px, py, pz, parent, w);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Functional + weight folding per point block (64 points, one wave).
+// Density partials written by the X/density kernel are summed in a fixed
+// order; grad rho = 2 sum dB X (eval_uvvar_gga_rks,
+// reference_local_host_work_driver.cxx:242-268).  Outputs the Z-matrix
+// coefficients  a = w vrho / 2,  c = 2 w vsigma grad rho
+// (eval_zmat_gga_vxc_rks, :678-713) and per-block sums of w eps rho and
+// w rho (host_integrator_exc_vxc.hpp:452-502).  func = 0: density only.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_functional(const WorkItem* __restrict__ pblk,
+                                                   const DevTask* __restrict__ tasks,
+                                                   const double* __restrict__ part,
+                                                   const double* __restrict__ weights,
+                                                   int func, int gga, int64_t nq,
+                                                   double* __restrict__ zcoef,
+                                                   double* __restrict__ red) {
+  const WorkItem wi = pblk[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int lane = threadIdx.x;
+  const int p = wi.a * kPtBlock + lane;
+  const int64_t ld = t.npts_pad;
+  const double* pp = part + t.part_off;
+  double rho = 0., gx = 0., gy = 0., gz = 0.;
+  const int nslot = 2 * t.ntile;
+  for (int sl = 0; sl < nslot; ++sl) {
+    const double* q = pp + (int64_t)sl * 4 * ld;
+    rho += q[p];
+    if (gga) {
+      gx += q[ld + p];
+      gy += q[2 * ld + p];
+      gz += q[3 * ld + p];
+    }
+  }
+  gx *= 2.0;
+  gy *= 2.0;
+  gz *= 2.0;
+  double e = 0., n = 0., za = 0., zx = 0., zy = 0., zz = 0.;
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    n = w * rho;
+    if (func != 0) {
+      const double sigma = gx * gx + gy * gy + gz * gz;
+      const XCOut o = dev_eval_functional(func, rho, sigma);
+      e = w * o.e;
+      za = 0.5 * w * o.vr;
+      const double c = 2.0 * w * o.vs;
+      zx = c * gx;
+      zy = c * gy;
+      zz = c * gz;
+    }
+  }
+  if (zcoef) {
+    double* zc = zcoef + t.q_off + p;
+    zc[0] = za;
+    if (gga) {
+      zc[nq] = zx;
+      zc[2 * nq] = zy;
+      zc[3 * nq] = zz;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (lane == 0) {
+    red[2 * (t.red_off + wi.a)] = e;
+    red[2 * (t.red_off + wi.a) + 1] = n;
+  }
+}
+
+int launch_functional(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
+                      const double* part, const double* weights, int func, int gga,
+                      double* zcoef, double* red) {
+  if (c.npblk == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_functional, dim3((unsigned)c.npblk), dim3(64), 0, s, pblk + c.pblk_off,
+                     tasks, part, weights, func, gga, c.nq, zcoef, red);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+// Sum of (e, n) pairs in a fixed order: deterministic EXC and NEL.
+__global__ __launch_bounds__(256) void k_reduce_sums(const double* __restrict__ red,
+                                                     int64_t nb, double* __restrict__ out) {
+  __shared__ double se[256], sn[256];
+  double e = 0., n = 0.;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) {
+    e += red[2 * i];
+    n += red[2 * i + 1];
+  }
+  se[threadIdx.x] = e;
+  sn[threadIdx.x] = n;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      se[threadIdx.x] += se[threadIdx.x + st];
+      sn[threadIdx.x] += sn[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = se[0];
+    out[1] = sn[0];
+  }
+}
+
+int launch_reduce_sums(hipStream_t s, const double* red, int64_t nb, double* out2) {
+  hipLaunchKernelGGL(k_reduce_sums, dim3(1), dim3(256), 0, s, red, nb, out2);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+// Upper triangle <- lower triangle through a 32x32 LDS tile
+// (symmetrize_vxc of incore_..._exc_vxc.hpp:376-385).
+__global__ __launch_bounds__(256) void k_symmetrize(int32_t n, double* __restrict__ a,
+                                                    int64_t lda) {
+  __shared__ double tile[32][33];
+  const int bi = blockIdx.x, bj = blockIdx.y;  // source tile rows bi (lower), cols bj
+  if (bi < bj) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bi * 32 + tx, c = bj * 32 + k;
+    tile[k][tx] = (r < n && c < n) ? a[r + (int64_t)c * lda] : 0.;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    // destination (row = bj*32+tx, col = bi*32+k) = source (row bi*32+k, col bj*32+tx)
+    const int r = bj * 32 + tx, c = bi * 32 + k;
+    if (r < n && c < n && r < c) a[r + (int64_t)c * lda] = tile[tx][k];
+  }
+}
+
+int launch_symmetrize(hipStream_t s, int32_t n, double* a, int64_t lda) {
+  const int nt = (n + 31) / 32;
+  if (nt == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt), dim3(256), 0, s, n, a, lda);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+// Functional on flat arrays (kernel-level parity entry point).
+__global__ __launch_bounds__(256) void k_functional_points(int func, int64_t n,
+                                                           const double* __restrict__ rho,
+                                                           const double* __restrict__ sigma,
+                                                           double* __restrict__ eps,
+                                                           double* __restrict__ vrho,
+                                                           double* __restrict__ vsigma) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  const XCOut o = dev_eval_functional(func, r, sigma ? sigma[i] : 0.0);
+  eps[i] = r > 0.0 ? o.e / r : 0.0;
+  vrho[i] = o.vr;
+  if (vsigma) vsigma[i] = o.vs;
+}
+
+int launch_functional_points(hipStream_t s, int func, int64_t n, const double* rho,
+                             const double* sigma, double* eps, double* vrho, double* vsigma) {
+  if (n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_functional_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     func, n, rho, sigma, eps, vrho, vsigma);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+}  // namespace xcq

@@ -1,0 +1,65 @@
+// SSF molecular weights through the device C ABI.
+// GauXC behaviour: src/molecular_weights/molecular_weights.cxx:27-31 and
+// device/device_molecular_weights.cxx:18 (refuses a second application,
+// SSF only on the device).
+#include <gauxc/molecular_weights.hpp>
+#include <gauxc/exceptions.hpp>
+#include <gauxc_amd/device.h>
+
+#include <algorithm>
+#include <cctype>
+#include <string>
+#include <vector>
+
+namespace GauXC {
+
+MolecularWeightsFactory::MolecularWeightsFactory(ExecutionSpace ex, std::string lwd,
+                                                 MolecularWeightsSettings s)
+    : ex_(ex), lwd_kernel_(std::move(lwd)), settings_(s) {
+  std::transform(lwd_kernel_.begin(), lwd_kernel_.end(), lwd_kernel_.begin(),
+                 [](unsigned char c) { return std::toupper(c); });
+  if (lwd_kernel_ != "DEFAULT" && lwd_kernel_ != "SCHEME1" && lwd_kernel_ != "SCHEME1-MIXED")
+    GAUXC_GENERIC_EXCEPTION("Local Work Driver Not Recognized: " + lwd_kernel_);
+}
+
+void MolecularWeights::modify_weights(LoadBalancer& lb) const {
+  if (lb.state().modified_weights_are_stored)
+    GAUXC_GENERIC_EXCEPTION("Attempting to Overwrite Modified Weights");
+  if (settings_.weight_alg != XCWeightAlg::SSF)
+    GAUXC_GENERIC_EXCEPTION("Only SSF partition weights are provided by the device backend");
+  if (ex_ != ExecutionSpace::Device)
+    GAUXC_GENERIC_EXCEPTION("MolecularWeights: only ExecutionSpace::Device is provided");
+
+  auto& tasks = lb.get_tasks();
+  const auto& mol = lb.molecule();
+  size_t npts = 0;
+  for (const auto& t : tasks) npts += t.points.size();
+  std::vector<double> pts(3 * npts), w(npts);
+  std::vector<int32_t> parent(npts);
+  size_t k = 0;
+  for (const auto& t : tasks)
+    for (size_t i = 0; i < t.points.size(); ++i, ++k) {
+      pts[3 * k] = t.points[i][0];
+      pts[3 * k + 1] = t.points[i][1];
+      pts[3 * k + 2] = t.points[i][2];
+      w[k] = t.weights[i];
+      parent[k] = t.iParent;
+    }
+  std::vector<double> xyz(3 * mol.natoms());
+  for (size_t a = 0; a < mol.natoms(); ++a) {
+    xyz[3 * a] = mol[a].x;
+    xyz[3 * a + 1] = mol[a].y;
+    xyz[3 * a + 2] = mol[a].z;
+  }
+  if (xcq_ssf_weights(device_, static_cast<int32_t>(mol.natoms()), xyz.data(),
+                      static_cast<int64_t>(npts), pts.data(), parent.data(), w.data()) !=
+      XCQ_SUCCESS)
+    GAUXC_GENERIC_EXCEPTION(std::string("SSF weights failed: ") + xcq_last_error());
+  k = 0;
+  for (auto& t : tasks)
+    for (size_t i = 0; i < t.weights.size(); ++i, ++k) t.weights[i] = w[k];
+  lb.state().modified_weights_are_stored = true;
+  lb.mark_modified();
+}
+
+}  // namespace GauXC

@@ -1,0 +1,265 @@
+// Atomic quadratures, octree micro-batching and molecular grid defaults.
+// Reference behaviour: src/grid_factory.cxx:35-69 (unpruned product grids),
+// src/molgrid_defaults.cxx:52-71 (Mura-Knowles scale 7.0 for alkali and
+// alkaline-earth metals, else 5.0) and :144-202 (standard grid sizes).
+#include <gauxc/molgrid.hpp>
+#include <gauxc/exceptions.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <set>
+#include <string>
+
+namespace GauXC {
+
+namespace {
+
+struct LebedevOrbit {
+  double a, b, c, w;
+  int n;
+};
+struct LebedevRule {
+  int npts, order, norbits;
+  const LebedevOrbit* orbits;
+};
+#include "lebedev_orbits.inc"
+
+// All distinct points reachable from (a,b,c) by coordinate permutation and
+// sign change; the count is checked against the tabulated orbit size.
+void expand_orbit(const LebedevOrbit& o, std::vector<std::array<double, 3>>& pts,
+                  std::vector<double>& wts) {
+  const double v[3] = {o.a, o.b, o.c};
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  std::set<std::array<double, 3>> seen;
+  for (int s = 0; s < 8; ++s)
+    for (const auto& p : perm) {
+      std::array<double, 3> q;
+      for (int d = 0; d < 3; ++d) {
+        double x = v[p[d]];
+        if ((s >> d) & 1) x = -x;
+        q[d] = (x == 0.) ? 0. : x;  // fold -0 into +0
+      }
+      if (seen.insert(q).second) {
+        pts.push_back(q);
+        wts.push_back(o.w);
+      }
+    }
+  if ((int)seen.size() != o.n) GAUXC_GENERIC_EXCEPTION("Lebedev orbit expansion mismatch");
+}
+
+void octree_partition(const std::vector<std::array<double, 3>>& pts, std::vector<size_t>& idx,
+                      size_t begin, size_t end, std::array<double, 3> lo,
+                      std::array<double, 3> up, size_t max_batch, std::vector<size_t>& offs) {
+  const size_t n = end - begin;
+  if (n == 0) return;
+  if (n <= max_batch) {
+    offs.push_back(end);
+    return;
+  }
+  std::array<double, 3> mid;
+  for (int d = 0; d < 3; ++d) mid[d] = 0.5 * (lo[d] + up[d]);
+  // Stable three-level split: x, then y, then z; child c has bit d set when
+  // coordinate d lies at or above the midpoint.
+  std::vector<size_t> tmp(idx.begin() + begin, idx.begin() + end);
+  size_t pos = begin;
+  for (int c = 0; c < 8; ++c) {
+    const size_t cbeg = pos;
+    for (size_t k : tmp) {
+      bool in = true;
+      for (int d = 0; d < 3 && in; ++d) {
+        const bool hi = pts[k][d] >= mid[d];
+        in = (((c >> d) & 1) != 0) == hi;
+      }
+      if (in) idx[pos++] = k;
+    }
+    std::array<double, 3> clo = lo, cup = up;
+    for (int d = 0; d < 3; ++d) {
+      if ((c >> d) & 1) clo[d] = mid[d];
+      else cup[d] = mid[d];
+    }
+    if (pos > cbeg) {
+      if (pos - cbeg == n) {
+        // All points coincide in one child (degenerate box): stop splitting.
+        offs.push_back(pos);
+      } else {
+        octree_partition(pts, idx, cbeg, pos, clo, cup, max_batch, offs);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+MicroBatcher::MicroBatcher(size_t max_batch, std::shared_ptr<const AtomicQuadrature> q)
+    : quad_(std::move(q)), max_batch_(max_batch) {
+  if (max_batch_ == 0) GAUXC_GENERIC_EXCEPTION("BatchSize must be positive");
+  const auto& pts = quad_->points;
+  order_.resize(pts.size());
+  std::iota(order_.begin(), order_.end(), size_t{0});
+  offsets_.assign(1, 0);
+  if (pts.empty()) return;
+  std::array<double, 3> lo = pts[0], up = pts[0];
+  for (const auto& p : pts)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], p[d]);
+      up[d] = std::max(up[d], p[d]);
+    }
+  octree_partition(pts, order_, 0, pts.size(), lo, up, max_batch_, offsets_);
+}
+
+GridBatch MicroBatcher::at(size_t ib, const std::array<double, 3>& c) const {
+  if (ib >= nbatches()) GAUXC_GENERIC_EXCEPTION("batch index out of range");
+  GridBatch b;
+  const size_t beg = offsets_[ib], end = offsets_[ib + 1];
+  b.points.reserve(end - beg);
+  b.weights.reserve(end - beg);
+  for (size_t k = beg; k < end; ++k) {
+    const auto& p = quad_->points[order_[k]];
+    b.points.push_back({p[0] + c[0], p[1] + c[1], p[2] + c[2]});
+    b.weights.push_back(quad_->weights[order_[k]]);
+  }
+  b.lo = b.points.front();
+  b.up = b.points.front();
+  for (const auto& p : b.points)
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = std::min(b.lo[d], p[d]);
+      b.up[d] = std::max(b.up[d], p[d]);
+    }
+  return b;
+}
+
+Grid::Grid(std::shared_ptr<const AtomicQuadrature> q, BatchSize bsz)
+    : quad_(std::move(q)),
+      batcher_(std::make_shared<MicroBatcher>(static_cast<size_t>(bsz.get()), quad_)) {}
+
+AtomicQuadrature AtomicGridFactory::mura_knowles(RadialSize nrad, RadialScale scale) {
+  AtomicQuadrature q;
+  const int64_t n = nrad.get();
+  const double s = scale.get();
+  q.points.resize(n);
+  q.weights.resize(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const double x = double(i + 1) / double(n + 1);
+    const double x3 = x * x * x;
+    const double r = -s * std::log(1. - x3);
+    const double dr = 3. * s * x * x / ((1. - x3) * double(n + 1));
+    q.points[i] = {r, 0., 0.};
+    q.weights[i] = dr * r * r;
+  }
+  return q;
+}
+
+AtomicQuadrature AtomicGridFactory::lebedev_laikov(AngularSize nang) {
+  for (const auto& rule : kLebedevRules) {
+    if (rule.npts != nang.get()) continue;
+    AtomicQuadrature q;
+    for (int k = 0; k < rule.norbits; ++k) expand_orbit(rule.orbits[k], q.points, q.weights);
+    return q;
+  }
+  GAUXC_GENERIC_EXCEPTION("Unsupported Lebedev grid size " + std::to_string(nang.get()));
+}
+
+std::shared_ptr<AtomicQuadrature> AtomicGridFactory::generate_unpruned_quadrature(
+    RadialQuad rq, RadialSize nrad, AngularSize nang, RadialScale rscal) {
+  if (rq != RadialQuad::MuraKnowles)
+    GAUXC_GENERIC_EXCEPTION("Only the Mura-Knowles radial quadrature is provided");
+  const auto rad = mura_knowles(nrad, rscal);
+  const auto ang = lebedev_laikov(nang);
+  auto q = std::make_shared<AtomicQuadrature>();
+  q->points.reserve(rad.npts() * ang.npts());
+  q->weights.reserve(rad.npts() * ang.npts());
+  for (size_t ir = 0; ir < rad.npts(); ++ir) {
+    const double r = rad.points[ir][0];
+    for (size_t ia = 0; ia < ang.npts(); ++ia) {
+      const auto& u = ang.points[ia];
+      q->points.push_back({r * u[0], r * u[1], r * u[2]});
+      q->weights.push_back(rad.weights[ir] * ang.weights[ia]);
+    }
+  }
+  return q;
+}
+
+Grid AtomicGridFactory::generate_unpruned_grid(RadialQuad rq, RadialSize nrad, AngularSize nang,
+                                               RadialScale rscal, BatchSize bsz) {
+  return Grid(generate_unpruned_quadrature(rq, nrad, nang, rscal), bsz);
+}
+
+RadialScale default_mk_radial_scaling_factor(AtomicNumber Z) {
+  switch (Z.get()) {
+    case 3: case 4: case 11: case 12: case 19: case 20:
+    case 37: case 38: case 55: case 56: case 87: case 88:
+      return RadialScale(7.0);
+    default:
+      return RadialScale(5.0);
+  }
+}
+
+RadialScale default_radial_scaling_factor(RadialQuad rq, AtomicNumber Z) {
+  if (rq == RadialQuad::MuraKnowles) return default_mk_radial_scaling_factor(Z);
+  GAUXC_GENERIC_EXCEPTION("Only the Mura-Knowles radial quadrature is provided");
+}
+
+std::tuple<RadialSize, AngularSize> default_grid_size(AtomicNumber Z, RadialQuad,
+                                                      AtomicGridSizeDefault s) {
+  switch (s) {
+    case AtomicGridSizeDefault::GM3: return {RadialSize(35), AngularSize(110)};
+    case AtomicGridSizeDefault::GM5: return {RadialSize(50), AngularSize(302)};
+    case AtomicGridSizeDefault::FineGrid: return {RadialSize(75), AngularSize(302)};
+    case AtomicGridSizeDefault::UltraFineGrid: return {RadialSize(99), AngularSize(590)};
+    case AtomicGridSizeDefault::SuperFineGrid:
+      return Z.get() <= 2 ? std::make_tuple(RadialSize(175), AngularSize(974))
+                          : std::make_tuple(RadialSize(250), AngularSize(974));
+  }
+  GAUXC_GENERIC_EXCEPTION("Not A Recognized Standard Grid");
+}
+
+UnprunedAtomicGridSpecification MolGridFactory::create_default_unpruned_grid_spec(
+    AtomicNumber Z, RadialQuad rq, AtomicGridSizeDefault s) {
+  auto [rsz, asz] = default_grid_size(Z, rq, s);
+  return UnprunedAtomicGridSpecification{rq, rsz, default_radial_scaling_factor(rq, Z), asz};
+}
+
+atomic_grid_spec_map MolGridFactory::create_default_gridmap(const Molecule& mol,
+                                                            PruningScheme ps, RadialQuad rq,
+                                                            AtomicGridSizeDefault s) {
+  if (ps != PruningScheme::Unpruned)
+    GAUXC_GENERIC_EXCEPTION("Only unpruned atomic grids are provided");
+  atomic_grid_spec_map m;
+  for (const auto& a : mol)
+    if (!m.count(a.Z)) m.emplace(a.Z, create_default_unpruned_grid_spec(a.Z, rq, s));
+  return m;
+}
+
+MolGrid::MolGrid(const atomic_grid_spec_map& specs, BatchSize bsz) {
+  for (const auto& [Z, sp] : specs)
+    grids_.emplace(Z, AtomicGridFactory::generate_unpruned_grid(sp.radial_quad, sp.radial_size,
+                                                                sp.angular_size,
+                                                                sp.radial_scale, bsz));
+}
+
+MolGrid MolGridFactory::create_default_molgrid(const Molecule& mol, PruningScheme ps,
+                                               BatchSize bsz, RadialQuad rq,
+                                               AtomicGridSizeDefault s) {
+  return MolGrid(create_default_gridmap(mol, ps, rq, s), bsz);
+}
+
+const Grid& MolGrid::get_grid(AtomicNumber Z) const {
+  auto it = grids_.find(Z);
+  if (it == grids_.end()) GAUXC_GENERIC_EXCEPTION("No grid for Z=" + std::to_string(Z.get()));
+  return it->second;
+}
+
+size_t MolGrid::max_nbatches() const {
+  size_t m = 0;
+  for (const auto& [Z, g] : grids_) m = std::max(m, g.batcher().nbatches());
+  return m;
+}
+
+int64_t MolGrid::max_batch_size() const {
+  int64_t m = 0;
+  for (const auto& [Z, g] : grids_) m = std::max<int64_t>(m, g.max_batch_size());
+  return m;
+}
+
+}  // namespace GauXC

@@ -1,0 +1,82 @@
+// Task generation ("load balancing") for the replicated integrator.
+//
+// GauXC counterpart: include/gauxc/load_balancer.hpp:71-198 and
+// src/load_balancer/host/replicated_host_load_balancer.cxx:22-193 with the
+// "petite" screening of petite_replicated_load_balancer.cxx:31-65:
+//  * for every atom and every micro-batch of its grid, keep each shell whose
+//    screening sphere (centre, cutoff radius) intersects the batch's box;
+//  * batches with no surviving shell are dropped;
+//  * tasks are ordered by (parent atom, shell list) and consecutive tasks
+//    with equal parent and shell list are merged (points concatenated).
+// With a single process every task is local, so GauXC's greedy rank
+// assignment reduces to keeping all of them.
+#pragma once
+#include <gauxc/basisset.hpp>
+#include <gauxc/enums.hpp>
+#include <gauxc/molecule.hpp>
+#include <gauxc/molgrid.hpp>
+#include <gauxc/runtime_environment.hpp>
+#include <gauxc/xc_task.hpp>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace GauXC {
+
+struct LoadBalancerState {
+  bool modified_weights_are_stored = false;
+};
+
+class LoadBalancer {
+  std::shared_ptr<const Molecule> mol_;
+  std::shared_ptr<const MolGrid> mg_;
+  std::shared_ptr<const BasisSet<double>> basis_;
+  std::shared_ptr<const MolMeta> meta_;
+  mutable std::vector<XCTask> tasks_;
+  mutable bool built_ = false;
+  LoadBalancerState state_;
+  uint64_t revision_ = 0;  // bumped whenever tasks may have changed
+
+  void build_tasks() const;
+
+ public:
+  LoadBalancer(const Molecule& mol, const MolGrid& mg, const BasisSet<double>& basis);
+  LoadBalancer(const LoadBalancer&);
+  LoadBalancer(LoadBalancer&&) noexcept = default;
+  ~LoadBalancer() = default;
+
+  const std::vector<XCTask>& get_tasks() const;
+  std::vector<XCTask>& get_tasks();
+  const Molecule& molecule() const { return *mol_; }
+  const MolMeta& molmeta() const { return *meta_; }
+  const MolGrid& molgrid() const { return *mg_; }
+  const BasisSet<double>& basis() const { return *basis_; }
+  LoadBalancerState& state() { return state_; }
+  const LoadBalancerState& state() const { return state_; }
+  size_t max_npts() const;
+  size_t total_npts() const;
+  /// Monotone counter that changes whenever the task list or its weights change.
+  uint64_t revision() const { return revision_; }
+  void mark_modified() { ++revision_; }
+};
+
+class LoadBalancerFactory {
+  ExecutionSpace ex_;
+  std::string kernel_;
+
+ public:
+  LoadBalancerFactory(ExecutionSpace ex, std::string kernel_name);
+  LoadBalancer get_instance(const RuntimeEnvironment& rt, const Molecule& mol, const MolGrid& mg,
+                            const BasisSet<double>& basis);
+  std::shared_ptr<LoadBalancer> get_shared_instance(const RuntimeEnvironment& rt,
+                                                    const Molecule& mol, const MolGrid& mg,
+                                                    const BasisSet<double>& basis);
+};
+
+namespace geometry {
+/// True when the sphere (center, rad) intersects the axis-aligned box [lo, up].
+bool cube_sphere_intersect(const double* lo, const double* up, const double* center, double rad);
+}
+
+}  // namespace GauXC

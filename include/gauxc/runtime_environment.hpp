@@ -1,0 +1,39 @@
+// Runtime environments (GauXC: include/gauxc/runtime_environment/decl.hpp:26-86).
+// This build runs one process per GPU and has no MPI layer: comm_rank() is 0
+// and comm_size() is 1.  DeviceRuntimeEnvironment carries the device ordinal
+// and the fraction of free device memory the integrator may use as its
+// working arena, or a caller-provided arena size.
+#pragma once
+#include <cstddef>
+#include <memory>
+
+namespace GauXC {
+
+class RuntimeEnvironment {
+ public:
+  RuntimeEnvironment() = default;
+  virtual ~RuntimeEnvironment() = default;
+  int comm_rank() const { return 0; }
+  int comm_size() const { return 1; }
+};
+
+class DeviceRuntimeEnvironment : public RuntimeEnvironment {
+  int device_ = 0;
+  double fill_fraction_ = 0.9;
+  void* user_mem_ = nullptr;
+  size_t user_mem_size_ = 0;
+
+ public:
+  explicit DeviceRuntimeEnvironment(double fill_fraction = 0.9, int device = 0)
+      : device_(device), fill_fraction_(fill_fraction) {}
+  /// Caller-provided device arena (device pointer and size in bytes).
+  DeviceRuntimeEnvironment(void* mem, size_t mem_sz, int device = 0)
+      : device_(device), fill_fraction_(0.), user_mem_(mem), user_mem_size_(mem_sz) {}
+  int device_id() const { return device_; }
+  double fill_fraction() const { return fill_fraction_; }
+  void* device_memory() const { return user_mem_; }
+  size_t device_memory_size() const { return user_mem_size_; }
+  bool owns_memory() const { return user_mem_ == nullptr; }
+};
+
+}  // namespace GauXC

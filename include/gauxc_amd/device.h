@@ -1,0 +1,134 @@
+/* gauxc_amd device layer: the C ABI between the GauXC-compatible C++ host
+ * library (libgauxc_amd.so) and the gfx950 HIP kernels
+ * (libgauxc_amd_device.so).  Plain-old-data arguments, int status codes, no
+ * exceptions or C++ types cross it.  The host library is its only caller in
+ * the product; tests call it directly for kernel-level parity.
+ *
+ * Each entry point states the GauXC step it replaces.  All GauXC citations
+ * are relative to the GauXC source tree.
+ *
+ * Status codes: XCQ_SUCCESS (0) or a non-zero error; xcq_last_error()
+ * returns the calling thread's last message.
+ */
+#ifndef GAUXC_AMD_DEVICE_H
+#define GAUXC_AMD_DEVICE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XCQ_SUCCESS 0
+#define XCQ_ERROR_INVALID_ARGUMENT 1
+#define XCQ_ERROR_HIP 2
+#define XCQ_ERROR_OUT_OF_MEMORY 3
+#define XCQ_ERROR_STATE 4
+#define XCQ_ERROR_NO_DEVICE 5
+
+/* Functionals evaluated on the device.  XCQ_FUNC_PBE0 is the XC part of
+ * PBE0 (0.75 PBE exchange + PBE correlation); the exact-exchange share is
+ * not an XC-quadrature quantity. */
+enum xcq_functional { XCQ_FUNC_SVWN5 = 1, XCQ_FUNC_PBE = 2, XCQ_FUNC_PBE0 = 3 };
+
+/* Arithmetic of the per-task contractions.
+ *   XCQ_PREC_FP64  : everything in FP64.
+ *   XCQ_PREC_MIXED : collocation in FP32; X = B*P_sub and B^T*Z on BF16 MFMA
+ *                    with split operands and FP32 accumulation; SSF weights,
+ *                    the functional at rho < 1e-6 and the EXC/VXC
+ *                    accumulation in FP64. */
+enum xcq_precision { XCQ_PREC_FP64 = 0, XCQ_PREC_MIXED = 1 };
+
+/* Contracted shell with normalised coefficients (GauXC Shell<double>,
+ * include/gauxc/shell.hpp:48-229). */
+typedef struct xcq_shell {
+  int32_t nprim, l, pure, pad_;
+  double alpha[32];
+  double coeff[32];
+  double origin[3];
+  double cutoff_radius;
+} xcq_shell;
+
+/* One quadrature task (GauXC XCTask, include/gauxc/xc_task.hpp:25-119).
+ * Points are x,y,z interleaved; point_offset indexes points (x3) and
+ * weights, shell_offset indexes the shell-list array.  Shell lists are
+ * ascending. */
+typedef struct xcq_task {
+  int32_t iparent, npts, nshells, nbe;
+  int64_t point_offset, shell_offset;
+  double dist_nearest;
+} xcq_task;
+
+typedef struct xcq_context_s* xcq_context;
+
+/* ---- runtime (GauXC DeviceRuntimeEnvironment / DeviceBackend,
+ *      src/runtime_environment/device/hip/hip_backend.cxx:16-131) ---- */
+int xcq_device_count(int* count);
+const char* xcq_last_error(void);
+/* arena_bytes = 0: size the working arena as fill_fraction of free memory. */
+int xcq_context_create(int device, size_t arena_bytes, double fill_fraction, xcq_context* ctx);
+int xcq_context_destroy(xcq_context ctx);
+int xcq_synchronize(xcq_context ctx);
+
+/* ---- static data (XCDeviceStackData::send_static_data_density_basis,
+ *      src/xc_integrator/xc_data/device/xc_device_stack_data.cxx:350-384) ---- */
+int xcq_upload_basis(xcq_context ctx, int32_t nshells, const xcq_shell* shells);
+/* Task table + points/weights, resident in HBM until replaced
+ * (replaces the per-batch pack_and_send of generate_buffers,
+ *  xc_device_stack_data.cxx:713-751). */
+int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int64_t npts_total,
+                     const double* points, const double* weights, int64_t nshell_entries,
+                     const int32_t* shell_lists);
+/* Density matrix, nbf x nbf column-major (host pointer). */
+int xcq_upload_density(xcq_context ctx, int32_t nbf, const double* P, int64_t ldp);
+
+/* ---- SSF partition weights (GauXC: DeviceMolecularWeights::modify_weights,
+ *      src/molecular_weights/device/device_molecular_weights.cxx:18, kernels
+ *      cuda_ssf_1d.cu:24-131; host reference weights.cxx:117-237) ----
+ * Host buffers in, host buffers out: points (3*npts), per-point parent atom,
+ * weights modified in place. */
+int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t npts,
+                    const double* points, const int32_t* parent, double* weights);
+/* Same, on the weights of the uploaded task set (device resident). */
+int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xyz);
+/* Copy the (possibly SSF-modified) resident weights back to the host. */
+int xcq_download_weights(xcq_context ctx, double* weights);
+
+/* ---- the EXC/VXC local work (IncoreReplicatedXCDeviceIntegrator::
+ *      exc_vxc_local_work_, incore_replicated_xc_device_integrator_exc_vxc.hpp:221-386):
+ *      collocation+gradient, X = 2 B P_sub, rho/grad rho/sigma, functional,
+ *      weight folding, EXC/NEL, Z, V_sub = B^T Z + Z^T B, scatter into VXC,
+ *      symmetrisation.  Runs asynchronously on the context's stream;
+ *      results stay in HBM until xcq_download_results.  want_vxc = 0 gives
+ *      the EXC-only path (eval_exc). ---- */
+int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc);
+/* Integrated density only (eval_integrate_den). */
+int xcq_integrate_den(xcq_context ctx, double* nel);
+int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc);
+
+/* ---- kernel-level entry points used by parity tests (host buffers) ---- */
+/* Collocation (+gradient if deriv = 1) of uploaded task `task`, written as
+ * [comp][nbe][npts] (comp = value, d/dx, d/dy, d/dz) in FP64.  precision
+ * selects the FP64 or FP32 collocation kernel. */
+int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision, double* out);
+/* Functional on n points (device evaluation of host arrays).  sigma/vsigma
+ * may be NULL for SVWN5. */
+int xcq_eval_functional(int device, int functional, int64_t n, const double* rho,
+                        const double* sigma, double* eps, double* vrho, double* vsigma);
+
+/* ---- instrumentation ----
+ * Per-stage device times of the last xcq_exc_vxc call in milliseconds,
+ * measured with HIP events on the context's stream.  Stage order:
+ * 0 collocation, 1 X + density reduction, 2 functional + EXC/NEL,
+ * 3 Z build, 4 B^T Z + scatter, 5 symmetrise, 6 total.  Returns the number
+ * of stages written (at most n). */
+int xcq_stage_times(xcq_context ctx, double* ms, int n);
+/* Number of kernel launches and tiles of the last call, for roofline
+ * accounting: out[0] = launches, out[1] = total GEMM tiles, out[2] = sum
+ * npts, out[3] = sum over tasks of npts*nbe*nbe. */
+int xcq_work_counters(xcq_context ctx, int64_t* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""Python access to the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle (oracle.cpp) restates GauXC's host reference integrator; it is
+the checker of the HIP path in tests/, in __graft_entry__.smoke(), and the
+CPU baseline timed by bench.py.  Product code never imports this package.
+``gg_ref()`` exposes GauXC's own gau2grid collocation compiled from the
+reference sources into oracle/_ref (present only where it was built).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+GG_REF = os.path.join(HERE, "_ref", "libgg_ref.so")
+
+FUNC = {"SVWN5": 1, "PBE": 2, "PBE0": 3}
+
+
+class OracleShell(ctypes.Structure):
+    _fields_ = [("nprim", ctypes.c_int32), ("l", ctypes.c_int32), ("pure", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("alpha", ctypes.c_double * 32),
+                ("coeff", ctypes.c_double * 32), ("origin", ctypes.c_double * 3),
+                ("cutoff_radius", ctypes.c_double)]
+
+
+class OracleTask(ctypes.Structure):
+    _fields_ = [("iparent", ctypes.c_int32), ("npts", ctypes.c_int32),
+                ("nshells", ctypes.c_int32), ("nbe", ctypes.c_int32),
+                ("point_offset", ctypes.c_int64), ("shell_offset", ctypes.c_int64),
+                ("dist_nearest", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        L.oracle_ssf_weights.argtypes = [ctypes.c_int32, vp, ctypes.c_int32, vp, vp, vp,
+                                         ctypes.c_int]
+        L.oracle_ssf_weights.restype = None
+        L.oracle_collocation.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, vp, vp,
+                                         vp]
+        L.oracle_collocation.restype = None
+        L.oracle_functional.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp, vp, vp]
+        L.oracle_functional.restype = ctypes.c_int
+        L.oracle_exc_vxc.argtypes = [ctypes.c_int, ctypes.c_int32, vp, vp, ctypes.c_int64,
+                                     ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64,
+                                     ctypes.c_int]
+        L.oracle_exc_vxc.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def shells_from(struct_array):
+    """Copy any shell struct array with nprim/l/pure/alpha/coeff/origin fields."""
+    n = len(struct_array)
+    out = (OracleShell * n)()
+    for i, s in enumerate(struct_array):
+        d = out[i]
+        d.nprim, d.l, d.pure = s.nprim, s.l, int(s.pure)
+        for k in range(s.nprim):
+            d.alpha[k] = s.alpha[k]
+            d.coeff[k] = s.coeff[k]
+        for k in range(3):
+            d.origin[k] = s.origin[k]
+    return out
+
+
+def _task_array(t):
+    nt = len(t["npts"])
+    arr = (OracleTask * nt)()
+    for i in range(nt):
+        arr[i] = OracleTask(int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
+                            int(t["nbe"][i]), int(t["point_offset"][i]),
+                            int(t["shell_offset"][i]), float(t["dist_nearest"][i]))
+    return arr
+
+
+def ssf_weights(xyz, t, nthreads=0):
+    """SSF-modified copy of t['weights'] (t: task dict of LoadBalancer.get_tasks())."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.array(t["weights"], dtype=np.float64, copy=True)
+    tasks = _task_array(t)
+    lib().oracle_ssf_weights(len(xyz), xyz.ctypes.data, len(t["npts"]), ctypes.addressof(tasks),
+                             pts.ctypes.data, w.ctypes.data, nthreads)
+    return w
+
+
+def collocation(shells, shell_list, points, deriv=True):
+    """[comp][npts][nbe] values (comp = value, d/dx, d/dy, d/dz)."""
+    sl = np.ascontiguousarray(shell_list, dtype=np.int32)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    nbe = 0
+    for s in sl:
+        sh = shells[int(s)]
+        nbe += 2 * sh.l + 1 if sh.pure else (sh.l + 1) * (sh.l + 2) // 2
+    out = np.zeros((4 if deriv else 1, len(pts), nbe))
+    ptrs = [out[k].ctypes.data for k in range(out.shape[0])] + [None] * (4 - out.shape[0])
+    lib().oracle_collocation(ctypes.addressof(shells), len(sl), sl.ctypes.data, len(pts),
+                             pts.ctypes.data, *ptrs)
+    return out
+
+
+def functional(func, rho, sigma=None):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    n = len(rho)
+    eps, vr, vs = np.zeros(n), np.zeros(n), np.zeros(n)
+    sig = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+    rc = lib().oracle_functional(FUNC[func], n, rho.ctypes.data,
+                                 None if sig is None else sig.ctypes.data, eps.ctypes.data,
+                                 vr.ctypes.data, None if sig is None else vs.ctypes.data)
+    assert rc == 0
+    return eps, vr, (vs if sig is not None else None)
+
+
+def exc_vxc(func, nbf, shells, P, t, weights=None, nthreads=0):
+    """(EXC, NEL, VXC) of the restated host integrator over task dict t."""
+    P = np.asfortranarray(P, dtype=np.float64)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    vxc = np.zeros((nbf, nbf), order="F")
+    exc = ctypes.c_double(0.)
+    nel = ctypes.c_double(0.)
+    tasks = _task_array(t)
+    rc = lib().oracle_exc_vxc(FUNC[func], nbf, ctypes.addressof(shells), P.ctypes.data, nbf,
+                              len(t["npts"]), ctypes.addressof(tasks), pts.ctypes.data,
+                              w.ctypes.data, sl.ctypes.data, ctypes.addressof(exc),
+                              ctypes.addressof(nel), vxc.ctypes.data, nbf, nthreads)
+    assert rc == 0
+    return exc.value, nel.value, vxc
+
+
+def gg_ref():
+    """GauXC's own gau2grid built from the reference sources, or None."""
+    if not os.path.exists(GG_REF):
+        return None
+    L = ctypes.CDLL(GG_REF)
+    vp = ctypes.c_void_p
+    L.gg_collocation_deriv1.argtypes = [ctypes.c_int, ctypes.c_ulong, vp, ctypes.c_ulong,
+                                        ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
+    L.gg_collocation_deriv1.restype = None
+    return L
+
+
+GG_SPHERICAL_CCA, GG_CARTESIAN_CCA = 300, 400  # gau2grid.h orderings

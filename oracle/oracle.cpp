@@ -1,0 +1,524 @@
+// CPU oracle of the XC-quadrature path -- test infrastructure, never shipped.
+// Every function names the GauXC file:line it restates.  See oracle.h.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// SSF weights: src/xc_integrator/local_work_driver/host/reference/weights.cxx:117-237,
+// constants src/xc_integrator/integrator_util/integrator_constants.hpp:17-20.
+// ---------------------------------------------------------------------------
+constexpr double kSSFa = 0.64;
+constexpr double kSSFTol = 1e-13;
+
+inline double frisch_g(double mu) {
+  const double s = mu / kSSFa, s2 = s * s, s3 = s * s2, s5 = s3 * s2, s7 = s5 * s2;
+  return (35. * (s - s3) + 21. * s5 - 5. * s7) / 16.;
+}
+
+// ---------------------------------------------------------------------------
+// Real solid harmonics in gau2grid's CCA order (m = -l..l), expressed on the
+// Cartesian monomials x^a y^b z^c of the shell (Helgaker-Jorgensen-Olsen
+// eq. 6.4.47-6.4.50; the normalisation N_lm reproduces gau2grid's
+// GG_SPHERICAL_CCA coefficients, e.g. l=2: sqrt(3)xy, sqrt(3)yz,
+// z^2-(x^2+y^2)/2, sqrt(3)xz, sqrt(3)/2 (x^2-y^2)).
+// ---------------------------------------------------------------------------
+struct SphTerm {
+  int a, b, c;
+  double coef;
+};
+
+double fact(int n) {
+  double r = 1.;
+  for (int k = 2; k <= n; ++k) r *= k;
+  return r;
+}
+double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.;
+  return fact(n) / (fact(k) * fact(n - k));
+}
+
+std::vector<std::vector<SphTerm>> solid_harmonics(int l) {
+  std::vector<std::vector<SphTerm>> out;
+  for (int m = -l; m <= l; ++m) {
+    const int am = std::abs(m);
+    const double N = std::sqrt(2. * fact(l + am) * fact(l - am) / (m == 0 ? 2. : 1.)) /
+                     (std::ldexp(1.0, am) * fact(l));
+    std::vector<SphTerm> terms;
+    // v runs over integers (m >= 0) or half-integers (m < 0); use 2v.
+    const int v2_start = (m < 0) ? 1 : 0;
+    for (int t = 0; t <= (l - am) / 2; ++t)
+      for (int u = 0; u <= t; ++u)
+        for (int v2 = v2_start; v2 <= am; v2 += 2) {
+          const int sgn_exp = t + (v2 - v2_start) / 2;
+          const double C = ((sgn_exp % 2) ? -1. : 1.) * std::pow(0.25, t) * binom(l, t) *
+                           binom(l - t, am + t) * binom(t, u) * binom(am, v2);
+          if (C == 0.) continue;
+          const int ax = 2 * t + am - 2 * u - v2;
+          const int ay = 2 * u + v2;
+          const int az = l - 2 * t - am;
+          bool merged = false;
+          for (auto& tm : terms)
+            if (tm.a == ax && tm.b == ay && tm.c == az) {
+              tm.coef += N * C;
+              merged = true;
+            }
+          if (!merged) terms.push_back({ax, ay, az, N * C});
+        }
+    out.push_back(terms);
+  }
+  return out;
+}
+
+struct SphTables {
+  std::vector<std::vector<SphTerm>> t[8];
+  SphTables() {
+    for (int l = 0; l < 8; ++l) t[l] = solid_harmonics(l);
+  }
+};
+const SphTables& sph_tables() {
+  static SphTables tb;
+  return tb;
+}
+
+inline double ipow(double x, int n) {
+  double r = 1.;
+  for (int i = 0; i < n; ++i) r *= x;
+  return r;
+}
+
+// Monomial x^a y^b z^c and its gradient times the radial part.
+// d/dx [x^a y^b z^c R] = a x^(a-1) y^b z^c R + x^a y^b z^c dR/dx.
+inline void monomial_eval(int a, int b, int c, double x, double y, double z, double R, double Rx,
+                          double Ry, double Rz, double* v, double* gx, double* gy, double* gz) {
+  const double xa = ipow(x, a), yb = ipow(y, b), zc = ipow(z, c);
+  const double mono = xa * yb * zc;
+  *v = mono * R;
+  if (gx) {
+    const double dxa = a ? a * ipow(x, a - 1) : 0.;
+    const double dyb = b ? b * ipow(y, b - 1) : 0.;
+    const double dzc = c ? c * ipow(z, c - 1) : 0.;
+    *gx = dxa * yb * zc * R + mono * Rx;
+    *gy = xa * dyb * zc * R + mono * Ry;
+    *gz = xa * yb * dzc * R + mono * Rz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Functionals (unpolarised).  ExchCXX's builtin kernels (libxc-derived) are
+// the reference; they are third-party and absent, so these are written from
+// the published forms:
+//   Slater exchange; the VWN correlation GauXC's "SVWN5" evaluates, which is
+//   the Vosko-Wilk-Nusair (1980) fit to the RPA correlation energy (libxc
+//   LDA_C_VWN_RPA parameters, paramagnetic: A=0.0310907, b=13.0720,
+//   c=42.7198, x0=-0.409286) -- pinned to 2e-15 in EXC by GauXC's
+//   benzene_svwn5 fixture; PBE exchange and correlation (Perdew-Burke-
+//   Ernzerhof 1996) with libxc's constants and the PW92 "modified"
+//   correlation parameters libxc's GGA_C_PBE uses.
+// Density thresholds follow the libxc defaults ExchCXX inherits: rho <= tol
+// gives zero output (1e-15; 1e-12 for PBE correlation).
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+
+inline void slater(double rho, double& e, double& v) {  // e = energy density
+  const double cx = -0.75 * std::cbrt(3. / kPi);
+  const double r13 = std::cbrt(rho);
+  e = cx * rho * r13;
+  v = 4. / 3. * cx * r13;
+}
+
+inline void vwn_rpa(double rho, double& e, double& v) {
+  const double A = 0.0310907, b = 13.0720, c = 42.7198, x0 = -0.409286;
+  const double rs = std::cbrt(3. / (4. * kPi * rho));
+  const double x = std::sqrt(rs);
+  const double X = x * x + b * x + c, X0 = x0 * x0 + b * x0 + c;
+  const double Q = std::sqrt(4. * c - b * b);
+  const double tx = 2. * x + b;
+  const double at = std::atan(Q / tx);
+  const double eps = A * (std::log(x * x / X) + 2. * b / Q * at -
+                          b * x0 / X0 *
+                              (std::log((x - x0) * (x - x0) / X) + 2. * (b + 2. * x0) / Q * at));
+  const double den = tx * tx + Q * Q;
+  const double deps_dx = A * (2. / x - tx / X - 4. * b / den -
+                              b * x0 / X0 * (2. / (x - x0) - tx / X - 4. * (b + 2. * x0) / den));
+  e = rho * eps;
+  v = eps - x / 6. * deps_dx;
+}
+
+inline void pbe_x(double rho, double sigma, double& e, double& vr, double& vs) {
+  const double kappa = 0.804, mu = 0.2195149727645171;
+  const double ax = -0.75 * std::cbrt(3. / kPi);
+  const double C = 1. / (4. * std::pow(3. * kPi * kPi, 2. / 3.));
+  const double r13 = std::cbrt(rho);
+  const double r43 = rho * r13;
+  const double s2 = C * sigma / (r43 * r43);
+  const double den = 1. + mu * s2 / kappa;
+  const double F = 1. + kappa - kappa / den;
+  const double dF = mu / (den * den);  // dF/d(s^2)
+  e = ax * r43 * F;
+  vr = 4. / 3. * ax * r13 * F - 8. / 3. * ax * r13 * dF * s2;
+  vs = ax * r43 * dF * C / (r43 * r43);
+}
+
+inline void pw92_mod(double rs, double& eps, double& deps) {
+  const double a = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382,
+               b4 = 0.49294;
+  const double srs = std::sqrt(rs);
+  const double Qd = 2. * a * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  const double dQd = 2. * a * (0.5 * b1 / srs + b2 + 1.5 * b3 * srs + 2. * b4 * rs);
+  const double L = std::log1p(1. / Qd);
+  eps = -2. * a * (1. + a1 * rs) * L;
+  deps = -2. * a * a1 * L + 2. * a * (1. + a1 * rs) * dQd / (Qd * Qd + Qd);
+}
+
+inline void pbe_c(double rho, double sigma, double& e, double& vr, double& vs) {
+  const double beta = 0.06672455060314922;
+  const double gamma = (1. - std::log(2.)) / (kPi * kPi);
+  const double y = beta / gamma;
+  const double rs = std::cbrt(3. / (4. * kPi * rho));
+  double ec, dec;
+  pw92_mod(rs, ec, dec);
+  const double D = kPi / (16. * std::cbrt(3. * kPi * kPi));
+  const double T = D * sigma / std::pow(rho, 7. / 3.);
+  const double E = std::exp(-ec / gamma);
+  const double A = y / (E - 1.);
+  const double N = 1. + A * T, Dn = 1. + A * T + A * A * T * T;
+  const double Q = T * N / Dn;
+  const double H = gamma * std::log1p(y * Q);
+  const double dH_dQ = gamma * y / (1. + y * Q);
+  const double dQ_dT = ((N + T * A) * Dn - T * N * (A + 2. * A * A * T)) / (Dn * Dn);
+  const double dQ_dA = T * T * (Dn - N * (1. + 2. * A * T)) / (Dn * Dn);
+  const double dA_de = y * E / (gamma * (E - 1.) * (E - 1.));
+  e = rho * (ec + H);
+  vr = ec + H - rs / 3. * dec * (1. + dH_dQ * dQ_dA * dA_de) - 7. / 3. * T * dH_dQ * dQ_dT;
+  vs = rho * dH_dQ * dQ_dT * D / std::pow(rho, 7. / 3.);
+}
+
+struct FuncOut {
+  double e, vr, vs;
+};
+
+inline FuncOut eval_func(int func, double rho, double sigma) {
+  FuncOut o{0., 0., 0.};
+  const double sig_tol = 2.1544346900318838e-43;
+  if (func == ORACLE_SVWN5) {
+    double e, v;
+    if (rho > 1e-15) {
+      slater(rho, e, v);
+      o.e += e;
+      o.vr += v;
+      vwn_rpa(rho, e, v);
+      o.e += e;
+      o.vr += v;
+    }
+    return o;
+  }
+  const double xs = (func == ORACLE_PBE0) ? 0.75 : 1.0;
+  const double s = std::max(sigma, sig_tol * sig_tol);
+  double e, vr, vs;
+  if (rho > 1e-15) {
+    pbe_x(rho, s, e, vr, vs);
+    o.e += xs * e;
+    o.vr += xs * vr;
+    o.vs += xs * vs;
+  }
+  if (rho > 1e-12) {
+    pbe_c(rho, s, e, vr, vs);
+    o.e += e;
+    o.vr += vr;
+    o.vs += vs;
+  }
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_ssf_weights(int32_t natoms, const double* xyz, int32_t ntasks,
+                        const oracle_task* tasks, const double* points, double* weights,
+                        int nthreads) {
+  // MolMeta::compute_rab / compute_dist_nearest (src/molmeta.cxx:28-60).
+  std::vector<double> rab((size_t)natoms * natoms, 0.);
+  for (int i = 0; i < natoms; ++i)
+    for (int j = 0; j < natoms; ++j) {
+      const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                   dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> part(natoms), dist(natoms);
+#pragma omp for schedule(dynamic)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const double cutoff = 0.5 * (1. - kSSFa) * t.dist_nearest;
+      for (int32_t ip = 0; ip < t.npts; ++ip) {
+        const double* p = points + 3 * (t.point_offset + ip);
+        double& w = weights[t.point_offset + ip];
+        for (int a = 0; a < natoms; ++a) {
+          const double dx = p[0] - xyz[3 * a], dy = p[1] - xyz[3 * a + 1],
+                       dz = p[2] - xyz[3 * a + 2];
+          dist[a] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        if (dist[t.iparent] < cutoff) continue;  // whole weight stays with the parent
+        std::fill(part.begin(), part.end(), 1.);
+        for (int i = 0; i < natoms; ++i)
+          for (int j = 0; j < i; ++j) {
+            if (!(part[i] > kSSFTol || part[j] > kSSFTol)) continue;
+            const double mu = (dist[i] - dist[j]) / rab[(size_t)j + (size_t)i * natoms];
+            if (mu <= -kSSFa) {
+              part[j] = 0.;
+            } else if (mu >= kSSFa) {
+              part[i] = 0.;
+            } else {
+              const double g = 0.5 * (1. - frisch_g(mu));
+              part[i] *= g;
+              part[j] *= 1. - g;
+            }
+          }
+        double sum = 0.;
+        for (int a = 0; a < natoms; ++a) sum += part[a];
+        w *= part[t.iparent] / sum;
+      }
+    }
+  }
+}
+
+void oracle_collocation(const oracle_shell* shells, int32_t nlist, const int32_t* shell_list,
+                        int32_t npts, const double* points, double* b, double* dx, double* dy,
+                        double* dz) {
+  int32_t nbe = 0;
+  for (int32_t i = 0; i < nlist; ++i) {
+    const auto& sh = shells[shell_list[i]];
+    nbe += sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2;
+  }
+  const auto& sph = sph_tables();
+  std::vector<double> cv, cgx, cgy, cgz;
+  for (int32_t ip = 0; ip < npts; ++ip) {
+    int32_t off = 0;
+    for (int32_t i = 0; i < nlist; ++i) {
+      const auto& sh = shells[shell_list[i]];
+      const double x = points[3 * ip] - sh.origin[0];
+      const double y = points[3 * ip + 1] - sh.origin[1];
+      const double z = points[3 * ip + 2] - sh.origin[2];
+      const double r2 = x * x + y * y + z * z;
+      double R = 0., dR = 0.;  // dR = sum -2 a c exp(-a r^2)
+      for (int k = 0; k < sh.nprim; ++k) {
+        const double e = sh.coeff[k] * std::exp(-sh.alpha[k] * r2);
+        R += e;
+        dR += -2. * sh.alpha[k] * e;
+      }
+      const double Rx = dR * x, Ry = dR * y, Rz = dR * z;
+      const int l = sh.l;
+      const int ncart = (l + 1) * (l + 2) / 2;
+      cv.assign(ncart, 0.);
+      cgx.assign(ncart, 0.);
+      cgy.assign(ncart, 0.);
+      cgz.assign(ncart, 0.);
+      // Cartesian CCA order: a from l down to 0, then b from l-a down to 0.
+      int ic = 0;
+      for (int a = l; a >= 0; --a)
+        for (int bb = l - a; bb >= 0; --bb, ++ic)
+          monomial_eval(a, bb, l - a - bb, x, y, z, R, Rx, Ry, Rz, &cv[ic],
+                        dx ? &cgx[ic] : nullptr, &cgy[ic], &cgz[ic]);
+      const size_t row = (size_t)ip * nbe;
+      if (!sh.pure) {
+        for (int c = 0; c < ncart; ++c) {
+          b[row + off + c] = cv[c];
+          if (dx) {
+            dx[row + off + c] = cgx[c];
+            dy[row + off + c] = cgy[c];
+            dz[row + off + c] = cgz[c];
+          }
+        }
+        off += ncart;
+      } else {
+        for (int m = 0; m < 2 * l + 1; ++m) {
+          double v = 0., gx = 0., gy = 0., gz = 0.;
+          for (const auto& tm : sph.t[l][m]) {
+            double mv, mx, my, mz;
+            monomial_eval(tm.a, tm.b, tm.c, x, y, z, R, Rx, Ry, Rz, &mv, &mx, &my, &mz);
+            v += tm.coef * mv;
+            gx += tm.coef * mx;
+            gy += tm.coef * my;
+            gz += tm.coef * mz;
+          }
+          b[row + off + m] = v;
+          if (dx) {
+            dx[row + off + m] = gx;
+            dy[row + off + m] = gy;
+            dz[row + off + m] = gz;
+          }
+        }
+        off += 2 * l + 1;
+      }
+    }
+  }
+}
+
+int oracle_functional(int func, int64_t n, const double* rho, const double* sigma, double* eps,
+                      double* vrho, double* vsigma) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  for (int64_t i = 0; i < n; ++i) {
+    const FuncOut o = eval_func(func, rho[i], sigma ? sigma[i] : 0.);
+    eps[i] = rho[i] > 0. ? o.e / rho[i] : 0.;
+    vrho[i] = o.vr;
+    if (vsigma) vsigma[i] = o.vs;
+  }
+  return 0;
+}
+
+int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const double* P,
+                   int64_t ldp, int32_t ntasks, const oracle_task* tasks, const double* points,
+                   const double* weights, const int32_t* shell_lists, double* exc, double* nel,
+                   double* vxc, int64_t ldvxc, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  // Zero VXC (host_integrator_exc_vxc.hpp:173-195).
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = 0; i < nbf; ++i) vxc[i + (size_t)j * ldvxc] = 0.;
+  // First AO of each shell (BasisSetMap).
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+  }
+  double EXC = 0., NEL = 0.;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> B, Bx, By, Bz, X, Z, Psub, V, rho, sig, gx, gy, gz, eps, vr, vs;
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic) reduction(+ : EXC, NEL)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      // Local -> global basis-function index (the compressed submatrix map of
+      // integrator_common.cxx:21-144 expressed per function).
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      B.assign((size_t)npts * nbe, 0.);
+      if (gga) {
+        Bx.assign(B.size(), 0.);
+        By.assign(B.size(), 0.);
+        Bz.assign(B.size(), 0.);
+      }
+      // eval_collocation(_gradient) (host_integrator_exc_vxc.hpp:378-380).
+      oracle_collocation(shells, t.nshells, sl, npts, pts, B.data(), gga ? Bx.data() : nullptr,
+                         gga ? By.data() : nullptr, gga ? Bz.data() : nullptr);
+      // eval_xmat: X = 2 * B * P_sub (reference_local_host_work_driver.cxx:123-146).
+      Psub.resize((size_t)nbe * nbe);
+      for (int32_t j = 0; j < nbe; ++j)
+        for (int32_t i = 0; i < nbe; ++i)
+          Psub[(size_t)i + (size_t)j * nbe] = P[bf[i] + (size_t)bf[j] * ldp];
+      X.assign((size_t)npts * nbe, 0.);
+      for (int32_t p = 0; p < npts; ++p) {
+        double* xr = X.data() + (size_t)p * nbe;
+        const double* br = B.data() + (size_t)p * nbe;
+        for (int32_t k = 0; k < nbe; ++k) {
+          const double bk = 2.0 * br[k];
+          const double* pc = Psub.data() + (size_t)k * nbe;  // P_sub symmetric: row k
+          for (int32_t j = 0; j < nbe; ++j) xr[j] += bk * pc[j];
+        }
+      }
+      // eval_uvvar_{lda,gga}_rks (:150-163, :242-268).
+      rho.assign(npts, 0.);
+      sig.assign(npts, 0.);
+      gx.assign(npts, 0.);
+      gy.assign(npts, 0.);
+      gz.assign(npts, 0.);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        double r = 0., ax = 0., ay = 0., az = 0.;
+        for (int32_t k = 0; k < nbe; ++k) {
+          r += B[o + k] * X[o + k];
+          if (gga) {
+            ax += Bx[o + k] * X[o + k];
+            ay += By[o + k] * X[o + k];
+            az += Bz[o + k] * X[o + k];
+          }
+        }
+        rho[p] = r;
+        if (gga) {
+          gx[p] = 2. * ax;
+          gy[p] = 2. * ay;
+          gz[p] = 2. * az;
+          sig[p] = gx[p] * gx[p] + gy[p] * gy[p] + gz[p] * gz[p];
+        }
+      }
+      // Functional, weight folding, EXC/NEL (:445-502).
+      eps.resize(npts);
+      vr.resize(npts);
+      vs.assign(npts, 0.);
+      oracle_functional(func, npts, rho.data(), gga ? sig.data() : nullptr, eps.data(), vr.data(),
+                        gga ? vs.data() : nullptr);
+      for (int32_t p = 0; p < npts; ++p) {
+        eps[p] *= w[p];
+        vr[p] *= w[p];
+        vs[p] *= w[p];
+        NEL += w[p] * rho[p];
+        EXC += eps[p] * rho[p];
+      }
+      // Z matrix (:586-604 LDA, :678-713 GGA).
+      Z.resize((size_t)npts * nbe);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        const double fl = 0.5 * vr[p];
+        const double fx = 2. * vs[p] * gx[p], fy = 2. * vs[p] * gy[p], fz = 2. * vs[p] * gz[p];
+        for (int32_t k = 0; k < nbe; ++k) {
+          double zv = fl * B[o + k];
+          if (gga) zv += fx * Bx[o + k] + fy * By[o + k] + fz * Bz[o + k];
+          Z[o + k] = zv;
+        }
+      }
+      // inc_vxc: V = B^T Z + Z^T B (lower), scattered with atomics (:1678-1692,
+      // host/util.hpp:130-168).
+      V.assign((size_t)nbe * nbe, 0.);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        for (int32_t i = 0; i < nbe; ++i) {
+          const double bi = B[o + i], zi = Z[o + i];
+          double* vrow = V.data() + (size_t)i * nbe;
+          for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[o + j] + zi * B[o + j];
+        }
+      }
+      for (int32_t i = 0; i < nbe; ++i)
+        for (int32_t j = 0; j <= i; ++j) {
+#pragma omp atomic
+          vxc[bf[i] + (size_t)bf[j] * ldvxc] += V[(size_t)i * nbe + j];
+        }
+    }
+  }
+  // Symmetrise lower -> upper (:577-599).
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = j + 1; i < nbf; ++i) vxc[j + (size_t)i * ldvxc] = vxc[i + (size_t)j * ldvxc];
+  *exc = EXC;
+  *nel = NEL;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+/* CPU oracle for the XC-quadrature path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++/OpenMP restatement of GauXC's host reference integrator
+ * (ReferenceReplicatedXCHostIntegrator::exc_vxc_local_work_,
+ * src/xc_integrator/replicated/host/reference_replicated_xc_host_integrator_exc_vxc.hpp:106-601)
+ * and of the pieces it calls.  It is the checker of the HIP path and the
+ * CPU baseline that bench.py times; nothing in the product links it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.
+ *
+ * Layouts: basis values are [npts][nbe] (nbe fastest, as GauXC's host
+ * driver), matrices are column-major with leading dimension ld.
+ */
+#ifndef GAUXC_ORACLE_H
+#define GAUXC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_shell {
+  int32_t nprim, l, pure, pad_;
+  double alpha[32];
+  double coeff[32]; /* normalised contraction coefficients */
+  double origin[3];
+  double cutoff_radius;
+} oracle_shell;
+
+typedef struct oracle_task {
+  int32_t iparent, npts, nshells, nbe;
+  int64_t point_offset; /* into the points (x,y,z interleaved) and weights arrays */
+  int64_t shell_offset; /* into the shell-list array */
+  double dist_nearest;
+} oracle_task;
+
+enum { ORACLE_SVWN5 = 1, ORACLE_PBE = 2, ORACLE_PBE0 = 3 };
+
+/* SSF partition weights, in place (weights.cxx:117-237). */
+void oracle_ssf_weights(int32_t natoms, const double* atom_xyz, int32_t ntasks,
+                        const oracle_task* tasks, const double* points, double* weights,
+                        int nthreads);
+
+/* Collocation (+gradient when dx != NULL) of one shell list at npts points;
+ * outputs [npts][nbe] (gau2grid_collocation.cxx:25-145 semantics). */
+void oracle_collocation(const oracle_shell* shells, int32_t nlist, const int32_t* shell_list,
+                        int32_t npts, const double* points, double* b, double* dx, double* dy,
+                        double* dz);
+
+/* Functional on n points: eps (per particle), vrho, vsigma (GGA only). */
+int oracle_functional(int func, int64_t n, const double* rho, const double* sigma, double* eps,
+                      double* vrho, double* vsigma);
+
+/* Full EXC/VXC local work over the given tasks (weights already partitioned).
+ * VXC is nbf x nbf column-major, symmetrised on return.  Returns 0 on success. */
+int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const double* P,
+                   int64_t ldp, int32_t ntasks, const oracle_task* tasks, const double* points,
+                   const double* weights, const int32_t* shell_lists, double* exc, double* nel,
+                   double* vxc, int64_t ldvxc, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
