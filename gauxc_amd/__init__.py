@@ -11,3 +11,4 @@ from .api import (  # noqa: F401
     Molecule, RuntimeEnvironment, XCIntegrator, XCIntegratorFactory, eval_functional_device,
     ssf_weights_device,
 )
+from . import workloads  # noqa: F401,E402

@@ -545,7 +545,7 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
     XCQ_HIP(hipEventRecord(ev[1], s));
     XCQ_HIP(hipEventRecord(ev[2], s));
     rc = launch_xmat_den_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, ctx->d_P, ctx->nbf,
-                              scr, ctx->d_part, gga);
+                              scr, ctx->d_part, gga, func == 0 ? 1.0 : 2.0);
     if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[3], s));
     XCQ_HIP(hipEventRecord(ev[4], s));

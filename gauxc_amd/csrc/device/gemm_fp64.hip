@@ -1,7 +1,8 @@
 // FP64 contractions of the XC path on gfx950 matrix cores
 // (v_mfma_f64_16x16x4_f64).
 //
-//  k_xmat_den : X = 2 B P_sub for a 64-point x 64-function tile, with the
+//  k_xmat_den : X = fac B P_sub (fac = 2 for RKS EXC/VXC, 1 for the
+//               integrated density) for a 64-point x 64-function tile, with the
 //               density reduction fused into the epilogue: X never reaches
 //               HBM.  Per point it emits the partial sums
 //                 rho_p += sum_j B_pj X_pj,  g_p += sum_j dB_pj X_pj
@@ -40,7 +41,8 @@ __global__ __launch_bounds__(256) void k_xmat_den(const WorkItem* __restrict__ t
                                                   const int32_t* __restrict__ bfmap,
                                                   const double* __restrict__ P, int64_t ldp,
                                                   const double* __restrict__ scratch,
-                                                  double* __restrict__ part, int gga) {
+                                                  double* __restrict__ part, int gga,
+                                                  double fac) {
   __shared__ double As[16][kLdsPad];
   __shared__ double Bs[16][kLdsPad];
   const WorkItem wi = tiles[blockIdx.x];
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void k_xmat_den(const WorkItem* __restrict__ t
     const int64_t rowP = kk < nbe ? (int64_t)bf[kk] : -1;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      Bs[b_k][b_n + 16 * i] = (rowP >= 0 && colP[i] >= 0) ? 2.0 * P[rowP + colP[i]] : 0.;
+      Bs[b_k][b_n + 16 * i] = (rowP >= 0 && colP[i] >= 0) ? fac * P[rowP + colP[i]] : 0.;
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
@@ -142,10 +144,10 @@ __global__ __launch_bounds__(256) void k_xmat_den(const WorkItem* __restrict__ t
 
 int launch_xmat_den_fp64(hipStream_t s, const Chunk& c, const WorkItem* g1, const DevTask* tasks,
                          const int32_t* bfmap, const double* P, int64_t ldp,
-                         const double* scratch, double* part, int gga) {
+                         const double* scratch, double* part, int gga, double fac) {
   if (c.ng1 == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_xmat_den, dim3((unsigned)c.ng1), dim3(256), 0, s, g1 + c.g1_off, tasks,
-                     bfmap, P, ldp, scratch, part, gga);
+                     bfmap, P, ldp, scratch, part, gga, fac);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

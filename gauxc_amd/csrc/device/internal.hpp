@@ -95,7 +95,7 @@ int launch_collocation_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                             const double* pz, double* scratch, int deriv);
 int launch_xmat_den_fp64(hipStream_t s, const Chunk& c, const WorkItem* g1, const DevTask* tasks,
                          const int32_t* bfmap, const double* P, int64_t ldp,
-                         const double* scratch, double* part, int gga);
+                         const double* scratch, double* part, int gga, double fac);
 int launch_functional(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
                       const double* part, const double* weights, int func, int gga,
                       double* zcoef, double* red);
