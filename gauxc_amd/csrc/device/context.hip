@@ -80,9 +80,7 @@ struct xcq_context_s {
   double *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_w = nullptr;
   int32_t *d_shell_list = nullptr, *d_bfmap = nullptr, *d_lsh = nullptr;
   WorkItem* d_work = nullptr;
-  int64_t work_pblk_base = 0, work_g1_base = 0, work_g2_base = 0;
   std::vector<Chunk> chunks;
-  std::vector<double> h_wbuf;  // (unused) host weights scratch
 
   // density / results
   double* d_P = nullptr;
@@ -99,10 +97,8 @@ struct xcq_context_s {
   double* d_zcoef = nullptr;
   int64_t max_part = 0, max_nq = 0, max_scratch_elems = 0;
 
-  // mixed precision scratch (owned by gemm_mixed.hip helpers)
-  void* d_mixed = nullptr;
-  size_t mixed_bytes = 0;
-  void* d_Phi = nullptr;  // split density matrix (hi/lo bf16 planes)
+  // packed bfloat16 slices of P for the mixed-precision contractions
+  void* d_Phi = nullptr;
 
   // timing
   std::vector<hipEvent_t> ev;  // [chunk][stage][2]
@@ -130,38 +126,17 @@ struct xcq_context_s {
     if (d_scratch) (void)hipFree(d_scratch);
     dfree(d_part);
     dfree(d_zcoef);
-    if (d_mixed) (void)hipFree(d_mixed);
     if (d_Phi) (void)hipFree(d_Phi);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace xcq {
-// Accessors used by the mixed-precision translation unit.
-struct CtxView {
-  hipStream_t stream;
-  const DevTask* tasks;
-  const DevShell* shells;
-  const int32_t *shell_list, *bfmap, *lsh;
-  const double *px, *py, *pz, *w;
-  const WorkItem* work;
-  double* P;
-  double* vxc;
-  int32_t nbf;
-  double* red;
-  double* part;
-  double* zcoef;
-  void* scratch;
-  size_t scratch_bytes;
-  void** mixed;
-  size_t* mixed_bytes;
-  void** Psplit;
-};
 CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream, c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap, c->d_lsh,
                  c->d_px,   c->d_py,    c->d_pz,     c->d_w,          c->d_work,  c->d_P,
                  c->d_vxc,  c->nbf,     c->d_red,    c->d_part,       c->d_zcoef, c->d_scratch,
-                 c->scratch_bytes, &c->d_mixed, &c->mixed_bytes, &c->d_Phi};
+                 c->scratch_bytes, &c->d_Phi};
 }
 }  // namespace xcq
 
@@ -402,11 +377,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   if (ctx->d_scratch) {
     (void)hipFree(ctx->d_scratch);
     ctx->d_scratch = nullptr;
-  }
-  if (ctx->d_mixed) {
-    (void)hipFree(ctx->d_mixed);
-    ctx->d_mixed = nullptr;
-    ctx->mixed_bytes = 0;
   }
   for (auto e : ctx->ev) (void)hipEventDestroy(e);
   ctx->ev.clear();

@@ -21,8 +21,10 @@
 //
 // Workgroup: 256 threads = 4 waves in a 2x2 arrangement, each wave owning a
 // 32x32 sub-tile = 2x2 MFMA 16x16 blocks.  K advances 16 per LDS stage.
-// MFMA operand layout (16x16x4 f64): lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; accumulator register r of lane l is D[4*(l/16)+r][l%16].
+// MFMA operand layout (16x16x4 f64, measured on gfx950): lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; accumulator register r of lane l is
+// D[4*r + l/16][l%16] (rows interleave across the four 16-lane groups,
+// unlike the FP32-output MFMAs whose register r holds row 4*(l/16) + r).
 #include "internal.hpp"
 
 namespace xcq {
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(256) void k_xmat_den(const WorkItem* __restrict__ t
   for (int bi = 0; bi < 2; ++bi) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+      const int m = m0 + wm * 32 + bi * 16 + 4 * r + (lane >> 4);
       double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj) {
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(256) void k_syr2k(const WorkItem* __restrict__ tile
       const int64_t cj = (int64_t)bf[j] * ldv;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        const int i = i0 + wm * 32 + bi * 16 + 4 * r + (lane >> 4);
         if (i < nbe && (!diag || i >= j)) atomicAdd(vxc + bf[i] + cj, acc[bi][bj][r]);
       }
     }

@@ -1,22 +1,388 @@
-// Mixed-precision pipeline (FP32 collocation, BF16 MFMA contractions with
-// split operands, FP64 accumulation of EXC/VXC).
+// Mixed-precision contractions of the XC path on gfx950
+// (v_mfma_f32_16x16x32_bf16).
+//
+// Numerics (chosen from a CPU emulation against the FP64 path):
+//  * B, dB and Z are stored in FP32; P is held as three bfloat16 slices
+//    P ~ p1 + p2 + p3 (24 significant bits).  Each FP32 operand element is
+//    split on the fly into three bfloat16 slices the same way (exact: 3 x 8
+//    bits cover the FP32 significand).
+//  * A product A*B is formed from the six slice products a_s * b_t with
+//    s + t <= 2 (dropped terms are ~2^-24 relative), accumulated in FP32
+//    inside one 32-deep K block (one MFMA per slice product) and added to
+//    FP64 accumulators across K blocks, so FP32 rounding does not grow with
+//    the contraction length (npts can reach 2e4 in the B^T Z product).
+//  * Densities, the functional, EXC/NEL and VXC accumulate in FP64.
+//
+// Kernels mirror the FP64 ones (gemm_fp64.hip): k_xmat_den_mx fuses the
+// per-point density reduction into the X = fac B P_sub epilogue,
+// k_syr2k_mx fuses the scatter-add into VXC into the B^T Z + Z^T B
+// epilogue.  Operand layout of the MFMA (measured on gfx950): lane l holds
+// A[l%16][8(l/16) .. 8(l/16)+7] and B[8(l/16) .. +7][l%16]; accumulator
+// register r of lane l is D[4(l/16)+r][l%16].
 #include "internal.hpp"
 
 namespace xcq {
 
-template <bool GRAD, typename C, typename T>
-__global__ void k_collocation_stub() {}
+int launch_collocation_fp32(hipStream_t s, const Chunk& c, const WorkItem* pblk,
+                            const DevTask* tasks, const DevShell* shells,
+                            const int32_t* shell_list, const int32_t* lsh_first,
+                            const double* px, const double* py, const double* pz,
+                            float* scratch, int deriv);
 
-int launch_collocation_fp32(hipStream_t, const Chunk&, const WorkItem*, const DevTask*,
-                            const DevShell*, const int32_t*, const int32_t*, const double*,
-                            const double*, const double*, float*, int) {
-  set_error("mixed-precision collocation is not built yet");
-  return XCQ_ERROR_STATE;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+constexpr int kKB = 32;    // K block = MFMA depth
+constexpr int kRow = 40;   // LDS row length in bf16 (32 + 8 pad, 80 B: 16-B aligned)
+
+__device__ __forceinline__ unsigned short bf_bits(__bf16 v) {
+  return __builtin_bit_cast(unsigned short, v);
 }
 
-int launch_mixed_pipeline(struct ::xcq_context_s*, const Chunk&, int, int, int, hipEvent_t*) {
-  set_error("mixed-precision pipeline is not built yet");
-  return XCQ_ERROR_STATE;
+__device__ __forceinline__ void split3f(float x, unsigned short& a, unsigned short& b,
+                                        unsigned short& c) {
+  const __bf16 h1 = (__bf16)x;
+  const float r1 = x - (float)h1;
+  const __bf16 h2 = (__bf16)r1;
+  const float r2 = r1 - (float)h2;
+  const __bf16 h3 = (__bf16)r2;
+  a = bf_bits(h1);
+  b = bf_bits(h2);
+  c = bf_bits(h3);
+}
+
+__device__ __forceinline__ v4f mfma_bf16(const unsigned short* a, const unsigned short* b,
+                                         v4f c) {
+  const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(a));
+  const bf16x8 bv = __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(b));
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// P (FP64, nbf x nbf) -> packed (p1, p2, p3, 0) bfloat16 slices.
+__global__ __launch_bounds__(256) void k_split_P(int64_t n, const double* __restrict__ P,
+                                                 uint2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = P[i];
+  const __bf16 h1 = (__bf16)(float)x;
+  const double r1 = x - (double)(float)h1;
+  const __bf16 h2 = (__bf16)(float)r1;
+  const double r2 = r1 - (double)(float)h2;
+  const __bf16 h3 = (__bf16)(float)r2;
+  uint2 v;
+  v.x = (unsigned)bf_bits(h1) | ((unsigned)bf_bits(h2) << 16);
+  v.y = (unsigned)bf_bits(h3);
+  out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_xmat_den_mx(const WorkItem* __restrict__ tiles,
+                                                     const DevTask* __restrict__ tasks,
+                                                     const int32_t* __restrict__ bfmap,
+                                                     const uint2* __restrict__ Ps, int64_t ldp,
+                                                     const float* __restrict__ scratch,
+                                                     double* __restrict__ part, int gga,
+                                                     float fac) {
+  __shared__ __attribute__((aligned(16))) unsigned short As[3][64][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[3][64][kRow];
+  const WorkItem wi = tiles[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = wi.a * kTile, n0 = wi.b * kTile;
+  const int64_t ld = t.npts_pad;
+  const int nbe = t.nbe;
+  const float* Bm = scratch + t.b_off;
+  const int32_t* bf = bfmap + t.bf_off;
+
+  const int r64 = tid & 63, kg = tid >> 6;  // loader: row (point / function), k group of 8
+  const int ncol = n0 + r64;
+  const int64_t colP = ncol < nbe ? (int64_t)bf[ncol] * ldp : -1;
+
+  double acc[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
+
+  for (int k0 = 0; k0 < nbe; k0 += kKB) {
+    {
+      us8 s1, s2, s3;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + kg * 8 + e;
+        const float x = k < nbe ? Bm[(int64_t)k * ld + m0 + r64] : 0.f;
+        unsigned short a, b, c;
+        split3f(x, a, b, c);
+        s1[e] = a;
+        s2[e] = b;
+        s3[e] = c;
+      }
+      *reinterpret_cast<us8*>(&As[0][r64][kg * 8]) = s1;
+      *reinterpret_cast<us8*>(&As[1][r64][kg * 8]) = s2;
+      *reinterpret_cast<us8*>(&As[2][r64][kg * 8]) = s3;
+    }
+    {
+      us8 s1, s2, s3;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + kg * 8 + e;
+        uint2 v = make_uint2(0u, 0u);
+        if (k < nbe && colP >= 0) v = Ps[(int64_t)bf[k] + colP];
+        s1[e] = (unsigned short)(v.x & 0xffffu);
+        s2[e] = (unsigned short)(v.x >> 16);
+        s3[e] = (unsigned short)(v.y & 0xffffu);
+      }
+      *reinterpret_cast<us8*>(&Bs[0][r64][kg * 8]) = s1;
+      *reinterpret_cast<us8*>(&Bs[1][r64][kg * 8]) = s2;
+      *reinterpret_cast<us8*>(&Bs[2][r64][kg * 8]) = s3;
+    }
+    __syncthreads();
+    const int kk = 8 * (lane >> 4);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int ra = wm * 32 + bi * 16 + (lane & 15);
+        const int rb = wn * 32 + bj * 16 + (lane & 15);
+        v4f c = {0.f, 0.f, 0.f, 0.f};
+        c = mfma_bf16(&As[0][ra][kk], &Bs[0][rb][kk], c);
+        c = mfma_bf16(&As[0][ra][kk], &Bs[1][rb][kk], c);
+        c = mfma_bf16(&As[1][ra][kk], &Bs[0][rb][kk], c);
+        c = mfma_bf16(&As[0][ra][kk], &Bs[2][rb][kk], c);
+        c = mfma_bf16(&As[1][ra][kk], &Bs[1][rb][kk], c);
+        c = mfma_bf16(&As[2][ra][kk], &Bs[0][rb][kk], c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[r];
+      }
+    __syncthreads();
+  }
+
+  const int64_t plane = (int64_t)nbe * ld;
+  double* pslot = part + t.part_off + (int64_t)(2 * wi.b + wn) * 4 * ld;
+  const double dfac = fac;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+      double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int n = n0 + wn * 32 + bj * 16 + (lane & 15);
+        if (n < nbe) {
+          const double x = dfac * acc[bi][bj][r];
+          const int64_t o = (int64_t)n * ld + m;
+          s0 += (double)Bm[o] * x;
+          if (gga) {
+            s1 += (double)Bm[plane + o] * x;
+            s2 += (double)Bm[2 * plane + o] * x;
+            s3 += (double)Bm[3 * plane + o] * x;
+          }
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        s0 += __shfl_xor(s0, off);
+        if (gga) {
+          s1 += __shfl_xor(s1, off);
+          s2 += __shfl_xor(s2, off);
+          s3 += __shfl_xor(s3, off);
+        }
+      }
+      if ((lane & 15) == 0) {
+        pslot[m] = s0;
+        if (gga) {
+          pslot[ld + m] = s1;
+          pslot[2 * ld + m] = s2;
+          pslot[3 * ld + m] = s3;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_zmat_f32(const WorkItem* __restrict__ pblk,
+                                                  const DevTask* __restrict__ tasks,
+                                                  const double* __restrict__ zcoef, int64_t nq,
+                                                  float* __restrict__ scratch, int gga) {
+  const WorkItem wi = pblk[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = wi.a * kPtBlock + lane;
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  const double* zc = zcoef + t.q_off + p;
+  const double a = zc[0];
+  const double cx = gga ? zc[nq] : 0., cy = gga ? zc[2 * nq] : 0., cz = gga ? zc[3 * nq] : 0.;
+  float* B = scratch + t.b_off + p;
+  float* Z = B + 4 * plane;
+  for (int j = wave; j < t.nbe; j += 4) {
+    const int64_t o = (int64_t)j * ld;
+    double z = a * (double)B[o];
+    if (gga)
+      z += cx * (double)B[plane + o] + cy * (double)B[2 * plane + o] +
+           cz * (double)B[3 * plane + o];
+    Z[o] = (float)z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ tiles,
+                                                  const DevTask* __restrict__ tasks,
+                                                  const int32_t* __restrict__ bfmap,
+                                                  const float* __restrict__ scratch,
+                                                  double* __restrict__ vxc, int64_t ldv) {
+  __shared__ __attribute__((aligned(16))) unsigned short Bi[3][64][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Zi[3][64][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Bj[3][64][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Zj[3][64][kRow];
+  const WorkItem wi = tiles[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int i0 = wi.a * kTile, j0 = wi.b * kTile;
+  const bool diag = wi.a == wi.b;
+  const int64_t ld = t.npts_pad;
+  const int nbe = t.nbe;
+  const int64_t plane = (int64_t)nbe * ld;
+  const float* Bm = scratch + t.b_off;
+  const float* Zm = Bm + 4 * plane;
+  const int f = tid & 63, pg = tid >> 6;  // loader: function row, point group of 8
+
+  double acc[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
+
+  auto stage = [&](const float* src, int row0, unsigned short (*dst)[64][kRow], int k0) {
+    us8 s1, s2, s3;
+    const int row = row0 + f;
+    const float* p = src + (int64_t)row * ld + k0 + pg * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = row < nbe ? p[e] : 0.f;
+      unsigned short a, b, c;
+      split3f(x, a, b, c);
+      s1[e] = a;
+      s2[e] = b;
+      s3[e] = c;
+    }
+    *reinterpret_cast<us8*>(&dst[0][f][pg * 8]) = s1;
+    *reinterpret_cast<us8*>(&dst[1][f][pg * 8]) = s2;
+    *reinterpret_cast<us8*>(&dst[2][f][pg * 8]) = s3;
+  };
+
+  for (int k0 = 0; k0 < ld; k0 += kKB) {
+    stage(Bm, i0, Bi, k0);
+    stage(Zm, i0, Zi, k0);
+    if (!diag) {
+      stage(Bm, j0, Bj, k0);
+      stage(Zm, j0, Zj, k0);
+    }
+    __syncthreads();
+    unsigned short(*BJ)[64][kRow] = diag ? Bi : Bj;
+    unsigned short(*ZJ)[64][kRow] = diag ? Zi : Zj;
+    const int kk = 8 * (lane >> 4);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int ra = wm * 32 + bi * 16 + (lane & 15);
+        const int rb = wn * 32 + bj * 16 + (lane & 15);
+        v4f c = {0.f, 0.f, 0.f, 0.f};
+        // B_i^T Z_j
+        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[0][rb][kk], c);
+        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[1][rb][kk], c);
+        c = mfma_bf16(&Bi[1][ra][kk], &ZJ[0][rb][kk], c);
+        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[2][rb][kk], c);
+        c = mfma_bf16(&Bi[1][ra][kk], &ZJ[1][rb][kk], c);
+        c = mfma_bf16(&Bi[2][ra][kk], &ZJ[0][rb][kk], c);
+        // Z_i^T B_j
+        c = mfma_bf16(&Zi[0][ra][kk], &BJ[0][rb][kk], c);
+        c = mfma_bf16(&Zi[0][ra][kk], &BJ[1][rb][kk], c);
+        c = mfma_bf16(&Zi[1][ra][kk], &BJ[0][rb][kk], c);
+        c = mfma_bf16(&Zi[0][ra][kk], &BJ[2][rb][kk], c);
+        c = mfma_bf16(&Zi[1][ra][kk], &BJ[1][rb][kk], c);
+        c = mfma_bf16(&Zi[2][ra][kk], &BJ[0][rb][kk], c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[r];
+      }
+    __syncthreads();
+  }
+
+  const int32_t* bf = bfmap + t.bf_off;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int j = j0 + wn * 32 + bj * 16 + (lane & 15);
+      if (j >= nbe) continue;
+      const int64_t cj = (int64_t)bf[j] * ldv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        if (i < nbe && (!diag || i >= j)) atomicAdd(vxc + bf[i] + cj, acc[bi][bj][r]);
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+int launch_mixed_pipeline(struct ::xcq_context_s* ctx, const Chunk& c, int func, int gga,
+                          int want_vxc, hipEvent_t* ev) {
+  const CtxView v = view_of(ctx);
+  hipStream_t s = v.stream;
+  const int64_t nel = (int64_t)v.nbf * v.nbf;
+  if (*v.Psplit == nullptr) {
+    XCQ_HIP(hipMalloc(v.Psplit, (size_t)nel * sizeof(uint2)));
+    hipLaunchKernelGGL(k_split_P, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, s, nel,
+                       (const double*)v.P, (uint2*)*v.Psplit);
+    XCQ_HIP(hipGetLastError());
+  }
+  float* scr = static_cast<float*>(v.scratch);
+  XCQ_HIP(hipEventRecord(ev[0], s));
+  int rc = launch_collocation_fp32(s, c, v.work, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
+                                   v.py, v.pz, scr, gga);
+  if (rc) return rc;
+  XCQ_HIP(hipEventRecord(ev[1], s));
+  XCQ_HIP(hipEventRecord(ev[2], s));
+  if (c.ng1) {
+    hipLaunchKernelGGL(k_xmat_den_mx, dim3((unsigned)c.ng1), dim3(256), 0, s, v.work + c.g1_off,
+                       v.tasks, v.bfmap, (const uint2*)*v.Psplit, (int64_t)v.nbf, scr, v.part,
+                       gga, func == 0 ? 1.0f : 2.0f);
+    XCQ_HIP(hipGetLastError());
+  }
+  XCQ_HIP(hipEventRecord(ev[3], s));
+  XCQ_HIP(hipEventRecord(ev[4], s));
+  rc = launch_functional(s, c, v.work, v.tasks, v.part, v.w, func, gga,
+                         want_vxc ? v.zcoef : nullptr, v.red);
+  if (rc) return rc;
+  XCQ_HIP(hipEventRecord(ev[5], s));
+  if (!want_vxc) return XCQ_SUCCESS;
+  XCQ_HIP(hipEventRecord(ev[6], s));
+  if (c.npblk) {
+    hipLaunchKernelGGL(k_zmat_f32, dim3((unsigned)c.npblk), dim3(256), 0, s, v.work + c.pblk_off,
+                       v.tasks, v.zcoef, c.nq, scr, gga);
+    XCQ_HIP(hipGetLastError());
+  }
+  XCQ_HIP(hipEventRecord(ev[7], s));
+  XCQ_HIP(hipEventRecord(ev[8], s));
+  if (c.ng2) {
+    hipLaunchKernelGGL(k_syr2k_mx, dim3((unsigned)c.ng2), dim3(256), 0, s, v.work + c.g2_off,
+                       v.tasks, v.bfmap, scr, v.vxc, (int64_t)v.nbf);
+    XCQ_HIP(hipGetLastError());
+  }
+  XCQ_HIP(hipEventRecord(ev[9], s));
+  return XCQ_SUCCESS;
 }
 
 }  // namespace xcq

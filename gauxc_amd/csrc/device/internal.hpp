@@ -82,8 +82,28 @@ struct Chunk {
   int64_t g2_off, ng2;        // SYR2K tiles (task, itile, jtile) with itile >= jtile
 };
 
+// View of a context's device buffers for the mixed-precision translation
+// unit (defined in context.hip).
+struct CtxView {
+  hipStream_t stream;
+  const DevTask* tasks;
+  const DevShell* shells;
+  const int32_t *shell_list, *bfmap, *lsh;
+  const double *px, *py, *pz, *w;
+  const WorkItem* work;
+  double* P;
+  double* vxc;
+  int32_t nbf;
+  double* red;
+  double* part;
+  double* zcoef;
+  void* scratch;
+  size_t scratch_bytes;
+  void** Psplit;  // nbf x nbf packed (p1, p2, p3, 0) bfloat16 slices of P
+};
+CtxView view_of(struct ::xcq_context_s* c);
+
 // ---- kernel launchers (defined in the .hip files) ---------------------------
-struct Context;
 
 int upload_sph_table(hipStream_t s);
 int launch_ssf(hipStream_t s, int32_t natoms, const double* atom_xyz, const double* rab,
