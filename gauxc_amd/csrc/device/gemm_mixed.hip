@@ -52,11 +52,21 @@ __device__ __forceinline__ void split3f(float x, unsigned short& a, unsigned sho
   c = bf_bits(h3);
 }
 
-__device__ __forceinline__ v4f mfma_bf16(const unsigned short* a, const unsigned short* b,
-                                         v4f c) {
-  const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(a));
-  const bf16x8 bv = __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(b));
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+__device__ __forceinline__ bf16x8 ldfrag(const unsigned short* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(p));
+}
+
+__device__ __forceinline__ v4f mfma_v(bf16x8 a, bf16x8 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Slice pairs (s, t) with s + t <= 2, largest terms first:
+// (0,0) (0,1) (1,0) (0,2) (1,1) (2,0).
+__device__ __forceinline__ constexpr int pair_a(int p) {
+  return p == 2 ? 1 : p == 4 ? 1 : p == 5 ? 2 : 0;
+}
+__device__ __forceinline__ constexpr int pair_b(int p) {
+  return p == 1 ? 1 : p == 3 ? 2 : p == 4 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -143,23 +153,37 @@ __global__ __launch_bounds__(256) void k_xmat_den_mx(const WorkItem* __restrict_
     }
     __syncthreads();
     const int kk = 8 * (lane >> 4);
+    // Operand fragments are read from LDS once per K block and reused by
+    // every slice product; the six products run over four independent
+    // accumulators so consecutive MFMAs do not wait on each other.
+    bf16x8 fa[2][3], fb[2][3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        fa[q][s] = ldfrag(&As[s][wm * 32 + q * 16 + (lane & 15)][kk]);
+        fb[q][s] = ldfrag(&Bs[s][wn * 32 + q * 16 + (lane & 15)][kk]);
+      }
+    __syncthreads();
+    v4f c[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const int ra = wm * 32 + bi * 16 + (lane & 15);
-        const int rb = wn * 32 + bj * 16 + (lane & 15);
-        v4f c = {0.f, 0.f, 0.f, 0.f};
-        c = mfma_bf16(&As[0][ra][kk], &Bs[0][rb][kk], c);
-        c = mfma_bf16(&As[0][ra][kk], &Bs[1][rb][kk], c);
-        c = mfma_bf16(&As[1][ra][kk], &Bs[0][rb][kk], c);
-        c = mfma_bf16(&As[0][ra][kk], &Bs[2][rb][kk], c);
-        c = mfma_bf16(&As[1][ra][kk], &Bs[1][rb][kk], c);
-        c = mfma_bf16(&As[2][ra][kk], &Bs[0][rb][kk], c);
+      for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[r];
-      }
-    __syncthreads();
+    for (int pr = 0; pr < 6; ++pr) {
+      const int sa = pair_a(pr), sb = pair_b(pr);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) c[bi][bj] = mfma_v(fa[bi][sa], fb[bj][sb], c[bi][bj]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
   }
 
   const int64_t plane = (int64_t)nbe * ld;
@@ -293,31 +317,41 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ t
     unsigned short(*BJ)[64][kRow] = diag ? Bi : Bj;
     unsigned short(*ZJ)[64][kRow] = diag ? Zi : Zj;
     const int kk = 8 * (lane >> 4);
+    bf16x8 fbi[2][3], fzi[2][3], fbj[2][3], fzj[2][3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int ra = wm * 32 + q * 16 + (lane & 15);
+        const int rb = wn * 32 + q * 16 + (lane & 15);
+        fbi[q][s] = ldfrag(&Bi[s][ra][kk]);
+        fzi[q][s] = ldfrag(&Zi[s][ra][kk]);
+        fbj[q][s] = ldfrag(&BJ[s][rb][kk]);
+        fzj[q][s] = ldfrag(&ZJ[s][rb][kk]);
+      }
+    __syncthreads();
+    v4f c[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const int ra = wm * 32 + bi * 16 + (lane & 15);
-        const int rb = wn * 32 + bj * 16 + (lane & 15);
-        v4f c = {0.f, 0.f, 0.f, 0.f};
-        // B_i^T Z_j
-        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[0][rb][kk], c);
-        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[1][rb][kk], c);
-        c = mfma_bf16(&Bi[1][ra][kk], &ZJ[0][rb][kk], c);
-        c = mfma_bf16(&Bi[0][ra][kk], &ZJ[2][rb][kk], c);
-        c = mfma_bf16(&Bi[1][ra][kk], &ZJ[1][rb][kk], c);
-        c = mfma_bf16(&Bi[2][ra][kk], &ZJ[0][rb][kk], c);
-        // Z_i^T B_j
-        c = mfma_bf16(&Zi[0][ra][kk], &BJ[0][rb][kk], c);
-        c = mfma_bf16(&Zi[0][ra][kk], &BJ[1][rb][kk], c);
-        c = mfma_bf16(&Zi[1][ra][kk], &BJ[0][rb][kk], c);
-        c = mfma_bf16(&Zi[0][ra][kk], &BJ[2][rb][kk], c);
-        c = mfma_bf16(&Zi[1][ra][kk], &BJ[1][rb][kk], c);
-        c = mfma_bf16(&Zi[2][ra][kk], &BJ[0][rb][kk], c);
+      for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[r];
-      }
-    __syncthreads();
+    for (int pr = 0; pr < 6; ++pr) {
+      const int sa = pair_a(pr), sb = pair_b(pr);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          c[bi][bj] = mfma_v(fbi[bi][sa], fzj[bj][sb], c[bi][bj]);  // B_i^T Z_j
+          c[bi][bj] = mfma_v(fzi[bi][sa], fbj[bj][sb], c[bi][bj]);  // Z_i^T B_j
+        }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
   }
 
   const int32_t* bf = bfmap + t.bf_off;
