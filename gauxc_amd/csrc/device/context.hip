@@ -5,11 +5,11 @@
 //   shells      DevShell[nshells]
 //   px/py/pz/w  SoA points and weights of all tasks, in upload order
 //   shell list, local->global basis map, local shell -> first function
-//   work lists  (task, point block), GEMM1 tiles, SYR2K tiles per chunk
+//   work lists  (task, point block) and SYR2K tiles per chunk
 //   P, VXC      nbf x nbf, column-major, ld = nbf
 //   red         (w eps rho, w rho) per point block -> deterministic sums
 // Chunk scratch (reused by every chunk): per task [5][nbe][npts_pad]
-// (B, dB/dx, dB/dy, dB/dz, Z), density partials, Z coefficients.  Tasks are
+// (B, dB/dx, dB/dy, dB/dz, Z) and the Z coefficients.  Tasks are
 // grouped greedily into chunks that fit the arena; each chunk is one launch
 // per stage, so every launch carries thousands of workgroups.
 #include "internal.hpp"
@@ -93,9 +93,8 @@ struct xcq_context_s {
   // chunk scratch
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
-  double* d_part = nullptr;
   double* d_zcoef = nullptr;
-  int64_t max_part = 0, max_nq = 0, max_scratch_elems = 0;
+  int64_t max_nq = 0, max_scratch_elems = 0;
 
   // packed bfloat16 slices of P for the mixed-precision contractions
   void* d_Phi = nullptr;
@@ -124,7 +123,6 @@ struct xcq_context_s {
     dfree(d_red);
     dfree(d_out2);
     if (d_scratch) (void)hipFree(d_scratch);
-    dfree(d_part);
     dfree(d_zcoef);
     if (d_Phi) (void)hipFree(d_Phi);
     if (stream) (void)hipStreamDestroy(stream);
@@ -135,7 +133,7 @@ namespace xcq {
 CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream, c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap, c->d_lsh,
                  c->d_px,   c->d_py,    c->d_pz,     c->d_w,          c->d_work,  c->d_P,
-                 c->d_vxc,  c->nbf,     c->d_red,    c->d_part,       c->d_zcoef, c->d_scratch,
+                 c->d_vxc,  c->nbf,     c->d_red,    c->d_zcoef, c->d_scratch,
                  c->scratch_bytes, &c->d_Phi};
 }
 }  // namespace xcq
@@ -311,11 +309,10 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
     budget = (size_t)std::max(0.0, std::min(avail * 0.5, 48.0 * (1ull << 30)));
   }
   auto task_bytes = [](const DevTask& d) {
-    return (size_t)5 * d.nbe * d.npts_pad * 8 + (size_t)2 * d.ntile * 4 * d.npts_pad * 8 +
-           (size_t)4 * d.npts_pad * 8;
+    return (size_t)5 * d.nbe * d.npts_pad * 8 + (size_t)4 * d.npts_pad * 8;
   };
   std::vector<Chunk> chunks;
-  std::vector<WorkItem> pbl, g1, g2;
+  std::vector<WorkItem> pbl, g2;
   {
     int32_t i = 0;
     while (i < ntasks) {
@@ -323,9 +320,8 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
       std::memset(&c, 0, sizeof(c));
       c.task_begin = i;
       size_t used = 0;
-      int64_t se = 0, nq = 0, np = 0;
+      int64_t se = 0, nq = 0;
       c.pblk_off = (int64_t)pbl.size();
-      c.g1_off = (int64_t)g1.size();
       c.g2_off = (int64_t)g2.size();
       while (i < ntasks) {
         auto& d = ht[i];
@@ -338,14 +334,10 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
         used += b;
         d.b_off = se;
         d.q_off = nq;
-        d.part_off = np;
         se += (int64_t)5 * d.nbe * d.npts_pad;
         nq += d.npts_pad;
-        np += (int64_t)2 * d.ntile * 4 * d.npts_pad;
         const int nmb = d.npts_pad / kPtBlock;
         for (int pb = 0; pb < nmb; ++pb) pbl.push_back(WorkItem{i, pb, 0, 0});
-        for (int mt = 0; mt < d.npts_pad / kTile; ++mt)
-          for (int nt = 0; nt < d.ntile; ++nt) g1.push_back(WorkItem{i, mt, nt, 0});
         for (int it = 0; it < d.ntile; ++it)
           for (int jt = 0; jt <= it; ++jt) g2.push_back(WorkItem{i, it, jt, 0});
         ++i;
@@ -353,9 +345,7 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
       c.task_end = i;
       c.scratch_elems = se;
       c.nq = nq;
-      c.npart = np;
       c.npblk = (int64_t)pbl.size() - c.pblk_off;
-      c.ng1 = (int64_t)g1.size() - c.g1_off;
       c.ng2 = (int64_t)g2.size() - c.g2_off;
       chunks.push_back(c);
     }
@@ -372,7 +362,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   dfree(ctx->d_lsh);
   dfree(ctx->d_work);
   dfree(ctx->d_red);
-  dfree(ctx->d_part);
   dfree(ctx->d_zcoef);
   if (ctx->d_scratch) {
     (void)hipFree(ctx->d_scratch);
@@ -382,11 +371,10 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   ctx->ev.clear();
   ctx->ntasks = 0;
 
-  int64_t max_se = 0, max_nq = 0, max_np = 0;
+  int64_t max_se = 0, max_nq = 0;
   for (const auto& c : chunks) {
     max_se = std::max(max_se, c.scratch_elems);
     max_nq = std::max(max_nq, c.nq);
-    max_np = std::max(max_np, c.npart);
   }
   std::vector<double> hx(npts_total), hy(npts_total), hz(npts_total);
   for (int64_t k = 0; k < npts_total; ++k) {
@@ -395,17 +383,11 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
     hz[k] = points[3 * k + 2];
   }
   std::vector<WorkItem> work;
-  work.reserve(pbl.size() + g1.size() + g2.size());
-  const int64_t base_p = 0, base_g1 = (int64_t)pbl.size(),
-                base_g2 = (int64_t)(pbl.size() + g1.size());
+  work.reserve(pbl.size() + g2.size());
+  const int64_t base_g2 = (int64_t)pbl.size();
   work.insert(work.end(), pbl.begin(), pbl.end());
-  work.insert(work.end(), g1.begin(), g1.end());
   work.insert(work.end(), g2.begin(), g2.end());
-  for (auto& c : chunks) {
-    c.pblk_off += base_p;
-    c.g1_off += base_g1;
-    c.g2_off += base_g2;
-  }
+  for (auto& c : chunks) c.g2_off += base_g2;
 
   int rc;
   if ((rc = dalloc(&ctx->d_tasks, ht.size()))) return rc;
@@ -418,7 +400,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
   if ((rc = dalloc(&ctx->d_work, work.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
-  if ((rc = dalloc(&ctx->d_part, (size_t)max_np))) return rc;
   if ((rc = dalloc(&ctx->d_zcoef, (size_t)4 * max_nq))) return rc;
   ctx->scratch_bytes = (size_t)max_se * sizeof(double);
   if (ctx->scratch_bytes) {
@@ -450,7 +431,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   ctx->ntasks = ntasks;
   ctx->npts_total = npts_total;
   ctx->total_pblocks = pblocks;
-  ctx->max_part = max_np;
   ctx->max_nq = max_nq;
   ctx->max_scratch_elems = max_se;
   int64_t sum_pts = 0, sum_pnn = 0;
@@ -503,7 +483,7 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
       rc = launch_mixed_pipeline(ctx, c, func, gga, want_vxc, ev);
       if (rc) return rc;
       launches += want_vxc ? 5 : 3;
-      tiles += c.ng1 + (want_vxc ? c.ng2 : 0);
+      tiles += c.npblk + (want_vxc ? c.ng2 : 0);
       continue;
     }
     double* scr = static_cast<double*>(ctx->d_scratch);
@@ -514,17 +494,17 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
     if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[1], s));
     XCQ_HIP(hipEventRecord(ev[2], s));
-    rc = launch_xmat_den_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, ctx->d_P, ctx->nbf,
-                              scr, ctx->d_part, gga, func == 0 ? 1.0 : 2.0);
+    // X, rho/grad rho, functional and EXC/NEL in one launch (stage 1);
+    // stage 2 stays as an empty interval so the stage table keeps its shape.
+    rc = launch_density_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, ctx->d_P, ctx->nbf,
+                             scr, ctx->d_w, func, gga, func == 0 ? 1.0 : 2.0,
+                             want_vxc ? ctx->d_zcoef : nullptr, ctx->d_red);
     if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[3], s));
     XCQ_HIP(hipEventRecord(ev[4], s));
-    rc = launch_functional(s, c, ctx->d_work, ctx->d_tasks, ctx->d_part, ctx->d_w, func, gga,
-                           want_vxc ? ctx->d_zcoef : nullptr, ctx->d_red);
-    if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[5], s));
-    launches += 3;
-    tiles += c.ng1;
+    launches += 2;
+    tiles += c.npblk;
     if (want_vxc) {
       XCQ_HIP(hipEventRecord(ev[6], s));
       rc = launch_zmat_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_zcoef, scr, gga);

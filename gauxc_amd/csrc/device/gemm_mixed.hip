@@ -13,10 +13,9 @@
 //    the contraction length (npts can reach 2e4 in the B^T Z product).
 //  * Densities, the functional, EXC/NEL and VXC accumulate in FP64.
 //
-// Kernels mirror the FP64 ones (gemm_fp64.hip): k_xmat_den_mx fuses the
-// per-point density reduction into the X = fac B P_sub epilogue,
-// k_syr2k_mx fuses the scatter-add into VXC into the B^T Z + Z^T B
-// epilogue.  Operand layout of the MFMA (measured on gfx950): lane l holds
+// Kernels mirror the FP64 ones (gemm_fp64.hip; the density stage is in
+// density.hip): k_syr2k_mx fuses the scatter-add into VXC into the
+// B^T Z + Z^T B epilogue.  Operand layout of the MFMA (measured on gfx950): lane l holds
 // A[l%16][8(l/16) .. 8(l/16)+7] and B[8(l/16) .. +7][l%16]; accumulator
 // register r of lane l is D[4(l/16)+r][l%16].
 #include "internal.hpp"
@@ -85,149 +84,6 @@ __global__ __launch_bounds__(256) void k_split_P(int64_t n, const double* __rest
   v.x = (unsigned)bf_bits(h1) | ((unsigned)bf_bits(h2) << 16);
   v.y = (unsigned)bf_bits(h3);
   out[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_xmat_den_mx(const WorkItem* __restrict__ tiles,
-                                                     const DevTask* __restrict__ tasks,
-                                                     const int32_t* __restrict__ bfmap,
-                                                     const uint2* __restrict__ Ps, int64_t ldp,
-                                                     const float* __restrict__ scratch,
-                                                     double* __restrict__ part, int gga,
-                                                     float fac) {
-  __shared__ __attribute__((aligned(16))) unsigned short As[3][64][kRow];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[3][64][kRow];
-  const WorkItem wi = tiles[blockIdx.x];
-  const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = wi.a * kTile, n0 = wi.b * kTile;
-  const int64_t ld = t.npts_pad;
-  const int nbe = t.nbe;
-  const float* Bm = scratch + t.b_off;
-  const int32_t* bf = bfmap + t.bf_off;
-
-  const int r64 = tid & 63, kg = tid >> 6;  // loader: row (point / function), k group of 8
-  const int ncol = n0 + r64;
-  const int64_t colP = ncol < nbe ? (int64_t)bf[ncol] * ldp : -1;
-
-  double acc[2][2][4];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
-
-  for (int k0 = 0; k0 < nbe; k0 += kKB) {
-    {
-      us8 s1, s2, s3;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + kg * 8 + e;
-        const float x = k < nbe ? Bm[(int64_t)k * ld + m0 + r64] : 0.f;
-        unsigned short a, b, c;
-        split3f(x, a, b, c);
-        s1[e] = a;
-        s2[e] = b;
-        s3[e] = c;
-      }
-      *reinterpret_cast<us8*>(&As[0][r64][kg * 8]) = s1;
-      *reinterpret_cast<us8*>(&As[1][r64][kg * 8]) = s2;
-      *reinterpret_cast<us8*>(&As[2][r64][kg * 8]) = s3;
-    }
-    {
-      us8 s1, s2, s3;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + kg * 8 + e;
-        uint2 v = make_uint2(0u, 0u);
-        if (k < nbe && colP >= 0) v = Ps[(int64_t)bf[k] + colP];
-        s1[e] = (unsigned short)(v.x & 0xffffu);
-        s2[e] = (unsigned short)(v.x >> 16);
-        s3[e] = (unsigned short)(v.y & 0xffffu);
-      }
-      *reinterpret_cast<us8*>(&Bs[0][r64][kg * 8]) = s1;
-      *reinterpret_cast<us8*>(&Bs[1][r64][kg * 8]) = s2;
-      *reinterpret_cast<us8*>(&Bs[2][r64][kg * 8]) = s3;
-    }
-    __syncthreads();
-    const int kk = 8 * (lane >> 4);
-    // Operand fragments are read from LDS once per K block and reused by
-    // every slice product; the six products run over four independent
-    // accumulators so consecutive MFMAs do not wait on each other.
-    bf16x8 fa[2][3], fb[2][3];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        fa[q][s] = ldfrag(&As[s][wm * 32 + q * 16 + (lane & 15)][kk]);
-        fb[q][s] = ldfrag(&Bs[s][wn * 32 + q * 16 + (lane & 15)][kk]);
-      }
-    __syncthreads();
-    v4f c[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int pr = 0; pr < 6; ++pr) {
-      const int sa = pair_a(pr), sb = pair_b(pr);
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) c[bi][bj] = mfma_v(fa[bi][sa], fb[bj][sb], c[bi][bj]);
-    }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
-  }
-
-  const int64_t plane = (int64_t)nbe * ld;
-  double* pslot = part + t.part_off + (int64_t)(2 * wi.b + wn) * 4 * ld;
-  const double dfac = fac;
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
-      double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const int n = n0 + wn * 32 + bj * 16 + (lane & 15);
-        if (n < nbe) {
-          const double x = dfac * acc[bi][bj][r];
-          const int64_t o = (int64_t)n * ld + m;
-          s0 += (double)Bm[o] * x;
-          if (gga) {
-            s1 += (double)Bm[plane + o] * x;
-            s2 += (double)Bm[2 * plane + o] * x;
-            s3 += (double)Bm[3 * plane + o] * x;
-          }
-        }
-      }
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        s0 += __shfl_xor(s0, off);
-        if (gga) {
-          s1 += __shfl_xor(s1, off);
-          s2 += __shfl_xor(s2, off);
-          s3 += __shfl_xor(s3, off);
-        }
-      }
-      if ((lane & 15) == 0) {
-        pslot[m] = s0;
-        if (gga) {
-          pslot[ld + m] = s1;
-          pslot[2 * ld + m] = s2;
-          pslot[3 * ld + m] = s3;
-        }
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -389,17 +245,12 @@ int launch_mixed_pipeline(struct ::xcq_context_s* ctx, const Chunk& c, int func,
   if (rc) return rc;
   XCQ_HIP(hipEventRecord(ev[1], s));
   XCQ_HIP(hipEventRecord(ev[2], s));
-  if (c.ng1) {
-    hipLaunchKernelGGL(k_xmat_den_mx, dim3((unsigned)c.ng1), dim3(256), 0, s, v.work + c.g1_off,
-                       v.tasks, v.bfmap, (const uint2*)*v.Psplit, (int64_t)v.nbf, scr, v.part,
-                       gga, func == 0 ? 1.0f : 2.0f);
-    XCQ_HIP(hipGetLastError());
-  }
+  rc = launch_density_mixed(s, c, v.work, v.tasks, v.bfmap, (const uint2*)*v.Psplit,
+                            (int64_t)v.nbf, scr, v.w, func, gga, func == 0 ? 1.0 : 2.0,
+                            want_vxc ? v.zcoef : nullptr, v.red);
+  if (rc) return rc;
   XCQ_HIP(hipEventRecord(ev[3], s));
   XCQ_HIP(hipEventRecord(ev[4], s));
-  rc = launch_functional(s, c, v.work, v.tasks, v.part, v.w, func, gga,
-                         want_vxc ? v.zcoef : nullptr, v.red);
-  if (rc) return rc;
   XCQ_HIP(hipEventRecord(ev[5], s));
   if (!want_vxc) return XCQ_SUCCESS;
   XCQ_HIP(hipEventRecord(ev[6], s));

@@ -55,7 +55,6 @@ struct DevTask {
   int64_t lsh_off;    // local shell -> first local basis function (nshells entries)
   int64_t b_off;      // chunk scratch: [5][nbe][npts_pad] (B, Bx, By, Bz, Z) in elements
   int64_t q_off;      // chunk per-point arrays (npts_pad entries)
-  int64_t part_off;   // chunk density partials: [2*ntile][4][npts_pad]
   int64_t red_off;    // global index of the task's first point block (EXC/NEL partials)
 };
 
@@ -63,22 +62,12 @@ struct WorkItem {  // (task, a, b): a/b are point-block or tile indices
   int32_t task, a, b, pad_;
 };
 
-// Spherical-harmonic transformation table: for shell l, rows m = 0..2l,
-// each a short list of (cartesian index, coefficient) pairs.
-struct SphTable {
-  int32_t start[kMaxL + 1][2 * kMaxL + 2];  // start[l][m] .. start[l][m+1]
-  int32_t cart[256];
-  double coef[256];
-};
-
 struct Chunk {
   int32_t task_begin, task_end;
   int64_t scratch_elems;  // elements of T (FP64 or FP32) used by B-family arrays
   int64_t nq;             // per-point array length
-  int64_t npart;          // density partial length
   // work lists (offsets into the concatenated device lists)
-  int64_t pblk_off, npblk;    // (task, point block)
-  int64_t g1_off, ng1;        // GEMM1 tiles (task, mtile, ntile)
+  int64_t pblk_off, npblk;    // (task, point block): collocation, density, Z
   int64_t g2_off, ng2;        // SYR2K tiles (task, itile, jtile) with itile >= jtile
 };
 
@@ -95,7 +84,6 @@ struct CtxView {
   double* vxc;
   int32_t nbf;
   double* red;
-  double* part;
   double* zcoef;
   void* scratch;
   size_t scratch_bytes;
@@ -113,12 +101,14 @@ int launch_collocation_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                             const DevTask* tasks, const DevShell* shells, const int32_t* shell_list,
                             const int32_t* lsh_first, const double* px, const double* py,
                             const double* pz, double* scratch, int deriv);
-int launch_xmat_den_fp64(hipStream_t s, const Chunk& c, const WorkItem* g1, const DevTask* tasks,
-                         const int32_t* bfmap, const double* P, int64_t ldp,
-                         const double* scratch, double* part, int gga, double fac);
-int launch_functional(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
-                      const double* part, const double* weights, int func, int gga,
-                      double* zcoef, double* red);
+int launch_density_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
+                        const DevTask* tasks, const int32_t* bfmap, const double* P,
+                        int64_t ldp, const double* scratch, const double* weights, int func,
+                        int gga, double fac, double* zcoef, double* red);
+int launch_density_mixed(hipStream_t s, const Chunk& c, const WorkItem* pblk,
+                         const DevTask* tasks, const int32_t* bfmap, const uint2* Ps,
+                         int64_t ldp, const float* scratch, const double* weights, int func,
+                         int gga, double fac, double* zcoef, double* red);
 int launch_zmat_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
                      const double* zcoef, double* scratch, int gga);
 int launch_syr2k_scatter_fp64(hipStream_t s, const Chunk& c, const WorkItem* g2,

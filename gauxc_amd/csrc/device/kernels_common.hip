@@ -77,86 +77,6 @@ int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* r
   return XCQ_SUCCESS;
 }
 
-// ---------------------------------------------------------------------------
-// Functional + weight folding per point block (64 points, one wave).
-// Density partials written by the X/density kernel are summed in a fixed
-// order; grad rho = 2 sum dB X (eval_uvvar_gga_rks,
-// reference_local_host_work_driver.cxx:242-268).  Outputs the Z-matrix
-// coefficients  a = w vrho / 2,  c = 2 w vsigma grad rho
-// (eval_zmat_gga_vxc_rks, :678-713) and per-block sums of w eps rho and
-// w rho (host_integrator_exc_vxc.hpp:452-502).  func = 0: density only.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_functional(const WorkItem* __restrict__ pblk,
-                                                   const DevTask* __restrict__ tasks,
-                                                   const double* __restrict__ part,
-                                                   const double* __restrict__ weights,
-                                                   int func, int gga, int64_t nq,
-                                                   double* __restrict__ zcoef,
-                                                   double* __restrict__ red) {
-  const WorkItem wi = pblk[blockIdx.x];
-  const DevTask t = tasks[wi.task];
-  const int lane = threadIdx.x;
-  const int p = wi.a * kPtBlock + lane;
-  const int64_t ld = t.npts_pad;
-  const double* pp = part + t.part_off;
-  double rho = 0., gx = 0., gy = 0., gz = 0.;
-  const int nslot = 2 * t.ntile;
-  for (int sl = 0; sl < nslot; ++sl) {
-    const double* q = pp + (int64_t)sl * 4 * ld;
-    rho += q[p];
-    if (gga) {
-      gx += q[ld + p];
-      gy += q[2 * ld + p];
-      gz += q[3 * ld + p];
-    }
-  }
-  gx *= 2.0;
-  gy *= 2.0;
-  gz *= 2.0;
-  double e = 0., n = 0., za = 0., zx = 0., zy = 0., zz = 0.;
-  if (p < t.npts) {
-    const double w = weights[t.pt_off + p];
-    n = w * rho;
-    if (func != 0) {
-      const double sigma = gx * gx + gy * gy + gz * gz;
-      const XCOut o = dev_eval_functional(func, rho, sigma);
-      e = w * o.e;
-      za = 0.5 * w * o.vr;
-      const double c = 2.0 * w * o.vs;
-      zx = c * gx;
-      zy = c * gy;
-      zz = c * gz;
-    }
-  }
-  if (zcoef) {
-    double* zc = zcoef + t.q_off + p;
-    zc[0] = za;
-    if (gga) {
-      zc[nq] = zx;
-      zc[2 * nq] = zy;
-      zc[3 * nq] = zz;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    e += __shfl_xor(e, off);
-    n += __shfl_xor(n, off);
-  }
-  if (lane == 0) {
-    red[2 * (t.red_off + wi.a)] = e;
-    red[2 * (t.red_off + wi.a) + 1] = n;
-  }
-}
-
-int launch_functional(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
-                      const double* part, const double* weights, int func, int gga,
-                      double* zcoef, double* red) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_functional, dim3((unsigned)c.npblk), dim3(64), 0, s, pblk + c.pblk_off,
-                     tasks, part, weights, func, gga, c.nq, zcoef, red);
-  XCQ_HIP(hipGetLastError());
-  return XCQ_SUCCESS;
-}
-
 // Sum of (e, n) pairs in a fixed order: deterministic EXC and NEL.
 __global__ __launch_bounds__(256) void k_reduce_sums(const double* __restrict__ red,
                                                      int64_t nb, double* __restrict__ out) {
