@@ -2,19 +2,18 @@
 // functional with weight folding, and the EXC/NEL block sums, in one launch.
 //
 // One 256-thread workgroup owns one (task, 64-point block) and walks every
-// 64-function column tile of the task; per column tile it accumulates the
-// 64x64 X tile over 16- (FP64) or 32-deep (mixed) K blocks on the matrix
-// cores, then folds the tile into per-point rho and grad rho partials kept
-// in LDS (rho_p = sum_j B_pj X_pj, grad rho_p = 2 sum_j dB_pj X_pj,
-// reference_local_host_work_driver.cxx:242-268).  After the last column
-// tile the block's 64 points are complete: the functional runs in the same
-// workgroup (host_integrator_exc_vxc.hpp:445-502) and writes the Z
-// coefficients (a = w vrho / 2, c = 2 w vsigma grad rho) and the block's
-// (w eps rho, w rho) sums.  X and the per-tile partials never reach HBM.
-//
-// Long-lived workgroups (one per point block instead of one per 64x64
-// tile) let the loads of the next K block overlap the matrix work of the
-// current one across the resident waves.
+// 64-function column tile of the task.  Per tile it forms the transposed
+// product X^T = fac P_sub B^T (64 functions x 64 points) on the matrix
+// cores, so that an accumulator register holds one function of 16
+// consecutive points across a lane group: the epilogue then reads B and
+// grad B rows with coalesced 16-point segments (the planes are stored
+// point-contiguous) and folds
+//   rho_p = sum_j B_pj X_pj,   grad rho_p = 2 sum_j dB_pj X_pj
+// (reference_local_host_work_driver.cxx:242-268) into per-point partials
+// kept in LDS.  After the last tile the block's 64 points are complete: the
+// functional runs in the same workgroup (host_integrator_exc_vxc.hpp:445-502)
+// and writes the Z coefficients (a = w vrho / 2, c = 2 w vsigma grad rho)
+// and the block's (w eps rho, w rho) sums.  X never reaches HBM.
 #include "internal.hpp"
 #include "functional.hpp"
 
@@ -57,26 +56,28 @@ __device__ __forceinline__ constexpr int sl_b(int p) {
   return p == 1 ? 1 : p == 3 ? 2 : p == 4 ? 1 : 0;
 }
 
-// Folds one X tile into the per-point partials held in LDS.
-// rowof(bi, r) gives the tile row of accumulator register r of block bi.
-template <typename T, typename RowOf>
-__device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], RowOf rowof, int m0, int n0,
+// Folds one X^T tile (rows = functions, columns = points) into the
+// per-point partials in LDS.  frow(bi, r) is the tile row (function) of
+// accumulator register r of row block bi for this lane; the column (point)
+// of lane l in column block bj is wn*32 + bj*16 + (l & 15).
+template <typename T, typename FRow>
+__device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow, int m0, int n0,
                                           int wm, int wn, int lane, int nbe, int64_t ld,
                                           int64_t plane, const T* Bm, int gga,
                                           double (*dens)[64][4]) {
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi) {
+  for (int bj = 0; bj < 2; ++bj) {
+    const int pl = wn * 32 + bj * 16 + (lane & 15);
+    const int64_t m = m0 + pl;
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = wm * 32 + bi * 16 + rowof(r);
-      const int m = m0 + row;
-      double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const int n = n0 + wn * 32 + bj * 16 + (lane & 15);
-        if (n < nbe) {
+      for (int r = 0; r < 4; ++r) {
+        const int j = n0 + wm * 32 + bi * 16 + frow(r);
+        if (j < nbe) {
           const double xv = x[bi][bj][r];
-          const int64_t o = (int64_t)n * ld + m;
+          const int64_t o = (int64_t)j * ld + m;
           s0 += (double)Bm[o] * xv;
           if (gga) {
             s1 += (double)Bm[plane + o] * xv;
@@ -85,22 +86,22 @@ __device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], RowOf rowo
           }
         }
       }
+    // Sum over the four 16-lane groups (different functions, same point).
 #pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        s0 += __shfl_xor(s0, off);
-        if (gga) {
-          s1 += __shfl_xor(s1, off);
-          s2 += __shfl_xor(s2, off);
-          s3 += __shfl_xor(s3, off);
-        }
+    for (int off = 16; off < 64; off <<= 1) {
+      s0 += __shfl_xor(s0, off);
+      if (gga) {
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+        s3 += __shfl_xor(s3, off);
       }
-      if ((lane & 15) == 0) {
-        dens[wn][row][0] += s0;
-        if (gga) {
-          dens[wn][row][1] += s1;
-          dens[wn][row][2] += s2;
-          dens[wn][row][3] += s3;
-        }
+    }
+    if (lane < 16) {
+      dens[wm][pl][0] += s0;
+      if (gga) {
+        dens[wm][pl][1] += s1;
+        dens[wm][pl][2] += s2;
+        dens[wm][pl][3] += s3;
       }
     }
   }
@@ -158,14 +159,14 @@ __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
 
 // ---------------------------------------------------------------------------
 // FP64 (v_mfma_f64_16x16x4_f64; accumulator register r of lane l holds row
-// 4r + l/16).
+// 4r + l/16).  Waves: wm = function half, wn = point half of the tile.
 __global__ __launch_bounds__(256) void k_density_f64(
     const WorkItem* __restrict__ pblk, const DevTask* __restrict__ tasks,
     const int32_t* __restrict__ bfmap, const double* __restrict__ P, int64_t ldp,
     const double* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
     double fac, int64_t nq, double* __restrict__ zcoef, double* __restrict__ red) {
-  __shared__ double As[16][kPad64];
-  __shared__ double Bs[16][kPad64];
+  __shared__ double Pt[16][kPad64];  // [k][function]: A operand (P_sub)
+  __shared__ double Bt[16][kPad64];  // [k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
   const WorkItem wi = pblk[blockIdx.x];
   const DevTask t = tasks[wi.task];
@@ -178,9 +179,9 @@ __global__ __launch_bounds__(256) void k_density_f64(
   const double* Bm = scratch + t.b_off;
   const int32_t* bf = bfmap + t.bf_off;
   for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
-  const int a_m = tid & 63, a_k = tid >> 6;
-  const int b_k = tid & 15, b_n = tid >> 4;
-  auto rowof = [&](int r) { return 4 * r + (lane >> 4); };
+  const int a_m = tid & 63, a_k = tid >> 6;  // plane loader: point, k (+4i)
+  const int b_k = tid & 15, b_n = tid >> 4;  // P loader: k, function (+16i)
+  auto frow = [&](int r) { return 4 * r + (lane >> 4); };
 
   for (int n0 = 0; n0 < nbe; n0 += 64) {
     int64_t colP[4];
@@ -198,21 +199,21 @@ __global__ __launch_bounds__(256) void k_density_f64(
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = a_k + 4 * i;
-        As[k][a_m] = (k0 + k < nbe) ? Bm[(int64_t)(k0 + k) * ld + m0 + a_m] : 0.;
+        Bt[k][a_m] = (k0 + k < nbe) ? Bm[(int64_t)(k0 + k) * ld + m0 + a_m] : 0.;
       }
       const int kk = k0 + b_k;
       const int64_t rowP = kk < nbe ? (int64_t)bf[kk] : -1;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        Bs[b_k][b_n + 16 * i] = (rowP >= 0 && colP[i] >= 0) ? fac * P[rowP + colP[i]] : 0.;
+        Pt[b_k][b_n + 16 * i] = (rowP >= 0 && colP[i] >= 0) ? fac * P[rowP + colP[i]] : 0.;
       __syncthreads();
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
-        const double a0 = As[kr][wm * 32 + (lane & 15)];
-        const double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-        const double b0 = Bs[kr][wn * 32 + (lane & 15)];
-        const double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
+        const double a0 = Pt[kr][wm * 32 + (lane & 15)];
+        const double a1 = Pt[kr][wm * 32 + 16 + (lane & 15)];
+        const double b0 = Bt[kr][wn * 32 + (lane & 15)];
+        const double b1 = Bt[kr][wn * 32 + 16 + (lane & 15)];
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(256) void k_density_f64(
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
-    fold_tile<double>(x, rowof, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+    fold_tile<double>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
   }
   __syncthreads();
   finish_points(t, wi.a, tid, dens, weights, func, gga, nq, zcoef, red);
@@ -242,8 +243,8 @@ __global__ __launch_bounds__(256) void k_density_mx(
     const int32_t* __restrict__ bfmap, const uint2* __restrict__ Ps, int64_t ldp,
     const float* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
     double fac, int64_t nq, double* __restrict__ zcoef, double* __restrict__ red) {
-  __shared__ __attribute__((aligned(16))) unsigned short As[3][64][kRowMx];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[3][64][kRowMx];
+  __shared__ __attribute__((aligned(16))) unsigned short Pt[3][64][kRowMx];  // [s][fn][k]
+  __shared__ __attribute__((aligned(16))) unsigned short Bt[3][64][kRowMx];  // [s][pt][k]
   __shared__ double dens[2][64][4];
   const WorkItem wi = pblk[blockIdx.x];
   const DevTask t = tasks[wi.task];
@@ -258,11 +259,13 @@ __global__ __launch_bounds__(256) void k_density_mx(
   for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
   const int r64 = tid & 63, kg = tid >> 6;
   const int kk = 8 * (lane >> 4);
-  auto rowof = [&](int r) { return 4 * (lane >> 4) + r; };
+  auto frow = [&](int r) { return 4 * (lane >> 4) + r; };
 
   for (int n0 = 0; n0 < nbe; n0 += 64) {
+    // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
+    // neighbouring threads (neighbouring n) touch neighbouring columns.
     const int ncol = n0 + r64;
-    const int64_t colP = ncol < nbe ? (int64_t)bf[ncol] * ldp : -1;
+    const int64_t colP = ncol < nbe ? (int64_t)bf[ncol] : -1;
     double acc[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -271,7 +274,7 @@ __global__ __launch_bounds__(256) void k_density_mx(
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
     for (int k0 = 0; k0 < nbe; k0 += 32) {
-      {
+      {  // B^T operand: 8 consecutive functions of one point per thread
         us8 s1, s2, s3;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -283,24 +286,24 @@ __global__ __launch_bounds__(256) void k_density_mx(
           s2[e] = b;
           s3[e] = c;
         }
-        *reinterpret_cast<us8*>(&As[0][r64][kg * 8]) = s1;
-        *reinterpret_cast<us8*>(&As[1][r64][kg * 8]) = s2;
-        *reinterpret_cast<us8*>(&As[2][r64][kg * 8]) = s3;
+        *reinterpret_cast<us8*>(&Bt[0][r64][kg * 8]) = s1;
+        *reinterpret_cast<us8*>(&Bt[1][r64][kg * 8]) = s2;
+        *reinterpret_cast<us8*>(&Bt[2][r64][kg * 8]) = s3;
       }
-      {
+      {  // P_sub operand: 8 consecutive functions of one column per thread
         us8 s1, s2, s3;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int k = k0 + kg * 8 + e;
           uint2 v = make_uint2(0u, 0u);
-          if (k < nbe && colP >= 0) v = Ps[(int64_t)bf[k] + colP];
+          if (k < nbe && colP >= 0) v = Ps[(int64_t)bf[k] * ldp + colP];
           s1[e] = (unsigned short)(v.x & 0xffffu);
           s2[e] = (unsigned short)(v.x >> 16);
           s3[e] = (unsigned short)(v.y & 0xffffu);
         }
-        *reinterpret_cast<us8*>(&Bs[0][r64][kg * 8]) = s1;
-        *reinterpret_cast<us8*>(&Bs[1][r64][kg * 8]) = s2;
-        *reinterpret_cast<us8*>(&Bs[2][r64][kg * 8]) = s3;
+        *reinterpret_cast<us8*>(&Pt[0][r64][kg * 8]) = s1;
+        *reinterpret_cast<us8*>(&Pt[1][r64][kg * 8]) = s2;
+        *reinterpret_cast<us8*>(&Pt[2][r64][kg * 8]) = s3;
       }
       __syncthreads();
       bf16x8 fa[2][3], fb[2][3];
@@ -308,8 +311,8 @@ __global__ __launch_bounds__(256) void k_density_mx(
       for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          fa[q][s] = frag(&As[s][wm * 32 + q * 16 + (lane & 15)][kk]);
-          fb[q][s] = frag(&Bs[s][wn * 32 + q * 16 + (lane & 15)][kk]);
+          fa[q][s] = frag(&Pt[s][wm * 32 + q * 16 + (lane & 15)][kk]);
+          fb[q][s] = frag(&Bt[s][wn * 32 + q * 16 + (lane & 15)][kk]);
         }
       __syncthreads();
       v4f c[2][2];
@@ -323,8 +326,8 @@ __global__ __launch_bounds__(256) void k_density_mx(
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj)
-            c[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[bi][sl_a(pr)],
-                                                                 fb[bj][sl_b(pr)], c[bi][bj], 0,
+            c[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[bi][sl_b(pr)],
+                                                                 fb[bj][sl_a(pr)], c[bi][bj], 0,
                                                                  0, 0);
       }
 #pragma unroll
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(256) void k_density_mx(
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) x[i][j][r] = fac * acc[i][j][r];
-    fold_tile<float>(x, rowof, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+    fold_tile<float>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
   }
   __syncthreads();
   finish_points(t, wi.a, tid, dens, weights, func, gga, nq, zcoef, red);

@@ -173,8 +173,14 @@ def test_device_collocation_fp32_storage(gpu, gx, orc):
         ref = orc.collocation(sh, t["shell_list"][s0:s0 + n], t["points"][p0:p0 + m], deriv=True)
         out = ctx.collocation(task, deriv=True, precision="mixed")
         for k in range(4):
-            # FP32 storage: relative rounding 2^-24 of each element.
-            assert np.all(np.abs(out[k] - ref[k].T) <= 6e-8 * np.abs(ref[k].T) + 1e-38)
+            # FP32 storage: relative rounding 2^-24 of each element, plus the
+            # FP64 evaluation's own cancellation noise near zeros of a
+            # gradient component (as in the FP64 collocation test).
+            r = ref[k].T
+            err = np.abs(out[k] - r)
+            bound = 6e-8 * np.abs(r) + 1e-14 * max(1.0, np.abs(r).max()) + 1e-38
+            worst = np.unravel_index(np.argmax(err - bound), err.shape)
+            assert np.all(err <= bound), (k, worst, out[k][worst], r[worst])
 
 
 @pytest.mark.parametrize("cfgname,n_water,grid,func",
