@@ -93,7 +93,6 @@ struct xcq_context_s {
   // chunk scratch
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
-  double* d_zcoef = nullptr;
   int64_t max_nq = 0, max_scratch_elems = 0;
 
   // packed bfloat16 slices of P for the mixed-precision contractions
@@ -123,7 +122,6 @@ struct xcq_context_s {
     dfree(d_red);
     dfree(d_out2);
     if (d_scratch) (void)hipFree(d_scratch);
-    dfree(d_zcoef);
     if (d_Phi) (void)hipFree(d_Phi);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -133,10 +131,30 @@ namespace xcq {
 CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream, c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap, c->d_lsh,
                  c->d_px,   c->d_py,    c->d_pz,     c->d_w,          c->d_work,  c->d_P,
-                 c->d_vxc,  c->nbf,     c->d_red,    c->d_zcoef, c->d_scratch,
+                 c->d_vxc,  c->nbf,     c->d_red,    c->d_scratch,
                  c->scratch_bytes, &c->d_Phi};
 }
 }  // namespace xcq
+
+namespace {
+// The command processor hands workgroup i of a launch to XCD i % 8.  Work
+// lists are built task by task, so neighbouring items share a task's B/Z
+// panels and P_sub gathers; this permutation gives each XCD runs of kXcdRun
+// consecutive items (one L2 then sees all of a run's reuse) while keeping
+// the list's large-first order at the granularity of 8 * kXcdRun items.
+// The final partial group keeps its natural order.
+constexpr size_t kXcds = 8, kXcdRun = 16;
+void xcd_interleave(std::vector<xcq::WorkItem>& v, size_t b, size_t e) {
+  constexpr size_t G = kXcds * kXcdRun;
+  const size_t n = e - b, full = n / G * G;
+  if (full == 0) return;
+  std::vector<xcq::WorkItem> tmp(v.begin() + b, v.begin() + b + full);
+  for (size_t i = 0; i < full; ++i) {
+    const size_t g = i / G, o = i % G, x = o / kXcdRun, q = o % kXcdRun;
+    v[b + g * G + q * kXcds + x] = tmp[i];
+  }
+}
+}  // namespace
 
 extern "C" {
 
@@ -347,6 +365,8 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
       c.nq = nq;
       c.npblk = (int64_t)pbl.size() - c.pblk_off;
       c.ng2 = (int64_t)g2.size() - c.g2_off;
+      xcd_interleave(pbl, (size_t)c.pblk_off, pbl.size());
+      xcd_interleave(g2, (size_t)c.g2_off, g2.size());
       chunks.push_back(c);
     }
   }
@@ -362,7 +382,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   dfree(ctx->d_lsh);
   dfree(ctx->d_work);
   dfree(ctx->d_red);
-  dfree(ctx->d_zcoef);
   if (ctx->d_scratch) {
     (void)hipFree(ctx->d_scratch);
     ctx->d_scratch = nullptr;
@@ -400,7 +419,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
   if ((rc = dalloc(&ctx->d_work, work.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
-  if ((rc = dalloc(&ctx->d_zcoef, (size_t)4 * max_nq))) return rc;
   ctx->scratch_bytes = (size_t)max_se * sizeof(double);
   if (ctx->scratch_bytes) {
     hipError_t e = hipMalloc(&ctx->d_scratch, ctx->scratch_bytes);
@@ -482,7 +500,7 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
     if (precision == XCQ_PREC_MIXED) {
       rc = launch_mixed_pipeline(ctx, c, func, gga, want_vxc, ev);
       if (rc) return rc;
-      launches += want_vxc ? 5 : 3;
+      launches += want_vxc ? 4 : 3;
       tiles += c.npblk + (want_vxc ? c.ng2 : 0);
       continue;
     }
@@ -494,11 +512,12 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
     if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[1], s));
     XCQ_HIP(hipEventRecord(ev[2], s));
-    // X, rho/grad rho, functional and EXC/NEL in one launch (stage 1);
-    // stage 2 stays as an empty interval so the stage table keeps its shape.
+    // X, rho/grad rho, functional, EXC/NEL and Z in one launch (stage 1);
+    // stages 2 (functional) and 3 (Z) stay as empty intervals so the stage
+    // table keeps its shape.
     rc = launch_density_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, ctx->d_P, ctx->nbf,
-                             scr, ctx->d_w, func, gga, func == 0 ? 1.0 : 2.0,
-                             want_vxc ? ctx->d_zcoef : nullptr, ctx->d_red);
+                             scr, ctx->d_w, func, gga, func == 0 ? 1.0 : 2.0, want_vxc,
+                             ctx->d_red);
     if (rc) return rc;
     XCQ_HIP(hipEventRecord(ev[3], s));
     XCQ_HIP(hipEventRecord(ev[4], s));
@@ -507,15 +526,13 @@ static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) 
     tiles += c.npblk;
     if (want_vxc) {
       XCQ_HIP(hipEventRecord(ev[6], s));
-      rc = launch_zmat_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_zcoef, scr, gga);
-      if (rc) return rc;
       XCQ_HIP(hipEventRecord(ev[7], s));
       XCQ_HIP(hipEventRecord(ev[8], s));
       rc = launch_syr2k_scatter_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, scr,
                                      ctx->d_vxc, ctx->nbf);
       if (rc) return rc;
       XCQ_HIP(hipEventRecord(ev[9], s));
-      launches += 2;
+      launches += 1;
       tiles += c.ng2;
     }
   }

@@ -1,5 +1,6 @@
 // Density stage of the XC path: X = fac B P_sub, rho / grad rho, the
-// functional with weight folding, and the EXC/NEL block sums, in one launch.
+// functional with weight folding, the EXC/NEL block sums and the Z matrix,
+// in one launch.
 //
 // One 256-thread workgroup owns one (task, 64-point block) and walks every
 // 64-function column tile of the task.  Per tile it forms the transposed
@@ -12,8 +13,17 @@
 // (reference_local_host_work_driver.cxx:242-268) into per-point partials
 // kept in LDS.  After the last tile the block's 64 points are complete: the
 // functional runs in the same workgroup (host_integrator_exc_vxc.hpp:445-502)
-// and writes the Z coefficients (a = w vrho / 2, c = 2 w vsigma grad rho)
-// and the block's (w eps rho, w rho) sums.  X never reaches HBM.
+// and gives the Z coefficients a = w vrho / 2, c = 2 w vsigma grad rho and
+// the block's (w eps rho, w rho) sums; the tail then writes
+//   Z = a B + c . grad B        (reference_local_host_work_driver.cxx:678-713)
+// for the block's points.  X never reaches HBM.
+//
+// The K loop is software-pipelined: step s+1's global loads are issued into
+// registers before step s's matrix instructions, and the LDS stage is double
+// buffered so one barrier per step suffices.  Out-of-range rows/columns are
+// read at a clamped (valid) index and zeroed by a select, so the loads are
+// branch-free.  K indices are uniform across a wave, which makes the
+// basis-map lookups scalar loads.
 #include "internal.hpp"
 #include "functional.hpp"
 
@@ -56,34 +66,47 @@ __device__ __forceinline__ constexpr int sl_b(int p) {
   return p == 1 ? 1 : p == 3 ? 2 : p == 4 ? 1 : 0;
 }
 
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Folds one X^T tile (rows = functions, columns = points) into the
-// per-point partials in LDS.  frow(bi, r) is the tile row (function) of
+// per-point partials in LDS.  frow(r) is the tile row (function) of
 // accumulator register r of row block bi for this lane; the column (point)
 // of lane l in column block bj is wn*32 + bj*16 + (l & 15).
 template <typename T, typename FRow>
 __device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow, int m0, int n0,
-                                          int wm, int wn, int lane, int nbe, int64_t ld,
-                                          int64_t plane, const T* Bm, int gga,
-                                          double (*dens)[64][4]) {
+                                          int wm, int wn, int lane, int nbe, int ld, int plane,
+                                          const T* Bm, int gga, double (*dens)[64][4]) {
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
     const int pl = wn * 32 + bj * 16 + (lane & 15);
-    const int64_t m = m0 + pl;
+    const int m = m0 + pl;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    T v0[2][4], v1[2][4], v2[2][4], v3[2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = n0 + wm * 32 + bi * 16 + frow(r);
-        if (j < nbe) {
-          const double xv = x[bi][bj][r];
-          const int64_t o = (int64_t)j * ld + m;
-          s0 += (double)Bm[o] * xv;
-          if (gga) {
-            s1 += (double)Bm[plane + o] * xv;
-            s2 += (double)Bm[2 * plane + o] * xv;
-            s3 += (double)Bm[3 * plane + o] * xv;
-          }
+        const int o = min(j, nbe - 1) * ld + m;
+        v0[bi][r] = Bm[o];
+        if (gga) {
+          v1[bi][r] = Bm[plane + o];
+          v2[bi][r] = Bm[2 * plane + o];
+          v3[bi][r] = Bm[3 * plane + o];
+        }
+      }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // Rows past nbe carry X = 0 (their P_sub column was zeroed), so the
+        // clamped B values they read contribute nothing.
+        const double xv = x[bi][bj][r];
+        s0 += (double)v0[bi][r] * xv;
+        if (gga) {
+          s1 += (double)v1[bi][r] * xv;
+          s2 += (double)v2[bi][r] * xv;
+          s3 += (double)v3[bi][r] * xv;
         }
       }
     // Sum over the four 16-lane groups (different functions, same point).
@@ -107,12 +130,13 @@ __device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow,
   }
 }
 
-// Functional on the block's 64 points (threads 0..63 = wave 0).
+// Functional on the block's 64 points (threads 0..63 = wave 0).  Writes the
+// Z coefficients (a, cx, cy, cz) of each point to zc and the block's
+// (EXC, NEL) partial sums to red.
 __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
-                                              double (*dens)[64][4],
+                                              double (*dens)[64][4], double (*zc)[64],
                                               const double* __restrict__ weights, int func,
-                                              int gga, int64_t nq, double* __restrict__ zcoef,
-                                              double* __restrict__ red) {
+                                              int gga, double* __restrict__ red) {
   if (tid >= 64) return;
   const int p = pb * kPtBlock + tid;
   const double rho = dens[0][tid][0] + dens[1][tid][0];
@@ -136,15 +160,10 @@ __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
       zz = c * gz;
     }
   }
-  if (zcoef) {
-    double* zc = zcoef + t.q_off + p;
-    zc[0] = za;
-    if (gga) {
-      zc[nq] = zx;
-      zc[2 * nq] = zy;
-      zc[3 * nq] = zz;
-    }
-  }
+  zc[0][tid] = za;
+  zc[1][tid] = zx;
+  zc[2][tid] = zy;
+  zc[3][tid] = zz;
   for (int off = 32; off > 0; off >>= 1) {
     e += __shfl_xor(e, off);
     n += __shfl_xor(n, off);
@@ -155,71 +174,115 @@ __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
   }
 }
 
+// Z = a B + c . grad B for the block's 64 points, every function row
+// (wave w handles rows w, w+4, ...; lane = point).
+template <typename T>
+__device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, double (*zc)[64],
+                                        int gga, T* Bm) {
+  const int lane = tid & 63, wave = uni(tid >> 6);
+  const int ld = t.npts_pad, nbe = t.nbe, plane = nbe * ld;
+  const double a = zc[0][lane], cx = zc[1][lane], cy = zc[2][lane], cz = zc[3][lane];
+  T* B = Bm + m0 + lane;
+  T* Z = B + 4 * plane;
+  int j = wave;
+  for (; j + 4 < nbe; j += 8) {
+    const int o0 = j * ld, o1 = (j + 4) * ld;
+    const double b0 = B[o0], b1 = B[o1];
+    double z0 = a * b0, z1 = a * b1;
+    if (gga) {
+      const double x0 = B[plane + o0], y0 = B[2 * plane + o0], q0 = B[3 * plane + o0];
+      const double x1 = B[plane + o1], y1 = B[2 * plane + o1], q1 = B[3 * plane + o1];
+      z0 += cx * x0 + cy * y0 + cz * q0;
+      z1 += cx * x1 + cy * y1 + cz * q1;
+    }
+    Z[o0] = (T)z0;
+    Z[o1] = (T)z1;
+  }
+  if (j < nbe) {
+    const int o = j * ld;
+    double z = a * (double)B[o];
+    if (gga) z += cx * (double)B[plane + o] + cy * (double)B[2 * plane + o] +
+                  cz * (double)B[3 * plane + o];
+    Z[o] = (T)z;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // FP64 (v_mfma_f64_16x16x4_f64; accumulator register r of lane l holds row
 // 4r + l/16).  Waves: wm = function half, wn = point half of the tile.
+// Loader roles: thread (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of
+// column c of both stages (B^T: point c; P_sub: function n0 + c).
 __global__ __launch_bounds__(256) void k_density_f64(
     const WorkItem* __restrict__ pblk, const DevTask* __restrict__ tasks,
     const int32_t* __restrict__ bfmap, const double* __restrict__ P, int64_t ldp,
-    const double* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
-    double fac, int64_t nq, double* __restrict__ zcoef, double* __restrict__ red) {
-  __shared__ double Pt[16][kPad64];  // [k][function]: A operand (P_sub)
-  __shared__ double Bt[16][kPad64];  // [k][point]:    B operand (B^T)
+    double* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
+    double fac, int want_z, double* __restrict__ red) {
+  __shared__ double Pt[2][16][kPad64];  // [buf][k][function]: A operand (P_sub)
+  __shared__ double Bt[2][16][kPad64];  // [buf][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
+  __shared__ double zc[4][64];
   const WorkItem wi = pblk[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int m0 = wi.a * kPtBlock;
-  const int64_t ld = t.npts_pad;
+  const int ld = t.npts_pad;
   const int nbe = t.nbe;
-  const int64_t plane = (int64_t)nbe * ld;
-  const double* Bm = scratch + t.b_off;
+  const int plane = nbe * ld;
+  double* Bm = scratch + t.b_off;
   const int32_t* bf = bfmap + t.bf_off;
   for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
-  const int a_m = tid & 63, a_k = tid >> 6;  // plane loader: point, k (+4i)
-  const int b_k = tid & 15, b_n = tid >> 4;  // P loader: k, function (+16i)
+  __syncthreads();
+  const int lc = lane, lq = wave;  // loader column / K quad
+  const double* bcol = Bm + m0 + lc;
   auto frow = [&](int r) { return 4 * r + (lane >> 4); };
+  const int nks = (nbe + 15) / 16;
 
   for (int n0 = 0; n0 < nbe; n0 += 64) {
-    int64_t colP[4];
+    const int ncol = n0 + lc;
+    const bool colok = ncol < nbe;
+    const double* Pc = P + bf[min(ncol, nbe - 1)];  // column bf[n] of P (P symmetric)
+    double rb[4], rp[4];
+    auto load = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int n = n0 + b_n + 16 * i;
-      colP[i] = n < nbe ? (int64_t)bf[n] * ldp : -1;
-    }
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + 4 * lq + e;  // wave-uniform
+        const int kc = min(k, nbe - 1);
+        const double b = bcol[kc * ld];
+        const double p = Pc[(int64_t)bf[kc] * ldp];
+        rb[e] = k < nbe ? b : 0.;
+        rp[e] = (k < nbe && colok) ? fac * p : 0.;
+      }
+    };
     v4d acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
-    for (int k0 = 0; k0 < nbe; k0 += 16) {
+    load(0);
+    for (int s = 0; s < nks; ++s) {
+      const int buf = s & 1;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = a_k + 4 * i;
-        Bt[k][a_m] = (k0 + k < nbe) ? Bm[(int64_t)(k0 + k) * ld + m0 + a_m] : 0.;
+      for (int e = 0; e < 4; ++e) {
+        Bt[buf][4 * lq + e][lc] = rb[e];
+        Pt[buf][4 * lq + e][lc] = rp[e];
       }
-      const int kk = k0 + b_k;
-      const int64_t rowP = kk < nbe ? (int64_t)bf[kk] : -1;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        Pt[b_k][b_n + 16 * i] = (rowP >= 0 && colP[i] >= 0) ? fac * P[rowP + colP[i]] : 0.;
       __syncthreads();
+      if (s + 1 < nks) load(16 * (s + 1));
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
-        const double a0 = Pt[kr][wm * 32 + (lane & 15)];
-        const double a1 = Pt[kr][wm * 32 + 16 + (lane & 15)];
-        const double b0 = Bt[kr][wn * 32 + (lane & 15)];
-        const double b1 = Bt[kr][wn * 32 + 16 + (lane & 15)];
+        const double a0 = Pt[buf][kr][wm * 32 + (lane & 15)];
+        const double a1 = Pt[buf][kr][wm * 32 + 16 + (lane & 15)];
+        const double b0 = Bt[buf][kr][wn * 32 + (lane & 15)];
+        const double b1 = Bt[buf][kr][wn * 32 + 16 + (lane & 15)];
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
         acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
       }
-      __syncthreads();
     }
     double x[2][2][4];
 #pragma unroll
@@ -229,43 +292,68 @@ __global__ __launch_bounds__(256) void k_density_f64(
 #pragma unroll
         for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
     fold_tile<double>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+    // The next tile's first stage write targets buffer 0, last read at step
+    // nks-2 (or nks-1 when nks == 1); all waves pass this barrier after it.
+    __syncthreads();
   }
-  __syncthreads();
-  finish_points(t, wi.a, tid, dens, weights, func, gga, nq, zcoef, red);
+  finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+  if (want_z) {
+    __syncthreads();
+    write_z<double>(t, m0, tid, zc, gga, Bm);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Mixed precision (v_mfma_f32_16x16x32_bf16, three-slice operands, FP32
 // inside one K block, FP64 across; accumulator register r of lane l holds
-// row 4(l/16) + r).
+// row 4(l/16) + r).  Loader roles: thread (c = tid & 63, g = tid >> 6)
+// loads K rows 8g..8g+7 of column c of both stages.
 __global__ __launch_bounds__(256) void k_density_mx(
     const WorkItem* __restrict__ pblk, const DevTask* __restrict__ tasks,
     const int32_t* __restrict__ bfmap, const uint2* __restrict__ Ps, int64_t ldp,
-    const float* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
-    double fac, int64_t nq, double* __restrict__ zcoef, double* __restrict__ red) {
-  __shared__ __attribute__((aligned(16))) unsigned short Pt[3][64][kRowMx];  // [s][fn][k]
-  __shared__ __attribute__((aligned(16))) unsigned short Bt[3][64][kRowMx];  // [s][pt][k]
+    float* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
+    double fac, int want_z, double* __restrict__ red) {
+  __shared__ __attribute__((aligned(16))) unsigned short Pt[2][3][64][kRowMx];  // [s][fn][k]
+  __shared__ __attribute__((aligned(16))) unsigned short Bt[2][3][64][kRowMx];  // [s][pt][k]
   __shared__ double dens[2][64][4];
+  __shared__ double zc[4][64];
   const WorkItem wi = pblk[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int m0 = wi.a * kPtBlock;
-  const int64_t ld = t.npts_pad;
+  const int ld = t.npts_pad;
   const int nbe = t.nbe;
-  const int64_t plane = (int64_t)nbe * ld;
-  const float* Bm = scratch + t.b_off;
+  const int plane = nbe * ld;
+  float* Bm = scratch + t.b_off;
   const int32_t* bf = bfmap + t.bf_off;
   for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
-  const int r64 = tid & 63, kg = tid >> 6;
+  __syncthreads();
+  const int lc = lane, lg = wave;
+  const float* bcol = Bm + m0 + lc;
   const int kk = 8 * (lane >> 4);
   auto frow = [&](int r) { return 4 * (lane >> 4) + r; };
+  const int nks = (nbe + 31) / 32;
 
   for (int n0 = 0; n0 < nbe; n0 += 64) {
     // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
-    // neighbouring threads (neighbouring n) touch neighbouring columns.
-    const int ncol = n0 + r64;
-    const int64_t colP = ncol < nbe ? (int64_t)bf[ncol] : -1;
+    // neighbouring lanes (neighbouring n) touch neighbouring columns.
+    const int ncol = n0 + lc;
+    const bool colok = ncol < nbe;
+    const uint2* Pc = Ps + bf[min(ncol, nbe - 1)];
+    float rb[8];
+    uint2 rp[8];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + 8 * lg + e;  // wave-uniform
+        const int kc = min(k, nbe - 1);
+        const float b = bcol[kc * ld];
+        const uint2 p = Pc[(int64_t)bf[kc] * ldp];
+        rb[e] = k < nbe ? b : 0.f;
+        rp[e] = (k < nbe && colok) ? p : make_uint2(0u, 0u);
+      }
+    };
     double acc[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -273,48 +361,39 @@ __global__ __launch_bounds__(256) void k_density_mx(
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
-    for (int k0 = 0; k0 < nbe; k0 += 32) {
-      {  // B^T operand: 8 consecutive functions of one point per thread
-        us8 s1, s2, s3;
+    load(0);
+    for (int s = 0; s < nks; ++s) {
+      const int buf = s & 1;
+      {
+        us8 s1, s2, s3, p1, p2, p3;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const int k = k0 + kg * 8 + e;
-          const float xv = k < nbe ? Bm[(int64_t)k * ld + m0 + r64] : 0.f;
           unsigned short a, b, c;
-          split3(xv, a, b, c);
+          split3(rb[e], a, b, c);
           s1[e] = a;
           s2[e] = b;
           s3[e] = c;
+          p1[e] = (unsigned short)(rp[e].x & 0xffffu);
+          p2[e] = (unsigned short)(rp[e].x >> 16);
+          p3[e] = (unsigned short)(rp[e].y & 0xffffu);
         }
-        *reinterpret_cast<us8*>(&Bt[0][r64][kg * 8]) = s1;
-        *reinterpret_cast<us8*>(&Bt[1][r64][kg * 8]) = s2;
-        *reinterpret_cast<us8*>(&Bt[2][r64][kg * 8]) = s3;
-      }
-      {  // P_sub operand: 8 consecutive functions of one column per thread
-        us8 s1, s2, s3;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int k = k0 + kg * 8 + e;
-          uint2 v = make_uint2(0u, 0u);
-          if (k < nbe && colP >= 0) v = Ps[(int64_t)bf[k] * ldp + colP];
-          s1[e] = (unsigned short)(v.x & 0xffffu);
-          s2[e] = (unsigned short)(v.x >> 16);
-          s3[e] = (unsigned short)(v.y & 0xffffu);
-        }
-        *reinterpret_cast<us8*>(&Pt[0][r64][kg * 8]) = s1;
-        *reinterpret_cast<us8*>(&Pt[1][r64][kg * 8]) = s2;
-        *reinterpret_cast<us8*>(&Pt[2][r64][kg * 8]) = s3;
+        *reinterpret_cast<us8*>(&Bt[buf][0][lc][8 * lg]) = s1;
+        *reinterpret_cast<us8*>(&Bt[buf][1][lc][8 * lg]) = s2;
+        *reinterpret_cast<us8*>(&Bt[buf][2][lc][8 * lg]) = s3;
+        *reinterpret_cast<us8*>(&Pt[buf][0][lc][8 * lg]) = p1;
+        *reinterpret_cast<us8*>(&Pt[buf][1][lc][8 * lg]) = p2;
+        *reinterpret_cast<us8*>(&Pt[buf][2][lc][8 * lg]) = p3;
       }
       __syncthreads();
+      if (s + 1 < nks) load(32 * (s + 1));
       bf16x8 fa[2][3], fb[2][3];
 #pragma unroll
       for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          fa[q][s] = frag(&Pt[s][wm * 32 + q * 16 + (lane & 15)][kk]);
-          fb[q][s] = frag(&Bt[s][wn * 32 + q * 16 + (lane & 15)][kk]);
+        for (int sl = 0; sl < 3; ++sl) {
+          fa[q][sl] = frag(&Pt[buf][sl][wm * 32 + q * 16 + (lane & 15)][kk]);
+          fb[q][sl] = frag(&Bt[buf][sl][wn * 32 + q * 16 + (lane & 15)][kk]);
         }
-      __syncthreads();
       v4f c[2][2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
@@ -345,30 +424,34 @@ __global__ __launch_bounds__(256) void k_density_mx(
 #pragma unroll
         for (int r = 0; r < 4; ++r) x[i][j][r] = fac * acc[i][j][r];
     fold_tile<float>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+    __syncthreads();
   }
-  __syncthreads();
-  finish_points(t, wi.a, tid, dens, weights, func, gga, nq, zcoef, red);
+  finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+  if (want_z) {
+    __syncthreads();
+    write_z<float>(t, m0, tid, zc, gga, Bm);
+  }
 }
 
 int launch_density_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                         const DevTask* tasks, const int32_t* bfmap, const double* P,
-                        int64_t ldp, const double* scratch, const double* weights, int func,
-                        int gga, double fac, double* zcoef, double* red) {
+                        int64_t ldp, double* scratch, const double* weights, int func, int gga,
+                        double fac, int want_z, double* red) {
   if (c.npblk == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_density_f64, dim3((unsigned)c.npblk), dim3(256), 0, s,
                      pblk + c.pblk_off, tasks, bfmap, P, ldp, scratch, weights, func, gga, fac,
-                     c.nq, zcoef, red);
+                     want_z, red);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
 
 int launch_density_mixed(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                          const DevTask* tasks, const int32_t* bfmap, const uint2* Ps,
-                         int64_t ldp, const float* scratch, const double* weights, int func,
-                         int gga, double fac, double* zcoef, double* red) {
+                         int64_t ldp, float* scratch, const double* weights, int func, int gga,
+                         double fac, int want_z, double* red) {
   if (c.npblk == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_density_mx, dim3((unsigned)c.npblk), dim3(256), 0, s, pblk + c.pblk_off,
-                     tasks, bfmap, Ps, ldp, scratch, weights, func, gga, fac, c.nq, zcoef, red);
+                     tasks, bfmap, Ps, ldp, scratch, weights, func, gga, fac, want_z, red);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

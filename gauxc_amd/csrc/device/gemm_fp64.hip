@@ -1,9 +1,7 @@
 // FP64 contractions of the XC path on gfx950 matrix cores
 // (v_mfma_f64_16x16x4_f64).
 //
-//  (X = fac B P_sub and the density reduction live in density.hip.)
-//  k_zmat     : Z = a B + c . grad B (eval_zmat_gga_vxc_rks,
-//               reference_local_host_work_driver.cxx:678-713).
+//  (X = fac B P_sub, the density reduction and Z live in density.hip.)
 //  k_syr2k    : V = B^T Z + Z^T B for lower-triangle 64x64 tiles with the
 //               scatter-add into VXC fused into the epilogue (GauXC: DSYR2K
 //               per task + sym_inc_by_submat, scheme1_base.cxx:1712-1766).
@@ -27,40 +25,11 @@ __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_zmat(const WorkItem* __restrict__ pblk,
-                                              const DevTask* __restrict__ tasks,
-                                              const double* __restrict__ zcoef, int64_t nq,
-                                              double* __restrict__ scratch, int gga) {
-  const WorkItem wi = pblk[blockIdx.x];
-  const DevTask t = tasks[wi.task];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int p = wi.a * kPtBlock + lane;
-  const int64_t ld = t.npts_pad;
-  const int64_t plane = (int64_t)t.nbe * ld;
-  const double* zc = zcoef + t.q_off + p;
-  const double a = zc[0];
-  const double cx = gga ? zc[nq] : 0., cy = gga ? zc[2 * nq] : 0., cz = gga ? zc[3 * nq] : 0.;
-  double* B = scratch + t.b_off + p;
-  double* Z = B + 4 * plane;
-  for (int j = wave; j < t.nbe; j += 4) {
-    const int64_t o = (int64_t)j * ld;
-    double z = a * B[o];
-    if (gga) z += cx * B[plane + o] + cy * B[2 * plane + o] + cz * B[3 * plane + o];
-    Z[o] = z;
-  }
-}
-
-int launch_zmat_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
-                     const double* zcoef, double* scratch, int gga) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_zmat, dim3((unsigned)c.npblk), dim3(256), 0, s, pblk + c.pblk_off, tasks,
-                     zcoef, c.nq, scratch, gga);
-  XCQ_HIP(hipGetLastError());
-  return XCQ_SUCCESS;
-}
-
-// ---------------------------------------------------------------------------
+// Loader: thread (p = tid & 15, f = tid >> 4) moves point k0 + p of rows
+// f, f+16, f+32, f+48 of each panel; 16 lanes cover 16 consecutive points
+// (128 B) of one row.  Rows past nbe are read at row nbe-1 and zeroed by a
+// select.  The next K step is loaded into registers while the current one
+// runs on the matrix cores.
 __global__ __launch_bounds__(256) void k_syr2k(const WorkItem* __restrict__ tiles,
                                                const DevTask* __restrict__ tasks,
                                                const int32_t* __restrict__ bfmap,
@@ -72,38 +41,65 @@ __global__ __launch_bounds__(256) void k_syr2k(const WorkItem* __restrict__ tile
   __shared__ double Zj[16][kLdsPad];
   const WorkItem wi = tiles[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int i0 = wi.a * kTile, j0 = wi.b * kTile;
   const bool diag = wi.a == wi.b;
-  const int64_t ld = t.npts_pad;
+  const int ld = t.npts_pad;
   const int nbe = t.nbe;
-  const int64_t plane = (int64_t)nbe * ld;
+  const int plane = nbe * ld;
   const double* Bm = scratch + t.b_off;
   const double* Zm = Bm + 4 * plane;
 
-  const int l_p = tid & 15, l_f = tid >> 4;  // loader: point, function (+16q)
+  const int l_p = tid & 15, l_f = tid >> 4;
+  int oi[4], oj[4];
+  bool ki[4], kj[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int fi = i0 + l_f + 16 * q, fj = j0 + l_f + 16 * q;
+    ki[q] = fi < nbe;
+    kj[q] = fj < nbe;
+    oi[q] = min(fi, nbe - 1) * ld + l_p;
+    oj[q] = min(fj, nbe - 1) * ld + l_p;
+  }
+  double rbi[4], rzi[4], rbj[4], rzj[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double b = Bm[oi[q] + k0], z = Zm[oi[q] + k0];
+      rbi[q] = ki[q] ? b : 0.;
+      rzi[q] = ki[q] ? z : 0.;
+    }
+    if (!diag) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double b = Bm[oj[q] + k0], z = Zm[oj[q] + k0];
+        rbj[q] = kj[q] ? b : 0.;
+        rzj[q] = kj[q] ? z : 0.;
+      }
+    }
+  };
+
   v4d acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
 
+  load(0);
   for (int k0 = 0; k0 < ld; k0 += 16) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int f = l_f + 16 * q;
-      const int fi = i0 + f, fj = j0 + f;
-      const int64_t oi = (int64_t)fi * ld + k0 + l_p;
-      const int64_t oj = (int64_t)fj * ld + k0 + l_p;
-      Bi[l_p][f] = fi < nbe ? Bm[oi] : 0.;
-      Zi[l_p][f] = fi < nbe ? Zm[oi] : 0.;
+      Bi[l_p][l_f + 16 * q] = rbi[q];
+      Zi[l_p][l_f + 16 * q] = rzi[q];
       if (!diag) {
-        Bj[l_p][f] = fj < nbe ? Bm[oj] : 0.;
-        Zj[l_p][f] = fj < nbe ? Zm[oj] : 0.;
+        Bj[l_p][l_f + 16 * q] = rbj[q];
+        Zj[l_p][l_f + 16 * q] = rzj[q];
       }
     }
     __syncthreads();
+    if (k0 + 16 < ld) load(k0 + 16);
     const double(*BJ)[kLdsPad] = diag ? Bi : Bj;
     const double(*ZJ)[kLdsPad] = diag ? Zi : Zj;
 #pragma unroll

@@ -87,33 +87,6 @@ __global__ __launch_bounds__(256) void k_split_P(int64_t n, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_zmat_f32(const WorkItem* __restrict__ pblk,
-                                                  const DevTask* __restrict__ tasks,
-                                                  const double* __restrict__ zcoef, int64_t nq,
-                                                  float* __restrict__ scratch, int gga) {
-  const WorkItem wi = pblk[blockIdx.x];
-  const DevTask t = tasks[wi.task];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int p = wi.a * kPtBlock + lane;
-  const int64_t ld = t.npts_pad;
-  const int64_t plane = (int64_t)t.nbe * ld;
-  const double* zc = zcoef + t.q_off + p;
-  const double a = zc[0];
-  const double cx = gga ? zc[nq] : 0., cy = gga ? zc[2 * nq] : 0., cz = gga ? zc[3 * nq] : 0.;
-  float* B = scratch + t.b_off + p;
-  float* Z = B + 4 * plane;
-  for (int j = wave; j < t.nbe; j += 4) {
-    const int64_t o = (int64_t)j * ld;
-    double z = a * (double)B[o];
-    if (gga)
-      z += cx * (double)B[plane + o] + cy * (double)B[2 * plane + o] +
-           cz * (double)B[3 * plane + o];
-    Z[o] = (float)z;
-  }
-}
-
-// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ tiles,
                                                   const DevTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ bfmap,
@@ -125,16 +98,23 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ t
   __shared__ __attribute__((aligned(16))) unsigned short Zj[3][64][kRow];
   const WorkItem wi = tiles[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int i0 = wi.a * kTile, j0 = wi.b * kTile;
   const bool diag = wi.a == wi.b;
-  const int64_t ld = t.npts_pad;
+  const int ld = t.npts_pad;
   const int nbe = t.nbe;
-  const int64_t plane = (int64_t)nbe * ld;
+  const int plane = nbe * ld;
   const float* Bm = scratch + t.b_off;
   const float* Zm = Bm + 4 * plane;
-  const int f = tid & 63, pg = tid >> 6;  // loader: function row, point group of 8
+  // Loader: thread (g = tid & 3, f = tid >> 2) moves points k0+8g..k0+8g+7
+  // of row f of each panel (four lanes cover one row's 128 B); rows past
+  // nbe are read at row nbe-1 and zeroed by a select.
+  const int pg = tid & 3, f = tid >> 2;
+  const int fi = i0 + f, fj = j0 + f;
+  const bool ki = fi < nbe, kj = fj < nbe;
+  const int oi = min(fi, nbe - 1) * ld + 8 * pg, oj = min(fj, nbe - 1) * ld + 8 * pg;
 
   double acc[2][2][4];
 #pragma unroll
@@ -144,13 +124,32 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ t
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
 
-  auto stage = [&](const float* src, int row0, unsigned short (*dst)[64][kRow], int k0) {
+  float4 pre[4][2];  // (Bi, Zi, Bj, Zj) x two float4
+  auto load = [&](int k0) {
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float4 b = *reinterpret_cast<const float4*>(Bm + oi + k0 + 4 * h);
+      const float4 z = *reinterpret_cast<const float4*>(Zm + oi + k0 + 4 * h);
+      pre[0][h] = ki ? b : z4;
+      pre[1][h] = ki ? z : z4;
+    }
+    if (!diag) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float4 b = *reinterpret_cast<const float4*>(Bm + oj + k0 + 4 * h);
+        const float4 z = *reinterpret_cast<const float4*>(Zm + oj + k0 + 4 * h);
+        pre[2][h] = kj ? b : z4;
+        pre[3][h] = kj ? z : z4;
+      }
+    }
+  };
+  auto stage = [&](const float4 (&v)[2], unsigned short (*dst)[64][kRow]) {
+    const float* pv = reinterpret_cast<const float*>(&v[0]);  // 8 consecutive points
     us8 s1, s2, s3;
-    const int row = row0 + f;
-    const float* p = src + (int64_t)row * ld + k0 + pg * 8;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const float x = row < nbe ? p[e] : 0.f;
+      const float x = pv[e];
       unsigned short a, b, c;
       split3f(x, a, b, c);
       s1[e] = a;
@@ -162,14 +161,16 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(const WorkItem* __restrict__ t
     *reinterpret_cast<us8*>(&dst[2][f][pg * 8]) = s3;
   };
 
+  load(0);
   for (int k0 = 0; k0 < ld; k0 += kKB) {
-    stage(Bm, i0, Bi, k0);
-    stage(Zm, i0, Zi, k0);
+    stage(pre[0], Bi);
+    stage(pre[1], Zi);
     if (!diag) {
-      stage(Bm, j0, Bj, k0);
-      stage(Zm, j0, Zj, k0);
+      stage(pre[2], Bj);
+      stage(pre[3], Zj);
     }
     __syncthreads();
+    if (k0 + kKB < ld) load(k0 + kKB);
     unsigned short(*BJ)[64][kRow] = diag ? Bi : Bj;
     unsigned short(*ZJ)[64][kRow] = diag ? Zi : Zj;
     const int kk = 8 * (lane >> 4);
@@ -246,19 +247,14 @@ int launch_mixed_pipeline(struct ::xcq_context_s* ctx, const Chunk& c, int func,
   XCQ_HIP(hipEventRecord(ev[1], s));
   XCQ_HIP(hipEventRecord(ev[2], s));
   rc = launch_density_mixed(s, c, v.work, v.tasks, v.bfmap, (const uint2*)*v.Psplit,
-                            (int64_t)v.nbf, scr, v.w, func, gga, func == 0 ? 1.0 : 2.0,
-                            want_vxc ? v.zcoef : nullptr, v.red);
+                            (int64_t)v.nbf, scr, v.w, func, gga, func == 0 ? 1.0 : 2.0, want_vxc,
+                            v.red);
   if (rc) return rc;
   XCQ_HIP(hipEventRecord(ev[3], s));
   XCQ_HIP(hipEventRecord(ev[4], s));
   XCQ_HIP(hipEventRecord(ev[5], s));
   if (!want_vxc) return XCQ_SUCCESS;
   XCQ_HIP(hipEventRecord(ev[6], s));
-  if (c.npblk) {
-    hipLaunchKernelGGL(k_zmat_f32, dim3((unsigned)c.npblk), dim3(256), 0, s, v.work + c.pblk_off,
-                       v.tasks, v.zcoef, c.nq, scr, gga);
-    XCQ_HIP(hipGetLastError());
-  }
   XCQ_HIP(hipEventRecord(ev[7], s));
   XCQ_HIP(hipEventRecord(ev[8], s));
   if (c.ng2) {

@@ -84,7 +84,6 @@ struct CtxView {
   double* vxc;
   int32_t nbf;
   double* red;
-  double* zcoef;
   void* scratch;
   size_t scratch_bytes;
   void** Psplit;  // nbf x nbf packed (p1, p2, p3, 0) bfloat16 slices of P
@@ -103,14 +102,12 @@ int launch_collocation_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                             const double* pz, double* scratch, int deriv);
 int launch_density_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                         const DevTask* tasks, const int32_t* bfmap, const double* P,
-                        int64_t ldp, const double* scratch, const double* weights, int func,
-                        int gga, double fac, double* zcoef, double* red);
+                        int64_t ldp, double* scratch, const double* weights, int func, int gga,
+                        double fac, int want_z, double* red);
 int launch_density_mixed(hipStream_t s, const Chunk& c, const WorkItem* pblk,
                          const DevTask* tasks, const int32_t* bfmap, const uint2* Ps,
-                         int64_t ldp, const float* scratch, const double* weights, int func,
-                         int gga, double fac, double* zcoef, double* red);
-int launch_zmat_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk, const DevTask* tasks,
-                     const double* zcoef, double* scratch, int gga);
+                         int64_t ldp, float* scratch, const double* weights, int func, int gga,
+                         double fac, int want_z, double* red);
 int launch_syr2k_scatter_fp64(hipStream_t s, const Chunk& c, const WorkItem* g2,
                               const DevTask* tasks, const int32_t* bfmap, const double* scratch,
                               double* vxc, int64_t ldv);
