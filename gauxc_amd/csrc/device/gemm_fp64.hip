@@ -122,20 +122,27 @@ __global__ __launch_bounds__(256) void k_syr2k(const WorkItem* __restrict__ tile
     __syncthreads();
   }
 
+  // Scatter-add into VXC.  Element (i, j) of the tile (i >= j locally) is
+  // added at (bf[j], bf[i]) of the column-major VXC: the 16 lanes of a lane
+  // group hold 16 consecutive j, so each atomic instruction touches a few
+  // 128-byte column segments instead of 16 columns.  The symmetrisation that
+  // follows sums the two triangles, so which triangle an element lands in
+  // does not matter.
   const int32_t* bf = bfmap + t.bf_off;
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bj = 0; bj < 2; ++bj) {
+    const int j = j0 + wn * 32 + bj * 16 + (lane & 15);
+    if (j >= nbe) continue;
+    const int rj = bf[j];
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const int j = j0 + wn * 32 + bj * 16 + (lane & 15);
-      if (j >= nbe) continue;
-      const int64_t cj = (int64_t)bf[j] * ldv;
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + wm * 32 + bi * 16 + 4 * r + (lane >> 4);
-        if (i < nbe && (!diag || i >= j)) atomicAdd(vxc + bf[i] + cj, acc[bi][bj][r]);
+        if (i < nbe && (!diag || i >= j))
+          atomicAdd(vxc + (int64_t)bf[i] * ldv + rj, acc[bi][bj][r]);
       }
-    }
+  }
 }
 
 int launch_syr2k_scatter_fp64(hipStream_t s, const Chunk& c, const WorkItem* g2,

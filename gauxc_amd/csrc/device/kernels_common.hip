@@ -110,21 +110,31 @@ int launch_reduce_sums(hipStream_t s, const double* red, int64_t nb, double* out
 
 // Upper triangle <- lower triangle through a 32x32 LDS tile
 // (symmetrize_vxc of incore_..._exc_vxc.hpp:376-385).
+// Symmetrisation of the accumulated VXC: each off-diagonal pair (a, b) holds
+// contributions in either triangle, and both entries become their sum
+// (GauXC fills the upper triangle from the lower one,
+// host_integrator_exc_vxc.hpp:577-599; here the scatter may land in either).
+// Workgroup (bi, bj), bi >= bj, owns the 32x32 tile pair (bi, bj) / (bj, bi).
 __global__ __launch_bounds__(256) void k_symmetrize(int32_t n, double* __restrict__ a,
                                                     int64_t lda) {
-  __shared__ double tile[32][33];
-  const int bi = blockIdx.x, bj = blockIdx.y;  // source tile rows bi (lower), cols bj
+  __shared__ double lo[32][33];
+  __shared__ double up[32][33];
+  const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi < bj) return;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
-    const int r = bi * 32 + tx, c = bj * 32 + k;
-    tile[k][tx] = (r < n && c < n) ? a[r + (int64_t)c * lda] : 0.;
+    const int r = bi * 32 + tx, c = bj * 32 + k;  // lower tile element (r, c)
+    lo[k][tx] = (r < n && c < n) ? a[r + (int64_t)c * lda] : 0.;
+    const int r2 = bj * 32 + tx, c2 = bi * 32 + k;  // upper tile element (r2, c2)
+    up[k][tx] = (r2 < n && c2 < n) ? a[r2 + (int64_t)c2 * lda] : 0.;
   }
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
-    // destination (row = bj*32+tx, col = bi*32+k) = source (row bi*32+k, col bj*32+tx)
-    const int r = bj * 32 + tx, c = bi * 32 + k;
-    if (r < n && c < n && r < c) a[r + (int64_t)c * lda] = tile[tx][k];
+    // lower (r = bi*32+tx, c = bj*32+k) pairs with upper (c, r) = up[tx][k]
+    const int r = bi * 32 + tx, c = bj * 32 + k;
+    if (r < n && c < n && r > c) a[r + (int64_t)c * lda] = lo[k][tx] + up[tx][k];
+    const int r2 = bj * 32 + tx, c2 = bi * 32 + k;
+    if (r2 < n && c2 < n && r2 < c2) a[r2 + (int64_t)c2 * lda] = up[k][tx] + lo[tx][k];
   }
 }
 
