@@ -87,6 +87,9 @@ DEVICE_SYMBOLS = [
     "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
+    "xcq_upload_molecules", "xcq_upload_tasks_batched", "xcq_upload_density_mol",
+    "xcq_download_results_mol", "xcq_set_tile_policy", "xcq_set_schedule",
+    "xcq_enable_task_timing", "xcq_task_times", "xcq_use_graph",
 ]
 
 _host = None
@@ -130,6 +133,16 @@ def device_lib():
                                                    c_vp, c_vp, c_vp, c_vp]),
             "xcq_stage_times": (ctypes.c_int, [H, c_vp, ctypes.c_int]),
             "xcq_work_counters": (ctypes.c_int, [H, c_vp, ctypes.c_int]),
+            "xcq_upload_molecules": (ctypes.c_int, [H, c_i32, c_vp, c_vp]),
+            "xcq_upload_tasks_batched": (ctypes.c_int, [H, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp,
+                                                        c_i64, c_vp]),
+            "xcq_upload_density_mol": (ctypes.c_int, [H, c_i32, c_i32, c_vp, c_i64]),
+            "xcq_download_results_mol": (ctypes.c_int, [H, c_i32, c_vp, c_vp, c_vp, c_i64]),
+            "xcq_set_tile_policy": (ctypes.c_int, [H, c_i32, c_i32]),
+            "xcq_set_schedule": (ctypes.c_int, [H, c_i32, c_vp, c_vp]),
+            "xcq_enable_task_timing": (ctypes.c_int, [H, ctypes.c_int]),
+            "xcq_task_times": (ctypes.c_int, [H, c_vp, c_vp]),
+            "xcq_use_graph": (ctypes.c_int, [H, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)

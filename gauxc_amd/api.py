@@ -409,6 +409,102 @@ class DeviceContext:
         self.lib.xcq_work_counters(self.h, c.ctypes.data, 4)
         return c
 
+    # ---- scheduling, timing and graph controls ----
+    def set_tile_policy(self, dens32_max_nbe, syr2k32_max_nbe):
+        N.xcq_check(self.lib.xcq_set_tile_policy(self.h, int(dens32_max_nbe),
+                                                 int(syr2k32_max_nbe)), "xcq_set_tile_policy")
+
+    def set_schedule(self, nqueues=0, dens_cost=None, syr2k_cost=None):
+        if dens_cost is None:
+            N.xcq_check(self.lib.xcq_set_schedule(self.h, int(nqueues), None, None),
+                        "xcq_set_schedule")
+            return
+        dc = np.ascontiguousarray(dens_cost, dtype=np.float64)
+        sc = np.ascontiguousarray(syr2k_cost, dtype=np.float64)
+        N.xcq_check(self.lib.xcq_set_schedule(self.h, int(nqueues), dc.ctypes.data,
+                                              sc.ctypes.data), "xcq_set_schedule")
+
+    def enable_task_timing(self, on=True):
+        N.xcq_check(self.lib.xcq_enable_task_timing(self.h, int(bool(on))),
+                    "xcq_enable_task_timing")
+
+    def task_times(self):
+        nt = len(self.tasks["npts"])
+        d = np.zeros(nt)
+        s = np.zeros(nt)
+        N.xcq_check(self.lib.xcq_task_times(self.h, d.ctypes.data, s.ctypes.data),
+                    "xcq_task_times")
+        return d, s
+
+    def use_graph(self, on=True):
+        N.xcq_check(self.lib.xcq_use_graph(self.h, int(bool(on))), "xcq_use_graph")
+
+
+class BatchedDeviceContext(DeviceContext):
+    """Several independent molecules evaluated in the same launches.
+
+    ``upload_molecules`` takes one list of xcq_shell arrays per molecule;
+    ``upload_tasks`` takes one task dict (LoadBalancer.get_tasks layout) per
+    molecule and concatenates them, tagging each task with its molecule."""
+
+    def upload_molecules(self, shell_arrays):
+        counts = np.array([len(a) for a in shell_arrays], dtype=np.int32)
+        allsh = (N.XcqShell * int(counts.sum()))()
+        k = 0
+        for a in shell_arrays:
+            for sh in a:
+                allsh[k] = sh
+                k += 1
+        N.xcq_check(self.lib.xcq_upload_molecules(self.h, len(counts), counts.ctypes.data, allsh),
+                    "xcq_upload_molecules")
+        self.mol_nshells = counts
+        self.mol_nbf = [None] * len(counts)
+
+    def upload_tasks(self, task_dicts):
+        sh_base = np.concatenate([[0], np.cumsum(self.mol_nshells)[:-1]]).astype(np.int64)
+        descs, mols, pts, ws, sls = [], [], [], [], []
+        pt_base = 0
+        sl_base = 0
+        for m, t in enumerate(task_dicts):
+            nt = len(t["npts"])
+            for i in range(nt):
+                descs.append((int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
+                              int(t["nbe"][i]), int(t["point_offset"][i]) + pt_base,
+                              int(t["shell_offset"][i]) + sl_base, float(t["dist_nearest"][i])))
+            mols.append(np.full(nt, m, dtype=np.int32))
+            pts.append(np.asarray(t["points"], dtype=np.float64).reshape(-1, 3))
+            ws.append(np.asarray(t["weights"], dtype=np.float64))
+            sls.append(np.asarray(t["shell_list"], dtype=np.int64) + sh_base[m])
+            pt_base += len(ws[-1])
+            sl_base += len(sls[-1])
+        nt = len(descs)
+        desc = (N.XcqTask * nt)(*[N.XcqTask(*d) for d in descs])
+        mol = np.ascontiguousarray(np.concatenate(mols))
+        P = np.ascontiguousarray(np.concatenate(pts))
+        W = np.ascontiguousarray(np.concatenate(ws))
+        SL = np.ascontiguousarray(np.concatenate(sls).astype(np.int32))
+        N.xcq_check(self.lib.xcq_upload_tasks_batched(self.h, nt, desc, mol.ctypes.data, len(W),
+                                                      P.ctypes.data, W.ctypes.data, len(SL),
+                                                      SL.ctypes.data), "xcq_upload_tasks_batched")
+        self.tasks = {"npts": np.array([d[1] for d in descs]),
+                      "nbe": np.array([d[3] for d in descs]), "molecule": mol}
+
+    def upload_density(self, mol, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        self.mol_nbf[mol] = P.shape[0]
+        N.xcq_check(self.lib.xcq_upload_density_mol(self.h, int(mol), P.shape[0], P.ctypes.data,
+                                                    P.shape[0]), "xcq_upload_density_mol")
+
+    def results(self, mol, want_vxc=True):
+        n = self.mol_nbf[mol]
+        exc = ctypes.c_double(0.)
+        nel = ctypes.c_double(0.)
+        vxc = np.zeros((n, n), order="F") if want_vxc else None
+        N.xcq_check(self.lib.xcq_download_results_mol(
+            self.h, int(mol), ctypes.addressof(exc), ctypes.addressof(nel),
+            vxc.ctypes.data if want_vxc else None, n), "xcq_download_results_mol")
+        return exc.value, nel.value, vxc
+
 
 def eval_functional_device(func, rho, sigma=None, device=0):
     rho = np.ascontiguousarray(rho, dtype=np.float64)

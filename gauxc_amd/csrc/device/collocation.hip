@@ -195,40 +195,36 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
   }
 }
 
-int launch_collocation_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
+int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first,
                             const double* px, const double* py, const double* pz,
                             double* scratch, int deriv) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
+  if (n == 0) return XCQ_SUCCESS;
   if (deriv)
-    hipLaunchKernelGGL((k_collocation<true, double, double>), dim3((unsigned)c.npblk), dim3(256),
-                       0, s, pblk + c.pblk_off, tasks, shells, shell_list, lsh_first, px, py, pz,
-                       scratch);
+    hipLaunchKernelGGL((k_collocation<true, double, double>), dim3((unsigned)n), dim3(256), 0, s,
+                       items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
   else
-    hipLaunchKernelGGL((k_collocation<false, double, double>), dim3((unsigned)c.npblk),
-                       dim3(256), 0, s, pblk + c.pblk_off, tasks, shells, shell_list, lsh_first,
-                       px, py, pz, scratch);
+    hipLaunchKernelGGL((k_collocation<false, double, double>), dim3((unsigned)n), dim3(256), 0,
+                       s, items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
 
 // Mixed precision: values evaluated in FP64, stored as FP32 (half the HBM
 // traffic of the B planes; storage rounding is the only FP32 step).
-int launch_collocation_fp32(hipStream_t s, const Chunk& c, const WorkItem* pblk,
+int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first,
                             const double* px, const double* py, const double* pz,
                             float* scratch, int deriv) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
+  if (n == 0) return XCQ_SUCCESS;
   if (deriv)
-    hipLaunchKernelGGL((k_collocation<true, double, float>), dim3((unsigned)c.npblk), dim3(256),
-                       0, s, pblk + c.pblk_off, tasks, shells, shell_list, lsh_first, px, py, pz,
-                       scratch);
+    hipLaunchKernelGGL((k_collocation<true, double, float>), dim3((unsigned)n), dim3(256), 0, s,
+                       items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
   else
-    hipLaunchKernelGGL((k_collocation<false, double, float>), dim3((unsigned)c.npblk),
-                       dim3(256), 0, s, pblk + c.pblk_off, tasks, shells, shell_list, lsh_first,
-                       px, py, pz, scratch);
+    hipLaunchKernelGGL((k_collocation<false, double, float>), dim3((unsigned)n), dim3(256), 0,
+                       s, items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

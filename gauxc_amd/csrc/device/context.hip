@@ -1,23 +1,30 @@
 // C ABI of the device layer (include/gauxc_amd/device.h): context, resident
-// data, chunk planning and the stage pipeline of one EXC/VXC evaluation.
+// data, chunk planning, work lists and the stage pipeline of one EXC/VXC
+// evaluation.
 //
 // HBM layout (all resident until replaced):
-//   shells      DevShell[nshells]
+//   shells      DevShell[nshells] of every molecule, molecule after molecule
 //   px/py/pz/w  SoA points and weights of all tasks, in upload order
-//   shell list, local->global basis map, local shell -> first function
-//   work lists  (task, point block) and SYR2K tiles per chunk
-//   P, VXC      nbf x nbf, column-major, ld = nbf
-//   red         (w eps rho, w rho) per point block -> deterministic sums
+//   shell list, local->molecule basis map, local shell -> first function
+//   work        per chunk: [density items, 32-wide | 64-wide]
+//                          [SYR2K tiles,   32-wide | 64-wide]
+//   qoff        queue tables of the persistent launches
+//   P, VXC      per molecule nbf x nbf, column-major, ld = nbf, concatenated
+//   P slices    packed bfloat16 slices of P (mixed precision), same offsets
+//   red         (w eps rho, w rho) per point block, molecule-major
 // Chunk scratch (reused by every chunk): per task [5][nbe][npts_pad]
-// (B, dB/dx, dB/dy, dB/dz, Z) and the Z coefficients.  Tasks are
-// grouped greedily into chunks that fit the arena; each chunk is one launch
-// per stage, so every launch carries thousands of workgroups.
+// (B, dB/dx, dB/dy, dB/dz, Z).  Tasks are grouped greedily into chunks that
+// fit the arena; each chunk is one launch per stage and tile width, so every
+// launch carries thousands of workgroups (or one persistent workgroup per
+// queue).
 #include "internal.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -25,14 +32,6 @@ namespace xcq {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& m) { g_last_error = m; }
-
-int launch_collocation_fp32(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                            const DevTask* tasks, const DevShell* shells,
-                            const int32_t* shell_list, const int32_t* lsh_first,
-                            const double* px, const double* py, const double* pz, float* scratch,
-                            int deriv);
-int launch_mixed_pipeline(struct ::xcq_context_s* ctx, const Chunk& c, int func, int gga,
-                          int want_vxc, hipEvent_t* ev);
 
 }  // namespace xcq
 
@@ -59,6 +58,58 @@ void dfree(T*& p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
+
+// The command processor hands workgroup i of a launch to XCD i % 8.  Work
+// lists are built task by task, so neighbouring items share a task's B/Z
+// panels and P_sub gathers; this permutation gives each XCD runs of kXcdRun
+// consecutive items (one L2 then sees all of a run's reuse) while keeping
+// the list's large-first order at the granularity of 8 * kXcdRun items.
+// The final partial group keeps its natural order.
+constexpr size_t kXcds = 8, kXcdRun = 16;
+void xcd_interleave(std::vector<WorkItem>& v, size_t b, size_t e) {
+  constexpr size_t G = kXcds * kXcdRun;
+  const size_t n = e - b, full = n / G * G;
+  if (full == 0) return;
+  std::vector<WorkItem> tmp(v.begin() + b, v.begin() + b + full);
+  for (size_t i = 0; i < full; ++i) {
+    const size_t g = i / G, o = i % G, x = o / kXcdRun, q = o % kXcdRun;
+    v[b + g * G + q * kXcds + x] = tmp[i];
+  }
+}
+
+// Longest-processing-time-first packing of items [b, e) of v (costs c) into
+// nq queues.  Rewrites v[b, e) queue after queue (each queue in decreasing
+// cost order) and appends the nq + 1 queue offsets (relative to b) to qoff.
+void lpt_pack(std::vector<WorkItem>& v, size_t b, size_t e, const std::vector<double>& c,
+              int nq, std::vector<int32_t>& qoff) {
+  const size_t n = e - b;
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), size_t(0));
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return c[x] > c[y]; });
+  using QE = std::pair<double, int>;  // (load, queue)
+  std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
+  for (int q = 0; q < nq; ++q) heap.push({0.0, q});
+  std::vector<std::vector<WorkItem>> qs(nq);
+  for (size_t k : order) {
+    QE top = heap.top();
+    heap.pop();
+    qs[top.second].push_back(v[b + k]);
+    top.first += c[k];
+    heap.push(top);
+  }
+  size_t pos = b;
+  qoff.push_back(0);
+  for (int q = 0; q < nq; ++q) {
+    for (const auto& w : qs[q]) v[pos++] = w;
+    qoff.push_back((int32_t)(pos - b));
+  }
+}
+
+struct MolInfo {
+  int32_t sh_begin = 0, nshells = 0, nbf = 0;
+  int64_t p_off = 0;
+  bool have_density = false;
+};
 }  // namespace
 
 struct xcq_context_s {
@@ -66,11 +117,18 @@ struct xcq_context_s {
   hipStream_t stream = nullptr;
   size_t arena_bytes = 0;  // user limit (0 = derive from free memory)
   double fill = 0.9;
+  int wall_khz = 100000;
 
-  // basis
+  // molecules and basis
   DevShell* d_shells = nullptr;
-  int32_t nshells = 0, nbf = 0;
-  std::vector<int32_t> sh_size, sh_first;
+  int32_t nshells = 0;
+  std::vector<int32_t> sh_size, sh_first, sh_mol;  // sh_first: molecule-local
+  std::vector<MolInfo> mols;
+  int32_t max_nbf = 0;
+  int64_t p_total = 0;
+  int32_t* d_mol_n = nullptr;
+  int64_t* d_mol_off = nullptr;
+  int64_t* d_mol_blk = nullptr;  // nmol + 1 block offsets into red
 
   // tasks
   int32_t ntasks = 0;
@@ -79,35 +137,49 @@ struct xcq_context_s {
   DevTask* d_tasks = nullptr;
   double *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_w = nullptr;
   int32_t *d_shell_list = nullptr, *d_bfmap = nullptr, *d_lsh = nullptr;
-  WorkItem* d_work = nullptr;
+
+  // work lists
   std::vector<Chunk> chunks;
+  std::vector<WorkItem> h_work;
+  std::vector<int32_t> h_qoff;
+  WorkItem* d_work = nullptr;
+  int32_t* d_qoff = nullptr;
+  int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
+  int32_t nqueues = 0;                                 // 0: one workgroup per item
+  std::vector<double> cost_d, cost_s;                  // per-task predicted costs
+  bool timing = false;
+  int64_t* d_tstamp = nullptr;
 
   // density / results
   double* d_P = nullptr;
+  uint2* d_Psplit = nullptr;
+  bool psplit_valid = false;
   double* d_vxc = nullptr;
   double* d_red = nullptr;
   double* d_out2 = nullptr;
-  bool have_density = false;
   int last_want_vxc = 0;
 
   // chunk scratch
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
-  int64_t max_nq = 0, max_scratch_elems = 0;
 
-  // packed bfloat16 slices of P for the mixed-precision contractions
-  void* d_Phi = nullptr;
+  // graph of the whole evaluation
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  int gkey = -1;
 
   // timing
   std::vector<hipEvent_t> ev;  // [chunk][stage][2]
-  double stage_ms[kStages + 1] = {0};
+  hipEvent_t ev_total[2] = {nullptr, nullptr};
+  bool last_graph = false;
   int64_t counters[4] = {0, 0, 0, 0};
 
-  ~xcq_context_s() {
-    (void)hipSetDevice(device);
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (auto e : ev) (void)hipEventDestroy(e);
-    dfree(d_shells);
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gkey = -1;
+  }
+  void free_tasks() {
     dfree(d_tasks);
     dfree(d_px);
     dfree(d_py);
@@ -117,43 +189,251 @@ struct xcq_context_s {
     dfree(d_bfmap);
     dfree(d_lsh);
     dfree(d_work);
-    dfree(d_P);
-    dfree(d_vxc);
+    dfree(d_qoff);
     dfree(d_red);
-    dfree(d_out2);
+    dfree(d_tstamp);
     if (d_scratch) (void)hipFree(d_scratch);
-    if (d_Phi) (void)hipFree(d_Phi);
+    d_scratch = nullptr;
+    scratch_bytes = 0;
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    chunks.clear();
+    h_work.clear();
+    h_qoff.clear();
+    ntasks = 0;
+  }
+  ~xcq_context_s() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    free_tasks();
+    for (auto e : ev_total)
+      if (e) (void)hipEventDestroy(e);
+    dfree(d_shells);
+    dfree(d_mol_n);
+    dfree(d_mol_off);
+    dfree(d_mol_blk);
+    dfree(d_P);
+    dfree(d_Psplit);
+    dfree(d_vxc);
+    dfree(d_out2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
-namespace xcq {
-CtxView view_of(xcq_context_s* c) {
-  return CtxView{c->stream, c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap, c->d_lsh,
-                 c->d_px,   c->d_py,    c->d_pz,     c->d_w,          c->d_work,  c->d_P,
-                 c->d_vxc,  c->nbf,     c->d_red,    c->d_scratch,
-                 c->scratch_bytes, &c->d_Phi};
-}
-}  // namespace xcq
-
 namespace {
-// The command processor hands workgroup i of a launch to XCD i % 8.  Work
-// lists are built task by task, so neighbouring items share a task's B/Z
-// panels and P_sub gathers; this permutation gives each XCD runs of kXcdRun
-// consecutive items (one L2 then sees all of a run's reuse) while keeping
-// the list's large-first order at the granularity of 8 * kXcdRun items.
-// The final partial group keeps its natural order.
-constexpr size_t kXcds = 8, kXcdRun = 16;
-void xcd_interleave(std::vector<xcq::WorkItem>& v, size_t b, size_t e) {
-  constexpr size_t G = kXcds * kXcdRun;
-  const size_t n = e - b, full = n / G * G;
-  if (full == 0) return;
-  std::vector<xcq::WorkItem> tmp(v.begin() + b, v.begin() + b + full);
-  for (size_t i = 0; i < full; ++i) {
-    const size_t g = i / G, o = i % G, x = o / kXcdRun, q = o % kXcdRun;
-    v[b + g * G + q * kXcds + x] = tmp[i];
-  }
+
+CtxView view_of(xcq_context_s* c) {
+  return CtxView{c->stream,  c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap,
+                 c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
+                 c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
+                 c->d_P,     c->d_Psplit, c->d_vxc,   c->d_red,        c->d_scratch};
 }
+
+// Heuristic per-item costs (used when no predicted task costs are set):
+// matrix-core work of the item, npts * rows * cols in the padded tile.
+double heur_dens_item(const DevTask& d) {
+  const double rows = std::ceil(d.nbe / double(d.nt_d)) * d.nt_d;
+  return (double)kPtBlock * rows * d.nbe;
+}
+double heur_syr2k_item(const DevTask& d) { return (double)d.npts_pad * d.nt_s * d.nt_s; }
+
+// Rebuild the work lists and queue tables of every chunk from the current
+// tile policy and schedule, and upload them.
+int build_work_lists(xcq_context_s* ctx) {
+  auto& W = ctx->h_work;
+  auto& Q = ctx->h_qoff;
+  W.clear();
+  Q.clear();
+  const bool have_cost = ctx->cost_d.size() == (size_t)ctx->ntasks &&
+                         ctx->cost_s.size() == (size_t)ctx->ntasks;
+  for (auto& d : ctx->h_tasks) {
+    d.nt_d = d.nbe <= ctx->dens32_max_nbe ? 32 : 64;
+    d.nt_s = d.nbe <= ctx->syr2k32_max_nbe ? 32 : 64;
+  }
+  auto finish_range = [&](WorkRange& r, const std::vector<double>& cost) {
+    r.n = (int64_t)W.size() - r.off;
+    r.nq = 0;
+    r.qoff_off = (int64_t)Q.size();
+    if (r.n == 0) return;
+    if (ctx->nqueues > 0) {
+      r.nq = (int32_t)std::min<int64_t>(ctx->nqueues, r.n);
+      lpt_pack(W, (size_t)r.off, W.size(), cost, r.nq, Q);
+    } else {
+      xcd_interleave(W, (size_t)r.off, W.size());
+    }
+  };
+  for (auto& c : ctx->chunks) {
+    for (int v = 0; v < 2; ++v) {  // density: 32-wide, then 64-wide tiles
+      const int nt = v == 0 ? 32 : 64;
+      c.dens[v].off = (int64_t)W.size();
+      std::vector<double> cost;
+      for (int32_t i = c.task_begin; i < c.task_end; ++i) {
+        const DevTask& d = ctx->h_tasks[i];
+        if (d.nt_d != nt) continue;
+        const int nmb = d.npts_pad / kPtBlock;
+        const double ci = have_cost ? ctx->cost_d[i] / nmb : heur_dens_item(d);
+        for (int pb = 0; pb < nmb; ++pb) {
+          W.push_back(WorkItem{i, pb, 0, 0});
+          cost.push_back(ci);
+        }
+      }
+      finish_range(c.dens[v], cost);
+    }
+    c.coll.off = c.dens[0].off;
+    c.coll.n = c.dens[0].n + c.dens[1].n;
+    c.coll.nq = 0;
+    for (int v = 0; v < 2; ++v) {  // SYR2K tiles
+      const int nt = v == 0 ? 32 : 64;
+      c.syr2k[v].off = (int64_t)W.size();
+      std::vector<double> cost;
+      for (int32_t i = c.task_begin; i < c.task_end; ++i) {
+        const DevTask& d = ctx->h_tasks[i];
+        if (d.nt_s != nt) continue;
+        const int nt_tiles = (d.nbe + nt - 1) / nt;
+        const int npairs = nt_tiles * (nt_tiles + 1) / 2;
+        const double ci = have_cost ? ctx->cost_s[i] / npairs : heur_syr2k_item(d);
+        for (int it = 0; it < nt_tiles; ++it)
+          for (int jt = 0; jt <= it; ++jt) {
+            W.push_back(WorkItem{i, it, jt, 0});
+            cost.push_back(ci);
+          }
+      }
+      finish_range(c.syr2k[v], cost);
+    }
+  }
+  dfree(ctx->d_work);
+  dfree(ctx->d_qoff);
+  dfree(ctx->d_tstamp);
+  int rc;
+  if ((rc = dalloc(&ctx->d_work, W.size()))) return rc;
+  if ((rc = dalloc(&ctx->d_qoff, Q.size()))) return rc;
+  if (ctx->timing && (rc = dalloc(&ctx->d_tstamp, 2 * W.size()))) return rc;
+  if (!W.empty())
+    XCQ_HIP(hipMemcpy(ctx->d_work, W.data(), W.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  if (!Q.empty())
+    XCQ_HIP(hipMemcpy(ctx->d_qoff, Q.data(), Q.size() * 4, hipMemcpyHostToDevice));
+  if (ctx->d_tstamp)
+    XCQ_HIP(hipMemset(ctx->d_tstamp, 0, 2 * W.size() * sizeof(int64_t)));
+  ctx->drop_graph();
+  return XCQ_SUCCESS;
+}
+
+// Enqueue one evaluation on the context's stream.  Events are recorded only
+// when `events` is set (not under graph capture).
+int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, bool events) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const CtxView v = view_of(ctx);
+  hipStream_t s = ctx->stream;
+  const double fac = func == 0 ? 1.0 : 2.0;
+  int rc;
+  if (want_vxc && (rc = launch_fill(s, ctx->d_vxc, ctx->p_total))) return rc;
+  int64_t launches = want_vxc ? 1 : 0, items = 0;
+  auto rec = [&](hipEvent_t e) { return events ? hipEventRecord(e, s) : hipSuccess; };
+  for (size_t ci = 0; ci < ctx->chunks.size(); ++ci) {
+    const Chunk& c = ctx->chunks[ci];
+    hipEvent_t* ev = ctx->ev.data() + ci * kStages * 2;
+    XCQ_HIP(rec(ev[0]));
+    const WorkItem* cw = v.work + c.coll.off;
+    if (precision == XCQ_PREC_FP64)
+      rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
+                                   v.py, v.pz, static_cast<double*>(v.scratch), gga);
+    else
+      rc = launch_collocation_fp32(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
+                                   v.py, v.pz, static_cast<float*>(v.scratch), gga);
+    if (rc) return rc;
+    XCQ_HIP(rec(ev[1]));
+    // X, rho/grad rho, functional, EXC/NEL and Z in one launch per tile
+    // width (stage 1); stages 2 (functional) and 3 (Z) stay as empty
+    // intervals so the stage table keeps its shape.
+    XCQ_HIP(rec(ev[2]));
+    for (int k = 0; k < 2; ++k)
+      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, precision, func, gga, fac,
+                               want_vxc)))
+        return rc;
+    XCQ_HIP(rec(ev[3]));
+    XCQ_HIP(rec(ev[4]));
+    XCQ_HIP(rec(ev[5]));
+    launches += 3;
+    items += c.coll.n;
+    if (want_vxc) {
+      XCQ_HIP(rec(ev[6]));
+      XCQ_HIP(rec(ev[7]));
+      XCQ_HIP(rec(ev[8]));
+      for (int k = 0; k < 2; ++k)
+        if ((rc = launch_syr2k(v, c.syr2k[k], k == 0 ? 32 : 64, precision))) return rc;
+      XCQ_HIP(rec(ev[9]));
+      launches += 2;
+      items += c.syr2k[0].n + c.syr2k[1].n;
+    }
+  }
+  const int32_t nmol = (int32_t)ctx->mols.size();
+  if (want_vxc) {
+    XCQ_HIP(rec(ctx->ev[10]));
+    if ((rc = launch_symmetrize(s, nmol, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off,
+                                ctx->d_vxc)))
+      return rc;
+    XCQ_HIP(rec(ctx->ev[11]));
+    ++launches;
+  }
+  if ((rc = launch_reduce_sums(s, nmol, ctx->d_mol_blk, ctx->d_red, ctx->d_out2))) return rc;
+  ctx->counters[0] = launches + 1;
+  ctx->counters[1] = items;
+  return XCQ_SUCCESS;
+}
+
+int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  for (const auto& m : ctx->mols)
+    XCQ_REQUIRE(m.have_density, XCQ_ERROR_STATE, "no density uploaded");
+  XCQ_REQUIRE(func == 0 || func == XCQ_FUNC_SVWN5 || func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
+  XCQ_REQUIRE(precision == XCQ_PREC_FP64 || precision == XCQ_PREC_MIXED,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown precision");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  if (precision == XCQ_PREC_MIXED && !ctx->psplit_valid) {
+    if (!ctx->d_Psplit) {
+      int rc = dalloc(&ctx->d_Psplit, (size_t)ctx->p_total);
+      if (rc) return rc;
+      ctx->drop_graph();
+    }
+    int rc = launch_split_P(s, ctx->p_total, ctx->d_P, ctx->d_Psplit);
+    if (rc) return rc;
+    ctx->psplit_valid = true;
+  }
+  ctx->last_want_vxc = want_vxc;
+  if (!ctx->use_graph) {
+    ctx->last_graph = false;
+    return enqueue_pipeline(ctx, func, precision, want_vxc, true);
+  }
+  // Graph mode: capture on first use of this (functional, precision,
+  // want_vxc) and replay afterwards.
+  const int key = (func * 4 + precision) * 2 + want_vxc;
+  if (!ctx->gexec || ctx->gkey != key) {
+    ctx->drop_graph();
+    XCQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_pipeline(ctx, func, precision, want_vxc, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    XCQ_HIP(e);
+    e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    XCQ_HIP(e);
+    ctx->gkey = key;
+  }
+  XCQ_HIP(hipEventRecord(ctx->ev_total[0], s));
+  XCQ_HIP(hipGraphLaunch(ctx->gexec, s));
+  XCQ_HIP(hipEventRecord(ctx->ev_total[1], s));
+  ctx->last_graph = true;
+  return XCQ_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -189,10 +469,17 @@ int xcq_context_create(int device, size_t arena_bytes, double fill_fraction, xcq
   ctx->device = device;
   ctx->arena_bytes = arena_bytes;
   ctx->fill = fill_fraction;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+      khz > 0)
+    ctx->wall_khz = khz;
+  else
+    (void)hipGetLastError();
   XCQ_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  XCQ_HIP(hipEventCreate(&ctx->ev_total[0]));
+  XCQ_HIP(hipEventCreate(&ctx->ev_total[1]));
   int rc = upload_sph_table(ctx->stream);
   if (rc) return rc;
-  if ((rc = dalloc(&ctx->d_out2, 2))) return rc;
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
   *out = ctx.release();
   return XCQ_SUCCESS;
@@ -210,59 +497,105 @@ int xcq_synchronize(xcq_context ctx) {
   return XCQ_SUCCESS;
 }
 
-int xcq_upload_basis(xcq_context ctx, int32_t nshells, const xcq_shell* shells) {
+int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshells,
+                         const xcq_shell* shells) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
-  XCQ_REQUIRE(nshells > 0 && shells, XCQ_ERROR_INVALID_ARGUMENT, "empty basis");
+  XCQ_REQUIRE(nmol > 0 && mol_nshells && shells, XCQ_ERROR_INVALID_ARGUMENT, "empty basis");
+  int32_t nshells = 0;
+  for (int32_t m = 0; m < nmol; ++m) {
+    XCQ_REQUIRE(mol_nshells[m] > 0, XCQ_ERROR_INVALID_ARGUMENT, "molecule without shells");
+    nshells += mol_nshells[m];
+  }
   XCQ_HIP(hipSetDevice(ctx->device));
   std::vector<DevShell> h(nshells);
+  std::vector<MolInfo> mols(nmol);
   ctx->sh_size.assign(nshells, 0);
-  ctx->sh_first.assign(nshells + 1, 0);
-  for (int32_t i = 0; i < nshells; ++i) {
-    const auto& s = shells[i];
-    XCQ_REQUIRE(s.l >= 0 && s.l <= kMaxL, XCQ_ERROR_INVALID_ARGUMENT,
-                "shell angular momentum out of range (0..4)");
-    XCQ_REQUIRE(s.nprim > 0 && s.nprim <= 32, XCQ_ERROR_INVALID_ARGUMENT,
-                "shell primitive count out of range (1..32)");
-    auto& d = h[i];
-    std::memset(&d, 0, sizeof(d));
-    for (int k = 0; k < s.nprim; ++k) {
-      d.alpha[k] = s.alpha[k];
-      d.coeff[k] = s.coeff[k];
+  ctx->sh_first.assign(nshells, 0);
+  ctx->sh_mol.assign(nshells, 0);
+  int32_t i = 0;
+  int64_t p_total = 0;
+  int32_t max_nbf = 0;
+  for (int32_t m = 0; m < nmol; ++m) {
+    mols[m].sh_begin = i;
+    mols[m].nshells = mol_nshells[m];
+    int32_t nbf = 0;
+    for (int32_t k = 0; k < mol_nshells[m]; ++k, ++i) {
+      const auto& s = shells[i];
+      XCQ_REQUIRE(s.l >= 0 && s.l <= kMaxL, XCQ_ERROR_INVALID_ARGUMENT,
+                  "shell angular momentum out of range (0..4)");
+      XCQ_REQUIRE(s.nprim > 0 && s.nprim <= 32, XCQ_ERROR_INVALID_ARGUMENT,
+                  "shell primitive count out of range (1..32)");
+      auto& d = h[i];
+      std::memset(&d, 0, sizeof(d));
+      for (int q = 0; q < s.nprim; ++q) {
+        d.alpha[q] = s.alpha[q];
+        d.coeff[q] = s.coeff[q];
+      }
+      for (int q = 0; q < 3; ++q) d.O[q] = s.origin[q];
+      d.nprim = s.nprim;
+      d.l = s.l;
+      d.pure = s.pure;
+      d.size = s.pure ? 2 * s.l + 1 : (s.l + 1) * (s.l + 2) / 2;
+      ctx->sh_size[i] = d.size;
+      ctx->sh_first[i] = nbf;
+      ctx->sh_mol[i] = m;
+      nbf += d.size;
     }
-    for (int k = 0; k < 3; ++k) d.O[k] = s.origin[k];
-    d.nprim = s.nprim;
-    d.l = s.l;
-    d.pure = s.pure;
-    d.size = s.pure ? 2 * s.l + 1 : (s.l + 1) * (s.l + 2) / 2;
-    ctx->sh_size[i] = d.size;
-    ctx->sh_first[i + 1] = ctx->sh_first[i] + d.size;
+    mols[m].nbf = nbf;
+    mols[m].p_off = p_total;
+    p_total += (int64_t)nbf * nbf;
+    max_nbf = std::max(max_nbf, nbf);
   }
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->drop_graph();
+  ctx->free_tasks();  // tasks refer to shell indices: require a new upload
   dfree(ctx->d_shells);
   int rc = dalloc(&ctx->d_shells, nshells);
   if (rc) return rc;
   XCQ_HIP(hipMemcpy(ctx->d_shells, h.data(), sizeof(DevShell) * nshells, hipMemcpyHostToDevice));
   ctx->nshells = nshells;
-  const int32_t nbf = ctx->sh_first[nshells];
-  if (nbf != ctx->nbf) {
+  const bool same_shape =
+      ctx->mols.size() == mols.size() &&
+      std::equal(mols.begin(), mols.end(), ctx->mols.begin(),
+                 [](const MolInfo& a, const MolInfo& b) { return a.nbf == b.nbf; });
+  if (!same_shape) {
     dfree(ctx->d_P);
+    dfree(ctx->d_Psplit);
     dfree(ctx->d_vxc);
-    if (ctx->d_Phi) {
-      (void)hipFree(ctx->d_Phi);
-      ctx->d_Phi = nullptr;
+    dfree(ctx->d_out2);
+    dfree(ctx->d_mol_n);
+    dfree(ctx->d_mol_off);
+    if ((rc = dalloc(&ctx->d_P, (size_t)p_total))) return rc;
+    if ((rc = dalloc(&ctx->d_vxc, (size_t)p_total))) return rc;
+    if ((rc = dalloc(&ctx->d_out2, (size_t)2 * nmol))) return rc;
+    if ((rc = dalloc(&ctx->d_mol_n, (size_t)nmol))) return rc;
+    if ((rc = dalloc(&ctx->d_mol_off, (size_t)nmol))) return rc;
+    std::vector<int32_t> mn(nmol);
+    std::vector<int64_t> mo(nmol);
+    for (int32_t m = 0; m < nmol; ++m) {
+      mn[m] = mols[m].nbf;
+      mo[m] = mols[m].p_off;
     }
-    ctx->nbf = nbf;
-    if ((rc = dalloc(&ctx->d_P, (size_t)nbf * nbf))) return rc;
-    if ((rc = dalloc(&ctx->d_vxc, (size_t)nbf * nbf))) return rc;
-    ctx->have_density = false;
+    XCQ_HIP(hipMemcpy(ctx->d_mol_n, mn.data(), 4 * (size_t)nmol, hipMemcpyHostToDevice));
+    XCQ_HIP(hipMemcpy(ctx->d_mol_off, mo.data(), 8 * (size_t)nmol, hipMemcpyHostToDevice));
+  } else {
+    for (int32_t m = 0; m < nmol; ++m) mols[m].have_density = ctx->mols[m].have_density;
   }
-  ctx->ntasks = 0;  // tasks refer to shell indices: require a new upload
+  ctx->mols = std::move(mols);
+  ctx->p_total = p_total;
+  ctx->max_nbf = max_nbf;
   return XCQ_SUCCESS;
 }
 
-int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int64_t npts_total,
-                     const double* points, const double* weights, int64_t nshell_entries,
-                     const int32_t* shell_lists) {
+int xcq_upload_basis(xcq_context ctx, int32_t nshells, const xcq_shell* shells) {
+  XCQ_REQUIRE(nshells > 0 && shells, XCQ_ERROR_INVALID_ARGUMENT, "empty basis");
+  return xcq_upload_molecules(ctx, 1, &nshells, shells);
+}
+
+int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* tasks,
+                             const int32_t* task_mol, int64_t npts_total, const double* points,
+                             const double* weights, int64_t nshell_entries,
+                             const int32_t* shell_lists) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_REQUIRE(ctx->nshells > 0, XCQ_ERROR_STATE, "basis must be uploaded before tasks");
   XCQ_REQUIRE(ntasks >= 0 && (ntasks == 0 || tasks), XCQ_ERROR_INVALID_ARGUMENT, "bad task table");
@@ -272,13 +605,16 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
               XCQ_ERROR_INVALID_ARGUMENT, "bad shell lists");
   XCQ_HIP(hipSetDevice(ctx->device));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  const int32_t nmol = (int32_t)ctx->mols.size();
 
   // Validate tasks and build the per-task device descriptors.
   std::vector<DevTask> ht(ntasks);
   std::vector<int32_t> bfmap, lsh;
-  int64_t pblocks = 0;
+  std::vector<int64_t> mol_blocks(nmol + 1, 0);
   for (int32_t i = 0; i < ntasks; ++i) {
     const auto& t = tasks[i];
+    const int32_t m = task_mol ? task_mol[i] : 0;
+    XCQ_REQUIRE(m >= 0 && m < nmol, XCQ_ERROR_INVALID_ARGUMENT, "task molecule out of range");
     XCQ_REQUIRE(t.npts >= 0 && t.nshells >= 0, XCQ_ERROR_INVALID_ARGUMENT, "negative task size");
     XCQ_REQUIRE(t.point_offset >= 0 && t.point_offset + t.npts <= npts_total,
                 XCQ_ERROR_INVALID_ARGUMENT, "task points out of range");
@@ -289,7 +625,10 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
     d.npts = t.npts;
     d.nshells = t.nshells;
     d.iparent = t.iparent;
-    d.npts_pad = (t.npts + kTile - 1) / kTile * kTile;
+    d.npts_pad = (t.npts + kPtBlock - 1) / kPtBlock * kPtBlock;
+    d.mol = m;
+    d.ldp = ctx->mols[m].nbf;
+    d.p_off = ctx->mols[m].p_off;
     d.pt_off = t.point_offset;
     d.sh_off = t.shell_offset;
     d.bf_off = (int64_t)bfmap.size();
@@ -299,6 +638,8 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
       const int32_t s = shell_lists[t.shell_offset + k];
       XCQ_REQUIRE(s >= 0 && s < ctx->nshells, XCQ_ERROR_INVALID_ARGUMENT,
                   "shell index out of range");
+      XCQ_REQUIRE(ctx->sh_mol[s] == m, XCQ_ERROR_INVALID_ARGUMENT,
+                  "task shell list names a shell of another molecule");
       XCQ_REQUIRE(s > prev, XCQ_ERROR_INVALID_ARGUMENT, "shell lists must be ascending");
       prev = s;
       lsh.push_back(nbe);
@@ -308,10 +649,19 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
     XCQ_REQUIRE(t.nbe == nbe, XCQ_ERROR_INVALID_ARGUMENT,
                 "task nbe does not match its shell list");
     d.nbe = nbe;
-    d.ntile = (nbe + kTile - 1) / kTile;
-    d.red_off = pblocks;
-    pblocks += d.npts_pad / kPtBlock;
+    mol_blocks[m + 1] += d.npts_pad / kPtBlock;
   }
+  // Point-block partials are numbered molecule by molecule so that each
+  // molecule's EXC/NEL is one contiguous reduction.
+  for (int32_t m = 0; m < nmol; ++m) mol_blocks[m + 1] += mol_blocks[m];
+  {
+    std::vector<int64_t> next(mol_blocks.begin(), mol_blocks.end() - 1);
+    for (auto& d : ht) {
+      d.red_off = next[d.mol];
+      next[d.mol] += d.npts_pad / kPtBlock;
+    }
+  }
+  const int64_t pblocks = mol_blocks[nmol];
 
   // Chunk planning against the arena.
   size_t free_b = 0, total_b = 0;
@@ -326,21 +676,16 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
     const double avail = ctx->fill * (double)free_b - (double)resident;
     budget = (size_t)std::max(0.0, std::min(avail * 0.5, 48.0 * (1ull << 30)));
   }
-  auto task_bytes = [](const DevTask& d) {
-    return (size_t)5 * d.nbe * d.npts_pad * 8 + (size_t)4 * d.npts_pad * 8;
-  };
+  auto task_bytes = [](const DevTask& d) { return (size_t)5 * d.nbe * d.npts_pad * 8; };
   std::vector<Chunk> chunks;
-  std::vector<WorkItem> pbl, g2;
+  int64_t max_se = 0;
   {
     int32_t i = 0;
     while (i < ntasks) {
       Chunk c;
-      std::memset(&c, 0, sizeof(c));
       c.task_begin = i;
       size_t used = 0;
-      int64_t se = 0, nq = 0;
-      c.pblk_off = (int64_t)pbl.size();
-      c.g2_off = (int64_t)g2.size();
+      int64_t se = 0;
       while (i < ntasks) {
         auto& d = ht[i];
         const size_t b = task_bytes(d);
@@ -351,63 +696,25 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
         }
         used += b;
         d.b_off = se;
-        d.q_off = nq;
         se += (int64_t)5 * d.nbe * d.npts_pad;
-        nq += d.npts_pad;
-        const int nmb = d.npts_pad / kPtBlock;
-        for (int pb = 0; pb < nmb; ++pb) pbl.push_back(WorkItem{i, pb, 0, 0});
-        for (int it = 0; it < d.ntile; ++it)
-          for (int jt = 0; jt <= it; ++jt) g2.push_back(WorkItem{i, it, jt, 0});
         ++i;
       }
       c.task_end = i;
       c.scratch_elems = se;
-      c.nq = nq;
-      c.npblk = (int64_t)pbl.size() - c.pblk_off;
-      c.ng2 = (int64_t)g2.size() - c.g2_off;
-      xcd_interleave(pbl, (size_t)c.pblk_off, pbl.size());
-      xcd_interleave(g2, (size_t)c.g2_off, g2.size());
+      max_se = std::max(max_se, se);
       chunks.push_back(c);
     }
   }
 
   // Release the previous task set and allocate the new one.
-  dfree(ctx->d_tasks);
-  dfree(ctx->d_px);
-  dfree(ctx->d_py);
-  dfree(ctx->d_pz);
-  dfree(ctx->d_w);
-  dfree(ctx->d_shell_list);
-  dfree(ctx->d_bfmap);
-  dfree(ctx->d_lsh);
-  dfree(ctx->d_work);
-  dfree(ctx->d_red);
-  if (ctx->d_scratch) {
-    (void)hipFree(ctx->d_scratch);
-    ctx->d_scratch = nullptr;
-  }
-  for (auto e : ctx->ev) (void)hipEventDestroy(e);
-  ctx->ev.clear();
-  ctx->ntasks = 0;
-
-  int64_t max_se = 0, max_nq = 0;
-  for (const auto& c : chunks) {
-    max_se = std::max(max_se, c.scratch_elems);
-    max_nq = std::max(max_nq, c.nq);
-  }
+  ctx->drop_graph();
+  ctx->free_tasks();
   std::vector<double> hx(npts_total), hy(npts_total), hz(npts_total);
   for (int64_t k = 0; k < npts_total; ++k) {
     hx[k] = points[3 * k];
     hy[k] = points[3 * k + 1];
     hz[k] = points[3 * k + 2];
   }
-  std::vector<WorkItem> work;
-  work.reserve(pbl.size() + g2.size());
-  const int64_t base_g2 = (int64_t)pbl.size();
-  work.insert(work.end(), pbl.begin(), pbl.end());
-  work.insert(work.end(), g2.begin(), g2.end());
-  for (auto& c : chunks) c.g2_off += base_g2;
-
   int rc;
   if ((rc = dalloc(&ctx->d_tasks, ht.size()))) return rc;
   if ((rc = dalloc(&ctx->d_px, npts_total))) return rc;
@@ -417,8 +724,9 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   if ((rc = dalloc(&ctx->d_shell_list, (size_t)nshell_entries))) return rc;
   if ((rc = dalloc(&ctx->d_bfmap, bfmap.size()))) return rc;
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
-  if ((rc = dalloc(&ctx->d_work, work.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
+  dfree(ctx->d_mol_blk);
+  if ((rc = dalloc(&ctx->d_mol_blk, mol_blocks.size()))) return rc;
   ctx->scratch_bytes = (size_t)max_se * sizeof(double);
   if (ctx->scratch_bytes) {
     hipError_t e = hipMalloc(&ctx->d_scratch, ctx->scratch_bytes);
@@ -431,7 +739,6 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   auto h2d = [](void* d, const void* h, size_t b) {
     return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess;
   };
-  XCQ_HIP(h2d(ctx->d_tasks, ht.data(), ht.size() * sizeof(DevTask)));
   XCQ_HIP(h2d(ctx->d_px, hx.data(), npts_total * 8));
   XCQ_HIP(h2d(ctx->d_py, hy.data(), npts_total * 8));
   XCQ_HIP(h2d(ctx->d_pz, hz.data(), npts_total * 8));
@@ -439,7 +746,7 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   XCQ_HIP(h2d(ctx->d_shell_list, shell_lists, nshell_entries * 4));
   XCQ_HIP(h2d(ctx->d_bfmap, bfmap.data(), bfmap.size() * 4));
   XCQ_HIP(h2d(ctx->d_lsh, lsh.data(), lsh.size() * 4));
-  XCQ_HIP(h2d(ctx->d_work, work.data(), work.size() * sizeof(WorkItem)));
+  XCQ_HIP(h2d(ctx->d_mol_blk, mol_blocks.data(), mol_blocks.size() * 8));
 
   ctx->ev.resize(chunks.size() * kStages * 2);
   for (auto& e : ctx->ev) XCQ_HIP(hipEventCreate(&e));
@@ -449,8 +756,10 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   ctx->ntasks = ntasks;
   ctx->npts_total = npts_total;
   ctx->total_pblocks = pblocks;
-  ctx->max_nq = max_nq;
-  ctx->max_scratch_elems = max_se;
+  ctx->cost_d.clear();
+  ctx->cost_s.clear();
+  if ((rc = build_work_lists(ctx))) return rc;
+  XCQ_HIP(h2d(ctx->d_tasks, ctx->h_tasks.data(), ctx->h_tasks.size() * sizeof(DevTask)));
   int64_t sum_pts = 0, sum_pnn = 0;
   for (const auto& d : ctx->h_tasks) {
     sum_pts += d.npts;
@@ -461,96 +770,120 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
   return XCQ_SUCCESS;
 }
 
-int xcq_upload_density(xcq_context ctx, int32_t nbf, const double* P, int64_t ldp) {
+int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int64_t npts_total,
+                     const double* points, const double* weights, int64_t nshell_entries,
+                     const int32_t* shell_lists) {
+  return xcq_upload_tasks_batched(ctx, ntasks, tasks, nullptr, npts_total, points, weights,
+                                  nshell_entries, shell_lists);
+}
+
+int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32_max_nbe) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
-  XCQ_REQUIRE(P, XCQ_ERROR_INVALID_ARGUMENT, "null density");
-  XCQ_REQUIRE(nbf == ctx->nbf && nbf > 0, XCQ_ERROR_INVALID_ARGUMENT,
-              "density dimension does not match the basis");
-  XCQ_REQUIRE(ldp >= nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldp < nbf");
   XCQ_HIP(hipSetDevice(ctx->device));
-  XCQ_HIP(hipMemcpy2DAsync(ctx->d_P, (size_t)nbf * 8, P, (size_t)ldp * 8, (size_t)nbf * 8, nbf,
-                           hipMemcpyHostToDevice, ctx->stream));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->have_density = true;
-  if (ctx->d_Phi) {  // invalidate the split copy of the previous density
-    (void)hipFree(ctx->d_Phi);
-    ctx->d_Phi = nullptr;
+  ctx->dens32_max_nbe = dens32_max_nbe;
+  ctx->syr2k32_max_nbe = syr2k32_max_nbe;
+  if (ctx->ntasks == 0) return XCQ_SUCCESS;
+  int rc = build_work_lists(ctx);
+  if (rc) return rc;
+  XCQ_HIP(hipMemcpy(ctx->d_tasks, ctx->h_tasks.data(), ctx->h_tasks.size() * sizeof(DevTask),
+                    hipMemcpyHostToDevice));
+  return XCQ_SUCCESS;
+}
+
+int xcq_set_schedule(xcq_context ctx, int32_t nqueues, const double* dens_cost,
+                     const double* syr2k_cost) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(nqueues >= 0, XCQ_ERROR_INVALID_ARGUMENT, "negative queue count");
+  XCQ_REQUIRE((dens_cost == nullptr) == (syr2k_cost == nullptr), XCQ_ERROR_INVALID_ARGUMENT,
+              "give both cost arrays or neither");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->nqueues = nqueues;
+  if (dens_cost) {
+    XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "costs need an uploaded task set");
+    ctx->cost_d.assign(dens_cost, dens_cost + ctx->ntasks);
+    ctx->cost_s.assign(syr2k_cost, syr2k_cost + ctx->ntasks);
+  } else {
+    ctx->cost_d.clear();
+    ctx->cost_s.clear();
+  }
+  if (ctx->ntasks == 0) return XCQ_SUCCESS;
+  return build_work_lists(ctx);
+}
+
+int xcq_enable_task_timing(xcq_context ctx, int enable) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->timing = enable != 0;
+  dfree(ctx->d_tstamp);
+  ctx->drop_graph();
+  if (ctx->timing && !ctx->h_work.empty()) {
+    int rc = dalloc(&ctx->d_tstamp, 2 * ctx->h_work.size());
+    if (rc) return rc;
+    XCQ_HIP(hipMemset(ctx->d_tstamp, 0, 2 * ctx->h_work.size() * sizeof(int64_t)));
   }
   return XCQ_SUCCESS;
 }
 
-static int run_pipeline(xcq_context ctx, int func, int precision, int want_vxc) {
-  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
-  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
-  XCQ_REQUIRE(ctx->have_density, XCQ_ERROR_STATE, "no density uploaded");
-  XCQ_REQUIRE(func == 0 || func == XCQ_FUNC_SVWN5 || func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0,
-              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
-  XCQ_REQUIRE(precision == XCQ_PREC_FP64 || precision == XCQ_PREC_MIXED,
-              XCQ_ERROR_INVALID_ARGUMENT, "unknown precision");
+int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us) {
+  XCQ_REQUIRE(ctx && dens_us && syr2k_us, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  XCQ_REQUIRE(ctx->timing && ctx->d_tstamp, XCQ_ERROR_STATE, "task timing is not enabled");
   XCQ_HIP(hipSetDevice(ctx->device));
-  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
-  hipStream_t s = ctx->stream;
-  if (want_vxc)
-    XCQ_HIP(hipMemsetAsync(ctx->d_vxc, 0, (size_t)ctx->nbf * ctx->nbf * 8, s));
-  int64_t launches = 0, tiles = 0;
-  for (size_t ci = 0; ci < ctx->chunks.size(); ++ci) {
-    const Chunk& c = ctx->chunks[ci];
-    hipEvent_t* ev = ctx->ev.data() + ci * kStages * 2;
-    int rc;
-    if (precision == XCQ_PREC_MIXED) {
-      rc = launch_mixed_pipeline(ctx, c, func, gga, want_vxc, ev);
-      if (rc) return rc;
-      launches += want_vxc ? 4 : 3;
-      tiles += c.npblk + (want_vxc ? c.ng2 : 0);
-      continue;
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int64_t> ts(2 * ctx->h_work.size());
+  XCQ_HIP(hipMemcpy(ts.data(), ctx->d_tstamp, ts.size() * 8, hipMemcpyDeviceToHost));
+  std::fill(dens_us, dens_us + ctx->ntasks, 0.0);
+  std::fill(syr2k_us, syr2k_us + ctx->ntasks, 0.0);
+  const double us_per_tick = 1000.0 / ctx->wall_khz;
+  auto add = [&](const WorkRange& r, double* out) {
+    for (int64_t k = r.off; k < r.off + r.n; ++k) {
+      const int64_t dt = ts[2 * k + 1] - ts[2 * k];
+      if (dt > 0) out[ctx->h_work[k].task] += dt * us_per_tick;
     }
-    double* scr = static_cast<double*>(ctx->d_scratch);
-    XCQ_HIP(hipEventRecord(ev[0], s));
-    rc = launch_collocation_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_shells,
-                                 ctx->d_shell_list, ctx->d_lsh, ctx->d_px, ctx->d_py, ctx->d_pz,
-                                 scr, gga);
-    if (rc) return rc;
-    XCQ_HIP(hipEventRecord(ev[1], s));
-    XCQ_HIP(hipEventRecord(ev[2], s));
-    // X, rho/grad rho, functional, EXC/NEL and Z in one launch (stage 1);
-    // stages 2 (functional) and 3 (Z) stay as empty intervals so the stage
-    // table keeps its shape.
-    rc = launch_density_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, ctx->d_P, ctx->nbf,
-                             scr, ctx->d_w, func, gga, func == 0 ? 1.0 : 2.0, want_vxc,
-                             ctx->d_red);
-    if (rc) return rc;
-    XCQ_HIP(hipEventRecord(ev[3], s));
-    XCQ_HIP(hipEventRecord(ev[4], s));
-    XCQ_HIP(hipEventRecord(ev[5], s));
-    launches += 2;
-    tiles += c.npblk;
-    if (want_vxc) {
-      XCQ_HIP(hipEventRecord(ev[6], s));
-      XCQ_HIP(hipEventRecord(ev[7], s));
-      XCQ_HIP(hipEventRecord(ev[8], s));
-      rc = launch_syr2k_scatter_fp64(s, c, ctx->d_work, ctx->d_tasks, ctx->d_bfmap, scr,
-                                     ctx->d_vxc, ctx->nbf);
-      if (rc) return rc;
-      XCQ_HIP(hipEventRecord(ev[9], s));
-      launches += 1;
-      tiles += c.ng2;
+  };
+  for (const auto& c : ctx->chunks)
+    for (int v = 0; v < 2; ++v) {
+      add(c.dens[v], dens_us);
+      add(c.syr2k[v], syr2k_us);
     }
-  }
-  hipEvent_t* evs = ctx->ev.data();  // reuse chunk 0 slots 10/11 for symmetrise
-  int rc = 0;
-  if (want_vxc) {
-    XCQ_HIP(hipEventRecord(evs[10], s));
-    rc = launch_symmetrize(s, ctx->nbf, ctx->d_vxc, ctx->nbf);
-    if (rc) return rc;
-    XCQ_HIP(hipEventRecord(evs[11], s));
-    ++launches;
-  }
-  rc = launch_reduce_sums(s, ctx->d_red, ctx->total_pblocks, ctx->d_out2);
-  if (rc) return rc;
-  ctx->counters[0] = launches + 1;
-  ctx->counters[1] = tiles;
-  ctx->last_want_vxc = want_vxc;
   return XCQ_SUCCESS;
+}
+
+int xcq_use_graph(xcq_context ctx, int enable) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->use_graph = enable != 0;
+  if (!ctx->use_graph) ctx->drop_graph();
+  return XCQ_SUCCESS;
+}
+
+int xcq_upload_density_mol(xcq_context ctx, int32_t mol, int32_t nbf, const double* P,
+                           int64_t ldp) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(P, XCQ_ERROR_INVALID_ARGUMENT, "null density");
+  XCQ_REQUIRE(mol >= 0 && mol < (int32_t)ctx->mols.size(), XCQ_ERROR_INVALID_ARGUMENT,
+              "molecule index out of range");
+  MolInfo& M = ctx->mols[mol];
+  XCQ_REQUIRE(nbf == M.nbf && nbf > 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "density dimension does not match the basis");
+  XCQ_REQUIRE(ldp >= nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldp < nbf");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipMemcpy2DAsync(ctx->d_P + M.p_off, (size_t)nbf * 8, P, (size_t)ldp * 8,
+                           (size_t)nbf * 8, nbf, hipMemcpyHostToDevice, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  M.have_density = true;
+  ctx->psplit_valid = false;  // the bfloat16 slices follow the new density
+  return XCQ_SUCCESS;
+}
+
+int xcq_upload_density(xcq_context ctx, int32_t nbf, const double* P, int64_t ldp) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "context holds several molecules: use xcq_upload_density_mol");
+  return xcq_upload_density_mol(ctx, 0, nbf, P, ldp);
 }
 
 int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc) {
@@ -560,6 +893,8 @@ int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc) {
 
 int xcq_integrate_den(xcq_context ctx, double* nel) {
   XCQ_REQUIRE(nel, XCQ_ERROR_INVALID_ARGUMENT, "null output");
+  XCQ_REQUIRE(ctx && ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "integrate_den needs a single-molecule context");
   int rc = run_pipeline(ctx, 0, XCQ_PREC_FP64, 0);
   if (rc) return rc;
   double out[2];
@@ -569,17 +904,20 @@ int xcq_integrate_den(xcq_context ctx, double* nel) {
   return XCQ_SUCCESS;
 }
 
-int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc) {
+int xcq_download_results_mol(xcq_context ctx, int32_t mol, double* exc, double* nel,
+                             double* vxc, int64_t ldvxc) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(mol >= 0 && mol < (int32_t)ctx->mols.size(), XCQ_ERROR_INVALID_ARGUMENT,
+              "molecule index out of range");
+  const MolInfo& M = ctx->mols[mol];
   XCQ_HIP(hipSetDevice(ctx->device));
   double out[2];
-  XCQ_HIP(hipMemcpyAsync(out, ctx->d_out2, 16, hipMemcpyDeviceToHost, ctx->stream));
+  XCQ_HIP(hipMemcpyAsync(out, ctx->d_out2 + 2 * mol, 16, hipMemcpyDeviceToHost, ctx->stream));
   if (vxc) {
     XCQ_REQUIRE(ctx->last_want_vxc, XCQ_ERROR_STATE, "last evaluation did not compute VXC");
-    XCQ_REQUIRE(ldvxc >= ctx->nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldvxc < nbf");
-    XCQ_HIP(hipMemcpy2DAsync(vxc, (size_t)ldvxc * 8, ctx->d_vxc, (size_t)ctx->nbf * 8,
-                             (size_t)ctx->nbf * 8, ctx->nbf, hipMemcpyDeviceToHost,
-                             ctx->stream));
+    XCQ_REQUIRE(ldvxc >= M.nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldvxc < nbf");
+    XCQ_HIP(hipMemcpy2DAsync(vxc, (size_t)ldvxc * 8, ctx->d_vxc + M.p_off, (size_t)M.nbf * 8,
+                             (size_t)M.nbf * 8, M.nbf, hipMemcpyDeviceToHost, ctx->stream));
   }
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
   if (exc) *exc = out[0];
@@ -587,25 +925,41 @@ int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc,
   return XCQ_SUCCESS;
 }
 
+int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "context holds several molecules: use xcq_download_results_mol");
+  return xcq_download_results_mol(ctx, 0, exc, nel, vxc, ldvxc);
+}
+
 int xcq_stage_times(xcq_context ctx, double* ms, int n) {
   XCQ_REQUIRE(ctx && ms, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
   XCQ_HIP(hipSetDevice(ctx->device));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
   double st[kStages + 1] = {0};
-  const int nst = ctx->last_want_vxc ? 5 : 3;
-  for (size_t ci = 0; ci < ctx->chunks.size(); ++ci)
-    for (int k = 0; k < nst; ++k) {
+  if (ctx->last_graph) {
+    // A replayed graph has no per-stage events: report the total only.
+    float f = 0.f;
+    if (hipEventElapsedTime(&f, ctx->ev_total[0], ctx->ev_total[1]) == hipSuccess)
+      st[kStages] = f;
+    else
+      (void)hipGetLastError();
+  } else {
+    const int nst = ctx->last_want_vxc ? 5 : 3;
+    for (size_t ci = 0; ci < ctx->chunks.size(); ++ci)
+      for (int k = 0; k < nst; ++k) {
+        float f = 0.f;
+        hipEvent_t* ev = ctx->ev.data() + ci * kStages * 2;
+        if (hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]) == hipSuccess) st[k] += f;
+        else (void)hipGetLastError();
+      }
+    if (ctx->last_want_vxc && !ctx->ev.empty()) {
       float f = 0.f;
-      hipEvent_t* ev = ctx->ev.data() + ci * kStages * 2;
-      if (hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]) == hipSuccess) st[k] += f;
+      if (hipEventElapsedTime(&f, ctx->ev[10], ctx->ev[11]) == hipSuccess) st[5] = f;
       else (void)hipGetLastError();
     }
-  if (ctx->last_want_vxc && !ctx->ev.empty()) {
-    float f = 0.f;
-    if (hipEventElapsedTime(&f, ctx->ev[10], ctx->ev[11]) == hipSuccess) st[5] = f;
-    else (void)hipGetLastError();
+    for (int k = 0; k < kStages; ++k) st[kStages] += st[k];
   }
-  for (int k = 0; k < kStages; ++k) st[kStages] += st[k];
   const int m = std::min(n, kStages + 1);
   for (int k = 0; k < m; ++k) ms[k] = st[k];
   return m;
@@ -698,6 +1052,8 @@ int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t 
 
 int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xyz) {
   XCQ_REQUIRE(ctx && ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "SSF on resident tasks needs a single-molecule context");
   XCQ_REQUIRE(natoms > 0 && atom_xyz, XCQ_ERROR_INVALID_ARGUMENT, "empty molecule");
   XCQ_HIP(hipSetDevice(ctx->device));
   std::vector<int32_t> parent(ctx->npts_total);
@@ -763,6 +1119,7 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
   XCQ_REQUIRE(precision == XCQ_PREC_FP64 || precision == XCQ_PREC_MIXED,
               XCQ_ERROR_INVALID_ARGUMENT, "unknown precision");
   XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
   const DevTask& d = ctx->h_tasks[task];
   const int nmb = d.npts_pad / kPtBlock;
   std::vector<WorkItem> items(nmb);
@@ -770,18 +1127,16 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
   WorkItem* d_items = nullptr;
   int rc = dalloc(&d_items, items.size());
   if (rc) return rc;
-  Chunk c;
-  std::memset(&c, 0, sizeof(c));
-  c.npblk = nmb;
   const int ncomp = deriv ? 4 : 1;
   const int64_t plane = (int64_t)d.nbe * d.npts_pad;
   std::vector<double> host((size_t)ncomp * plane);
-  hipError_t e = hipMemcpy(d_items, items.data(), items.size() * sizeof(WorkItem),
-                           hipMemcpyHostToDevice);
+  hipError_t e = nmb ? hipMemcpy(d_items, items.data(), items.size() * sizeof(WorkItem),
+                                 hipMemcpyHostToDevice)
+                     : hipSuccess;
   if (e == hipSuccess && nmb > 0) {
     if (precision == XCQ_PREC_FP64) {
       double* scr = static_cast<double*>(ctx->d_scratch);
-      rc = launch_collocation_fp64(ctx->stream, c, d_items, ctx->d_tasks, ctx->d_shells,
+      rc = launch_collocation_fp64(ctx->stream, d_items, nmb, ctx->d_tasks, ctx->d_shells,
                                    ctx->d_shell_list, ctx->d_lsh, ctx->d_px, ctx->d_py,
                                    ctx->d_pz, scr, deriv);
       if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
@@ -790,7 +1145,7 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
     } else {
       // FP32 collocation: same layout in float inside the scratch.
       float* scr = static_cast<float*>(ctx->d_scratch);
-      rc = launch_collocation_fp32(ctx->stream, c, d_items, ctx->d_tasks, ctx->d_shells,
+      rc = launch_collocation_fp32(ctx->stream, d_items, nmb, ctx->d_tasks, ctx->d_shells,
                                    ctx->d_shell_list, ctx->d_lsh, ctx->d_px, ctx->d_py,
                                    ctx->d_pz, scr, deriv);
       if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);

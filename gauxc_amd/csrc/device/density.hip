@@ -24,6 +24,11 @@
 // read at a clamped (valid) index and zeroed by a select, so the loads are
 // branch-free.  K indices are uniform across a wave, which makes the
 // basis-map lookups scalar loads.
+//
+// Both kernels are templated on the function tile NT (32 or 64): a task's
+// rows are covered by ceil(nbe / NT) tiles, so small tasks are not padded to
+// 64.  A launch either runs one workgroup per (task, block) item or, in
+// queued mode, one persistent workgroup per queue of items.
 #include "internal.hpp"
 #include "functional.hpp"
 
@@ -72,8 +77,8 @@ __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlan
 // per-point partials in LDS.  frow(r) is the tile row (function) of
 // accumulator register r of row block bi for this lane; the column (point)
 // of lane l in column block bj is wn*32 + bj*16 + (l & 15).
-template <typename T, typename FRow>
-__device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow, int m0, int n0,
+template <typename T, int BI, typename FRow>
+__device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow, int m0, int n0,
                                           int wm, int wn, int lane, int nbe, int ld, int plane,
                                           const T* Bm, int gga, double (*dens)[64][4]) {
 #pragma unroll
@@ -81,12 +86,12 @@ __device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow,
     const int pl = wn * 32 + bj * 16 + (lane & 15);
     const int m = m0 + pl;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
-    T v0[2][4], v1[2][4], v2[2][4], v3[2][4];
+    T v0[BI][4], v1[BI][4], v2[BI][4], v3[BI][4];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = n0 + wm * 32 + bi * 16 + frow(r);
+        const int j = n0 + wm * 16 * BI + bi * 16 + frow(r);
         const int o = min(j, nbe - 1) * ld + m;
         v0[bi][r] = Bm[o];
         if (gga) {
@@ -96,7 +101,7 @@ __device__ __forceinline__ void fold_tile(const double (&x)[2][2][4], FRow frow,
         }
       }
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         // Rows past nbe carry X = 0 (their P_sub column was zeroed), so the
@@ -211,95 +216,108 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
 
 // ---------------------------------------------------------------------------
 // FP64 (v_mfma_f64_16x16x4_f64; accumulator register r of lane l holds row
-// 4r + l/16).  Waves: wm = function half, wn = point half of the tile.
-// Loader roles: thread (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of
-// column c of both stages (B^T: point c; P_sub: function n0 + c).
+// 4r + l/16).  Waves: wm = function half (NT/2 rows = BI blocks of 16),
+// wn = point half of the 64-point block.  Loader roles: thread
+// (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of column c of the B^T
+// stage (point c) and, for c < NT, of the P_sub stage (function n0 + c).
+template <int NT>
 __global__ __launch_bounds__(256) void k_density_f64(
-    const WorkItem* __restrict__ pblk, const DevTask* __restrict__ tasks,
-    const int32_t* __restrict__ bfmap, const double* __restrict__ P, int64_t ldp,
-    double* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
-    double fac, int want_z, double* __restrict__ red) {
-  __shared__ double Pt[2][16][kPad64];  // [buf][k][function]: A operand (P_sub)
+    QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
+    const double* __restrict__ Pall, double* __restrict__ scratch,
+    const double* __restrict__ weights, int func, int gga, double fac, int want_z,
+    double* __restrict__ red) {
+  constexpr int BI = NT / 32;
+  __shared__ double Pt[2][16][NT + 4];  // [buf][k][function]: A operand (P_sub)
   __shared__ double Bt[2][16][kPad64];  // [buf][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
-  const WorkItem wi = pblk[blockIdx.x];
-  const DevTask t = tasks[wi.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = wi.a * kPtBlock;
-  const int ld = t.npts_pad;
-  const int nbe = t.nbe;
-  const int plane = nbe * ld;
-  double* Bm = scratch + t.b_off;
-  const int32_t* bf = bfmap + t.bf_off;
-  for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
-  __syncthreads();
   const int lc = lane, lq = wave;  // loader column / K quad
-  const double* bcol = Bm + m0 + lc;
   auto frow = [&](int r) { return 4 * r + (lane >> 4); };
-  const int nks = (nbe + 15) / 16;
 
-  for (int n0 = 0; n0 < nbe; n0 += 64) {
-    const int ncol = n0 + lc;
-    const bool colok = ncol < nbe;
-    const double* Pc = P + bf[min(ncol, nbe - 1)];  // column bf[n] of P (P symmetric)
-    double rb[4], rp[4];
-    auto load = [&](int k0) {
+  XCQ_FOR_EACH_ITEM(qa, it) {
+    const int64_t t0 = qa.tstamp ? wall_clock64() : 0;
+    const WorkItem wi = qa.items[it];
+    const DevTask t = tasks[wi.task];
+    const int m0 = wi.a * kPtBlock;
+    const int ld = t.npts_pad;
+    const int nbe = t.nbe;
+    const int plane = nbe * ld;
+    const int64_t ldp = t.ldp;
+    const double* P = Pall + t.p_off;
+    double* Bm = scratch + t.b_off;
+    const int32_t* bf = bfmap + t.bf_off;
+    for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    __syncthreads();
+    const double* bcol = Bm + m0 + lc;
+    const int nks = (nbe + 15) / 16;
+
+    for (int n0 = 0; n0 < nbe; n0 += NT) {
+      const int ncol = n0 + lc;
+      const bool colok = lc < NT && ncol < nbe;
+      const double* Pc = P + bf[min(ncol, nbe - 1)];  // column bf[n] of P (P symmetric)
+      double rb[4], rp[4];
+      auto load = [&](int k0) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + 4 * lq + e;  // wave-uniform
-        const int kc = min(k, nbe - 1);
-        const double b = bcol[kc * ld];
-        const double p = Pc[(int64_t)bf[kc] * ldp];
-        rb[e] = k < nbe ? b : 0.;
-        rp[e] = (k < nbe && colok) ? fac * p : 0.;
+        for (int e = 0; e < 4; ++e) {
+          const int k = k0 + 4 * lq + e;  // wave-uniform
+          const int kc = min(k, nbe - 1);
+          const double b = bcol[kc * ld];
+          const double p = Pc[(int64_t)bf[kc] * ldp];
+          rb[e] = k < nbe ? b : 0.;
+          rp[e] = (k < nbe && colok) ? fac * p : 0.;
+        }
+      };
+      v4d acc[BI][2];
+#pragma unroll
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
+      load(0);
+      for (int s = 0; s < nks; ++s) {
+        const int buf = s & 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          Bt[buf][4 * lq + e][lc] = rb[e];
+          if (lc < NT) Pt[buf][4 * lq + e][lc] = rp[e];
+        }
+        __syncthreads();
+        if (s + 1 < nks) load(16 * (s + 1));
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const int kr = ks * 4 + (lane >> 4);
+          double a[BI];
+#pragma unroll
+          for (int bi = 0; bi < BI; ++bi) a[bi] = Pt[buf][kr][wm * 16 * BI + bi * 16 + (lane & 15)];
+          const double b0 = Bt[buf][kr][wn * 32 + (lane & 15)];
+          const double b1 = Bt[buf][kr][wn * 32 + 16 + (lane & 15)];
+#pragma unroll
+          for (int bi = 0; bi < BI; ++bi) {
+            acc[bi][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b0, acc[bi][0], 0, 0, 0);
+            acc[bi][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b1, acc[bi][1], 0, 0, 0);
+          }
+        }
       }
-    };
-    v4d acc[2][2];
+      double x[BI][2][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
-    load(0);
-    for (int s = 0; s < nks; ++s) {
-      const int buf = s & 1;
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        Bt[buf][4 * lq + e][lc] = rb[e];
-        Pt[buf][4 * lq + e][lc] = rp[e];
-      }
+          for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
+      fold_tile<double, BI>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      // The next tile's first stage write targets buffer 0, last read at step
+      // nks-2 (or nks-1 when nks == 1); all waves pass this barrier after it.
       __syncthreads();
-      if (s + 1 < nks) load(16 * (s + 1));
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        const double a0 = Pt[buf][kr][wm * 32 + (lane & 15)];
-        const double a1 = Pt[buf][kr][wm * 32 + 16 + (lane & 15)];
-        const double b0 = Bt[buf][kr][wn * 32 + (lane & 15)];
-        const double b1 = Bt[buf][kr][wn * 32 + 16 + (lane & 15)];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-      }
     }
-    double x[2][2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
-    fold_tile<double>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
-    // The next tile's first stage write targets buffer 0, last read at step
-    // nks-2 (or nks-1 when nks == 1); all waves pass this barrier after it.
+    finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+    if (want_z) {
+      __syncthreads();
+      write_z<double>(t, m0, tid, zc, gga, Bm);
+    }
+    stamp_item(qa, it, t0);
     __syncthreads();
-  }
-  finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
-  if (want_z) {
-    __syncthreads();
-    write_z<double>(t, m0, tid, zc, gga, Bm);
   }
 }
 
@@ -307,151 +325,168 @@ __global__ __launch_bounds__(256) void k_density_f64(
 // Mixed precision (v_mfma_f32_16x16x32_bf16, three-slice operands, FP32
 // inside one K block, FP64 across; accumulator register r of lane l holds
 // row 4(l/16) + r).  Loader roles: thread (c = tid & 63, g = tid >> 6)
-// loads K rows 8g..8g+7 of column c of both stages.
+// loads K rows 8g..8g+7 of column c of the B^T stage and, for c < NT, of
+// the P_sub stage.
+template <int NT>
 __global__ __launch_bounds__(256) void k_density_mx(
-    const WorkItem* __restrict__ pblk, const DevTask* __restrict__ tasks,
-    const int32_t* __restrict__ bfmap, const uint2* __restrict__ Ps, int64_t ldp,
-    float* __restrict__ scratch, const double* __restrict__ weights, int func, int gga,
-    double fac, int want_z, double* __restrict__ red) {
-  __shared__ __attribute__((aligned(16))) unsigned short Pt[2][3][64][kRowMx];  // [s][fn][k]
+    QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
+    const uint2* __restrict__ Psall, float* __restrict__ scratch,
+    const double* __restrict__ weights, int func, int gga, double fac, int want_z,
+    double* __restrict__ red) {
+  constexpr int BI = NT / 32;
+  __shared__ __attribute__((aligned(16))) unsigned short Pt[2][3][NT][kRowMx];  // [s][fn][k]
   __shared__ __attribute__((aligned(16))) unsigned short Bt[2][3][64][kRowMx];  // [s][pt][k]
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
-  const WorkItem wi = pblk[blockIdx.x];
-  const DevTask t = tasks[wi.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = wi.a * kPtBlock;
-  const int ld = t.npts_pad;
-  const int nbe = t.nbe;
-  const int plane = nbe * ld;
-  float* Bm = scratch + t.b_off;
-  const int32_t* bf = bfmap + t.bf_off;
-  for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
-  __syncthreads();
   const int lc = lane, lg = wave;
-  const float* bcol = Bm + m0 + lc;
   const int kk = 8 * (lane >> 4);
   auto frow = [&](int r) { return 4 * (lane >> 4) + r; };
-  const int nks = (nbe + 31) / 32;
 
-  for (int n0 = 0; n0 < nbe; n0 += 64) {
-    // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
-    // neighbouring lanes (neighbouring n) touch neighbouring columns.
-    const int ncol = n0 + lc;
-    const bool colok = ncol < nbe;
-    const uint2* Pc = Ps + bf[min(ncol, nbe - 1)];
-    float rb[8];
-    uint2 rp[8];
-    auto load = [&](int k0) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + 8 * lg + e;  // wave-uniform
-        const int kc = min(k, nbe - 1);
-        const float b = bcol[kc * ld];
-        const uint2 p = Pc[(int64_t)bf[kc] * ldp];
-        rb[e] = k < nbe ? b : 0.f;
-        rp[e] = (k < nbe && colok) ? p : make_uint2(0u, 0u);
-      }
-    };
-    double acc[2][2][4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
-    load(0);
-    for (int s = 0; s < nks; ++s) {
-      const int buf = s & 1;
-      {
-        us8 s1, s2, s3, p1, p2, p3;
+  XCQ_FOR_EACH_ITEM(qa, it) {
+    const int64_t t0 = qa.tstamp ? wall_clock64() : 0;
+    const WorkItem wi = qa.items[it];
+    const DevTask t = tasks[wi.task];
+    const int m0 = wi.a * kPtBlock;
+    const int ld = t.npts_pad;
+    const int nbe = t.nbe;
+    const int plane = nbe * ld;
+    const int64_t ldp = t.ldp;
+    const uint2* Ps = Psall + t.p_off;
+    float* Bm = scratch + t.b_off;
+    const int32_t* bf = bfmap + t.bf_off;
+    for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    __syncthreads();
+    const float* bcol = Bm + m0 + lc;
+    const int nks = (nbe + 31) / 32;
+
+    for (int n0 = 0; n0 < nbe; n0 += NT) {
+      // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
+      // neighbouring lanes (neighbouring n) touch neighbouring columns.
+      const int ncol = n0 + lc;
+      const bool colok = lc < NT && ncol < nbe;
+      const uint2* Pc = Ps + bf[min(ncol, nbe - 1)];
+      float rb[8];
+      uint2 rp[8];
+      auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          unsigned short a, b, c;
-          split3(rb[e], a, b, c);
-          s1[e] = a;
-          s2[e] = b;
-          s3[e] = c;
-          p1[e] = (unsigned short)(rp[e].x & 0xffffu);
-          p2[e] = (unsigned short)(rp[e].x >> 16);
-          p3[e] = (unsigned short)(rp[e].y & 0xffffu);
+          const int k = k0 + 8 * lg + e;  // wave-uniform
+          const int kc = min(k, nbe - 1);
+          const float b = bcol[kc * ld];
+          const uint2 p = Pc[(int64_t)bf[kc] * ldp];
+          rb[e] = k < nbe ? b : 0.f;
+          rp[e] = (k < nbe && colok) ? p : make_uint2(0u, 0u);
         }
-        *reinterpret_cast<us8*>(&Bt[buf][0][lc][8 * lg]) = s1;
-        *reinterpret_cast<us8*>(&Bt[buf][1][lc][8 * lg]) = s2;
-        *reinterpret_cast<us8*>(&Bt[buf][2][lc][8 * lg]) = s3;
-        *reinterpret_cast<us8*>(&Pt[buf][0][lc][8 * lg]) = p1;
-        *reinterpret_cast<us8*>(&Pt[buf][1][lc][8 * lg]) = p2;
-        *reinterpret_cast<us8*>(&Pt[buf][2][lc][8 * lg]) = p3;
-      }
-      __syncthreads();
-      if (s + 1 < nks) load(32 * (s + 1));
-      bf16x8 fa[2][3], fb[2][3];
+      };
+      double acc[BI][2][4];
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
+      for (int a = 0; a < BI; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
+      load(0);
+      for (int s = 0; s < nks; ++s) {
+        const int buf = s & 1;
+        {
+          us8 s1, s2, s3, p1, p2, p3;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            unsigned short a, b, c;
+            split3(rb[e], a, b, c);
+            s1[e] = a;
+            s2[e] = b;
+            s3[e] = c;
+            p1[e] = (unsigned short)(rp[e].x & 0xffffu);
+            p2[e] = (unsigned short)(rp[e].x >> 16);
+            p3[e] = (unsigned short)(rp[e].y & 0xffffu);
+          }
+          *reinterpret_cast<us8*>(&Bt[buf][0][lc][8 * lg]) = s1;
+          *reinterpret_cast<us8*>(&Bt[buf][1][lc][8 * lg]) = s2;
+          *reinterpret_cast<us8*>(&Bt[buf][2][lc][8 * lg]) = s3;
+          if (lc < NT) {
+            *reinterpret_cast<us8*>(&Pt[buf][0][lc][8 * lg]) = p1;
+            *reinterpret_cast<us8*>(&Pt[buf][1][lc][8 * lg]) = p2;
+            *reinterpret_cast<us8*>(&Pt[buf][2][lc][8 * lg]) = p3;
+          }
+        }
+        __syncthreads();
+        if (s + 1 < nks) load(32 * (s + 1));
+        bf16x8 fa[BI][3], fb[2][3];
 #pragma unroll
         for (int sl = 0; sl < 3; ++sl) {
-          fa[q][sl] = frag(&Pt[buf][sl][wm * 32 + q * 16 + (lane & 15)][kk]);
-          fb[q][sl] = frag(&Bt[buf][sl][wn * 32 + q * 16 + (lane & 15)][kk]);
+#pragma unroll
+          for (int q = 0; q < BI; ++q)
+            fa[q][sl] = frag(&Pt[buf][sl][wm * 16 * BI + q * 16 + (lane & 15)][kk]);
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+            fb[q][sl] = frag(&Bt[buf][sl][wn * 32 + q * 16 + (lane & 15)][kk]);
         }
-      v4f c[2][2];
+        v4f c[BI][2];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+          for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int pr = 0; pr < 6; ++pr) {
+        for (int pr = 0; pr < 6; ++pr) {
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+          for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+              c[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  fa[bi][sl_b(pr)], fb[bj][sl_a(pr)], c[bi][bj], 0, 0, 0);
+        }
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj)
-            c[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[bi][sl_b(pr)],
-                                                                 fb[bj][sl_a(pr)], c[bi][bj], 0,
-                                                                 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
       }
+      double x[BI][2][4];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+      for (int i = 0; i < BI; ++i)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
+          for (int r = 0; r < 4; ++r) x[i][j][r] = fac * acc[i][j][r];
+      fold_tile<float, BI>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      __syncthreads();
     }
-    double x[2][2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) x[i][j][r] = fac * acc[i][j][r];
-    fold_tile<float>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+    finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+    if (want_z) {
+      __syncthreads();
+      write_z<float>(t, m0, tid, zc, gga, Bm);
+    }
+    stamp_item(qa, it, t0);
     __syncthreads();
-  }
-  finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
-  if (want_z) {
-    __syncthreads();
-    write_z<float>(t, m0, tid, zc, gga, Bm);
   }
 }
 
-int launch_density_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                        const DevTask* tasks, const int32_t* bfmap, const double* P,
-                        int64_t ldp, double* scratch, const double* weights, int func, int gga,
-                        double fac, int want_z, double* red) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_density_f64, dim3((unsigned)c.npblk), dim3(256), 0, s,
-                     pblk + c.pblk_off, tasks, bfmap, P, ldp, scratch, weights, func, gga, fac,
-                     want_z, red);
-  XCQ_HIP(hipGetLastError());
-  return XCQ_SUCCESS;
-}
-
-int launch_density_mixed(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                         const DevTask* tasks, const int32_t* bfmap, const uint2* Ps,
-                         int64_t ldp, float* scratch, const double* weights, int func, int gga,
-                         double fac, int want_z, double* red) {
-  if (c.npblk == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_density_mx, dim3((unsigned)c.npblk), dim3(256), 0, s, pblk + c.pblk_off,
-                     tasks, bfmap, Ps, ldp, scratch, weights, func, gga, fac, want_z, red);
+int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, int func,
+                   int gga, double fac, int want_z) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  const QueueArgs qa = queue_args(v, r);
+  const dim3 grid(grid_of(r)), block(256);
+  if (precision == XCQ_PREC_FP64) {
+    double* scr = static_cast<double*>(v.scratch);
+    if (nt == 32)
+      hipLaunchKernelGGL(k_density_f64<32>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P,
+                         scr, v.w, func, gga, fac, want_z, v.red);
+    else
+      hipLaunchKernelGGL(k_density_f64<64>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P,
+                         scr, v.w, func, gga, fac, want_z, v.red);
+  } else {
+    float* scr = static_cast<float*>(v.scratch);
+    if (nt == 32)
+      hipLaunchKernelGGL(k_density_mx<32>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
+                         v.Psplit, scr, v.w, func, gga, fac, want_z, v.red);
+    else
+      hipLaunchKernelGGL(k_density_mx<64>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
+                         v.Psplit, scr, v.w, func, gga, fac, want_z, v.red);
+  }
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -33,8 +33,7 @@ void set_error(const std::string& msg);
 
 // ---- device data ------------------------------------------------------------
 constexpr int kMaxL = 4;          // highest angular momentum the kernels evaluate
-constexpr int kTile = 64;         // GEMM tile edge (points and basis functions)
-constexpr int kPtBlock = 64;      // points per collocation / functional block
+constexpr int kPtBlock = 64;      // points per collocation / density block (npts_pad unit)
 
 struct DevShell {
   double alpha[32];
@@ -44,35 +43,55 @@ struct DevShell {
 };
 
 // One task as laid out on the device.  Offsets into resident arrays are
-// absolute; offsets into the chunk scratch are relative to the chunk.
+// absolute; b_off is relative to the chunk scratch.
 struct DevTask {
   int32_t npts, nbe, nshells, iparent;
-  int32_t npts_pad;   // npts rounded up to kTile
-  int32_t ntile;      // ceil(nbe / kTile)
+  int32_t npts_pad;   // npts rounded up to kPtBlock
+  int32_t mol;        // molecule of the task
+  int32_t ldp;        // nbf of the molecule = leading dimension of its P and VXC
+  int32_t nt_d;       // function tile of the density kernel (32 or 64)
+  int32_t nt_s;       // function tile of the SYR2K kernel (32 or 64)
+  int32_t pad_;
   int64_t pt_off;     // points (SoA) / weights
   int64_t sh_off;     // shell list
-  int64_t bf_off;     // local -> global basis function map (nbe entries)
+  int64_t bf_off;     // local -> molecule basis function map (nbe entries)
   int64_t lsh_off;    // local shell -> first local basis function (nshells entries)
   int64_t b_off;      // chunk scratch: [5][nbe][npts_pad] (B, Bx, By, Bz, Z) in elements
-  int64_t q_off;      // chunk per-point arrays (npts_pad entries)
   int64_t red_off;    // global index of the task's first point block (EXC/NEL partials)
+  int64_t p_off;      // offset of the molecule's P (and P slices, VXC) in elements
 };
 
 struct WorkItem {  // (task, a, b): a/b are point-block or tile indices
   int32_t task, a, b, pad_;
 };
 
-struct Chunk {
-  int32_t task_begin, task_end;
-  int64_t scratch_elems;  // elements of T (FP64 or FP32) used by B-family arrays
-  int64_t nq;             // per-point array length
-  // work lists (offsets into the concatenated device lists)
-  int64_t pblk_off, npblk;    // (task, point block): collocation, density, Z
-  int64_t g2_off, ng2;        // SYR2K tiles (task, itile, jtile) with itile >= jtile
+// A launchable work list: items [off, off + n) of the context's work array.
+// nq > 0 runs it as a persistent launch of nq workgroups, workgroup q taking
+// items [qoff[q], qoff[q+1]) of the list in order (queue table at qoff_off
+// of the context's queue array, nq + 1 entries).
+struct WorkRange {
+  int64_t off = 0, n = 0;
+  int64_t qoff_off = 0;
+  int32_t nq = 0, pad_ = 0;
 };
 
-// View of a context's device buffers for the mixed-precision translation
-// unit (defined in context.hip).
+// Launch arguments common to every queued stage kernel.
+struct QueueArgs {
+  const WorkItem* items;  // first item of the list
+  const int32_t* qoff;    // queue table (nullptr: one workgroup per item)
+  int64_t* tstamp;        // per-item (start, end) wall clock, or nullptr
+};
+
+struct Chunk {
+  int32_t task_begin = 0, task_end = 0;
+  int64_t scratch_elems = 0;  // elements of T (FP64 or FP32) used by B-family arrays
+  WorkRange coll;             // all point blocks (collocation) = dens[0] followed by dens[1]
+  WorkRange dens[2];          // density blocks of tasks with 32- / 64-function tiles
+  WorkRange syr2k[2];         // SYR2K tiles (it >= jt) with 32- / 64-function tiles
+};
+
+// View of a context's device buffers for the translation units that run
+// pipeline stages (defined in context.hip).
 struct CtxView {
   hipStream_t stream;
   const DevTask* tasks;
@@ -80,39 +99,62 @@ struct CtxView {
   const int32_t *shell_list, *bfmap, *lsh;
   const double *px, *py, *pz, *w;
   const WorkItem* work;
-  double* P;
+  const int32_t* qoff;
+  int64_t* tstamp;  // nullptr unless task timing is on
+  const double* P;
+  const uint2* Psplit;
   double* vxc;
-  int32_t nbf;
   double* red;
   void* scratch;
-  size_t scratch_bytes;
-  void** Psplit;  // nbf x nbf packed (p1, p2, p3, 0) bfloat16 slices of P
 };
-CtxView view_of(struct ::xcq_context_s* c);
+
+inline QueueArgs queue_args(const CtxView& v, const WorkRange& r) {
+  return QueueArgs{v.work + r.off, r.nq ? v.qoff + r.qoff_off : nullptr,
+                   v.tstamp ? v.tstamp + 2 * r.off : nullptr};
+}
+inline unsigned grid_of(const WorkRange& r) { return (unsigned)(r.nq ? r.nq : r.n); }
+
+// Item loop of a queued stage kernel: one item per workgroup, or the
+// workgroup's queue.  The body must end with all threads done with LDS.
+#define XCQ_FOR_EACH_ITEM(qa, it_var)                                              \
+  for (int64_t it_var = (qa).qoff ? (qa).qoff[blockIdx.x] : (int64_t)blockIdx.x,  \
+               it_end_ = (qa).qoff ? (qa).qoff[blockIdx.x + 1] : it_var + 1;       \
+       it_var < it_end_; ++it_var)
+
+__device__ __forceinline__ void stamp_item(const QueueArgs& qa, int64_t it, int64_t t0) {
+  if (qa.tstamp && threadIdx.x == 0) {
+    qa.tstamp[2 * it] = t0;
+    qa.tstamp[2 * it + 1] = wall_clock64();
+  }
+}
 
 // ---- kernel launchers (defined in the .hip files) ---------------------------
-
 int upload_sph_table(hipStream_t s);
 int launch_ssf(hipStream_t s, int32_t natoms, const double* atom_xyz, const double* rab,
                const double* dist_nearest, int64_t npts, const double* px, const double* py,
                const double* pz, const int32_t* parent, double* w);
-int launch_collocation_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                            const DevTask* tasks, const DevShell* shells, const int32_t* shell_list,
-                            const int32_t* lsh_first, const double* px, const double* py,
-                            const double* pz, double* scratch, int deriv);
-int launch_density_fp64(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                        const DevTask* tasks, const int32_t* bfmap, const double* P,
-                        int64_t ldp, double* scratch, const double* weights, int func, int gga,
-                        double fac, int want_z, double* red);
-int launch_density_mixed(hipStream_t s, const Chunk& c, const WorkItem* pblk,
-                         const DevTask* tasks, const int32_t* bfmap, const uint2* Ps,
-                         int64_t ldp, float* scratch, const double* weights, int func, int gga,
-                         double fac, int want_z, double* red);
-int launch_syr2k_scatter_fp64(hipStream_t s, const Chunk& c, const WorkItem* g2,
-                              const DevTask* tasks, const int32_t* bfmap, const double* scratch,
-                              double* vxc, int64_t ldv);
-int launch_symmetrize(hipStream_t s, int32_t n, double* a, int64_t lda);
-int launch_reduce_sums(hipStream_t s, const double* red, int64_t nblocks, double* out2);
+int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,
+                            const DevTask* tasks, const DevShell* shells,
+                            const int32_t* shell_list, const int32_t* lsh_first, const double* px,
+                            const double* py, const double* pz, double* scratch, int deriv);
+int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
+                            const DevTask* tasks, const DevShell* shells,
+                            const int32_t* shell_list, const int32_t* lsh_first, const double* px,
+                            const double* py, const double* pz, float* scratch, int deriv);
+// Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
+// whose tasks use `nt`-function tiles.
+int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, int func,
+                   int gga, double fac, int want_z);
+// B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
+int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);
+int launch_split_P(hipStream_t s, int64_t n, const double* P, uint2* out);
+int launch_fill(hipStream_t s, double* p, int64_t n);
+// Symmetrise the VXC of nmol molecules: mol_n[m] = nbf, mol_off[m] = offset.
+int launch_symmetrize(hipStream_t s, int32_t nmol, int32_t max_n, const int32_t* mol_n,
+                      const int64_t* mol_off, double* vxc);
+// Per-molecule EXC/NEL: red pairs [blk_off[m], blk_off[m+1]) -> out2[2m], out2[2m+1].
+int launch_reduce_sums(hipStream_t s, int32_t nmol, const int64_t* blk_off, const double* red,
+                       double* out2);
 int launch_functional_points(hipStream_t s, int func, int64_t n, const double* rho,
                              const double* sigma, double* eps, double* vrho, double* vsigma);
 

@@ -2,6 +2,8 @@
 // functional with weight folding and EXC/NEL block sums, the deterministic
 // final reduction, and VXC symmetrisation.
 #include "internal.hpp"
+
+#include <algorithm>
 #include "functional.hpp"
 
 
@@ -77,12 +79,16 @@ int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* r
   return XCQ_SUCCESS;
 }
 
-// Sum of (e, n) pairs in a fixed order: deterministic EXC and NEL.
-__global__ __launch_bounds__(256) void k_reduce_sums(const double* __restrict__ red,
-                                                     int64_t nb, double* __restrict__ out) {
+// Per-molecule sums of the (e, n) block pairs in a fixed order:
+// deterministic EXC and NEL.  Workgroup m reduces blocks
+// [blk_off[m], blk_off[m+1]).
+__global__ __launch_bounds__(256) void k_reduce_sums(const int64_t* __restrict__ blk_off,
+                                                     const double* __restrict__ red,
+                                                     double* __restrict__ out) {
   __shared__ double se[256], sn[256];
+  const int m = blockIdx.x;
   double e = 0., n = 0.;
-  for (int64_t i = threadIdx.x; i < nb; i += 256) {
+  for (int64_t i = blk_off[m] + threadIdx.x; i < blk_off[m + 1]; i += 256) {
     e += red[2 * i];
     n += red[2 * i + 1];
   }
@@ -97,30 +103,50 @@ __global__ __launch_bounds__(256) void k_reduce_sums(const double* __restrict__ 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = se[0];
-    out[1] = sn[0];
+    out[2 * m] = se[0];
+    out[2 * m + 1] = sn[0];
   }
 }
 
-int launch_reduce_sums(hipStream_t s, const double* red, int64_t nb, double* out2) {
-  hipLaunchKernelGGL(k_reduce_sums, dim3(1), dim3(256), 0, s, red, nb, out2);
+int launch_reduce_sums(hipStream_t s, int32_t nmol, const int64_t* blk_off, const double* red,
+                       double* out2) {
+  if (nmol == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_reduce_sums, dim3((unsigned)nmol), dim3(256), 0, s, blk_off, red, out2);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
 
-// Upper triangle <- lower triangle through a 32x32 LDS tile
-// (symmetrize_vxc of incore_..._exc_vxc.hpp:376-385).
+// Zero fill (used instead of a memset node so the pipeline can be captured
+// into a graph).
+__global__ __launch_bounds__(256) void k_fill_zero(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    p[i] = 0.;
+}
+
+int launch_fill(hipStream_t s, double* p, int64_t n) {
+  if (n == 0) return XCQ_SUCCESS;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
 // Symmetrisation of the accumulated VXC: each off-diagonal pair (a, b) holds
 // contributions in either triangle, and both entries become their sum
 // (GauXC fills the upper triangle from the lower one,
 // host_integrator_exc_vxc.hpp:577-599; here the scatter may land in either).
-// Workgroup (bi, bj), bi >= bj, owns the 32x32 tile pair (bi, bj) / (bj, bi).
-__global__ __launch_bounds__(256) void k_symmetrize(int32_t n, double* __restrict__ a,
-                                                    int64_t lda) {
+// Workgroup (bi, bj, m), bi >= bj, owns the 32x32 tile pair (bi, bj) /
+// (bj, bi) of molecule m's nbf x nbf matrix at offset mol_off[m].
+__global__ __launch_bounds__(256) void k_symmetrize(const int32_t* __restrict__ mol_n,
+                                                    const int64_t* __restrict__ mol_off,
+                                                    double* __restrict__ vxc) {
   __shared__ double lo[32][33];
   __shared__ double up[32][33];
   const int bi = blockIdx.x, bj = blockIdx.y;
-  if (bi < bj) return;
+  const int n = mol_n[blockIdx.z];
+  if (bi < bj || bj * 32 >= n || bi * 32 >= n) return;
+  const int64_t lda = n;
+  double* a = vxc + mol_off[blockIdx.z];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
     const int r = bi * 32 + tx, c = bj * 32 + k;  // lower tile element (r, c)
@@ -138,10 +164,11 @@ __global__ __launch_bounds__(256) void k_symmetrize(int32_t n, double* __restric
   }
 }
 
-int launch_symmetrize(hipStream_t s, int32_t n, double* a, int64_t lda) {
-  const int nt = (n + 31) / 32;
-  if (nt == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt), dim3(256), 0, s, n, a, lda);
+int launch_symmetrize(hipStream_t s, int32_t nmol, int32_t max_n, const int32_t* mol_n,
+                      const int64_t* mol_off, double* vxc) {
+  const int nt = (max_n + 31) / 32;
+  if (nt == 0 || nmol == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, nmol), dim3(256), 0, s, mol_n, mol_off, vxc);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

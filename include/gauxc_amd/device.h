@@ -82,6 +82,48 @@ int xcq_upload_tasks(xcq_context ctx, int32_t ntasks, const xcq_task* tasks, int
 /* Density matrix, nbf x nbf column-major (host pointer). */
 int xcq_upload_density(xcq_context ctx, int32_t nbf, const double* P, int64_t ldp);
 
+/* ---- batched evaluation of independent molecules in one context (no GauXC
+ *      counterpart: GauXC runs one molecule per integrator; this batches the
+ *      task lists of many molecules into the same launches) ----
+ * Shells of nmol molecules, concatenated molecule by molecule; mol_nshells[m]
+ * shells belong to molecule m.  xcq_upload_basis is the nmol = 1 case. */
+int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshells,
+                         const xcq_shell* shells);
+/* Tasks of several molecules: task_mol[i] is the molecule of task i (NULL:
+ * every task belongs to molecule 0); shell lists hold global shell indices
+ * of that molecule's shells.  xcq_upload_tasks is task_mol = NULL. */
+int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* tasks,
+                             const int32_t* task_mol, int64_t npts_total, const double* points,
+                             const double* weights, int64_t nshell_entries,
+                             const int32_t* shell_lists);
+int xcq_upload_density_mol(xcq_context ctx, int32_t mol, int32_t nbf, const double* P,
+                           int64_t ldp);
+int xcq_download_results_mol(xcq_context ctx, int32_t mol, double* exc, double* nel,
+                             double* vxc, int64_t ldvxc);
+
+/* ---- scheduling (no GauXC counterpart beyond the task sort of
+ *      incore_replicated_xc_device_integrator_exc_vxc.hpp:253-257) ----
+ * Tile policy: tasks with nbe <= dens32_max_nbe run the density stage with
+ * 32-function tiles (else 64); likewise syr2k32_max_nbe for B^T Z. */
+int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32_max_nbe);
+/* nqueues = 0: one workgroup per work item (large-first upload order,
+ * XCD-interleaved).  nqueues > 0: persistent launches of nqueues workgroups
+ * per stage list, each draining its own queue; work items are packed
+ * longest-first into the queues by per-task predicted costs (dens_cost,
+ * syr2k_cost, ntasks entries each, split evenly over a task's items), or by
+ * the tile-flop heuristic when both are NULL. */
+int xcq_set_schedule(xcq_context ctx, int32_t nqueues, const double* dens_cost,
+                     const double* syr2k_cost);
+/* Per-item wall-clock stamps in the density and B^T Z kernels; after a run,
+ * xcq_task_times returns per-task sums in microseconds (ntasks entries
+ * each): the timings the cost model trains on. */
+int xcq_enable_task_timing(xcq_context ctx, int enable);
+int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us);
+/* Capture the whole evaluation into a HIP graph on first use of a
+ * (functional, precision, want_vxc) combination and replay it afterwards.
+ * Stage times then report only the total (stage 6). */
+int xcq_use_graph(xcq_context ctx, int enable);
+
 /* ---- SSF partition weights (GauXC: DeviceMolecularWeights::modify_weights,
  *      src/molecular_weights/device/device_molecular_weights.cxx:18, kernels
  *      cuda_ssf_1d.cu:24-131; host reference weights.cxx:117-237) ----

@@ -1,0 +1,138 @@
+"""Scheduling machinery of the device path: tile-width variants, persistent
+queued launches, graph capture/replay, batched molecules and per-task
+timing.  Each must leave the results unchanged (to FP64 summation-order
+noise) against the default one-workgroup-per-item launches, which the
+parity tests pin to the oracle and GauXC's fixtures."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _tasks(gx, n_water, grid, seed=0, cfgname="c1"):
+    mol, bs, P, cfg = gx.workloads.make_config(cfgname, seed=seed, n_water=n_water)
+    lb = gx.workloads.build_tasks(mol, bs, grid=grid)
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    return bs, P, lb.get_tasks()
+
+
+def _ctx(gx, bs, t, P):
+    ctx = gx.DeviceContext(0)
+    ctx.upload_basis(bs.xcq_shells())
+    ctx.upload_tasks(t)
+    ctx.upload_density(P)
+    return ctx
+
+
+def _run(ctx, prec, func="PBE"):
+    ctx.exc_vxc(func, prec, True)
+    return ctx.results()
+
+
+@pytest.fixture(scope="module")
+def small(gx, gpu):
+    # 4 waters, UltraFineGrid: nbe spans ~20..250, so both tile widths occur.
+    return _tasks(gx, 4, "UltraFineGrid")
+
+
+@pytest.mark.parametrize("prec,tol_e,tol_v", [("fp64", 1e-11, 1e-12), ("mixed", 1e-9, 1e-10)])
+def test_tile_policy_variants(gx, small, prec, tol_e, tol_v):
+    """All-64, all-32 and split tile policies give the same EXC/VXC."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    ctx.set_tile_policy(0, 0)  # every task on 64-wide tiles
+    e0, n0, v0 = _run(ctx, prec)
+    for pol in [(100000, 100000), (48, 96), (96, 48)]:
+        _flush(f"tile policy {pol} {prec}")
+        ctx.set_tile_policy(*pol)
+        e1, n1, v1 = _run(ctx, prec)
+        assert abs(e1 - e0) <= tol_e
+        assert abs(n1 - n0) <= tol_e
+        assert np.abs(v1 - v0).max() <= tol_v
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_queued_schedule_matches_direct(gx, small, prec):
+    """Persistent queues (heuristic and explicit costs) reproduce the
+    one-workgroup-per-item launch."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    e0, n0, v0 = _run(ctx, prec)
+    nt = len(t["npts"])
+    cost = t["npts"].astype(float) * t["nbe"].astype(float) ** 2
+    for nq, dc, sc in [(256, None, None), (1000, cost, 0.5 * cost), (7, cost[::-1].copy(), cost)]:
+        _flush(f"queues nq={nq} {prec}")
+        ctx.set_schedule(nq, dc, sc)
+        e1, n1, v1 = _run(ctx, prec)
+        assert abs(e1 - e0) <= 1e-11
+        assert np.abs(v1 - v0).max() <= 1e-12
+    ctx.set_schedule(0)
+    assert nt > 0
+
+
+def test_graph_replay_matches_eager(gx, small):
+    """Graph capture of the whole evaluation: several replays, with eager
+    work and synchronisations between them, each equal to the eager run."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    e0, n0, v0 = _run(ctx, "fp64")
+    em, nm, vm = _run(ctx, "mixed")
+    ctx.use_graph(True)
+    for rep in range(3):
+        _flush(f"graph replay {rep}")
+        e1, n1, v1 = _run(ctx, "fp64")
+        assert abs(e1 - e0) <= 1e-11
+        assert np.abs(v1 - v0).max() <= 1e-12
+        assert ctx.stage_times()[6] > 0
+        # eager-equivalent work in between: a mixed graph, then a host read
+        e2, n2, v2 = _run(ctx, "mixed")
+        assert abs(e2 - em) <= 1e-11
+        assert np.abs(v2 - vm).max() <= 1e-12
+        ctx.synchronize()
+    ctx.use_graph(False)
+    e3, n3, v3 = _run(ctx, "fp64")
+    assert abs(e3 - e0) <= 1e-11
+
+
+def test_batched_molecules_match_single(gx, gpu):
+    """Three molecules in one context: each molecule's EXC/NEL/VXC equals its
+    own single-molecule evaluation (FP64 and mixed)."""
+    systems = [_tasks(gx, nw, "FineGrid", seed=s) for nw, s in [(2, 3), (3, 4), (2, 5)]]
+    b = gx.BatchedDeviceContext(0)
+    b.upload_molecules([bs.xcq_shells() for bs, _, _ in systems])
+    b.upload_tasks([t for _, _, t in systems])
+    for m, (_, P, _) in enumerate(systems):
+        b.upload_density(m, P)
+    for prec in ["fp64", "mixed"]:
+        _flush(f"batched {prec}")
+        b.exc_vxc("PBE", prec, True)
+        for m, (bs, P, t) in enumerate(systems):
+            e, n, v = b.results(m)
+            ctx = _ctx(gx, bs, t, P)
+            e1, n1, v1 = _run(ctx, prec)
+            assert abs(e - e1) <= 1e-11, (m, e, e1)
+            assert abs(n - n1) <= 1e-11
+            assert np.abs(v - v1).max() <= 1e-12
+    rc = b.lib.xcq_download_results(b.h, None, None, None, 0)
+    assert rc != 0 and b"several molecules" in b.lib.xcq_last_error()
+
+
+def test_task_timing_records_every_task(gx, small):
+    """Per-item wall-clock stamps aggregate to a positive time per task."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    ctx.enable_task_timing(True)
+    _run(ctx, "mixed")
+    d, s = ctx.task_times()
+    assert d.shape == (len(t["npts"]),)
+    assert np.all(d > 0) and np.all(s > 0)
+    # bigger tasks take longer on average
+    w = t["npts"].astype(float) * t["nbe"].astype(float) ** 2
+    big = w > np.quantile(w, 0.9)
+    little = w < np.quantile(w, 0.1)
+    assert d[big].mean() > d[little].mean()
+    assert s[big].mean() > s[little].mean()
