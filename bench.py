@@ -7,10 +7,17 @@ Workload (one "step" = one full EXC+VXC evaluation over every task):
   P = 2 C C^T with C of width 120.  Inputs (tasks, weights, P, basis) are
   resident in HBM before the timed region; VXC stays in HBM.
 
+Both precisions run on the same resident inputs.  Before timing, the device
+autotuner picks the tile thresholds and queue count for each precision
+(untimed); the timed steps replay the captured graph of the evaluation.
+``value`` is the mixed-precision rate; the FP64 rate, the speed-up and the
+mixed-vs-FP64 |dEXC| / max|dVXC| are reported beside it.
+
 Prints one JSON line (see the driver contract) with a roofline entry for
-the dominant kernel, measured with HIP events on the kernel's stream, and a
-CPU baseline: the oracle (a restatement of GauXC's host integrator) timed on
-a bounded sample of the same tasks with all host threads.
+the dominant stage (its kernel launches timed with HIP events on the
+context's stream, in an eager step) and a CPU baseline: the oracle (a
+restatement of GauXC's host integrator) timed on a bounded sample of the
+same tasks with all host threads.
 """
 import argparse
 import json
@@ -26,10 +33,67 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symmetrize"]
+# Kernels of each stage (the names rocprofv3 reports).
+STAGE_KERNELS = {
+    ("collocation", "fp64"): ["k_collocation<true, double, double>"],
+    ("collocation", "mixed"): ["k_collocation<true, double, float>"],
+    ("density", "fp64"): ["k_density_f64<32>", "k_density_f64<64>"],
+    ("density", "mixed"): ["k_density_mx<32>", "k_density_mx<64>"],
+    ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
+    ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32>", "k_syr2k_mx<64>"],
+}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_c1_24water.json")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic(stage, prec, ts):
+    """Algorithmic (bytes, flops) of one evaluation's launches of a stage.
+
+    Per task t (npts, nbe; s = 8 B for FP64 planes, 4 B for mixed):
+      collocation  writes B, dB/dx, dB/dy, dB/dz: 4 s npts nbe
+                   (+ 32 npts for points and weights)
+      density      reads B, grad B, writes Z: 5 s npts nbe; gathers P_sub
+                   once: 8 nbe^2; flops 2 npts nbe^2 (X = B P_sub)
+      syr2k        reads B and Z: 2 s npts nbe; VXC lower triangle
+                   read+write: 8 nbe (nbe+1); flops 2 npts nbe^2
+                   (B^T Z + Z^T B, lower triangle)
+    """
+    npts = ts["npts"].astype(np.float64)
+    nbe = ts["nbe"].astype(np.float64)
+    s = 8.0 if prec == "fp64" else 4.0
+    pn = float(np.sum(npts * nbe))
+    pnn = float(np.sum(npts * nbe * nbe))
+    nn = float(np.sum(nbe * nbe))
+    if stage == "collocation":
+        return 4 * s * pn + 32 * float(npts.sum()), 0.0
+    if stage == "density":
+        return 5 * s * pn + 8 * nn, 2 * pnn
+    if stage == "syr2k_scatter":
+        return 2 * s * pn + 8 * float(np.sum(nbe * (nbe + 1))), 2 * pnn
+    return 0.0, 0.0
+
+
+def measured_traffic(stage, prec):
+    """HBM bytes of the stage's launches from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy
+    calibration factors it records), or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            pmc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    total = 0.0
+    found = False
+    for k in STAGE_KERNELS.get((stage, prec), []):
+        e = pmc.get("kernels", {}).get(k)
+        if e:
+            total += (e["fetch_kib"] * pmc["fetch_scale"] + e["write_kib"] * pmc["write_scale"]) * 1024
+            found = True
+    return total if found else None
 
 
 def main():
@@ -41,6 +105,11 @@ def main():
     ap.add_argument("--n-water", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tile-policy", default=None,
+                    help="D,S,Q: fixed density / SYR2K 32-wide thresholds and queue count "
+                         "for every precision (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count)")
@@ -83,21 +152,40 @@ def main():
     precisions = ["fp64", "mixed"] if args.precision == "auto" else [args.precision]
     results = {}
     for prec in precisions:
+        tuned = None
+        if args.tile_policy:
+            d_, s_, q_ = (int(x) for x in args.tile_policy.split(","))
+            ctx.set_tile_policy(d_, s_)
+            ctx.set_schedule(q_)
+            tuned = (d_, s_, q_, None)
+        elif not args.no_autotune:
+            tic = time.perf_counter()
+            tuned = ctx.autotune("PBE", prec)
+            log(f"{prec}: autotune -> dens32<=nbe {tuned[0]}, syr2k32<=nbe {tuned[1]}, "
+                f"queues {tuned[2]} ({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
+        # Eager steps: per-stage HIP-event times of the kernels.
+        ctx.use_graph(False)
+        stage = np.zeros(7)
+        for _ in range(3):
+            ctx.exc_vxc("PBE", prec, True)
+            stage += ctx.stage_times()
+        stage /= 3
+        # Timed steps: graph replays (or eager launches with --no-graph).
+        ctx.use_graph(not args.no_graph)
         for _ in range(args.warmup):
             ctx.exc_vxc("PBE", prec, True)
         ctx.synchronize()
-        stage = np.zeros(7)
         tic = time.perf_counter()
         for _ in range(args.steps):
             ctx.exc_vxc("PBE", prec, True)
-            ctx.synchronize()
-            stage += ctx.stage_times()
+        ctx.synchronize()
         toc = time.perf_counter()
         exc, nel, vxc = ctx.results(want_vxc=True)
         ms = (toc - tic) * 1e3 / args.steps
-        results[prec] = dict(ms=ms, stage=stage / args.steps, exc=exc, nel=nel, vxc=vxc)
+        results[prec] = dict(ms=ms, stage=stage, exc=exc, nel=nel, vxc=vxc, tuned=tuned)
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
-            f"stages(ms)={np.round(stage / args.steps, 3).tolist()}")
+            f"stages(ms)={np.round(stage, 3).tolist()}")
+    ctx.use_graph(False)
     head = "mixed" if "mixed" in results else precisions[0]
     r = results[head]
     accuracy = None
@@ -111,25 +199,28 @@ def main():
             "speedup_mixed_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],
         }
         log(f"accuracy/speed: {accuracy}")
-    counters = ctx.work_counters()
-    sum_pnn = float(counters[3])  # sum_t npts * nbe^2
-    gemm_flops = 2.0 * sum_pnn  # one GEMM: 2 npts nbe^2 (counted once, unsplit)
-    names = ["collocation", "xmat_density", "functional", "zmat", "syr2k_scatter", "symmetrize"]
+
+    # Roofline of the dominant stage: algorithmic bytes and flops of its
+    # launches over their event-timed duration; bound by arithmetic intensity
+    # against the ridge point of the unit it computes on.
     k_dom = int(np.argmax(r["stage"][:6]))
-    dom_ms = r["stage"][k_dom]
-    if names[k_dom] in ("xmat_density", "syr2k_scatter"):
-        achieved = gemm_flops / (dom_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved,
-                    peak=PEAK_FP64_TFLOPS if head == "fp64" else PEAK_BF16_TFLOPS,
-                    unit="TFLOP/s", traffic=None, kernel=names[k_dom])
+    name = STAGES[k_dom]
+    dom_ms = float(r["stage"][k_dom])
+    abytes, aflops = algorithmic(name, head, ts)
+    peak_mfma = PEAK_FP64_TFLOPS if head == "fp64" else PEAK_BF16_TFLOPS
+    ridge = peak_mfma * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if aflops > 0 and aflops / abytes > ridge:
+        roof = dict(bound="mfma", achieved=aflops / (dom_ms * 1e-3) / 1e12, peak=peak_mfma,
+                    unit="TFLOP/s")
     else:
-        # collocation writes 4 planes of npts*nbe values (8 B each in FP64)
-        sum_pn = float(np.sum(ts["npts"].astype(np.int64) * ts["nbe"].astype(np.int64)))
-        bytes_ = sum_pn * 4 * 8
-        achieved = bytes_ / (dom_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
-                    traffic=None, kernel=names[k_dom])
+        roof = dict(bound="hbm", achieved=abytes / (dom_ms * 1e-3) / 1e9, peak=PEAK_HBM_GBS,
+                    unit="GB/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = measured_traffic(name, head)
+    roof["kernel"] = " + ".join(STAGE_KERNELS.get((name, head), [name]))
+    roof["algorithmic_bytes"] = abytes
+    roof["algorithmic_flops"] = aflops
+    roof["stage_ms"] = dom_ms
 
     cpu = None
     if not args.no_cpu:
@@ -139,15 +230,17 @@ def main():
         "metric": "PBE EXC+VXC grid pts/s, mixed vs FP64 on 1 GPU, at |dEXC|<=1e-6 Eh",
         "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "bf16+f32+f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "bf16+f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.n_water}-water cluster, PBE, 99x590 grid, "
-                               f"{cfg['nbf']} bf, {npts_total} points, precision {head}",
+                               f"{cfg['nbf']} bf, {npts_total} points",
                    "precision": head, "nbf": cfg["nbf"], "points": npts_total,
-                   "tasks": int(len(ts["npts"]))},
+                   "tasks": int(len(ts["npts"])), "graph": not args.no_graph,
+                   "autotuned": {p: results[p]["tuned"] for p in results}},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "stage_ms": dict(zip(names, [float(x) for x in r["stage"][:6]])),
+        "stage_ms": dict(zip(STAGES, [float(x) for x in r["stage"][:6]])),
         "exc": r["exc"],
         "mixed_vs_fp64": accuracy,
     }
@@ -198,7 +291,6 @@ def cpu_baseline(bs, P, ts, cfg, budget_s):
     idx, pts, elapsed = run(2 * nthreads)
     if elapsed < budget_s / 3 and len(idx) < ntask:
         idx, pts, elapsed = run(int(len(idx) * budget_s / max(elapsed, 1e-3)))
-    stride = ntask / max(1, len(idx))
     log(f"cpu baseline: {len(idx)} tasks, {pts} points in {elapsed:.2f}s on {nthreads} threads")
     return {"value": pts / elapsed, "unit": "grid points/s", "cores": nthreads, "kind": "port",
             "sample": f"{len(idx)} of {ntask} tasks (evenly spaced, {pts} points) of the "

@@ -6,7 +6,7 @@ Native libraries (built in-tree by ``python -m gauxc_amd.build``):
 """
 from ._native import GauXCError, device_count  # noqa: F401
 from .api import (  # noqa: F401
-    BasisSet, DeviceContext, DeviceRuntimeEnvironment, Functional, LoadBalancer,
+    BasisSet, BatchedDeviceContext, DeviceContext, DeviceRuntimeEnvironment, Functional, LoadBalancer,
     LoadBalancerFactory, MolecularWeights, MolecularWeightsFactory, MolGrid, MolGridFactory,
     Molecule, RuntimeEnvironment, XCIntegrator, XCIntegratorFactory, eval_functional_device,
     ssf_weights_device,

@@ -89,7 +89,7 @@ DEVICE_SYMBOLS = [
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
     "xcq_upload_molecules", "xcq_upload_tasks_batched", "xcq_upload_density_mol",
     "xcq_download_results_mol", "xcq_set_tile_policy", "xcq_set_schedule",
-    "xcq_enable_task_timing", "xcq_task_times", "xcq_use_graph",
+    "xcq_enable_task_timing", "xcq_task_times", "xcq_use_graph", "xcq_autotune",
 ]
 
 _host = None
@@ -143,6 +143,8 @@ def device_lib():
             "xcq_enable_task_timing": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_task_times": (ctypes.c_int, [H, c_vp, c_vp]),
             "xcq_use_graph": (ctypes.c_int, [H, ctypes.c_int]),
+            "xcq_autotune": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, c_vp, c_vp, c_vp,
+                                            c_vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)

@@ -439,6 +439,17 @@ class DeviceContext:
     def use_graph(self, on=True):
         N.xcq_check(self.lib.xcq_use_graph(self.h, int(bool(on))), "xcq_use_graph")
 
+    def autotune(self, func="PBE", precision="mixed"):
+        """Device autotuner (xcq_autotune): returns the chosen
+        (dens32_max_nbe, syr2k32_max_nbe, nqueues, best total ms)."""
+        d, s, q = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_double()
+        N.xcq_check(self.lib.xcq_autotune(self.h, N.XCQ_FUNC[func], N.XCQ_PREC[precision],
+                                          ctypes.addressof(d), ctypes.addressof(s),
+                                          ctypes.addressof(q), ctypes.addressof(ms)),
+                    "xcq_autotune")
+        return d.value, s.value, q.value, ms.value
+
 
 class BatchedDeviceContext(DeviceContext):
     """Several independent molecules evaluated in the same launches.

@@ -860,6 +860,87 @@ int xcq_use_graph(xcq_context ctx, int enable) {
   return XCQ_SUCCESS;
 }
 
+int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
+                 int32_t* syr2k32_max_nbe, int32_t* nqueues, double* best_ms) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  const bool graph = ctx->use_graph;
+  ctx->use_graph = false;  // stage timing needs the eager path
+  ctx->drop_graph();
+  // Median of three timed runs of (density, B^T Z) stage times.
+  auto measure = [&](double* dens_ms, double* syr_ms, double* tot_ms) -> int {
+    std::vector<double> d, sy, tt;
+    for (int rep = 0; rep < 4; ++rep) {
+      int rc = run_pipeline(ctx, functional, precision, 1);
+      if (rc) return rc;
+      double st[kStages + 1];
+      xcq_stage_times(ctx, st, kStages + 1);
+      if (rep == 0) continue;  // warm-up
+      d.push_back(st[1]);
+      sy.push_back(st[4]);
+      tt.push_back(st[kStages]);
+    }
+    auto med = [](std::vector<double>& v) {
+      std::sort(v.begin(), v.end());
+      return v[v.size() / 2];
+    };
+    *dens_ms = med(d);
+    *syr_ms = med(sy);
+    *tot_ms = med(tt);
+    return XCQ_SUCCESS;
+  };
+  const int32_t cand[] = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30};
+  int32_t best_d = ctx->dens32_max_nbe, best_s = ctx->syr2k32_max_nbe;
+  double bd = 1e300, bs = 1e300, tot = 0.;
+  int rc;
+  // The two stages read disjoint thresholds: sweep both at once and keep the
+  // best of each.
+  for (int32_t c : cand) {
+    if ((rc = xcq_set_tile_policy(ctx, c, c))) return rc;
+    double dm, sm;
+    if ((rc = measure(&dm, &sm, &tot))) return rc;
+    if (dm < bd) {
+      bd = dm;
+      best_d = c;
+    }
+    if (sm < bs) {
+      bs = sm;
+      best_s = c;
+    }
+  }
+  if ((rc = xcq_set_tile_policy(ctx, best_d, best_s))) return rc;
+  // Persistent queues (flop-heuristic packing) against one workgroup per item.
+  int dev_cus = 0;
+  if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    dev_cus = 256;
+  }
+  const std::vector<double> keep_d = ctx->cost_d, keep_s = ctx->cost_s;
+  int32_t best_q = 0;
+  double bt = 1e300;
+  for (int32_t q : {0, 2 * dev_cus, 4 * dev_cus}) {
+    if ((rc = xcq_set_schedule(ctx, q, keep_d.empty() ? nullptr : keep_d.data(),
+                               keep_s.empty() ? nullptr : keep_s.data())))
+      return rc;
+    double dm, sm, tm;
+    if ((rc = measure(&dm, &sm, &tm))) return rc;
+    if (tm < bt) {
+      bt = tm;
+      best_q = q;
+    }
+  }
+  if ((rc = xcq_set_schedule(ctx, best_q, keep_d.empty() ? nullptr : keep_d.data(),
+                             keep_s.empty() ? nullptr : keep_s.data())))
+    return rc;
+  ctx->use_graph = graph;
+  if (dens32_max_nbe) *dens32_max_nbe = best_d;
+  if (syr2k32_max_nbe) *syr2k32_max_nbe = best_s;
+  if (nqueues) *nqueues = best_q;
+  if (best_ms) *best_ms = bt;
+  return XCQ_SUCCESS;
+}
+
 int xcq_upload_density_mol(xcq_context ctx, int32_t mol, int32_t nbf, const double* P,
                            int64_t ldp) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");

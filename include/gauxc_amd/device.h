@@ -123,6 +123,13 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us);
  * (functional, precision, want_vxc) combination and replay it afterwards.
  * Stage times then report only the total (stage 6). */
 int xcq_use_graph(xcq_context ctx, int enable);
+/* Autotuner: times the uploaded workload on the device and keeps the
+ * fastest settings.  Sweeps the 32/64 tile thresholds of the density and
+ * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}), then
+ * the persistent-queue count {0, 2, 4 per CU}; leaves the context on the
+ * best and reports it (any output pointer may be NULL). */
+int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
+                 int32_t* syr2k32_max_nbe, int32_t* nqueues, double* best_ms);
 
 /* ---- SSF partition weights (GauXC: DeviceMolecularWeights::modify_weights,
  *      src/molecular_weights/device/device_molecular_weights.cxx:18, kernels

@@ -136,3 +136,18 @@ def test_task_timing_records_every_task(gx, small):
     little = w < np.quantile(w, 0.1)
     assert d[big].mean() > d[little].mean()
     assert s[big].mean() > s[little].mean()
+
+
+def test_autotune_keeps_results(gx, small):
+    """The autotuner only picks tile thresholds and a queue count: the
+    evaluation after tuning matches the one before."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    e0, n0, v0 = _run(ctx, "mixed")
+    d, s, q, ms = ctx.autotune("PBE", "mixed")
+    _flush(f"autotune -> {d} {s} {q} {ms:.3f} ms")
+    cand = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30}
+    assert d in cand and s in cand and q >= 0 and ms > 0
+    e1, n1, v1 = _run(ctx, "mixed")
+    assert abs(e1 - e0) <= 1e-9
+    assert np.abs(v1 - v0).max() <= 1e-10
