@@ -90,6 +90,8 @@ DEVICE_SYMBOLS = [
     "xcq_upload_molecules", "xcq_upload_tasks_batched", "xcq_upload_density_mol",
     "xcq_download_results_mol", "xcq_set_tile_policy", "xcq_set_schedule",
     "xcq_enable_task_timing", "xcq_task_times", "xcq_use_graph", "xcq_autotune",
+    "xcq_mlp_create", "xcq_mlp_destroy", "xcq_mlp_num_params", "xcq_mlp_get_params",
+    "xcq_mlp_set_params", "xcq_mlp_forward", "xcq_mlp_gradients", "xcq_mlp_train",
 ]
 
 _host = None
@@ -145,6 +147,16 @@ def device_lib():
             "xcq_use_graph": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_autotune": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, c_vp, c_vp, c_vp,
                                             c_vp]),
+            "xcq_mlp_create": (ctypes.c_int, [ctypes.c_int, c_i32, c_i32, c_i32, c_i32,
+                                              ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+            "xcq_mlp_destroy": (ctypes.c_int, [H]),
+            "xcq_mlp_num_params": (ctypes.c_int, [H, c_vp]),
+            "xcq_mlp_get_params": (ctypes.c_int, [H, c_vp]),
+            "xcq_mlp_set_params": (ctypes.c_int, [H, c_vp]),
+            "xcq_mlp_forward": (ctypes.c_int, [H, c_i64, c_vp, c_vp]),
+            "xcq_mlp_gradients": (ctypes.c_int, [H, c_i64, c_vp, c_vp, c_vp, c_vp]),
+            "xcq_mlp_train": (ctypes.c_int, [H, c_i64, c_vp, c_vp, c_i32, c_i32, ctypes.c_float,
+                                             ctypes.c_uint64, c_vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)

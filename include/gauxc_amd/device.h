@@ -177,6 +177,31 @@ int xcq_stage_times(xcq_context ctx, double* ms, int n);
  * npts, out[3] = sum over tasks of npts*nbe*nbe. */
 int xcq_work_counters(xcq_context ctx, int64_t* out, int n);
 
+/* ---- cost-model network of the task scheduler (no GauXC counterpart:
+ *      GauXC orders tasks by the flop heuristic XCTask::cost_exc_vxc,
+ *      include/gauxc/xc_task.hpp:112-114) ----
+ * FP32 MLP  nin -> [Linear -> ReLU] x nlayers (width nhidden) -> Linear ->
+ * nout, trained on the device by this library's own forward, backward and
+ * Adam kernels.  Parameters are one vector W1, b1, W2, b2, ... with W_l
+ * stored [out][in] row-major.  Rows of x / t are row-major [n][nin] /
+ * [n][nout] host arrays.  Loss: mean squared error. */
+typedef struct xcq_mlp_s* xcq_mlp;
+int xcq_mlp_create(int device, int32_t nin, int32_t nhidden, int32_t nlayers, int32_t nout,
+                   uint64_t seed, xcq_mlp* out);
+int xcq_mlp_destroy(xcq_mlp m);
+int xcq_mlp_num_params(xcq_mlp m, int64_t* n);
+int xcq_mlp_get_params(xcq_mlp m, float* host);
+/* Setting parameters also resets the optimiser state. */
+int xcq_mlp_set_params(xcq_mlp m, const float* host);
+int xcq_mlp_forward(xcq_mlp m, int64_t n, const float* x, float* y);
+/* Loss and its gradient with respect to the parameter vector (no update). */
+int xcq_mlp_gradients(xcq_mlp m, int64_t n, const float* x, const float* t, float* loss,
+                      float* grad);
+/* Adam (lr, beta 0.9 / 0.999) over shuffled minibatches of `batch` rows for
+ * `epochs` passes; final_loss is the loss of the last minibatch. */
+int xcq_mlp_train(xcq_mlp m, int64_t n, const float* x, const float* t, int32_t epochs,
+                  int32_t batch, float lr, uint64_t seed, float* final_loss);
+
 #ifdef __cplusplus
 }
 #endif

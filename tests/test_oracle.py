@@ -191,3 +191,32 @@ def test_oracle_functional_limits(orc):
     ecv = e_v[0] + 0.75 * (3 / np.pi) ** (1 / 3) * rs1 ** (1 / 3)
     assert -0.0800 < ecv < -0.0780
     assert np.all(v_lda < 0) and np.all(v_pbe < 0)
+
+
+def test_oracle_mlp_gradient_matches_finite_differences():
+    """The cost-model network's CPU oracle: analytic gradient against
+    central differences of its own loss (pins forward and backward)."""
+    import oracle.mlp as M
+    rng = np.random.default_rng(7)
+    widths, shapes, n = M.layout(6, 24, 3, 2)
+    p = rng.normal(size=n) * 0.4
+    x = rng.normal(size=(32, 6))
+    t = rng.normal(size=(32, 2))
+    loss, g = M.loss_and_grad(p, shapes, x, t)
+    acts, _ = M.forward(p, shapes, x)
+    assert np.isclose(loss, np.mean((acts[-1] - t) ** 2))
+    h = 1e-6
+    for k in rng.choice(n, 60, replace=False):
+        e = np.zeros(n)
+        e[k] = h
+        fd = (M.loss_and_grad(p + e, shapes, x, t)[0] - M.loss_and_grad(p - e, shapes, x, t)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-6 * abs(g[k]), (k, fd, g[k])
+    # a planted mistake (bias gradient of the last layer doubled) is caught
+    off, o, i = shapes[-1]
+    g_bad = g.copy()
+    g_bad[off + o * i:off + o * i + o] *= 2
+    k = off + o * i
+    e = np.zeros(n)
+    e[k] = h
+    fd = (M.loss_and_grad(p + e, shapes, x, t)[0] - M.loss_and_grad(p - e, shapes, x, t)[0]) / (2 * h)
+    assert abs(fd - g_bad[k]) > 1e-6
