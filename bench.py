@@ -102,7 +102,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
-    ap.add_argument("--n-water", type=int, default=24)
+    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
+                    help="c1: 24-water cluster (the metric's workload, default); "
+                         "c4: 150-water cluster in a 17 A box (7800 bf)")
+    ap.add_argument("--n-water", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -134,7 +137,7 @@ def main():
 
     assert gx.device_count() > 0, "bench.py needs a HIP device"
     t0 = time.time()
-    mol, bs, P, cfg = workloads.make_config("c1", seed=0, n_water=args.n_water)
+    mol, bs, P, cfg = workloads.make_config(args.config, seed=0, n_water=args.n_water)
     lb = workloads.build_tasks(mol, bs, grid=cfg["grid"])
     gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
     t = lb.get_tasks()
@@ -233,7 +236,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "bf16+f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.n_water}-water cluster, PBE, 99x590 grid, "
+        "config": {"workload": f"{cfg['n_water']}-water cluster, PBE, 99x590 grid, "
                                f"{cfg['nbf']} bf, {npts_total} points",
                    "precision": head, "nbf": cfg["nbf"], "points": npts_total,
                    "tasks": int(len(ts["npts"])), "graph": not args.no_graph,

@@ -64,7 +64,11 @@ struct DeviceXCIntegratorImpl::State {
   }
 
   void ensure_context() {
-    if (!ctx) check(xcq_context_create(device, 0, fill_fraction, &ctx), "xcq_context_create");
+    if (ctx) return;
+    check(xcq_context_create(device, 0, fill_fraction, &ctx), "xcq_context_create");
+    // SCF calls repeat the same evaluation with a new P: capture it once as a
+    // graph and replay it.
+    check(xcq_use_graph(ctx, 1), "xcq_use_graph");
   }
 
   void ensure_static_data() {
