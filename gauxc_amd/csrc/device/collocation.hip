@@ -69,12 +69,21 @@ int upload_sph_table(hipStream_t s) {
   return XCQ_SUCCESS;
 }
 
+// bfloat16 (round to nearest even) of the part of v that its FP32 rounding
+// drops: with it the stored value plane carries ~32 significant bits.
+__device__ __forceinline__ unsigned short resid_bf16(double v) {
+  const float hi = (float)v;
+  return __builtin_bit_cast(unsigned short, (__bf16)(float)(v - (double)hi));
+}
+
 // Values (and gradients) of one shell at one point, written to the scratch.
 // T is the storage type (double or float); arithmetic is in double for the
-// FP64 kernel and float for the FP32 kernel (type C).
+// FP64 kernel and float for the FP32 kernel (type C).  With FP32 storage the
+// value plane also gets its bfloat16 residual (lo, same indexing as out).
 template <int L, bool GRAD, typename C, typename T>
 __device__ __forceinline__ void shell_eval(C x, C y, C z, C R, C dR, bool pure, bool valid,
-                                           T* out, int64_t ld, int64_t plane) {
+                                           T* out, unsigned short* lo, int64_t ld,
+                                           int64_t plane) {
   constexpr int NC = (L + 1) * (L + 2) / 2;
   C xp[L + 1], yp[L + 1], zp[L + 1];
   xp[0] = yp[0] = zp[0] = C(1);
@@ -120,6 +129,7 @@ __device__ __forceinline__ void shell_eval(C x, C y, C z, C R, C dR, bool pure, 
         }
       }
       out[m * ld] = valid ? T(v) : T(0);
+      if constexpr (sizeof(T) == 4) lo[m * ld] = valid ? resid_bf16(v) : (unsigned short)0;
       if (GRAD) {
         out[plane + m * ld] = valid ? T(gx) : T(0);
         out[2 * plane + m * ld] = valid ? T(gy) : T(0);
@@ -131,6 +141,7 @@ __device__ __forceinline__ void shell_eval(C x, C y, C z, C R, C dR, bool pure, 
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       out[k * ld] = valid ? T(cv[k]) : T(0);
+      if constexpr (sizeof(T) == 4) lo[k * ld] = valid ? resid_bf16(cv[k]) : (unsigned short)0;
       if (GRAD) {
         out[plane + k * ld] = valid ? T(cx[k]) : T(0);
         out[2 * plane + k * ld] = valid ? T(cy[k]) : T(0);
@@ -165,6 +176,9 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
   T* base = scratch + t.b_off + p;
+  // FP32 storage: bfloat16 residual plane of the values after the five
+  // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).
+  unsigned short* lo_base = reinterpret_cast<unsigned short*>(scratch + t.b_off + 5 * plane) + p;
   for (int s = wave; s < t.nshells; s += 4) {
     const int ish = shell_list[t.sh_off + s];
     const int bf0 = lsh_first[t.lsh_off + s];
@@ -185,12 +199,13 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
       dR += C(-2) * C(sh->alpha[k]) * e;
     }
     T* out = base + (int64_t)bf0 * ld;
+    unsigned short* lo = lo_base + (int64_t)bf0 * ld;
     switch (l) {
-      case 0: shell_eval<0, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, ld, plane); break;
-      case 1: shell_eval<1, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, ld, plane); break;
-      case 2: shell_eval<2, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, ld, plane); break;
-      case 3: shell_eval<3, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, ld, plane); break;
-      default: shell_eval<4, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, ld, plane); break;
+      case 0: shell_eval<0, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
+      case 1: shell_eval<1, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
+      case 2: shell_eval<2, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
+      case 3: shell_eval<3, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
+      default: shell_eval<4, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
     }
   }
 }

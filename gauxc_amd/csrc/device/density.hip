@@ -64,14 +64,31 @@ __device__ __forceinline__ bf16x8 frag(const unsigned short* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(p));
 }
 
-__device__ __forceinline__ constexpr int sl_a(int p) {
-  return p == 2 ? 1 : p == 4 ? 1 : p == 5 ? 2 : 0;
+// Correction products (P slice, B slice) after the leading p0 b0:
+// (0,1) (1,0) (0,2) (1,1) (2,0) (0,3); b3 is the stored bfloat16 residual of
+// the value plane.
+__device__ __forceinline__ constexpr int corr_p(int k) {
+  return k == 1 ? 1 : k == 3 ? 1 : k == 4 ? 2 : 0;
 }
-__device__ __forceinline__ constexpr int sl_b(int p) {
-  return p == 1 ? 1 : p == 3 ? 2 : p == 4 ? 1 : 0;
+__device__ __forceinline__ constexpr int corr_b(int k) {
+  return k == 0 ? 1 : k == 2 ? 2 : k == 3 ? 1 : k == 5 ? 3 : 0;
 }
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ double bf16_val(unsigned short u) {
+  return (double)__builtin_bit_cast(float, (unsigned)u << 16);
+}
+
+// Value plane of FP32 storage: FP32 value + bfloat16 residual (plane 5 of
+// the task's scratch, 2 bytes per element).  nullptr for FP64 storage.
+template <typename T>
+__device__ __forceinline__ const unsigned short* lo_plane(const T* Bm, int plane) {
+  if constexpr (sizeof(T) == 4)
+    return reinterpret_cast<const unsigned short*>(Bm + 5 * (int64_t)plane);
+  else
+    return nullptr;
+}
 
 // Folds one X^T tile (rows = functions, columns = points) into the
 // per-point partials in LDS.  frow(r) is the tile row (function) of
@@ -86,34 +103,38 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
     const int pl = wn * 32 + bj * 16 + (lane & 15);
     const int m = m0 + pl;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
-    T v0[BI][4], v1[BI][4], v2[BI][4], v3[BI][4];
+    const unsigned short* lo = lo_plane(Bm, plane);
 #pragma unroll
-    for (int bi = 0; bi < BI; ++bi)
+    for (int bi = 0; bi < BI; ++bi) {
+      T v0[4], v1[4], v2[4], v3[4];
+      unsigned short l0[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = n0 + wm * 16 * BI + bi * 16 + frow(r);
         const int o = min(j, nbe - 1) * ld + m;
-        v0[bi][r] = Bm[o];
+        v0[r] = Bm[o];
+        if constexpr (sizeof(T) == 4) l0[r] = lo[o];
         if (gga) {
-          v1[bi][r] = Bm[plane + o];
-          v2[bi][r] = Bm[2 * plane + o];
-          v3[bi][r] = Bm[3 * plane + o];
+          v1[r] = Bm[plane + o];
+          v2[r] = Bm[2 * plane + o];
+          v3[r] = Bm[3 * plane + o];
         }
       }
-#pragma unroll
-    for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         // Rows past nbe carry X = 0 (their P_sub column was zeroed), so the
         // clamped B values they read contribute nothing.
         const double xv = x[bi][bj][r];
-        s0 += (double)v0[bi][r] * xv;
+        double b = (double)v0[r];
+        if constexpr (sizeof(T) == 4) b += bf16_val(l0[r]);
+        s0 += b * xv;
         if (gga) {
-          s1 += (double)v1[bi][r] * xv;
-          s2 += (double)v2[bi][r] * xv;
-          s3 += (double)v3[bi][r] * xv;
+          s1 += (double)v1[r] * xv;
+          s2 += (double)v2[r] * xv;
+          s3 += (double)v3[r] * xv;
         }
       }
+    }
     // Sum over the four 16-lane groups (different functions, same point).
 #pragma unroll
     for (int off = 16; off < 64; off <<= 1) {
@@ -189,10 +210,16 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
   const double a = zc[0][lane], cx = zc[1][lane], cy = zc[2][lane], cz = zc[3][lane];
   T* B = Bm + m0 + lane;
   T* Z = B + 4 * plane;
+  const unsigned short* lo = lo_plane(Bm, plane);
+  if constexpr (sizeof(T) == 4) lo += m0 + lane;
   int j = wave;
   for (; j + 4 < nbe; j += 8) {
     const int o0 = j * ld, o1 = (j + 4) * ld;
-    const double b0 = B[o0], b1 = B[o1];
+    double b0 = B[o0], b1 = B[o1];
+    if constexpr (sizeof(T) == 4) {
+      b0 += bf16_val(lo[o0]);
+      b1 += bf16_val(lo[o1]);
+    }
     double z0 = a * b0, z1 = a * b1;
     if (gga) {
       const double x0 = B[plane + o0], y0 = B[2 * plane + o0], q0 = B[3 * plane + o0];
@@ -205,7 +232,9 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
   }
   if (j < nbe) {
     const int o = j * ld;
-    double z = a * (double)B[o];
+    double bv = B[o];
+    if constexpr (sizeof(T) == 4) bv += bf16_val(lo[o]);
+    double z = a * bv;
     if (gga) z += cx * (double)B[plane + o] + cy * (double)B[2 * plane + o] +
                   cz * (double)B[3 * plane + o];
     Z[o] = (T)z;
@@ -221,7 +250,7 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
 // (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of column c of the B^T
 // stage (point c) and, for c < NT, of the P_sub stage (function n0 + c).
 template <int NT>
-__global__ __launch_bounds__(256) void k_density_f64(
+__global__ __launch_bounds__(256, 2) void k_density_f64(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, int gga, double fac, int want_z,
@@ -322,20 +351,22 @@ __global__ __launch_bounds__(256) void k_density_f64(
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision (v_mfma_f32_16x16x32_bf16, three-slice operands, FP32
-// inside one K block, FP64 across; accumulator register r of lane l holds
-// row 4(l/16) + r).  Loader roles: thread (c = tid & 63, g = tid >> 6)
+// Mixed precision (v_mfma_f32_16x16x32_bf16; P in three bfloat16 slices,
+// B in four: three exact slices of its FP32 part and the stored residual;
+// leading and correction products in separate FP32 accumulators inside one
+// K block, FP64 across; accumulator register r of lane l holds row
+// 4(l/16) + r).  Loader roles: thread (c = tid & 63, g = tid >> 6)
 // loads K rows 8g..8g+7 of column c of the B^T stage and, for c < NT, of
 // the P_sub stage.
 template <int NT>
-__global__ __launch_bounds__(256) void k_density_mx(
+__global__ __launch_bounds__(256, 2) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const uint2* __restrict__ Psall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, int gga, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
   __shared__ __attribute__((aligned(16))) unsigned short Pt[2][3][NT][kRowMx];  // [s][fn][k]
-  __shared__ __attribute__((aligned(16))) unsigned short Bt[2][3][64][kRowMx];  // [s][pt][k]
+  __shared__ __attribute__((aligned(16))) unsigned short Bt[2][4][64][kRowMx];  // [s][pt][k]
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
@@ -359,6 +390,7 @@ __global__ __launch_bounds__(256) void k_density_mx(
     for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
     __syncthreads();
     const float* bcol = Bm + m0 + lc;
+    const unsigned short* lcol = lo_plane(Bm, plane) + m0 + lc;
     const int nks = (nbe + 31) / 32;
 
     for (int n0 = 0; n0 < nbe; n0 += NT) {
@@ -368,6 +400,7 @@ __global__ __launch_bounds__(256) void k_density_mx(
       const bool colok = lc < NT && ncol < nbe;
       const uint2* Pc = Ps + bf[min(ncol, nbe - 1)];
       float rb[8];
+      unsigned short rl[8];
       uint2 rp[8];
       auto load = [&](int k0) {
 #pragma unroll
@@ -375,8 +408,10 @@ __global__ __launch_bounds__(256) void k_density_mx(
           const int k = k0 + 8 * lg + e;  // wave-uniform
           const int kc = min(k, nbe - 1);
           const float b = bcol[kc * ld];
+          const unsigned short l = lcol[kc * ld];
           const uint2 p = Pc[(int64_t)bf[kc] * ldp];
           rb[e] = k < nbe ? b : 0.f;
+          rl[e] = k < nbe ? l : (unsigned short)0;
           rp[e] = (k < nbe && colok) ? p : make_uint2(0u, 0u);
         }
       };
@@ -403,9 +438,13 @@ __global__ __launch_bounds__(256) void k_density_mx(
             p2[e] = (unsigned short)(rp[e].x >> 16);
             p3[e] = (unsigned short)(rp[e].y & 0xffffu);
           }
+          us8 s4;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) s4[e] = rl[e];
           *reinterpret_cast<us8*>(&Bt[buf][0][lc][8 * lg]) = s1;
           *reinterpret_cast<us8*>(&Bt[buf][1][lc][8 * lg]) = s2;
           *reinterpret_cast<us8*>(&Bt[buf][2][lc][8 * lg]) = s3;
+          *reinterpret_cast<us8*>(&Bt[buf][3][lc][8 * lg]) = s4;
           if (lc < NT) {
             *reinterpret_cast<us8*>(&Pt[buf][0][lc][8 * lg]) = p1;
             *reinterpret_cast<us8*>(&Pt[buf][1][lc][8 * lg]) = p2;
@@ -414,36 +453,34 @@ __global__ __launch_bounds__(256) void k_density_mx(
         }
         __syncthreads();
         if (s + 1 < nks) load(32 * (s + 1));
-        bf16x8 fa[BI][3], fb[2][3];
+        bf16x8 fa[BI][3];
 #pragma unroll
-        for (int sl = 0; sl < 3; ++sl) {
+        for (int sl = 0; sl < 3; ++sl)
 #pragma unroll
           for (int q = 0; q < BI; ++q)
             fa[q][sl] = frag(&Pt[buf][sl][wm * 16 * BI + q * 16 + (lane & 15)][kk]);
+        // Leading product p0 b0 in its own FP32 accumulator, the six
+        // correction products (p0 b1, p1 b0, p0 b2, p1 b1, p2 b0, p0 b3) in
+        // another: the large partial sum is rounded once per K block.
 #pragma unroll
-          for (int q = 0; q < 2; ++q)
-            fb[q][sl] = frag(&Bt[buf][sl][wn * 32 + q * 16 + (lane & 15)][kk]);
+        for (int bj = 0; bj < 2; ++bj) {
+          bf16x8 fb[4];
+#pragma unroll
+          for (int sl = 0; sl < 4; ++sl)
+            fb[sl] = frag(&Bt[buf][sl][wn * 32 + bj * 16 + (lane & 15)][kk]);
+#pragma unroll
+          for (int bi = 0; bi < BI; ++bi) {
+            const v4f cm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                fa[bi][0], fb[0], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            v4f cc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int pr = 0; pr < 6; ++pr)
+              cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[bi][corr_p(pr)], fb[corr_b(pr)],
+                                                           cc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)cm[r] + (double)cc[r];
+          }
         }
-        v4f c[BI][2];
-#pragma unroll
-        for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-          for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int pr = 0; pr < 6; ++pr) {
-#pragma unroll
-          for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-              c[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  fa[bi][sl_b(pr)], fb[bj][sl_a(pr)], c[bi][bj], 0, 0, 0);
-        }
-#pragma unroll
-        for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-          for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
       }
       double x[BI][2][4];
 #pragma unroll

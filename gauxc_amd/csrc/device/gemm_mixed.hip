@@ -2,15 +2,21 @@
 // (v_mfma_f32_16x16x32_bf16).
 //
 // Numerics (chosen from a CPU emulation against the FP64 path):
-//  * B, dB and Z are stored in FP32; P is held as three bfloat16 slices
-//    P ~ p1 + p2 + p3 (24 significant bits).  Each FP32 operand element is
-//    split on the fly into three bfloat16 slices the same way (exact: 3 x 8
-//    bits cover the FP32 significand).
-//  * A product A*B is formed from the six slice products a_s * b_t with
-//    s + t <= 2 (dropped terms are ~2^-24 relative), accumulated in FP32
-//    inside one 32-deep K block (one MFMA per slice product) and added to
-//    FP64 accumulators across K blocks, so FP32 rounding does not grow with
-//    the contraction length (npts can reach 2e4 in the B^T Z product).
+//  * dB/dx, dB/dy, dB/dz and Z are stored in FP32.  The value plane B is
+//    stored as FP32 plus a bfloat16 residual (plane 5 of the task scratch),
+//    about 32 significant bits: FP32 rounding of B was the largest error
+//    term of the EXC/VXC budget.
+//  * P is held as three bfloat16 slices p0 + p1 + p2 (24 significant bits).
+//    Each FP32 operand element is split on the fly into three bfloat16
+//    slices (exact: 3 x 8 bits cover the FP32 significand); B's residual is
+//    its fourth slice b3.
+//  * A product is formed from the leading slice product (one MFMA, its own
+//    FP32 accumulator, so the large partial sum is rounded once per K block)
+//    and six correction products (b0 z1, b1 z0, b0 z2, b1 z1, b2 z0, b3 z0;
+//    dropped terms are ~2^-24 relative) in a second FP32 accumulator, both
+//    added to FP64 accumulators after every 32-deep K block, so FP32
+//    rounding does not grow with the contraction length (npts can reach
+//    2e4 in the B^T Z product).
 //  * Densities, the functional, EXC/NEL and VXC accumulate in FP64.
 //
 // Kernels mirror the FP64 ones (gemm_fp64.hip; the density stage is in
@@ -53,13 +59,14 @@ __device__ __forceinline__ v4f mfma_v(bf16x8 a, bf16x8 b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Slice pairs (s, t) with s + t <= 2, largest terms first:
-// (0,0) (0,1) (1,0) (0,2) (1,1) (2,0).
-__device__ __forceinline__ constexpr int pair_a(int p) {
-  return p == 2 ? 1 : p == 4 ? 1 : p == 5 ? 2 : 0;
+// Correction products (B slice, Z slice) after the leading b0 z0:
+// (0,1) (1,0) (0,2) (1,1) (2,0) (3,0); slice 3 of B is the stored bfloat16
+// residual of the value plane.
+__device__ __forceinline__ constexpr int corr_b(int k) {
+  return k == 1 ? 1 : k == 3 ? 1 : k == 4 ? 2 : k == 5 ? 3 : 0;
 }
-__device__ __forceinline__ constexpr int pair_b(int p) {
-  return p == 1 ? 1 : p == 3 ? 2 : p == 4 ? 1 : 0;
+__device__ __forceinline__ constexpr int corr_z(int k) {
+  return k == 0 ? 1 : k == 2 ? 2 : k == 3 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -88,15 +95,15 @@ __global__ __launch_bounds__(256) void k_split_P(int64_t n, const double* __rest
 // f & 31 of the i panels (f < 32) or of the j panels (f >= 32).  Rows past
 // nbe are read at row nbe-1 and zeroed by a select.
 template <int NT>
-__global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
+__global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
                                                   const DevTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ bfmap,
                                                   const float* __restrict__ scratch,
                                                   double* __restrict__ vxc_all) {
   constexpr int BI = NT / 32;
-  __shared__ __attribute__((aligned(16))) unsigned short Bi[3][NT][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Bi[4][NT][kRow];
   __shared__ __attribute__((aligned(16))) unsigned short Zi[3][NT][kRow];
-  __shared__ __attribute__((aligned(16))) unsigned short Bj[3][NT][kRow];
+  __shared__ __attribute__((aligned(16))) unsigned short Bj[4][NT][kRow];
   __shared__ __attribute__((aligned(16))) unsigned short Zj[3][NT][kRow];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -118,6 +125,8 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
     const int plane = nbe * ld;
     const float* Bm = scratch + t.b_off;
     const float* Zm = Bm + 4 * plane;
+    // bfloat16 residual of the value plane (plane 5, 2 bytes per element)
+    const unsigned short* Lm = reinterpret_cast<const unsigned short*>(Bm + 5 * (int64_t)plane);
     const int64_t ldv = t.ldp;
     double* vxc = vxc_all + t.p_off;
     const int fi = i0 + frow, fj = j0 + frow;
@@ -134,8 +143,10 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
         for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
 
     float4 pre[4][2];  // (Bi, Zi, Bj, Zj) x two float4
+    uint4 prel[2];     // residuals of Bi, Bj (8 bfloat16 each)
     auto load = [&](int k0) {
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint4 zl = make_uint4(0u, 0u, 0u, 0u);
       if (do_i) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -144,6 +155,8 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
           pre[0][h] = ki ? b : z4;
           pre[1][h] = ki ? z : z4;
         }
+        const uint4 l = *reinterpret_cast<const uint4*>(Lm + oi + k0);
+        prel[0] = ki ? l : zl;
       }
       if (ldj) {
 #pragma unroll
@@ -153,6 +166,8 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
           pre[2][h] = kj ? b : z4;
           pre[3][h] = kj ? z : z4;
         }
+        const uint4 l = *reinterpret_cast<const uint4*>(Lm + oj + k0);
+        prel[1] = kj ? l : zl;
       }
     };
     auto stage = [&](const float4 (&v)[2], unsigned short (*dst)[NT][kRow]) {
@@ -171,47 +186,62 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
       *reinterpret_cast<us8*>(&dst[2][frow][pg * 8]) = s3;
     };
 
+    auto stage_lo = [&](const uint4& l, unsigned short (*dst)[NT][kRow]) {
+      *reinterpret_cast<uint4*>(&dst[3][frow][pg * 8]) = l;
+    };
+
     load(0);
     for (int k0 = 0; k0 < ld; k0 += kKB) {
       if (do_i) {
         stage(pre[0], Bi);
+        stage_lo(prel[0], Bi);
         stage(pre[1], Zi);
       }
       if (ldj) {
         stage(pre[2], Bj);
+        stage_lo(prel[1], Bj);
         stage(pre[3], Zj);
       }
       __syncthreads();
       if (k0 + kKB < ld) load(k0 + kKB);
       unsigned short(*BJ)[NT][kRow] = diag ? Bi : Bj;
       unsigned short(*ZJ)[NT][kRow] = diag ? Zi : Zj;
-      bf16x8 fbi[BI][3], fzi[BI][3], fbj[BI][3], fzj[BI][3];
+      bf16x8 fbi[BI][4], fzi[BI][3], fbj[BI][4], fzj[BI][3];
 #pragma unroll
       for (int q = 0; q < BI; ++q)
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
+        for (int s = 0; s < 4; ++s) {
           const int ra = wm * 16 * BI + q * 16 + (lane & 15);
           const int rb = wn * 16 * BI + q * 16 + (lane & 15);
           fbi[q][s] = ldfrag(&Bi[s][ra][kk]);
-          fzi[q][s] = ldfrag(&Zi[s][ra][kk]);
           fbj[q][s] = ldfrag(&BJ[s][rb][kk]);
-          fzj[q][s] = ldfrag(&ZJ[s][rb][kk]);
+          if (s < 3) {
+            fzi[q][s] = ldfrag(&Zi[s][ra][kk]);
+            fzj[q][s] = ldfrag(&ZJ[s][rb][kk]);
+          }
         }
       __syncthreads();
-      v4f c[BI][BI];
+      // Leading products b0 z0 in one FP32 accumulator, the corrections
+      // (b0 z1, b1 z0, b0 z2, b1 z1, b2 z0, b3 z0; b3 = stored residual) in
+      // another, for B_i^T Z_j and Z_i^T B_j alike.
+      v4f cm[BI][BI], cc[BI][BI];
 #pragma unroll
       for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < BI; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int bj = 0; bj < BI; ++bj) {
+          cm[bi][bj] = mfma_v(fbi[bi][0], fzj[bj][0], v4f{0.f, 0.f, 0.f, 0.f});
+          cm[bi][bj] = mfma_v(fzi[bi][0], fbj[bj][0], cm[bi][bj]);
+          cc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll
       for (int pr = 0; pr < 6; ++pr) {
-        const int sa = pair_a(pr), sb = pair_b(pr);
+        const int sb = corr_b(pr), sz = corr_z(pr);
 #pragma unroll
         for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
           for (int bj = 0; bj < BI; ++bj) {
-            c[bi][bj] = mfma_v(fbi[bi][sa], fzj[bj][sb], c[bi][bj]);  // B_i^T Z_j
-            c[bi][bj] = mfma_v(fzi[bi][sa], fbj[bj][sb], c[bi][bj]);  // Z_i^T B_j
+            cc[bi][bj] = mfma_v(fbi[bi][sb], fzj[bj][sz], cc[bi][bj]);  // B_i^T Z_j
+            cc[bi][bj] = mfma_v(fzi[bi][sz], fbj[bj][sb], cc[bi][bj]);  // Z_i^T B_j
           }
       }
 #pragma unroll
@@ -219,7 +249,8 @@ __global__ __launch_bounds__(256) void k_syr2k_mx(QueueArgs qa,
 #pragma unroll
         for (int bj = 0; bj < BI; ++bj)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
+          for (int r = 0; r < 4; ++r)
+            acc[bi][bj][r] += (double)cm[bi][bj][r] + (double)cc[bi][bj][r];
     }
 
     // Scatter-add into VXC in transposed order (see k_syr2k).
