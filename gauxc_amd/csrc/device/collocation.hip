@@ -175,10 +175,11 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
   }
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
-  T* base = scratch + t.b_off + p;
+  const int64_t boff = sizeof(T) == 4 ? t.b_off_mx : t.b_off;
+  T* base = scratch + boff + p;
   // FP32 storage: bfloat16 residual plane of the values after the five
   // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).
-  unsigned short* lo_base = reinterpret_cast<unsigned short*>(scratch + t.b_off + 5 * plane) + p;
+  unsigned short* lo_base = reinterpret_cast<unsigned short*>(scratch + boff + 5 * plane) + p;
   for (int s = wave; s < t.nshells; s += 4) {
     const int ish = shell_list[t.sh_off + s];
     const int bf0 = lsh_first[t.lsh_off + s];

@@ -676,16 +676,19 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
     const double avail = ctx->fill * (double)free_b - (double)resident;
     budget = (size_t)std::max(0.0, std::min(avail * 0.5, 48.0 * (1ull << 30)));
   }
+  // FP64 storage needs 40 bytes per (point, function) of a task, FP32 storage
+  // 22 (five float planes + the value plane's bfloat16 residual); the arena
+  // is sized for the larger.
   auto task_bytes = [](const DevTask& d) { return (size_t)5 * d.nbe * d.npts_pad * 8; };
   std::vector<Chunk> chunks;
-  int64_t max_se = 0;
+  int64_t max_bytes = 0;
   {
     int32_t i = 0;
     while (i < ntasks) {
       Chunk c;
       c.task_begin = i;
       size_t used = 0;
-      int64_t se = 0;
+      int64_t se = 0, se_mx = 0;
       while (i < ntasks) {
         auto& d = ht[i];
         const size_t b = task_bytes(d);
@@ -695,13 +698,16 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
           break;
         }
         used += b;
+        const int64_t plane = (int64_t)d.nbe * d.npts_pad;  // npts_pad is a multiple of 64
         d.b_off = se;
-        se += (int64_t)5 * d.nbe * d.npts_pad;
+        d.b_off_mx = se_mx;
+        se += 5 * plane;
+        se_mx += 5 * plane + plane / 2;
         ++i;
       }
       c.task_end = i;
       c.scratch_elems = se;
-      max_se = std::max(max_se, se);
+      max_bytes = std::max(max_bytes, std::max(se * 8, se_mx * 4));
       chunks.push_back(c);
     }
   }
@@ -727,7 +733,7 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
   dfree(ctx->d_mol_blk);
   if ((rc = dalloc(&ctx->d_mol_blk, mol_blocks.size()))) return rc;
-  ctx->scratch_bytes = (size_t)max_se * sizeof(double);
+  ctx->scratch_bytes = (size_t)max_bytes;
   if (ctx->scratch_bytes) {
     hipError_t e = hipMalloc(&ctx->d_scratch, ctx->scratch_bytes);
     if (e != hipSuccess) {
@@ -1232,7 +1238,7 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
       if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
       std::vector<float> hf(host.size());
       if (rc == XCQ_SUCCESS && e == hipSuccess)
-        e = hipMemcpy(hf.data(), scr + d.b_off, 4 * hf.size(), hipMemcpyDeviceToHost);
+        e = hipMemcpy(hf.data(), scr + d.b_off_mx, 4 * hf.size(), hipMemcpyDeviceToHost);
       for (size_t k = 0; k < hf.size(); ++k) host[k] = hf[k];
     }
   }

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
     const int plane = nbe * ld;
     const int64_t ldp = t.ldp;
     const uint2* Ps = Psall + t.p_off;
-    float* Bm = scratch + t.b_off;
+    float* Bm = scratch + t.b_off_mx;
     const int32_t* bf = bfmap + t.bf_off;
     for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
     __syncthreads();

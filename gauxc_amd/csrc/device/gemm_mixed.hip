@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
     const int ld = t.npts_pad;
     const int nbe = t.nbe;
     const int plane = nbe * ld;
-    const float* Bm = scratch + t.b_off;
+    const float* Bm = scratch + t.b_off_mx;
     const float* Zm = Bm + 4 * plane;
     // bfloat16 residual of the value plane (plane 5, 2 bytes per element)
     const unsigned short* Lm = reinterpret_cast<const unsigned short*>(Bm + 5 * (int64_t)plane);

@@ -56,7 +56,11 @@ struct DevTask {
   int64_t sh_off;     // shell list
   int64_t bf_off;     // local -> molecule basis function map (nbe entries)
   int64_t lsh_off;    // local shell -> first local basis function (nshells entries)
-  int64_t b_off;      // chunk scratch: [5][nbe][npts_pad] (B, Bx, By, Bz, Z) in elements
+  int64_t b_off;      // chunk scratch, FP64 storage: [5][nbe][npts_pad] doubles
+                      // (B, Bx, By, Bz, Z), offset in doubles
+  int64_t b_off_mx;   // chunk scratch, FP32 storage: the same five planes in
+                      // floats + the value plane's bfloat16 residual
+                      // [nbe][npts_pad], offset in floats (5.5 planes per task)
   int64_t red_off;    // global index of the task's first point block (EXC/NEL partials)
   int64_t p_off;      // offset of the molecule's P (and P slices, VXC) in elements
 };
