@@ -10,7 +10,7 @@
 //                          [SYR2K tiles,   32-wide | 64-wide]
 //   qoff        queue tables of the persistent launches
 //   P, VXC      per molecule nbf x nbf, column-major, ld = nbf, concatenated
-//   P slices    packed bfloat16 slices of P (mixed precision), same offsets
+//   P (FP32)    FP32 copy of P (mixed precision), same offsets
 //   red         (w eps rho, w rho) per point block, molecule-major
 // Chunk scratch (reused by every chunk): per task [5][nbe][npts_pad]
 // (B, dB/dx, dB/dy, dB/dz, Z).  Tasks are grouped greedily into chunks that
@@ -152,8 +152,8 @@ struct xcq_context_s {
 
   // density / results
   double* d_P = nullptr;
-  uint2* d_Psplit = nullptr;
-  bool psplit_valid = false;
+  float* d_Pf32 = nullptr;  // FP32 copy of P (mixed precision)
+  bool pf32_valid = false;
   double* d_vxc = nullptr;
   double* d_red = nullptr;
   double* d_out2 = nullptr;
@@ -214,7 +214,7 @@ struct xcq_context_s {
     dfree(d_mol_off);
     dfree(d_mol_blk);
     dfree(d_P);
-    dfree(d_Psplit);
+    dfree(d_Pf32);
     dfree(d_vxc);
     dfree(d_out2);
     if (stream) (void)hipStreamDestroy(stream);
@@ -227,7 +227,7 @@ CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream,  c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap,
                  c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
                  c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
-                 c->d_P,     c->d_Psplit, c->d_vxc,   c->d_red,        c->d_scratch};
+                 c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch};
 }
 
 // Heuristic per-item costs (used when no predicted task costs are set):
@@ -393,15 +393,15 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
               XCQ_ERROR_INVALID_ARGUMENT, "unknown precision");
   XCQ_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  if (precision == XCQ_PREC_MIXED && !ctx->psplit_valid) {
-    if (!ctx->d_Psplit) {
-      int rc = dalloc(&ctx->d_Psplit, (size_t)ctx->p_total);
+  if (precision == XCQ_PREC_MIXED && !ctx->pf32_valid) {
+    if (!ctx->d_Pf32) {
+      int rc = dalloc(&ctx->d_Pf32, (size_t)ctx->p_total);
       if (rc) return rc;
       ctx->drop_graph();
     }
-    int rc = launch_split_P(s, ctx->p_total, ctx->d_P, ctx->d_Psplit);
+    int rc = launch_P_to_f32(s, ctx->p_total, ctx->d_P, ctx->d_Pf32);
     if (rc) return rc;
-    ctx->psplit_valid = true;
+    ctx->pf32_valid = true;
   }
   ctx->last_want_vxc = want_vxc;
   if (!ctx->use_graph) {
@@ -560,7 +560,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
                  [](const MolInfo& a, const MolInfo& b) { return a.nbf == b.nbf; });
   if (!same_shape) {
     dfree(ctx->d_P);
-    dfree(ctx->d_Psplit);
+    dfree(ctx->d_Pf32);
     dfree(ctx->d_vxc);
     dfree(ctx->d_out2);
     dfree(ctx->d_mol_n);
@@ -962,7 +962,7 @@ int xcq_upload_density_mol(xcq_context ctx, int32_t mol, int32_t nbf, const doub
                            (size_t)nbf * 8, nbf, hipMemcpyHostToDevice, ctx->stream));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
   M.have_density = true;
-  ctx->psplit_valid = false;  // the bfloat16 slices follow the new density
+  ctx->pf32_valid = false;  // the FP32 copy follows the new density
   return XCQ_SUCCESS;
 }
 

@@ -35,44 +35,15 @@
 namespace xcq {
 
 using v4d = double __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
 constexpr int kPad64 = 68;  // FP64 stage row (doubles)
-constexpr int kRowMx = 40;  // mixed stage row (bf16): 32 + 8 pad
 
-__device__ __forceinline__ unsigned short bfbits(__bf16 v) {
-  return __builtin_bit_cast(unsigned short, v);
-}
 
-__device__ __forceinline__ void split3(float x, unsigned short& a, unsigned short& b,
-                                       unsigned short& c) {
-  const __bf16 h1 = (__bf16)x;
-  const float r1 = x - (float)h1;
-  const __bf16 h2 = (__bf16)r1;
-  const float r2 = r1 - (float)h2;
-  const __bf16 h3 = (__bf16)r2;
-  a = bfbits(h1);
-  b = bfbits(h2);
-  c = bfbits(h3);
-}
 
-__device__ __forceinline__ bf16x8 frag(const unsigned short* p) {
-  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const us8*>(p));
-}
 
-// Correction products (P slice, B slice) after the leading p0 b0:
-// (0,1) (1,0) (0,2) (1,1) (2,0) (0,3); b3 is the stored bfloat16 residual of
-// the value plane.
-__device__ __forceinline__ constexpr int corr_p(int k) {
-  return k == 1 ? 1 : k == 3 ? 1 : k == 4 ? 2 : 0;
-}
-__device__ __forceinline__ constexpr int corr_b(int k) {
-  return k == 0 ? 1 : k == 2 ? 2 : k == 3 ? 1 : k == 5 ? 3 : 0;
-}
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
@@ -351,28 +322,32 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision (v_mfma_f32_16x16x32_bf16; P in three bfloat16 slices,
-// B in four: three exact slices of its FP32 part and the stored residual;
-// leading and correction products in separate FP32 accumulators inside one
-// K block, FP64 across; accumulator register r of lane l holds row
-// 4(l/16) + r).  Loader roles: thread (c = tid & 63, g = tid >> 6)
-// loads K rows 8g..8g+7 of column c of the B^T stage and, for c < NT, of
-// the P_sub stage.
+// Mixed precision: FP32 operands on v_mfma_f32_16x16x4_f32 (lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; accumulator register r of lane l is row
+// 4(l/16) + r), FP32 accumulation inside one 32-deep K block (eight chained
+// MFMAs), FP64 across blocks.  The FP32 matrix cores round to nearest on
+// every accumulation; the BF16 ones drop the low bits of the smaller
+// products of a dot product, a bias towards zero that compounds into the
+// integrated density (measured on gfx950: -0.6 ulp per 32-term BF16 dot
+// product of positive terms spanning 20 binades, -0.004 ulp for the FP32
+// MFMA on the same data).  The B^T operand is the FP32 part of the value
+// plane; its bfloat16 residual enters the rho fold and Z.  P is gathered
+// from an FP32 copy.  Waves and loader roles as in the FP64 kernel, with K
+// rows 8g..8g+7 per loader thread.
 template <int NT>
 __global__ __launch_bounds__(256, 2) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
-    const uint2* __restrict__ Psall, float* __restrict__ scratch,
+    const float* __restrict__ Pfall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, int gga, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
-  __shared__ __attribute__((aligned(16))) unsigned short Pt[2][3][NT][kRowMx];  // [s][fn][k]
-  __shared__ __attribute__((aligned(16))) unsigned short Bt[2][4][64][kRowMx];  // [s][pt][k]
+  __shared__ float Pt[2][32][NT + 4];  // [buf][k][function]: A operand (P_sub)
+  __shared__ float Bt[2][32][68];      // [buf][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int lc = lane, lg = wave;
-  const int kk = 8 * (lane >> 4);
   auto frow = [&](int r) { return 4 * (lane >> 4) + r; };
 
   XCQ_FOR_EACH_ITEM(qa, it) {
@@ -384,13 +359,12 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
     const int nbe = t.nbe;
     const int plane = nbe * ld;
     const int64_t ldp = t.ldp;
-    const uint2* Ps = Psall + t.p_off;
+    const float* Pf = Pfall + t.p_off;
     float* Bm = scratch + t.b_off_mx;
     const int32_t* bf = bfmap + t.bf_off;
     for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
     __syncthreads();
     const float* bcol = Bm + m0 + lc;
-    const unsigned short* lcol = lo_plane(Bm, plane) + m0 + lc;
     const int nks = (nbe + 31) / 32;
 
     for (int n0 = 0; n0 < nbe; n0 += NT) {
@@ -398,21 +372,17 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
       // neighbouring lanes (neighbouring n) touch neighbouring columns.
       const int ncol = n0 + lc;
       const bool colok = lc < NT && ncol < nbe;
-      const uint2* Pc = Ps + bf[min(ncol, nbe - 1)];
-      float rb[8];
-      unsigned short rl[8];
-      uint2 rp[8];
+      const float* Pc = Pf + bf[min(ncol, nbe - 1)];
+      float rb[8], rp[8];
       auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int k = k0 + 8 * lg + e;  // wave-uniform
           const int kc = min(k, nbe - 1);
           const float b = bcol[kc * ld];
-          const unsigned short l = lcol[kc * ld];
-          const uint2 p = Pc[(int64_t)bf[kc] * ldp];
+          const float p = Pc[(int64_t)bf[kc] * ldp];
           rb[e] = k < nbe ? b : 0.f;
-          rl[e] = k < nbe ? l : (unsigned short)0;
-          rp[e] = (k < nbe && colok) ? p : make_uint2(0u, 0u);
+          rp[e] = (k < nbe && colok) ? (float)fac * p : 0.f;
         }
       };
       double acc[BI][2][4];
@@ -425,71 +395,41 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
       load(0);
       for (int s = 0; s < nks; ++s) {
         const int buf = s & 1;
-        {
-          us8 s1, s2, s3, p1, p2, p3;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            unsigned short a, b, c;
-            split3(rb[e], a, b, c);
-            s1[e] = a;
-            s2[e] = b;
-            s3[e] = c;
-            p1[e] = (unsigned short)(rp[e].x & 0xffffu);
-            p2[e] = (unsigned short)(rp[e].x >> 16);
-            p3[e] = (unsigned short)(rp[e].y & 0xffffu);
-          }
-          us8 s4;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) s4[e] = rl[e];
-          *reinterpret_cast<us8*>(&Bt[buf][0][lc][8 * lg]) = s1;
-          *reinterpret_cast<us8*>(&Bt[buf][1][lc][8 * lg]) = s2;
-          *reinterpret_cast<us8*>(&Bt[buf][2][lc][8 * lg]) = s3;
-          *reinterpret_cast<us8*>(&Bt[buf][3][lc][8 * lg]) = s4;
-          if (lc < NT) {
-            *reinterpret_cast<us8*>(&Pt[buf][0][lc][8 * lg]) = p1;
-            *reinterpret_cast<us8*>(&Pt[buf][1][lc][8 * lg]) = p2;
-            *reinterpret_cast<us8*>(&Pt[buf][2][lc][8 * lg]) = p3;
-          }
+        for (int e = 0; e < 8; ++e) {
+          Bt[buf][8 * lg + e][lc] = rb[e];
+          if (lc < NT) Pt[buf][8 * lg + e][lc] = rp[e];
         }
         __syncthreads();
         if (s + 1 < nks) load(32 * (s + 1));
-        bf16x8 fa[BI][3];
+        v4f c[BI][2];
 #pragma unroll
-        for (int sl = 0; sl < 3; ++sl)
+        for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-          for (int q = 0; q < BI; ++q)
-            fa[q][sl] = frag(&Pt[buf][sl][wm * 16 * BI + q * 16 + (lane & 15)][kk]);
-        // Leading product p0 b0 in its own FP32 accumulator, the six
-        // correction products (p0 b1, p1 b0, p0 b2, p1 b1, p2 b0, p0 b3) in
-        // another: the large partial sum is rounded once per K block.
+          for (int bj = 0; bj < 2; ++bj) c[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-          bf16x8 fb[4];
+        for (int ks = 0; ks < 8; ++ks) {
+          const int kr = ks * 4 + (lane >> 4);
+          float a_[BI];
 #pragma unroll
-          for (int sl = 0; sl < 4; ++sl)
-            fb[sl] = frag(&Bt[buf][sl][wn * 32 + bj * 16 + (lane & 15)][kk]);
+          for (int bi = 0; bi < BI; ++bi) a_[bi] = Pt[buf][kr][wm * 16 * BI + bi * 16 + (lane & 15)];
+          const float b0 = Bt[buf][kr][wn * 32 + (lane & 15)];
+          const float b1 = Bt[buf][kr][wn * 32 + 16 + (lane & 15)];
 #pragma unroll
           for (int bi = 0; bi < BI; ++bi) {
-            const v4f cm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                fa[bi][0], fb[0], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            v4f cc = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int pr = 0; pr < 6; ++pr)
-              cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[bi][corr_p(pr)], fb[corr_b(pr)],
-                                                           cc, 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)cm[r] + (double)cc[r];
+            c[bi][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_[bi], b0, c[bi][0], 0, 0, 0);
+            c[bi][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_[bi], b1, c[bi][1], 0, 0, 0);
           }
         }
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
       }
-      double x[BI][2][4];
-#pragma unroll
-      for (int i = 0; i < BI; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) x[i][j][r] = fac * acc[i][j][r];
-      fold_tile<float, BI>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      fold_tile<float, BI>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      // The next tile's first stage write targets buffer 0 (see k_density_f64).
       __syncthreads();
     }
     finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
@@ -519,10 +459,10 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, 
     float* scr = static_cast<float*>(v.scratch);
     if (nt == 32)
       hipLaunchKernelGGL(k_density_mx<32>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
-                         v.Psplit, scr, v.w, func, gga, fac, want_z, v.red);
+                         v.Pf32, scr, v.w, func, gga, fac, want_z, v.red);
     else
       hipLaunchKernelGGL(k_density_mx<64>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
-                         v.Psplit, scr, v.w, func, gga, fac, want_z, v.red);
+                         v.Pf32, scr, v.w, func, gga, fac, want_z, v.red);
   }
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;

@@ -70,21 +70,11 @@ __device__ __forceinline__ constexpr int corr_z(int k) {
 }
 
 // ---------------------------------------------------------------------------
-// P (FP64, nbf x nbf) -> packed (p1, p2, p3, 0) bfloat16 slices.
-__global__ __launch_bounds__(256) void k_split_P(int64_t n, const double* __restrict__ P,
-                                                 uint2* __restrict__ out) {
+// P (FP64, nbf x nbf) -> FP32 copy (the mixed density stage's P_sub source).
+__global__ __launch_bounds__(256) void k_P_to_f32(int64_t n, const double* __restrict__ P,
+                                                  float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double x = P[i];
-  const __bf16 h1 = (__bf16)(float)x;
-  const double r1 = x - (double)(float)h1;
-  const __bf16 h2 = (__bf16)(float)r1;
-  const double r2 = r1 - (double)(float)h2;
-  const __bf16 h3 = (__bf16)(float)r2;
-  uint2 v;
-  v.x = (unsigned)bf_bits(h1) | ((unsigned)bf_bits(h2) << 16);
-  v.y = (unsigned)bf_bits(h3);
-  out[i] = v;
+  if (i < n) out[i] = (float)P[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -288,9 +278,9 @@ int launch_syr2k_mixed(const CtxView& v, const WorkRange& r, int nt) {
   return XCQ_SUCCESS;
 }
 
-int launch_split_P(hipStream_t s, int64_t n, const double* P, uint2* out) {
+int launch_P_to_f32(hipStream_t s, int64_t n, const double* P, float* out) {
   if (n == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_split_P, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, P, out);
+  hipLaunchKernelGGL(k_P_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, P, out);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -106,7 +106,7 @@ struct CtxView {
   const int32_t* qoff;
   int64_t* tstamp;  // nullptr unless task timing is on
   const double* P;
-  const uint2* Psplit;
+  const float* Pf32;  // FP32 copy of P (mixed-precision density stage)
   double* vxc;
   double* red;
   void* scratch;
@@ -151,7 +151,7 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, 
                    int gga, double fac, int want_z);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);
-int launch_split_P(hipStream_t s, int64_t n, const double* P, uint2* out);
+int launch_P_to_f32(hipStream_t s, int64_t n, const double* P, float* out);
 int launch_fill(hipStream_t s, double* p, int64_t n);
 // Symmetrise the VXC of nmol molecules: mol_n[m] = nbf, mol_off[m] = offset.
 int launch_symmetrize(hipStream_t s, int32_t nmol, int32_t max_n, const int32_t* mol_n,
