@@ -257,16 +257,19 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
       const int ncol = n0 + lc;
       const bool colok = lc < NT && ncol < nbe;
       const double* Pc = P + bf[min(ncol, nbe - 1)];  // column bf[n] of P (P symmetric)
+      // Prefetch registers hold raw loads; the masks and the factor are
+      // applied when they are staged, so no load is waited for before the
+      // matrix instructions of the current step are issued.
       double rb[4], rp[4];
       auto load = [&](int k0) {
+        int rowp[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rowp[e] = bf[min(k0 + 4 * lq + e, nbe - 1)];  // scalar
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int k = k0 + 4 * lq + e;  // wave-uniform
-          const int kc = min(k, nbe - 1);
-          const double b = bcol[kc * ld];
-          const double p = Pc[(int64_t)bf[kc] * ldp];
-          rb[e] = k < nbe ? b : 0.;
-          rp[e] = (k < nbe && colok) ? fac * p : 0.;
+          const int kc = min(k0 + 4 * lq + e, nbe - 1);
+          rb[e] = bcol[kc * ld];
+          rp[e] = Pc[(int64_t)rowp[e] * ldp];
         }
       };
       v4d acc[BI][2];
@@ -279,8 +282,9 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
         const int buf = s & 1;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          Bt[buf][4 * lq + e][lc] = rb[e];
-          if (lc < NT) Pt[buf][4 * lq + e][lc] = rp[e];
+          const bool kok = 16 * s + 4 * lq + e < nbe;
+          Bt[buf][4 * lq + e][lc] = kok ? rb[e] : 0.;
+          if (lc < NT) Pt[buf][4 * lq + e][lc] = (kok && colok) ? fac * rp[e] : 0.;
         }
         __syncthreads();
         if (s + 1 < nks) load(16 * (s + 1));
@@ -373,18 +377,20 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
       const int ncol = n0 + lc;
       const bool colok = lc < NT && ncol < nbe;
       const float* Pc = Pf + bf[min(ncol, nbe - 1)];
+      // Raw prefetch (masks and factor applied when staged; see k_density_f64).
       float rb[8], rp[8];
       auto load = [&](int k0) {
+        int rowp[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) rowp[e] = bf[min(k0 + 8 * lg + e, nbe - 1)];  // scalar
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const int k = k0 + 8 * lg + e;  // wave-uniform
-          const int kc = min(k, nbe - 1);
-          const float b = bcol[kc * ld];
-          const float p = Pc[(int64_t)bf[kc] * ldp];
-          rb[e] = k < nbe ? b : 0.f;
-          rp[e] = (k < nbe && colok) ? (float)fac * p : 0.f;
+          const int kc = min(k0 + 8 * lg + e, nbe - 1);
+          rb[e] = bcol[kc * ld];
+          rp[e] = Pc[(int64_t)rowp[e] * ldp];
         }
       };
+      const float ffac = (float)fac;
       double acc[BI][2][4];
 #pragma unroll
       for (int a = 0; a < BI; ++a)
@@ -397,8 +403,9 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
         const int buf = s & 1;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          Bt[buf][8 * lg + e][lc] = rb[e];
-          if (lc < NT) Pt[buf][8 * lg + e][lc] = rp[e];
+          const bool kok = 32 * s + 8 * lg + e < nbe;
+          Bt[buf][8 * lg + e][lc] = kok ? rb[e] : 0.f;
+          if (lc < NT) Pt[buf][8 * lg + e][lc] = (kok && colok) ? ffac * rp[e] : 0.f;
         }
         __syncthreads();
         if (s + 1 < nks) load(32 * (s + 1));

@@ -68,20 +68,20 @@ __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __re
       oi[q] = min(fi, nbe - 1) * ld + l_p;
       oj[q] = min(fj, nbe - 1) * ld + l_p;
     }
+    // Raw prefetch registers; the row masks are applied when staged, so the
+    // loads stay in flight across the current step's matrix instructions.
     double rbi[NQ], rzi[NQ], rbj[NQ], rzj[NQ];
     auto load = [&](int k0) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const double b = Bm[oi[q] + k0], z = Zm[oi[q] + k0];
-        rbi[q] = ki[q] ? b : 0.;
-        rzi[q] = ki[q] ? z : 0.;
+        rbi[q] = Bm[oi[q] + k0];
+        rzi[q] = Zm[oi[q] + k0];
       }
       if (!diag) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          const double b = Bm[oj[q] + k0], z = Zm[oj[q] + k0];
-          rbj[q] = kj[q] ? b : 0.;
-          rzj[q] = kj[q] ? z : 0.;
+          rbj[q] = Bm[oj[q] + k0];
+          rzj[q] = Zm[oj[q] + k0];
         }
       }
     };
@@ -96,11 +96,11 @@ __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __re
     for (int k0 = 0; k0 < ld; k0 += 16) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        Bi[l_p][l_f + 16 * q] = rbi[q];
-        Zi[l_p][l_f + 16 * q] = rzi[q];
+        Bi[l_p][l_f + 16 * q] = ki[q] ? rbi[q] : 0.;
+        Zi[l_p][l_f + 16 * q] = ki[q] ? rzi[q] : 0.;
         if (!diag) {
-          Bj[l_p][l_f + 16 * q] = rbj[q];
-          Zj[l_p][l_f + 16 * q] = rzj[q];
+          Bj[l_p][l_f + 16 * q] = kj[q] ? rbj[q] : 0.;
+          Zj[l_p][l_f + 16 * q] = kj[q] ? rzj[q] : 0.;
         }
       }
       __syncthreads();

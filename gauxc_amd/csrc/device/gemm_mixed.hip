@@ -132,41 +132,35 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
 
+    // Raw prefetch registers; the row masks are applied when staged, so the
+    // loads stay in flight across the current step's matrix instructions.
     float4 pre[4][2];  // (Bi, Zi, Bj, Zj) x two float4
     uint4 prel[2];     // residuals of Bi, Bj (8 bfloat16 each)
     auto load = [&](int k0) {
-      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-      const uint4 zl = make_uint4(0u, 0u, 0u, 0u);
       if (do_i) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const float4 b = *reinterpret_cast<const float4*>(Bm + oi + k0 + 4 * h);
-          const float4 z = *reinterpret_cast<const float4*>(Zm + oi + k0 + 4 * h);
-          pre[0][h] = ki ? b : z4;
-          pre[1][h] = ki ? z : z4;
+          pre[0][h] = *reinterpret_cast<const float4*>(Bm + oi + k0 + 4 * h);
+          pre[1][h] = *reinterpret_cast<const float4*>(Zm + oi + k0 + 4 * h);
         }
-        const uint4 l = *reinterpret_cast<const uint4*>(Lm + oi + k0);
-        prel[0] = ki ? l : zl;
+        prel[0] = *reinterpret_cast<const uint4*>(Lm + oi + k0);
       }
       if (ldj) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const float4 b = *reinterpret_cast<const float4*>(Bm + oj + k0 + 4 * h);
-          const float4 z = *reinterpret_cast<const float4*>(Zm + oj + k0 + 4 * h);
-          pre[2][h] = kj ? b : z4;
-          pre[3][h] = kj ? z : z4;
+          pre[2][h] = *reinterpret_cast<const float4*>(Bm + oj + k0 + 4 * h);
+          pre[3][h] = *reinterpret_cast<const float4*>(Zm + oj + k0 + 4 * h);
         }
-        const uint4 l = *reinterpret_cast<const uint4*>(Lm + oj + k0);
-        prel[1] = kj ? l : zl;
+        prel[1] = *reinterpret_cast<const uint4*>(Lm + oj + k0);
       }
     };
-    auto stage = [&](const float4 (&v)[2], unsigned short (*dst)[NT][kRow]) {
+    auto stage = [&](const float4 (&v)[2], bool ok, unsigned short (*dst)[NT][kRow]) {
       const float* pv = reinterpret_cast<const float*>(&v[0]);  // 8 consecutive points
       us8 s1, s2, s3;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         unsigned short a, b, c;
-        split3f(pv[e], a, b, c);
+        split3f(ok ? pv[e] : 0.f, a, b, c);
         s1[e] = a;
         s2[e] = b;
         s3[e] = c;
@@ -176,21 +170,21 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
       *reinterpret_cast<us8*>(&dst[2][frow][pg * 8]) = s3;
     };
 
-    auto stage_lo = [&](const uint4& l, unsigned short (*dst)[NT][kRow]) {
-      *reinterpret_cast<uint4*>(&dst[3][frow][pg * 8]) = l;
+    auto stage_lo = [&](const uint4& l, bool ok, unsigned short (*dst)[NT][kRow]) {
+      *reinterpret_cast<uint4*>(&dst[3][frow][pg * 8]) = ok ? l : make_uint4(0u, 0u, 0u, 0u);
     };
 
     load(0);
     for (int k0 = 0; k0 < ld; k0 += kKB) {
       if (do_i) {
-        stage(pre[0], Bi);
-        stage_lo(prel[0], Bi);
-        stage(pre[1], Zi);
+        stage(pre[0], ki, Bi);
+        stage_lo(prel[0], ki, Bi);
+        stage(pre[1], ki, Zi);
       }
       if (ldj) {
-        stage(pre[2], Bj);
-        stage_lo(prel[1], Bj);
-        stage(pre[3], Zj);
+        stage(pre[2], kj, Bj);
+        stage_lo(prel[1], kj, Bj);
+        stage(pre[3], kj, Zj);
       }
       __syncthreads();
       if (k0 + kKB < ld) load(k0 + kKB);
