@@ -53,27 +53,31 @@ def log(msg):
 def algorithmic(stage, prec, ts):
     """Algorithmic (bytes, flops) of one evaluation's launches of a stage.
 
-    Per task t (npts, nbe; s = 8 B for FP64 planes, 4 B for mixed):
-      collocation  writes B, dB/dx, dB/dy, dB/dz: 4 s npts nbe
-                   (+ 32 npts for points and weights)
-      density      reads B, grad B, writes Z: 5 s npts nbe; gathers P_sub
-                   once: 8 nbe^2; flops 2 npts nbe^2 (X = B P_sub)
-      syr2k        reads B and Z: 2 s npts nbe; VXC lower triangle
-                   read+write: 8 nbe (nbe+1); flops 2 npts nbe^2
-                   (B^T Z + Z^T B, lower triangle)
+    Per task t (npts, nbe).  Bytes per (point, function) element of the
+    planes: FP64 stores B, dB/dx, dB/dy, dB/dz, Z at 8 B each; mixed stores
+    the value plane B at 6 B (FP32 + bfloat16 residual) and the other four
+    at 4 B.  P_sub is gathered once per task (8 B FP64, 4 B FP32 copy).
+      collocation  writes B and grad B (+ 32 npts for points and weights)
+      density      reads B and grad B, writes Z; gathers P_sub;
+                   flops 2 npts nbe^2 (X = B P_sub)
+      syr2k        reads B and Z; VXC lower triangle read + write
+                   8 nbe (nbe + 1); flops 2 npts nbe^2 (B^T Z + Z^T B, lower)
     """
     npts = ts["npts"].astype(np.float64)
     nbe = ts["nbe"].astype(np.float64)
-    s = 8.0 if prec == "fp64" else 4.0
+    if prec == "fp64":
+        bv, bg, bz, bp = 8.0, 8.0, 8.0, 8.0
+    else:
+        bv, bg, bz, bp = 6.0, 4.0, 4.0, 4.0
     pn = float(np.sum(npts * nbe))
     pnn = float(np.sum(npts * nbe * nbe))
     nn = float(np.sum(nbe * nbe))
     if stage == "collocation":
-        return 4 * s * pn + 32 * float(npts.sum()), 0.0
+        return (bv + 3 * bg) * pn + 32 * float(npts.sum()), 0.0
     if stage == "density":
-        return 5 * s * pn + 8 * nn, 2 * pnn
+        return (bv + 3 * bg + bz) * pn + bp * nn, 2 * pnn
     if stage == "syr2k_scatter":
-        return 2 * s * pn + 8 * float(np.sum(nbe * (nbe + 1))), 2 * pnn
+        return (bv + bz) * pn + 8 * float(np.sum(nbe * (nbe + 1))), 2 * pnn
     return 0.0, 0.0
 
 
