@@ -106,9 +106,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
-    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c3", "c4"],
                     help="c1: 24-water cluster (the metric's workload, default); "
+                         "c3: 64 independent clusters of 2-40 waters, batched; "
                          "c4: 150-water cluster in a 17 A box (7800 bf)")
+    ap.add_argument("--learned", action="store_true",
+                    help="also compare the learned cost model against GauXC's flop "
+                         "heuristic (prediction error and queued wall time)")
+    ap.add_argument("--n-mol", type=int, default=64, help="molecules of the c3 batch")
     ap.add_argument("--n-water", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -140,6 +145,8 @@ def main():
     from gauxc_amd import workloads
 
     assert gx.device_count() > 0, "bench.py needs a HIP device"
+    if args.config == "c3":
+        return run_batched(args)
     t0 = time.time()
     mol, bs, P, cfg = workloads.make_config(args.config, seed=0, n_water=args.n_water)
     lb = workloads.build_tasks(mol, bs, grid=cfg["grid"])
@@ -193,6 +200,9 @@ def main():
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
             f"stages(ms)={np.round(stage, 3).tolist()}")
     ctx.use_graph(False)
+    learned = None
+    if args.learned:
+        learned = learned_schedule(ctx, bs, ts)
     head = "mixed" if "mixed" in results else precisions[0]
     r = results[head]
     accuracy = None
@@ -251,6 +261,172 @@ def main():
         "exc": r["exc"],
         "mixed_vs_fp64": accuracy,
     }
+    if learned is not None:
+        line["learned_schedule"] = learned
+    print(json.dumps(line), flush=True)
+
+
+def time_eval(ctx, prec, reps=3):
+    """Median wall time (ms) of `reps` synchronised evaluations."""
+    ctx.exc_vxc("PBE", prec, True)
+    ctx.synchronize()
+    ts = []
+    for _ in range(reps):
+        tic = time.perf_counter()
+        ctx.exc_vxc("PBE", prec, True)
+        ctx.synchronize()
+        ts.append((time.perf_counter() - tic) * 1e3)
+    return float(np.median(ts))
+
+
+def measure_task_times(ctx, prec="mixed", reps=3):
+    """Per-task (density, B^T Z) times in microseconds, median of `reps`."""
+    ctx.enable_task_timing(True)
+    ctx.exc_vxc("PBE", prec, True)
+    out = []
+    for _ in range(reps):
+        ctx.exc_vxc("PBE", prec, True)
+        out.append(np.stack(ctx.task_times(), axis=1))
+    ctx.enable_task_timing(False)
+    return np.median(np.stack(out), axis=0)
+
+
+def learned_schedule(ctx, bs, ts, prec="mixed"):
+    """Learned cost model against GauXC's cost_exc_vxc heuristic on the
+    uploaded workload: train on half of the tasks' measured times, report the
+    prediction error on the other half, then the wall time of persistent
+    queues packed by each model (and of plain one-workgroup-per-item
+    dispatch)."""
+    from gauxc_amd.costmodel import HeuristicModel, MLPCostModel, task_features
+    tic = time.perf_counter()
+    times = measure_task_times(ctx, prec)
+    feats = task_features(ts, bs.xcq_shells())
+    nt = len(times)
+    perm = np.random.default_rng(0).permutation(nt)
+    tr, te = perm[: nt // 2], perm[nt // 2:]
+    sub = lambda idx: {"npts": ts["npts"][idx], "nbe": ts["nbe"][idx]}
+    heur = HeuristicModel().fit(sub(tr), times[tr])
+    mlp = MLPCostModel(0, seed=1)
+    loss = mlp.fit(feats[tr], times[tr], epochs=300, batch=512, lr=1e-3)
+    mae_h = np.abs(heur.predict(sub(te)) - times[te]).mean(axis=0)
+    mae_m = np.abs(mlp.predict(feats[te]) - times[te]).mean(axis=0)
+    nq = 4 * 256
+    wall = {"direct": None, "queues_heuristic": None, "queues_mlp": None}
+    ctx.set_schedule(0)
+    wall["direct"] = time_eval(ctx, prec)
+    hp = heur.predict(ts)
+    ctx.set_schedule(nq, hp[:, 0], hp[:, 1])
+    wall["queues_heuristic"] = time_eval(ctx, prec)
+    mp = mlp.predict(feats)
+    ctx.set_schedule(nq, mp[:, 0], mp[:, 1])
+    wall["queues_mlp"] = time_eval(ctx, prec)
+    ctx.set_schedule(0)
+    res = {"tasks": int(nt), "train_loss": float(loss),
+           "mae_us_density_syr2k": {"heuristic": mae_h.tolist(), "mlp": mae_m.tolist()},
+           "mean_task_us": times.mean(axis=0).tolist(), "queues": nq,
+           "wall_ms": wall, "seconds": time.perf_counter() - tic}
+    log(f"learned schedule: {res}")
+    return res
+
+
+def run_batched(args):
+    """64 independent water clusters (2-40 waters, seeded) with the 24-water
+    basis/grid recipe, each with its own P, evaluated in one context: every
+    task of every molecule in the same launches.  The autotuner and the cost
+    model are trained on the first 8 molecules; the full batch then runs as
+    persistent queues packed by the learned model, captured in a graph."""
+    import gauxc_amd as gx
+    from gauxc_amd import workloads
+    rng = np.random.default_rng(20)
+    nws = rng.integers(2, 41, size=args.n_mol)
+    mols = []
+    t0 = time.time()
+    cfg = dict(workloads.CONFIGS["c1"])
+    for m, nw in enumerate(nws):
+        box = 10.0 * (nw / 24.0) ** (1.0 / 3.0)  # the 24-water cluster's density
+        Z, xyz = workloads.water_cluster(int(nw), box, 100 + m)
+        l, pure, alpha, coeff, origin = workloads.even_tempered_basis(Z, xyz, cfg["recipe"])
+        mol = gx.Molecule(Z, xyz)
+        bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True,
+                         shell_tolerance=cfg["tol"])
+        P = workloads.density(bs.nbf(), max(1, min(bs.nbf(), 5 * int(nw))), 100 + m)
+        lb = workloads.build_tasks(mol, bs, grid=cfg["grid"])
+        gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+        t = lb.get_tasks()
+        order = np.argsort(-(t["npts"].astype(np.int64) * t["nbe"].astype(np.int64)),
+                           kind="stable")
+        mols.append((bs, P, reorder_tasks(t, order)))
+        if (m + 1) % 8 == 0:
+            log(f"c3 setup: {m + 1} of {len(nws)} molecules ({time.time() - t0:.1f}s)")
+    npts_total = int(sum(int(t["npts"].sum()) for _, _, t in mols))
+    ntasks = int(sum(len(t["npts"]) for _, _, t in mols))
+    log(f"c3 setup: {len(mols)} molecules, waters {int(nws.min())}-{int(nws.max())} "
+        f"(sum {int(nws.sum())}), {ntasks} tasks, {npts_total} points "
+        f"({time.time() - t0:.1f}s)")
+
+    def context(sel):
+        b = gx.BatchedDeviceContext(0)
+        b.upload_molecules([mols[i][0].xcq_shells() for i in sel])
+        b.upload_tasks([mols[i][2] for i in sel])
+        for k, i in enumerate(sel):
+            b.upload_density(k, mols[i][1])
+        return b
+
+    # Tuning and training on a subset.
+    from gauxc_amd.costmodel import MLPCostModel, task_features
+    sub = list(range(min(8, len(mols))))
+    sc = context(sub)
+    tuned = {p: sc.autotune("PBE", p) for p in ("fp64", "mixed")}
+    log(f"c3 autotune on {len(sub)} molecules: {tuned}")
+    sc.set_tile_policy(*tuned["mixed"][:2])
+    times = measure_task_times(sc, "mixed")
+    feats_sub = np.concatenate([task_features(mols[i][2], mols[i][0].xcq_shells()) for i in sub])
+    mlp = MLPCostModel(0, seed=2)
+    mlp.fit(feats_sub, times, epochs=300, batch=512, lr=1e-3)
+    sc.close()
+
+    full = context(range(len(mols)))
+    feats = np.concatenate([task_features(t, bs.xcq_shells()) for bs, _, t in mols])
+    pred = mlp.predict(feats)
+    nq = 4 * 256
+    out = {}
+    for prec in ("fp64", "mixed"):
+        full.set_tile_policy(*tuned[prec][:2])
+        full.use_graph(False)
+        full.set_schedule(0)
+        direct = time_eval(full, prec, reps=2)
+        full.set_schedule(nq, pred[:, 0], pred[:, 1])
+        full.use_graph(True)
+        for _ in range(args.warmup):
+            full.exc_vxc("PBE", prec, True)
+        full.synchronize()
+        tic = time.perf_counter()
+        for _ in range(args.steps):
+            full.exc_vxc("PBE", prec, True)
+        full.synchronize()
+        ms = (time.perf_counter() - tic) * 1e3 / args.steps
+        res = [full.results(i, want_vxc=True) for i in range(len(mols))]
+        out[prec] = dict(ms=ms, direct_ms=direct, res=res)
+        log(f"c3 {prec}: {ms:.1f} ms/step (queues {nq}, learned costs, graph); "
+            f"direct dispatch {direct:.1f} ms")
+        full.use_graph(False)
+    de = max(abs(a[0] - b[0]) for a, b in zip(out["mixed"]["res"], out["fp64"]["res"]))
+    dv = max(float(np.abs(a[2] - b[2]).max()) for a, b in zip(out["mixed"]["res"],
+                                                              out["fp64"]["res"]))
+    line = {"metric": "PBE EXC+VXC grid pts/s, batched molecules, mixed vs FP64 on 1 GPU",
+            "value": npts_total / (out["mixed"]["ms"] * 1e-3), "unit": "grid points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": out["mixed"]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16+f32+f64", "data": "synthetic",
+            "config": {"workload": f"{len(mols)} water clusters of {int(nws.min())}-"
+                                   f"{int(nws.max())} waters, PBE, 99x590 grid, "
+                                   f"{ntasks} tasks, {npts_total} points",
+                       "molecules": len(mols), "queues": nq, "graph": True,
+                       "autotuned_on_subset": {k: list(v) for k, v in tuned.items()}},
+            "fp64_ms_per_step": out["fp64"]["ms"],
+            "direct_dispatch_ms": {p: out[p]["direct_ms"] for p in out},
+            "speedup_mixed_vs_fp64": out["fp64"]["ms"] / out["mixed"]["ms"],
+            "max_dEXC_mixed_vs_fp64": de, "max_dVXC_mixed_vs_fp64": dv}
     print(json.dumps(line), flush=True)
 
 
