@@ -9,7 +9,9 @@ summary (calls, total and average duration) as markdown.
 the same two passes on ``bench.py --copy-calibration`` (three 1 GiB device
 copies), and writes per-kernel KiB per dispatch plus the factors that map
 each counter onto the copy's known byte count.  bench.py reads that file
-to report measured HBM bytes (``roofline.traffic``).
+to report measured HBM bytes (``roofline.traffic``).  A markdown table of
+the same numbers is written beside it (``OUT.md``); ``python -m
+gauxc_amd.profiling table OUT.json`` rewrites that table from the file.
 
 rocprofv3 runs as a child process with the program itself after ``--``;
 this module never touches the GPU.
@@ -85,7 +87,39 @@ def traffic(out_json, bench_args, work_dir):
            "kernels": kernels}
     with open(out_json, "w") as f:
         json.dump(doc, f, indent=1)
+    table(out_json)
     print(json.dumps({"fetch_scale": fetch_scale, "write_scale": write_scale}), flush=True)
+
+
+def table(in_json):
+    """Markdown rendering of a traffic summary: per-kernel counters per
+    dispatch and the scaled HBM bytes, written to the same path with .md."""
+    with open(in_json) as f:
+        doc = json.load(f)
+    fs, ws = doc["fetch_scale"], doc["write_scale"]
+    rows = []
+    for k, e in doc["kernels"].items():
+        gb = (e["fetch_kib"] * fs + e["write_kib"] * ws) * 1024 / 1e9
+        rows.append((gb, k, e))
+    rows.sort(key=lambda r: -r[0])
+    out_md = os.path.splitext(in_json)[0] + ".md"
+    name = os.path.basename(in_json)
+    with open(out_md, "w") as f:
+        f.write("# HBM traffic per kernel launch (rocprofv3 PMC)\n\n")
+        f.write("Commands (one pass per counter, then the same two counters on a plain copy):\n\n")
+        for c in ("command_fetch", "command_write", "command_copy"):
+            f.write("    " + doc[c] + "\n")
+        f.write(f"\n(driven by `python -m gauxc_amd.profiling traffic {name} -- <bench args>`; "
+                f"bench.py reads `{name}` for `roofline.traffic`).\n\n")
+        f.write(f"Copy calibration: three 1 GiB copies read {doc['copy_fetch_kib']:.0f} KiB by "
+                f"FETCH_SIZE and wrote {doc['copy_write_kib']:.0f} KiB by WRITE_SIZE, so "
+                f"FETCH_SIZE is scaled by {fs:.4f} and WRITE_SIZE by {ws:.4f}.\n\n")
+        f.write("| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
+                "| HBM GB/dispatch (scaled) |\n|---|---|---|---|---|\n")
+        for gb, k, e in rows:
+            f.write(f"| `{k}` | {e['dispatches']} | {e['fetch_kib']:.0f} | {e['write_kib']:.0f} "
+                    f"| {gb:.2f} |\n")
+    print(f"[profiling] wrote {out_md}", flush=True)
 
 
 def stats(out_md, bench_args, work_dir, timeout=300):
@@ -120,6 +154,9 @@ def stats(out_md, bench_args, work_dir, timeout=300):
 
 
 def main(argv):
+    if len(argv) == 2 and argv[0] == "table":
+        table(os.path.abspath(argv[1]))
+        return 0
     if len(argv) < 3 or argv[0] not in ("stats", "traffic") or "--" not in argv:
         print(__doc__)
         return 2
