@@ -38,8 +38,8 @@ STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symm
 STAGE_KERNELS = {
     ("collocation", "fp64"): ["k_collocation<true, double, double>"],
     ("collocation", "mixed"): ["k_collocation<true, double, float>"],
-    ("density", "fp64"): ["k_density_f64<32>", "k_density_f64<64>"],
-    ("density", "mixed"): ["k_density_mx<32>", "k_density_mx<64>"],
+    ("density", "fp64"): ["k_density_f64<32, true>", "k_density_f64<64, true>"],
+    ("density", "mixed"): ["k_density_mx<32, true>", "k_density_mx<64, true>"],
     ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
     ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32>", "k_syr2k_mx<64>"],
 }

@@ -157,7 +157,9 @@ struct xcq_context_s {
   double* d_vxc = nullptr;
   double* d_red = nullptr;
   double* d_out2 = nullptr;
-  int last_want_vxc = 0;
+  double* d_corr = nullptr;  // [nmol][kCorrBlocks] partials of the P32 correction
+  int last_want_vxc = 0;     // the last evaluation returns VXC
+  int last_vxc_eval = 0;     // the last evaluation computed VXC
 
   // chunk scratch
   void* d_scratch = nullptr;
@@ -217,6 +219,7 @@ struct xcq_context_s {
     dfree(d_Pf32);
     dfree(d_vxc);
     dfree(d_out2);
+    dfree(d_corr);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -368,15 +371,24 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     }
   }
   const int32_t nmol = (int32_t)ctx->mols.size();
+  // Mixed precision: EXC gets the first-order correction for the FP32
+  // rounding of P, fac * sum (P - P32) . VXC (run_pipeline always asks for
+  // VXC in that precision).
+  const bool corr = precision == XCQ_PREC_MIXED && want_vxc && func != 0;
   if (want_vxc) {
     XCQ_HIP(rec(ctx->ev[10]));
     if ((rc = launch_symmetrize(s, nmol, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off,
                                 ctx->d_vxc)))
       return rc;
+    if (corr && (rc = launch_p32_correction(s, nmol, ctx->d_mol_n, ctx->d_mol_off, ctx->d_P,
+                                            ctx->d_Pf32, ctx->d_vxc, ctx->d_corr)))
+      return rc;
     XCQ_HIP(rec(ctx->ev[11]));
-    ++launches;
+    launches += corr ? 2 : 1;
   }
-  if ((rc = launch_reduce_sums(s, nmol, ctx->d_mol_blk, ctx->d_red, ctx->d_out2))) return rc;
+  if ((rc = launch_reduce_sums(s, nmol, ctx->d_mol_blk, ctx->d_red,
+                               corr ? ctx->d_corr : nullptr, fac, ctx->d_out2)))
+    return rc;
   ctx->counters[0] = launches + 1;
   ctx->counters[1] = items;
   return XCQ_SUCCESS;
@@ -404,6 +416,10 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
     ctx->pf32_valid = true;
   }
   ctx->last_want_vxc = want_vxc;
+  // The mixed path's EXC correction needs VXC, so EXC-only requests in that
+  // precision evaluate it too (and do not return it).
+  if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
+  ctx->last_vxc_eval = want_vxc;
   if (!ctx->use_graph) {
     ctx->last_graph = false;
     return enqueue_pipeline(ctx, func, precision, want_vxc, true);
@@ -563,11 +579,13 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
     dfree(ctx->d_Pf32);
     dfree(ctx->d_vxc);
     dfree(ctx->d_out2);
+    dfree(ctx->d_corr);
     dfree(ctx->d_mol_n);
     dfree(ctx->d_mol_off);
     if ((rc = dalloc(&ctx->d_P, (size_t)p_total))) return rc;
     if ((rc = dalloc(&ctx->d_vxc, (size_t)p_total))) return rc;
     if ((rc = dalloc(&ctx->d_out2, (size_t)2 * nmol))) return rc;
+    if ((rc = dalloc(&ctx->d_corr, (size_t)kCorrBlocks * nmol))) return rc;
     if ((rc = dalloc(&ctx->d_mol_n, (size_t)nmol))) return rc;
     if ((rc = dalloc(&ctx->d_mol_off, (size_t)nmol))) return rc;
     std::vector<int32_t> mn(nmol);
@@ -1032,7 +1050,7 @@ int xcq_stage_times(xcq_context ctx, double* ms, int n) {
     else
       (void)hipGetLastError();
   } else {
-    const int nst = ctx->last_want_vxc ? 5 : 3;
+    const int nst = ctx->last_vxc_eval ? 5 : 3;
     for (size_t ci = 0; ci < ctx->chunks.size(); ++ci)
       for (int k = 0; k < nst; ++k) {
         float f = 0.f;
@@ -1040,7 +1058,7 @@ int xcq_stage_times(xcq_context ctx, double* ms, int n) {
         if (hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]) == hipSuccess) st[k] += f;
         else (void)hipGetLastError();
       }
-    if (ctx->last_want_vxc && !ctx->ev.empty()) {
+    if (ctx->last_vxc_eval && !ctx->ev.empty()) {
       float f = 0.f;
       if (hipEventElapsedTime(&f, ctx->ev[10], ctx->ev[11]) == hipSuccess) st[5] = f;
       else (void)hipGetLastError();

@@ -65,10 +65,10 @@ __device__ __forceinline__ const unsigned short* lo_plane(const T* Bm, int plane
 // per-point partials in LDS.  frow(r) is the tile row (function) of
 // accumulator register r of row block bi for this lane; the column (point)
 // of lane l in column block bj is wn*32 + bj*16 + (l & 15).
-template <typename T, int BI, typename FRow>
+template <typename T, int BI, bool GGA, typename FRow>
 __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow, int m0, int n0,
                                           int wm, int wn, int lane, int nbe, int ld, int plane,
-                                          const T* Bm, int gga, double (*dens)[64][4]) {
+                                          const T* Bm, double (*dens)[64][4]) {
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
     const int pl = wn * 32 + bj * 16 + (lane & 15);
@@ -85,7 +85,7 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
         const int o = min(j, nbe - 1) * ld + m;
         v0[r] = Bm[o];
         if constexpr (sizeof(T) == 4) l0[r] = lo[o];
-        if (gga) {
+        if constexpr (GGA) {
           v1[r] = Bm[plane + o];
           v2[r] = Bm[2 * plane + o];
           v3[r] = Bm[3 * plane + o];
@@ -99,7 +99,7 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
         double b = (double)v0[r];
         if constexpr (sizeof(T) == 4) b += bf16_val(l0[r]);
         s0 += b * xv;
-        if (gga) {
+        if constexpr (GGA) {
           s1 += (double)v1[r] * xv;
           s2 += (double)v2[r] * xv;
           s3 += (double)v3[r] * xv;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
 #pragma unroll
     for (int off = 16; off < 64; off <<= 1) {
       s0 += __shfl_xor(s0, off);
-      if (gga) {
+      if constexpr (GGA) {
         s1 += __shfl_xor(s1, off);
         s2 += __shfl_xor(s2, off);
         s3 += __shfl_xor(s3, off);
@@ -118,7 +118,7 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
     }
     if (lane < 16) {
       dens[wm][pl][0] += s0;
-      if (gga) {
+      if constexpr (GGA) {
         dens[wm][pl][1] += s1;
         dens[wm][pl][2] += s2;
         dens[wm][pl][3] += s3;
@@ -171,44 +171,68 @@ __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
   }
 }
 
-// Z = a B + c . grad B for the block's 64 points, every function row
-// (wave w handles rows w, w+4, ...; lane = point).
+// Four consecutive elements of a plane (16-byte aligned for FP32, 32 for FP64).
 template <typename T>
+using Vec4 = T __attribute__((ext_vector_type(4)));
+
+// Z = a B + c . grad B for the block's 64 points, every function row.
+// Thread (q = tid & 15, g = tid >> 4) owns points m0 + 4q .. 4q + 3 of rows
+// g, g + 16, ...  Each pass loads kZRows rows with 4-wide vector loads
+// before any of them is used, so every thread keeps up to 5 kZRows loads in
+// flight instead of waiting on each row in turn (4 rows of FP32, 2 of FP64,
+// which fit the register budget of the GEMM phase).  Row offsets are clamped
+// to a valid row; rows past nbe are not stored.
+template <typename T, bool GGA>
 __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, double (*zc)[64],
-                                        int gga, T* Bm) {
-  const int lane = tid & 63, wave = uni(tid >> 6);
+                                        T* Bm) {
+  constexpr int kZRows = sizeof(T) == 4 ? 4 : 2;
+  const int q = tid & 15, g = tid >> 4;
   const int ld = t.npts_pad, nbe = t.nbe, plane = nbe * ld;
-  const double a = zc[0][lane], cx = zc[1][lane], cy = zc[2][lane], cz = zc[3][lane];
-  T* B = Bm + m0 + lane;
-  T* Z = B + 4 * plane;
-  const unsigned short* lo = lo_plane(Bm, plane);
-  if constexpr (sizeof(T) == 4) lo += m0 + lane;
-  int j = wave;
-  for (; j + 4 < nbe; j += 8) {
-    const int o0 = j * ld, o1 = (j + 4) * ld;
-    double b0 = B[o0], b1 = B[o1];
-    if constexpr (sizeof(T) == 4) {
-      b0 += bf16_val(lo[o0]);
-      b1 += bf16_val(lo[o1]);
-    }
-    double z0 = a * b0, z1 = a * b1;
-    if (gga) {
-      const double x0 = B[plane + o0], y0 = B[2 * plane + o0], q0 = B[3 * plane + o0];
-      const double x1 = B[plane + o1], y1 = B[2 * plane + o1], q1 = B[3 * plane + o1];
-      z0 += cx * x0 + cy * y0 + cz * q0;
-      z1 += cx * x1 + cy * y1 + cz * q1;
-    }
-    Z[o0] = (T)z0;
-    Z[o1] = (T)z1;
+  double a[4], cx[4], cy[4], cz[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    a[e] = zc[0][4 * q + e];
+    cx[e] = zc[1][4 * q + e];
+    cy[e] = zc[2][4 * q + e];
+    cz[e] = zc[3][4 * q + e];
   }
-  if (j < nbe) {
-    const int o = j * ld;
-    double bv = B[o];
-    if constexpr (sizeof(T) == 4) bv += bf16_val(lo[o]);
-    double z = a * bv;
-    if (gga) z += cx * (double)B[plane + o] + cy * (double)B[2 * plane + o] +
-                  cz * (double)B[3 * plane + o];
-    Z[o] = (T)z;
+  T* B = Bm + m0 + 4 * q;
+  const unsigned short* lo = lo_plane(Bm, plane);
+  if constexpr (sizeof(T) == 4) lo += m0 + 4 * q;
+  for (int j0 = g; j0 < nbe; j0 += 16 * kZRows) {
+    Vec4<T> b[kZRows], x[kZRows], y[kZRows], z[kZRows];
+    uint2 l[kZRows];
+#pragma unroll
+    for (int u = 0; u < kZRows; ++u) {
+      const int o = min(j0 + 16 * u, nbe - 1) * ld;
+      b[u] = *reinterpret_cast<const Vec4<T>*>(B + o);
+      if constexpr (sizeof(T) == 4) l[u] = *reinterpret_cast<const uint2*>(lo + o);
+      if constexpr (GGA) {
+        x[u] = *reinterpret_cast<const Vec4<T>*>(B + plane + o);
+        y[u] = *reinterpret_cast<const Vec4<T>*>(B + 2 * plane + o);
+        z[u] = *reinterpret_cast<const Vec4<T>*>(B + 3 * plane + o);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kZRows; ++u) {
+      const int j = j0 + 16 * u;
+      if (j >= nbe) break;
+      Vec4<T> out;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double bv = (double)b[u][e];
+        if constexpr (sizeof(T) == 4) {
+          // bfloat16 residuals of points 2h, 2h + 1 are the low and high
+          // halves of word h.
+          const unsigned w = e < 2 ? l[u].x : l[u].y;
+          bv += (double)__builtin_bit_cast(float, (e & 1) ? (w & 0xffff0000u) : (w << 16));
+        }
+        double zv = a[e] * bv;
+        if constexpr (GGA) zv += cx[e] * (double)x[u][e] + cy[e] * (double)y[u][e] + cz[e] * (double)z[u][e];
+        out[e] = (T)zv;
+      }
+      *reinterpret_cast<Vec4<T>*>(B + 4 * plane + j * ld) = out;
+    }
   }
 }
 
@@ -220,11 +244,11 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
 // wn = point half of the 64-point block.  Loader roles: thread
 // (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of column c of the B^T
 // stage (point c) and, for c < NT, of the P_sub stage (function n0 + c).
-template <int NT>
+template <int NT, bool GGA>
 __global__ __launch_bounds__(256, 2) void k_density_f64(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const double* __restrict__ Pall, double* __restrict__ scratch,
-    const double* __restrict__ weights, int func, int gga, double fac, int want_z,
+    const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
   __shared__ double Pt[2][16][NT + 4];  // [buf][k][function]: A operand (P_sub)
@@ -310,15 +334,15 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
-      fold_tile<double, BI>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      fold_tile<double, BI, GGA>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
       // The next tile's first stage write targets buffer 0, last read at step
       // nks-2 (or nks-1 when nks == 1); all waves pass this barrier after it.
       __syncthreads();
     }
-    finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+    finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<double>(t, m0, tid, zc, gga, Bm);
+      write_z<double, GGA>(t, m0, tid, zc, Bm);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -338,11 +362,11 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 // plane; its bfloat16 residual enters the rho fold and Z.  P is gathered
 // from an FP32 copy.  Waves and loader roles as in the FP64 kernel, with K
 // rows 8g..8g+7 per loader thread.
-template <int NT>
+template <int NT, bool GGA>
 __global__ __launch_bounds__(256, 2) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const float* __restrict__ Pfall, float* __restrict__ scratch,
-    const double* __restrict__ weights, int func, int gga, double fac, int want_z,
+    const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
   __shared__ float Pt[2][32][NT + 4];  // [buf][k][function]: A operand (P_sub)
@@ -435,14 +459,14 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
       }
-      fold_tile<float, BI>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, gga, dens);
+      fold_tile<float, BI, GGA>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
       // The next tile's first stage write targets buffer 0 (see k_density_f64).
       __syncthreads();
     }
-    finish_points(t, wi.a, tid, dens, zc, weights, func, gga, red);
+    finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<float>(t, m0, tid, zc, gga, Bm);
+      write_z<float, GGA>(t, m0, tid, zc, Bm);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -454,22 +478,22 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, 
   if (r.n == 0) return XCQ_SUCCESS;
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
+  auto go = [&](auto kern, auto P, auto scr) {
+    hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, P, scr, v.w, func,
+                       fac, want_z, v.red);
+  };
   if (precision == XCQ_PREC_FP64) {
     double* scr = static_cast<double*>(v.scratch);
     if (nt == 32)
-      hipLaunchKernelGGL(k_density_f64<32>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P,
-                         scr, v.w, func, gga, fac, want_z, v.red);
+      gga ? go(k_density_f64<32, true>, v.P, scr) : go(k_density_f64<32, false>, v.P, scr);
     else
-      hipLaunchKernelGGL(k_density_f64<64>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P,
-                         scr, v.w, func, gga, fac, want_z, v.red);
+      gga ? go(k_density_f64<64, true>, v.P, scr) : go(k_density_f64<64, false>, v.P, scr);
   } else {
     float* scr = static_cast<float*>(v.scratch);
     if (nt == 32)
-      hipLaunchKernelGGL(k_density_mx<32>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
-                         v.Pf32, scr, v.w, func, gga, fac, want_z, v.red);
+      gga ? go(k_density_mx<32, true>, v.Pf32, scr) : go(k_density_mx<32, false>, v.Pf32, scr);
     else
-      hipLaunchKernelGGL(k_density_mx<64>, grid, block, 0, v.stream, qa, v.tasks, v.bfmap,
-                         v.Pf32, scr, v.w, func, gga, fac, want_z, v.red);
+      gga ? go(k_density_mx<64, true>, v.Pf32, scr) : go(k_density_mx<64, false>, v.Pf32, scr);
   }
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;

@@ -79,12 +79,54 @@ int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* r
   return XCQ_SUCCESS;
 }
 
+// First-order correction of EXC for the FP32 rounding of P in the mixed
+// path.  With rho = fac sum_ij P_ij phi_i phi_j, dEXC/dP_ij = fac VXC_ij
+// (VXC symmetrised), so
+//   EXC(P) = EXC(P32) + fac sum_ij (P - P32)_ij VXC_ij + O(|P - P32|^2),
+// where P32 is the FP32 copy the density stage multiplies with.  Workgroup
+// (b, m) sums elements b*256 + tid + k*kCorrBlocks*256 of molecule m's n x n
+// matrices and writes its partial to part[m * kCorrBlocks + b];
+// k_reduce_sums adds the partials of each molecule in a fixed order.
+__global__ __launch_bounds__(256) void k_p32_correction(const int32_t* __restrict__ mol_n,
+                                                        const int64_t* __restrict__ mol_off,
+                                                        const double* __restrict__ P,
+                                                        const float* __restrict__ Pf,
+                                                        const double* __restrict__ vxc,
+                                                        double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int m = blockIdx.y;
+  const int64_t n2 = (int64_t)mol_n[m] * mol_n[m];
+  const int64_t off = mol_off[m];
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += kCorrBlocks * 256)
+    s += (P[off + i] - (double)Pf[off + i]) * vxc[off + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)m * kCorrBlocks + blockIdx.x] = sh[0];
+}
+
+int launch_p32_correction(hipStream_t s, int32_t nmol, const int32_t* mol_n,
+                          const int64_t* mol_off, const double* P, const float* Pf,
+                          const double* vxc, double* part) {
+  if (nmol == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_p32_correction, dim3(kCorrBlocks, (unsigned)nmol), dim3(256), 0, s, mol_n,
+                     mol_off, P, Pf, vxc, part);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
 // Per-molecule sums of the (e, n) block pairs in a fixed order:
 // deterministic EXC and NEL.  Workgroup m reduces blocks
-// [blk_off[m], blk_off[m+1]).
+// [blk_off[m], blk_off[m+1]) and, when corr is given, adds
+// cfac * sum_b corr[m * kCorrBlocks + b] to EXC.
 __global__ __launch_bounds__(256) void k_reduce_sums(const int64_t* __restrict__ blk_off,
                                                      const double* __restrict__ red,
-                                                     double* __restrict__ out) {
+                                                     const double* __restrict__ corr,
+                                                     double cfac, double* __restrict__ out) {
   __shared__ double se[256], sn[256];
   const int m = blockIdx.x;
   double e = 0., n = 0.;
@@ -103,15 +145,19 @@ __global__ __launch_bounds__(256) void k_reduce_sums(const int64_t* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[2 * m] = se[0];
+    double c = 0.;
+    if (corr)
+      for (int b = 0; b < kCorrBlocks; ++b) c += corr[(int64_t)m * kCorrBlocks + b];
+    out[2 * m] = se[0] + cfac * c;
     out[2 * m + 1] = sn[0];
   }
 }
 
 int launch_reduce_sums(hipStream_t s, int32_t nmol, const int64_t* blk_off, const double* red,
-                       double* out2) {
+                       const double* corr, double cfac, double* out2) {
   if (nmol == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_reduce_sums, dim3((unsigned)nmol), dim3(256), 0, s, blk_off, red, out2);
+  hipLaunchKernelGGL(k_reduce_sums, dim3((unsigned)nmol), dim3(256), 0, s, blk_off, red, corr,
+                     cfac, out2);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -209,6 +209,37 @@ def test_device_mixed_matches_fp64(gpu, gx, orc, cfgname, n_water, grid, func):
         assert np.abs(vmx - vxc_o).max() <= 1e-7
 
 
+def test_mixed_p32_correction(gpu, gx):
+    """The mixed path multiplies with an FP32 copy P32 of P and corrects EXC
+    to first order, EXC(P) = EXC(P32) + 2 sum (P - P32) . VXC.  P32 + d with
+    |d| a quarter ulp of P32 has the same FP32 copy, so the mixed EXC
+    difference between P32 + d and P32 comes from the correction alone; it
+    must match the FP64 path's difference (a first-order quantity, second
+    order ~1e-15 here).  EXC-only requests give the same corrected EXC."""
+    _flush("mixed P32 correction c1 n_water=4 PBE")
+    mol, bs, P, cfg = gx.workloads.make_config("c1", seed=0, n_water=4)
+    lb = gx.workloads.build_tasks(mol, bs, grid="UltraFineGrid")
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    t = lb.get_tasks()
+    P32 = np.asfortranarray(P.astype(np.float32).astype(np.float64))
+    Pd = np.asfortranarray(P32 * (1.0 + 2.0 ** -26))
+    assert np.array_equal(Pd.astype(np.float32), P32.astype(np.float32))
+    ctx = _context_for(gx, bs, t, P32)
+    e = {}
+    for name, mat in (("base", P32), ("pert", Pd)):
+        ctx.upload_density(mat)
+        for prec in ("fp64", "mixed"):
+            ctx.exc_vxc("PBE", prec, True)
+            e[name, prec] = ctx.results(want_vxc=False)[0]
+        ctx.exc_vxc("PBE", "mixed", False)
+        assert ctx.results(want_vxc=False)[0] == e[name, "mixed"]
+    d64 = e["pert", "fp64"] - e["base", "fp64"]
+    dmx = e["pert", "mixed"] - e["base", "mixed"]
+    print(f"dEXC fp64 {d64:.6e} mixed {dmx:.6e}", flush=True)
+    assert abs(d64) > 1e-8
+    assert abs(dmx - d64) <= 1e-3 * abs(d64) + 1e-12
+
+
 def test_mixed_through_gauxc_api(gpu, gx):
     """The "Scheme1-Mixed" local work driver through the C API on GauXC's
     benzene PBE0 fixture."""
