@@ -36,8 +36,8 @@ PEAK_HBM_GBS = 8000.0
 STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symmetrize"]
 # Kernels of each stage (the names rocprofv3 reports).
 STAGE_KERNELS = {
-    ("collocation", "fp64"): ["k_collocation<true, double, double>"],
-    ("collocation", "mixed"): ["k_collocation<true, double, float>"],
+    ("collocation", "fp64"): ["k_collocation<true, double, 3>"],
+    ("collocation", "mixed"): ["k_collocation<true, float, 3>"],
     ("density", "fp64"): ["k_density_f64<32, true>", "k_density_f64<64, true>"],
     ("density", "mixed"): ["k_density_mx<32, true>", "k_density_mx<64, true>"],
     ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],

@@ -19,17 +19,37 @@
 
 namespace xcq {
 
-__constant__ double c_sph[kMaxL + 1][2 * kMaxL + 1][(kMaxL + 1) * (kMaxL + 2) / 2];
+// ---- real solid harmonics, tabulated at compile time --------------------
+// Closed form of Helgaker, Jorgensen & Olsen, "Molecular Electronic-Structure
+// Theory", eq. 6.4.47-6.4.50:
+//   S_lm = N_lm sum_{t,u,v} C^{lm}_{tuv} x^{2t+|m|-2(u+v)} y^{2(u+v)} z^{l-2t-|m|},
+//   C = (-1)^{t+v-v_m} (1/4)^t binom(l,t) binom(l-t,|m|+t) binom(t,u) binom(|m|,2v),
+//   N_lm = sqrt(2 (l+|m|)! (l-|m|)! / 2^{delta_m0}) / (2^|m| l!),
+// v_m = 0 (m >= 0) or 1/2 (m < 0), v running in steps of 1 from v_m.  The
+// table is a constant expression, so the unrolled transforms below multiply
+// by immediate coefficients and skip the zero ones.
+constexpr int kNCart = (kMaxL + 1) * (kMaxL + 2) / 2;
 
-static double h_fact(int n) {
+constexpr double cx_fact(int n) {
   double r = 1.;
   for (int k = 2; k <= n; ++k) r *= k;
   return r;
 }
-static double h_binom(int n, int k) {
-  return (k < 0 || k > n) ? 0. : h_fact(n) / (h_fact(k) * h_fact(n - k));
+constexpr double cx_binom(int n, int k) {
+  return (k < 0 || k > n) ? 0. : cx_fact(n) / (cx_fact(k) * cx_fact(n - k));
 }
-static int cart_index(int l, int a, int b) {  // CCA position of x^a y^b z^(l-a-b)
+// Newton's iteration from above; agrees with a correctly rounded square
+// root to within one unit in the last place.
+constexpr double cx_sqrt(double x) {
+  double r = x > 1. ? x : 1.;
+  for (int i = 0; i < 200; ++i) {
+    const double n = 0.5 * (r + x / r);
+    if (!(n < r)) break;
+    r = n;
+  }
+  return r;
+}
+constexpr int cx_cart_index(int l, int a, int b) {  // CCA position of x^a y^b z^(l-a-b)
   int idx = 0;
   for (int aa = l; aa >= 0; --aa)
     for (int bb = l - aa; bb >= 0; --bb, ++idx)
@@ -37,37 +57,39 @@ static int cart_index(int l, int a, int b) {  // CCA position of x^a y^b z^(l-a-
   return -1;
 }
 
-// Real solid harmonics from the closed form of Helgaker, Jorgensen & Olsen,
-// "Molecular Electronic-Structure Theory", eq. 6.4.47-6.4.50:
-//   S_lm = N_lm sum_{t,u,v} C^{lm}_{tuv} x^{2t+|m|-2(u+v)} y^{2(u+v)} z^{l-2t-|m|},
-//   C = (-1)^{t+v-v_m} (1/4)^t binom(l,t) binom(l-t,|m|+t) binom(t,u) binom(|m|,2v),
-//   N_lm = sqrt(2 (l+|m|)! (l-|m|)! / 2^{delta_m0}) / (2^|m| l!),
-// v_m = 0 (m >= 0) or 1/2 (m < 0), v running in steps of 1 from v_m.
-int upload_sph_table(hipStream_t s) {
-  static double tbl[kMaxL + 1][2 * kMaxL + 1][(kMaxL + 1) * (kMaxL + 2) / 2];
-  for (auto& a : tbl)
-    for (auto& b : a)
-      for (auto& c : b) c = 0.;
+struct SphTable {
+  double c[kMaxL + 1][2 * kMaxL + 1][kNCart];
+};
+
+constexpr SphTable make_sph_table() {
+  SphTable t{};
   for (int l = 0; l <= kMaxL; ++l)
     for (int m = -l; m <= l; ++m) {
       const int am = m < 0 ? -m : m;
-      const double N = std::sqrt(2. * h_fact(l + am) * h_fact(l - am) / (m == 0 ? 2. : 1.)) /
-                       (std::ldexp(1.0, am) * h_fact(l));
+      double pow2 = 1.;
+      for (int k = 0; k < am; ++k) pow2 *= 2.;
+      const double N =
+          cx_sqrt(2. * cx_fact(l + am) * cx_fact(l - am) / (m == 0 ? 2. : 1.)) / (pow2 * cx_fact(l));
       const int v2s = m < 0 ? 1 : 0;  // 2v starts at 0 (m >= 0) or 1 (m < 0)
-      for (int t = 0; t <= (l - am) / 2; ++t)
-        for (int u = 0; u <= t; ++u)
+      for (int tt = 0; tt <= (l - am) / 2; ++tt) {
+        double q = 1.;
+        for (int k = 0; k < tt; ++k) q *= 0.25;
+        for (int u = 0; u <= tt; ++u)
           for (int v2 = v2s; v2 <= am; v2 += 2) {
-            const int se = t + (v2 - v2s) / 2;
-            const double C = ((se & 1) ? -1. : 1.) * std::pow(0.25, t) * h_binom(l, t) *
-                             h_binom(l - t, am + t) * h_binom(t, u) * h_binom(am, v2);
-            const int ax = 2 * t + am - 2 * u - v2, ay = 2 * u + v2;
-            tbl[l][m + l][cart_index(l, ax, ay)] += N * C;
+            const int se = tt + (v2 - v2s) / 2;
+            const double C = ((se & 1) ? -1. : 1.) * q * cx_binom(l, tt) *
+                             cx_binom(l - tt, am + tt) * cx_binom(tt, u) * cx_binom(am, v2);
+            const int ax = 2 * tt + am - 2 * u - v2, ay = 2 * u + v2;
+            t.c[l][m + l][cx_cart_index(l, ax, ay)] += N * C;
           }
+      }
     }
-  XCQ_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sph), tbl, sizeof(tbl), 0,
-                                 hipMemcpyHostToDevice, s));
-  return XCQ_SUCCESS;
+  return t;
 }
+
+__device__ constexpr SphTable kSph = make_sph_table();
+
+int upload_sph_table(hipStream_t) { return XCQ_SUCCESS; }
 
 // bfloat16 (round to nearest even) of the part of v that its FP32 rounding
 // drops: with it the stored value plane carries ~32 significant bits.
@@ -76,83 +98,121 @@ __device__ __forceinline__ unsigned short resid_bf16(double v) {
   return __builtin_bit_cast(unsigned short, (__bf16)(float)(v - (double)hi));
 }
 
-// Values (and gradients) of one shell at one point, written to the scratch.
-// T is the storage type (double or float); arithmetic is in double for the
-// FP64 kernel and float for the FP32 kernel (type C).  With FP32 storage the
-// value plane also gets its bfloat16 residual (lo, same indexing as out).
-template <int L, bool GRAD, typename C, typename T>
-__device__ __forceinline__ void shell_eval(C x, C y, C z, C R, C dR, bool pure, bool valid,
-                                           T* out, unsigned short* lo, int64_t ld,
-                                           int64_t plane) {
-  constexpr int NC = (L + 1) * (L + 2) / 2;
-  C xp[L + 1], yp[L + 1], zp[L + 1];
-  xp[0] = yp[0] = zp[0] = C(1);
+// Values (and gradients) of one shell at one point, written to the scratch
+// in double arithmetic.  T is the storage type (double or float); with FP32
+// storage the value plane also gets its bfloat16 residual (lo, same
+// indexing as out).  Spherical shells accumulate their 2L+1 outputs while
+// the Cartesian monomials are generated, so no Cartesian array is kept.
+// Stores go through a buffer resource over the task's scratch: the byte
+// offset of a row is a scalar (row0 = offset of the shell's first function
+// at the block's first point), the lane offset one shared register.
+typedef unsigned U32x2 __attribute__((ext_vector_type(2)));
+
+struct TaskStore {
+  __amdgpu_buffer_rsrc_t rs;
+  int vo4, vo2;       // lane byte offsets for 4-/8-byte and 2-byte elements
+  int lo_base;        // byte offset of the residual plane
+};
+
+template <typename T>
+__device__ __forceinline__ void store_elem(const TaskStore& ts, int64_t idx, T v) {
+  const int so = (int)(idx * (int64_t)sizeof(T));
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), ts.rs, ts.vo4, so, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U32x2, v), ts.rs, ts.vo4 * 2, so,
+                                          0);
+}
+
+template <int L, bool GRAD, typename T>
+__device__ __forceinline__ void shell_eval(double x, double y, double z, double R, double dR,
+                                           bool pure, bool valid, const TaskStore& ts,
+                                           int64_t row0, int64_t ld, int64_t plane) {
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.;
 #pragma unroll
   for (int k = 1; k <= L; ++k) {
     xp[k] = xp[k - 1] * x;
     yp[k] = yp[k - 1] * y;
     zp[k] = zp[k - 1] * z;
   }
-  C cv[NC], cx[NC], cy[NC], cz[NC];
-  const C Rx = dR * x, Ry = dR * y, Rz = dR * z;
-  int ic = 0;
-#pragma unroll
-  for (int a = L; a >= 0; --a) {
-#pragma unroll
-    for (int b = L - a; b >= 0; --b) {
-      const int c = L - a - b;
-      const C mono = xp[a] * yp[b] * zp[c];
-      cv[ic] = mono * R;
-      if (GRAD) {
-        const C dxm = a ? C(a) * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] : C(0);
-        const C dym = b ? C(b) * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] : C(0);
-        const C dzm = c ? C(c) * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] : C(0);
-        cx[ic] = dxm * R + mono * Rx;
-        cy[ic] = dym * R + mono * Ry;
-        cz[ic] = dzm * R + mono * Rz;
-      }
-      ++ic;
+  const double Rx = dR * x, Ry = dR * y, Rz = dR * z;
+  auto put = [&](int f, double v, double gx, double gy, double gz) {
+    const int64_t r = row0 + f * ld;
+    store_elem<T>(ts, r, valid ? T(v) : T(0));
+    if constexpr (sizeof(T) == 4)
+      __builtin_amdgcn_raw_buffer_store_b16(
+          (short)(valid ? resid_bf16(v) : (unsigned short)0), ts.rs, ts.vo2,
+          ts.lo_base + (int)(2 * r), 0);
+    if constexpr (GRAD) {
+      store_elem<T>(ts, plane + r, valid ? T(gx) : T(0));
+      store_elem<T>(ts, 2 * plane + r, valid ? T(gy) : T(0));
+      store_elem<T>(ts, 3 * plane + r, valid ? T(gz) : T(0));
     }
-  }
+  };
+  // Cartesian term ic = (a, b, c) in CCA order: value and gradient.
+  auto term = [&](int a, int b, double& cv, double& gx, double& gy, double& gz) {
+    const int c = L - a - b;
+    const double mono = xp[a] * yp[b] * zp[c];
+    cv = mono * R;
+    if constexpr (GRAD) {
+      const double dxm = a ? double(a) * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] : 0.;
+      const double dym = b ? double(b) * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] : 0.;
+      const double dzm = c ? double(c) * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] : 0.;
+      gx = dxm * R + mono * Rx;
+      gy = dym * R + mono * Ry;
+      gz = dzm * R + mono * Rz;
+    }
+  };
   if (pure) {
+    double v[2 * L + 1], wx[2 * L + 1], wy[2 * L + 1], wz[2 * L + 1];
 #pragma unroll
-    for (int m = 0; m < 2 * L + 1; ++m) {
-      C v = 0, gx = 0, gy = 0, gz = 0;
+    for (int m = 0; m < 2 * L + 1; ++m) v[m] = wx[m] = wy[m] = wz[m] = 0.;
+    int ic = 0;
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const C cf = C(c_sph[L][m][k]);
-        v += cf * cv[k];
-        if (GRAD) {
-          gx += cf * cx[k];
-          gy += cf * cy[k];
-          gz += cf * cz[k];
+    for (int a = L; a >= 0; --a) {
+#pragma unroll
+      for (int b = L - a; b >= 0; --b) {
+        double cv, gx = 0., gy = 0., gz = 0.;
+        term(a, b, cv, gx, gy, gz);
+#pragma unroll
+        for (int m = 0; m < 2 * L + 1; ++m) {
+          const double cf = kSph.c[L][m][ic];
+          if (cf != 0.) {
+            v[m] += cf * cv;
+            if constexpr (GRAD) {
+              wx[m] += cf * gx;
+              wy[m] += cf * gy;
+              wz[m] += cf * gz;
+            }
+          }
         }
-      }
-      out[m * ld] = valid ? T(v) : T(0);
-      if constexpr (sizeof(T) == 4) lo[m * ld] = valid ? resid_bf16(v) : (unsigned short)0;
-      if (GRAD) {
-        out[plane + m * ld] = valid ? T(gx) : T(0);
-        out[2 * plane + m * ld] = valid ? T(gy) : T(0);
-        out[3 * plane + m * ld] = valid ? T(gz) : T(0);
+        ++ic;
       }
     }
-  } else {
-    // Cartesian functions in CCA order.
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      out[k * ld] = valid ? T(cv[k]) : T(0);
-      if constexpr (sizeof(T) == 4) lo[k * ld] = valid ? resid_bf16(cv[k]) : (unsigned short)0;
-      if (GRAD) {
-        out[plane + k * ld] = valid ? T(cx[k]) : T(0);
-        out[2 * plane + k * ld] = valid ? T(cy[k]) : T(0);
-        out[3 * plane + k * ld] = valid ? T(cz[k]) : T(0);
+    for (int m = 0; m < 2 * L + 1; ++m) put(m, v[m], wx[m], wy[m], wz[m]);
+  } else {
+    int ic = 0;
+#pragma unroll
+    for (int a = L; a >= 0; --a) {
+#pragma unroll
+      for (int b = L - a; b >= 0; --b) {
+        double cv, gx = 0., gy = 0., gz = 0.;
+        term(a, b, cv, gx, gy, gz);
+        put(ic, cv, gx, gy, gz);
+        ++ic;
       }
     }
   }
 }
 
-template <bool GRAD, typename C, typename T>
-__global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict__ blocks,
+// Occupancy target by the highest angular momentum a launch evaluates:
+// spherical f (LMAX 3) with gradients fits 168 registers, g (LMAX 4) 256.
+constexpr int coll_waves(int lmax) { return lmax <= 2 ? 4 : (lmax == 3 ? 3 : 2); }
+
+template <bool GRAD, typename T, int LMAX>
+__global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const WorkItem* __restrict__ blocks,
                                                      const DevTask* __restrict__ tasks,
                                                      const DevShell* __restrict__ shells,
                                                      const int32_t* __restrict__ shell_list,
@@ -163,9 +223,9 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
                                                      T* __restrict__ scratch) {
   const WorkItem wi = blocks[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int lane = threadIdx.x & 63;
+  const unsigned lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int p = wi.a * kPtBlock + lane;
+  const int p = wi.a * kPtBlock + (int)lane;
   const bool valid = p < t.npts;
   double gx = 0., gy = 0., gz = 0.;
   if (valid) {
@@ -176,10 +236,15 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
   const int64_t boff = sizeof(T) == 4 ? t.b_off_mx : t.b_off;
-  T* base = scratch + boff + p;
+  const int64_t m0 = (int64_t)wi.a * kPtBlock;
   // FP32 storage: bfloat16 residual plane of the values after the five
-  // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).
-  unsigned short* lo_base = reinterpret_cast<unsigned short*>(scratch + boff + 5 * plane) + p;
+  // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).  Offsets within the task's
+  // region stay below 2^31 bytes (checked when the chunks are planned).
+  TaskStore ts;
+  ts.rs = __builtin_amdgcn_make_buffer_rsrc(scratch + boff, 0, -1, 0x00020000);
+  ts.vo4 = (int)(lane * 4u);
+  ts.vo2 = (int)(lane * 2u);
+  ts.lo_base = (int)(5 * plane * (int64_t)sizeof(T));
   for (int s = wave; s < t.nshells; s += 4) {
     const int ish = shell_list[t.sh_off + s];
     const int bf0 = lsh_first[t.lsh_off + s];
@@ -187,62 +252,72 @@ __global__ __launch_bounds__(256) void k_collocation(const WorkItem* __restrict_
     const int l = sh->l;
     const int nprim = sh->nprim;
     const bool pure = sh->pure != 0;
-    // Relative coordinates in double, then evaluated in C.
-    const C x = C(gx - sh->O[0]), y = C(gy - sh->O[1]), z = C(gz - sh->O[2]);
-    const C r2 = x * x + y * y + z * z;
-    C R = 0, dR = 0;
+    const double x = gx - sh->O[0], y = gy - sh->O[1], z = gz - sh->O[2];
+    const double r2 = x * x + y * y + z * z;
+    double R = 0., dR = 0.;
     for (int k = 0; k < nprim; ++k) {
-      C ex;
-      if constexpr (sizeof(C) == 4) ex = expf(-C(sh->alpha[k]) * r2);
-      else ex = exp(-C(sh->alpha[k]) * r2);
-      const C e = C(sh->coeff[k]) * ex;
+      const double e = sh->coeff[k] * exp(-sh->alpha[k] * r2);
       R += e;
-      dR += C(-2) * C(sh->alpha[k]) * e;
+      dR += -2. * sh->alpha[k] * e;
     }
-    T* out = base + (int64_t)bf0 * ld;
-    unsigned short* lo = lo_base + (int64_t)bf0 * ld;
-    switch (l) {
-      case 0: shell_eval<0, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
-      case 1: shell_eval<1, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
-      case 2: shell_eval<2, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
-      case 3: shell_eval<3, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
-      default: shell_eval<4, GRAD, C, T>(x, y, z, R, dR, pure, valid, out, lo, ld, plane); break;
-    }
+    const int64_t row0 = (int64_t)bf0 * ld + m0;
+    if (l == 0) shell_eval<0, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+    else if (l == 1) shell_eval<1, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+    else if (LMAX == 2 || l == 2)
+      shell_eval<2, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+    else if (LMAX == 3 || l == 3)
+      shell_eval<(LMAX < 3 ? 2 : 3), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+    else
+      shell_eval<(LMAX < 4 ? 2 : 4), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
   }
+}
+
+// One block per (task, 64-point block) item; lmax is the highest angular
+// momentum among the basis shells (the kernel evaluates l <= max(lmax, 2)).
+template <typename T>
+static int launch_collocation(hipStream_t s, const WorkItem* items, int64_t n,
+                              const DevTask* tasks, const DevShell* shells,
+                              const int32_t* shell_list, const int32_t* lsh_first,
+                              const double* px, const double* py, const double* pz, T* scratch,
+                              int deriv, int lmax) {
+  if (n == 0) return XCQ_SUCCESS;
+  XCQ_REQUIRE(lmax >= 0 && lmax <= kMaxL, XCQ_ERROR_INVALID_ARGUMENT,
+              "collocation: angular momentum out of range");
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(256), 0, s, items, tasks, shells,
+                       shell_list, lsh_first, px, py, pz, scratch);
+  };
+  if (deriv) {
+    if (lmax <= 2) go(k_collocation<true, T, 2>);
+    else if (lmax == 3) go(k_collocation<true, T, 3>);
+    else go(k_collocation<true, T, 4>);
+  } else {
+    if (lmax <= 2) go(k_collocation<false, T, 2>);
+    else if (lmax == 3) go(k_collocation<false, T, 3>);
+    else go(k_collocation<false, T, 4>);
+  }
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
 }
 
 int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first,
                             const double* px, const double* py, const double* pz,
-                            double* scratch, int deriv) {
-  if (n == 0) return XCQ_SUCCESS;
-  if (deriv)
-    hipLaunchKernelGGL((k_collocation<true, double, double>), dim3((unsigned)n), dim3(256), 0, s,
-                       items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
-  else
-    hipLaunchKernelGGL((k_collocation<false, double, double>), dim3((unsigned)n), dim3(256), 0,
-                       s, items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
-  XCQ_HIP(hipGetLastError());
-  return XCQ_SUCCESS;
+                            double* scratch, int deriv, int lmax) {
+  return launch_collocation<double>(s, items, n, tasks, shells, shell_list, lsh_first, px, py,
+                                    pz, scratch, deriv, lmax);
 }
 
-// Mixed precision: values evaluated in FP64, stored as FP32 (half the HBM
-// traffic of the B planes; storage rounding is the only FP32 step).
+// Mixed precision: values evaluated in FP64, stored as FP32 plus the value
+// plane's bfloat16 residual.
 int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first,
                             const double* px, const double* py, const double* pz,
-                            float* scratch, int deriv) {
-  if (n == 0) return XCQ_SUCCESS;
-  if (deriv)
-    hipLaunchKernelGGL((k_collocation<true, double, float>), dim3((unsigned)n), dim3(256), 0, s,
-                       items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
-  else
-    hipLaunchKernelGGL((k_collocation<false, double, float>), dim3((unsigned)n), dim3(256), 0,
-                       s, items, tasks, shells, shell_list, lsh_first, px, py, pz, scratch);
-  XCQ_HIP(hipGetLastError());
-  return XCQ_SUCCESS;
+                            float* scratch, int deriv, int lmax) {
+  return launch_collocation<float>(s, items, n, tasks, shells, shell_list, lsh_first, px, py,
+                                   pz, scratch, deriv, lmax);
 }
 
 }  // namespace xcq

@@ -125,6 +125,7 @@ struct xcq_context_s {
   std::vector<int32_t> sh_size, sh_first, sh_mol;  // sh_first: molecule-local
   std::vector<MolInfo> mols;
   int32_t max_nbf = 0;
+  int32_t max_l = 0;  // highest shell angular momentum
   int64_t p_total = 0;
   int32_t* d_mol_n = nullptr;
   int64_t* d_mol_off = nullptr;
@@ -340,10 +341,10 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     const WorkItem* cw = v.work + c.coll.off;
     if (precision == XCQ_PREC_FP64)
       rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
-                                   v.py, v.pz, static_cast<double*>(v.scratch), gga);
+                                   v.py, v.pz, static_cast<double*>(v.scratch), gga, ctx->max_l);
     else
       rc = launch_collocation_fp32(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
-                                   v.py, v.pz, static_cast<float*>(v.scratch), gga);
+                                   v.py, v.pz, static_cast<float*>(v.scratch), gga, ctx->max_l);
     if (rc) return rc;
     XCQ_HIP(rec(ev[1]));
     // X, rho/grad rho, functional, EXC/NEL and Z in one launch per tile
@@ -530,7 +531,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
   ctx->sh_mol.assign(nshells, 0);
   int32_t i = 0;
   int64_t p_total = 0;
-  int32_t max_nbf = 0;
+  int32_t max_nbf = 0, max_l = 0;
   for (int32_t m = 0; m < nmol; ++m) {
     mols[m].sh_begin = i;
     mols[m].nshells = mol_nshells[m];
@@ -552,6 +553,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
       d.l = s.l;
       d.pure = s.pure;
       d.size = s.pure ? 2 * s.l + 1 : (s.l + 1) * (s.l + 2) / 2;
+      max_l = std::max(max_l, s.l);
       ctx->sh_size[i] = d.size;
       ctx->sh_first[i] = nbf;
       ctx->sh_mol[i] = m;
@@ -602,6 +604,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
   ctx->mols = std::move(mols);
   ctx->p_total = p_total;
   ctx->max_nbf = max_nbf;
+  ctx->max_l = max_l;
   return XCQ_SUCCESS;
 }
 
@@ -746,6 +749,7 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
   if ((rc = dalloc(&ctx->d_pz, npts_total))) return rc;
   if ((rc = dalloc(&ctx->d_w, npts_total))) return rc;
   if ((rc = dalloc(&ctx->d_shell_list, (size_t)nshell_entries))) return rc;
+  bfmap.resize(bfmap.size() + kBfPad, 0);
   if ((rc = dalloc(&ctx->d_bfmap, bfmap.size()))) return rc;
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
@@ -1243,7 +1247,7 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
       double* scr = static_cast<double*>(ctx->d_scratch);
       rc = launch_collocation_fp64(ctx->stream, d_items, nmb, ctx->d_tasks, ctx->d_shells,
                                    ctx->d_shell_list, ctx->d_lsh, ctx->d_px, ctx->d_py,
-                                   ctx->d_pz, scr, deriv);
+                                   ctx->d_pz, scr, deriv, ctx->max_l);
       if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
       if (rc == XCQ_SUCCESS && e == hipSuccess)
         e = hipMemcpy(host.data(), scr + d.b_off, 8 * host.size(), hipMemcpyDeviceToHost);
@@ -1252,7 +1256,7 @@ int xcq_task_collocation(xcq_context ctx, int32_t task, int deriv, int precision
       float* scr = static_cast<float*>(ctx->d_scratch);
       rc = launch_collocation_fp32(ctx->stream, d_items, nmb, ctx->d_tasks, ctx->d_shells,
                                    ctx->d_shell_list, ctx->d_lsh, ctx->d_px, ctx->d_py,
-                                   ctx->d_pz, scr, deriv);
+                                   ctx->d_pz, scr, deriv, ctx->max_l);
       if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
       std::vector<float> hf(host.size());
       if (rc == XCQ_SUCCESS && e == hipSuccess)

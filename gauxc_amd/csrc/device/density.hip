@@ -360,8 +360,52 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 // product of positive terms spanning 20 binades, -0.004 ulp for the FP32
 // MFMA on the same data).  The B^T operand is the FP32 part of the value
 // plane; its bfloat16 residual enters the rho fold and Z.  P is gathered
-// from an FP32 copy.  Waves and loader roles as in the FP64 kernel, with K
-// rows 8g..8g+7 per loader thread.
+// from an FP32 copy; the factor fac (a power of two) scales the
+// accumulated X, which is exact.
+//
+// Operand staging goes straight from global memory to LDS
+// (global_load_lds_dword: lane i of a wave writes dword i of a 256-byte LDS
+// row), through a ring of kStages K blocks, so kStages - 1 blocks of loads
+// are in flight while the matrix cores work on the current one, and no
+// register holds staged data.  Thread (lane c, wave g) issues, per K block,
+// the eight B^T rows 8g..8g+7 (lane = point) and the eight P_sub rows
+// (lane = function c < NT).  P rows past nbe and functions past nbe are
+// read from a zero row, B^T rows past nbe at a clamped valid row (finite,
+// multiplied by those zero P rows).
+constexpr int kStages = 3;
+constexpr int kLoadsPerStage = 16;  // global_load_lds instructions per thread per K block
+
+__device__ float g_zero_row[64];  // never written: zero-initialised device memory
+
+// Waits until at most n of this wave's vector memory operations are
+// outstanding, then joins the workgroup barrier.  The LDS rows written by
+// global_load_lds are complete once the issuing wave's counter has passed
+// them (vector memory operations complete in order), so after the barrier
+// every wave's loads of the awaited block are in LDS.  One asm statement
+// with a memory clobber keeps the compiler from moving LDS accesses across
+// it.
+__device__ __forceinline__ void wait_loads_barrier(int n) {
+  switch (n) {
+    case 0: asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); break;
+    case kLoadsPerStage:
+      asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
+      break;
+    default: asm volatile("s_waitcnt vmcnt(32)\n\ts_barrier" ::: "memory"); break;
+  }
+}
+static_assert(kLoadsPerStage == 16 && kStages == 3, "wait_loads_barrier covers two blocks ahead");
+
+// One dword per lane from global memory into the LDS row at lds_row (lane i
+// writes lds_row[i]); lds_row must be the same for the whole wave.  Issued
+// as inline asm so that the compiler's own wait insertion, which cannot tell
+// the ring slots apart, adds no wait for these loads before LDS reads; the
+// ring waits above cover them.  The low 32 bits of a generic pointer into
+// LDS are the LDS byte offset that M0 takes.
+__device__ __forceinline__ void load_lds(const float* src, float* lds_row) {
+  const unsigned off = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)lds_row);
+  asm volatile("global_load_lds_dword %0, off" ::"v"(src), "{m0}"(off) : "memory");
+}
+
 template <int NT, bool GGA>
 __global__ __launch_bounds__(256, 2) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
@@ -369,8 +413,8 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
     const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
-  __shared__ float Pt[2][32][NT + 4];  // [buf][k][function]: A operand (P_sub)
-  __shared__ float Bt[2][32][68];      // [buf][k][point]:    B operand (B^T)
+  __shared__ float Pt[kStages][32][NT + 4];  // [slot][k][function]: A operand (P_sub)
+  __shared__ float Bt[kStages][32][68];      // [slot][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
@@ -391,6 +435,9 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
     float* Bm = scratch + t.b_off_mx;
     const int32_t* bf = bfmap + t.bf_off;
     for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    // Stores of the previous item may still count against vmcnt; the ring
+    // waits below assume only this item's staging loads are outstanding.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const float* bcol = Bm + m0 + lc;
     const int nks = (nbe + 31) / 32;
@@ -401,20 +448,24 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
       const int ncol = n0 + lc;
       const bool colok = lc < NT && ncol < nbe;
       const float* Pc = Pf + bf[min(ncol, nbe - 1)];
-      // Raw prefetch (masks and factor applied when staged; see k_density_f64).
-      float rb[8], rp[8];
-      auto load = [&](int k0) {
+      auto issue = [&](int s) {
+        const int slot = s % kStages, k0 = 32 * s + 8 * lg;
+        // Eight consecutive map entries (uniform: one scalar load); entries
+        // past nbe lie in the map's zero padding and are not used.
         int rowp[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) rowp[e] = bf[min(k0 + 8 * lg + e, nbe - 1)];  // scalar
+        for (int e = 0; e < 8; ++e) rowp[e] = bf[k0 + e];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const int kc = min(k0 + 8 * lg + e, nbe - 1);
-          rb[e] = bcol[kc * ld];
-          rp[e] = Pc[(int64_t)rowp[e] * ldp];
+          const int k = k0 + e;
+          load_lds(bcol + min(k, nbe - 1) * ld, &Bt[slot][8 * lg + e][0]);
+          if (lc < NT) {
+            const float* src =
+                (k < nbe && colok) ? Pc + (int64_t)rowp[e] * ldp : g_zero_row + lc;
+            load_lds(src, &Pt[slot][8 * lg + e][0]);
+          }
         }
       };
-      const float ffac = (float)fac;
       double acc[BI][2][4];
 #pragma unroll
       for (int a = 0; a < BI; ++a)
@@ -422,17 +473,12 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
-      load(0);
+      for (int s = 0; s < kStages - 1 && s < nks; ++s) issue(s);
       for (int s = 0; s < nks; ++s) {
-        const int buf = s & 1;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const bool kok = 32 * s + 8 * lg + e < nbe;
-          Bt[buf][8 * lg + e][lc] = kok ? rb[e] : 0.f;
-          if (lc < NT) Pt[buf][8 * lg + e][lc] = (kok && colok) ? ffac * rp[e] : 0.f;
-        }
-        __syncthreads();
-        if (s + 1 < nks) load(32 * (s + 1));
+        const int slot = s % kStages;
+        // Blocks s+1 .. min(s + kStages - 2, nks - 1) may stay in flight.
+        wait_loads_barrier(kLoadsPerStage * min(kStages - 2, nks - 1 - s));
+        if (s + kStages - 1 < nks) issue(s + kStages - 1);
         v4f c[BI][2];
 #pragma unroll
         for (int bi = 0; bi < BI; ++bi)
@@ -443,9 +489,10 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
           const int kr = ks * 4 + (lane >> 4);
           float a_[BI];
 #pragma unroll
-          for (int bi = 0; bi < BI; ++bi) a_[bi] = Pt[buf][kr][wm * 16 * BI + bi * 16 + (lane & 15)];
-          const float b0 = Bt[buf][kr][wn * 32 + (lane & 15)];
-          const float b1 = Bt[buf][kr][wn * 32 + 16 + (lane & 15)];
+          for (int bi = 0; bi < BI; ++bi)
+            a_[bi] = Pt[slot][kr][wm * 16 * BI + bi * 16 + (lane & 15)];
+          const float b0 = Bt[slot][kr][wn * 32 + (lane & 15)];
+          const float b1 = Bt[slot][kr][wn * 32 + 16 + (lane & 15)];
 #pragma unroll
           for (int bi = 0; bi < BI; ++bi) {
             c[bi][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_[bi], b0, c[bi][0], 0, 0, 0);
@@ -459,8 +506,15 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bi][bj][r];
       }
+#pragma unroll
+      for (int a = 0; a < BI; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[a][b][r] *= fac;
       fold_tile<float, BI, GGA>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
-      // The next tile's first stage write targets buffer 0 (see k_density_f64).
+      // The next tile's first loads target slots 0 and 1; every wave has
+      // finished reading them (last K blocks of this tile) at this barrier.
       __syncthreads();
     }
     finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);

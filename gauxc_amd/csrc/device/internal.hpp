@@ -34,6 +34,8 @@ void set_error(const std::string& msg);
 // ---- device data ------------------------------------------------------------
 constexpr int kMaxL = 4;          // highest angular momentum the kernels evaluate
 constexpr int kPtBlock = 64;      // points per collocation / density block (npts_pad unit)
+constexpr int kBfPad = 32;        // zero entries after the last task's basis map: K blocks
+                                  // of 32 rows read up to 31 entries past a task's nbe
 
 struct DevShell {
   double alpha[32];
@@ -140,11 +142,13 @@ int launch_ssf(hipStream_t s, int32_t natoms, const double* atom_xyz, const doub
 int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first, const double* px,
-                            const double* py, const double* pz, double* scratch, int deriv);
+                            const double* py, const double* pz, double* scratch, int deriv,
+                            int lmax);
 int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first, const double* px,
-                            const double* py, const double* pz, float* scratch, int deriv);
+                            const double* py, const double* pz, float* scratch, int deriv,
+                            int lmax);
 // Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
 // whose tasks use `nt`-function tiles.
 int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, int func,
