@@ -111,12 +111,12 @@ typedef unsigned U32x2 __attribute__((ext_vector_type(2)));
 struct TaskStore {
   __amdgpu_buffer_rsrc_t rs;
   int vo4, vo2;       // lane byte offsets for 4-/8-byte and 2-byte elements
-  int lo_base;        // byte offset of the residual plane
+  unsigned lo_base;   // byte offset of the residual plane
 };
 
 template <typename T>
 __device__ __forceinline__ void store_elem(const TaskStore& ts, int64_t idx, T v) {
-  const int so = (int)(idx * (int64_t)sizeof(T));
+  const int so = (int)(unsigned)(idx * (int64_t)sizeof(T));
   if constexpr (sizeof(T) == 4)
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), ts.rs, ts.vo4, so, 0);
   else
@@ -143,7 +143,7 @@ __device__ __forceinline__ void shell_eval(double x, double y, double z, double 
     if constexpr (sizeof(T) == 4)
       __builtin_amdgcn_raw_buffer_store_b16(
           (short)(valid ? resid_bf16(v) : (unsigned short)0), ts.rs, ts.vo2,
-          ts.lo_base + (int)(2 * r), 0);
+          (int)((unsigned)ts.lo_base + (unsigned)(2 * r)), 0);
     if constexpr (GRAD) {
       store_elem<T>(ts, plane + r, valid ? T(gx) : T(0));
       store_elem<T>(ts, 2 * plane + r, valid ? T(gy) : T(0));
@@ -207,6 +207,9 @@ __device__ __forceinline__ void shell_eval(double x, double y, double z, double 
   }
 }
 
+constexpr int kCollShells = 256;  // shells staged per pass
+constexpr int kCollPrims = 4;     // primitives per shell staged in LDS
+
 // Occupancy target by the highest angular momentum a launch evaluates:
 // spherical f (LMAX 3) with gradients fits 168 registers, g (LMAX 4) 256.
 constexpr int coll_waves(int lmax) { return lmax <= 2 ? 4 : (lmax == 3 ? 3 : 2); }
@@ -238,27 +241,61 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
   const int64_t boff = sizeof(T) == 4 ? t.b_off_mx : t.b_off;
   const int64_t m0 = (int64_t)wi.a * kPtBlock;
   // FP32 storage: bfloat16 residual plane of the values after the five
-  // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).  Offsets within the task's
-  // region stay below 2^31 bytes (checked when the chunks are planned).
+  // FP32 planes (B, dB/dx, dB/dy, dB/dz, Z).  Byte offsets within the
+  // task's region are below 2^32 (checked when the chunks are planned); the
+  // buffer unit adds them as unsigned 32-bit values.
   TaskStore ts;
   ts.rs = __builtin_amdgcn_make_buffer_rsrc(scratch + boff, 0, -1, 0x00020000);
   ts.vo4 = (int)(lane * 4u);
   ts.vo2 = (int)(lane * 2u);
-  ts.lo_base = (int)(5 * plane * (int64_t)sizeof(T));
-  for (int s = wave; s < t.nshells; s += 4) {
-    const int ish = shell_list[t.sh_off + s];
-    const int bf0 = lsh_first[t.lsh_off + s];
-    const DevShell* sh = shells + ish;
-    const int l = sh->l;
-    const int nprim = sh->nprim;
-    const bool pure = sh->pure != 0;
-    const double x = gx - sh->O[0], y = gy - sh->O[1], z = gz - sh->O[2];
+  ts.lo_base = (unsigned)(5 * plane * (int64_t)sizeof(T));
+  // Shell parameters of the task, staged in LDS once per pass of up to
+  // kCollShells shells by one thread each, so the shell loop reads them with
+  // LDS broadcasts instead of a chain of dependent scalar loads per shell.
+  // Primitives past the first kCollPrims are read from the shell record.
+  __shared__ double s_O[3][kCollShells];
+  __shared__ double s_a[kCollPrims][kCollShells], s_c[kCollPrims][kCollShells];
+  __shared__ int s_info[kCollShells];  // l | pure << 8 | nprim << 16
+  __shared__ int s_bf0[kCollShells], s_ish[kCollShells];
+  const int tid = threadIdx.x;
+  for (int pass = 0; pass < t.nshells; pass += kCollShells) {
+    const int nsh = min(kCollShells, t.nshells - pass);
+    __syncthreads();  // the previous pass is done with the staged shells
+    for (int q = tid; q < nsh; q += 256) {
+      const int ish = shell_list[t.sh_off + pass + q];
+      const DevShell* sh = shells + ish;
+      s_ish[q] = ish;
+      s_bf0[q] = lsh_first[t.lsh_off + pass + q];
+      s_info[q] = sh->l | (sh->pure ? 1 << 8 : 0) | (sh->nprim << 16);
+      for (int d = 0; d < 3; ++d) s_O[d][q] = sh->O[d];
+      const int np = min(sh->nprim, kCollPrims);
+      for (int k = 0; k < np; ++k) {
+        s_a[k][q] = sh->alpha[k];
+        s_c[k][q] = sh->coeff[k];
+      }
+    }
+    __syncthreads();
+  for (int s = wave; s < nsh; s += 4) {
+    const int info = __builtin_amdgcn_readfirstlane(s_info[s]);
+    const int bf0 = __builtin_amdgcn_readfirstlane(s_bf0[s]);
+    const int l = info & 0xff;
+    const bool pure = (info >> 8) & 1;
+    const int nprim = info >> 16;
+    const double x = gx - s_O[0][s], y = gy - s_O[1][s], z = gz - s_O[2][s];
     const double r2 = x * x + y * y + z * z;
     double R = 0., dR = 0.;
-    for (int k = 0; k < nprim; ++k) {
-      const double e = sh->coeff[k] * exp(-sh->alpha[k] * r2);
+    for (int k = 0; k < nprim && k < kCollPrims; ++k) {
+      const double a = s_a[k][s], e = s_c[k][s] * exp(-a * r2);
       R += e;
-      dR += -2. * sh->alpha[k] * e;
+      dR += -2. * a * e;
+    }
+    if (nprim > kCollPrims) {
+      const DevShell* sh = shells + __builtin_amdgcn_readfirstlane(s_ish[s]);
+      for (int k = kCollPrims; k < nprim; ++k) {
+        const double e = sh->coeff[k] * exp(-sh->alpha[k] * r2);
+        R += e;
+        dR += -2. * sh->alpha[k] * e;
+      }
     }
     const int64_t row0 = (int64_t)bf0 * ld + m0;
     if (l == 0) shell_eval<0, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
@@ -269,6 +306,7 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
       shell_eval<(LMAX < 3 ? 2 : 3), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
     else
       shell_eval<(LMAX < 4 ? 2 : 4), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+  }
   }
 }
 

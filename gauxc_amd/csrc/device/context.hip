@@ -713,6 +713,9 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
       while (i < ntasks) {
         auto& d = ht[i];
         const size_t b = task_bytes(d);
+        // Kernels address a task's planes with 32-bit byte offsets.
+        XCQ_REQUIRE(b < (size_t(1) << 32), XCQ_ERROR_INVALID_ARGUMENT,
+                    "task too large: its basis function and point planes exceed 4 GiB");
         if (used + b > budget) {
           XCQ_REQUIRE(i > c.task_begin, XCQ_ERROR_OUT_OF_MEMORY,
                       "Insufficient memory to even start!");
