@@ -31,7 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
+# Matrix unit each stage's mixed-precision kernels run on (the FP64 path
+# uses the FP64 MFMA throughout): the density stage's X GEMM is FP32, the
+# B^T Z + Z^T B contraction BF16 slices.
+MIXED_PEAK = {"density": PEAK_FP32_TFLOPS, "syr2k_scatter": PEAK_BF16_TFLOPS}
 PEAK_HBM_GBS = 8000.0
 STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symmetrize"]
 # Kernels of each stage (the names rocprofv3 reports).
@@ -224,7 +229,7 @@ def main():
     name = STAGES[k_dom]
     dom_ms = float(r["stage"][k_dom])
     abytes, aflops = algorithmic(name, head, ts)
-    peak_mfma = PEAK_FP64_TFLOPS if head == "fp64" else PEAK_BF16_TFLOPS
+    peak_mfma = PEAK_FP64_TFLOPS if head == "fp64" else MIXED_PEAK.get(name, PEAK_BF16_TFLOPS)
     ridge = peak_mfma * 1e12 / (PEAK_HBM_GBS * 1e9)
     if aflops > 0 and aflops / abytes > ridge:
         roof = dict(bound="mfma", achieved=aflops / (dom_ms * 1e-3) / 1e12, peak=peak_mfma,

@@ -365,9 +365,9 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 //
 // Operand staging goes straight from global memory to LDS
 // (global_load_lds_dword: lane i of a wave writes dword i of a 256-byte LDS
-// row), through a ring of kStages K blocks, so kStages - 1 blocks of loads
-// are in flight while the matrix cores work on the current one, and no
-// register holds staged data.  Thread (lane c, wave g) issues, per K block,
+// row), through a ring of kStages K blocks, so up to kStages - 1 blocks of
+// loads are in flight while the matrix cores work on the current one, and
+// no register holds staged data.  Thread (lane c, wave g) issues, per K block,
 // the eight B^T rows 8g..8g+7 (lane = point) and the eight P_sub rows
 // (lane = function c < NT).  P rows past nbe and functions past nbe are
 // read from a zero row, B^T rows past nbe at a clamped valid row (finite,
@@ -393,7 +393,7 @@ __device__ __forceinline__ void wait_loads_barrier(int n) {
     default: asm volatile("s_waitcnt vmcnt(32)\n\ts_barrier" ::: "memory"); break;
   }
 }
-static_assert(kLoadsPerStage == 16 && kStages == 3, "wait_loads_barrier covers two blocks ahead");
+static_assert(kLoadsPerStage == 16 && kStages <= 4, "wait_loads_barrier covers up to two blocks");
 
 // One dword per lane from global memory into the LDS row at lds_row (lane i
 // writes lds_row[i]); lds_row must be the same for the whole wave.  Issued
