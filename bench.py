@@ -44,7 +44,8 @@ STAGE_KERNELS = {
     ("collocation", "fp64"): ["k_collocation<true, double, 3>"],
     ("collocation", "mixed"): ["k_collocation<true, float, 3>"],
     ("density", "fp64"): ["k_density_f64<32, true>", "k_density_f64<64, true>"],
-    ("density", "mixed"): ["k_density_mx<32, true>", "k_density_mx<64, true>"],
+    ("density", "mixed"): ["k_density_mx<32, true, false>", "k_density_mx<64, true, true>",
+                           "k_density_mx<64, true, false>"],
     ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
     ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32>", "k_syr2k_mx<64>"],
 }

@@ -414,6 +414,10 @@ class DeviceContext:
         N.xcq_check(self.lib.xcq_set_tile_policy(self.h, int(dens32_max_nbe),
                                                  int(syr2k32_max_nbe)), "xcq_set_tile_policy")
 
+    def set_density_occupancy(self, hiocc_max_nbe):
+        N.xcq_check(self.lib.xcq_set_density_occupancy(self.h, int(hiocc_max_nbe)),
+                    "xcq_set_density_occupancy")
+
     def set_schedule(self, nqueues=0, dens_cost=None, syr2k_cost=None):
         if dens_cost is None:
             N.xcq_check(self.lib.xcq_set_schedule(self.h, int(nqueues), None, None),

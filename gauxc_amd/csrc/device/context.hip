@@ -146,6 +146,7 @@ struct xcq_context_s {
   WorkItem* d_work = nullptr;
   int32_t* d_qoff = nullptr;
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
+  int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
   int32_t nqueues = 0;                                 // 0: one workgroup per item
   std::vector<double> cost_d, cost_s;                  // per-task predicted costs
   bool timing = false;
@@ -253,6 +254,7 @@ int build_work_lists(xcq_context_s* ctx) {
                          ctx->cost_s.size() == (size_t)ctx->ntasks;
   for (auto& d : ctx->h_tasks) {
     d.nt_d = d.nbe <= ctx->dens32_max_nbe ? 32 : 64;
+    d.dens_cls = d.nt_d == 32 ? 0 : (d.nbe <= ctx->dens_hiocc_max_nbe ? 1 : 2);
     d.nt_s = d.nbe <= ctx->syr2k32_max_nbe ? 32 : 64;
   }
   auto finish_range = [&](WorkRange& r, const std::vector<double>& cost) {
@@ -268,13 +270,12 @@ int build_work_lists(xcq_context_s* ctx) {
     }
   };
   for (auto& c : ctx->chunks) {
-    for (int v = 0; v < 2; ++v) {  // density: 32-wide, then 64-wide tiles
-      const int nt = v == 0 ? 32 : 64;
+    for (int v = 0; v < 3; ++v) {  // density classes (DevTask::dens_cls)
       c.dens[v].off = (int64_t)W.size();
       std::vector<double> cost;
       for (int32_t i = c.task_begin; i < c.task_end; ++i) {
         const DevTask& d = ctx->h_tasks[i];
-        if (d.nt_d != nt) continue;
+        if (d.dens_cls != v) continue;
         const int nmb = d.npts_pad / kPtBlock;
         const double ci = have_cost ? ctx->cost_d[i] / nmb : heur_dens_item(d);
         for (int pb = 0; pb < nmb; ++pb) {
@@ -285,7 +286,7 @@ int build_work_lists(xcq_context_s* ctx) {
       finish_range(c.dens[v], cost);
     }
     c.coll.off = c.dens[0].off;
-    c.coll.n = c.dens[0].n + c.dens[1].n;
+    c.coll.n = c.dens[0].n + c.dens[1].n + c.dens[2].n;
     c.coll.nq = 0;
     for (int v = 0; v < 2; ++v) {  // SYR2K tiles
       const int nt = v == 0 ? 32 : 64;
@@ -351,14 +352,14 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     // width (stage 1); stages 2 (functional) and 3 (Z) stay as empty
     // intervals so the stage table keeps its shape.
     XCQ_HIP(rec(ev[2]));
-    for (int k = 0; k < 2; ++k)
-      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, precision, func, gga, fac,
+    for (int k = 0; k < 3; ++k)
+      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, k == 1, precision, func, gga, fac,
                                want_vxc)))
         return rc;
     XCQ_HIP(rec(ev[3]));
     XCQ_HIP(rec(ev[4]));
     XCQ_HIP(rec(ev[5]));
-    launches += 3;
+    launches += 4;
     items += c.coll.n;
     if (want_vxc) {
       XCQ_HIP(rec(ev[6]));
@@ -822,6 +823,19 @@ int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32
   return XCQ_SUCCESS;
 }
 
+int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->dens_hiocc_max_nbe = hiocc_max_nbe;
+  if (ctx->ntasks == 0) return XCQ_SUCCESS;
+  int rc = build_work_lists(ctx);
+  if (rc) return rc;
+  XCQ_HIP(hipMemcpy(ctx->d_tasks, ctx->h_tasks.data(), ctx->h_tasks.size() * sizeof(DevTask),
+                    hipMemcpyHostToDevice));
+  return XCQ_SUCCESS;
+}
+
 int xcq_set_schedule(xcq_context ctx, int32_t nqueues, const double* dens_cost,
                      const double* syr2k_cost) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
@@ -874,11 +888,10 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us) {
       if (dt > 0) out[ctx->h_work[k].task] += dt * us_per_tick;
     }
   };
-  for (const auto& c : ctx->chunks)
-    for (int v = 0; v < 2; ++v) {
-      add(c.dens[v], dens_us);
-      add(c.syr2k[v], syr2k_us);
-    }
+  for (const auto& c : ctx->chunks) {
+    for (int v = 0; v < 3; ++v) add(c.dens[v], dens_us);
+    for (int v = 0; v < 2; ++v) add(c.syr2k[v], syr2k_us);
+  }
   return XCQ_SUCCESS;
 }
 
@@ -940,6 +953,22 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
     }
   }
   if ((rc = xcq_set_tile_policy(ctx, best_d, best_s))) return rc;
+  // Mixed precision: which of the remaining (64-wide) density tasks run in
+  // the three-workgroups-per-CU configuration.
+  if (precision == XCQ_PREC_MIXED) {
+    int32_t best_h = ctx->dens_hiocc_max_nbe;
+    double bh = 1e300;
+    for (int32_t c : cand) {
+      if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
+      double dm, sm;
+      if ((rc = measure(&dm, &sm, &tot))) return rc;
+      if (dm < bh) {
+        bh = dm;
+        best_h = c;
+      }
+    }
+    if ((rc = xcq_set_density_occupancy(ctx, best_h))) return rc;
+  }
   // Persistent queues (flop-heuristic packing) against one workgroup per item.
   int dev_cus = 0;
   if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=

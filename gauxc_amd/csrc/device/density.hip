@@ -182,10 +182,9 @@ using Vec4 = T __attribute__((ext_vector_type(4)));
 // flight instead of waiting on each row in turn (4 rows of FP32, 2 of FP64,
 // which fit the register budget of the GEMM phase).  Row offsets are clamped
 // to a valid row; rows past nbe are not stored.
-template <typename T, bool GGA>
+template <typename T, bool GGA, int kZRows = sizeof(T) == 4 ? 4 : 2>
 __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, double (*zc)[64],
                                         T* Bm) {
-  constexpr int kZRows = sizeof(T) == 4 ? 4 : 2;
   const int q = tid & 15, g = tid >> 4;
   const int ld = t.npts_pad, nbe = t.nbe, plane = nbe * ld;
   double a[4], cx[4], cy[4], cz[4];
@@ -372,8 +371,16 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 // (lane = function c < NT).  P rows past nbe and functions past nbe are
 // read from a zero row, B^T rows past nbe at a clamped valid row (finite,
 // multiplied by those zero P rows).
-constexpr int kStages = 3;
 constexpr int kLoadsPerStage = 16;  // global_load_lds instructions per thread per K block
+
+// Two configurations of the kernel: the default one keeps three K blocks in
+// the ring at two workgroups per CU; the high-occupancy one (HIOCC) keeps
+// two blocks in a smaller ring and fits three workgroups per CU (LDS
+// 41 KiB, at most 168 registers), which hides the fold's and Z's memory
+// latency better on small tasks.
+template <bool HIOCC>
+constexpr int ring_stages() { return HIOCC ? 2 : 3; }
+constexpr int density_waves(int nt, bool hiocc) { return hiocc ? 3 : 2; }
 
 __device__ float g_zero_row[64];  // never written: zero-initialised device memory
 
@@ -393,7 +400,7 @@ __device__ __forceinline__ void wait_loads_barrier(int n) {
     default: asm volatile("s_waitcnt vmcnt(32)\n\ts_barrier" ::: "memory"); break;
   }
 }
-static_assert(kLoadsPerStage == 16 && kStages <= 4, "wait_loads_barrier covers up to two blocks");
+static_assert(kLoadsPerStage == 16, "wait_loads_barrier counts 16 loads per block");
 
 // One dword per lane from global memory into the LDS row at lds_row (lane i
 // writes lds_row[i]); lds_row must be the same for the whole wave.  Issued
@@ -406,13 +413,14 @@ __device__ __forceinline__ void load_lds(const float* src, float* lds_row) {
   asm volatile("global_load_lds_dword %0, off" ::"v"(src), "{m0}"(off) : "memory");
 }
 
-template <int NT, bool GGA>
-__global__ __launch_bounds__(256, 2) void k_density_mx(
+template <int NT, bool GGA, bool HIOCC>
+__global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const float* __restrict__ Pfall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
+  constexpr int kStages = ring_stages<HIOCC>();
   __shared__ float Pt[kStages][32][NT + 4];  // [slot][k][function]: A operand (P_sub)
   __shared__ float Bt[kStages][32][68];      // [slot][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
@@ -520,15 +528,15 @@ __global__ __launch_bounds__(256, 2) void k_density_mx(
     finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<float, GGA>(t, m0, tid, zc, Bm);
+      write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
   }
 }
 
-int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, int func,
-                   int gga, double fac, int want_z) {
+int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
+                   int func, int gga, double fac, int want_z) {
   if (r.n == 0) return XCQ_SUCCESS;
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
@@ -544,10 +552,13 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, 
       gga ? go(k_density_f64<64, true>, v.P, scr) : go(k_density_f64<64, false>, v.P, scr);
   } else {
     float* scr = static_cast<float*>(v.scratch);
+    const float* Pf = v.Pf32;
     if (nt == 32)
-      gga ? go(k_density_mx<32, true>, v.Pf32, scr) : go(k_density_mx<32, false>, v.Pf32, scr);
+      gga ? go(k_density_mx<32, true, false>, Pf, scr) : go(k_density_mx<32, false, false>, Pf, scr);
+    else if (hiocc)
+      gga ? go(k_density_mx<64, true, true>, Pf, scr) : go(k_density_mx<64, false, true>, Pf, scr);
     else
-      gga ? go(k_density_mx<64, true>, v.Pf32, scr) : go(k_density_mx<64, false>, v.Pf32, scr);
+      gga ? go(k_density_mx<64, true, false>, Pf, scr) : go(k_density_mx<64, false, false>, Pf, scr);
   }
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;

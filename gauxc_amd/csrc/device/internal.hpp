@@ -53,7 +53,8 @@ struct DevTask {
   int32_t ldp;        // nbf of the molecule = leading dimension of its P and VXC
   int32_t nt_d;       // function tile of the density kernel (32 or 64)
   int32_t nt_s;       // function tile of the SYR2K kernel (32 or 64)
-  int32_t pad_;
+  int32_t dens_cls;   // density work list: 0 = 32-wide, 1 = 64-wide at three
+                      // workgroups per CU, 2 = 64-wide at two
   int64_t pt_off;     // points (SoA) / weights
   int64_t sh_off;     // shell list
   int64_t bf_off;     // local -> molecule basis function map (nbe entries)
@@ -91,8 +92,8 @@ struct QueueArgs {
 struct Chunk {
   int32_t task_begin = 0, task_end = 0;
   int64_t scratch_elems = 0;  // elements of T (FP64 or FP32) used by B-family arrays
-  WorkRange coll;             // all point blocks (collocation) = dens[0] followed by dens[1]
-  WorkRange dens[2];          // density blocks of tasks with 32- / 64-function tiles
+  WorkRange coll;             // all point blocks (collocation) = dens[0..2] back to back
+  WorkRange dens[3];          // density blocks by class (DevTask::dens_cls)
   WorkRange syr2k[2];         // SYR2K tiles (it >= jt) with 32- / 64-function tiles
 };
 
@@ -150,9 +151,10 @@ int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
                             const double* py, const double* pz, float* scratch, int deriv,
                             int lmax);
 // Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
-// whose tasks use `nt`-function tiles.
-int launch_density(const CtxView& v, const WorkRange& r, int nt, int precision, int func,
-                   int gga, double fac, int want_z);
+// whose tasks use `nt`-function tiles (hiocc: the three-workgroups-per-CU
+// configuration of the mixed kernel).
+int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
+                   int func, int gga, double fac, int want_z);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);
 int launch_P_to_f32(hipStream_t s, int64_t n, const double* P, float* out);

@@ -106,6 +106,11 @@ int xcq_download_results_mol(xcq_context ctx, int32_t mol, double* exc, double* 
  * Tile policy: tasks with nbe <= dens32_max_nbe run the density stage with
  * 32-function tiles (else 64); likewise syr2k32_max_nbe for B^T Z. */
 int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32_max_nbe);
+/* Mixed precision: density tasks with 64-function tiles and
+ * nbe <= hiocc_max_nbe run in the kernel configuration that fits three
+ * workgroups per CU (two-block operand ring); larger ones keep the
+ * three-block ring at two workgroups per CU.  0 (default) disables it. */
+int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
 /* nqueues = 0: one workgroup per work item (large-first upload order,
  * XCD-interleaved).  nqueues > 0: persistent launches of nqueues workgroups
  * per stage list, each draining its own queue; work items are packed
@@ -125,9 +130,11 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us);
 int xcq_use_graph(xcq_context ctx, int enable);
 /* Autotuner: times the uploaded workload on the device and keeps the
  * fastest settings.  Sweeps the 32/64 tile thresholds of the density and
- * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}), then
- * the persistent-queue count {0, 2, 4 per CU}; leaves the context on the
- * best and reports it (any output pointer may be NULL). */
+ * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}), for
+ * mixed precision the density occupancy threshold over the same buckets,
+ * then the persistent-queue count {0, 2, 4 per CU}; leaves the context on
+ * the best and reports the thresholds and queue count (any output pointer
+ * may be NULL). */
 int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
                  int32_t* syr2k32_max_nbe, int32_t* nqueues, double* best_ms);
 

@@ -55,6 +55,27 @@ def test_tile_policy_variants(gx, small, prec, tol_e, tol_v):
         assert np.abs(v1 - v0).max() <= tol_v
 
 
+def test_density_occupancy_variants(gx, small):
+    """The mixed density kernel's three-workgroups-per-CU configuration
+    (two-block operand ring) computes the same arithmetic as the default
+    one: identical EXC and NEL (fixed-order reductions) for every threshold,
+    also with queues; VXC to the run-to-run noise of its FP64 atomics."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    ctx.set_tile_policy(48, 96)
+    ctx.set_density_occupancy(0)
+    e0, n0, v0 = _run(ctx, "mixed")
+    for h, nq in [(100000, 0), (128, 0), (100000, 300)]:
+        _flush(f"density occupancy threshold {h} queues {nq}")
+        ctx.set_density_occupancy(h)
+        ctx.set_schedule(nq)
+        e1, n1, v1 = _run(ctx, "mixed")
+        assert e1 == e0 and n1 == n0
+        assert np.abs(v1 - v0).max() <= 1e-12
+    ctx.set_schedule(0)
+    ctx.set_density_occupancy(0)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_queued_schedule_matches_direct(gx, small, prec):
     """Persistent queues (heuristic and explicit costs) reproduce the
