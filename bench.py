@@ -126,8 +126,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-policy", default=None,
-                    help="D,S,Q: fixed density / SYR2K 32-wide thresholds and queue count "
-                         "for every precision (skips the autotuner)")
+                    help="D,S,Q[,H]: fixed density / SYR2K 32-wide thresholds, queue count "
+                         "and (mixed) density high-occupancy threshold for every precision "
+                         "(skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count)")
@@ -174,15 +175,21 @@ def main():
     for prec in precisions:
         tuned = None
         if args.tile_policy:
-            d_, s_, q_ = (int(x) for x in args.tile_policy.split(","))
+            vals = [int(x) for x in args.tile_policy.split(",")]
+            d_, s_, q_ = vals[:3]
+            h_ = vals[3] if len(vals) > 3 else 0
             ctx.set_tile_policy(d_, s_)
+            ctx.set_density_occupancy(h_)
             ctx.set_schedule(q_)
-            tuned = (d_, s_, q_, None)
+            tuned = (d_, s_, q_, None, h_)
         elif not args.no_autotune:
             tic = time.perf_counter()
             tuned = ctx.autotune("PBE", prec)
+            pol = ctx.tile_policy()
+            tuned = tuned + (pol[2],)
             log(f"{prec}: autotune -> dens32<=nbe {tuned[0]}, syr2k32<=nbe {tuned[1]}, "
-                f"queues {tuned[2]} ({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
+                f"queues {tuned[2]}, density hi-occupancy<=nbe {pol[2]} "
+                f"({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
         # Eager steps: per-stage HIP-event times of the kernels.
         ctx.use_graph(False)
         stage = np.zeros(7)

@@ -823,6 +823,15 @@ int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32
   return XCQ_SUCCESS;
 }
 
+int xcq_get_tile_policy(xcq_context ctx, int32_t* out4) {
+  XCQ_REQUIRE(ctx && out4, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  out4[0] = ctx->dens32_max_nbe;
+  out4[1] = ctx->syr2k32_max_nbe;
+  out4[2] = ctx->dens_hiocc_max_nbe;
+  out4[3] = ctx->nqueues;
+  return XCQ_SUCCESS;
+}
+
 int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_HIP(hipSetDevice(ctx->device));

@@ -111,6 +111,9 @@ int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32
  * workgroups per CU (two-block operand ring); larger ones keep the
  * three-block ring at two workgroups per CU.  0 (default) disables it. */
 int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
+/* Current settings: out4 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
+ * nqueues}. */
+int xcq_get_tile_policy(xcq_context ctx, int32_t* out4);
 /* nqueues = 0: one workgroup per work item (large-first upload order,
  * XCD-interleaved).  nqueues > 0: persistent launches of nqueues workgroups
  * per stage list, each draining its own queue; work items are packed
