@@ -26,6 +26,8 @@
 // register r of lane l is D[4(l/16)+r][l%16].
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace xcq {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -84,8 +86,8 @@ __global__ __launch_bounds__(256) void k_P_to_f32(int64_t n, const double* __res
 // row's 128 B): for NT = 64 row f of the i and j panels, for NT = 32 row
 // f & 31 of the i panels (f < 32) or of the j panels (f >= 32).  Rows past
 // nbe are read at row nbe-1 and zeroed by a select.
-template <int NT>
-__global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
+template <int NT, int PF = 1>
+__global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa,
                                                   const DevTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ bfmap,
                                                   const float* __restrict__ scratch,
@@ -100,8 +102,6 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
   const int wm = wave & 1, wn = wave >> 1;
   const int pg = tid & 3, f = tid >> 2;
   const int frow = NT == 64 ? f : (f & 31);
-  const bool do_i = NT == 64 || f < 32;  // this thread stages i-panel rows
-  const bool do_j = NT == 64 || f >= 32;  // ... and/or j-panel rows
   const int kk = 8 * (lane >> 4);
 
   XCQ_FOR_EACH_ITEM(qa, it) {
@@ -122,7 +122,6 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
     const int fi = i0 + frow, fj = j0 + frow;
     const bool ki = fi < nbe, kj = fj < nbe;
     const int oi = min(fi, nbe - 1) * ld + 8 * pg, oj = min(fj, nbe - 1) * ld + 8 * pg;
-    const bool ldj = do_j && !diag;
 
     double acc[BI][BI][4];
 #pragma unroll
@@ -132,26 +131,31 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.;
 
-    // Raw prefetch registers; the row masks are applied when staged, so the
-    // loads stay in flight across the current step's matrix instructions.
-    float4 pre[4][2];  // (Bi, Zi, Bj, Zj) x two float4
-    uint4 prel[2];     // residuals of Bi, Bj (8 bfloat16 each)
-    auto load = [&](int k0) {
-      if (do_i) {
+    // Raw prefetch registers, a ring of PF K blocks (PF = 3 for 32-wide
+    // tiles: two blocks stay in flight while one is staged and multiplied;
+    // 64-wide tiles have no register room for more than one).  The row
+    // masks are applied when staged, so the loads stay in flight across the
+    // matrix instructions.
+    // Panel sides this thread loads: both (i then j) for 64-wide tiles, one
+    // (i for f < 32, j otherwise) for 32-wide ones; the j side is skipped on
+    // diagonal tiles (it equals the i side).
+    constexpr int SIDES = NT == 64 ? 2 : 1;
+    auto side_id = [&](int sd) { return NT == 64 ? sd : (f < 32 ? 0 : 1); };
+    auto side_on = [&](int sd) { return side_id(sd) == 0 || !diag; };
+    float4 pre[PF][SIDES][2][2];  // [slot][side][B, Z][two float4]
+    uint4 prel[PF][SIDES];        // residuals of B (8 bfloat16)
+    auto load = [&](int k0, auto S) {
+      constexpr int sl = decltype(S)::value;
+#pragma unroll
+      for (int sd = 0; sd < SIDES; ++sd) {
+        if (!side_on(sd)) continue;
+        const int o = (side_id(sd) == 0 ? oi : oj) + k0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          pre[0][h] = *reinterpret_cast<const float4*>(Bm + oi + k0 + 4 * h);
-          pre[1][h] = *reinterpret_cast<const float4*>(Zm + oi + k0 + 4 * h);
+          pre[sl][sd][0][h] = *reinterpret_cast<const float4*>(Bm + o + 4 * h);
+          pre[sl][sd][1][h] = *reinterpret_cast<const float4*>(Zm + o + 4 * h);
         }
-        prel[0] = *reinterpret_cast<const uint4*>(Lm + oi + k0);
-      }
-      if (ldj) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          pre[2][h] = *reinterpret_cast<const float4*>(Bm + oj + k0 + 4 * h);
-          pre[3][h] = *reinterpret_cast<const float4*>(Zm + oj + k0 + 4 * h);
-        }
-        prel[1] = *reinterpret_cast<const uint4*>(Lm + oj + k0);
+        prel[sl][sd] = *reinterpret_cast<const uint4*>(Lm + o);
       }
     };
     auto stage = [&](const float4 (&v)[2], bool ok, unsigned short (*dst)[NT][kRow]) {
@@ -174,20 +178,23 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
       *reinterpret_cast<uint4*>(&dst[3][frow][pg * 8]) = ok ? l : make_uint4(0u, 0u, 0u, 0u);
     };
 
-    load(0);
-    for (int k0 = 0; k0 < ld; k0 += kKB) {
-      if (do_i) {
-        stage(pre[0], ki, Bi);
-        stage_lo(prel[0], ki, Bi);
-        stage(pre[1], ki, Zi);
-      }
-      if (ldj) {
-        stage(pre[2], kj, Bj);
-        stage_lo(prel[1], kj, Bj);
-        stage(pre[3], kj, Zj);
+    // One K block: stage ring slot S, refill it with block k0 + PF * kKB,
+    // multiply.
+    auto step = [&](int k0, auto S) {
+      constexpr int sl = decltype(S)::value;
+#pragma unroll
+      for (int sd = 0; sd < SIDES; ++sd) {
+        if (!side_on(sd)) continue;
+        const bool i_side = side_id(sd) == 0;
+        const bool ok = i_side ? ki : kj;
+        unsigned short(*db)[NT][kRow] = i_side ? Bi : Bj;
+        unsigned short(*dz)[NT][kRow] = i_side ? Zi : Zj;
+        stage(pre[sl][sd][0], ok, db);
+        stage_lo(prel[sl][sd], ok, db);
+        stage(pre[sl][sd][1], ok, dz);
       }
       __syncthreads();
-      if (k0 + kKB < ld) load(k0 + kKB);
+      if (k0 + PF * kKB < ld) load(k0 + PF * kKB, S);
       unsigned short(*BJ)[NT][kRow] = diag ? Bi : Bj;
       unsigned short(*ZJ)[NT][kRow] = diag ? Zi : Zj;
       bf16x8 fbi[BI][4], fzi[BI][3], fbj[BI][4], fzj[BI][3];
@@ -235,6 +242,18 @@ __global__ __launch_bounds__(256, 2) void k_syr2k_mx(QueueArgs qa,
 #pragma unroll
           for (int r = 0; r < 4; ++r)
             acc[bi][bj][r] += (double)cm[bi][bj][r] + (double)cc[bi][bj][r];
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, (PF > 1 ? 1 : 0)>;
+    using I2 = std::integral_constant<int, (PF > 2 ? 2 : 0)>;
+    load(0, I0{});
+    if (PF > 1 && kKB < ld) load(kKB, I1{});
+    if (PF > 2 && 2 * kKB < ld) load(2 * kKB, I2{});
+    for (int k0 = 0; k0 < ld; k0 += PF * kKB) {
+      step(k0, I0{});
+      if (PF > 1 && k0 + kKB < ld) step(k0 + kKB, I1{});
+      if (PF > 2 && k0 + 2 * kKB < ld) step(k0 + 2 * kKB, I2{});
     }
 
     // Scatter-add into VXC in transposed order (see k_syr2k).
