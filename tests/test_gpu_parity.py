@@ -250,3 +250,78 @@ def test_mixed_through_gauxc_api(gpu, gx):
     exc, vxc = integ.eval_exc_vxc(d["density"])
     assert abs(exc - d["exc"][0]) <= 1e-6
     assert np.abs(vxc - d["vxc"]).max() <= 1e-7
+
+
+def _high_l_system(gx, lmax):
+    """Two atoms carrying every angular momentum up to lmax, spherical and
+    Cartesian from l = 2 on, some shells contracted (two primitives, one
+    with six: more than the kernel stages per shell in LDS)."""
+    Z = [8, 1]
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 1.8]])
+    l, pure, alpha, coeff, origin = [], [], [], [], []
+    for a, r in zip(Z, xyz):
+        for am in range(lmax + 1):
+            for p in ([False] if am < 2 else [True, False]):
+                l.append(am)
+                pure.append(p)
+                alpha.append([1.7 / (am + 1), 0.45])
+                coeff.append([0.6, 0.8])
+                origin.append(r)
+        l.append(0)
+        pure.append(False)
+        alpha.append([30.0, 9.0, 3.0, 1.0, 0.35, 0.12])
+        coeff.append([0.05, 0.15, 0.3, 0.4, 0.3, 0.1])
+        origin.append(r)
+    mol = gx.Molecule(Z, xyz)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=1e-10)
+    P = gx.workloads.density(bs.nbf(), 4, 7)
+    lb = gx.workloads.build_tasks(mol, bs, grid="GM3")
+    return mol, bs, P, lb.get_tasks()
+
+
+@pytest.mark.parametrize("lmax", [3, 4])
+def test_device_collocation_high_l(gpu, gx, orc, lmax):
+    """Collocation kernels instantiated for the basis' highest angular
+    momentum (3 and 4) against the oracle (itself pinned to GauXC's compiled
+    gau2grid for l = 0..4), FP64 and FP32 storage."""
+    _flush(f"collocation lmax={lmax}")
+    mol, bs, P, t = _high_l_system(gx, lmax)
+    ctx = _context_for(gx, bs, t, P)
+    sh = orc.shells_from(bs.xcq_shells())
+    for task in sorted(set([0, len(t["npts"]) // 2, len(t["npts"]) - 1])):
+        s0, n = t["shell_offset"][task], t["nshells"][task]
+        p0, m = t["point_offset"][task], t["npts"][task]
+        ref = orc.collocation(sh, t["shell_list"][s0:s0 + n], t["points"][p0:p0 + m], deriv=True)
+        out = ctx.collocation(task, deriv=True, precision="fp64")
+        for k in range(4):
+            np.testing.assert_allclose(out[k], ref[k].T, rtol=1e-12,
+                                       atol=1e-14 * max(1.0, np.abs(ref[k]).max()))
+        out = ctx.collocation(task, deriv=True, precision="mixed")
+        for k in range(4):
+            r = ref[k].T
+            bound = 6e-8 * np.abs(r) + 1e-14 * max(1.0, np.abs(r).max()) + 1e-38
+            assert np.all(np.abs(out[k] - r) <= bound), (lmax, task, k)
+
+
+@pytest.mark.parametrize("lmax", [4])
+def test_device_fp64_matches_oracle_high_l(gpu, gx, orc, lmax):
+    """The whole FP64 pipeline on a basis with g functions and contracted
+    shells against the oracle (PBE), and mixed against FP64."""
+    _flush(f"pipeline lmax={lmax} fp64 vs oracle")
+    mol, bs, P, t = _high_l_system(gx, lmax)
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 1.8]])
+    w = orc.ssf_weights(xyz, t)
+    exc_o, nel_o, vxc_o = orc.exc_vxc("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), P, t,
+                                      weights=w)
+    t2 = dict(t)
+    t2["weights"] = w
+    ctx = _context_for(gx, bs, t2, P)
+    ctx.exc_vxc("PBE", "fp64", True)
+    exc, nel, vxc = ctx.results()
+    assert abs(exc - exc_o) <= 1e-10
+    assert abs(nel - nel_o) <= 1e-10
+    assert np.abs(vxc - vxc_o).max() <= 1e-9
+    ctx.exc_vxc("PBE", "mixed", True)
+    emx, nmx, vmx = ctx.results()
+    assert abs(emx - exc) <= 1e-6
+    assert np.abs(vmx - vxc).max() <= 1e-7
