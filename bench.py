@@ -45,7 +45,7 @@ STAGE_KERNELS = {
     ("collocation", "mixed"): ["k_collocation<true, float, 3>"],
     ("density", "fp64"): ["k_density_f64<32, true>", "k_density_f64<64, true>"],
     ("density", "mixed"): ["k_density_mx<32, true, false>", "k_density_mx<64, true, true>",
-                           "k_density_mx<64, true, false>"],
+                           "k_density_mx<64, true, false>", "k_density_mx<128, true, false>"],
     ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
     ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32>", "k_syr2k_mx<64>"],
 }
@@ -126,9 +126,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-policy", default=None,
-                    help="D,S,Q[,H]: fixed density / SYR2K 32-wide thresholds, queue count "
-                         "and (mixed) density high-occupancy threshold for every precision "
-                         "(skips the autotuner)")
+                    help="D,S,Q[,H[,W]]: fixed density / SYR2K 32-wide thresholds, queue "
+                         "count, (mixed) density high-occupancy threshold and 128-wide "
+                         "minimum nbe for every precision (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count)")
@@ -178,18 +178,20 @@ def main():
             vals = [int(x) for x in args.tile_policy.split(",")]
             d_, s_, q_ = vals[:3]
             h_ = vals[3] if len(vals) > 3 else 0
+            w_ = vals[4] if len(vals) > 4 else 1 << 30
             ctx.set_tile_policy(d_, s_)
             ctx.set_density_occupancy(h_)
+            ctx.set_density_wide(w_)
             ctx.set_schedule(q_)
-            tuned = (d_, s_, q_, None, h_)
+            tuned = (d_, s_, q_, None, h_, w_)
         elif not args.no_autotune:
             tic = time.perf_counter()
             tuned = ctx.autotune("PBE", prec)
             pol = ctx.tile_policy()
-            tuned = tuned + (pol[2],)
+            tuned = tuned + (pol[2], pol[4])
             log(f"{prec}: autotune -> dens32<=nbe {tuned[0]}, syr2k32<=nbe {tuned[1]}, "
-                f"queues {tuned[2]}, density hi-occupancy<=nbe {pol[2]} "
-                f"({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
+                f"queues {tuned[2]}, density hi-occupancy<=nbe {pol[2]}, "
+                f"128-wide>=nbe {pol[4]} ({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
         # Eager steps: per-stage HIP-event times of the kernels.
         ctx.use_graph(False)
         stage = np.zeros(7)

@@ -415,11 +415,15 @@ class DeviceContext:
                                                  int(syr2k32_max_nbe)), "xcq_set_tile_policy")
 
     def tile_policy(self):
-        """(dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe, nqueues)."""
-        out = np.zeros(4, dtype=np.int32)
+        """(dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe, nqueues, wide_min_nbe)."""
+        out = np.zeros(5, dtype=np.int32)
         N.xcq_check(self.lib.xcq_get_tile_policy(self.h, out.ctypes.data),
                     "xcq_get_tile_policy")
         return tuple(int(x) for x in out)
+
+    def set_density_wide(self, wide_min_nbe):
+        N.xcq_check(self.lib.xcq_set_density_wide(self.h, int(wide_min_nbe)),
+                    "xcq_set_density_wide")
 
     def set_density_occupancy(self, hiocc_max_nbe):
         N.xcq_check(self.lib.xcq_set_density_occupancy(self.h, int(hiocc_max_nbe)),

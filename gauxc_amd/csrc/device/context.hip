@@ -147,6 +147,7 @@ struct xcq_context_s {
   int32_t* d_qoff = nullptr;
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
   int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
+  int32_t dens128_min_nbe = 1 << 30;  // 128-wide (mixed) density tiles from this nbe on
   int32_t nqueues = 0;                                 // 0: one workgroup per item
   std::vector<double> cost_d, cost_s;                  // per-task predicted costs
   bool timing = false;
@@ -253,8 +254,10 @@ int build_work_lists(xcq_context_s* ctx) {
   const bool have_cost = ctx->cost_d.size() == (size_t)ctx->ntasks &&
                          ctx->cost_s.size() == (size_t)ctx->ntasks;
   for (auto& d : ctx->h_tasks) {
-    d.nt_d = d.nbe <= ctx->dens32_max_nbe ? 32 : 64;
-    d.dens_cls = d.nt_d == 32 ? 0 : (d.nbe <= ctx->dens_hiocc_max_nbe ? 1 : 2);
+    d.nt_d = d.nbe <= ctx->dens32_max_nbe ? 32 : (d.nbe >= ctx->dens128_min_nbe ? 128 : 64);
+    d.dens_cls = d.nt_d == 32    ? 0
+                 : d.nt_d == 128 ? 3
+                                 : (d.nbe <= ctx->dens_hiocc_max_nbe ? 1 : 2);
     d.nt_s = d.nbe <= ctx->syr2k32_max_nbe ? 32 : 64;
   }
   auto finish_range = [&](WorkRange& r, const std::vector<double>& cost) {
@@ -270,7 +273,7 @@ int build_work_lists(xcq_context_s* ctx) {
     }
   };
   for (auto& c : ctx->chunks) {
-    for (int v = 0; v < 3; ++v) {  // density classes (DevTask::dens_cls)
+    for (int v = 0; v < 4; ++v) {  // density classes (DevTask::dens_cls)
       c.dens[v].off = (int64_t)W.size();
       std::vector<double> cost;
       for (int32_t i = c.task_begin; i < c.task_end; ++i) {
@@ -286,7 +289,7 @@ int build_work_lists(xcq_context_s* ctx) {
       finish_range(c.dens[v], cost);
     }
     c.coll.off = c.dens[0].off;
-    c.coll.n = c.dens[0].n + c.dens[1].n + c.dens[2].n;
+    c.coll.n = c.dens[0].n + c.dens[1].n + c.dens[2].n + c.dens[3].n;
     c.coll.nq = 0;
     for (int v = 0; v < 2; ++v) {  // SYR2K tiles
       const int nt = v == 0 ? 32 : 64;
@@ -352,14 +355,14 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     // width (stage 1); stages 2 (functional) and 3 (Z) stay as empty
     // intervals so the stage table keeps its shape.
     XCQ_HIP(rec(ev[2]));
-    for (int k = 0; k < 3; ++k)
-      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, k == 1, precision, func, gga, fac,
-                               want_vxc)))
+    for (int k = 0; k < 4; ++k)
+      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : (k == 3 ? 128 : 64), k == 1,
+                               precision, func, gga, fac, want_vxc)))
         return rc;
     XCQ_HIP(rec(ev[3]));
     XCQ_HIP(rec(ev[4]));
     XCQ_HIP(rec(ev[5]));
-    launches += 4;
+    launches += 5;
     items += c.coll.n;
     if (want_vxc) {
       XCQ_HIP(rec(ev[6]));
@@ -823,12 +826,26 @@ int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32
   return XCQ_SUCCESS;
 }
 
-int xcq_get_tile_policy(xcq_context ctx, int32_t* out4) {
-  XCQ_REQUIRE(ctx && out4, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
-  out4[0] = ctx->dens32_max_nbe;
-  out4[1] = ctx->syr2k32_max_nbe;
-  out4[2] = ctx->dens_hiocc_max_nbe;
-  out4[3] = ctx->nqueues;
+int xcq_get_tile_policy(xcq_context ctx, int32_t* out5) {
+  XCQ_REQUIRE(ctx && out5, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  out5[0] = ctx->dens32_max_nbe;
+  out5[1] = ctx->syr2k32_max_nbe;
+  out5[2] = ctx->dens_hiocc_max_nbe;
+  out5[3] = ctx->nqueues;
+  out5[4] = ctx->dens128_min_nbe;
+  return XCQ_SUCCESS;
+}
+
+int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->dens128_min_nbe = wide_min_nbe;
+  if (ctx->ntasks == 0) return XCQ_SUCCESS;
+  int rc = build_work_lists(ctx);
+  if (rc) return rc;
+  XCQ_HIP(hipMemcpy(ctx->d_tasks, ctx->h_tasks.data(), ctx->h_tasks.size() * sizeof(DevTask),
+                    hipMemcpyHostToDevice));
   return XCQ_SUCCESS;
 }
 
@@ -898,7 +915,7 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us) {
     }
   };
   for (const auto& c : ctx->chunks) {
-    for (int v = 0; v < 3; ++v) add(c.dens[v], dens_us);
+    for (int v = 0; v < 4; ++v) add(c.dens[v], dens_us);
     for (int v = 0; v < 2; ++v) add(c.syr2k[v], syr2k_us);
   }
   return XCQ_SUCCESS;
@@ -977,6 +994,19 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
       }
     }
     if ((rc = xcq_set_density_occupancy(ctx, best_h))) return rc;
+    // ... and from which nbe on 128-wide tiles are used.
+    int32_t best_w = ctx->dens128_min_nbe;
+    double bw = 1e300;
+    for (int32_t c : {128, 192, 256, 384, 512, 1 << 30}) {
+      if ((rc = xcq_set_density_wide(ctx, c))) return rc;
+      double dm, sm;
+      if ((rc = measure(&dm, &sm, &tot))) return rc;
+      if (dm < bw) {
+        bw = dm;
+        best_w = c;
+      }
+    }
+    if ((rc = xcq_set_density_wide(ctx, best_w))) return rc;
   }
   // Persistent queues (flop-heuristic packing) against one workgroup per item.
   int dev_cus = 0;

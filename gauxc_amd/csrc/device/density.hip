@@ -371,16 +371,17 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
 // (lane = function c < NT).  P rows past nbe and functions past nbe are
 // read from a zero row, B^T rows past nbe at a clamped valid row (finite,
 // multiplied by those zero P rows).
-constexpr int kLoadsPerStage = 16;  // global_load_lds instructions per thread per K block
 
 // Two configurations of the kernel: the default one keeps three K blocks in
 // the ring at two workgroups per CU; the high-occupancy one (HIOCC) keeps
 // two blocks in a smaller ring and fits three workgroups per CU (LDS
 // 41 KiB, at most 168 registers), which hides the fold's and Z's memory
 // latency better on small tasks.
-template <bool HIOCC>
-constexpr int ring_stages() { return HIOCC ? 2 : 3; }
-constexpr int density_waves(int nt, bool hiocc) { return hiocc ? 3 : 2; }
+// 128-function tiles (large tasks) keep a two-block ring at two workgroups
+// per CU (57 KiB of LDS).
+template <int NT, bool HIOCC>
+constexpr int ring_stages() { return (HIOCC || NT == 128) ? 2 : 3; }
+constexpr int density_waves(int nt, bool hiocc) { return (hiocc && nt != 128) ? 3 : 2; }
 
 __device__ float g_zero_row[64];  // never written: zero-initialised device memory
 
@@ -394,13 +395,16 @@ __device__ float g_zero_row[64];  // never written: zero-initialised device memo
 __device__ __forceinline__ void wait_loads_barrier(int n) {
   switch (n) {
     case 0: asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); break;
-    case kLoadsPerStage:
-      asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
-      break;
-    default: asm volatile("s_waitcnt vmcnt(32)\n\ts_barrier" ::: "memory"); break;
+    case 16: asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory"); break;
+    case 24: asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory"); break;
+    case 32: asm volatile("s_waitcnt vmcnt(32)\n\ts_barrier" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(48)\n\ts_barrier" ::: "memory"); break;
   }
 }
-static_assert(kLoadsPerStage == 16, "wait_loads_barrier counts 16 loads per block");
+// Staging loads per thread per K block: eight B^T rows plus eight P_sub rows
+// per 64 columns of the tile.
+constexpr int loads_per_stage(int nt) { return 8 * (1 + (nt + 63) / 64); }
+static_assert(loads_per_stage(64) == 16 && loads_per_stage(128) == 24, "ring wait counts");
 
 // One dword per lane from global memory into the LDS row at lds_row (lane i
 // writes lds_row[i]); lds_row must be the same for the whole wave.  Issued
@@ -420,7 +424,9 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red) {
   constexpr int BI = NT / 32;
-  constexpr int kStages = ring_stages<HIOCC>();
+  constexpr int kStages = ring_stages<NT, HIOCC>();
+  constexpr int NPC = (NT + 63) / 64;  // 64-column chunks of a P_sub stage row
+  constexpr int kLoadsPerStage = loads_per_stage(NT);
   __shared__ float Pt[kStages][32][NT + 4];  // [slot][k][function]: A operand (P_sub)
   __shared__ float Bt[kStages][32][68];      // [slot][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
@@ -453,9 +459,14 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     for (int n0 = 0; n0 < nbe; n0 += NT) {
       // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
       // neighbouring lanes (neighbouring n) touch neighbouring columns.
-      const int ncol = n0 + lc;
-      const bool colok = lc < NT && ncol < nbe;
-      const float* Pc = Pf + bf[min(ncol, nbe - 1)];
+      const float* Pc[NPC];
+      bool colok[NPC];
+#pragma unroll
+      for (int h = 0; h < NPC; ++h) {
+        const int ncol = n0 + 64 * h + lc;
+        colok[h] = lc < NT && ncol < nbe;
+        Pc[h] = Pf + bf[min(ncol, nbe - 1)];
+      }
       auto issue = [&](int s) {
         const int slot = s % kStages, k0 = 32 * s + 8 * lg;
         // Eight consecutive map entries (uniform: one scalar load); entries
@@ -467,11 +478,13 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
         for (int e = 0; e < 8; ++e) {
           const int k = k0 + e;
           load_lds(bcol + min(k, nbe - 1) * ld, &Bt[slot][8 * lg + e][0]);
-          if (lc < NT) {
-            const float* src =
-                (k < nbe && colok) ? Pc + (int64_t)rowp[e] * ldp : g_zero_row + lc;
-            load_lds(src, &Pt[slot][8 * lg + e][0]);
-          }
+#pragma unroll
+          for (int h = 0; h < NPC; ++h)
+            if (lc < NT) {
+              const float* src =
+                  (k < nbe && colok[h]) ? Pc[h] + (int64_t)rowp[e] * ldp : g_zero_row + lc;
+              load_lds(src, &Pt[slot][8 * lg + e][64 * h]);
+            }
         }
       };
       double acc[BI][2][4];
@@ -555,6 +568,9 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
     const float* Pf = v.Pf32;
     if (nt == 32)
       gga ? go(k_density_mx<32, true, false>, Pf, scr) : go(k_density_mx<32, false, false>, Pf, scr);
+    else if (nt == 128)
+      gga ? go(k_density_mx<128, true, false>, Pf, scr)
+          : go(k_density_mx<128, false, false>, Pf, scr);
     else if (hiocc)
       gga ? go(k_density_mx<64, true, true>, Pf, scr) : go(k_density_mx<64, false, true>, Pf, scr);
     else

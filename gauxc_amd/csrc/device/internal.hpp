@@ -51,10 +51,11 @@ struct DevTask {
   int32_t npts_pad;   // npts rounded up to kPtBlock
   int32_t mol;        // molecule of the task
   int32_t ldp;        // nbf of the molecule = leading dimension of its P and VXC
-  int32_t nt_d;       // function tile of the density kernel (32 or 64)
+  int32_t nt_d;       // function tile of the density kernel (32, 64 or 128; the FP64
+                      // kernel runs 128-wide tasks with 64-wide tiles)
   int32_t nt_s;       // function tile of the SYR2K kernel (32 or 64)
   int32_t dens_cls;   // density work list: 0 = 32-wide, 1 = 64-wide at three
-                      // workgroups per CU, 2 = 64-wide at two
+                      // workgroups per CU, 2 = 64-wide at two, 3 = 128-wide
   int64_t pt_off;     // points (SoA) / weights
   int64_t sh_off;     // shell list
   int64_t bf_off;     // local -> molecule basis function map (nbe entries)
@@ -92,8 +93,8 @@ struct QueueArgs {
 struct Chunk {
   int32_t task_begin = 0, task_end = 0;
   int64_t scratch_elems = 0;  // elements of T (FP64 or FP32) used by B-family arrays
-  WorkRange coll;             // all point blocks (collocation) = dens[0..2] back to back
-  WorkRange dens[3];          // density blocks by class (DevTask::dens_cls)
+  WorkRange coll;             // all point blocks (collocation) = dens[0..3] back to back
+  WorkRange dens[4];          // density blocks by class (DevTask::dens_cls)
   WorkRange syr2k[2];         // SYR2K tiles (it >= jt) with 32- / 64-function tiles
 };
 

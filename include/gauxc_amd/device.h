@@ -111,9 +111,13 @@ int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32
  * workgroups per CU (two-block operand ring); larger ones keep the
  * three-block ring at two workgroups per CU.  0 (default) disables it. */
 int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
-/* Current settings: out4 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
- * nqueues}. */
-int xcq_get_tile_policy(xcq_context ctx, int32_t* out4);
+/* Mixed precision: density tasks with nbe >= wide_min_nbe (and above the
+ * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).
+ * Default: off (2^30). */
+int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe);
+/* Current settings: out5 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
+ * nqueues, wide_min_nbe}. */
+int xcq_get_tile_policy(xcq_context ctx, int32_t* out5);
 /* nqueues = 0: one workgroup per work item (large-first upload order,
  * XCD-interleaved).  nqueues > 0: persistent launches of nqueues workgroups
  * per stage list, each draining its own queue; work items are packed
@@ -134,10 +138,11 @@ int xcq_use_graph(xcq_context ctx, int enable);
 /* Autotuner: times the uploaded workload on the device and keeps the
  * fastest settings.  Sweeps the 32/64 tile thresholds of the density and
  * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}), for
- * mixed precision the density occupancy threshold over the same buckets,
- * then the persistent-queue count {0, 2, 4 per CU}; leaves the context on
- * the best and reports the thresholds and queue count (any output pointer
- * may be NULL). */
+ * mixed precision the density occupancy threshold over the same buckets and
+ * the 128-wide threshold over {128, 192, 256, 384, 512, off}, then the
+ * persistent-queue count {0, 2, 4 per CU}; leaves the context on the best
+ * and reports the 32-wide thresholds and queue count (any output pointer
+ * may be NULL; xcq_get_tile_policy returns the rest). */
 int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
                  int32_t* syr2k32_max_nbe, int32_t* nqueues, double* best_ms);
 

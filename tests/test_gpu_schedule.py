@@ -76,6 +76,29 @@ def test_density_occupancy_variants(gx, small):
     ctx.set_density_occupancy(0)
 
 
+def test_density_wide_tiles(gx, small):
+    """128-function density tiles (mixed) for the larger tasks: the same X
+    accumulation per element, the rho/grad-rho fold summed in another order,
+    so EXC/NEL agree to summation noise."""
+    bs, P, t = small
+    assert t["nbe"].max() > 128  # some tasks span both 64-column halves of a tile
+    ctx = _ctx(gx, bs, t, P)
+    ctx.set_tile_policy(48, 96)
+    e0, n0, v0 = _run(ctx, "mixed")
+    for w, h in [(100, 0), (129, 100000), (0, 0)]:
+        _flush(f"density 128-wide from nbe {w}, high occupancy to {h}")
+        ctx.set_density_wide(w)
+        ctx.set_density_occupancy(h)
+        assert ctx.tile_policy()[4] == w
+        e1, n1, v1 = _run(ctx, "mixed")
+        assert abs(e1 - e0) <= 1e-11 * abs(e0)
+        assert abs(n1 - n0) <= 1e-11 * abs(n0)
+        assert np.abs(v1 - v0).max() <= 1e-12
+        ctx.exc_vxc("PBE", "fp64", True)  # FP64 runs 128-wide tasks on 64-wide tiles
+    ctx.set_density_wide(1 << 30)
+    ctx.set_density_occupancy(0)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_queued_schedule_matches_direct(gx, small, prec):
     """Persistent queues (heuristic and explicit costs) reproduce the
