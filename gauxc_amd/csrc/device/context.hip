@@ -161,6 +161,7 @@ struct xcq_context_s {
   double* d_red = nullptr;
   double* d_out2 = nullptr;
   double* d_corr = nullptr;  // [nmol][kCorrBlocks] partials of the P32 correction
+  double* d_rpart = nullptr;  // [nmol][kRedBlocks][2] partials of EXC/NEL
   int last_want_vxc = 0;     // the last evaluation returns VXC
   int last_vxc_eval = 0;     // the last evaluation computed VXC
 
@@ -223,6 +224,7 @@ struct xcq_context_s {
     dfree(d_vxc);
     dfree(d_out2);
     dfree(d_corr);
+    dfree(d_rpart);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -391,7 +393,7 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     XCQ_HIP(rec(ctx->ev[11]));
     launches += corr ? 2 : 1;
   }
-  if ((rc = launch_reduce_sums(s, nmol, ctx->d_mol_blk, ctx->d_red,
+  if ((rc = launch_reduce_sums(s, nmol, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart,
                                corr ? ctx->d_corr : nullptr, fac, ctx->d_out2)))
     return rc;
   ctx->counters[0] = launches + 1;
@@ -586,12 +588,14 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
     dfree(ctx->d_vxc);
     dfree(ctx->d_out2);
     dfree(ctx->d_corr);
+    dfree(ctx->d_rpart);
     dfree(ctx->d_mol_n);
     dfree(ctx->d_mol_off);
     if ((rc = dalloc(&ctx->d_P, (size_t)p_total))) return rc;
     if ((rc = dalloc(&ctx->d_vxc, (size_t)p_total))) return rc;
     if ((rc = dalloc(&ctx->d_out2, (size_t)2 * nmol))) return rc;
     if ((rc = dalloc(&ctx->d_corr, (size_t)kCorrBlocks * nmol))) return rc;
+    if ((rc = dalloc(&ctx->d_rpart, (size_t)2 * kRedBlocks * nmol))) return rc;
     if ((rc = dalloc(&ctx->d_mol_n, (size_t)nmol))) return rc;
     if ((rc = dalloc(&ctx->d_mol_off, (size_t)nmol))) return rc;
     std::vector<int32_t> mn(nmol);

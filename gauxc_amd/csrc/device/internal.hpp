@@ -163,16 +163,19 @@ int launch_fill(hipStream_t s, double* p, int64_t n);
 // Symmetrise the VXC of nmol molecules: mol_n[m] = nbf, mol_off[m] = offset.
 int launch_symmetrize(hipStream_t s, int32_t nmol, int32_t max_n, const int32_t* mol_n,
                       const int64_t* mol_off, double* vxc);
-// Partials per molecule of the FP32-rounding correction of EXC.
-constexpr int kCorrBlocks = 32;
+// Partials per molecule of the FP32-rounding correction of EXC and of the
+// EXC/NEL block sums.
+constexpr int kCorrBlocks = 128;
+constexpr int kRedBlocks = 64;
 // part[m * kCorrBlocks + b] = partial sums of (P - P32) . VXC over molecule m.
 int launch_p32_correction(hipStream_t s, int32_t nmol, const int32_t* mol_n,
                           const int64_t* mol_off, const double* P, const float* Pf,
                           const double* vxc, double* part);
 // Per-molecule EXC/NEL: red pairs [blk_off[m], blk_off[m+1]) -> out2[2m], out2[2m+1],
-// plus cfac * (the molecule's correction partials) on EXC when corr is not null.
+// plus cfac * (the molecule's correction partials) on EXC when corr is not null;
+// part holds 2 * kRedBlocks doubles per molecule.
 int launch_reduce_sums(hipStream_t s, int32_t nmol, const int64_t* blk_off, const double* red,
-                       const double* corr, double cfac, double* out2);
+                       double* part, const double* corr, double cfac, double* out2);
 int launch_functional_points(hipStream_t s, int func, int64_t n, const double* rho,
                              const double* sigma, double* eps, double* vrho, double* vsigma);
 

@@ -86,7 +86,7 @@ int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* r
 // where P32 is the FP32 copy the density stage multiplies with.  Workgroup
 // (b, m) sums elements b*256 + tid + k*kCorrBlocks*256 of molecule m's n x n
 // matrices and writes its partial to part[m * kCorrBlocks + b];
-// k_reduce_sums adds the partials of each molecule in a fixed order.
+// k_reduce_final adds the partials of each molecule in a fixed order.
 __global__ __launch_bounds__(256) void k_p32_correction(const int32_t* __restrict__ mol_n,
                                                         const int64_t* __restrict__ mol_off,
                                                         const double* __restrict__ P,
@@ -120,17 +120,19 @@ int launch_p32_correction(hipStream_t s, int32_t nmol, const int32_t* mol_n,
 }
 
 // Per-molecule sums of the (e, n) block pairs in a fixed order:
-// deterministic EXC and NEL.  Workgroup m reduces blocks
-// [blk_off[m], blk_off[m+1]) and, when corr is given, adds
-// cfac * sum_b corr[m * kCorrBlocks + b] to EXC.
-__global__ __launch_bounds__(256) void k_reduce_sums(const int64_t* __restrict__ blk_off,
-                                                     const double* __restrict__ red,
-                                                     const double* __restrict__ corr,
-                                                     double cfac, double* __restrict__ out) {
+// deterministic EXC and NEL.  First pass: workgroup (b, m) sums pairs
+// blk_off[m] + b*256 + tid + k*kRedBlocks*256 into part[m][b]; second pass:
+// one wave per molecule adds the kRedBlocks partials (and, when corr is
+// given, cfac times the molecule's kCorrBlocks correction partials) in
+// order.
+__global__ __launch_bounds__(256) void k_reduce_partials(const int64_t* __restrict__ blk_off,
+                                                         const double* __restrict__ red,
+                                                         double* __restrict__ part) {
   __shared__ double se[256], sn[256];
-  const int m = blockIdx.x;
+  const int m = blockIdx.y;
   double e = 0., n = 0.;
-  for (int64_t i = blk_off[m] + threadIdx.x; i < blk_off[m + 1]; i += 256) {
+  for (int64_t i = blk_off[m] + (int64_t)blockIdx.x * 256 + threadIdx.x; i < blk_off[m + 1];
+       i += (int64_t)kRedBlocks * 256) {
     e += red[2 * i];
     n += red[2 * i + 1];
   }
@@ -145,19 +147,34 @@ __global__ __launch_bounds__(256) void k_reduce_sums(const int64_t* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double c = 0.;
-    if (corr)
-      for (int b = 0; b < kCorrBlocks; ++b) c += corr[(int64_t)m * kCorrBlocks + b];
-    out[2 * m] = se[0] + cfac * c;
-    out[2 * m + 1] = sn[0];
+    part[2 * ((int64_t)m * kRedBlocks + blockIdx.x)] = se[0];
+    part[2 * ((int64_t)m * kRedBlocks + blockIdx.x) + 1] = sn[0];
   }
 }
 
+__global__ __launch_bounds__(64) void k_reduce_final(const double* __restrict__ part,
+                                                     const double* __restrict__ corr, double cfac,
+                                                     double* __restrict__ out) {
+  const int m = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  double e = 0., n = 0., c = 0.;
+  for (int b = 0; b < kRedBlocks; ++b) {
+    e += part[2 * ((int64_t)m * kRedBlocks + b)];
+    n += part[2 * ((int64_t)m * kRedBlocks + b) + 1];
+  }
+  if (corr)
+    for (int b = 0; b < kCorrBlocks; ++b) c += corr[(int64_t)m * kCorrBlocks + b];
+  out[2 * m] = e + cfac * c;
+  out[2 * m + 1] = n;
+}
+
 int launch_reduce_sums(hipStream_t s, int32_t nmol, const int64_t* blk_off, const double* red,
-                       const double* corr, double cfac, double* out2) {
+                       double* part, const double* corr, double cfac, double* out2) {
   if (nmol == 0) return XCQ_SUCCESS;
-  hipLaunchKernelGGL(k_reduce_sums, dim3((unsigned)nmol), dim3(256), 0, s, blk_off, red, corr,
-                     cfac, out2);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(kRedBlocks, (unsigned)nmol), dim3(256), 0, s,
+                     blk_off, red, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)nmol), dim3(64), 0, s, part, corr, cfac,
+                     out2);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
