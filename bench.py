@@ -47,7 +47,7 @@ STAGE_KERNELS = {
     ("density", "mixed"): ["k_density_mx<32, true, false>", "k_density_mx<64, true, true>",
                            "k_density_mx<64, true, false>", "k_density_mx<128, true, false>"],
     ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
-    ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32>", "k_syr2k_mx<64>"],
+    ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32, 1>", "k_syr2k_mx<64, 1>"],
 }
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_c1_24water.json")
 
