@@ -127,22 +127,107 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
   }
 }
 
+// Fold through LDS (mixed kernel, tiles of up to 64 functions): the X^T
+// tile (rows = functions, FP64) is written over the operand ring, then
+// thread t takes points 4(t%16)..4(t%16)+3 of rows t/16, t/16 + 16, ... and
+// reads B, its residual and grad B there with 16-byte vector loads (four
+// points per load, against one in the accumulator layout).  Partials are
+// summed over the wave's four row groups by shuffles; lanes 0..15 of wave w
+// add them to dens[w].  The caller has passed a barrier after its last
+// reads of the ring; this function ends after its last reads of it.
+constexpr int kXLd = 66;  // LDS row stride of the X^T tile in doubles
+
+template <int NT, int BI, bool GGA>
+__device__ __forceinline__ void fold_tile_lds(const double (&x)[BI][2][4], double* xt, int m0,
+                                              int n0, int wm, int wn, int lane, int tid, int nbe,
+                                              int ld, int plane, const float* Bm,
+                                              double (*dens)[64][4]) {
+#pragma unroll
+  for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        xt[(wm * 16 * BI + bi * 16 + 4 * (lane >> 4) + r) * kXLd + wn * 32 + bj * 16 +
+           (lane & 15)] = x[bi][bj][r];
+  __syncthreads();
+  const int q4 = 4 * (tid & 15);
+  const int wave = tid >> 6;
+  double s[4][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[e][c] = 0.;
+  const unsigned short* lo = reinterpret_cast<const unsigned short*>(Bm + 5 * (int64_t)plane);
+#pragma unroll
+  for (int k = 0; k < NT / 16; ++k) {
+    const int f = (tid >> 4) + 16 * k;
+    // Rows past nbe carry X = 0; their (clamped) B values contribute nothing.
+    const int o = min(n0 + f, nbe - 1) * ld + m0 + q4;
+    const float4 b = *reinterpret_cast<const float4*>(Bm + o);
+    const uint2 l = *reinterpret_cast<const uint2*>(lo + o);
+    float4 gxv, gyv, gzv;
+    if constexpr (GGA) {
+      gxv = *reinterpret_cast<const float4*>(Bm + plane + o);
+      gyv = *reinterpret_cast<const float4*>(Bm + 2 * plane + o);
+      gzv = *reinterpret_cast<const float4*>(Bm + 3 * plane + o);
+    }
+    const double* xr = xt + f * kXLd + q4;
+    const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const unsigned w = e < 2 ? l.x : l.y;
+      const double bv = (double)bb[e] +
+                        (double)__builtin_bit_cast(float, (e & 1) ? (w & 0xffff0000u) : (w << 16));
+      const double xv = xr[e];
+      s[e][0] += bv * xv;
+      if constexpr (GGA) {
+        const float gxa[4] = {gxv.x, gxv.y, gxv.z, gxv.w};
+        const float gya[4] = {gyv.x, gyv.y, gyv.z, gyv.w};
+        const float gza[4] = {gzv.x, gzv.y, gzv.z, gzv.w};
+        s[e][1] += (double)gxa[e] * xv;
+        s[e][2] += (double)gya[e] * xv;
+        s[e][3] += (double)gza[e] * xv;
+      }
+    }
+  }
+  // Lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same points (other rows).
+#pragma unroll
+  for (int off = 16; off < 64; off <<= 1)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int c = 0; c < (GGA ? 4 : 1); ++c) s[e][c] += __shfl_xor(s[e][c], off);
+  if ((tid & 63) < 16)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int c = 0; c < (GGA ? 4 : 1); ++c) dens[wave][q4 + e][c] += s[e][c];
+}
+
 // Functional on the block's 64 points (threads 0..63 = wave 0).  Writes the
 // Z coefficients (a, cx, cy, cz) of each point to zc and the block's
 // (EXC, NEL) partial sums to red.
+template <int ND = 2>
 __device__ __forceinline__ void finish_points(const DevTask& t, int pb, int tid,
                                               double (*dens)[64][4], double (*zc)[64],
                                               const double* __restrict__ weights, int func,
                                               int gga, double* __restrict__ red) {
   if (tid >= 64) return;
   const int p = pb * kPtBlock + tid;
-  const double rho = dens[0][tid][0] + dens[1][tid][0];
-  double gx = 0., gy = 0., gz = 0.;
-  if (gga) {
-    gx = 2.0 * (dens[0][tid][1] + dens[1][tid][1]);
-    gy = 2.0 * (dens[0][tid][2] + dens[1][tid][2]);
-    gz = 2.0 * (dens[0][tid][3] + dens[1][tid][3]);
+  double rho = 0., gx = 0., gy = 0., gz = 0.;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    rho += dens[d][tid][0];
+    if (gga) {
+      gx += dens[d][tid][1];
+      gy += dens[d][tid][2];
+      gz += dens[d][tid][3];
+    }
   }
+  gx *= 2.0;
+  gy *= 2.0;
+  gz *= 2.0;
   double e = 0., n = 0., za = 0., zx = 0., zy = 0., zz = 0.;
   if (p < t.npts) {
     const double w = weights[t.pt_off + p];
@@ -427,9 +512,16 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   constexpr int kStages = ring_stages<NT, HIOCC>();
   constexpr int NPC = (NT + 63) / 64;  // 64-column chunks of a P_sub stage row
   constexpr int kLoadsPerStage = loads_per_stage(NT);
-  __shared__ float Pt[kStages][32][NT + 4];  // [slot][k][function]: A operand (P_sub)
-  __shared__ float Bt[kStages][32][68];      // [slot][k][point]:    B operand (B^T)
-  __shared__ double dens[2][64][4];
+  // Operand ring: P_sub stages [slot][k][function] (A operand) then B^T
+  // stages [slot][k][point] (B operand); after a tile's last K block the
+  // X^T tile of the LDS fold (NT <= 64) is written over it.
+  constexpr int kPtFloats = kStages * 32 * (NT + 4);
+  constexpr int kRingFloats = kPtFloats + kStages * 32 * 68;
+  static_assert(NT > 64 || NT * kXLd * 2 <= kRingFloats, "X^T tile must fit the ring");
+  __shared__ __attribute__((aligned(16))) float ring[kRingFloats];
+  float(*Pt)[32][NT + 4] = reinterpret_cast<float(*)[32][NT + 4]>(ring);
+  float(*Bt)[32][68] = reinterpret_cast<float(*)[32][68]>(ring + kPtFloats);
+  __shared__ double dens[4][64][4];
   __shared__ double zc[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
@@ -448,7 +540,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     const float* Pf = Pfall + t.p_off;
     float* Bm = scratch + t.b_off_mx;
     const int32_t* bf = bfmap + t.bf_off;
-    for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    for (int i = tid; i < 4 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
     // Stores of the previous item may still count against vmcnt; the ring
     // waits below assume only this item's staging loads are outstanding.
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -533,12 +625,18 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[a][b][r] *= fac;
-      fold_tile<float, BI, GGA>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
-      // The next tile's first loads target slots 0 and 1; every wave has
-      // finished reading them (last K blocks of this tile) at this barrier.
+      if constexpr (NT <= 64) {
+        __syncthreads();  // every wave is done with the ring's last K block
+        fold_tile_lds<NT, BI, GGA>(acc, reinterpret_cast<double*>(ring), m0, n0, wm, wn, lane,
+                                   tid, nbe, ld, plane, Bm, dens);
+      } else {
+        fold_tile<float, BI, GGA>(acc, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
+      }
+      // The next tile's first loads overwrite the ring; every wave has
+      // finished reading it (last K blocks, or the X^T tile) at this barrier.
       __syncthreads();
     }
-    finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
+    finish_points<4>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
       write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm);
