@@ -421,6 +421,10 @@ class DeviceContext:
                     "xcq_get_tile_policy")
         return tuple(int(x) for x in out)
 
+    def set_vxc_drop_tolerance(self, tol):
+        N.xcq_check(self.lib.xcq_set_vxc_drop_tolerance(self.h, float(tol)),
+                    "xcq_set_vxc_drop_tolerance")
+
     def set_density_wide(self, wide_min_nbe):
         N.xcq_check(self.lib.xcq_set_density_wide(self.h, int(wide_min_nbe)),
                     "xcq_set_density_wide")

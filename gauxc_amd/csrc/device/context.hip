@@ -148,6 +148,7 @@ struct xcq_context_s {
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
   int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
   int32_t dens128_min_nbe = 1 << 30;  // 128-wide (mixed) density tiles from this nbe on
+  double vxc_drop = kDefaultVxcDrop;  // see xcq_set_vxc_drop_tolerance
   int32_t nqueues = 0;                                 // 0: one workgroup per item
   std::vector<double> cost_d, cost_s;                  // per-task predicted costs
   bool timing = false;
@@ -235,7 +236,8 @@ CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream,  c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap,
                  c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
                  c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
-                 c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch};
+                 c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch,
+                 c->vxc_drop};
 }
 
 // Heuristic per-item costs (used when no predicted task costs are set):
@@ -837,6 +839,17 @@ int xcq_get_tile_policy(xcq_context ctx, int32_t* out5) {
   out5[2] = ctx->dens_hiocc_max_nbe;
   out5[3] = ctx->nqueues;
   out5[4] = ctx->dens128_min_nbe;
+  return XCQ_SUCCESS;
+}
+
+int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(tol >= 0.0 && tol <= 1e-10, XCQ_ERROR_INVALID_ARGUMENT,
+              "VXC drop tolerance must lie in [0, 1e-10]");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->vxc_drop = tol;
+  ctx->drop_graph();  // the captured launches carry the old value
   return XCQ_SUCCESS;
 }
 

@@ -33,7 +33,7 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __restrict__ tasks,
                                                const int32_t* __restrict__ bfmap,
                                                const double* __restrict__ scratch,
-                                               double* __restrict__ vxc_all) {
+                                               double* __restrict__ vxc_all, double drop) {
   constexpr int BI = NT / 32, NQ = NT / 16, LP = NT + 4;
   __shared__ double Bi[16][LP];
   __shared__ double Zi[16][LP];
@@ -148,7 +148,8 @@ __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __re
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = i0 + wm * 16 * BI + bi * 16 + 4 * r + (lane >> 4);
-            if (i < nbe && (!diag || i >= j))
+            // Negligible contributions are not added (NaN still is).
+            if (i < nbe && (!diag || i >= j) && !(fabs(acc[bi][bj][r]) < drop))
               atomicAdd(vxc + (int64_t)bf[i] * ldv + rj, acc[bi][bj][r]);
           }
       }
@@ -166,10 +167,10 @@ int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision) {
   const double* scr = static_cast<const double*>(v.scratch);
   if (nt == 32)
     hipLaunchKernelGGL(k_syr2k<32>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc);
+                       v.bfmap, scr, v.vxc, v.vxc_drop);
   else
     hipLaunchKernelGGL(k_syr2k<64>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc);
+                       v.bfmap, scr, v.vxc, v.vxc_drop);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

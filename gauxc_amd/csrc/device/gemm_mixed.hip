@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa
                                                   const DevTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ bfmap,
                                                   const float* __restrict__ scratch,
-                                                  double* __restrict__ vxc_all) {
+                                                  double* __restrict__ vxc_all, double drop) {
   constexpr int BI = NT / 32;
   __shared__ __attribute__((aligned(16))) unsigned short Bi[4][NT][kRow];
   __shared__ __attribute__((aligned(16))) unsigned short Zi[3][NT][kRow];
@@ -268,7 +268,8 @@ __global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = i0 + wm * 16 * BI + bi * 16 + 4 * (lane >> 4) + r;
-            if (i < nbe && (!diag || i >= j))
+            // Negligible contributions are not added (NaN still is).
+            if (i < nbe && (!diag || i >= j) && !(fabs(acc[bi][bj][r]) < drop))
               atomicAdd(vxc + (int64_t)bf[i] * ldv + rj, acc[bi][bj][r]);
           }
       }
@@ -283,10 +284,10 @@ int launch_syr2k_mixed(const CtxView& v, const WorkRange& r, int nt) {
   const float* scr = static_cast<const float*>(v.scratch);
   if (nt == 32)
     hipLaunchKernelGGL(k_syr2k_mx<32>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc);
+                       v.bfmap, scr, v.vxc, v.vxc_drop);
   else
     hipLaunchKernelGGL(k_syr2k_mx<64>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc);
+                       v.bfmap, scr, v.vxc, v.vxc_drop);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

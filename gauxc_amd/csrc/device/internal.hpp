@@ -34,6 +34,13 @@ void set_error(const std::string& msg);
 // ---- device data ------------------------------------------------------------
 constexpr int kMaxL = 4;          // highest angular momentum the kernels evaluate
 constexpr int kPtBlock = 64;      // points per collocation / density block (npts_pad unit)
+// Default |v| below which a task's B^T Z contribution is not scattered
+// into VXC: far below the FP64 resolution of any VXC element above 1e-2,
+// and its absolute effect (at most this times the number of tasks sharing a
+// function pair) far below the parity bars.  Most scatter-adds of large,
+// diffuse systems are of such negligible pairs (about three in four on a
+// 150-water cluster), and each is an FP64 atomic.
+constexpr double kDefaultVxcDrop = 1e-18;
 constexpr int kBfPad = 32;        // zero entries after the last task's basis map: K blocks
                                   // of 32 rows read up to 31 entries past a task's nbe
 
@@ -114,6 +121,7 @@ struct CtxView {
   double* vxc;
   double* red;
   void* scratch;
+  double vxc_drop;  // B^T Z contributions with |v| < vxc_drop are not added to VXC
 };
 
 inline QueueArgs queue_args(const CtxView& v, const WorkRange& r) {

@@ -115,6 +115,11 @@ int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
  * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).
  * Default: off (2^30). */
 int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe);
+/* B^T Z contributions with |v| < tol are not scattered into VXC (each
+ * scatter-add is an FP64 atomic; on large diffuse systems most are of
+ * negligible function pairs).  Default 1e-18; 0 adds every contribution.
+ * Allowed range [0, 1e-10].  No GauXC counterpart (GauXC adds all). */
+int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol);
 /* Current settings: out5 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
  * nqueues, wide_min_nbe}. */
 int xcq_get_tile_policy(xcq_context ctx, int32_t* out5);

@@ -76,6 +76,22 @@ def test_density_occupancy_variants(gx, small):
     ctx.set_density_occupancy(0)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_vxc_drop_tolerance(gx, small, prec):
+    """Skipping scatter-adds below the drop tolerance (default 1e-18) leaves
+    EXC/NEL unchanged and VXC within 1e-13 of adding every contribution."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    e0, n0, v0 = _run(ctx, prec)
+    ctx.set_vxc_drop_tolerance(0.0)
+    e1, n1, v1 = _run(ctx, prec)
+    assert e1 == e0 and n1 == n0
+    assert np.abs(v1 - v0).max() <= 1e-13
+    ctx.set_vxc_drop_tolerance(1e-18)
+    with pytest.raises(gx.GauXCError):
+        ctx.set_vxc_drop_tolerance(-1.0)
+
+
 def test_density_wide_tiles(gx, small):
     """128-function density tiles (mixed) for the larger tasks: the same X
     accumulation per element, the rho/grad-rho fold summed in another order,
