@@ -163,6 +163,8 @@ struct xcq_context_s {
   double* d_out2 = nullptr;
   double* d_corr = nullptr;  // [nmol][kCorrBlocks] partials of the P32 correction
   double* d_rpart = nullptr;  // [nmol][kRedBlocks][2] partials of EXC/NEL
+  double* d_tmax = nullptr;   // bound table: one {float, float} per function granule
+  int64_t tmax_total = 0;
   int last_want_vxc = 0;     // the last evaluation returns VXC
   int last_vxc_eval = 0;     // the last evaluation computed VXC
 
@@ -198,6 +200,8 @@ struct xcq_context_s {
     dfree(d_work);
     dfree(d_qoff);
     dfree(d_red);
+    dfree(d_tmax);
+    tmax_total = 0;
     dfree(d_tstamp);
     if (d_scratch) (void)hipFree(d_scratch);
     d_scratch = nullptr;
@@ -237,7 +241,7 @@ CtxView view_of(xcq_context_s* c) {
                  c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
                  c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
                  c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch,
-                 c->vxc_drop};
+                 c->vxc_drop, reinterpret_cast<unsigned*>(c->d_tmax)};
 }
 
 // Heuristic per-item costs (used when no predicted task costs are set):
@@ -340,6 +344,7 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
   const double fac = func == 0 ? 1.0 : 2.0;
   int rc;
   if (want_vxc && (rc = launch_fill(s, ctx->d_vxc, ctx->p_total))) return rc;
+  if (want_vxc && (rc = launch_fill(s, ctx->d_tmax, ctx->tmax_total))) return rc;
   int64_t launches = want_vxc ? 1 : 0, items = 0;
   auto rec = [&](hipEvent_t e) { return events ? hipEventRecord(e, s) : hipSuccess; };
   for (size_t ci = 0; ci < ctx->chunks.size(); ++ci) {
@@ -692,6 +697,11 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
       next[d.mol] += d.npts_pad / kPtBlock;
     }
   }
+  int64_t tmax_total = 0;
+  for (auto& d : ht) {
+    d.tmax_off = tmax_total;
+    tmax_total += (d.nbe + kGran - 1) / kGran;
+  }
   const int64_t pblocks = mol_blocks[nmol];
 
   // Chunk planning against the arena.
@@ -766,6 +776,9 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
   if ((rc = dalloc(&ctx->d_bfmap, bfmap.size()))) return rc;
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
+  dfree(ctx->d_tmax);
+  if ((rc = dalloc(&ctx->d_tmax, (size_t)std::max<int64_t>(tmax_total, 1)))) return rc;
+  ctx->tmax_total = tmax_total;
   dfree(ctx->d_mol_blk);
   if ((rc = dalloc(&ctx->d_mol_blk, mol_blocks.size()))) return rc;
   ctx->scratch_bytes = (size_t)max_bytes;

@@ -267,9 +267,15 @@ using Vec4 = T __attribute__((ext_vector_type(4)));
 // flight instead of waiting on each row in turn (4 rows of FP32, 2 of FP64,
 // which fit the register budget of the GEMM phase).  Row offsets are clamped
 // to a valid row; rows past nbe are not stored.
+// Also records, per granule of 32 functions, max |B| and max |Z| over the
+// block's points (B with its residual; float bits, so unsigned max orders
+// them): LDS atomics into smax, then one device atomic per granule into the
+// task's bound table (direct device atomics when the task has more granules
+// than smax holds).  The B^T Z stage skips a tile pair when
+// npts * (max|B_i| max|Z_j| + max|Z_i| max|B_j|) is below the drop tolerance.
 template <typename T, bool GGA, int kZRows = sizeof(T) == 4 ? 4 : 2>
 __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, double (*zc)[64],
-                                        T* Bm) {
+                                        T* Bm, unsigned (*smax)[2], unsigned* tmax) {
   const int q = tid & 15, g = tid >> 4;
   const int ld = t.npts_pad, nbe = t.nbe, plane = nbe * ld;
   double a[4], cx[4], cy[4], cz[4];
@@ -316,7 +322,36 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
         out[e] = (T)zv;
       }
       *reinterpret_cast<Vec4<T>*>(B + 4 * plane + j * ld) = out;
+      if (tmax) {
+        float mb = 0.f, mz = 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          mb = fmaxf(mb, fabsf((float)b[u][e]));
+          mz = fmaxf(mz, fabsf((float)out[e]));
+        }
+        // The 16 lanes of this row (same g) hold its 64 points.
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+          mb = fmaxf(mb, __shfl_xor(mb, off));
+          mz = fmaxf(mz, __shfl_xor(mz, off));
+        }
+        if (q == 0) {
+          const int gr = j / kGran;
+          if (gr < kGranLds) {
+            atomicMax(&smax[gr][0], __float_as_uint(mb));
+            atomicMax(&smax[gr][1], __float_as_uint(mz));
+          } else {
+            atomicMax(tmax + 2 * gr, __float_as_uint(mb));
+            atomicMax(tmax + 2 * gr + 1, __float_as_uint(mz));
+          }
+        }
+      }
     }
+  }
+  if (tmax) {
+    __syncthreads();
+    const int ng = min((nbe + kGran - 1) / kGran, kGranLds);
+    if (tid < 2 * ng) atomicMax(tmax + tid, smax[tid >> 1][tid & 1]);
   }
 }
 
@@ -333,12 +368,13 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
-    double* __restrict__ red) {
+    double* __restrict__ red, unsigned* __restrict__ tmax) {
   constexpr int BI = NT / 32;
   __shared__ double Pt[2][16][NT + 4];  // [buf][k][function]: A operand (P_sub)
   __shared__ double Bt[2][16][kPad64];  // [buf][k][point]:    B operand (B^T)
   __shared__ double dens[2][64][4];
   __shared__ double zc[4][64];
+  __shared__ unsigned smax[kGranLds][2];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int lc = lane, lq = wave;  // loader column / K quad
@@ -357,6 +393,7 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
     double* Bm = scratch + t.b_off;
     const int32_t* bf = bfmap + t.bf_off;
     for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    if (tid < 2 * kGranLds) (&smax[0][0])[tid] = 0u;
     __syncthreads();
     const double* bcol = Bm + m0 + lc;
     const int nks = (nbe + 15) / 16;
@@ -426,7 +463,7 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
     finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<double, GGA>(t, m0, tid, zc, Bm);
+      write_z<double, GGA>(t, m0, tid, zc, Bm, smax, tmax ? tmax + 2 * t.tmax_off : nullptr);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -507,7 +544,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const float* __restrict__ Pfall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
-    double* __restrict__ red) {
+    double* __restrict__ red, unsigned* __restrict__ tmax) {
   constexpr int BI = NT / 32;
   constexpr int kStages = ring_stages<NT, HIOCC>();
   constexpr int NPC = (NT + 63) / 64;  // 64-column chunks of a P_sub stage row
@@ -523,6 +560,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   float(*Bt)[32][68] = reinterpret_cast<float(*)[32][68]>(ring + kPtFloats);
   __shared__ double dens[4][64][4];
   __shared__ double zc[4][64];
+  __shared__ unsigned smax[kGranLds][2];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int lc = lane, lg = wave;
@@ -541,6 +579,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     float* Bm = scratch + t.b_off_mx;
     const int32_t* bf = bfmap + t.bf_off;
     for (int i = tid; i < 4 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    if (tid < 2 * kGranLds) (&smax[0][0])[tid] = 0u;
     // Stores of the previous item may still count against vmcnt; the ring
     // waits below assume only this item's staging loads are outstanding.
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -639,7 +678,8 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     finish_points<4>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm);
+      write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm, smax,
+                                        tmax ? tmax + 2 * t.tmax_off : nullptr);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -652,8 +692,11 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
   auto go = [&](auto kern, auto P, auto scr) {
+    // The bound table is filled only when VXC is computed and contributions
+    // may be dropped.
+    unsigned* tm = (want_z && v.vxc_drop > 0.0) ? v.tmax : nullptr;
     hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, P, scr, v.w, func,
-                       fac, want_z, v.red);
+                       fac, want_z, v.red, tm);
   };
   if (precision == XCQ_PREC_FP64) {
     double* scr = static_cast<double*>(v.scratch);

@@ -33,7 +33,8 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __restrict__ tasks,
                                                const int32_t* __restrict__ bfmap,
                                                const double* __restrict__ scratch,
-                                               double* __restrict__ vxc_all, double drop) {
+                                               double* __restrict__ vxc_all, double drop,
+    const unsigned* __restrict__ tmax) {
   constexpr int BI = NT / 32, NQ = NT / 16, LP = NT + 4;
   __shared__ double Bi[16][LP];
   __shared__ double Zi[16][LP];
@@ -50,6 +51,12 @@ __global__ __launch_bounds__(256) void k_syr2k(QueueArgs qa, const DevTask* __re
     const DevTask t = tasks[wi.task];
     const int i0 = wi.a * NT, j0 = wi.b * NT;
     const bool diag = wi.a == wi.b;
+    // Tile pairs whose every element would be dropped are skipped whole
+    // (uniform across the workgroup).
+    if (tile_pair_negligible(t, tmax, i0, j0, NT, drop)) {
+      stamp_item(qa, it, t0);
+      continue;
+    }
     const int ld = t.npts_pad;
     const int nbe = t.nbe;
     const int plane = nbe * ld;
@@ -167,10 +174,10 @@ int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision) {
   const double* scr = static_cast<const double*>(v.scratch);
   if (nt == 32)
     hipLaunchKernelGGL(k_syr2k<32>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc, v.vxc_drop);
+                       v.bfmap, scr, v.vxc, v.vxc_drop, v.vxc_drop > 0.0 ? v.tmax : nullptr);
   else
     hipLaunchKernelGGL(k_syr2k<64>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc, v.vxc_drop);
+                       v.bfmap, scr, v.vxc, v.vxc_drop, v.vxc_drop > 0.0 ? v.tmax : nullptr);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

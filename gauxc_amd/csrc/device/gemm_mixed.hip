@@ -91,7 +91,8 @@ __global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa
                                                   const DevTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ bfmap,
                                                   const float* __restrict__ scratch,
-                                                  double* __restrict__ vxc_all, double drop) {
+                                                  double* __restrict__ vxc_all, double drop,
+    const unsigned* __restrict__ tmax) {
   constexpr int BI = NT / 32;
   __shared__ __attribute__((aligned(16))) unsigned short Bi[4][NT][kRow];
   __shared__ __attribute__((aligned(16))) unsigned short Zi[3][NT][kRow];
@@ -110,6 +111,12 @@ __global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa
     const DevTask t = tasks[wi.task];
     const int i0 = wi.a * NT, j0 = wi.b * NT;
     const bool diag = wi.a == wi.b;
+    // Tile pairs whose every element would be dropped are skipped whole
+    // (uniform across the workgroup).
+    if (tile_pair_negligible(t, tmax, i0, j0, NT, drop)) {
+      stamp_item(qa, it, t0);
+      continue;
+    }
     const int ld = t.npts_pad;
     const int nbe = t.nbe;
     const int plane = nbe * ld;
@@ -284,10 +291,10 @@ int launch_syr2k_mixed(const CtxView& v, const WorkRange& r, int nt) {
   const float* scr = static_cast<const float*>(v.scratch);
   if (nt == 32)
     hipLaunchKernelGGL(k_syr2k_mx<32>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc, v.vxc_drop);
+                       v.bfmap, scr, v.vxc, v.vxc_drop, v.vxc_drop > 0.0 ? v.tmax : nullptr);
   else
     hipLaunchKernelGGL(k_syr2k_mx<64>, dim3(grid_of(r)), dim3(256), 0, v.stream, qa, v.tasks,
-                       v.bfmap, scr, v.vxc, v.vxc_drop);
+                       v.bfmap, scr, v.vxc, v.vxc_drop, v.vxc_drop > 0.0 ? v.tmax : nullptr);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -73,6 +73,8 @@ struct DevTask {
                       // floats + the value plane's bfloat16 residual
                       // [nbe][npts_pad], offset in floats (5.5 planes per task)
   int64_t red_off;    // global index of the task's first point block (EXC/NEL partials)
+  int64_t tmax_off;   // first of the task's ceil(nbe / 32) function granules in the
+                      // bound table (max |B| and max |Z| per 32 functions)
   int64_t p_off;      // offset of the molecule's P (and P slices, VXC) in elements
 };
 
@@ -122,7 +124,11 @@ struct CtxView {
   double* red;
   void* scratch;
   double vxc_drop;  // B^T Z contributions with |v| < vxc_drop are not added to VXC
+  unsigned* tmax;   // per granule {max |B|, max |Z|} as float bits (the density stage
+                    // fills it, the B^T Z stage skips tile pairs bounded below vxc_drop)
 };
+constexpr int kGran = 32;      // functions per granule of the bound table
+constexpr int kGranLds = 64;   // granules a density workgroup reduces in LDS (nbe <= 2048)
 
 inline QueueArgs queue_args(const CtxView& v, const WorkRange& r) {
   return QueueArgs{v.work + r.off, r.nq ? v.qoff + r.qoff_off : nullptr,
@@ -136,6 +142,29 @@ inline unsigned grid_of(const WorkRange& r) { return (unsigned)(r.nq ? r.nq : r.
   for (int64_t it_var = (qa).qoff ? (qa).qoff[blockIdx.x] : (int64_t)blockIdx.x,  \
                it_end_ = (qa).qoff ? (qa).qoff[blockIdx.x + 1] : it_var + 1;       \
        it_var < it_end_; ++it_var)
+
+// True when every element of the B^T Z + Z^T B tile pair (rows i0.., columns
+// j0.., nt wide) of task t is bounded below `drop` by the granule maxima the
+// density stage recorded: |sum_p B_pi Z_pj| <= npts max|B_i| max|Z_j|.
+// The factor 1.01 covers the rounding of the stored maxima and of the
+// products.
+__device__ __forceinline__ bool tile_pair_negligible(const DevTask& t, const unsigned* tmax,
+                                                     int i0, int j0, int nt, double drop) {
+  if (!tmax) return false;
+  const unsigned* tm = tmax + 2 * t.tmax_off;
+  const int ng = (t.nbe + kGran - 1) / kGran;
+  float bi = 0.f, zi = 0.f, bj = 0.f, zj = 0.f;
+  for (int g = i0 / kGran; g < min((i0 + nt) / kGran, ng); ++g) {
+    bi = fmaxf(bi, __uint_as_float(tm[2 * g]));
+    zi = fmaxf(zi, __uint_as_float(tm[2 * g + 1]));
+  }
+  for (int g = j0 / kGran; g < min((j0 + nt) / kGran, ng); ++g) {
+    bj = fmaxf(bj, __uint_as_float(tm[2 * g]));
+    zj = fmaxf(zj, __uint_as_float(tm[2 * g + 1]));
+  }
+  const double bound = 1.01 * (double)t.npts * ((double)bi * zj + (double)zi * bj);
+  return bound < drop;
+}
 
 __device__ __forceinline__ void stamp_item(const QueueArgs& qa, int64_t it, int64_t t0) {
   if (qa.tstamp && threadIdx.x == 0) {
