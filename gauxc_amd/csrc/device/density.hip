@@ -323,11 +323,13 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
       }
       *reinterpret_cast<Vec4<T>*>(B + 4 * plane + j * ld) = out;
       if (tmax) {
+        // A NaN counts as unbounded, so screening never hides it.
         float mb = 0.f, mz = 0.f;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          mb = fmaxf(mb, fabsf((float)b[u][e]));
-          mz = fmaxf(mz, fabsf((float)out[e]));
+          const float fb = fabsf((float)b[u][e]), fz = fabsf((float)out[e]);
+          mb = fmaxf(mb, fb != fb ? INFINITY : fb);
+          mz = fmaxf(mz, fz != fz ? INFINITY : fz);
         }
         // The 16 lanes of this row (same g) hold its 64 points.
 #pragma unroll
