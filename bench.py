@@ -126,9 +126,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tile-policy", default=None,
-                    help="D,S,Q[,H[,W]]: fixed density / SYR2K 32-wide thresholds, queue "
-                         "count, (mixed) density high-occupancy threshold and 128-wide "
-                         "minimum nbe for every precision (skips the autotuner)")
+                    help="D,S,Q[,H[,W[,M]]]: fixed density / SYR2K 32-wide thresholds, "
+                         "queue count, (mixed) density high-occupancy threshold, 128-wide "
+                         "minimum nbe and basis evaluation mode for every precision "
+                         "(skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count)")
@@ -179,19 +180,22 @@ def main():
             d_, s_, q_ = vals[:3]
             h_ = vals[3] if len(vals) > 3 else 0
             w_ = vals[4] if len(vals) > 4 else 1 << 30
+            m_ = vals[5] if len(vals) > 5 else 1
             ctx.set_tile_policy(d_, s_)
             ctx.set_density_occupancy(h_)
             ctx.set_density_wide(w_)
+            ctx.set_density_basis_eval(m_)
             ctx.set_schedule(q_)
-            tuned = (d_, s_, q_, None, h_, w_)
+            tuned = (d_, s_, q_, None, h_, w_, ctx.density_basis_eval())
         elif not args.no_autotune:
             tic = time.perf_counter()
             tuned = ctx.autotune("PBE", prec)
             pol = ctx.tile_policy()
-            tuned = tuned + (pol[2], pol[4])
+            tuned = tuned + (pol[2], pol[4], ctx.density_basis_eval())
             log(f"{prec}: autotune -> dens32<=nbe {tuned[0]}, syr2k32<=nbe {tuned[1]}, "
                 f"queues {tuned[2]}, density hi-occupancy<=nbe {pol[2]}, "
-                f"128-wide>=nbe {pol[4]} ({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
+                f"128-wide>=nbe {pol[4]}, basis evaluation mode {tuned[6]} "
+                f"({tuned[3]:.2f} ms; {time.perf_counter() - tic:.1f}s)")
         # Eager steps: per-stage HIP-event times of the kernels.
         ctx.use_graph(False)
         stage = np.zeros(7)
@@ -392,7 +396,9 @@ def run_batched(args):
     sub = list(range(min(8, len(mols))))
     sc = context(sub)
     tuned = {p: sc.autotune("PBE", p) for p in ("fp64", "mixed")}
-    log(f"c3 autotune on {len(sub)} molecules: {tuned}")
+    pol = sc.tile_policy()  # the mixed tuning's density settings
+    mixed_extra = dict(hiocc=pol[2], wide=pol[4], basis_eval=sc.density_basis_eval())
+    log(f"c3 autotune on {len(sub)} molecules: {tuned}, mixed density {mixed_extra}")
     sc.set_tile_policy(*tuned["mixed"][:2])
     times = measure_task_times(sc, "mixed")
     feats_sub = np.concatenate([task_features(mols[i][2], mols[i][0].xcq_shells()) for i in sub])
@@ -405,6 +411,9 @@ def run_batched(args):
     pred = mlp.predict(feats)
     nq = 4 * 256
     out = {}
+    full.set_density_occupancy(mixed_extra["hiocc"])
+    full.set_density_wide(mixed_extra["wide"])
+    full.set_density_basis_eval(mixed_extra["basis_eval"])
     for prec in ("fp64", "mixed"):
         full.set_tile_policy(*tuned[prec][:2])
         full.use_graph(False)

@@ -90,6 +90,7 @@ DEVICE_SYMBOLS = [
     "xcq_upload_molecules", "xcq_upload_tasks_batched", "xcq_upload_density_mol",
     "xcq_download_results_mol", "xcq_set_tile_policy", "xcq_set_density_occupancy",
     "xcq_get_tile_policy", "xcq_set_density_wide", "xcq_set_vxc_drop_tolerance",
+    "xcq_set_density_basis_eval", "xcq_get_density_basis_eval",
     "xcq_set_schedule",
     "xcq_enable_task_timing", "xcq_task_times", "xcq_use_graph", "xcq_autotune",
     "xcq_mlp_create", "xcq_mlp_destroy", "xcq_mlp_num_params", "xcq_mlp_get_params",
@@ -146,6 +147,8 @@ def device_lib():
             "xcq_set_density_occupancy": (ctypes.c_int, [H, c_i32]),
             "xcq_get_tile_policy": (ctypes.c_int, [H, c_vp]),
             "xcq_set_density_wide": (ctypes.c_int, [H, c_i32]),
+            "xcq_set_density_basis_eval": (ctypes.c_int, [H, c_i32]),
+            "xcq_get_density_basis_eval": (ctypes.c_int, [H, ctypes.POINTER(c_i32)]),
             "xcq_set_vxc_drop_tolerance": (ctypes.c_int, [H, ctypes.c_double]),
             "xcq_set_schedule": (ctypes.c_int, [H, c_i32, c_vp, c_vp]),
             "xcq_enable_task_timing": (ctypes.c_int, [H, ctypes.c_int]),

@@ -425,6 +425,20 @@ class DeviceContext:
         N.xcq_check(self.lib.xcq_set_vxc_drop_tolerance(self.h, float(tol)),
                     "xcq_set_vxc_drop_tolerance")
 
+    def set_density_basis_eval(self, mode):
+        """Mixed precision: 1 (or True) = the density stage evaluates basis
+        values/gradients for its fold and Z; 2 = for Z only; 0 (or False) =
+        both read the planes the collocation stage stored."""
+        N.xcq_check(self.lib.xcq_set_density_basis_eval(self.h, int(mode)),
+                    "xcq_set_density_basis_eval")
+
+    def density_basis_eval(self):
+        """The basis evaluation mode in effect (see set_density_basis_eval)."""
+        m = ctypes.c_int32(0)
+        N.xcq_check(self.lib.xcq_get_density_basis_eval(self.h, ctypes.byref(m)),
+                    "xcq_get_density_basis_eval")
+        return int(m.value)
+
     def set_density_wide(self, wide_min_nbe):
         N.xcq_check(self.lib.xcq_set_density_wide(self.h, int(wide_min_nbe)),
                     "xcq_set_density_wide")

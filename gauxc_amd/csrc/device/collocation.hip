@@ -1,12 +1,6 @@
 // Shell collocation (+gradient) on gfx950.
 //
-// Semantics: GauXC's collocation (gau2grid_collocation.cxx:25-145 on the
-// host, collocation_device_masked_combined_kernel_deriv1 on the device):
-//   B_mu(r) = R(|r-A|) Y_mu(r-A),  R(r) = sum_p c_p exp(-a_p r^2),
-//   d/dx B = (dY/dx) R + Y x dR/dr / r = (dY/dx) R - 2 x Y sum_p a_p c_p e^{-a_p r^2},
-// Cartesian functions in CCA order (x^l, x^(l-1)y, ..., z^l), spherical
-// functions as real solid harmonics in CCA order m = -l..l, unnormalised as
-// gau2grid's GG_SPHERICAL_CCA.
+// Semantics and the per-shell evaluation: shell_eval.hpp.
 //
 // Layout: task t owns [comp][nbe][npts_pad] in the chunk scratch
 // (comp = B, dB/dx, dB/dy, dB/dz), point index fastest, padded points 0.
@@ -14,80 +8,11 @@
 // the four waves stride over the task's shells.  Stores of one wave are
 // 64 consecutive points of one basis function: fully coalesced.
 #include "internal.hpp"
+#include "shell_eval.hpp"
 
 #include <cmath>
 
 namespace xcq {
-
-// ---- real solid harmonics, tabulated at compile time --------------------
-// Closed form of Helgaker, Jorgensen & Olsen, "Molecular Electronic-Structure
-// Theory", eq. 6.4.47-6.4.50:
-//   S_lm = N_lm sum_{t,u,v} C^{lm}_{tuv} x^{2t+|m|-2(u+v)} y^{2(u+v)} z^{l-2t-|m|},
-//   C = (-1)^{t+v-v_m} (1/4)^t binom(l,t) binom(l-t,|m|+t) binom(t,u) binom(|m|,2v),
-//   N_lm = sqrt(2 (l+|m|)! (l-|m|)! / 2^{delta_m0}) / (2^|m| l!),
-// v_m = 0 (m >= 0) or 1/2 (m < 0), v running in steps of 1 from v_m.  The
-// table is a constant expression, so the unrolled transforms below multiply
-// by immediate coefficients and skip the zero ones.
-constexpr int kNCart = (kMaxL + 1) * (kMaxL + 2) / 2;
-
-constexpr double cx_fact(int n) {
-  double r = 1.;
-  for (int k = 2; k <= n; ++k) r *= k;
-  return r;
-}
-constexpr double cx_binom(int n, int k) {
-  return (k < 0 || k > n) ? 0. : cx_fact(n) / (cx_fact(k) * cx_fact(n - k));
-}
-// Newton's iteration from above; agrees with a correctly rounded square
-// root to within one unit in the last place.
-constexpr double cx_sqrt(double x) {
-  double r = x > 1. ? x : 1.;
-  for (int i = 0; i < 200; ++i) {
-    const double n = 0.5 * (r + x / r);
-    if (!(n < r)) break;
-    r = n;
-  }
-  return r;
-}
-constexpr int cx_cart_index(int l, int a, int b) {  // CCA position of x^a y^b z^(l-a-b)
-  int idx = 0;
-  for (int aa = l; aa >= 0; --aa)
-    for (int bb = l - aa; bb >= 0; --bb, ++idx)
-      if (aa == a && bb == b) return idx;
-  return -1;
-}
-
-struct SphTable {
-  double c[kMaxL + 1][2 * kMaxL + 1][kNCart];
-};
-
-constexpr SphTable make_sph_table() {
-  SphTable t{};
-  for (int l = 0; l <= kMaxL; ++l)
-    for (int m = -l; m <= l; ++m) {
-      const int am = m < 0 ? -m : m;
-      double pow2 = 1.;
-      for (int k = 0; k < am; ++k) pow2 *= 2.;
-      const double N =
-          cx_sqrt(2. * cx_fact(l + am) * cx_fact(l - am) / (m == 0 ? 2. : 1.)) / (pow2 * cx_fact(l));
-      const int v2s = m < 0 ? 1 : 0;  // 2v starts at 0 (m >= 0) or 1 (m < 0)
-      for (int tt = 0; tt <= (l - am) / 2; ++tt) {
-        double q = 1.;
-        for (int k = 0; k < tt; ++k) q *= 0.25;
-        for (int u = 0; u <= tt; ++u)
-          for (int v2 = v2s; v2 <= am; v2 += 2) {
-            const int se = tt + (v2 - v2s) / 2;
-            const double C = ((se & 1) ? -1. : 1.) * q * cx_binom(l, tt) *
-                             cx_binom(l - tt, am + tt) * cx_binom(tt, u) * cx_binom(am, v2);
-            const int ax = 2 * tt + am - 2 * u - v2, ay = 2 * u + v2;
-            t.c[l][m + l][cx_cart_index(l, ax, ay)] += N * C;
-          }
-      }
-    }
-  return t;
-}
-
-__device__ constexpr SphTable kSph = make_sph_table();
 
 int upload_sph_table(hipStream_t) { return XCQ_SUCCESS; }
 
@@ -101,9 +26,7 @@ __device__ __forceinline__ unsigned short resid_bf16(double v) {
 // Values (and gradients) of one shell at one point, written to the scratch
 // in double arithmetic.  T is the storage type (double or float); with FP32
 // storage the value plane also gets its bfloat16 residual (lo, same
-// indexing as out).  Spherical shells accumulate their 2L+1 outputs while
-// the Cartesian monomials are generated, so no Cartesian array is kept.
-// Stores go through a buffer resource over the task's scratch: the byte
+// indexing as out).  Stores go through a buffer resource over the task's scratch: the byte
 // offset of a row is a scalar (row0 = offset of the shell's first function
 // at the block's first point), the lane offset one shared register.
 typedef unsigned U32x2 __attribute__((ext_vector_type(2)));
@@ -128,16 +51,8 @@ template <int L, bool GRAD, typename T>
 __device__ __forceinline__ void shell_eval(double x, double y, double z, double R, double dR,
                                            bool pure, bool valid, const TaskStore& ts,
                                            int64_t row0, int64_t ld, int64_t plane) {
-  double xp[L + 1], yp[L + 1], zp[L + 1];
-  xp[0] = yp[0] = zp[0] = 1.;
-#pragma unroll
-  for (int k = 1; k <= L; ++k) {
-    xp[k] = xp[k - 1] * x;
-    yp[k] = yp[k - 1] * y;
-    zp[k] = zp[k - 1] * z;
-  }
-  const double Rx = dR * x, Ry = dR * y, Rz = dR * z;
-  auto put = [&](int f, double v, double gx, double gy, double gz) {
+  shell_functions<L, GRAD>(x, y, z, R, dR, pure,
+                           [&](int f, double v, double gx, double gy, double gz) {
     const int64_t r = row0 + f * ld;
     store_elem<T>(ts, r, valid ? T(v) : T(0));
     if constexpr (sizeof(T) == 4)
@@ -149,62 +64,7 @@ __device__ __forceinline__ void shell_eval(double x, double y, double z, double 
       store_elem<T>(ts, 2 * plane + r, valid ? T(gy) : T(0));
       store_elem<T>(ts, 3 * plane + r, valid ? T(gz) : T(0));
     }
-  };
-  // Cartesian term ic = (a, b, c) in CCA order: value and gradient.
-  auto term = [&](int a, int b, double& cv, double& gx, double& gy, double& gz) {
-    const int c = L - a - b;
-    const double mono = xp[a] * yp[b] * zp[c];
-    cv = mono * R;
-    if constexpr (GRAD) {
-      const double dxm = a ? double(a) * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] : 0.;
-      const double dym = b ? double(b) * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] : 0.;
-      const double dzm = c ? double(c) * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] : 0.;
-      gx = dxm * R + mono * Rx;
-      gy = dym * R + mono * Ry;
-      gz = dzm * R + mono * Rz;
-    }
-  };
-  if (pure) {
-    double v[2 * L + 1], wx[2 * L + 1], wy[2 * L + 1], wz[2 * L + 1];
-#pragma unroll
-    for (int m = 0; m < 2 * L + 1; ++m) v[m] = wx[m] = wy[m] = wz[m] = 0.;
-    int ic = 0;
-#pragma unroll
-    for (int a = L; a >= 0; --a) {
-#pragma unroll
-      for (int b = L - a; b >= 0; --b) {
-        double cv, gx = 0., gy = 0., gz = 0.;
-        term(a, b, cv, gx, gy, gz);
-#pragma unroll
-        for (int m = 0; m < 2 * L + 1; ++m) {
-          const double cf = kSph.c[L][m][ic];
-          if (cf != 0.) {
-            v[m] += cf * cv;
-            if constexpr (GRAD) {
-              wx[m] += cf * gx;
-              wy[m] += cf * gy;
-              wz[m] += cf * gz;
-            }
-          }
-        }
-        ++ic;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 2 * L + 1; ++m) put(m, v[m], wx[m], wy[m], wz[m]);
-  } else {
-    int ic = 0;
-#pragma unroll
-    for (int a = L; a >= 0; --a) {
-#pragma unroll
-      for (int b = L - a; b >= 0; --b) {
-        double cv, gx = 0., gy = 0., gz = 0.;
-        term(a, b, cv, gx, gy, gz);
-        put(ic, cv, gx, gy, gz);
-        ++ic;
-      }
-    }
-  }
+  });
 }
 
 constexpr int kCollShells = 256;  // shells staged per pass

@@ -121,6 +121,7 @@ struct xcq_context_s {
 
   // molecules and basis
   DevShell* d_shells = nullptr;
+  DevShellC* d_shc = nullptr;
   int32_t nshells = 0;
   std::vector<int32_t> sh_size, sh_first, sh_mol;  // sh_first: molecule-local
   std::vector<MolInfo> mols;
@@ -148,6 +149,9 @@ struct xcq_context_s {
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
   int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
   int32_t dens128_min_nbe = 1 << 30;  // 128-wide (mixed) density tiles from this nbe on
+  // Mixed density stage evaluates B / grad B itself (basis up to l = 3);
+  // otherwise it reads the gradient planes the collocation stage stores.
+  int32_t dens_in_stage = 1;
   double vxc_drop = kDefaultVxcDrop;  // see xcq_set_vxc_drop_tolerance
   int32_t nqueues = 0;                                 // 0: one workgroup per item
   std::vector<double> cost_d, cost_s;                  // per-task predicted costs
@@ -221,6 +225,7 @@ struct xcq_context_s {
     for (auto e : ev_total)
       if (e) (void)hipEventDestroy(e);
     dfree(d_shells);
+    dfree(d_shc);
     dfree(d_mol_n);
     dfree(d_mol_off);
     dfree(d_mol_blk);
@@ -241,7 +246,8 @@ CtxView view_of(xcq_context_s* c) {
                  c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
                  c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
                  c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch,
-                 c->vxc_drop, reinterpret_cast<unsigned*>(c->d_tmax)};
+                 c->vxc_drop, reinterpret_cast<unsigned*>(c->d_tmax),
+                 c->max_l <= kInStageMaxL ? c->dens_in_stage : 0, c->d_shc};
 }
 
 // Heuristic per-item costs (used when no predicted task costs are set):
@@ -357,7 +363,8 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
                                    v.py, v.pz, static_cast<double*>(v.scratch), gga, ctx->max_l);
     else
       rc = launch_collocation_fp32(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
-                                   v.py, v.pz, static_cast<float*>(v.scratch), gga, ctx->max_l);
+                                   v.py, v.pz, static_cast<float*>(v.scratch),
+                                   gga && v.in_stage != 1, ctx->max_l);
     if (rc) return rc;
     XCQ_HIP(rec(ev[1]));
     // X, rho/grad rho, functional, EXC/NEL and Z in one launch per tile
@@ -581,9 +588,18 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
   ctx->drop_graph();
   ctx->free_tasks();  // tasks refer to shell indices: require a new upload
   dfree(ctx->d_shells);
+  dfree(ctx->d_shc);
   int rc = dalloc(&ctx->d_shells, nshells);
   if (rc) return rc;
   XCQ_HIP(hipMemcpy(ctx->d_shells, h.data(), sizeof(DevShell) * nshells, hipMemcpyHostToDevice));
+  std::vector<DevShellC> hc(nshells);
+  for (int32_t k = 0; k < nshells; ++k) {
+    const DevShell& d = h[k];
+    hc[k] = DevShellC{d.O[0], d.O[1], d.O[2], d.alpha[0], d.coeff[0],
+                      d.l | (d.pure ? 1 << 8 : 0) | (d.nprim << 16), 0};
+  }
+  if ((rc = dalloc(&ctx->d_shc, nshells))) return rc;
+  XCQ_HIP(hipMemcpy(ctx->d_shc, hc.data(), sizeof(DevShellC) * nshells, hipMemcpyHostToDevice));
   ctx->nshells = nshells;
   const bool same_shape =
       ctx->mols.size() == mols.size() &&
@@ -879,6 +895,23 @@ int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe) {
   return XCQ_SUCCESS;
 }
 
+int xcq_set_density_basis_eval(xcq_context ctx, int32_t in_stage) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  XCQ_REQUIRE(in_stage >= 0 && in_stage <= 2, XCQ_ERROR_INVALID_ARGUMENT,
+              "density basis evaluation mode must be 0, 1 or 2");
+  ctx->dens_in_stage = in_stage;
+  ctx->drop_graph();  // the captured launches follow the old setting
+  return XCQ_SUCCESS;
+}
+
+int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode) {
+  XCQ_REQUIRE(ctx && mode, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  *mode = ctx->max_l <= kInStageMaxL ? ctx->dens_in_stage : 0;
+  return XCQ_SUCCESS;
+}
+
 int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_HIP(hipSetDevice(ctx->device));
@@ -976,7 +1009,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
       double st[kStages + 1];
       xcq_stage_times(ctx, st, kStages + 1);
       if (rep == 0) continue;  // warm-up
-      d.push_back(st[1]);
+      d.push_back(st[0] + st[1]);
       sy.push_back(st[4]);
       tt.push_back(st[kStages]);
     }
@@ -993,8 +1026,9 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
   int32_t best_d = ctx->dens32_max_nbe, best_s = ctx->syr2k32_max_nbe;
   double bd = 1e300, bs = 1e300, tot = 0.;
   int rc;
-  // The two stages read disjoint thresholds: sweep both at once and keep the
-  // best of each.
+  // The density-side times below are collocation + density: the basis
+  // evaluation mode moves work between the two.  The two stages' tile
+  // thresholds are disjoint: sweep both at once and keep the best of each.
   for (int32_t c : cand) {
     if ((rc = xcq_set_tile_policy(ctx, c, c))) return rc;
     double dm, sm;
@@ -1009,34 +1043,62 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
     }
   }
   if ((rc = xcq_set_tile_policy(ctx, best_d, best_s))) return rc;
-  // Mixed precision: which of the remaining (64-wide) density tasks run in
-  // the three-workgroups-per-CU configuration.
   if (precision == XCQ_PREC_MIXED) {
-    int32_t best_h = ctx->dens_hiocc_max_nbe;
-    double bh = 1e300;
-    for (int32_t c : cand) {
-      if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
-      double dm, sm;
-      if ((rc = measure(&dm, &sm, &tot))) return rc;
-      if (dm < bh) {
-        bh = dm;
-        best_h = c;
+    // Per basis evaluation mode (in-stage first, when the basis allows it):
+    // the 32-wide threshold, which 64-wide tasks run at three workgroups per
+    // CU (stored planes only), and from which nbe on 128-wide tiles are
+    // used; the mode with the best collocation + density time wins.
+    struct Pol {
+      int32_t mode, d, h, w;
+      double ms;
+    };
+    std::vector<int32_t> modes;
+    if (ctx->max_l <= kInStageMaxL) modes.push_back(1);
+    modes.push_back(0);
+    Pol best{ctx->dens_in_stage, best_d, ctx->dens_hiocc_max_nbe, ctx->dens128_min_nbe, 1e300};
+    for (int32_t mode : modes) {
+      if ((rc = xcq_set_density_basis_eval(ctx, mode))) return rc;
+      Pol p{mode, best_d, 0, 1 << 30, 1e300};
+      if ((rc = xcq_set_density_occupancy(ctx, 0))) return rc;
+      if ((rc = xcq_set_density_wide(ctx, 1 << 30))) return rc;
+      for (int32_t c : cand) {
+        if ((rc = xcq_set_tile_policy(ctx, c, best_s))) return rc;
+        double dm, sm;
+        if ((rc = measure(&dm, &sm, &tot))) return rc;
+        if (dm < p.ms) {
+          p.ms = dm;
+          p.d = c;
+        }
       }
-    }
-    if ((rc = xcq_set_density_occupancy(ctx, best_h))) return rc;
-    // ... and from which nbe on 128-wide tiles are used.
-    int32_t best_w = ctx->dens128_min_nbe;
-    double bw = 1e300;
-    for (int32_t c : {128, 192, 256, 384, 512, 1 << 30}) {
-      if ((rc = xcq_set_density_wide(ctx, c))) return rc;
-      double dm, sm;
-      if ((rc = measure(&dm, &sm, &tot))) return rc;
-      if (dm < bw) {
-        bw = dm;
-        best_w = c;
+      if ((rc = xcq_set_tile_policy(ctx, p.d, best_s))) return rc;
+      if (mode == 0) {
+        for (int32_t c : cand) {
+          if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
+          double dm, sm;
+          if ((rc = measure(&dm, &sm, &tot))) return rc;
+          if (dm < p.ms) {
+            p.ms = dm;
+            p.h = c;
+          }
+        }
+        if ((rc = xcq_set_density_occupancy(ctx, p.h))) return rc;
       }
+      for (int32_t c : {128, 192, 256, 384, 512, 1 << 30}) {
+        if ((rc = xcq_set_density_wide(ctx, c))) return rc;
+        double dm, sm;
+        if ((rc = measure(&dm, &sm, &tot))) return rc;
+        if (dm < p.ms) {
+          p.ms = dm;
+          p.w = c;
+        }
+      }
+      if (p.ms < best.ms) best = p;
     }
-    if ((rc = xcq_set_density_wide(ctx, best_w))) return rc;
+    best_d = best.d;
+    if ((rc = xcq_set_density_basis_eval(ctx, best.mode))) return rc;
+    if ((rc = xcq_set_tile_policy(ctx, best.d, best_s))) return rc;
+    if ((rc = xcq_set_density_occupancy(ctx, best.h))) return rc;
+    if ((rc = xcq_set_density_wide(ctx, best.w))) return rc;
   }
   // Persistent queues (flop-heuristic packing) against one workgroup per item.
   int dev_cus = 0;

@@ -31,6 +31,10 @@
 // queued mode, one persistent workgroup per queue of items.
 #include "internal.hpp"
 #include "functional.hpp"
+#include "shell_eval.hpp"
+
+#include <climits>
+#include <type_traits>
 
 namespace xcq {
 
@@ -357,6 +361,185 @@ __device__ __forceinline__ void write_z(const DevTask& t, int m0, int tid, doubl
   }
 }
 
+// ---- basis functions evaluated inside the density stage -------------------
+// With the in-stage evaluation the collocation stage stores only the value
+// plane (and its residual), which the X product and the B^T Z stage read;
+// the fold and Z evaluate B and grad B of the block's points from the shell
+// parameters where they use them (FP64), so the three gradient planes are
+// neither written nor read twice.  Wave w takes the shells s = w (mod 4)
+// of a function range; lane = point of the 64-point block, so a shell's
+// parameters are wave-uniform (scalar loads) and each function's values
+// form one 64-point row.
+
+// A shell as one wave evaluates it: its first local function, its index
+// and the compact record's fields, all wave-uniform.
+struct ShellU {
+  int bf0, ish, info;
+  double ox, oy, oz, a0, c0;
+};
+
+__device__ __forceinline__ double readlane_d(double v, int j) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, j);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), j);
+  return __builtin_bit_cast(double, (unsigned long long)hi << 32 | lo);
+}
+
+// Calls sink(f, v, gx, gy, gz) for the functions of shell u at the lane's
+// point (rx, ry, rz): R = sum c e^{-a r2}, dR = -2 sum a c e^{-a r2}; the
+// primitives past the first come from the full shell record.
+template <bool GRAD, typename Sink>
+__device__ __forceinline__ void eval_shell(const ShellArgs& sa, const ShellU& u, double rx,
+                                           double ry, double rz, Sink&& sink) {
+  const int l = u.info & 0xff, nprim = u.info >> 16;
+  const bool pure = (u.info >> 8) & 1;
+  const double x = rx - u.ox, y = ry - u.oy, z = rz - u.oz;
+  const double r2 = x * x + y * y + z * z;
+  const double e0 = u.c0 * exp(-u.a0 * r2);
+  double R = e0, dR = -2. * u.a0 * e0;
+  if (nprim > 1) {
+    const DevShell* sh = sa.shells + u.ish;
+    for (int k = 1; k < nprim; ++k) {
+      const double a = sh->alpha[k], e = sh->coeff[k] * exp(-a * r2);
+      R += e;
+      dR += -2. * a * e;
+    }
+  }
+  shell_functions_l<kInStageMaxL, GRAD, true>(l, x, y, z, R, dR, pure, sink);
+}
+
+// Calls body(ShellU) for the task's shells that intersect local functions
+// [.., nhi) from s_first (the shell containing the range's first
+// function), this wave's share (every fourth); returns the shell
+// containing nhi.  Lane i loads the map entries and the compact record of
+// shell s_first + i (64 at a time, one vector load each), which the wave
+// then broadcasts shell by shell.  The local first-function map is
+// ascending, so the shells starting below nhi are a prefix counted by a
+// ballot.
+template <typename Body>
+__device__ __forceinline__ int for_range_shells(const DevTask& t, const ShellArgs& sa,
+                                                int s_first, int nhi, int wave, int lane,
+                                                Body&& body) {
+  int next = s_first;
+  for (int sc = s_first; sc < t.nshells; sc += 64) {
+    const int s = sc + lane;
+    const bool ok = s < t.nshells;
+    const int f0 = ok ? sa.lsh[t.lsh_off + s] : INT_MAX;
+    const int ish = ok ? sa.shell_list[t.sh_off + s] : 0;
+    const uint64_t in = __builtin_amdgcn_ballot_w64(f0 < nhi);
+    const uint64_t le = __builtin_amdgcn_ballot_w64(f0 <= nhi);
+    const int cnt = __popcll(in);
+    if (le) next = sc + __popcll(le) - 1;
+    if (wave < cnt) {
+      const DevShellC c = sa.shc[ish];
+      for (int j = wave; j < cnt; j += 4) {
+        ShellU u;
+        u.bf0 = __builtin_amdgcn_readlane(f0, j);
+        u.ish = __builtin_amdgcn_readlane(ish, j);
+        u.info = __builtin_amdgcn_readlane(c.info, j);
+        u.ox = readlane_d(c.ox, j);
+        u.oy = readlane_d(c.oy, j);
+        u.oz = readlane_d(c.oz, j);
+        u.a0 = readlane_d(c.a0, j);
+        u.c0 = readlane_d(c.c0, j);
+        body(u);
+      }
+    }
+    if (cnt < 64) break;
+  }
+  return next;
+}
+
+// Folds rows [nlo, nhi) of the X^T tile in LDS (row n - nlo, column =
+// point) into this lane's partials r: rho += B X, grad rho / 2 += grad B X.
+template <bool GGA>
+__device__ __forceinline__ int fold_in_stage(const DevTask& t, const ShellArgs& sa, int s_first,
+                                             int nlo, int nhi, int wave, int lane,
+                                             const double* xt, double rx, double ry, double rz,
+                                             double (&r)[4]) {
+  return for_range_shells(t, sa, s_first, nhi, wave, lane, [&](const ShellU& u) {
+    eval_shell<GGA>(sa, u, rx, ry, rz, [&](int f, double v, double gx, double gy, double gz) {
+      const int n = u.bf0 + f;  // uniform
+      if (n >= nlo && n < nhi) {
+        const double xv = xt[(n - nlo) * kXLd + lane];
+        r[0] += v * xv;
+        if constexpr (GGA) {
+          r[1] += gx * xv;
+          r[2] += gy * xv;
+          r[3] += gz * xv;
+        }
+      }
+    });
+  });
+}
+
+// Wave-wide max of v (every lane gets it).
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) v = max(v, (unsigned)__shfl_xor((int)v, off));
+  return v;
+}
+
+// Z = a B + c . grad B of the block's points for every function of the
+// task, with B and grad B evaluated here; stored in FP32 (plane 4), one
+// coalesced 64-point row per function.  Padded points have zero
+// coefficients (so Z = 0) and are evaluated at the task's last point.
+// Also records the granule maxima of |B| and |Z| as write_z does: per lane
+// as the bits of |x| under unsigned max (which orders every NaN above
+// infinity), reduced across the wave when the granule changes (each wave
+// meets the granules in ascending order) and clamped to infinity there, so
+// a NaN is never screened out.
+template <bool GGA>
+__device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa, int m0,
+                                           int wave, int lane, int tid, double rx, double ry,
+                                           double rz, double (*zc)[64], float* Bm,
+                                           unsigned (*smax)[2], unsigned* tmax) {
+  const int ld = t.npts_pad, nbe = t.nbe;
+  const int64_t plane = (int64_t)nbe * ld;
+  const double za = zc[0][lane], zx = zc[1][lane], zy = zc[2][lane], zz = zc[3][lane];
+  float* Zp = Bm + 4 * plane + m0 + lane;
+  constexpr unsigned kInfBits = 0x7f800000u, kAbs = 0x7fffffffu;
+  int cur = -1;
+  unsigned mb = 0u, mz = 0u;
+  auto flush = [&]() {
+    const unsigned b = min(wave_max(mb), kInfBits), z = min(wave_max(mz), kInfBits);
+    if (lane == 0) {
+      if (cur < kGranLds) {
+        atomicMax(&smax[cur][0], b);
+        atomicMax(&smax[cur][1], z);
+      } else {
+        atomicMax(tmax + 2 * cur, b);
+        atomicMax(tmax + 2 * cur + 1, z);
+      }
+    }
+  };
+  for_range_shells(t, sa, 0, nbe, wave, lane, [&](const ShellU& u) {
+    eval_shell<GGA>(sa, u, rx, ry, rz, [&](int f, double v, double gx, double gy, double gz) {
+      const int n = u.bf0 + f;  // uniform
+      double zv = za * v;
+      if constexpr (GGA) zv += zx * gx + zy * gy + zz * gz;
+      const float zf = (float)zv;
+      Zp[(int64_t)n * ld] = zf;
+      if (tmax) {
+        const int g = __builtin_amdgcn_readfirstlane(n / kGran);
+        if (g != cur) {
+          if (cur >= 0) flush();
+          cur = g;
+          mb = mz = 0u;
+        }
+        mb = max(mb, __float_as_uint((float)v) & kAbs);
+        mz = max(mz, __float_as_uint(zf) & kAbs);
+      }
+    });
+  });
+  if (tmax) {
+    if (cur >= 0) flush();
+    __syncthreads();
+    const int ng = min((nbe + kGran - 1) / kGran, kGranLds);
+    if (tid < 2 * ng) atomicMax(tmax + tid, smax[tid >> 1][tid & 1]);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -541,12 +724,15 @@ __device__ __forceinline__ void load_lds(const float* src, float* lds_row) {
   asm volatile("global_load_lds_dword %0, off" ::"v"(src), "{m0}"(off) : "memory");
 }
 
-template <int NT, bool GGA, bool HIOCC>
+// INSTAGE = 1: the fold and Z evaluate B and grad B themselves
+// (fold_in_stage / z_in_stage); 2: only Z does, the fold reads the stored
+// planes; 0: both read the planes the collocation stage stored.
+template <int NT, bool GGA, bool HIOCC, int INSTAGE>
 __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const float* __restrict__ Pfall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
-    double* __restrict__ red, unsigned* __restrict__ tmax) {
+    double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa) {
   constexpr int BI = NT / 32;
   constexpr int kStages = ring_stages<NT, HIOCC>();
   constexpr int NPC = (NT + 63) / 64;  // 64-column chunks of a P_sub stage row
@@ -557,6 +743,8 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   constexpr int kPtFloats = kStages * 32 * (NT + 4);
   constexpr int kRingFloats = kPtFloats + kStages * 32 * 68;
   static_assert(NT > 64 || NT * kXLd * 2 <= kRingFloats, "X^T tile must fit the ring");
+  static_assert(INSTAGE != 1 || (NT < 64 ? NT : 64) * kXLd * 2 <= kRingFloats,
+                "X^T rows of one in-stage fold must fit the ring");
   __shared__ __attribute__((aligned(16))) float ring[kRingFloats];
   float(*Pt)[32][NT + 4] = reinterpret_cast<float(*)[32][NT + 4]>(ring);
   float(*Bt)[32][68] = reinterpret_cast<float(*)[32][68]>(ring + kPtFloats);
@@ -588,6 +776,18 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     __syncthreads();
     const float* bcol = Bm + m0 + lc;
     const int nks = (nbe + 31) / 32;
+    // In-stage basis evaluation: this lane's point (padded points take the
+    // task's last point: finite values, which meet zero X and zero Z
+    // coefficients), the shell cursor and this wave's density partials.
+    const int pl = min(m0 + lane, t.npts - 1);
+    double rx = 0., ry = 0., rz = 0.;
+    if constexpr (INSTAGE != 0) {
+      rx = sa.px[t.pt_off + pl];
+      ry = sa.py[t.pt_off + pl];
+      rz = sa.pz[t.pt_off + pl];
+    }
+    int s_cur = 0;
+    double part[4] = {0., 0., 0., 0.};
 
     for (int n0 = 0; n0 < nbe; n0 += NT) {
       // P is symmetric: read P(k, n) as row bf[k], column bf[n] so that
@@ -666,7 +866,29 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[a][b][r] *= fac;
-      if constexpr (NT <= 64) {
+      if constexpr (INSTAGE == 1) {
+        // The X^T tile goes through LDS in halves of 64 rows (the row block
+        // of waves with wm = h for 128-wide tiles).
+        double* xt = reinterpret_cast<double*>(ring);
+#pragma unroll
+        for (int h = 0; h < (NT + 63) / 64; ++h) {
+          const int nlo = n0 + 64 * h;
+          if (nlo >= nbe) break;
+          __syncthreads();  // the ring's last K block, or the previous half, is consumed
+          if (NT <= 64 || wm == h)
+#pragma unroll
+            for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+              for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  xt[((NT <= 64 ? wm * 16 * BI : 0) + bi * 16 + 4 * (lane >> 4) + r) * kXLd +
+                     wn * 32 + bj * 16 + (lane & 15)] = acc[bi][bj][r];
+          __syncthreads();
+          s_cur = fold_in_stage<GGA>(t, sa, s_cur, nlo, min(nlo + min(NT, 64), nbe), wave,
+                                     lane, xt, rx, ry, rz, part);
+        }
+      } else if constexpr (NT <= 64) {
         __syncthreads();  // every wave is done with the ring's last K block
         fold_tile_lds<NT, BI, GGA>(acc, reinterpret_cast<double*>(ring), m0, n0, wm, wn, lane,
                                    tid, nbe, ld, plane, Bm, dens);
@@ -677,11 +899,20 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
       // finished reading it (last K blocks, or the X^T tile) at this barrier.
       __syncthreads();
     }
+    if constexpr (INSTAGE == 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dens[wave][lane][c] = part[c];
+      __syncthreads();
+    }
     finish_points<4>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm, smax,
-                                        tmax ? tmax + 2 * t.tmax_off : nullptr);
+      if constexpr (INSTAGE != 0)
+        z_in_stage<GGA>(t, sa, m0, wave, lane, tid, rx, ry, rz, zc, Bm, smax,
+                        tmax ? tmax + 2 * t.tmax_off : nullptr);
+      else
+        write_z<float, GGA, HIOCC ? 2 : 4>(t, m0, tid, zc, Bm, smax,
+                                          tmax ? tmax + 2 * t.tmax_off : nullptr);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -693,31 +924,54 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
   if (r.n == 0) return XCQ_SUCCESS;
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
-  auto go = [&](auto kern, auto P, auto scr) {
-    // The bound table is filled only when VXC is computed and contributions
-    // may be dropped.
-    unsigned* tm = (want_z && v.vxc_drop > 0.0) ? v.tmax : nullptr;
-    hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, P, scr, v.w, func,
-                       fac, want_z, v.red, tm);
-  };
+  // The bound table is filled only when VXC is computed and contributions
+  // may be dropped.
+  unsigned* tm = (want_z && v.vxc_drop > 0.0) ? v.tmax : nullptr;
   if (precision == XCQ_PREC_FP64) {
     double* scr = static_cast<double*>(v.scratch);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P, scr, v.w,
+                         func, fac, want_z, v.red, tm);
+    };
     if (nt == 32)
-      gga ? go(k_density_f64<32, true>, v.P, scr) : go(k_density_f64<32, false>, v.P, scr);
+      gga ? go(k_density_f64<32, true>) : go(k_density_f64<32, false>);
     else
-      gga ? go(k_density_f64<64, true>, v.P, scr) : go(k_density_f64<64, false>, v.P, scr);
+      gga ? go(k_density_f64<64, true>) : go(k_density_f64<64, false>);
   } else {
     float* scr = static_cast<float*>(v.scratch);
-    const float* Pf = v.Pf32;
-    if (nt == 32)
-      gga ? go(k_density_mx<32, true, false>, Pf, scr) : go(k_density_mx<32, false, false>, Pf, scr);
-    else if (nt == 128)
-      gga ? go(k_density_mx<128, true, false>, Pf, scr)
-          : go(k_density_mx<128, false, false>, Pf, scr);
-    else if (hiocc)
-      gga ? go(k_density_mx<64, true, true>, Pf, scr) : go(k_density_mx<64, false, true>, Pf, scr);
+    const ShellArgs sa{v.shells, v.shc, v.shell_list, v.lsh, v.px, v.py, v.pz};
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.Pf32, scr, v.w,
+                         func, fac, want_z, v.red, tm, sa);
+    };
+    // Template arguments: tile width, GGA, high occupancy, in-stage mode.
+    auto pick = [&](auto g, auto ins) {
+      constexpr bool G = decltype(g)::value;
+      constexpr int I = decltype(ins)::value;
+      if (nt == 32) {
+        go(k_density_mx<32, G, false, I>);
+      } else if (nt == 128) {
+        go(k_density_mx<128, G, false, I>);
+      } else if constexpr (I == 0) {
+        if (hiocc) go(k_density_mx<64, G, true, 0>);
+        else go(k_density_mx<64, G, false, 0>);
+      } else {
+        // The in-stage evaluation needs more than the 168 registers of the
+        // three-workgroups-per-CU configuration; it runs at two.
+        go(k_density_mx<64, G, false, I>);
+      }
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    using M0 = std::integral_constant<int, 0>;
+    using M1 = std::integral_constant<int, 1>;
+    using M2 = std::integral_constant<int, 2>;
+    if (v.in_stage == 1)
+      gga ? pick(T_{}, M1{}) : pick(F_{}, M1{});
+    else if (v.in_stage == 2)
+      gga ? pick(T_{}, M2{}) : pick(F_{}, M2{});
     else
-      gga ? go(k_density_mx<64, true, false>, Pf, scr) : go(k_density_mx<64, false, false>, Pf, scr);
+      gga ? pick(T_{}, M0{}) : pick(F_{}, M0{});
   }
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;

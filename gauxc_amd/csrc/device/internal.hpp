@@ -51,6 +51,14 @@ struct DevShell {
   int32_t nprim, l, pure, size;
 };
 
+// Compact copy of a shell's centre, first primitive and shape (48 bytes),
+// which the density stage's basis evaluation loads 64 shells at a time.
+struct DevShellC {
+  double ox, oy, oz, a0, c0;
+  int32_t info;  // l | pure << 8 | nprim << 16
+  int32_t pad_;
+};
+
 // One task as laid out on the device.  Offsets into resident arrays are
 // absolute; b_off is relative to the chunk scratch.
 struct DevTask {
@@ -126,7 +134,20 @@ struct CtxView {
   double vxc_drop;  // B^T Z contributions with |v| < vxc_drop are not added to VXC
   unsigned* tmax;   // per granule {max |B|, max |Z|} as float bits (the density stage
                     // fills it, the B^T Z stage skips tile pairs bounded below vxc_drop)
+  int in_stage;     // mixed density stage evaluates B / grad B itself; collocation
+                    // then stores only the value plane and its residual
+  const DevShellC* shc;
 };
+// Basis data of the density stage's own evaluation of B and grad B.
+struct ShellArgs {
+  const DevShell* shells;       // primitives past the first
+  const DevShellC* shc;
+  const int32_t* shell_list;    // task shell lists (molecule shell indices)
+  const int32_t* lsh;           // first local function of each task shell (ascending)
+  const double *px, *py, *pz;   // point coordinates
+};
+constexpr int kInStageMaxL = 3;  // highest angular momentum it evaluates
+
 constexpr int kGran = 32;      // functions per granule of the bound table
 constexpr int kGranLds = 64;   // granules a density workgroup reduces in LDS (nbe <= 2048)
 

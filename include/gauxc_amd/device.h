@@ -115,6 +115,17 @@ int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
  * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).
  * Default: off (2^30). */
 int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe);
+/* Mixed precision: 1 (default) = the density stage evaluates the basis
+ * functions' values and gradients at its points itself (bases up to l = 3)
+ * for its rho fold and for Z, so the collocation stage stores only the
+ * value plane (GauXC stores B and grad B,
+ * collocation_device_masked_combined_kernel_deriv1, and reads them back in
+ * its rho and Z kernels); 2 = only Z evaluates them (the fold reads the
+ * stored planes); 0 = both read the stored planes.  Bases with higher
+ * angular momentum always use the stored planes. */
+int xcq_set_density_basis_eval(xcq_context ctx, int32_t in_stage);
+/* The basis evaluation mode in effect (0 when the basis exceeds l = 3). */
+int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode);
 /* B^T Z contributions with |v| < tol are not scattered into VXC (each
  * scatter-add is an FP64 atomic; on large diffuse systems most are of
  * negligible function pairs).  Default 1e-18; 0 adds every contribution.
@@ -142,12 +153,15 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us);
 int xcq_use_graph(xcq_context ctx, int enable);
 /* Autotuner: times the uploaded workload on the device and keeps the
  * fastest settings.  Sweeps the 32/64 tile thresholds of the density and
- * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}), for
- * mixed precision the density occupancy threshold over the same buckets and
- * the 128-wide threshold over {128, 192, 256, 384, 512, off}, then the
- * persistent-queue count {0, 2, 4 per CU}; leaves the context on the best
- * and reports the 32-wide thresholds and queue count (any output pointer
- * may be NULL; xcq_get_tile_policy returns the rest). */
+ * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}); for
+ * mixed precision, per basis evaluation mode (in-stage when the basis
+ * allows it, then stored planes), the density 32-wide threshold, the
+ * occupancy threshold (stored planes) over the same buckets and the
+ * 128-wide threshold over {128, 192, 256, 384, 512, off}, keeping the mode
+ * with the fastest collocation + density; then the persistent-queue count
+ * {0, 2, 4 per CU}.  Leaves the context on the best and reports the 32-wide
+ * thresholds and queue count (any output pointer may be NULL;
+ * xcq_get_tile_policy and xcq_get_density_basis_eval return the rest). */
 int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
                  int32_t* syr2k32_max_nbe, int32_t* nqueues, double* best_ms);
 

@@ -303,7 +303,7 @@ def test_device_collocation_high_l(gpu, gx, orc, lmax):
             assert np.all(np.abs(out[k] - r) <= bound), (lmax, task, k)
 
 
-@pytest.mark.parametrize("lmax", [4])
+@pytest.mark.parametrize("lmax", [3, 4])
 def test_device_fp64_matches_oracle_high_l(gpu, gx, orc, lmax):
     """The whole FP64 pipeline on a basis with g functions and contracted
     shells against the oracle (PBE), and mixed against FP64."""

@@ -115,6 +115,34 @@ def test_density_wide_tiles(gx, small):
     ctx.set_density_occupancy(0)
 
 
+@pytest.mark.parametrize("func", ["PBE", "SVWN5"])
+def test_density_basis_eval_in_stage(gx, small, func):
+    """The mixed density stage's own FP64 evaluation of B and grad B (the
+    default) against reading the planes the collocation stage stored (FP32,
+    the value with its bfloat16 residual): both within the mixed bars of
+    the FP64 path, for 32-, 64- and 128-wide tiles and with queues."""
+    bs, P, t = small
+    ctx = _ctx(gx, bs, t, P)
+    e0, n0, v0 = _run(ctx, "fp64", func)
+    for dens32, wide, nq in [(48, 1 << 30, 0), (0, 0, 0), (100000, 1 << 30, 0), (48, 160, 300)]:
+        ctx.set_tile_policy(dens32, 96)
+        ctx.set_density_wide(wide)
+        ctx.set_schedule(nq)
+        res = {}
+        for in_stage in (True, False):
+            _flush(f"basis eval in stage={in_stage} {func} dens32={dens32} wide={wide} nq={nq}")
+            ctx.set_density_basis_eval(in_stage)
+            res[in_stage] = _run(ctx, "mixed", func)
+            e, n, v = res[in_stage]
+            assert abs(e - e0) <= 1e-6 and abs(n - n0) <= 1e-6
+            assert np.abs(v - v0).max() <= 1e-7
+        (e1, n1, v1), (e2, n2, v2) = res[True], res[False]
+        assert abs(e1 - e2) <= 1e-7 and abs(n1 - n2) <= 1e-7
+        assert np.abs(v1 - v2).max() <= 1e-8
+    ctx.set_schedule(0)
+    ctx.set_density_basis_eval(True)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_queued_schedule_matches_direct(gx, small, prec):
     """Persistent queues (heuristic and explicit costs) reproduce the
@@ -208,6 +236,7 @@ def test_autotune_keeps_results(gx, small):
     _flush(f"autotune -> {d} {s} {q} {ms:.3f} ms")
     cand = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30}
     assert d in cand and s in cand and q >= 0 and ms > 0
+    assert ctx.density_basis_eval() in (0, 1)
     e1, n1, v1 = _run(ctx, "mixed")
     assert abs(e1 - e0) <= 1e-9
     assert np.abs(v1 - v0).max() <= 1e-10
