@@ -39,16 +39,28 @@ PEAK_BF16_TFLOPS = 2500.0
 MIXED_PEAK = {"density": PEAK_FP32_TFLOPS, "syr2k_scatter": PEAK_BF16_TFLOPS}
 PEAK_HBM_GBS = 8000.0
 STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symmetrize"]
-# Kernels of each stage (the names rocprofv3 reports).
-STAGE_KERNELS = {
-    ("collocation", "fp64"): ["k_collocation<true, double, 3>"],
-    ("collocation", "mixed"): ["k_collocation<true, float, 3>"],
-    ("density", "fp64"): ["k_density_f64<32, true>", "k_density_f64<64, true>"],
-    ("density", "mixed"): ["k_density_mx<32, true, false>", "k_density_mx<64, true, true>",
-                           "k_density_mx<64, true, false>", "k_density_mx<128, true, false>"],
-    ("syr2k_scatter", "fp64"): ["k_syr2k<32>", "k_syr2k<64>"],
-    ("syr2k_scatter", "mixed"): ["k_syr2k_mx<32, 1>", "k_syr2k_mx<64, 1>"],
-}
+# Kernels of each stage (the names rocprofv3 reports); the mixed path's
+# kernels depend on its basis evaluation mode (0: stored planes, 1 / 3:
+# evaluated in the density stage, 3 with Z from its own launch).
+
+
+def stage_kernels(stage, prec, mode=0):
+    if prec == "fp64":
+        return {"collocation": ["k_collocation<true, double, 3>"],
+                "density": ["k_density_f64<32, true>", "k_density_f64<64, true>"],
+                "syr2k_scatter": ["k_syr2k<32>", "k_syr2k<64>"]}.get(stage, [])
+    if stage == "collocation":
+        return ["k_collocation<true, float, 3>" if mode == 0 else "k_collocation<false, float, 3>"]
+    if stage == "density":
+        ks = [f"k_density_mx<{nt}, true, false, {mode}>" for nt in (32, 64, 128)]
+        return ks + (["k_density_mx<64, true, true, 0>"] if mode == 0 else [])
+    if stage == "zmat":
+        return ["k_z_in_stage<true, 3>"] if mode == 3 else []
+    if stage == "syr2k_scatter":
+        return ["k_syr2k_mx<32, 1>", "k_syr2k_mx<64, 1>"]
+    return []
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_c1_24water.json")
 
 
@@ -56,7 +68,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic(stage, prec, ts):
+def algorithmic(stage, prec, ts, mode=0):
     """Algorithmic (bytes, flops) of one evaluation's launches of a stage.
 
     Per task t (npts, nbe).  Bytes per (point, function) element of the
@@ -68,6 +80,11 @@ def algorithmic(stage, prec, ts):
                    flops 2 npts nbe^2 (X = B P_sub)
       syr2k        reads B and Z; VXC lower triangle read + write
                    8 nbe (nbe + 1); flops 2 npts nbe^2 (B^T Z + Z^T B, lower)
+    With the mixed path's in-stage basis evaluation (mode 1 or 3) the
+    collocation stage writes only the value plane, the density stage reads
+    only its FP32 part (the X operand) and in mode 3 writes four FP64 Z
+    coefficients per point instead of Z, which the Z launch ("zmat") reads
+    before writing Z.
     """
     npts = ts["npts"].astype(np.float64)
     nbe = ts["nbe"].astype(np.float64)
@@ -78,6 +95,14 @@ def algorithmic(stage, prec, ts):
     pn = float(np.sum(npts * nbe))
     pnn = float(np.sum(npts * nbe * nbe))
     nn = float(np.sum(nbe * nbe))
+    if prec != "fp64" and mode in (1, 3):
+        if stage == "collocation":
+            return bv * pn + 32 * float(npts.sum()), 0.0
+        if stage == "density":
+            zout = 32 * float(npts.sum()) if mode == 3 else bz * pn
+            return 4.0 * pn + zout + bp * nn, 2 * pnn
+        if stage == "zmat" and mode == 3:
+            return bz * pn + 32 * float(npts.sum()), 0.0
     if stage == "collocation":
         return (bv + 3 * bg) * pn + 32 * float(npts.sum()), 0.0
     if stage == "density":
@@ -87,7 +112,7 @@ def algorithmic(stage, prec, ts):
     return 0.0, 0.0
 
 
-def measured_traffic(stage, prec):
+def measured_traffic(stage, prec, mode=0):
     """HBM bytes of the stage's launches from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy
     calibration factors it records), or None."""
@@ -98,7 +123,7 @@ def measured_traffic(stage, prec):
         return None
     total = 0.0
     found = False
-    for k in STAGE_KERNELS.get((stage, prec), []):
+    for k in stage_kernels(stage, prec, mode):
         e = pmc.get("kernels", {}).get(k)
         if e:
             total += (e["fetch_kib"] * pmc["fetch_scale"] + e["write_kib"] * pmc["write_scale"]) * 1024
@@ -180,7 +205,7 @@ def main():
             d_, s_, q_ = vals[:3]
             h_ = vals[3] if len(vals) > 3 else 0
             w_ = vals[4] if len(vals) > 4 else 1 << 30
-            m_ = vals[5] if len(vals) > 5 else 1
+            m_ = vals[5] if len(vals) > 5 else 3
             ctx.set_tile_policy(d_, s_)
             ctx.set_density_occupancy(h_)
             ctx.set_density_wide(w_)
@@ -215,7 +240,8 @@ def main():
         toc = time.perf_counter()
         exc, nel, vxc = ctx.results(want_vxc=True)
         ms = (toc - tic) * 1e3 / args.steps
-        results[prec] = dict(ms=ms, stage=stage, exc=exc, nel=nel, vxc=vxc, tuned=tuned)
+        results[prec] = dict(ms=ms, stage=stage, exc=exc, nel=nel, vxc=vxc, tuned=tuned,
+                             mode=ctx.density_basis_eval() if prec == "mixed" else 0)
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
             f"stages(ms)={np.round(stage, 3).tolist()}")
     ctx.use_graph(False)
@@ -242,7 +268,7 @@ def main():
     k_dom = int(np.argmax(r["stage"][:6]))
     name = STAGES[k_dom]
     dom_ms = float(r["stage"][k_dom])
-    abytes, aflops = algorithmic(name, head, ts)
+    abytes, aflops = algorithmic(name, head, ts, r["mode"])
     peak_mfma = PEAK_FP64_TFLOPS if head == "fp64" else MIXED_PEAK.get(name, PEAK_BF16_TFLOPS)
     ridge = peak_mfma * 1e12 / (PEAK_HBM_GBS * 1e9)
     if aflops > 0 and aflops / abytes > ridge:
@@ -252,8 +278,9 @@ def main():
         roof = dict(bound="hbm", achieved=abytes / (dom_ms * 1e-3) / 1e9, peak=PEAK_HBM_GBS,
                     unit="GB/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic(name, head)
-    roof["kernel"] = " + ".join(STAGE_KERNELS.get((name, head), [name]))
+    roof["traffic"] = measured_traffic(name, head, r["mode"])
+    roof["kernel"] = " + ".join(stage_kernels(name, head, r["mode"]) or [name])
+    roof["basis_eval_mode"] = r["mode"]
     roof["algorithmic_bytes"] = abytes
     roof["algorithmic_flops"] = aflops
     roof["stage_ms"] = dom_ms

@@ -74,6 +74,75 @@ constexpr int kCollPrims = 4;     // primitives per shell staged in LDS
 // spherical f (LMAX 3) with gradients fits 168 registers, g (LMAX 4) 256.
 constexpr int coll_waves(int lmax) { return lmax <= 2 ? 4 : (lmax == 3 ? 3 : 2); }
 
+// Shell parameters of a task in LDS, one pass of up to kCollShells shells.
+struct ShellStage {
+  double O[3][kCollShells];
+  double a[kCollPrims][kCollShells], c[kCollPrims][kCollShells];
+  int info[kCollShells];  // l | pure << 8 | nprim << 16
+  int bf0[kCollShells], ish[kCollShells];
+};
+
+// Walks the task's shells for the lane's point (gx, gy, gz): the shell
+// parameters are staged in LDS once per pass of up to kCollShells shells by
+// one thread each, so the shell loop reads them with LDS broadcasts instead
+// of a chain of dependent scalar loads per shell; primitives past the first
+// kCollPrims are read from the shell record.  Wave w takes shells w, w + 4,
+// ... (ascending) and calls body(l, pure, first local function, x, y, z,
+// R, dR) with the displacement from the centre, R = sum c e^{-a r^2} and
+// dR = -2 sum a c e^{-a r^2}.
+template <typename Body>
+__device__ __forceinline__ void for_block_shells(const DevTask& t,
+                                                 const DevShell* __restrict__ shells,
+                                                 const int32_t* __restrict__ shell_list,
+                                                 const int32_t* __restrict__ lsh_first,
+                                                 double gx, double gy, double gz, ShellStage& st,
+                                                 Body&& body) {
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int pass = 0; pass < t.nshells; pass += kCollShells) {
+    const int nsh = min(kCollShells, t.nshells - pass);
+    __syncthreads();  // the previous pass is done with the staged shells
+    for (int q = tid; q < nsh; q += 256) {
+      const int ish = shell_list[t.sh_off + pass + q];
+      const DevShell* sh = shells + ish;
+      st.ish[q] = ish;
+      st.bf0[q] = lsh_first[t.lsh_off + pass + q];
+      st.info[q] = sh->l | (sh->pure ? 1 << 8 : 0) | (sh->nprim << 16);
+      for (int d = 0; d < 3; ++d) st.O[d][q] = sh->O[d];
+      const int np = min(sh->nprim, kCollPrims);
+      for (int k = 0; k < np; ++k) {
+        st.a[k][q] = sh->alpha[k];
+        st.c[k][q] = sh->coeff[k];
+      }
+    }
+    __syncthreads();
+    for (int s = wave; s < nsh; s += 4) {
+      const int info = __builtin_amdgcn_readfirstlane(st.info[s]);
+      const int bf0 = __builtin_amdgcn_readfirstlane(st.bf0[s]);
+      const int l = info & 0xff;
+      const bool pure = (info >> 8) & 1;
+      const int nprim = info >> 16;
+      const double x = gx - st.O[0][s], y = gy - st.O[1][s], z = gz - st.O[2][s];
+      const double r2 = x * x + y * y + z * z;
+      double R = 0., dR = 0.;
+      for (int k = 0; k < nprim && k < kCollPrims; ++k) {
+        const double a = st.a[k][s], e = st.c[k][s] * exp(-a * r2);
+        R += e;
+        dR += -2. * a * e;
+      }
+      if (nprim > kCollPrims) {
+        const DevShell* sh = shells + __builtin_amdgcn_readfirstlane(st.ish[s]);
+        for (int k = kCollPrims; k < nprim; ++k) {
+          const double e = sh->coeff[k] * exp(-sh->alpha[k] * r2);
+          R += e;
+          dR += -2. * sh->alpha[k] * e;
+        }
+      }
+      body(l, pure, bf0, x, y, z, R, dR);
+    }
+  }
+}
+
 template <bool GRAD, typename T, int LMAX>
 __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const WorkItem* __restrict__ blocks,
                                                      const DevTask* __restrict__ tasks,
@@ -87,7 +156,6 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
   const WorkItem wi = blocks[blockIdx.x];
   const DevTask t = tasks[wi.task];
   const unsigned lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int p = wi.a * kPtBlock + (int)lane;
   const bool valid = p < t.npts;
   double gx = 0., gy = 0., gz = 0.;
@@ -109,54 +177,10 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
   ts.vo4 = (int)(lane * 4u);
   ts.vo2 = (int)(lane * 2u);
   ts.lo_base = (unsigned)(5 * plane * (int64_t)sizeof(T));
-  // Shell parameters of the task, staged in LDS once per pass of up to
-  // kCollShells shells by one thread each, so the shell loop reads them with
-  // LDS broadcasts instead of a chain of dependent scalar loads per shell.
-  // Primitives past the first kCollPrims are read from the shell record.
-  __shared__ double s_O[3][kCollShells];
-  __shared__ double s_a[kCollPrims][kCollShells], s_c[kCollPrims][kCollShells];
-  __shared__ int s_info[kCollShells];  // l | pure << 8 | nprim << 16
-  __shared__ int s_bf0[kCollShells], s_ish[kCollShells];
-  const int tid = threadIdx.x;
-  for (int pass = 0; pass < t.nshells; pass += kCollShells) {
-    const int nsh = min(kCollShells, t.nshells - pass);
-    __syncthreads();  // the previous pass is done with the staged shells
-    for (int q = tid; q < nsh; q += 256) {
-      const int ish = shell_list[t.sh_off + pass + q];
-      const DevShell* sh = shells + ish;
-      s_ish[q] = ish;
-      s_bf0[q] = lsh_first[t.lsh_off + pass + q];
-      s_info[q] = sh->l | (sh->pure ? 1 << 8 : 0) | (sh->nprim << 16);
-      for (int d = 0; d < 3; ++d) s_O[d][q] = sh->O[d];
-      const int np = min(sh->nprim, kCollPrims);
-      for (int k = 0; k < np; ++k) {
-        s_a[k][q] = sh->alpha[k];
-        s_c[k][q] = sh->coeff[k];
-      }
-    }
-    __syncthreads();
-  for (int s = wave; s < nsh; s += 4) {
-    const int info = __builtin_amdgcn_readfirstlane(s_info[s]);
-    const int bf0 = __builtin_amdgcn_readfirstlane(s_bf0[s]);
-    const int l = info & 0xff;
-    const bool pure = (info >> 8) & 1;
-    const int nprim = info >> 16;
-    const double x = gx - s_O[0][s], y = gy - s_O[1][s], z = gz - s_O[2][s];
-    const double r2 = x * x + y * y + z * z;
-    double R = 0., dR = 0.;
-    for (int k = 0; k < nprim && k < kCollPrims; ++k) {
-      const double a = s_a[k][s], e = s_c[k][s] * exp(-a * r2);
-      R += e;
-      dR += -2. * a * e;
-    }
-    if (nprim > kCollPrims) {
-      const DevShell* sh = shells + __builtin_amdgcn_readfirstlane(s_ish[s]);
-      for (int k = kCollPrims; k < nprim; ++k) {
-        const double e = sh->coeff[k] * exp(-sh->alpha[k] * r2);
-        R += e;
-        dR += -2. * sh->alpha[k] * e;
-      }
-    }
+  __shared__ ShellStage st;
+  for_block_shells(t, shells, shell_list, lsh_first, gx, gy, gz, st,
+                   [&](int l, bool pure, int bf0, double x, double y, double z, double R,
+                       double dR) {
     const int64_t row0 = (int64_t)bf0 * ld + m0;
     if (l == 0) shell_eval<0, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
     else if (l == 1) shell_eval<1, GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
@@ -166,8 +190,130 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
       shell_eval<(LMAX < 3 ? 2 : 3), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
     else
       shell_eval<(LMAX < 4 ? 2 : 4), GRAD, T>(x, y, z, R, dR, pure, valid, ts, row0, ld, plane);
+  });
+}
+
+// Z = a B + c . grad B for one (task, 64-point block), FP32 storage, with B
+// and grad B evaluated here in FP64 (the mixed path's in-stage mode that
+// leaves Z to this launch): the block's coefficients (a, cx, cy, cz per
+// point) come from the density stage, zcoef[block][4][64] with the block
+// numbered as the EXC/NEL partials.  Z goes to plane 4 of the task's
+// scratch, one coalesced 64-point row per function.  Padded points have
+// zero coefficients (Z = 0) and are evaluated at the task's last point.
+// Also records the bound table's granule maxima of |B| and |Z|: per lane
+// the bits of |x| under unsigned max (every NaN orders above infinity) per
+// shell and per current granule; a wave reduces them (clamped to infinity)
+// into LDS when it moves to another granule (its shells ascend), and a
+// shell that spans two granules counts in both, which can only loosen the
+// bounds.
+template <bool GGA, int LMAX>
+__global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const WorkItem* __restrict__ blocks,
+                                                      const DevTask* __restrict__ tasks,
+                                                      const DevShell* __restrict__ shells,
+                                                      const int32_t* __restrict__ shell_list,
+                                                      const int32_t* __restrict__ lsh_first,
+                                                      const double* __restrict__ px,
+                                                      const double* __restrict__ py,
+                                                      const double* __restrict__ pz,
+                                                      const double* __restrict__ zcoef,
+                                                      float* __restrict__ scratch,
+                                                      unsigned* __restrict__ tmax_all) {
+  const WorkItem wi = blocks[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int pl = min(wi.a * kPtBlock + lane, t.npts - 1);
+  const double gx = px[t.pt_off + pl], gy = py[t.pt_off + pl], gz = pz[t.pt_off + pl];
+  const double* zb = zcoef + (t.red_off + wi.a) * 256;
+  const double za = zb[lane], zx = zb[64 + lane], zy = zb[128 + lane], zz = zb[192 + lane];
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  float* Zp = scratch + t.b_off_mx + 4 * plane + (int64_t)wi.a * kPtBlock + lane;
+  unsigned* tmax = tmax_all ? tmax_all + 2 * t.tmax_off : nullptr;
+  __shared__ ShellStage st;
+  __shared__ unsigned smax[kGranLds][2];
+  if (tid < 2 * kGranLds) (&smax[0][0])[tid] = 0u;
+  constexpr unsigned kInfBits = 0x7f800000u, kAbs = 0x7fffffffu;
+  int cur = -1;               // granule of the running maxima (uniform)
+  unsigned mb = 0u, mz = 0u;  // running maxima of granule cur
+  auto flush = [&]() {
+    unsigned b = mb, z = mz;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      b = max(b, (unsigned)__shfl_xor((int)b, off));
+      z = max(z, (unsigned)__shfl_xor((int)z, off));
+    }
+    b = min(b, kInfBits);
+    z = min(z, kInfBits);
+    if (lane == 0) {
+      if (cur < kGranLds) {
+        atomicMax(&smax[cur][0], b);
+        atomicMax(&smax[cur][1], z);
+      } else {
+        atomicMax(tmax + 2 * cur, b);
+        atomicMax(tmax + 2 * cur + 1, z);
+      }
+    }
+  };
+  for_block_shells(t, shells, shell_list, lsh_first, gx, gy, gz, st,
+                   [&](int l, bool pure, int bf0, double x, double y, double z, double R,
+                       double dR) {
+    unsigned sb = 0u, sz = 0u;  // this shell's maxima
+    shell_functions_l<LMAX, GGA, true>(l, x, y, z, R, dR, pure,
+                                       [&](int f, double v, double vx, double vy, double vz) {
+      double zv = za * v;
+      if constexpr (GGA) zv += zx * vx + zy * vy + zz * vz;
+      const float zf = (float)zv;
+      Zp[(int64_t)(bf0 + f) * ld] = zf;
+      sb = max(sb, __float_as_uint((float)v) & kAbs);
+      sz = max(sz, __float_as_uint(zf) & kAbs);
+    });
+    if (tmax) {
+      const int size = pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+      const int g0 = bf0 / kGran, g1 = (bf0 + size - 1) / kGran;  // uniform
+      if (g0 != cur) {
+        if (cur >= 0) flush();
+        cur = g0;
+        mb = mz = 0u;
+      }
+      mb = max(mb, sb);
+      mz = max(mz, sz);
+      if (g1 != g0) {
+        flush();
+        cur = g1;
+        mb = sb;
+        mz = sz;
+      }
+    }
+  });
+  if (tmax) {
+    if (cur >= 0) flush();
+    __syncthreads();
+    const int ng = min((t.nbe + kGran - 1) / kGran, kGranLds);
+    if (tid < 2 * ng) atomicMax(tmax + tid, smax[tid >> 1][tid & 1]);
   }
+}
+
+int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const DevTask* tasks,
+                      const DevShell* shells, const int32_t* shell_list,
+                      const int32_t* lsh_first, const double* px, const double* py,
+                      const double* pz, const double* zcoef, float* scratch, unsigned* tmax,
+                      int gga, int lmax) {
+  if (n == 0) return XCQ_SUCCESS;
+  XCQ_REQUIRE(lmax >= 0 && lmax <= kInStageMaxL, XCQ_ERROR_INVALID_ARGUMENT,
+              "in-stage Z: angular momentum out of range");
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(256), 0, s, items, tasks, shells,
+                       shell_list, lsh_first, px, py, pz, zcoef, scratch, tmax);
+  };
+  if (gga) {
+    if (lmax <= 2) go(k_z_in_stage<true, 2>);
+    else go(k_z_in_stage<true, 3>);
+  } else {
+    if (lmax <= 2) go(k_z_in_stage<false, 2>);
+    else go(k_z_in_stage<false, 3>);
   }
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
 }
 
 // One block per (task, 64-point block) item; lmax is the highest angular

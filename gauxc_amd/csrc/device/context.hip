@@ -149,9 +149,10 @@ struct xcq_context_s {
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
   int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
   int32_t dens128_min_nbe = 1 << 30;  // 128-wide (mixed) density tiles from this nbe on
-  // Mixed density stage evaluates B / grad B itself (basis up to l = 3);
-  // otherwise it reads the gradient planes the collocation stage stores.
-  int32_t dens_in_stage = 1;
+  // Mixed density stage's basis evaluation (bases up to l = 3): 1 = its
+  // fold and Z evaluate B / grad B, 3 = its fold does and a separate launch
+  // forms Z, 0 = both read the planes the collocation stage stores.
+  int32_t dens_in_stage = 3;
   double vxc_drop = kDefaultVxcDrop;  // see xcq_set_vxc_drop_tolerance
   int32_t nqueues = 0;                                 // 0: one workgroup per item
   std::vector<double> cost_d, cost_s;                  // per-task predicted costs
@@ -164,6 +165,8 @@ struct xcq_context_s {
   bool pf32_valid = false;
   double* d_vxc = nullptr;
   double* d_red = nullptr;
+  double* d_zc = nullptr;   // Z coefficients per point block (in-stage mode 3), lazily
+  int64_t pblocks = 0;      // point blocks of the uploaded tasks
   double* d_out2 = nullptr;
   double* d_corr = nullptr;  // [nmol][kCorrBlocks] partials of the P32 correction
   double* d_rpart = nullptr;  // [nmol][kRedBlocks][2] partials of EXC/NEL
@@ -204,6 +207,8 @@ struct xcq_context_s {
     dfree(d_work);
     dfree(d_qoff);
     dfree(d_red);
+    dfree(d_zc);
+    pblocks = 0;
     dfree(d_tmax);
     tmax_total = 0;
     dfree(d_tstamp);
@@ -247,7 +252,7 @@ CtxView view_of(xcq_context_s* c) {
                  c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
                  c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch,
                  c->vxc_drop, reinterpret_cast<unsigned*>(c->d_tmax),
-                 c->max_l <= kInStageMaxL ? c->dens_in_stage : 0, c->d_shc};
+                 c->max_l <= kInStageMaxL ? c->dens_in_stage : 0, c->d_shc, c->d_zc};
 }
 
 // Heuristic per-item costs (used when no predicted task costs are set):
@@ -364,12 +369,13 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     else
       rc = launch_collocation_fp32(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
                                    v.py, v.pz, static_cast<float*>(v.scratch),
-                                   gga && v.in_stage != 1, ctx->max_l);
+                                   gga && v.in_stage == 0, ctx->max_l);
     if (rc) return rc;
     XCQ_HIP(rec(ev[1]));
     // X, rho/grad rho, functional, EXC/NEL and Z in one launch per tile
-    // width (stage 1); stages 2 (functional) and 3 (Z) stay as empty
-    // intervals so the stage table keeps its shape.
+    // width (stage 1); stage 2 (functional) stays an empty interval so the
+    // stage table keeps its shape, and so does stage 3 (Z) except in the
+    // mixed path's in-stage mode 3, where it times the Z launch.
     XCQ_HIP(rec(ev[2]));
     for (int k = 0; k < 4; ++k)
       if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : (k == 3 ? 128 : 64), k == 1,
@@ -382,6 +388,15 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     items += c.coll.n;
     if (want_vxc) {
       XCQ_HIP(rec(ev[6]));
+      // In-stage mode 3: Z (stage 3) from the coefficients the density
+      // stage left.
+      if (precision == XCQ_PREC_MIXED && v.in_stage == 3) {
+        if ((rc = launch_z_in_stage(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                    v.px, v.py, v.pz, v.zc, static_cast<float*>(v.scratch),
+                                    v.vxc_drop > 0.0 ? v.tmax : nullptr, gga, ctx->max_l)))
+          return rc;
+        ++launches;
+      }
       XCQ_HIP(rec(ev[7]));
       XCQ_HIP(rec(ev[8]));
       for (int k = 0; k < 2; ++k)
@@ -440,6 +455,11 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
   if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
+  if (precision == XCQ_PREC_MIXED && want_vxc && view_of(ctx).in_stage == 3 && !ctx->d_zc) {
+    int rc = dalloc(&ctx->d_zc, (size_t)256 * std::max<int64_t>(ctx->pblocks, 1));
+    if (rc) return rc;
+    ctx->drop_graph();
+  }
   ctx->last_vxc_eval = want_vxc;
   if (!ctx->use_graph) {
     ctx->last_graph = false;
@@ -792,6 +812,7 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
   if ((rc = dalloc(&ctx->d_bfmap, bfmap.size()))) return rc;
   if ((rc = dalloc(&ctx->d_lsh, lsh.size()))) return rc;
   if ((rc = dalloc(&ctx->d_red, (size_t)2 * std::max<int64_t>(pblocks, 1)))) return rc;
+  ctx->pblocks = pblocks;
   dfree(ctx->d_tmax);
   if ((rc = dalloc(&ctx->d_tmax, (size_t)std::max<int64_t>(tmax_total, 1)))) return rc;
   ctx->tmax_total = tmax_total;
@@ -899,8 +920,8 @@ int xcq_set_density_basis_eval(xcq_context ctx, int32_t in_stage) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_HIP(hipSetDevice(ctx->device));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
-  XCQ_REQUIRE(in_stage >= 0 && in_stage <= 2, XCQ_ERROR_INVALID_ARGUMENT,
-              "density basis evaluation mode must be 0, 1 or 2");
+  XCQ_REQUIRE(in_stage == 0 || in_stage == 1 || in_stage == 3, XCQ_ERROR_INVALID_ARGUMENT,
+              "density basis evaluation mode must be 0, 1 or 3");
   ctx->dens_in_stage = in_stage;
   ctx->drop_graph();  // the captured launches follow the old setting
   return XCQ_SUCCESS;
@@ -1053,7 +1074,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
       double ms;
     };
     std::vector<int32_t> modes;
-    if (ctx->max_l <= kInStageMaxL) modes.push_back(1);
+    if (ctx->max_l <= kInStageMaxL) modes.push_back(3);
     modes.push_back(0);
     Pol best{ctx->dens_in_stage, best_d, ctx->dens_hiocc_max_nbe, ctx->dens128_min_nbe, 1e300};
     for (int32_t mode : modes) {

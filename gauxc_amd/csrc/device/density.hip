@@ -484,11 +484,12 @@ __device__ __forceinline__ unsigned wave_max(unsigned v) {
 // task, with B and grad B evaluated here; stored in FP32 (plane 4), one
 // coalesced 64-point row per function.  Padded points have zero
 // coefficients (so Z = 0) and are evaluated at the task's last point.
-// Also records the granule maxima of |B| and |Z| as write_z does: per lane
-// as the bits of |x| under unsigned max (which orders every NaN above
-// infinity), reduced across the wave when the granule changes (each wave
-// meets the granules in ascending order) and clamped to infinity there, so
-// a NaN is never screened out.
+// Also records the granule maxima of |B| and |Z| as write_z does, per
+// shell: each lane keeps the bits of |x| under unsigned max (which orders
+// every NaN above infinity) for the shell and for the current granule; the
+// wave reduces and clamps them to infinity when it moves to another granule
+// (each wave meets the granules in ascending order).  A shell that spans
+// two granules counts in both, which can only loosen the bounds.
 template <bool GGA>
 __device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa, int m0,
                                            int wave, int lane, int tid, double rx, double ry,
@@ -499,8 +500,8 @@ __device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa
   const double za = zc[0][lane], zx = zc[1][lane], zy = zc[2][lane], zz = zc[3][lane];
   float* Zp = Bm + 4 * plane + m0 + lane;
   constexpr unsigned kInfBits = 0x7f800000u, kAbs = 0x7fffffffu;
-  int cur = -1;
-  unsigned mb = 0u, mz = 0u;
+  int cur = -1;               // granule of the running maxima (uniform)
+  unsigned mb = 0u, mz = 0u;  // running maxima of granule cur
   auto flush = [&]() {
     const unsigned b = min(wave_max(mb), kInfBits), z = min(wave_max(mz), kInfBits);
     if (lane == 0) {
@@ -514,23 +515,33 @@ __device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa
     }
   };
   for_range_shells(t, sa, 0, nbe, wave, lane, [&](const ShellU& u) {
+    unsigned sb = 0u, sz = 0u;  // this shell's maxima
     eval_shell<GGA>(sa, u, rx, ry, rz, [&](int f, double v, double gx, double gy, double gz) {
-      const int n = u.bf0 + f;  // uniform
       double zv = za * v;
       if constexpr (GGA) zv += zx * gx + zy * gy + zz * gz;
       const float zf = (float)zv;
-      Zp[(int64_t)n * ld] = zf;
-      if (tmax) {
-        const int g = __builtin_amdgcn_readfirstlane(n / kGran);
-        if (g != cur) {
-          if (cur >= 0) flush();
-          cur = g;
-          mb = mz = 0u;
-        }
-        mb = max(mb, __float_as_uint((float)v) & kAbs);
-        mz = max(mz, __float_as_uint(zf) & kAbs);
-      }
+      Zp[(int64_t)(u.bf0 + f) * ld] = zf;
+      sb = max(sb, __float_as_uint((float)v) & kAbs);
+      sz = max(sz, __float_as_uint(zf) & kAbs);
     });
+    if (tmax) {
+      const int l = u.info & 0xff;
+      const int size = (u.info >> 8) & 1 ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+      const int g0 = u.bf0 / kGran, g1 = (u.bf0 + size - 1) / kGran;  // uniform
+      if (g0 != cur) {
+        if (cur >= 0) flush();
+        cur = g0;
+        mb = mz = 0u;
+      }
+      mb = max(mb, sb);
+      mz = max(mz, sz);
+      if (g1 != g0) {
+        flush();
+        cur = g1;
+        mb = sb;
+        mz = sz;
+      }
+    }
   });
   if (tmax) {
     if (cur >= 0) flush();
@@ -725,14 +736,16 @@ __device__ __forceinline__ void load_lds(const float* src, float* lds_row) {
 }
 
 // INSTAGE = 1: the fold and Z evaluate B and grad B themselves
-// (fold_in_stage / z_in_stage); 2: only Z does, the fold reads the stored
-// planes; 0: both read the planes the collocation stage stored.
+// (fold_in_stage / z_in_stage); 3: the fold does, and the block's Z
+// coefficients go to zbuf for a separate Z launch; 0: both read the planes
+// the collocation stage stored.
 template <int NT, bool GGA, bool HIOCC, int INSTAGE>
 __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const float* __restrict__ Pfall, float* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
-    double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa) {
+    double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa,
+    double* __restrict__ zbuf) {
   constexpr int BI = NT / 32;
   constexpr int kStages = ring_stages<NT, HIOCC>();
   constexpr int NPC = (NT + 63) / 64;  // 64-column chunks of a P_sub stage row
@@ -743,7 +756,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   constexpr int kPtFloats = kStages * 32 * (NT + 4);
   constexpr int kRingFloats = kPtFloats + kStages * 32 * 68;
   static_assert(NT > 64 || NT * kXLd * 2 <= kRingFloats, "X^T tile must fit the ring");
-  static_assert(INSTAGE != 1 || (NT < 64 ? NT : 64) * kXLd * 2 <= kRingFloats,
+  static_assert((INSTAGE != 1 && INSTAGE != 3) || (NT < 64 ? NT : 64) * kXLd * 2 <= kRingFloats,
                 "X^T rows of one in-stage fold must fit the ring");
   __shared__ __attribute__((aligned(16))) float ring[kRingFloats];
   float(*Pt)[32][NT + 4] = reinterpret_cast<float(*)[32][NT + 4]>(ring);
@@ -866,7 +879,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[a][b][r] *= fac;
-      if constexpr (INSTAGE == 1) {
+      if constexpr (INSTAGE == 1 || INSTAGE == 3) {
         // The X^T tile goes through LDS in halves of 64 rows (the row block
         // of waves with wm = h for 128-wide tiles).
         double* xt = reinterpret_cast<double*>(ring);
@@ -899,7 +912,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
       // finished reading it (last K blocks, or the X^T tile) at this barrier.
       __syncthreads();
     }
-    if constexpr (INSTAGE == 1) {
+    if constexpr (INSTAGE == 1 || INSTAGE == 3) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) dens[wave][lane][c] = part[c];
       __syncthreads();
@@ -907,7 +920,9 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
     finish_points<4>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      if constexpr (INSTAGE != 0)
+      if constexpr (INSTAGE == 3)
+        zbuf[(t.red_off + wi.a) * 256 + tid] = (&zc[0][0])[tid];
+      else if constexpr (INSTAGE == 1)
         z_in_stage<GGA>(t, sa, m0, wave, lane, tid, rx, ry, rz, zc, Bm, smax,
                         tmax ? tmax + 2 * t.tmax_off : nullptr);
       else
@@ -942,7 +957,7 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
     const ShellArgs sa{v.shells, v.shc, v.shell_list, v.lsh, v.px, v.py, v.pz};
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.Pf32, scr, v.w,
-                         func, fac, want_z, v.red, tm, sa);
+                         func, fac, want_z, v.red, tm, sa, v.zc);
     };
     // Template arguments: tile width, GGA, high occupancy, in-stage mode.
     auto pick = [&](auto g, auto ins) {
@@ -965,11 +980,11 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
     using F_ = std::false_type;
     using M0 = std::integral_constant<int, 0>;
     using M1 = std::integral_constant<int, 1>;
-    using M2 = std::integral_constant<int, 2>;
+    using M3 = std::integral_constant<int, 3>;
     if (v.in_stage == 1)
       gga ? pick(T_{}, M1{}) : pick(F_{}, M1{});
-    else if (v.in_stage == 2)
-      gga ? pick(T_{}, M2{}) : pick(F_{}, M2{});
+    else if (v.in_stage == 3)
+      gga ? pick(T_{}, M3{}) : pick(F_{}, M3{});
     else
       gga ? pick(T_{}, M0{}) : pick(F_{}, M0{});
   }

@@ -125,6 +125,7 @@ __device__ __forceinline__ void shell_functions(double x, double y, double z, do
 #pragma unroll
       for (int m = 0; m < 2 * L + 1; ++m) {
         double v = 0., wx = 0., wy = 0., wz = 0.;
+        bool first = true;  // the first nonzero term sets the sums (no "+ 0")
         int ic = 0;
 #pragma unroll
         for (int a = L; a >= 0; --a) {
@@ -134,12 +135,13 @@ __device__ __forceinline__ void shell_functions(double x, double y, double z, do
             if (cf != 0.) {
               double cv, gx = 0., gy = 0., gz = 0.;
               term(a, b, cv, gx, gy, gz);
-              v += cf * cv;
+              v = first ? cf * cv : v + cf * cv;
               if constexpr (GRAD) {
-                wx += cf * gx;
-                wy += cf * gy;
-                wz += cf * gz;
+                wx = first ? cf * gx : wx + cf * gx;
+                wy = first ? cf * gy : wy + cf * gy;
+                wz = first ? cf * gz : wz + cf * gz;
               }
+              first = false;
             }
             ++ic;
           }

@@ -115,14 +115,15 @@ int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
  * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).
  * Default: off (2^30). */
 int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe);
-/* Mixed precision: 1 (default) = the density stage evaluates the basis
- * functions' values and gradients at its points itself (bases up to l = 3)
- * for its rho fold and for Z, so the collocation stage stores only the
- * value plane (GauXC stores B and grad B,
+/* Mixed precision, where the rho fold and Z get the basis functions'
+ * values and gradients (GauXC stores B and grad B,
  * collocation_device_masked_combined_kernel_deriv1, and reads them back in
- * its rho and Z kernels); 2 = only Z evaluates them (the fold reads the
- * stored planes); 0 = both read the stored planes.  Bases with higher
- * angular momentum always use the stored planes. */
+ * its rho and Z kernels): 3 (default) = the density stage evaluates them
+ * at its points for the fold and leaves Z to a separate launch that
+ * evaluates them again; 1 = the density stage evaluates them for both;
+ * 0 = both read the planes the collocation stage stores.  In modes 1 and 3
+ * the collocation stage stores only the value plane.  Bases with angular
+ * momentum above 3 always use the stored planes.  Other values: error. */
 int xcq_set_density_basis_eval(xcq_context ctx, int32_t in_stage);
 /* The basis evaluation mode in effect (0 when the basis exceeds l = 3). */
 int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode);

@@ -117,10 +117,12 @@ def test_density_wide_tiles(gx, small):
 
 @pytest.mark.parametrize("func", ["PBE", "SVWN5"])
 def test_density_basis_eval_in_stage(gx, small, func):
-    """The mixed density stage's own FP64 evaluation of B and grad B (the
-    default) against reading the planes the collocation stage stored (FP32,
-    the value with its bfloat16 residual): both within the mixed bars of
-    the FP64 path, for 32-, 64- and 128-wide tiles and with queues."""
+    """The mixed density stage's own FP64 evaluation of B and grad B (for
+    the fold and Z: mode 1; for the fold, with Z from a separate launch: 3)
+    against reading the planes the collocation stage stored (mode 0: FP32,
+    the value with its bfloat16 residual): all within the mixed bars of the
+    FP64 path and close to each other, for 32-, 64- and 128-wide tiles and
+    with queues."""
     bs, P, t = small
     ctx = _ctx(gx, bs, t, P)
     e0, n0, v0 = _run(ctx, "fp64", func)
@@ -129,18 +131,26 @@ def test_density_basis_eval_in_stage(gx, small, func):
         ctx.set_density_wide(wide)
         ctx.set_schedule(nq)
         res = {}
-        for in_stage in (True, False):
-            _flush(f"basis eval in stage={in_stage} {func} dens32={dens32} wide={wide} nq={nq}")
-            ctx.set_density_basis_eval(in_stage)
-            res[in_stage] = _run(ctx, "mixed", func)
-            e, n, v = res[in_stage]
+        for mode in (1, 3, 0):
+            _flush(f"basis eval mode {mode} {func} dens32={dens32} wide={wide} nq={nq}")
+            ctx.set_density_basis_eval(mode)
+            assert ctx.density_basis_eval() == mode
+            res[mode] = _run(ctx, "mixed", func)
+            e, n, v = res[mode]
             assert abs(e - e0) <= 1e-6 and abs(n - n0) <= 1e-6
             assert np.abs(v - v0).max() <= 1e-7
-        (e1, n1, v1), (e2, n2, v2) = res[True], res[False]
-        assert abs(e1 - e2) <= 1e-7 and abs(n1 - n2) <= 1e-7
-        assert np.abs(v1 - v2).max() <= 1e-8
+        for mode in (1, 3):
+            e, n, v = res[mode]
+            assert abs(e - res[0][0]) <= 1e-7 and abs(n - res[0][1]) <= 1e-7
+            assert np.abs(v - res[0][2]).max() <= 1e-8
+        # Modes 1 and 3 evaluate the same fold and Z values in FP64.
+        assert abs(res[1][0] - res[3][0]) <= 1e-11
+        assert np.abs(res[1][2] - res[3][2]).max() <= 1e-12
     ctx.set_schedule(0)
-    ctx.set_density_basis_eval(True)
+    ctx.set_density_basis_eval(3)
+    for bad in (2, 4):
+        with pytest.raises(gx.GauXCError):
+            ctx.set_density_basis_eval(bad)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -236,7 +246,7 @@ def test_autotune_keeps_results(gx, small):
     _flush(f"autotune -> {d} {s} {q} {ms:.3f} ms")
     cand = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30}
     assert d in cand and s in cand and q >= 0 and ms > 0
-    assert ctx.density_basis_eval() in (0, 1)
+    assert ctx.density_basis_eval() in (0, 3)
     e1, n1, v1 = _run(ctx, "mixed")
     assert abs(e1 - e0) <= 1e-9
     assert np.abs(v1 - v0).max() <= 1e-10
