@@ -1092,7 +1092,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
         }
       }
       if ((rc = xcq_set_tile_policy(ctx, p.d, best_s))) return rc;
-      if (mode == 0) {
+      if (mode != 1) {
         for (int32_t c : cand) {
           if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
           double dm, sm;

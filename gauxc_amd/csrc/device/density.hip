@@ -967,12 +967,13 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
         go(k_density_mx<32, G, false, I>);
       } else if (nt == 128) {
         go(k_density_mx<128, G, false, I>);
-      } else if constexpr (I == 0) {
-        if (hiocc) go(k_density_mx<64, G, true, 0>);
-        else go(k_density_mx<64, G, false, 0>);
+      } else if constexpr (I != 1) {
+        if (hiocc) go(k_density_mx<64, G, true, I>);
+        else go(k_density_mx<64, G, false, I>);
       } else {
-        // The in-stage evaluation needs more than the 168 registers of the
-        // three-workgroups-per-CU configuration; it runs at two.
+        // The in-stage fold with the in-kernel Z pass needs more than the
+        // 168 registers of the three-workgroups-per-CU configuration; it
+        // runs at two.
         go(k_density_mx<64, G, false, I>);
       }
     };

@@ -126,13 +126,16 @@ def test_density_basis_eval_in_stage(gx, small, func):
     bs, P, t = small
     ctx = _ctx(gx, bs, t, P)
     e0, n0, v0 = _run(ctx, "fp64", func)
-    for dens32, wide, nq in [(48, 1 << 30, 0), (0, 0, 0), (100000, 1 << 30, 0), (48, 160, 300)]:
+    for dens32, wide, nq, hiocc in [(48, 1 << 30, 0, 0), (0, 0, 0, 0), (100000, 1 << 30, 0, 0),
+                                    (48, 160, 300, 0), (0, 1 << 30, 0, 100000)]:
         ctx.set_tile_policy(dens32, 96)
         ctx.set_density_wide(wide)
+        ctx.set_density_occupancy(hiocc)
         ctx.set_schedule(nq)
         res = {}
         for mode in (1, 3, 0):
-            _flush(f"basis eval mode {mode} {func} dens32={dens32} wide={wide} nq={nq}")
+            _flush(f"basis eval mode {mode} {func} dens32={dens32} wide={wide} nq={nq} "
+                   f"hiocc={hiocc}")
             ctx.set_density_basis_eval(mode)
             assert ctx.density_basis_eval() == mode
             res[mode] = _run(ctx, "mixed", func)
@@ -147,6 +150,7 @@ def test_density_basis_eval_in_stage(gx, small, func):
         assert abs(res[1][0] - res[3][0]) <= 1e-11
         assert np.abs(res[1][2] - res[3][2]).max() <= 1e-12
     ctx.set_schedule(0)
+    ctx.set_density_occupancy(0)
     ctx.set_density_basis_eval(3)
     for bad in (2, 4):
         with pytest.raises(gx.GauXCError):
