@@ -45,17 +45,18 @@ STAGES = ["collocation", "density", "functional", "zmat", "syr2k_scatter", "symm
 
 
 def stage_kernels(stage, prec, mode=0):
-    if prec == "fp64":
-        return {"collocation": ["k_collocation<true, double, 3>"],
-                "density": ["k_density_f64<32, true>", "k_density_f64<64, true>"],
-                "syr2k_scatter": ["k_syr2k<32>", "k_syr2k<64>"]}.get(stage, [])
+    t = "double" if prec == "fp64" else "float"
     if stage == "collocation":
-        return ["k_collocation<true, float, 3>" if mode == 0 else "k_collocation<false, float, 3>"]
-    if stage == "density":
-        ks = [f"k_density_mx<{nt}, true, false, {mode}>" for nt in (32, 64, 128)]
-        return ks + (["k_density_mx<64, true, true, 0>"] if mode == 0 else [])
+        return [f"k_collocation<{'true' if mode == 0 else 'false'}, {t}, 3>"]
     if stage == "zmat":
-        return ["k_z_in_stage<true, 3>"] if mode == 3 else []
+        return [f"k_z_in_stage<true, 3, {t}>"] if mode == 3 else []
+    if prec == "fp64":
+        return {"density": [f"k_density_f64<{nt}, true, {mode}>" for nt in (32, 64)],
+                "syr2k_scatter": ["k_syr2k<32>", "k_syr2k<64>"]}.get(stage, [])
+    if stage == "density":
+        return [f"k_density_mx<{nt}, true, {hi}, {mode}>"
+                for nt, hi in ((32, "false"), (64, "false"), (64, "true"), (128, "false"))
+                if not (hi == "true" and mode == 1)]
     if stage == "syr2k_scatter":
         return ["k_syr2k_mx<32, 1>", "k_syr2k_mx<64, 1>"]
     return []
@@ -80,11 +81,11 @@ def algorithmic(stage, prec, ts, mode=0):
                    flops 2 npts nbe^2 (X = B P_sub)
       syr2k        reads B and Z; VXC lower triangle read + write
                    8 nbe (nbe + 1); flops 2 npts nbe^2 (B^T Z + Z^T B, lower)
-    With the mixed path's in-stage basis evaluation (mode 1 or 3) the
+    With the in-stage basis evaluation (mode 1 or 3; FP64 runs 3) the
     collocation stage writes only the value plane, the density stage reads
-    only its FP32 part (the X operand) and in mode 3 writes four FP64 Z
-    coefficients per point instead of Z, which the Z launch ("zmat") reads
-    before writing Z.
+    only the X operand's part of it (FP32 for mixed, FP64) and in mode 3
+    writes four FP64 Z coefficients per point instead of Z, which the Z
+    launch ("zmat") reads before writing Z.
     """
     npts = ts["npts"].astype(np.float64)
     nbe = ts["nbe"].astype(np.float64)
@@ -95,12 +96,13 @@ def algorithmic(stage, prec, ts, mode=0):
     pn = float(np.sum(npts * nbe))
     pnn = float(np.sum(npts * nbe * nbe))
     nn = float(np.sum(nbe * nbe))
-    if prec != "fp64" and mode in (1, 3):
+    if mode in (1, 3):
+        xop = 8.0 if prec == "fp64" else 4.0
         if stage == "collocation":
             return bv * pn + 32 * float(npts.sum()), 0.0
         if stage == "density":
             zout = 32 * float(npts.sum()) if mode == 3 else bz * pn
-            return 4.0 * pn + zout + bp * nn, 2 * pnn
+            return xop * pn + zout + bp * nn, 2 * pnn
         if stage == "zmat" and mode == 3:
             return bz * pn + 32 * float(npts.sum()), 0.0
     if stage == "collocation":
@@ -240,8 +242,11 @@ def main():
         toc = time.perf_counter()
         exc, nel, vxc = ctx.results(want_vxc=True)
         ms = (toc - tic) * 1e3 / args.steps
+        mode = ctx.density_basis_eval()
+        if prec == "fp64" and mode:
+            mode = 3  # the FP64 kernels' in-stage mode
         results[prec] = dict(ms=ms, stage=stage, exc=exc, nel=nel, vxc=vxc, tuned=tuned,
-                             mode=ctx.density_basis_eval() if prec == "mixed" else 0)
+                             mode=mode)
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
             f"stages(ms)={np.round(stage, 3).tolist()}")
     ctx.use_graph(False)

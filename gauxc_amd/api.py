@@ -426,9 +426,9 @@ class DeviceContext:
                     "xcq_set_vxc_drop_tolerance")
 
     def set_density_basis_eval(self, mode):
-        """Mixed precision: 3 (default) = the density stage evaluates basis
-        values/gradients for its fold and a separate launch for Z; 1 = the
-        density stage for both; 0 (or False) = both read the planes the
+        """3 (default) = the density stage evaluates basis values/gradients
+        for its fold and a separate launch for Z; 1 = the mixed density stage
+        for both (FP64 runs 3); 0 (or False) = both read the planes the
         collocation stage stored."""
         N.xcq_check(self.lib.xcq_set_density_basis_eval(self.h, int(mode)),
                     "xcq_set_density_basis_eval")

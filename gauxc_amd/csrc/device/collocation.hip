@@ -11,6 +11,7 @@
 #include "shell_eval.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace xcq {
 
@@ -193,12 +194,13 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
   });
 }
 
-// Z = a B + c . grad B for one (task, 64-point block), FP32 storage, with B
-// and grad B evaluated here in FP64 (the mixed path's in-stage mode that
-// leaves Z to this launch): the block's coefficients (a, cx, cy, cz per
-// point) come from the density stage, zcoef[block][4][64] with the block
-// numbered as the EXC/NEL partials.  Z goes to plane 4 of the task's
-// scratch, one coalesced 64-point row per function.  Padded points have
+// Z = a B + c . grad B for one (task, 64-point block), stored as T (FP32
+// for the mixed path, FP64 for the FP64 path), with B and grad B evaluated
+// here in FP64 (the in-stage mode that leaves Z to this launch): the
+// block's coefficients (a, cx, cy, cz per point) come from the density
+// stage, zcoef[block][4][64] with the block numbered as the EXC/NEL
+// partials.  Z goes to plane 4 of the task's scratch, one coalesced
+// 64-point row per function.  Padded points have
 // zero coefficients (Z = 0) and are evaluated at the task's last point.
 // Also records the bound table's granule maxima of |B| and |Z|: per lane
 // the bits of |x| under unsigned max (every NaN orders above infinity) per
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
 // into LDS when it moves to another granule (its shells ascend), and a
 // shell that spans two granules counts in both, which can only loosen the
 // bounds.
-template <bool GGA, int LMAX>
+template <bool GGA, int LMAX, typename T>
 __global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const WorkItem* __restrict__ blocks,
                                                       const DevTask* __restrict__ tasks,
                                                       const DevShell* __restrict__ shells,
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const Wor
                                                       const double* __restrict__ py,
                                                       const double* __restrict__ pz,
                                                       const double* __restrict__ zcoef,
-                                                      float* __restrict__ scratch,
+                                                      T* __restrict__ scratch,
                                                       unsigned* __restrict__ tmax_all) {
   const WorkItem wi = blocks[blockIdx.x];
   const DevTask t = tasks[wi.task];
@@ -227,7 +229,8 @@ __global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const Wor
   const double za = zb[lane], zx = zb[64 + lane], zy = zb[128 + lane], zz = zb[192 + lane];
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
-  float* Zp = scratch + t.b_off_mx + 4 * plane + (int64_t)wi.a * kPtBlock + lane;
+  T* Zp = scratch + (sizeof(T) == 4 ? t.b_off_mx : t.b_off) + 4 * plane +
+          (int64_t)wi.a * kPtBlock + lane;
   unsigned* tmax = tmax_all ? tmax_all + 2 * t.tmax_off : nullptr;
   __shared__ ShellStage st;
   __shared__ unsigned smax[kGranLds][2];
@@ -262,10 +265,9 @@ __global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const Wor
                                        [&](int f, double v, double vx, double vy, double vz) {
       double zv = za * v;
       if constexpr (GGA) zv += zx * vx + zy * vy + zz * vz;
-      const float zf = (float)zv;
-      Zp[(int64_t)(bf0 + f) * ld] = zf;
+      Zp[(int64_t)(bf0 + f) * ld] = (T)zv;
       sb = max(sb, __float_as_uint((float)v) & kAbs);
-      sz = max(sz, __float_as_uint(zf) & kAbs);
+      sz = max(sz, __float_as_uint((float)zv) & kAbs);
     });
     if (tmax) {
       const int size = pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
@@ -296,22 +298,27 @@ __global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const Wor
 int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const DevTask* tasks,
                       const DevShell* shells, const int32_t* shell_list,
                       const int32_t* lsh_first, const double* px, const double* py,
-                      const double* pz, const double* zcoef, float* scratch, unsigned* tmax,
-                      int gga, int lmax) {
+                      const double* pz, const double* zcoef, void* scratch, int precision,
+                      unsigned* tmax, int gga, int lmax) {
   if (n == 0) return XCQ_SUCCESS;
   XCQ_REQUIRE(lmax >= 0 && lmax <= kInStageMaxL, XCQ_ERROR_INVALID_ARGUMENT,
               "in-stage Z: angular momentum out of range");
-  auto go = [&](auto kern) {
+  auto go = [&](auto kern, auto* scr) {
     hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(256), 0, s, items, tasks, shells,
-                       shell_list, lsh_first, px, py, pz, zcoef, scratch, tmax);
+                       shell_list, lsh_first, px, py, pz, zcoef, scr, tmax);
   };
-  if (gga) {
-    if (lmax <= 2) go(k_z_in_stage<true, 2>);
-    else go(k_z_in_stage<true, 3>);
-  } else {
-    if (lmax <= 2) go(k_z_in_stage<false, 2>);
-    else go(k_z_in_stage<false, 3>);
-  }
+  auto pick = [&](auto* scr) {
+    using T = std::remove_pointer_t<decltype(scr)>;
+    if (gga) {
+      if (lmax <= 2) go(k_z_in_stage<true, 2, T>, scr);
+      else go(k_z_in_stage<true, 3, T>, scr);
+    } else {
+      if (lmax <= 2) go(k_z_in_stage<false, 2, T>, scr);
+      else go(k_z_in_stage<false, 3, T>, scr);
+    }
+  };
+  if (precision == XCQ_PREC_FP64) pick(static_cast<double*>(scratch));
+  else pick(static_cast<float*>(scratch));
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -348,9 +348,17 @@ int build_work_lists(xcq_context_s* ctx) {
 
 // Enqueue one evaluation on the context's stream.  Events are recorded only
 // when `events` is set (not under graph capture).
+// The basis evaluation mode a precision runs: the FP64 kernels have the
+// stored planes (0) and the in-stage fold with a Z launch (3).
+int effective_mode(const CtxView& v, int precision) {
+  if (precision == XCQ_PREC_FP64) return v.in_stage != 0 ? 3 : 0;
+  return v.in_stage;
+}
+
 int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, bool events) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
   const CtxView v = view_of(ctx);
+  const int mode = effective_mode(v, precision);
   hipStream_t s = ctx->stream;
   const double fac = func == 0 ? 1.0 : 2.0;
   int rc;
@@ -365,11 +373,12 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
     const WorkItem* cw = v.work + c.coll.off;
     if (precision == XCQ_PREC_FP64)
       rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
-                                   v.py, v.pz, static_cast<double*>(v.scratch), gga, ctx->max_l);
+                                   v.py, v.pz, static_cast<double*>(v.scratch),
+                                   gga && mode == 0, ctx->max_l);
     else
       rc = launch_collocation_fp32(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh, v.px,
                                    v.py, v.pz, static_cast<float*>(v.scratch),
-                                   gga && v.in_stage == 0, ctx->max_l);
+                                   gga && mode == 0, ctx->max_l);
     if (rc) return rc;
     XCQ_HIP(rec(ev[1]));
     // X, rho/grad rho, functional, EXC/NEL and Z in one launch per tile
@@ -390,9 +399,9 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
       XCQ_HIP(rec(ev[6]));
       // In-stage mode 3: Z (stage 3) from the coefficients the density
       // stage left.
-      if (precision == XCQ_PREC_MIXED && v.in_stage == 3) {
+      if (mode == 3) {
         if ((rc = launch_z_in_stage(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                    v.px, v.py, v.pz, v.zc, static_cast<float*>(v.scratch),
+                                    v.px, v.py, v.pz, v.zc, v.scratch, precision,
                                     v.vxc_drop > 0.0 ? v.tmax : nullptr, gga, ctx->max_l)))
           return rc;
         ++launches;
@@ -455,7 +464,7 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
   if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
-  if (precision == XCQ_PREC_MIXED && want_vxc && view_of(ctx).in_stage == 3 && !ctx->d_zc) {
+  if (want_vxc && effective_mode(view_of(ctx), precision) == 3 && !ctx->d_zc) {
     int rc = dalloc(&ctx->d_zc, (size_t)256 * std::max<int64_t>(ctx->pblocks, 1));
     if (rc) return rc;
     ctx->drop_graph();
@@ -1030,7 +1039,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
       double st[kStages + 1];
       xcq_stage_times(ctx, st, kStages + 1);
       if (rep == 0) continue;  // warm-up
-      d.push_back(st[0] + st[1]);
+      d.push_back(st[0] + st[1] + st[3]);
       sy.push_back(st[4]);
       tt.push_back(st[kStages]);
     }
@@ -1047,8 +1056,8 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
   int32_t best_d = ctx->dens32_max_nbe, best_s = ctx->syr2k32_max_nbe;
   double bd = 1e300, bs = 1e300, tot = 0.;
   int rc;
-  // The density-side times below are collocation + density: the basis
-  // evaluation mode moves work between the two.  The two stages' tile
+  // The density-side times below are collocation + density + the Z launch:
+  // the basis evaluation mode moves work between them.  The two stages' tile
   // thresholds are disjoint: sweep both at once and keep the best of each.
   for (int32_t c : cand) {
     if ((rc = xcq_set_tile_policy(ctx, c, c))) return rc;
@@ -1064,11 +1073,13 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
     }
   }
   if ((rc = xcq_set_tile_policy(ctx, best_d, best_s))) return rc;
-  if (precision == XCQ_PREC_MIXED) {
+  {
     // Per basis evaluation mode (in-stage first, when the basis allows it):
-    // the 32-wide threshold, which 64-wide tasks run at three workgroups per
-    // CU (stored planes only), and from which nbe on 128-wide tiles are
-    // used; the mode with the best collocation + density time wins.
+    // the 32-wide threshold and, for mixed precision, which 64-wide tasks
+    // run at three workgroups per CU and from which nbe on 128-wide tiles
+    // are used; the mode with the best collocation + density (+ Z launch)
+    // time wins.
+    const bool mixed = precision == XCQ_PREC_MIXED;
     struct Pol {
       int32_t mode, d, h, w;
       double ms;
@@ -1092,7 +1103,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
         }
       }
       if ((rc = xcq_set_tile_policy(ctx, p.d, best_s))) return rc;
-      if (mode != 1) {
+      if (mixed && mode != 1) {
         for (int32_t c : cand) {
           if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
           double dm, sm;
@@ -1105,6 +1116,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
         if ((rc = xcq_set_density_occupancy(ctx, p.h))) return rc;
       }
       for (int32_t c : {128, 192, 256, 384, 512, 1 << 30}) {
+        if (!mixed) break;
         if ((rc = xcq_set_density_wide(ctx, c))) return rc;
         double dm, sm;
         if ((rc = measure(&dm, &sm, &tot))) return rc;

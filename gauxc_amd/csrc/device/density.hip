@@ -559,16 +559,25 @@ __device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa
 // wn = point half of the 64-point block.  Loader roles: thread
 // (c = tid & 63, q = tid >> 6) loads K rows 4q..4q+3 of column c of the B^T
 // stage (point c) and, for c < NT, of the P_sub stage (function n0 + c).
-template <int NT, bool GGA>
+// INSTAGE = 3: the fold evaluates B and grad B itself (fold_in_stage, from
+// the X^T tile written over the stages) and the block's Z coefficients go
+// to zbuf for the separate Z launch; 0: the fold and Z read the planes the
+// collocation stage stored.
+template <int NT, bool GGA, int INSTAGE>
 __global__ __launch_bounds__(256, 2) void k_density_f64(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
-    double* __restrict__ red, unsigned* __restrict__ tmax) {
+    double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa,
+    double* __restrict__ zbuf) {
   constexpr int BI = NT / 32;
-  __shared__ double Pt[2][16][NT + 4];  // [buf][k][function]: A operand (P_sub)
-  __shared__ double Bt[2][16][kPad64];  // [buf][k][point]:    B operand (B^T)
-  __shared__ double dens[2][64][4];
+  constexpr int kPtD = 2 * 16 * (NT + 4), kStageD = kPtD + 2 * 16 * kPad64;
+  static_assert(INSTAGE == 0 || NT * kXLd <= kStageD, "X^T tile must fit the stages");
+  __shared__ double stage[kStageD];
+  // [buf][k][function]: A operand (P_sub); [buf][k][point]: B operand (B^T)
+  double(*Pt)[16][NT + 4] = reinterpret_cast<double(*)[16][NT + 4]>(stage);
+  double(*Bt)[16][kPad64] = reinterpret_cast<double(*)[16][kPad64]>(stage + kPtD);
+  __shared__ double dens[4][64][4];
   __shared__ double zc[4][64];
   __shared__ unsigned smax[kGranLds][2];
   const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
@@ -588,11 +597,23 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
     const double* P = Pall + t.p_off;
     double* Bm = scratch + t.b_off;
     const int32_t* bf = bfmap + t.bf_off;
-    for (int i = tid; i < 2 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
+    for (int i = tid; i < 4 * 64 * 4; i += 256) (&dens[0][0][0])[i] = 0.;
     if (tid < 2 * kGranLds) (&smax[0][0])[tid] = 0u;
     __syncthreads();
     const double* bcol = Bm + m0 + lc;
     const int nks = (nbe + 15) / 16;
+    // In-stage fold: this lane's point (padded points take the task's last
+    // point: finite values, which meet zero X), the shell cursor and this
+    // wave's density partials.
+    double rx = 0., ry = 0., rz = 0.;
+    if constexpr (INSTAGE != 0) {
+      const int pl = min(m0 + lane, t.npts - 1);
+      rx = sa.px[t.pt_off + pl];
+      ry = sa.py[t.pt_off + pl];
+      rz = sa.pz[t.pt_off + pl];
+    }
+    int s_cur = 0;
+    double part[4] = {0., 0., 0., 0.};
 
     for (int n0 = 0; n0 < nbe; n0 += NT) {
       const int ncol = n0 + lc;
@@ -644,22 +665,47 @@ __global__ __launch_bounds__(256, 2) void k_density_f64(
           }
         }
       }
-      double x[BI][2][4];
+      if constexpr (INSTAGE != 0) {
+        double* xt = stage;
+        __syncthreads();  // every wave is done with the last K step's stage
 #pragma unroll
-      for (int i = 0; i < BI; ++i)
+        for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+          for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
-      fold_tile<double, BI, GGA>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
+            for (int r = 0; r < 4; ++r)
+              xt[(wm * 16 * BI + bi * 16 + frow(r)) * kXLd + wn * 32 + bj * 16 + (lane & 15)] =
+                  acc[bi][bj][r];
+        __syncthreads();
+        s_cur = fold_in_stage<GGA>(t, sa, s_cur, n0, min(n0 + NT, nbe), wave, lane, xt, rx, ry,
+                                   rz, part);
+      } else {
+        double x[BI][2][4];
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[i][j][r] = acc[i][j][r];
+        fold_tile<double, BI, GGA>(x, frow, m0, n0, wm, wn, lane, nbe, ld, plane, Bm, dens);
+      }
       // The next tile's first stage write targets buffer 0, last read at step
-      // nks-2 (or nks-1 when nks == 1); all waves pass this barrier after it.
+      // nks-2 (or nks-1 when nks == 1), or the X^T tile; all waves pass this
+      // barrier after it.
       __syncthreads();
     }
-    finish_points(t, wi.a, tid, dens, zc, weights, func, GGA, red);
+    if constexpr (INSTAGE != 0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dens[wave][lane][c] = part[c];
+      __syncthreads();
+    }
+    finish_points<INSTAGE != 0 ? 4 : 2>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
     if (want_z) {
       __syncthreads();
-      write_z<double, GGA>(t, m0, tid, zc, Bm, smax, tmax ? tmax + 2 * t.tmax_off : nullptr);
+      if constexpr (INSTAGE != 0)
+        zbuf[(t.red_off + wi.a) * 256 + tid] = (&zc[0][0])[tid];
+      else
+        write_z<double, GGA>(t, m0, tid, zc, Bm, smax, tmax ? tmax + 2 * t.tmax_off : nullptr);
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -942,19 +988,27 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
   // The bound table is filled only when VXC is computed and contributions
   // may be dropped.
   unsigned* tm = (want_z && v.vxc_drop > 0.0) ? v.tmax : nullptr;
+  const ShellArgs sa{v.shells, v.shc, v.shell_list, v.lsh, v.px, v.py, v.pz};
   if (precision == XCQ_PREC_FP64) {
     double* scr = static_cast<double*>(v.scratch);
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P, scr, v.w,
-                         func, fac, want_z, v.red, tm);
+                         func, fac, want_z, v.red, tm, sa, v.zc);
     };
-    if (nt == 32)
-      gga ? go(k_density_f64<32, true>) : go(k_density_f64<32, false>);
-    else
-      gga ? go(k_density_f64<64, true>) : go(k_density_f64<64, false>);
+    // Any in-stage mode runs the FP64 kernel's one (mode 3).
+    if (v.in_stage != 0) {
+      if (nt == 32)
+        gga ? go(k_density_f64<32, true, 3>) : go(k_density_f64<32, false, 3>);
+      else
+        gga ? go(k_density_f64<64, true, 3>) : go(k_density_f64<64, false, 3>);
+    } else {
+      if (nt == 32)
+        gga ? go(k_density_f64<32, true, 0>) : go(k_density_f64<32, false, 0>);
+      else
+        gga ? go(k_density_f64<64, true, 0>) : go(k_density_f64<64, false, 0>);
+    }
   } else {
     float* scr = static_cast<float*>(v.scratch);
-    const ShellArgs sa{v.shells, v.shc, v.shell_list, v.lsh, v.px, v.py, v.pz};
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.Pf32, scr, v.w,
                          func, fac, want_z, v.red, tm, sa, v.zc);

@@ -134,9 +134,9 @@ struct CtxView {
   double vxc_drop;  // B^T Z contributions with |v| < vxc_drop are not added to VXC
   unsigned* tmax;   // per granule {max |B|, max |Z|} as float bits (the density stage
                     // fills it, the B^T Z stage skips tile pairs bounded below vxc_drop)
-  int in_stage;     // mixed density stage's basis evaluation: 0 = reads the stored
-                    // B / grad B planes; 1 = evaluates them for its fold and Z; 3 = for
-                    // its fold, Z left to a separate launch
+  int in_stage;     // density stage's basis evaluation: 0 = reads the stored B /
+                    // grad B planes; 1 = evaluates them for its fold and Z (mixed; the
+                    // FP64 path runs 3); 3 = for its fold, Z left to a separate launch
   const DevShellC* shc;
   double* zc;       // Z coefficients per point block [4][64] (in-stage mode 3)
 };
@@ -211,13 +211,14 @@ int launch_collocation_fp32(hipStream_t s, const WorkItem* items, int64_t n,
                             const int32_t* shell_list, const int32_t* lsh_first, const double* px,
                             const double* py, const double* pz, float* scratch, int deriv,
                             int lmax);
-// Z = a B + c . grad B with B / grad B evaluated in FP64 (FP32 storage; in-stage
-// mode 3), coefficients from zcoef, granule maxima into tmax (may be null).
+// Z = a B + c . grad B with B / grad B evaluated in FP64 (in-stage mode 3),
+// stored in the precision's scratch layout, coefficients from zcoef, granule
+// maxima into tmax (may be null).
 int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const DevTask* tasks,
                       const DevShell* shells, const int32_t* shell_list,
                       const int32_t* lsh_first, const double* px, const double* py,
-                      const double* pz, const double* zcoef, float* scratch, unsigned* tmax,
-                      int gga, int lmax);
+                      const double* pz, const double* zcoef, void* scratch, int precision,
+                      unsigned* tmax, int gga, int lmax);
 // Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
 // whose tasks use `nt`-function tiles (hiocc: the three-workgroups-per-CU
 // configuration of the mixed kernel).

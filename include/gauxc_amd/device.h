@@ -115,15 +115,15 @@ int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
  * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).
  * Default: off (2^30). */
 int xcq_set_density_wide(xcq_context ctx, int32_t wide_min_nbe);
-/* Mixed precision, where the rho fold and Z get the basis functions'
- * values and gradients (GauXC stores B and grad B,
- * collocation_device_masked_combined_kernel_deriv1, and reads them back in
- * its rho and Z kernels): 3 (default) = the density stage evaluates them
- * at its points for the fold and leaves Z to a separate launch that
- * evaluates them again; 1 = the density stage evaluates them for both;
- * 0 = both read the planes the collocation stage stores.  In modes 1 and 3
- * the collocation stage stores only the value plane.  Bases with angular
- * momentum above 3 always use the stored planes.  Other values: error. */
+/* Where the rho fold and Z get the basis functions' values and gradients
+ * (GauXC stores B and grad B, collocation_device_masked_combined_kernel_deriv1,
+ * and reads them back in its rho and Z kernels): 3 (default) = the density
+ * stage evaluates them at its points for the fold and leaves Z to a
+ * separate launch that evaluates them again; 1 = the mixed density stage
+ * evaluates them for both (the FP64 path runs mode 3); 0 = both read the
+ * planes the collocation stage stores.  In modes 1 and 3 the collocation
+ * stage stores only the value plane.  Bases with angular momentum above 3
+ * always use the stored planes.  Other values: error. */
 int xcq_set_density_basis_eval(xcq_context ctx, int32_t in_stage);
 /* The basis evaluation mode in effect (0 when the basis exceeds l = 3). */
 int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode);
@@ -154,13 +154,12 @@ int xcq_task_times(xcq_context ctx, double* dens_us, double* syr2k_us);
 int xcq_use_graph(xcq_context ctx, int enable);
 /* Autotuner: times the uploaded workload on the device and keeps the
  * fastest settings.  Sweeps the 32/64 tile thresholds of the density and
- * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}); for
- * mixed precision, per basis evaluation mode (in-stage when the basis
- * allows it, then stored planes), the density 32-wide threshold, the
- * occupancy threshold (stored planes) over the same buckets and the
- * 128-wide threshold over {128, 192, 256, 384, 512, off}, keeping the mode
- * with the fastest collocation + density; then the persistent-queue count
- * {0, 2, 4 per CU}.  Leaves the context on the best and reports the 32-wide
+ * B^T Z stages (nbe buckets {0, 32, 48, 64, 96, 128, 192, 256, all}); per
+ * basis evaluation mode (3 when the basis allows it, then 0) the density
+ * 32-wide threshold and, for mixed precision, the occupancy threshold over
+ * the same buckets and the 128-wide threshold over {128, 192, 256, 384,
+ * 512, off}, keeping the mode with the fastest collocation + density + Z
+ * launch; then the persistent-queue count {0, 2, 4 per CU}.  Leaves the context on the best and reports the 32-wide
  * thresholds and queue count (any output pointer may be NULL;
  * xcq_get_tile_policy and xcq_get_density_basis_eval return the rest). */
 int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32_max_nbe,
