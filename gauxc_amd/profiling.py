@@ -14,7 +14,9 @@ the same numbers is written beside it (``OUT.md``); ``python -m
 gauxc_amd.profiling table OUT.json`` rewrites that table from the file.
 
 rocprofv3 runs as a child process with the program itself after ``--``;
-this module never touches the GPU.
+this module never touches the GPU.  Raw rocprofv3 output goes under
+``$XCQ_PROFILE_DIR`` (default: a directory in the system's temporary
+directory).
 """
 import csv
 import json
@@ -162,7 +164,8 @@ def main(argv):
         return 2
     mode, out = argv[0], os.path.abspath(argv[1])
     bench_args = argv[argv.index("--") + 1:]
-    work = os.environ.get("XCQ_PROFILE_DIR", os.path.join(ROOT, "gpurun_out", "profiling"))
+    work = os.environ.get("XCQ_PROFILE_DIR",
+                          os.path.join(tempfile.gettempdir(), "xcq_profiling"))
     os.makedirs(work, exist_ok=True)
     if mode == "stats":
         stats(out, bench_args, work)

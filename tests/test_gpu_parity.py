@@ -1,7 +1,7 @@
 """Parity of the HIP path with the oracle and with GauXC's golden data.
 
 All calls go through the C ABIs (GauXC C API in libgauxc_amd.so, device C
-ABI in libgauxc_amd_device.so).  FP64 bars:
+ABI in libgauxc_amd_device.so).  FP64 bars (the path's parity tolerances):
 |dEXC| <= 1e-10 Eh, max|dVXC_ij| <= 1e-9 against the oracle, and GauXC's
 own ||VXC - VXC_ref||_F / nbf < 1e-10 against its fixtures.
 """
@@ -188,7 +188,7 @@ def test_device_collocation_fp32_storage(gpu, gx, orc):
                           ("c1", 4, "UltraFineGrid", "PBE")])
 def test_device_mixed_matches_fp64(gpu, gx, orc, cfgname, n_water, grid, func):
     """Mixed precision against the FP64 device path and the oracle at the
-    |dEXC| <= 1e-6 Eh, max|dVXC_ij| <= 1e-7."""
+    mixed-precision bars: |dEXC| <= 1e-6 Eh, max|dVXC_ij| <= 1e-7."""
     _flush(f"mixed {cfgname} n_water={n_water} {func}")
     mol, bs, P, cfg = gx.workloads.make_config(cfgname, seed=0, n_water=n_water)
     lb = gx.workloads.build_tasks(mol, bs, grid=grid)
