@@ -164,8 +164,8 @@ def main(argv):
         return 2
     mode, out = argv[0], os.path.abspath(argv[1])
     bench_args = argv[argv.index("--") + 1:]
-    work = os.environ.get("XCQ_PROFILE_DIR",
-                          os.path.join(tempfile.gettempdir(), "xcq_profiling"))
+    work = os.path.abspath(os.environ.get("XCQ_PROFILE_DIR",
+                                          os.path.join(tempfile.gettempdir(), "xcq_profiling")))
     os.makedirs(work, exist_ok=True)
     if mode == "stats":
         stats(out, bench_args, work)
