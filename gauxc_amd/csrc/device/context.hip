@@ -1075,10 +1075,10 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
   if ((rc = xcq_set_tile_policy(ctx, best_d, best_s))) return rc;
   {
     // Per basis evaluation mode (in-stage first, when the basis allows it):
-    // the 32-wide threshold and, for mixed precision, which 64-wide tasks
-    // run at three workgroups per CU and from which nbe on 128-wide tiles
-    // are used; the mode with the best collocation + density (+ Z launch)
-    // time wins.
+    // the 32-wide threshold, which 64-wide tasks run at three workgroups per
+    // CU (mixed precision, and the FP64 in-stage kernel) and, for mixed
+    // precision, from which nbe on 128-wide tiles are used; the mode with
+    // the best collocation + density (+ Z launch) time wins.
     const bool mixed = precision == XCQ_PREC_MIXED;
     struct Pol {
       int32_t mode, d, h, w;
@@ -1103,7 +1103,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
         }
       }
       if ((rc = xcq_set_tile_policy(ctx, p.d, best_s))) return rc;
-      if (mixed && mode != 1) {
+      if ((mixed && mode != 1) || (!mixed && mode == 3)) {
         for (int32_t c : cand) {
           if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
           double dm, sm;

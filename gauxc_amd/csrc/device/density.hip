@@ -562,9 +562,11 @@ __device__ __forceinline__ void z_in_stage(const DevTask& t, const ShellArgs& sa
 // INSTAGE = 3: the fold evaluates B and grad B itself (fold_in_stage, from
 // the X^T tile written over the stages) and the block's Z coefficients go
 // to zbuf for the separate Z launch; 0: the fold and Z read the planes the
-// collocation stage stored.
-template <int NT, bool GGA, int INSTAGE>
-__global__ __launch_bounds__(256, 2) void k_density_f64(
+// collocation stage stored.  HIOCC (in-stage, 64-wide tiles): a register
+// budget for three workgroups per CU, which spills a few registers of the
+// fold and still hides more latency.
+template <int NT, bool GGA, int INSTAGE, bool HIOCC = false>
+__global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
     QueueArgs qa, const DevTask* __restrict__ tasks, const int32_t* __restrict__ bfmap,
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
@@ -999,6 +1001,8 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
     if (v.in_stage != 0) {
       if (nt == 32)
         gga ? go(k_density_f64<32, true, 3>) : go(k_density_f64<32, false, 3>);
+      else if (hiocc)
+        gga ? go(k_density_f64<64, true, 3, true>) : go(k_density_f64<64, false, 3, true>);
       else
         gga ? go(k_density_f64<64, true, 3>) : go(k_density_f64<64, false, 3>);
     } else {

@@ -106,10 +106,10 @@ int xcq_download_results_mol(xcq_context ctx, int32_t mol, double* exc, double* 
  * Tile policy: tasks with nbe <= dens32_max_nbe run the density stage with
  * 32-function tiles (else 64); likewise syr2k32_max_nbe for B^T Z. */
 int xcq_set_tile_policy(xcq_context ctx, int32_t dens32_max_nbe, int32_t syr2k32_max_nbe);
-/* Mixed precision: density tasks with 64-function tiles and
- * nbe <= hiocc_max_nbe run in the kernel configuration that fits three
- * workgroups per CU (two-block operand ring); larger ones keep the
- * three-block ring at two workgroups per CU.  0 (default) disables it. */
+/* Density tasks with 64-function tiles and nbe <= hiocc_max_nbe run in
+ * the kernel configuration that fits three workgroups per CU (mixed: a
+ * two-block operand ring; FP64: the in-stage kernel's register budget);
+ * larger ones keep two workgroups per CU.  0 (default) disables it. */
 int xcq_set_density_occupancy(xcq_context ctx, int32_t hiocc_max_nbe);
 /* Mixed precision: density tasks with nbe >= wide_min_nbe (and above the
  * 32-wide threshold) use 128-function tiles (the FP64 path keeps 64).

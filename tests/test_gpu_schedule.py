@@ -161,16 +161,17 @@ def test_density_basis_eval_in_stage(gx, small, func):
 def test_density_basis_eval_fp64(gx, small, func):
     """FP64 path: the in-stage fold with the separate Z launch (modes 1 and 3
     both run it) against the stored planes: the same FP64 values up to the
-    order of the spherical transform's sums, for 32- and 64-wide tiles and
-    with queues."""
+    order of the spherical transform's sums, for 32- and 64-wide tiles, the
+    three-workgroups-per-CU configuration and queues."""
     bs, P, t = small
     ctx = _ctx(gx, bs, t, P)
-    for dens32, nq in [(48, 0), (0, 0), (100000, 300)]:
+    for dens32, nq, hiocc in [(48, 0, 0), (0, 0, 100000), (100000, 300, 0), (0, 300, 100000)]:
         ctx.set_tile_policy(dens32, 96)
+        ctx.set_density_occupancy(hiocc)
         ctx.set_schedule(nq)
         res = {}
         for mode in (3, 1, 0):
-            _flush(f"fp64 basis eval mode {mode} {func} dens32={dens32} nq={nq}")
+            _flush(f"fp64 basis eval mode {mode} {func} dens32={dens32} nq={nq} hiocc={hiocc}")
             ctx.set_density_basis_eval(mode)
             res[mode] = _run(ctx, "fp64", func)
         for mode in (3, 1):
@@ -178,6 +179,7 @@ def test_density_basis_eval_fp64(gx, small, func):
             assert abs(e - res[0][0]) <= 1e-11 and abs(n - res[0][1]) <= 1e-11
             assert np.abs(v - res[0][2]).max() <= 1e-12
     ctx.set_schedule(0)
+    ctx.set_density_occupancy(0)
     ctx.set_density_basis_eval(3)
 
 
