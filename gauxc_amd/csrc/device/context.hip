@@ -760,7 +760,7 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
     budget = ctx->arena_bytes;
   } else {
     const double avail = ctx->fill * (double)free_b - (double)resident;
-    budget = (size_t)std::max(0.0, std::min(avail * 0.5, 48.0 * (1ull << 30)));
+    budget = (size_t)std::max(0.0, std::min(avail * 0.5, 128.0 * (1ull << 30)));
   }
   // FP64 storage needs 40 bytes per (point, function) of a task, FP32 storage
   // 22 (five float planes + the value plane's bfloat16 residual); the arena
