@@ -29,6 +29,13 @@
 // rows are covered by ceil(nbe / NT) tiles, so small tasks are not padded to
 // 64.  A launch either runs one workgroup per (task, block) item or, in
 // queued mode, one persistent workgroup per queue of items.
+//
+// Basis evaluation mode (template argument INSTAGE): 0 reads B and grad B
+// from the planes the collocation stage stored; in the in-stage modes the
+// fold evaluates them at the block's points from the shell parameters
+// (fold_in_stage) and Z is formed either in the same kernel (mode 1,
+// z_in_stage) or by a separate launch from the Z coefficients this kernel
+// leaves (mode 3, k_z_in_stage in collocation.hip).
 #include "internal.hpp"
 #include "functional.hpp"
 #include "shell_eval.hpp"
