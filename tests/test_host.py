@@ -189,3 +189,28 @@ def test_no_cpu_fallback_without_device(gx):
     d, mol, bs, lb = benzene_setup("svwn5", grid="GM3")
     with pytest.raises(gx.GauXCError, match="SSF weights failed"):
         gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+
+
+def test_bench_roofline_model_follows_basis_eval_mode():
+    """bench.py's algorithmic bytes follow the basis evaluation mode (the
+    in-stage modes move no gradient planes), and its kernel names match the
+    committed rocprofv3 traffic summary, so roofline.traffic is found."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    ts = {"npts": np.array([64, 130, 500]), "nbe": np.array([10, 40, 300])}
+    for prec in ("fp64", "mixed"):
+        b0, f0 = bench.algorithmic("density", prec, ts, 0)
+        b3, f3 = bench.algorithmic("density", prec, ts, 3)
+        c0, _ = bench.algorithmic("collocation", prec, ts, 0)
+        c3, _ = bench.algorithmic("collocation", prec, ts, 3)
+        z3, _ = bench.algorithmic("zmat", prec, ts, 3)
+        assert f0 == f3 > 0
+        assert b3 < b0 and c3 < c0 and z3 > 0
+        assert bench.stage_kernels("zmat", prec, 0) == []
+        assert len(bench.stage_kernels("zmat", prec, 3)) == 1
+    assert bench.measured_traffic("density", "mixed", 3) is not None
+    assert bench.measured_traffic("syr2k_scatter", "mixed", 3) is not None
