@@ -809,10 +809,14 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   // stages [slot][k][point] (B operand); after a tile's last K block the
   // X^T tile of the LDS fold (NT <= 64) is written over it.
   constexpr int kPtFloats = kStages * 32 * (NT + 4);
-  constexpr int kRingFloats = kPtFloats + kStages * 32 * 68;
+  constexpr int kStageFloats = kPtFloats + kStages * 32 * 68;
+  // In-stage folds take the whole X^T tile at once (128-wide tiles: the
+  // buffer grows to 66 KiB, still two workgroups per CU), so no wave holds
+  // its accumulators while another row block is folded.
+  constexpr bool kWholeXt = INSTAGE == 1 || INSTAGE == 3;
+  constexpr int kRingFloats =
+      kWholeXt && NT * kXLd * 2 > kStageFloats ? NT * kXLd * 2 : kStageFloats;
   static_assert(NT > 64 || NT * kXLd * 2 <= kRingFloats, "X^T tile must fit the ring");
-  static_assert((INSTAGE != 1 && INSTAGE != 3) || (NT < 64 ? NT : 64) * kXLd * 2 <= kRingFloats,
-                "X^T rows of one in-stage fold must fit the ring");
   __shared__ __attribute__((aligned(16))) float ring[kRingFloats];
   float(*Pt)[32][NT + 4] = reinterpret_cast<float(*)[32][NT + 4]>(ring);
   float(*Bt)[32][68] = reinterpret_cast<float(*)[32][68]>(ring + kPtFloats);
@@ -935,26 +939,25 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[a][b][r] *= fac;
       if constexpr (INSTAGE == 1 || INSTAGE == 3) {
-        // The X^T tile goes through LDS in halves of 64 rows (the row block
-        // of waves with wm = h for 128-wide tiles).
+        // The whole X^T tile goes to LDS (row = tile row), then is folded
+        // in slices of up to 64 rows.
         double* xt = reinterpret_cast<double*>(ring);
+        __syncthreads();  // every wave is done with the ring's last K block
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              xt[(wm * 16 * BI + bi * 16 + 4 * (lane >> 4) + r) * kXLd + wn * 32 + bj * 16 +
+                 (lane & 15)] = acc[bi][bj][r];
+        __syncthreads();
 #pragma unroll
         for (int h = 0; h < (NT + 63) / 64; ++h) {
           const int nlo = n0 + 64 * h;
           if (nlo >= nbe) break;
-          __syncthreads();  // the ring's last K block, or the previous half, is consumed
-          if (NT <= 64 || wm == h)
-#pragma unroll
-            for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-              for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                  xt[((NT <= 64 ? wm * 16 * BI : 0) + bi * 16 + 4 * (lane >> 4) + r) * kXLd +
-                     wn * 32 + bj * 16 + (lane & 15)] = acc[bi][bj][r];
-          __syncthreads();
           s_cur = fold_in_stage<GGA>(t, sa, s_cur, nlo, min(nlo + min(NT, 64), nbe), wave,
-                                     lane, xt, rx, ry, rz, part);
+                                     lane, xt + 64 * h * kXLd, rx, ry, rz, part);
         }
       } else if constexpr (NT <= 64) {
         __syncthreads();  // every wave is done with the ring's last K block
