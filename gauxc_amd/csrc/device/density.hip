@@ -751,7 +751,8 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
 // 41 KiB, at most 168 registers), which hides the fold's and Z's memory
 // latency better on small tasks.
 // 128-function tiles (large tasks) keep a two-block ring at two workgroups
-// per CU (57 KiB of LDS).
+// per CU (60 KiB of LDS; 76 KiB in the in-stage modes, whose buffer holds
+// the whole X^T tile).
 template <int NT, bool HIOCC>
 constexpr int ring_stages() { return (HIOCC || NT == 128) ? 2 : 3; }
 constexpr int density_waves(int nt, bool hiocc) { return (hiocc && nt != 128) ? 3 : 2; }
