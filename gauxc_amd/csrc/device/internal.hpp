@@ -231,9 +231,10 @@ int launch_fill(hipStream_t s, double* p, int64_t n);
 // Symmetrise the VXC of nmol molecules: mol_n[m] = nbf, mol_off[m] = offset.
 int launch_symmetrize(hipStream_t s, int32_t nmol, int32_t max_n, const int32_t* mol_n,
                       const int64_t* mol_off, double* vxc);
-// Partials per molecule of the FP32-rounding correction of EXC and of the
-// EXC/NEL block sums.
-constexpr int kCorrBlocks = 128;
+// Partials per molecule of the FP32-rounding correction of EXC (four
+// workgroups per CU, enough to stream a large molecule's P, P32 and VXC at
+// HBM rate) and of the EXC/NEL block sums.
+constexpr int kCorrBlocks = 1024;
 constexpr int kRedBlocks = 64;
 // part[m * kCorrBlocks + b] = partial sums of (P - P32) . VXC over molecule m.
 int launch_p32_correction(hipStream_t s, int32_t nmol, const int32_t* mol_n,
