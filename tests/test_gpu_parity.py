@@ -53,6 +53,20 @@ def test_device_ssf_matches_fixture_and_oracle(gpu, gx, orc):
     assert np.abs(w_d - w_o).max() <= 1e-12 * np.abs(w_o).max()
 
 
+def test_device_ssf_water_cluster_matches_oracle(gpu, gx, orc):
+    """SSF weights on an 8-water cluster (24 atoms, FineGrid): most atom
+    pairs are far apart relative to the point, the case where the kernel
+    skips factors that are exactly 1."""
+    _flush("ssf weights 8 waters FineGrid vs oracle")
+    mol, bs, P, cfg = gx.workloads.make_config("c1", seed=1, n_water=8, grid="FineGrid")
+    lb = gx.workloads.build_tasks(mol, bs, grid="FineGrid")
+    t = lb.get_tasks()
+    w_o = orc.ssf_weights(cfg["xyz"], t)
+    w_d = gx.ssf_weights_device(cfg["xyz"], t["points"], np.repeat(t["iparent"], t["npts"]),
+                                t["weights"])
+    assert np.abs(w_d - w_o).max() <= 1e-12 * np.abs(w_o).max()
+
+
 def _context_for(gx, bs, t, P, arena=0):
     ctx = gx.DeviceContext(0, arena_bytes=arena)
     ctx.upload_basis(bs.xcq_shells())
