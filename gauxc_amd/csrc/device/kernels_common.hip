@@ -50,12 +50,23 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
   const double dpar = dist(par);
   if (dpar < 0.5 * (1.0 - a) * dnear[par]) return;
   double dmin = dpar;
-  for (int A = 0; A < natoms; ++A) dmin = fmin(dmin, dist(A));
+  int inear = par;
+  for (int A = 0; A < natoms; ++A) {
+    const double d = dist(A);
+    if (d < dmin) {
+      dmin = d;
+      inear = A;
+    }
+  }
   const double bound = (1.0 + a) / (1.0 - a) * dmin * (1.0 + 1e-10) + 1e-12;
   double sum = 0.0, ppar = 0.0;
   for (int A = 0; A < natoms; ++A) {
     const double dA = dist(A);
     if (dA > bound) continue;
+    // The nearest atom's factor first: when it is exactly 0 the product
+    // below is exactly 0 as well (every factor lies in [0, 1]).
+    if (A != inear && ssf_s((dA - dmin) / rab[(size_t)A + (size_t)inear * natoms]) == 0.0)
+      continue;
     double PA = 1.0;
     for (int B = 0; B < natoms && PA > 0.0; ++B) {
       if (B == A) continue;
