@@ -71,9 +71,14 @@ __device__ __forceinline__ void shell_eval(double x, double y, double z, double 
 constexpr int kCollShells = 256;  // shells staged per pass
 constexpr int kCollPrims = 4;     // primitives per shell staged in LDS
 
-// Occupancy target by the highest angular momentum a launch evaluates:
-// spherical f (LMAX 3) with gradients fits 168 registers, g (LMAX 4) 256.
-constexpr int coll_waves(int lmax) { return lmax <= 2 ? 4 : (lmax == 3 ? 3 : 2); }
+// Occupancy target (waves per SIMD) by the highest angular momentum a
+// launch evaluates: spherical f (LMAX 3) with gradients fits 168 registers,
+// g (LMAX 4) 256.  Values alone up to f: six waves for FP64 planes (no
+// spills), five for FP32 planes (six spill and measured slower: 24 waters,
+// 1.23 ms at five, 1.29 ms at six, 1.32 ms at four).
+constexpr int coll_waves(int lmax, bool grad, bool f64) {
+  return lmax <= 2 ? 4 : lmax == 3 ? (grad ? 3 : f64 ? 6 : 5) : 2;
+}
 
 // Shell parameters of a task in LDS, one pass of up to kCollShells shells.
 struct ShellStage {
@@ -145,7 +150,7 @@ __device__ __forceinline__ void for_block_shells(const DevTask& t,
 }
 
 template <bool GRAD, typename T, int LMAX>
-__global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const WorkItem* __restrict__ blocks,
+__global__ __launch_bounds__(256, coll_waves(LMAX, GRAD, sizeof(T) == 8)) void k_collocation(const WorkItem* __restrict__ blocks,
                                                      const DevTask* __restrict__ tasks,
                                                      const DevShell* __restrict__ shells,
                                                      const int32_t* __restrict__ shell_list,
@@ -207,9 +212,11 @@ __global__ __launch_bounds__(256, coll_waves(LMAX)) void k_collocation(const Wor
 // shell and per current granule; a wave reduces them (clamped to infinity)
 // into LDS when it moves to another granule (its shells ascend), and a
 // shell that spans two granules counts in both, which can only loosen the
-// bounds.
+// bounds.  Four waves per SIMD: up to f with gradients the kernel then
+// spills 18 registers and still runs faster than at three (24 waters,
+// 1.47 ms against 1.60 ms); at five it spills over 50.
 template <bool GGA, int LMAX, typename T>
-__global__ __launch_bounds__(256, LMAX <= 2 ? 4 : 3) void k_z_in_stage(const WorkItem* __restrict__ blocks,
+__global__ __launch_bounds__(256, 4) void k_z_in_stage(const WorkItem* __restrict__ blocks,
                                                       const DevTask* __restrict__ tasks,
                                                       const DevShell* __restrict__ shells,
                                                       const int32_t* __restrict__ shell_list,
