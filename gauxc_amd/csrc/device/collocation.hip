@@ -212,9 +212,8 @@ __global__ __launch_bounds__(256, coll_waves(LMAX, GRAD, sizeof(T) == 8)) void k
 // shell and per current granule; a wave reduces them (clamped to infinity)
 // into LDS when it moves to another granule (its shells ascend), and a
 // shell that spans two granules counts in both, which can only loosen the
-// bounds.  Four waves per SIMD: up to f with gradients the kernel then
-// spills 18 registers and still runs faster than at three (24 waters,
-// 1.47 ms against 1.60 ms); at five it spills over 50.
+// bounds.  Four waves per SIMD: with the gradient folded along the
+// coefficient vector (shell_functions_z) up to f fits 114 registers.
 template <bool GGA, int LMAX, typename T>
 __global__ __launch_bounds__(256, 4) void k_z_in_stage(const WorkItem* __restrict__ blocks,
                                                       const DevTask* __restrict__ tasks,
@@ -268,14 +267,21 @@ __global__ __launch_bounds__(256, 4) void k_z_in_stage(const WorkItem* __restric
                    [&](int l, bool pure, int bf0, double x, double y, double z, double R,
                        double dR) {
     unsigned sb = 0u, sz = 0u;  // this shell's maxima
-    shell_functions_l<LMAX, GGA, true>(l, x, y, z, R, dR, pure,
-                                       [&](int f, double v, double vx, double vy, double vz) {
-      double zv = za * v;
-      if constexpr (GGA) zv += zx * vx + zy * vy + zz * vz;
+    auto put = [&](int f, double v, double zv) {
       Zp[(int64_t)(bf0 + f) * ld] = (T)zv;
       sb = max(sb, __float_as_uint((float)v) & kAbs);
       sz = max(sz, __float_as_uint((float)zv) & kAbs);
-    });
+    };
+    if constexpr (GGA) {
+      // Z_f = a B_f + c . grad B_f, the gradient folded along c.
+      const double A = za * R + dR * (zx * x + zy * y + zz * z);
+      shell_functions_z_l<LMAX>(l, x, y, z, R, A, zx, zy, zz, pure, put);
+    } else {
+      shell_functions_l<LMAX, false, true>(l, x, y, z, R, dR, pure,
+                                           [&](int f, double v, double, double, double) {
+        put(f, v, za * v);
+      });
+    }
     if (tmax) {
       const int size = pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
       const int g0 = bf0 / kGran, g1 = (bf0 + size - 1) / kGran;  // uniform

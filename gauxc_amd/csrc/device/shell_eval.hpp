@@ -192,6 +192,84 @@ __device__ __forceinline__ void shell_functions(double x, double y, double z, do
   }
 }
 
+// For the Z matrix: calls sink(f, v, zf) with v = B_f and
+// zf = a B_f + u . grad B_f for each function f of the shell.  With
+// B = R Y and grad R = dR r this is zf = A Y_f + R (u . grad Y_f),
+// A = a R + dR (u . r), so each Cartesian term needs its monomial and the
+// monomial's derivative along u, and a spherical output two sums (one
+// output at a time, as the LEAN form above) instead of four.  The caller
+// passes A.
+template <int L, typename Sink>
+__device__ __forceinline__ void shell_functions_z(double x, double y, double z, double R,
+                                                  double A, double ux, double uy, double uz,
+                                                  bool pure, Sink&& sink) {
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.;
+#pragma unroll
+  for (int k = 1; k <= L; ++k) {
+    xp[k] = xp[k - 1] * x;
+    yp[k] = yp[k - 1] * y;
+    zp[k] = zp[k - 1] * z;
+  }
+  // Cartesian term (a, b, c = L - a - b): monomial and its derivative along u.
+  auto term = [&](int a, int b, double& mono, double& du) {
+    const int c = L - a - b;
+    mono = xp[a] * yp[b] * zp[c];
+    du = 0.;
+    if (a) du += double(a) * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] * ux;
+    if (b) du += double(b) * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] * uy;
+    if (c) du += double(c) * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] * uz;
+  };
+  if (pure) {
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m) {
+      double y0 = 0., yu = 0.;  // sum cf * monomial, sum cf * derivative
+      bool first = true;
+      int ic = 0;
+#pragma unroll
+      for (int a = L; a >= 0; --a) {
+#pragma unroll
+        for (int b = L - a; b >= 0; --b) {
+          const double cf = kSph.c[L][m][ic];
+          if (cf != 0.) {
+            double mono, du;
+            term(a, b, mono, du);
+            y0 = first ? cf * mono : y0 + cf * mono;
+            yu = first ? cf * du : yu + cf * du;
+            first = false;
+          }
+          ++ic;
+        }
+      }
+      sink(m, R * y0, A * y0 + R * yu);
+    }
+  } else {
+    int ic = 0;
+#pragma unroll
+    for (int a = L; a >= 0; --a) {
+#pragma unroll
+      for (int b = L - a; b >= 0; --b) {
+        double mono, du;
+        term(a, b, mono, du);
+        sink(ic, R * mono, A * mono + R * du);
+        ++ic;
+      }
+    }
+  }
+}
+
+template <int LMAX, typename Sink>
+__device__ __forceinline__ void shell_functions_z_l(int l, double x, double y, double z,
+                                                    double R, double A, double ux, double uy,
+                                                    double uz, bool pure, Sink&& sink) {
+  if (l == 0) shell_functions_z<0>(x, y, z, R, A, ux, uy, uz, pure, sink);
+  else if (l == 1) shell_functions_z<1>(x, y, z, R, A, ux, uy, uz, pure, sink);
+  else if (LMAX <= 2 || l == 2) shell_functions_z<2>(x, y, z, R, A, ux, uy, uz, pure, sink);
+  else if (LMAX == 3 || l == 3)
+    shell_functions_z<(LMAX < 3 ? 2 : 3)>(x, y, z, R, A, ux, uy, uz, pure, sink);
+  else shell_functions_z<(LMAX < 4 ? 2 : 4)>(x, y, z, R, A, ux, uy, uz, pure, sink);
+}
+
 // The same for a run-time l <= max(LMAX, 2) (l above that is not evaluated).
 template <int LMAX, bool GRAD, bool LEAN = false, typename Sink>
 __device__ __forceinline__ void shell_functions_l(int l, double x, double y, double z, double R,
