@@ -395,15 +395,20 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
 // Calls sink(f, v, gx, gy, gz) for the functions of shell u at the lane's
 // point (rx, ry, rz): R = sum c e^{-a r2}, dR = -2 sum a c e^{-a r2}; the
 // primitives past the first come from the full shell record.
-template <bool GRAD, typename Sink>
-__device__ __forceinline__ void eval_shell(const ShellArgs& sa, const ShellU& u, double rx,
-                                           double ry, double rz, Sink&& sink) {
-  const int l = u.info & 0xff, nprim = u.info >> 16;
-  const bool pure = (u.info >> 8) & 1;
-  const double x = rx - u.ox, y = ry - u.oy, z = rz - u.oz;
+// Displacement (x, y, z) of the lane's point from shell u's centre and the
+// radial part R = sum c e^{-a r2}, dR = -2 sum a c e^{-a r2}; the
+// primitives past the first come from the full shell record.
+__device__ __forceinline__ void shell_radial(const ShellArgs& sa, const ShellU& u, double rx,
+                                             double ry, double rz, double& x, double& y,
+                                             double& z, double& R, double& dR) {
+  const int nprim = u.info >> 16;
+  x = rx - u.ox;
+  y = ry - u.oy;
+  z = rz - u.oz;
   const double r2 = x * x + y * y + z * z;
   const double e0 = u.c0 * exp(-u.a0 * r2);
-  double R = e0, dR = -2. * u.a0 * e0;
+  R = e0;
+  dR = -2. * u.a0 * e0;
   if (nprim > 1) {
     const DevShell* sh = sa.shells + u.ish;
     for (int k = 1; k < nprim; ++k) {
@@ -412,6 +417,15 @@ __device__ __forceinline__ void eval_shell(const ShellArgs& sa, const ShellU& u,
       dR += -2. * a * e;
     }
   }
+}
+
+template <bool GRAD, typename Sink>
+__device__ __forceinline__ void eval_shell(const ShellArgs& sa, const ShellU& u, double rx,
+                                           double ry, double rz, Sink&& sink) {
+  const int l = u.info & 0xff;
+  const bool pure = (u.info >> 8) & 1;
+  double x, y, z, R, dR;
+  shell_radial(sa, u, rx, ry, rz, x, y, z, R, dR);
   shell_functions_l<kInStageMaxL, GRAD, true>(l, x, y, z, R, dR, pure, sink);
 }
 
@@ -459,24 +473,41 @@ __device__ __forceinline__ int for_range_shells(const DevTask& t, const ShellArg
 
 // Folds rows [nlo, nhi) of the X^T tile in LDS (row n - nlo, column =
 // point) into this lane's partials r: rho += B X, grad rho / 2 += grad B X.
+// With B_f = R Y_f and grad B_f = R grad Y_f + dR r Y_f, a shell adds
+// R sy to rho and R sg + dR r sy to grad rho / 2, where sy = sum Y_f X_f
+// and sg = sum grad Y_f X_f over its functions in the range: the functions
+// are formed from their angular factors alone.
 template <bool GGA>
 __device__ __forceinline__ int fold_in_stage(const DevTask& t, const ShellArgs& sa, int s_first,
                                              int nlo, int nhi, int wave, int lane,
                                              const double* xt, double rx, double ry, double rz,
                                              double (&r)[4]) {
   return for_range_shells(t, sa, s_first, nhi, wave, lane, [&](const ShellU& u) {
-    eval_shell<GGA>(sa, u, rx, ry, rz, [&](int f, double v, double gx, double gy, double gz) {
-      const int n = u.bf0 + f;  // uniform
-      if (n >= nlo && n < nhi) {
-        const double xv = xt[(n - nlo) * kXLd + lane];
-        r[0] += v * xv;
-        if constexpr (GGA) {
-          r[1] += gx * xv;
-          r[2] += gy * xv;
-          r[3] += gz * xv;
-        }
-      }
-    });
+    const int l = u.info & 0xff;
+    const bool pure = (u.info >> 8) & 1;
+    double x, y, z, R, dR;
+    shell_radial(sa, u, rx, ry, rz, x, y, z, R, dR);
+    double sy = 0., sgx = 0., sgy = 0., sgz = 0.;
+    shell_functions_l<kInStageMaxL, GGA, true, true>(
+        l, x, y, z, R, dR, pure, [&](int f, double v, double gx, double gy, double gz) {
+          const int n = u.bf0 + f;  // uniform
+          if (n >= nlo && n < nhi) {
+            const double xv = xt[(n - nlo) * kXLd + lane];
+            sy += v * xv;
+            if constexpr (GGA) {
+              sgx += gx * xv;
+              sgy += gy * xv;
+              sgz += gz * xv;
+            }
+          }
+        });
+    r[0] += R * sy;
+    if constexpr (GGA) {
+      const double d = dR * sy;
+      r[1] += R * sgx + d * x;
+      r[2] += R * sgy + d * y;
+      r[3] += R * sgz + d * z;
+    }
   });
 }
 

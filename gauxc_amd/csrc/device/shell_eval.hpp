@@ -91,7 +91,10 @@ __device__ constexpr SphTable kSph = make_sph_table();
 // accumulate their 2L+1 outputs while the Cartesian monomials are
 // generated, so no Cartesian array is kept; with LEAN they are formed one
 // output at a time instead (fewer live registers, a few terms recomputed).
-template <int L, bool GRAD, bool LEAN = false, typename Sink>
+// With ANG only the angular factors are formed: v = Y_f and the gradient
+// grad Y_f (R and dR are not read), for callers that apply the radial part
+// once per shell.
+template <int L, bool GRAD, bool LEAN = false, bool ANG = false, typename Sink>
 __device__ __forceinline__ void shell_functions(double x, double y, double z, double R,
                                                 double dR, bool pure, Sink&& sink) {
   double xp[L + 1], yp[L + 1], zp[L + 1];
@@ -107,14 +110,20 @@ __device__ __forceinline__ void shell_functions(double x, double y, double z, do
   auto term = [&](int a, int b, double& cv, double& gx, double& gy, double& gz) {
     const int c = L - a - b;
     const double mono = xp[a] * yp[b] * zp[c];
-    cv = mono * R;
+    cv = ANG ? mono : mono * R;
     if constexpr (GRAD) {
       const double dxm = a ? double(a) * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] : 0.;
       const double dym = b ? double(b) * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] : 0.;
       const double dzm = c ? double(c) * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] : 0.;
-      gx = dxm * R + mono * Rx;
-      gy = dym * R + mono * Ry;
-      gz = dzm * R + mono * Rz;
+      if constexpr (ANG) {
+        gx = dxm;
+        gy = dym;
+        gz = dzm;
+      } else {
+        gx = dxm * R + mono * Rx;
+        gy = dym * R + mono * Ry;
+        gz = dzm * R + mono * Rz;
+      }
     }
   };
   if (pure) {
@@ -271,15 +280,15 @@ __device__ __forceinline__ void shell_functions_z_l(int l, double x, double y, d
 }
 
 // The same for a run-time l <= max(LMAX, 2) (l above that is not evaluated).
-template <int LMAX, bool GRAD, bool LEAN = false, typename Sink>
+template <int LMAX, bool GRAD, bool LEAN = false, bool ANG = false, typename Sink>
 __device__ __forceinline__ void shell_functions_l(int l, double x, double y, double z, double R,
                                                   double dR, bool pure, Sink&& sink) {
-  if (l == 0) shell_functions<0, GRAD, LEAN>(x, y, z, R, dR, pure, sink);
-  else if (l == 1) shell_functions<1, GRAD, LEAN>(x, y, z, R, dR, pure, sink);
-  else if (LMAX <= 2 || l == 2) shell_functions<2, GRAD, LEAN>(x, y, z, R, dR, pure, sink);
+  if (l == 0) shell_functions<0, GRAD, LEAN, ANG>(x, y, z, R, dR, pure, sink);
+  else if (l == 1) shell_functions<1, GRAD, LEAN, ANG>(x, y, z, R, dR, pure, sink);
+  else if (LMAX <= 2 || l == 2) shell_functions<2, GRAD, LEAN, ANG>(x, y, z, R, dR, pure, sink);
   else if (LMAX == 3 || l == 3)
-    shell_functions<(LMAX < 3 ? 2 : 3), GRAD, LEAN>(x, y, z, R, dR, pure, sink);
-  else shell_functions<(LMAX < 4 ? 2 : 4), GRAD, LEAN>(x, y, z, R, dR, pure, sink);
+    shell_functions<(LMAX < 3 ? 2 : 3), GRAD, LEAN, ANG>(x, y, z, R, dR, pure, sink);
+  else shell_functions<(LMAX < 4 ? 2 : 4), GRAD, LEAN, ANG>(x, y, z, R, dR, pure, sink);
 }
 
 }  // namespace xcq
