@@ -212,10 +212,12 @@ __global__ __launch_bounds__(256, coll_waves(LMAX, GRAD, sizeof(T) == 8)) void k
 // shell and per current granule; a wave reduces them (clamped to infinity)
 // into LDS when it moves to another granule (its shells ascend), and a
 // shell that spans two granules counts in both, which can only loosen the
-// bounds.  Four waves per SIMD: with the gradient folded along the
-// coefficient vector (shell_functions_z) up to f fits 114 registers.
+// bounds.  Five waves per SIMD: with the gradient folded along the
+// coefficient vector (shell_functions_z) up to f needs 114 registers; at
+// five waves 8 spill and the launch is still faster (24 waters, FP32 Z
+// 1.30 -> 1.20 ms, FP64 1.63 -> 1.49 ms).
 template <bool GGA, int LMAX, typename T>
-__global__ __launch_bounds__(256, 4) void k_z_in_stage(const WorkItem* __restrict__ blocks,
+__global__ __launch_bounds__(256, 5) void k_z_in_stage(const WorkItem* __restrict__ blocks,
                                                       const DevTask* __restrict__ tasks,
                                                       const DevShell* __restrict__ shells,
                                                       const int32_t* __restrict__ shell_list,
