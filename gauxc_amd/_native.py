@@ -10,6 +10,8 @@ Python or CPU fallback for any product entry point.
 import ctypes
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 HOST_LIB = os.path.join(LIB_DIR, "libgauxc_amd.so")
@@ -27,6 +29,11 @@ class GauXCStatus(ctypes.Structure):
 
 class GauXCHandle(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("ptr", ctypes.c_void_p)]
+
+
+class GauXCRuntimeHandle(ctypes.Structure):
+    """GauXCRuntimeEnvironment: {header, host environment, device environment}."""
+    _fields_ = [("type", ctypes.c_int), ("ptr", ctypes.c_void_p), ("device_ptr", ctypes.c_void_p)]
 
 
 class GauXCAtom(ctypes.Structure):
@@ -54,6 +61,25 @@ class XcqTask(ctypes.Structure):
                 ("point_offset", c_i64), ("shell_offset", c_i64), ("dist_nearest", c_dbl)]
 
 
+# The same record as a numpy structured dtype (identical 40-byte layout), so
+# task tables of a million entries are packed without a Python loop.
+XCQ_TASK_DTYPE = np.dtype([("iparent", "<i4"), ("npts", "<i4"), ("nshells", "<i4"),
+                           ("nbe", "<i4"), ("point_offset", "<i8"), ("shell_offset", "<i8"),
+                           ("dist_nearest", "<f8")], align=True)
+assert XCQ_TASK_DTYPE.itemsize == ctypes.sizeof(XcqTask)
+
+
+def pack_tasks(t, point_base=0, shell_base=0):
+    """xcq_task records of a task dict (LoadBalancer.get_tasks layout)."""
+    nt = len(t["npts"])
+    out = np.zeros(nt, dtype=XCQ_TASK_DTYPE)
+    for k in ("iparent", "npts", "nshells", "nbe", "dist_nearest"):
+        out[k] = t[k]
+    out["point_offset"] = np.asarray(t["point_offset"], dtype=np.int64) + point_base
+    out["shell_offset"] = np.asarray(t["shell_offset"], dtype=np.int64) + shell_base
+    return out
+
+
 # Enumerations (include/gauxc/c/gauxc.h, GauXC order)
 RADIAL_MURA_KNOWLES = 1
 GRID = {"FineGrid": 0, "UltraFineGrid": 1, "SuperFineGrid": 2, "GM3": 3, "GM5": 4}
@@ -70,6 +96,8 @@ HOST_SYMBOLS = [
     "gauxc_basisset_new_from_shells", "gauxc_basisset_delete", "gauxc_basisset_nbf",
     "gauxc_basisset_get_shells", "gauxc_molgrid_new_default", "gauxc_molgrid_delete",
     "gauxc_runtime_environment_new", "gauxc_device_runtime_environment_new",
+    "gauxc_device_runtime_environment_new_mem", "gauxc_runtime_environment_comm_rank",
+    "gauxc_runtime_environment_comm_size", "gauxc_object_delete", "gauxc_objects_delete",
     "gauxc_runtime_environment_delete", "gauxc_load_balancer_factory_new",
     "gauxc_load_balancer_factory_delete", "gauxc_load_balancer_factory_get_instance",
     "gauxc_load_balancer_delete", "gauxc_load_balancer_task_counts",
@@ -82,9 +110,11 @@ HOST_SYMBOLS = [
     "gauxc_integrator_device_context",
 ]
 DEVICE_SYMBOLS = [
-    "xcq_device_count", "xcq_last_error", "xcq_context_create", "xcq_context_destroy",
+    "xcq_device_count", "xcq_last_error", "xcq_context_create", "xcq_context_create_mem",
+    "xcq_context_destroy",
     "xcq_synchronize", "xcq_upload_basis", "xcq_upload_tasks", "xcq_upload_density",
-    "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_download_weights", "xcq_exc_vxc",
+    "xcq_ssf_weights", "xcq_ssf_weights_tasks",
+    "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
     "xcq_upload_molecules", "xcq_upload_tasks_batched", "xcq_upload_density_mol",
@@ -119,6 +149,8 @@ def device_lib():
             "xcq_last_error": (ctypes.c_char_p, []),
             "xcq_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, c_dbl,
                                                   ctypes.POINTER(H)]),
+            "xcq_context_create_mem": (ctypes.c_int, [ctypes.c_int, c_vp, ctypes.c_size_t,
+                                                      ctypes.POINTER(H)]),
             "xcq_context_destroy": (ctypes.c_int, [H]),
             "xcq_synchronize": (ctypes.c_int, [H]),
             "xcq_upload_basis": (ctypes.c_int, [H, c_i32, c_vp]),
@@ -182,6 +214,7 @@ def host_lib():
         S = ctypes.POINTER(GauXCStatus)
         Hd = GauXCHandle
         PH = ctypes.POINTER(GauXCHandle)
+        Rt = GauXCRuntimeHandle
         sig = {
             "gauxc_status_delete": (None, [S]),
             "gauxc_molecule_new_from_atoms": (Hd, [S, c_vp, ctypes.c_size_t]),
@@ -194,12 +227,17 @@ def host_lib():
             "gauxc_molgrid_new_default": (Hd, [S, Hd, ctypes.c_int, c_i64, ctypes.c_int,
                                                ctypes.c_int]),
             "gauxc_molgrid_delete": (None, [S, PH]),
-            "gauxc_runtime_environment_new": (Hd, [S]),
-            "gauxc_device_runtime_environment_new": (Hd, [S, c_dbl]),
-            "gauxc_runtime_environment_delete": (None, [S, PH]),
+            "gauxc_runtime_environment_new": (Rt, [S]),
+            "gauxc_device_runtime_environment_new": (Rt, [S, c_dbl]),
+            "gauxc_device_runtime_environment_new_mem": (Rt, [S, c_vp, ctypes.c_size_t]),
+            "gauxc_runtime_environment_delete": (None, [S, ctypes.POINTER(Rt)]),
+            "gauxc_runtime_environment_comm_rank": (ctypes.c_int, [S, Rt]),
+            "gauxc_runtime_environment_comm_size": (ctypes.c_int, [S, Rt]),
+            "gauxc_object_delete": (None, [S, ctypes.POINTER(c_vp)]),
+            "gauxc_objects_delete": (None, [S, ctypes.POINTER(c_vp), ctypes.c_size_t]),
             "gauxc_load_balancer_factory_new": (Hd, [S, ctypes.c_int, ctypes.c_char_p]),
             "gauxc_load_balancer_factory_delete": (None, [S, PH]),
-            "gauxc_load_balancer_factory_get_instance": (Hd, [S, Hd, Hd, Hd, Hd, Hd]),
+            "gauxc_load_balancer_factory_get_instance": (Hd, [S, Hd, Rt, Hd, Hd, Hd]),
             "gauxc_load_balancer_delete": (None, [S, PH]),
             "gauxc_load_balancer_task_counts": (None, [S, Hd, c_vp]),
             "gauxc_load_balancer_export_tasks": (None, [S, Hd, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -143,16 +143,45 @@ class MolGridFactory:
 
 
 class RuntimeEnvironment(_Handle):
+    """Host runtime environment (GauXC RuntimeEnvironment): device work runs
+    on device 0 with the default fill fraction 0.9."""
     _delete = "gauxc_runtime_environment_delete"
 
-    def __init__(self, fill_fraction=0.9):
+    def __init__(self):
         st = N.Status()
-        h = N.host_lib().gauxc_device_runtime_environment_new(st.ptr, float(fill_fraction))
+        h = N.host_lib().gauxc_runtime_environment_new(st.ptr)
         st.raise_if()
         super().__init__(h)
 
+    def __del__(self):
+        try:
+            if self.h is not None and (self.h.ptr or self.h.device_ptr):
+                st = N.Status()
+                N.host_lib().gauxc_runtime_environment_delete(st.ptr, ctypes.byref(self.h))
+        except Exception:
+            pass
 
-DeviceRuntimeEnvironment = RuntimeEnvironment
+    def comm_rank(self):
+        st = N.Status()
+        r = N.host_lib().gauxc_runtime_environment_comm_rank(st.ptr, self.h)
+        st.raise_if()
+        return r
+
+
+class DeviceRuntimeEnvironment(RuntimeEnvironment):
+    """GauXC DeviceRuntimeEnvironment: the integrator's working arena is
+    ``fill_fraction`` of the free device memory, or the caller's device
+    buffer ``mem`` (an integer device address, 256-byte aligned) of
+    ``mem_size`` bytes."""
+
+    def __init__(self, fill_fraction=0.9, mem=None, mem_size=0):
+        st = N.Status()
+        if mem is None:
+            h = N.host_lib().gauxc_device_runtime_environment_new(st.ptr, float(fill_fraction))
+        else:
+            h = N.host_lib().gauxc_device_runtime_environment_new_mem(st.ptr, mem, mem_size)
+        st.raise_if()
+        _Handle.__init__(self, h)
 
 
 class LoadBalancer(_Handle):
@@ -339,15 +368,11 @@ class DeviceContext:
 
     def upload_tasks(self, t):
         nt = len(t["npts"])
-        desc = (N.XcqTask * nt)()
-        for i in range(nt):
-            desc[i] = N.XcqTask(int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
-                                int(t["nbe"][i]), int(t["point_offset"][i]),
-                                int(t["shell_offset"][i]), float(t["dist_nearest"][i]))
+        desc = N.pack_tasks(t)
         pts = np.ascontiguousarray(t["points"], dtype=np.float64)
         w = np.ascontiguousarray(t["weights"], dtype=np.float64)
         sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
-        N.xcq_check(self.lib.xcq_upload_tasks(self.h, nt, desc, len(w), pts.ctypes.data,
+        N.xcq_check(self.lib.xcq_upload_tasks(self.h, nt, desc.ctypes.data, len(w), pts.ctypes.data,
                                               w.ctypes.data, len(sl), sl.ctypes.data),
                     "xcq_upload_tasks")
         self.tasks = t
@@ -512,27 +537,24 @@ class BatchedDeviceContext(DeviceContext):
         sl_base = 0
         for m, t in enumerate(task_dicts):
             nt = len(t["npts"])
-            for i in range(nt):
-                descs.append((int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
-                              int(t["nbe"][i]), int(t["point_offset"][i]) + pt_base,
-                              int(t["shell_offset"][i]) + sl_base, float(t["dist_nearest"][i])))
+            descs.append(N.pack_tasks(t, pt_base, sl_base))
             mols.append(np.full(nt, m, dtype=np.int32))
             pts.append(np.asarray(t["points"], dtype=np.float64).reshape(-1, 3))
             ws.append(np.asarray(t["weights"], dtype=np.float64))
             sls.append(np.asarray(t["shell_list"], dtype=np.int64) + sh_base[m])
             pt_base += len(ws[-1])
             sl_base += len(sls[-1])
-        nt = len(descs)
-        desc = (N.XcqTask * nt)(*[N.XcqTask(*d) for d in descs])
+        desc = np.concatenate(descs)
+        nt = len(desc)
         mol = np.ascontiguousarray(np.concatenate(mols))
         P = np.ascontiguousarray(np.concatenate(pts))
         W = np.ascontiguousarray(np.concatenate(ws))
         SL = np.ascontiguousarray(np.concatenate(sls).astype(np.int32))
-        N.xcq_check(self.lib.xcq_upload_tasks_batched(self.h, nt, desc, mol.ctypes.data, len(W),
+        N.xcq_check(self.lib.xcq_upload_tasks_batched(self.h, nt, desc.ctypes.data,
+                                                      mol.ctypes.data, len(W),
                                                       P.ctypes.data, W.ctypes.data, len(SL),
                                                       SL.ctypes.data), "xcq_upload_tasks_batched")
-        self.tasks = {"npts": np.array([d[1] for d in descs]),
-                      "nbe": np.array([d[3] for d in descs]), "molecule": mol}
+        self.tasks = {"npts": desc["npts"].copy(), "nbe": desc["nbe"].copy(), "molecule": mol}
 
     def upload_density(self, mol, P):
         P = np.asfortranarray(P, dtype=np.float64)

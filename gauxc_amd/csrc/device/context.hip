@@ -117,6 +117,10 @@ struct xcq_context_s {
   hipStream_t stream = nullptr;
   size_t arena_bytes = 0;  // user limit (0 = derive from free memory)
   double fill = 0.9;
+  // Caller-owned device buffer used as the chunk scratch arena
+  // (xcq_context_create_mem); never freed by the context.
+  void* user_arena = nullptr;
+  size_t user_arena_bytes = 0;
   int wall_khz = 100000;
 
   // molecules and basis
@@ -212,7 +216,7 @@ struct xcq_context_s {
     dfree(d_tmax);
     tmax_total = 0;
     dfree(d_tstamp);
-    if (d_scratch) (void)hipFree(d_scratch);
+    if (d_scratch && d_scratch != user_arena) (void)hipFree(d_scratch);
     d_scratch = nullptr;
     scratch_bytes = 0;
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -551,6 +555,22 @@ int xcq_context_create(int device, size_t arena_bytes, double fill_fraction, xcq
   return XCQ_SUCCESS;
 }
 
+int xcq_context_create_mem(int device, void* mem, size_t mem_bytes, xcq_context* out) {
+  XCQ_REQUIRE(out, XCQ_ERROR_INVALID_ARGUMENT, "null context pointer");
+  *out = nullptr;
+  XCQ_REQUIRE(mem || mem_bytes == 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "null device buffer with a non-zero size");
+  XCQ_REQUIRE(((uintptr_t)mem & 255) == 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "the device buffer must be 256-byte aligned");
+  xcq_context ctx = nullptr;
+  int rc = xcq_context_create(device, mem_bytes ? mem_bytes : 1, 0.9, &ctx);
+  if (rc) return rc;
+  ctx->user_arena = mem;
+  ctx->user_arena_bytes = mem_bytes;
+  *out = ctx;
+  return XCQ_SUCCESS;
+}
+
 int xcq_context_destroy(xcq_context ctx) {
   delete ctx;
   return XCQ_SUCCESS;
@@ -756,7 +776,9 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
                           (size_t)nshell_entries * 4 + (size_t)ntasks * sizeof(DevTask) +
                           (size_t)pblocks * 16 * 8;
   size_t budget;
-  if (ctx->arena_bytes) {
+  if (ctx->user_arena) {
+    budget = ctx->user_arena_bytes;
+  } else if (ctx->arena_bytes) {
     budget = ctx->arena_bytes;
   } else {
     const double avail = ctx->fill * (double)free_b - (double)resident;
@@ -828,7 +850,10 @@ int xcq_upload_tasks_batched(xcq_context ctx, int32_t ntasks, const xcq_task* ta
   dfree(ctx->d_mol_blk);
   if ((rc = dalloc(&ctx->d_mol_blk, mol_blocks.size()))) return rc;
   ctx->scratch_bytes = (size_t)max_bytes;
-  if (ctx->scratch_bytes) {
+  if (ctx->user_arena) {
+    // Every chunk fits the caller's buffer (the planner's budget).
+    ctx->d_scratch = ctx->user_arena;
+  } else if (ctx->scratch_bytes) {
     hipError_t e = hipMalloc(&ctx->d_scratch, ctx->scratch_bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();

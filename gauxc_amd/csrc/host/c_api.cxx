@@ -207,18 +207,31 @@ void gauxc_molgrid_delete(GauXCStatus* st, GauXCMolGrid* g) {
 
 GauXCRuntimeEnvironment gauxc_runtime_environment_new(GauXCStatus* st) {
   return guarded(st, [] {
-    return make_handle<GauXCRuntimeEnvironment>(GauXC_Type_RuntimeEnvironment,
-                                                static_cast<RuntimeEnvironment*>(
-                                                    new DeviceRuntimeEnvironment(0.9)));
+    GauXCRuntimeEnvironment e{};
+    e.hdr.type = GauXC_Type_RuntimeEnvironment;
+    e.ptr = new RuntimeEnvironment();
+    return e;
   });
 }
 
 GauXCRuntimeEnvironment gauxc_device_runtime_environment_new(GauXCStatus* st, double fill) {
   return guarded(st, [&] {
     if (!(fill > 0. && fill <= 1.)) throw std::runtime_error("fill_fraction must be in (0, 1]");
-    return make_handle<GauXCRuntimeEnvironment>(
-        GauXC_Type_RuntimeEnvironment,
-        static_cast<RuntimeEnvironment*>(new DeviceRuntimeEnvironment(fill)));
+    GauXCRuntimeEnvironment e{};
+    e.hdr.type = GauXC_Type_RuntimeEnvironment;
+    e.device_ptr = new DeviceRuntimeEnvironment(fill);
+    return e;
+  });
+}
+
+GauXCRuntimeEnvironment gauxc_device_runtime_environment_new_mem(GauXCStatus* st, void* mem,
+                                                                 size_t mem_sz) {
+  return guarded(st, [&] {
+    if (!mem && mem_sz) throw std::runtime_error("null device buffer with a non-zero size");
+    GauXCRuntimeEnvironment e{};
+    e.hdr.type = GauXC_Type_RuntimeEnvironment;
+    e.device_ptr = new DeviceRuntimeEnvironment(mem, mem_sz);
+    return e;
   });
 }
 
@@ -226,8 +239,26 @@ void gauxc_runtime_environment_delete(GauXCStatus* st, GauXCRuntimeEnvironment* 
   guarded_void(st, [&] {
     if (!e) return;
     delete static_cast<RuntimeEnvironment*>(e->ptr);
+    delete static_cast<DeviceRuntimeEnvironment*>(e->device_ptr);
     e->ptr = nullptr;
+    e->device_ptr = nullptr;
   });
+}
+
+static const RuntimeEnvironment& runtime_of(const GauXCRuntimeEnvironment& e) {
+  void* p = e.device_ptr ? e.device_ptr : e.ptr;
+  if (e.hdr.type != GauXC_Type_RuntimeEnvironment || !p)
+    throw std::runtime_error("Invalid RuntimeEnvironment handle");
+  if (e.device_ptr) return *static_cast<const DeviceRuntimeEnvironment*>(e.device_ptr);
+  return *static_cast<const RuntimeEnvironment*>(e.ptr);
+}
+
+int gauxc_runtime_environment_comm_rank(GauXCStatus* st, const GauXCRuntimeEnvironment e) {
+  return guarded(st, [&] { return runtime_of(e).comm_rank(); });
+}
+
+int gauxc_runtime_environment_comm_size(GauXCStatus* st, const GauXCRuntimeEnvironment e) {
+  return guarded(st, [&] { return runtime_of(e).comm_size(); });
 }
 
 GauXCLoadBalancerFactory gauxc_load_balancer_factory_new(GauXCStatus* st,
@@ -256,7 +287,7 @@ GauXCLoadBalancer gauxc_load_balancer_factory_get_instance(GauXCStatus* st,
                                                            const GauXCBasisSet basis) {
   return guarded(st, [&] {
     auto& fac = deref<LoadBalancerFactory>(f, GauXC_Type_LoadBalancerFactory, "factory");
-    const auto& rt = deref<RuntimeEnvironment>(env, GauXC_Type_RuntimeEnvironment, "runtime");
+    const auto& rt = runtime_of(env);
     const auto& m = deref<Molecule>(mol, GauXC_Type_Molecule, "molecule");
     const auto& g = deref<MolGrid>(mg, GauXC_Type_MolGrid, "molgrid");
     const auto& b = deref<BasisSet<double>>(basis, GauXC_Type_BasisSet, "basis");
@@ -445,6 +476,50 @@ void* gauxc_integrator_device_context(GauXCStatus* st, const GauXCIntegrator in)
     auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
     return I.device_impl().device_context();
   });
+}
+
+// Type-dispatched deletion (GauXC src/c-api/c_types.cxx:29-109): the handle's
+// header names its type.
+void gauxc_object_delete(GauXCStatus* st, void** obj) {
+  set_status(st, 0, nullptr);
+  if (!obj || !*obj) return;
+  switch (static_cast<const GauXCHeader*>(*obj)->type) {
+    case GauXC_Type_Molecule: gauxc_molecule_delete(st, static_cast<GauXCMolecule*>(*obj)); break;
+    case GauXC_Type_BasisSet: gauxc_basisset_delete(st, static_cast<GauXCBasisSet*>(*obj)); break;
+    case GauXC_Type_MolGrid: gauxc_molgrid_delete(st, static_cast<GauXCMolGrid*>(*obj)); break;
+    case GauXC_Type_RuntimeEnvironment:
+      gauxc_runtime_environment_delete(st, static_cast<GauXCRuntimeEnvironment*>(*obj));
+      break;
+    case GauXC_Type_LoadBalancer:
+      gauxc_load_balancer_delete(st, static_cast<GauXCLoadBalancer*>(*obj));
+      break;
+    case GauXC_Type_LoadBalancerFactory:
+      gauxc_load_balancer_factory_delete(st, static_cast<GauXCLoadBalancerFactory*>(*obj));
+      break;
+    case GauXC_Type_MolecularWeights:
+      gauxc_molecular_weights_delete(st, static_cast<GauXCMolecularWeights*>(*obj));
+      break;
+    case GauXC_Type_MolecularWeightsFactory:
+      gauxc_molecular_weights_factory_delete(st,
+                                             static_cast<GauXCMolecularWeightsFactory*>(*obj));
+      break;
+    case GauXC_Type_Functional:
+      gauxc_functional_delete(st, static_cast<GauXCFunctional*>(*obj));
+      break;
+    case GauXC_Type_Integrator:
+      gauxc_integrator_delete(st, static_cast<GauXCIntegrator*>(*obj));
+      break;
+    default: set_status(st, 1, "Unknown object type in gauxc_object_delete"); break;
+  }
+}
+
+void gauxc_objects_delete(GauXCStatus* st, void** objs, size_t n) {
+  set_status(st, 0, nullptr);
+  for (size_t i = 0; objs && i < n; ++i) {
+    if (!objs[i]) continue;
+    gauxc_object_delete(st, &objs[i]);
+    if (st && st->code != 0) return;
+  }
 }
 
 }  // extern "C"

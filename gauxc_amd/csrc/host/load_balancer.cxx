@@ -26,14 +26,15 @@ bool cube_sphere_intersect(const double* lo, const double* up, const double* c, 
 }  // namespace geometry
 
 LoadBalancer::LoadBalancer(const Molecule& mol, const MolGrid& mg,
-                           const BasisSet<double>& basis)
+                           const BasisSet<double>& basis, const RuntimeEnvironment& rt)
     : mol_(std::make_shared<Molecule>(mol)),
       mg_(std::make_shared<MolGrid>(mg)),
       basis_(std::make_shared<BasisSet<double>>(basis)),
-      meta_(std::make_shared<MolMeta>(mol)) {}
+      meta_(std::make_shared<MolMeta>(mol)),
+      rt_(copy_runtime(rt)) {}
 
 LoadBalancer::LoadBalancer(const LoadBalancer& o)
-    : mol_(o.mol_), mg_(o.mg_), basis_(o.basis_), meta_(o.meta_), tasks_(o.tasks_),
+    : mol_(o.mol_), mg_(o.mg_), basis_(o.basis_), meta_(o.meta_), rt_(o.rt_), tasks_(o.tasks_),
       built_(o.built_), state_(o.state_), revision_(o.revision_) {}
 
 void LoadBalancer::build_tasks() const {
@@ -121,10 +122,10 @@ LoadBalancerFactory::LoadBalancerFactory(ExecutionSpace ex, std::string kernel_n
     GAUXC_GENERIC_EXCEPTION("Load Balancer Kernel Not Recognized: " + kernel_);
 }
 
-LoadBalancer LoadBalancerFactory::get_instance(const RuntimeEnvironment&, const Molecule& mol,
+LoadBalancer LoadBalancerFactory::get_instance(const RuntimeEnvironment& rt, const Molecule& mol,
                                                const MolGrid& mg,
                                                const BasisSet<double>& basis) {
-  return LoadBalancer(mol, mg, basis);
+  return LoadBalancer(mol, mg, basis, rt);
 }
 
 std::shared_ptr<LoadBalancer> LoadBalancerFactory::get_shared_instance(

@@ -51,7 +51,11 @@ void MolecularWeights::modify_weights(LoadBalancer& lb) const {
     xyz[3 * a + 1] = mol[a].y;
     xyz[3 * a + 2] = mol[a].z;
   }
-  if (xcq_ssf_weights(device_, static_cast<int32_t>(mol.natoms()), xyz.data(),
+  // The load balancer's device environment names the device (GauXC runs
+  // the device weights on the runtime it was given).
+  const auto* drt = as_device_runtime_ptr(lb.runtime());
+  const int device = drt ? drt->device_id() : device_;
+  if (xcq_ssf_weights(device, static_cast<int32_t>(mol.natoms()), xyz.data(),
                       static_cast<int64_t>(npts), pts.data(), parent.data(), w.data()) !=
       XCQ_SUCCESS)
     GAUXC_GENERIC_EXCEPTION(std::string("SSF weights failed: ") + xcq_last_error());

@@ -51,8 +51,10 @@ struct DeviceXCIntegratorImpl::State {
   std::shared_ptr<functional_type> func;
   std::shared_ptr<LoadBalancer> lb;
   DevicePrecision prec;
-  int device;
-  double fill_fraction;
+  int device = 0;
+  double fill_fraction = 0.9;
+  void* user_mem = nullptr;  // caller-owned device arena (DeviceRuntimeEnvironment(mem, sz))
+  size_t user_mem_size = 0;
   xcq_context ctx = nullptr;
   uint64_t uploaded_revision = ~uint64_t(0);
   bool basis_uploaded = false;
@@ -65,7 +67,11 @@ struct DeviceXCIntegratorImpl::State {
 
   void ensure_context() {
     if (ctx) return;
-    check(xcq_context_create(device, 0, fill_fraction, &ctx), "xcq_context_create");
+    if (user_mem)
+      check(xcq_context_create_mem(device, user_mem, user_mem_size, &ctx),
+            "xcq_context_create_mem");
+    else
+      check(xcq_context_create(device, 0, fill_fraction, &ctx), "xcq_context_create");
     // SCF calls repeat the same evaluation with a new P: capture it once as a
     // graph and replay it.
     check(xcq_use_graph(ctx, 1), "xcq_use_graph");
@@ -151,15 +157,24 @@ struct DeviceXCIntegratorImpl::State {
 
 DeviceXCIntegratorImpl::DeviceXCIntegratorImpl(std::shared_ptr<functional_type> func,
                                                std::shared_ptr<LoadBalancer> lb,
-                                               DevicePrecision prec, int device,
-                                               double fill_fraction)
+                                               DevicePrecision prec)
     : s_(std::make_unique<State>()) {
   if (!func || !lb) GAUXC_GENERIC_EXCEPTION("null functional or load balancer");
   s_->func = std::move(func);
   s_->lb = std::move(lb);
   s_->prec = prec;
-  s_->device = device;
-  s_->fill_fraction = fill_fraction;
+  // GauXC sizes its device arena from the DeviceRuntimeEnvironment the load
+  // balancer holds: fill_fraction of the free memory, or the caller's buffer
+  // (device_runtime_environment_impl.hpp:42-50, xc_device_stack_data.cxx:713-751).
+  if (const auto* drt = as_device_runtime_ptr(s_->lb->runtime())) {
+    s_->device = drt->device_id();
+    if (drt->owns_memory()) {
+      s_->fill_fraction = drt->fill_fraction();
+    } else {
+      s_->user_mem = drt->device_memory();
+      s_->user_mem_size = drt->device_memory_size();
+    }
+  }
 }
 
 DeviceXCIntegratorImpl::~DeviceXCIntegratorImpl() = default;
@@ -243,8 +258,7 @@ std::shared_ptr<detail::DeviceXCIntegratorImpl> make_device_integrator_impl(
   if (rk != "DEFAULT" && rk != "BASICMPI" && rk != "NCCL")
     GAUXC_GENERIC_EXCEPTION("Reduction Kernel Not Recognized: " + reduction_kernel);
   if (!func || !func->initialized()) GAUXC_PIMPL_NOT_INITIALIZED();
-  return std::make_shared<detail::DeviceXCIntegratorImpl>(std::move(func), std::move(lb), prec,
-                                                          0, 0.9);
+  return std::make_shared<detail::DeviceXCIntegratorImpl>(std::move(func), std::move(lb), prec);
 }
 
 }  // namespace GauXC

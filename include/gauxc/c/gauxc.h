@@ -91,7 +91,6 @@ typedef struct GauXCShell {
 GAUXC_HANDLE(GauXCMolecule)
 GAUXC_HANDLE(GauXCBasisSet)
 GAUXC_HANDLE(GauXCMolGrid)
-GAUXC_HANDLE(GauXCRuntimeEnvironment)
 GAUXC_HANDLE(GauXCLoadBalancer)
 GAUXC_HANDLE(GauXCLoadBalancerFactory)
 GAUXC_HANDLE(GauXCMolecularWeights)
@@ -99,6 +98,16 @@ GAUXC_HANDLE(GauXCMolecularWeightsFactory)
 GAUXC_HANDLE(GauXCFunctional)
 GAUXC_HANDLE(GauXCIntegrator)
 #undef GAUXC_HANDLE
+
+/* Runtime environment handle of a device build of GauXC
+ * (include/gauxc/c/runtime_environment.h:31-37 with GAUXC_HAS_DEVICE):
+ * `ptr` holds a host RuntimeEnvironment, `device_ptr` a
+ * DeviceRuntimeEnvironment; exactly one of them is set. */
+typedef struct GauXCRuntimeEnvironment {
+  GauXCHeader hdr;
+  void* ptr;
+  void* device_ptr;
+} GauXCRuntimeEnvironment;
 
 typedef struct GauXCMolecularWeightsSettings {
   enum GauXC_XCWeightAlg weight_alg;
@@ -130,11 +139,20 @@ GauXCMolGrid gauxc_molgrid_new_default(GauXCStatus* status, const GauXCMolecule 
                                        enum GauXC_AtomicGridSizeDefault grid_size);
 void gauxc_molgrid_delete(GauXCStatus* status, GauXCMolGrid* molgrid);
 
-/* Runtime environment (GauXC c/runtime_environment.h; no MPI communicator) */
+/* Runtime environment (GauXC c/runtime_environment.h; no MPI communicator).
+ * A host environment runs device work on device 0 with fill fraction 0.9.
+ * The device environment's fill fraction sizes the integrator's working
+ * arena from free device memory; _new_mem hands the integrator a caller-owned
+ * device buffer (256-byte aligned) as that arena instead
+ * (runtime_environment.h:89-108). */
 GauXCRuntimeEnvironment gauxc_runtime_environment_new(GauXCStatus* status);
 GauXCRuntimeEnvironment gauxc_device_runtime_environment_new(GauXCStatus* status,
                                                              double fill_fraction);
+GauXCRuntimeEnvironment gauxc_device_runtime_environment_new_mem(GauXCStatus* status, void* mem,
+                                                                 size_t mem_sz);
 void gauxc_runtime_environment_delete(GauXCStatus* status, GauXCRuntimeEnvironment* env);
+int gauxc_runtime_environment_comm_rank(GauXCStatus* status, const GauXCRuntimeEnvironment env);
+int gauxc_runtime_environment_comm_size(GauXCStatus* status, const GauXCRuntimeEnvironment env);
 
 /* Load balancer (GauXC c/load_balancer.h) */
 GauXCLoadBalancerFactory gauxc_load_balancer_factory_new(GauXCStatus* status,
@@ -202,6 +220,12 @@ void gauxc_integrator_eval_exc_vxc_rks(GauXCStatus* status, const GauXCIntegrato
 /* Extension: the integrator's device context (an xcq_context of
  * include/gauxc_amd/device.h) for stage timings; NULL before first use. */
 void* gauxc_integrator_device_context(GauXCStatus* status, const GauXCIntegrator integrator);
+
+/* Type-dispatched deletion (GauXC c/types.h:49-76): *handle points at any
+ * handle struct above; its header selects the delete function, which
+ * clears the handle's pointer.  NULL entries are skipped. */
+void gauxc_object_delete(GauXCStatus* status, void** handle);
+void gauxc_objects_delete(GauXCStatus* status, void** handles, size_t nhandles);
 
 #ifdef __cplusplus
 }

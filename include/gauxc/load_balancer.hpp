@@ -33,6 +33,7 @@ class LoadBalancer {
   std::shared_ptr<const MolGrid> mg_;
   std::shared_ptr<const BasisSet<double>> basis_;
   std::shared_ptr<const MolMeta> meta_;
+  std::shared_ptr<const RuntimeEnvironment> rt_;
   mutable std::vector<XCTask> tasks_;
   mutable bool built_ = false;
   LoadBalancerState state_;
@@ -41,7 +42,8 @@ class LoadBalancer {
   void build_tasks() const;
 
  public:
-  LoadBalancer(const Molecule& mol, const MolGrid& mg, const BasisSet<double>& basis);
+  LoadBalancer(const Molecule& mol, const MolGrid& mg, const BasisSet<double>& basis,
+               const RuntimeEnvironment& rt = RuntimeEnvironment());
   LoadBalancer(const LoadBalancer&);
   LoadBalancer(LoadBalancer&&) noexcept = default;
   ~LoadBalancer() = default;
@@ -52,6 +54,9 @@ class LoadBalancer {
   const MolMeta& molmeta() const { return *meta_; }
   const MolGrid& molgrid() const { return *mg_; }
   const BasisSet<double>& basis() const { return *basis_; }
+  /// The runtime environment this load balancer was built with (GauXC
+  /// include/gauxc/load_balancer.hpp:115-116).
+  const RuntimeEnvironment& runtime() const { return *rt_; }
   LoadBalancerState& state() { return state_; }
   const LoadBalancerState& state() const { return state_; }
   size_t max_npts() const;

@@ -77,9 +77,10 @@ class DeviceXCIntegratorImpl {
   std::unique_ptr<State> s_;
 
  public:
+  /// The device, the fill fraction or the caller's device arena come from
+  /// the load balancer's runtime environment (lb->runtime()).
   DeviceXCIntegratorImpl(std::shared_ptr<functional_type> func,
-                         std::shared_ptr<LoadBalancer> lb, DevicePrecision prec, int device,
-                         double fill_fraction);
+                         std::shared_ptr<LoadBalancer> lb, DevicePrecision prec);
   ~DeviceXCIntegratorImpl();
   void eval_exc_vxc(int64_t m, int64_t n, const double* P, int64_t ldp, double* VXC,
                     int64_t ldvxc, double* EXC, const IntegratorSettingsXC& s);

@@ -67,6 +67,15 @@ int xcq_device_count(int* count);
 const char* xcq_last_error(void);
 /* arena_bytes = 0: size the working arena as fill_fraction of free memory. */
 int xcq_context_create(int device, size_t arena_bytes, double fill_fraction, xcq_context* ctx);
+/* Context whose chunk scratch arena (the per-task basis-function planes,
+ * the bulk of the device memory an evaluation needs) is the caller's device
+ * buffer `mem` of `mem_bytes` bytes, 256-byte aligned, owned by the caller
+ * and never freed here (GauXC DeviceRuntimeEnvironment(mem, mem_sz),
+ * include/gauxc/runtime_environment/decl.hpp:71-72).  Task tables, points,
+ * P and VXC stay in the context's own allocations.  An upload whose largest
+ * task does not fit the buffer fails with "Insufficient memory to even
+ * start!" (xc_device_stack_data.cxx:720-721). */
+int xcq_context_create_mem(int device, void* mem, size_t mem_bytes, xcq_context* ctx);
 int xcq_context_destroy(xcq_context ctx);
 int xcq_synchronize(xcq_context ctx);
 

@@ -54,6 +54,8 @@ def lib():
                                      ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64,
                                      ctypes.c_int]
         L.oracle_exc_vxc.restype = ctypes.c_int
+        L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_use_blas.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -73,13 +75,17 @@ def shells_from(struct_array):
     return out
 
 
+_TASK_DTYPE = np.dtype([("iparent", "<i4"), ("npts", "<i4"), ("nshells", "<i4"),
+                        ("nbe", "<i4"), ("point_offset", "<i8"), ("shell_offset", "<i8"),
+                        ("dist_nearest", "<f8")], align=True)
+assert _TASK_DTYPE.itemsize == ctypes.sizeof(OracleTask)
+
+
 def _task_array(t):
-    nt = len(t["npts"])
-    arr = (OracleTask * nt)()
-    for i in range(nt):
-        arr[i] = OracleTask(int(t["iparent"][i]), int(t["npts"][i]), int(t["nshells"][i]),
-                            int(t["nbe"][i]), int(t["point_offset"][i]),
-                            int(t["shell_offset"][i]), float(t["dist_nearest"][i]))
+    """oracle_task records of a task dict, as a numpy structured array."""
+    arr = np.zeros(len(t["npts"]), dtype=_TASK_DTYPE)
+    for k in _TASK_DTYPE.names:
+        arr[k] = t[k]
     return arr
 
 
@@ -89,7 +95,7 @@ def ssf_weights(xyz, t, nthreads=0):
     pts = np.ascontiguousarray(t["points"], dtype=np.float64)
     w = np.array(t["weights"], dtype=np.float64, copy=True)
     tasks = _task_array(t)
-    lib().oracle_ssf_weights(len(xyz), xyz.ctypes.data, len(t["npts"]), ctypes.addressof(tasks),
+    lib().oracle_ssf_weights(len(xyz), xyz.ctypes.data, len(t["npts"]), tasks.ctypes.data,
                              pts.ctypes.data, w.ctypes.data, nthreads)
     return w
 
@@ -132,11 +138,33 @@ def exc_vxc(func, nbf, shells, P, t, weights=None, nthreads=0):
     nel = ctypes.c_double(0.)
     tasks = _task_array(t)
     rc = lib().oracle_exc_vxc(FUNC[func], nbf, ctypes.addressof(shells), P.ctypes.data, nbf,
-                              len(t["npts"]), ctypes.addressof(tasks), pts.ctypes.data,
+                              len(t["npts"]), tasks.ctypes.data, pts.ctypes.data,
                               w.ctypes.data, sl.ctypes.data, ctypes.addressof(exc),
                               ctypes.addressof(nel), vxc.ctypes.data, nbf, nthreads)
     assert rc == 0
     return exc.value, nel.value, vxc
+
+
+# Host BLAS libraries tried, in order, for the CPU baseline (MKL's single
+# dynamic library first, then OpenBLAS and the reference BLAS of the system).
+BLAS_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1",
+                   "libmkl_rt.so", "libopenblas.so.0", "libopenblas.so", "libblas.so.3"]
+
+
+def use_blas(path="auto"):
+    """Route the oracle's two per-task products through a host BLAS.
+
+    path="auto" tries BLAS_CANDIDATES; None returns to the plain loops.
+    Returns the loaded library's path, or None when the loops are in use."""
+    L = lib()
+    buf = ctypes.create_string_buffer(512)
+    if path is None:
+        L.oracle_use_blas(None, buf, len(buf))
+        return None
+    for cand in (BLAS_CANDIDATES if path == "auto" else [path]):
+        if L.oracle_use_blas(cand.encode(), buf, len(buf)) == 0:
+            return buf.value.decode()
+    return None
 
 
 def gg_ref():

@@ -2,8 +2,11 @@
 // Every function names the GauXC file:line it restates.  See oracle.h.
 #include "oracle.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -12,6 +15,20 @@
 #endif
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Optional host BLAS (GauXC's host integrator calls dgemm_/dsyr2k_ through
+// host/blas.cxx:29,38).  Loaded at run time by oracle_use_blas; when absent
+// the plain loops below compute the same products.
+// ---------------------------------------------------------------------------
+using dgemm_t = void (*)(const char*, const char*, const int*, const int*, const int*,
+                         const double*, const double*, const int*, const double*, const int*,
+                         const double*, double*, const int*);
+using dsyr2k_t = void (*)(const char*, const char*, const int*, const int*, const double*,
+                          const double*, const int*, const double*, const int*, const double*,
+                          double*, const int*);
+dgemm_t g_dgemm = nullptr;
+dsyr2k_t g_dsyr2k = nullptr;
 
 // ---------------------------------------------------------------------------
 // SSF weights: src/xc_integrator/local_work_driver/host/reference/weights.cxx:117-237,
@@ -381,6 +398,31 @@ int oracle_functional(int func, int64_t n, const double* rho, const double* sigm
   return 0;
 }
 
+int oracle_use_blas(const char* path, char* which, int nwhich) {
+  g_dgemm = nullptr;
+  g_dsyr2k = nullptr;
+  if (which && nwhich > 0) which[0] = 0;
+  if (!path || !path[0]) return 0;
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 1;
+  // MKL: sequential BLAS inside the OpenMP task loop (as GauXC's host
+  // integrator, which threads over tasks).
+  using set_layer_t = int (*)(int);
+  if (auto set_layer = (set_layer_t)dlsym(h, "MKL_Set_Threading_Layer")) set_layer(1);
+  using set_threads_t = void (*)(int);
+  if (auto set_threads = (set_threads_t)dlsym(h, "openblas_set_num_threads")) set_threads(1);
+  auto gemm = (dgemm_t)dlsym(h, "dgemm_");
+  auto syr2k = (dsyr2k_t)dlsym(h, "dsyr2k_");
+  if (!gemm || !syr2k) {
+    dlclose(h);
+    return 2;
+  }
+  g_dgemm = gemm;
+  g_dsyr2k = syr2k;
+  if (which && nwhich > 0) std::snprintf(which, (size_t)nwhich, "%s", path);
+  return 0;
+}
+
 int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const double* P,
                    int64_t ldp, int32_t ntasks, const oracle_task* tasks, const double* points,
                    const double* weights, const int32_t* shell_lists, double* exc, double* nel,
@@ -436,13 +478,20 @@ int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const doub
         for (int32_t i = 0; i < nbe; ++i)
           Psub[(size_t)i + (size_t)j * nbe] = P[bf[i] + (size_t)bf[j] * ldp];
       X.assign((size_t)npts * nbe, 0.);
-      for (int32_t p = 0; p < npts; ++p) {
-        double* xr = X.data() + (size_t)p * nbe;
-        const double* br = B.data() + (size_t)p * nbe;
-        for (int32_t k = 0; k < nbe; ++k) {
-          const double bk = 2.0 * br[k];
-          const double* pc = Psub.data() + (size_t)k * nbe;  // P_sub symmetric: row k
-          for (int32_t j = 0; j < nbe; ++j) xr[j] += bk * pc[j];
+      if (g_dgemm) {
+        // B is [npts][nbe] = column-major nbe x npts: X = 2 P_sub B.
+        const double two = 2.0, zero = 0.0;
+        g_dgemm("N", "N", &nbe, &npts, &nbe, &two, Psub.data(), &nbe, B.data(), &nbe, &zero,
+                X.data(), &nbe);
+      } else {
+        for (int32_t p = 0; p < npts; ++p) {
+          double* xr = X.data() + (size_t)p * nbe;
+          const double* br = B.data() + (size_t)p * nbe;
+          for (int32_t k = 0; k < nbe; ++k) {
+            const double bk = 2.0 * br[k];
+            const double* pc = Psub.data() + (size_t)k * nbe;  // P_sub symmetric: row k
+            for (int32_t j = 0; j < nbe; ++j) xr[j] += bk * pc[j];
+          }
         }
       }
       // eval_uvvar_{lda,gga}_rks (:150-163, :242-268).
@@ -498,12 +547,20 @@ int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const doub
       // inc_vxc: V = B^T Z + Z^T B (lower), scattered with atomics (:1678-1692,
       // host/util.hpp:130-168).
       V.assign((size_t)nbe * nbe, 0.);
-      for (int32_t p = 0; p < npts; ++p) {
-        const size_t o = (size_t)p * nbe;
-        for (int32_t i = 0; i < nbe; ++i) {
-          const double bi = B[o + i], zi = Z[o + i];
-          double* vrow = V.data() + (size_t)i * nbe;
-          for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[o + j] + zi * B[o + j];
+      if (g_dsyr2k) {
+        // Column-major nbe x npts operands: the upper triangle of B Z^T + Z B^T
+        // is V[i * nbe + j] for j <= i, the layout of the loops below.
+        const double one = 1.0, zero = 0.0;
+        g_dsyr2k("U", "N", &nbe, &npts, &one, B.data(), &nbe, Z.data(), &nbe, &zero, V.data(),
+                 &nbe);
+      } else {
+        for (int32_t p = 0; p < npts; ++p) {
+          const size_t o = (size_t)p * nbe;
+          for (int32_t i = 0; i < nbe; ++i) {
+            const double bi = B[o + i], zi = Z[o + i];
+            double* vrow = V.data() + (size_t)i * nbe;
+            for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[o + j] + zi * B[o + j];
+          }
         }
       }
       for (int32_t i = 0; i < nbe; ++i)

@@ -51,6 +51,13 @@ void oracle_collocation(const oracle_shell* shells, int32_t nlist, const int32_t
 int oracle_functional(int func, int64_t n, const double* rho, const double* sigma, double* eps,
                       double* vrho, double* vsigma);
 
+/* Host BLAS for the two per-task products (dgemm_, dsyr2k_ with 32-bit
+ * integers, as GauXC's host/blas.cxx:29,38 calls them), loaded with dlopen
+ * from `path`; NULL or "" returns to the plain loops.  Returns 0 on success
+ * (the loaded path is copied to `which`), 1 if the library does not load,
+ * 2 if it lacks either symbol (the loops stay in use). */
+int oracle_use_blas(const char* path, char* which, int nwhich);
+
 /* Full EXC/VXC local work over the given tasks (weights already partitioned).
  * VXC is nbf x nbf column-major, symmetrised on return.  Returns 0 on success. */
 int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const double* P,

@@ -53,3 +53,46 @@ def benzene_setup(tag, grid="UltraFineGrid"):
     lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
                                                                  mg, bs)
     return d, mol, bs, lb
+
+
+CALLER_SRC = os.path.join(ROOT, "tests", "cxx", "caller.cxx")
+CALLER_BIN = os.path.join(ROOT, "tests", "cxx", "_build", "caller")
+
+
+def build_caller():
+    """The C++ caller program (tests/cxx/caller.cxx) against include/ and
+    libgauxc_amd.so; rebuilt when the source is newer than the binary."""
+    import subprocess
+    lib = os.path.join(ROOT, "gauxc_amd", "lib")
+    if (not os.path.exists(CALLER_BIN)
+            or os.path.getmtime(CALLER_BIN) < os.path.getmtime(CALLER_SRC)):
+        os.makedirs(os.path.dirname(CALLER_BIN), exist_ok=True)
+        subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+                        CALLER_SRC, "-o", CALLER_BIN, "-L" + lib, "-lgauxc_amd",
+                        "-Wl,-rpath," + lib, "-ldl"], check=True)
+    return CALLER_BIN
+
+
+def write_caller_input(path, Z, xyz, l, pure, alpha, coeff, origin, P):
+    """Input file of tests/cxx/caller.cxx (uncontracted shells)."""
+    with open(path, "wb") as f:
+        f.write(np.int64(len(Z)).tobytes())
+        for z, r in zip(Z, xyz):
+            f.write(np.int64(z).tobytes())
+            f.write(np.asarray(r, dtype=np.float64).tobytes())
+        f.write(np.int64(len(l)).tobytes())
+        for i in range(len(l)):
+            f.write(np.int32(l[i]).tobytes())
+            f.write(np.int32(int(pure[i])).tobytes())
+            f.write(np.float64(alpha[i][0]).tobytes())
+            f.write(np.float64(coeff[i][0]).tobytes())
+            f.write(np.asarray(origin[i], dtype=np.float64).tobytes())
+        f.write(np.int64(P.shape[0]).tobytes())
+        f.write(np.asfortranarray(P, dtype=np.float64).tobytes(order="F"))
+
+
+def run_caller(args, timeout=120):
+    import subprocess
+    r = subprocess.run([build_caller(), *args], capture_output=True, text=True,
+                       timeout=timeout)
+    return r.returncode, r.stdout + r.stderr

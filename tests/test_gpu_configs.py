@@ -1,0 +1,183 @@
+"""The benchmark configurations at their stated sizes, on the device path.
+
+* 24-water cluster (1,248 functions, 99 x 590 grid, 23,343 tasks, PBE):
+  FP64 through GauXC's API against the oracle on every task, at the FP64
+  bars |dEXC| <= 1e-10 Eh and max|dVXC_ij| <= 1e-9 (the VXC drop tolerance
+  at its default); mixed precision ("Scheme1-Mixed") against the oracle and
+  against FP64 at |dEXC| <= 1e-6 Eh and max|dVXC_ij| <= 1e-7.
+* 150-water cluster (7,800 functions, 172,543 tasks): mixed against FP64 at
+  the mixed bars, NEL included; FP64 against the oracle on an evenly spaced
+  sample of the tasks at the FP64 bars.
+* 64 independent clusters of 2-40 waters in one batched context: every
+  molecule's mixed result against its FP64 result at the mixed bars.
+
+GauXC's own end-to-end check is tests/xc_integrator.cxx:199-216.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_EXC, FP64_VXC = 1e-10, 1e-9
+MIXED_EXC, MIXED_VXC = 1e-6, 1e-7
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1(gx, orc, gpu):
+    """24-water inputs and the oracle's result (oracle SSF weights, every task)."""
+    _flush("config 24 waters: setup + oracle")
+    mol, bs, P, cfg = gx.workloads.make_config("c1", seed=0)
+    lb = gx.workloads.build_tasks(mol, bs, grid=cfg["grid"])
+    t = lb.get_tasks()
+    w = orc.ssf_weights(cfg["xyz"], t)
+    ref = orc.exc_vxc("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), P, t, weights=w)
+    return mol, bs, P, cfg, lb, t, w, ref
+
+
+def _integrator(gx, lb, lwd):
+    return gx.XCIntegratorFactory("Device", "Replicated", "Default", lwd,
+                                  "Default").get_instance(gx.Functional("PBE"), lb)
+
+
+def test_config1_fp64_vs_oracle(gx, c1):
+    mol, bs, P, cfg, lb, t, w_o, (exc_o, nel_o, vxc_o) = c1
+    assert len(t["npts"]) > 20000 and bs.nbf() == 1248
+    _flush("config 24 waters: device SSF weights")
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    w_d = lb.get_tasks()["weights"]
+    assert np.abs(w_d - w_o).max() <= 1e-12 * np.abs(w_o).max()
+    _flush("config 24 waters: FP64 through the GauXC API")
+    exc, vxc = _integrator(gx, lb, "Default").eval_exc_vxc(P)
+    _flush(f"dEXC={exc - exc_o:.3e} max|dVXC|={np.abs(vxc - vxc_o).max():.3e}")
+    assert abs(exc - exc_o) <= FP64_EXC
+    assert np.abs(vxc - vxc_o).max() <= FP64_VXC
+    assert np.linalg.norm(vxc - vxc_o) / bs.nbf() < 1e-10
+
+
+def test_config2_mixed_vs_oracle_and_fp64(gx, c1):
+    mol, bs, P, cfg, lb, t, w_o, (exc_o, nel_o, vxc_o) = c1
+    if not lb.weights_modified():
+        gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    _flush("config 24 waters: mixed through the GauXC API")
+    emx, vmx = _integrator(gx, lb, "Scheme1-Mixed").eval_exc_vxc(P)
+    e64, v64 = _integrator(gx, lb, "Default").eval_exc_vxc(P)
+    _flush(f"mixed-oracle dEXC={emx - exc_o:.3e} max|dVXC|={np.abs(vmx - vxc_o).max():.3e}; "
+           f"mixed-fp64 dEXC={emx - e64:.3e} max|dVXC|={np.abs(vmx - v64).max():.3e}")
+    for e_ref, v_ref in ((exc_o, vxc_o), (e64, v64)):
+        assert abs(emx - e_ref) <= MIXED_EXC
+        assert np.abs(vmx - v_ref).max() <= MIXED_VXC
+    assert np.array_equal(vmx, vmx.T)
+    # Integrated density through the device context: NEL of both precisions
+    # against the oracle's.
+    tt = lb.get_tasks()
+    ctx = gx.DeviceContext(0)
+    ctx.upload_basis(bs.xcq_shells())
+    ctx.upload_tasks(tt)
+    ctx.upload_density(P)
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
+        ctx.exc_vxc("PBE", prec, True)
+        nel = ctx.results(want_vxc=False)[1]
+        assert abs(nel - nel_o) <= tol, (prec, nel, nel_o)
+
+
+def _subset(t, idx):
+    out = {k: t[k][idx] for k in ("iparent", "npts", "nshells", "nbe", "dist_nearest")}
+    pts, w, sl = [], [], []
+    for i in idx:
+        a, n = t["point_offset"][i], t["npts"][i]
+        b, m = t["shell_offset"][i], t["nshells"][i]
+        pts.append(t["points"][a:a + n])
+        w.append(t["weights"][a:a + n])
+        sl.append(t["shell_list"][b:b + m])
+    out["points"] = np.concatenate(pts)
+    out["weights"] = np.concatenate(w)
+    out["shell_list"] = np.concatenate(sl)
+    out["point_offset"] = np.concatenate([[0], np.cumsum(out["npts"])[:-1]]).astype(np.int64)
+    out["shell_offset"] = np.concatenate([[0], np.cumsum(out["nshells"])[:-1]]).astype(np.int64)
+    return out
+
+
+def test_config4_mixed_vs_fp64(gx, orc, gpu):
+    _flush("config 150 waters: setup")
+    mol, bs, P, cfg = gx.workloads.make_config("c4", seed=0)
+    lb = gx.workloads.build_tasks(mol, bs, grid=cfg["grid"])
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    t = lb.get_tasks()
+    assert bs.nbf() == 7800 and len(t["npts"]) > 150000
+    ctx = gx.DeviceContext(0)
+    ctx.upload_basis(bs.xcq_shells())
+    ctx.upload_tasks(t)
+    ctx.upload_density(P)
+    _flush("config 150 waters: FP64")
+    ctx.exc_vxc("PBE", "fp64", True)
+    e64, n64, v64 = ctx.results()
+    _flush("config 150 waters: mixed")
+    ctx.exc_vxc("PBE", "mixed", True)
+    emx, nmx, vmx = ctx.results()
+    _flush(f"EXC={e64:.10f} dEXC={emx - e64:.3e} dNEL={nmx - n64:.3e} "
+           f"max|dVXC|={np.abs(vmx - v64).max():.3e}")
+    assert abs(emx - e64) <= MIXED_EXC
+    assert abs(nmx - n64) <= MIXED_EXC
+    assert np.abs(vmx - v64).max() <= MIXED_VXC
+    ctx.close()
+    del vmx
+    # FP64 against the oracle on every 40th task (the same shapes; the full
+    # set is 26M points).
+    idx = np.arange(0, len(t["npts"]), 40)
+    sub = _subset(t, idx)
+    _flush(f"config 150 waters: FP64 vs oracle on {len(idx)} tasks")
+    exc_o, nel_o, vxc_o = orc.exc_vxc("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), P, sub)
+    sc = gx.DeviceContext(0)
+    sc.upload_basis(bs.xcq_shells())
+    sc.upload_tasks(sub)
+    sc.upload_density(P)
+    sc.exc_vxc("PBE", "fp64", True)
+    e, n, v = sc.results()
+    _flush(f"subset dEXC={e - exc_o:.3e} max|dVXC|={np.abs(v - vxc_o).max():.3e}")
+    assert abs(e - exc_o) <= FP64_EXC
+    assert abs(n - nel_o) <= FP64_EXC
+    assert np.abs(v - vxc_o).max() <= FP64_VXC
+
+
+def test_config3_batched_mixed_vs_fp64(gx, gpu):
+    """All 64 clusters of the batched configuration in one context."""
+    rng = np.random.default_rng(20)
+    nws = rng.integers(2, 41, size=64)
+    recipe = gx.workloads.CONFIGS["c1"]
+    mols = []
+    for m, nw in enumerate(nws):
+        box = 10.0 * (nw / 24.0) ** (1.0 / 3.0)
+        Z, xyz = gx.workloads.water_cluster(int(nw), box, 100 + m)
+        l, pure, alpha, coeff, origin = gx.workloads.even_tempered_basis(Z, xyz, recipe["recipe"])
+        mol = gx.Molecule(Z, xyz)
+        bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True,
+                         shell_tolerance=recipe["tol"])
+        P = gx.workloads.density(bs.nbf(), max(1, min(bs.nbf(), 5 * int(nw))), 100 + m)
+        lb = gx.workloads.build_tasks(mol, bs, grid=recipe["grid"])
+        gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+        mols.append((bs, P, lb.get_tasks()))
+        if (m + 1) % 16 == 0:
+            _flush(f"batched setup {m + 1}/64")
+    b = gx.BatchedDeviceContext(0)
+    b.upload_molecules([bs.xcq_shells() for bs, _, _ in mols])
+    b.upload_tasks([t for _, _, t in mols])
+    for m, (_, P, _) in enumerate(mols):
+        b.upload_density(m, P)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        _flush(f"batched 64 clusters {prec}")
+        b.exc_vxc("PBE", prec, True)
+        res[prec] = [b.results(m) for m in range(len(mols))]
+    worst_e = worst_v = 0.0
+    for m in range(len(mols)):
+        (e64, n64, v64), (emx, nmx, vmx) = res["fp64"][m], res["mixed"][m]
+        worst_e = max(worst_e, abs(emx - e64))
+        worst_v = max(worst_v, float(np.abs(vmx - v64).max()))
+        assert abs(emx - e64) <= MIXED_EXC, (m, int(nws[m]), emx - e64)
+        assert abs(nmx - n64) <= MIXED_EXC, (m, int(nws[m]))
+        assert np.abs(vmx - v64).max() <= MIXED_VXC, (m, int(nws[m]))
+    _flush(f"batched worst dEXC={worst_e:.3e} max|dVXC|={worst_v:.3e}")

@@ -100,6 +100,7 @@ def test_density_wide_tiles(gx, small):
     assert t["nbe"].max() > 128  # some tasks span both 64-column halves of a tile
     ctx = _ctx(gx, bs, t, P)
     ctx.set_tile_policy(48, 96)
+    f0, fn0, fv0 = _run(ctx, "fp64")
     e0, n0, v0 = _run(ctx, "mixed")
     for w, h in [(100, 0), (129, 100000), (0, 0)]:
         _flush(f"density 128-wide from nbe {w}, high occupancy to {h}")
@@ -110,7 +111,10 @@ def test_density_wide_tiles(gx, small):
         assert abs(e1 - e0) <= 1e-11 * abs(e0)
         assert abs(n1 - n0) <= 1e-11 * abs(n0)
         assert np.abs(v1 - v0).max() <= 1e-12
-        ctx.exc_vxc("PBE", "fp64", True)  # FP64 runs 128-wide tasks on 64-wide tiles
+        # FP64 runs 128-wide tasks on 64-wide tiles: the same result as before.
+        f1, fn1, fv1 = _run(ctx, "fp64")
+        assert abs(f1 - f0) <= 1e-12 * abs(f0) and abs(fn1 - fn0) <= 1e-12 * abs(fn0)
+        assert np.abs(fv1 - fv0).max() <= 1e-13
     ctx.set_density_wide(1 << 30)
     ctx.set_density_occupancy(0)
 

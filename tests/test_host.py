@@ -212,5 +212,79 @@ def test_bench_roofline_model_follows_basis_eval_mode():
         assert b3 < b0 and c3 < c0 and z3 > 0
         assert bench.stage_kernels("zmat", prec, 0) == []
         assert len(bench.stage_kernels("zmat", prec, 3)) == 1
-    assert bench.measured_traffic("density", "mixed", 3) is not None
-    assert bench.measured_traffic("syr2k_scatter", "mixed", 3) is not None
+    assert bench.measured_traffic("c1", "density", "mixed", 3) is not None
+    assert bench.measured_traffic("c1", "syr2k_scatter", "mixed", 3) is not None
+    assert bench.measured_traffic("c3", "density", "mixed", 3) is None
+
+
+def _caller_input(gx, tmp_path):
+    from conftest import write_caller_input
+    Z, xyz = gx.workloads.water_cluster(3, 6.0, 0)
+    l, pure, alpha, coeff, origin = gx.workloads.even_tempered_basis(Z, xyz,
+                                                                     gx.workloads.BASIS_C0)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=1e-10)
+    P = gx.workloads.density(bs.nbf(), 15, 0)
+    path = str(tmp_path / "caller_input.bin")
+    write_caller_input(path, Z, xyz, l, pure, alpha, coeff, origin, P)
+    mol = gx.Molecule(Z, xyz)
+    return path, mol, bs, P
+
+
+def test_cxx_caller_builds_and_matches_c_api(gx, tmp_path):
+    """A C++ program written against include/gauxc/*.hpp (GauXC's call
+    sequence) builds, links libgauxc_amd.so and produces the same task set as
+    the C API on the same molecule, basis and grid."""
+    from conftest import run_caller
+    path, mol, bs, P = _caller_input(gx, tmp_path)
+    rc, out = run_caller([path, "host"])
+    assert rc == 0, out
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles", "FineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol, mg,
+                                                                 bs)
+    nt, npt, nsh = lb.task_counts()
+    assert f"tasks {nt} points {npt} shell_entries {nsh} nbf {bs.nbf()}" in out
+
+
+def test_c_api_runtime_and_object_delete(gx):
+    """Runtime environments (host, device fill fraction, caller arena), the
+    communicator queries and type-dispatched deletion of every handle kind."""
+    import ctypes
+    from gauxc_amd import _native as N
+    L = N.host_lib()
+    st = N.Status()
+    rt = gx.RuntimeEnvironment()
+    assert rt.h.ptr and not rt.h.device_ptr and rt.comm_rank() == 0
+    drt = gx.DeviceRuntimeEnvironment(0.5)
+    assert drt.h.device_ptr and not drt.h.ptr
+    assert L.gauxc_runtime_environment_comm_size(st.ptr, drt.h) == 1
+    mrt = gx.DeviceRuntimeEnvironment(mem=256 * 1024, mem_size=4096)
+    assert mrt.h.device_ptr
+    with pytest.raises(gx.GauXCError, match="fill_fraction"):
+        gx.DeviceRuntimeEnvironment(1.5)
+    # A load balancer built with a device environment keeps it.
+    d, mol, bs, lb = benzene_setup("svwn5", grid="GM3")
+    lb2 = gx.LoadBalancerFactory("Host", "Default").get_instance(drt, mol,
+                                                                  gx.MolGridFactory.create_default_molgrid(
+                                                                      mol, "Unpruned", 512,
+                                                                      "MuraKnowles", "GM3"), bs)
+    assert lb2.task_counts() == lb.task_counts()
+    # gauxc_object_delete dispatches on the header and clears the handle.
+    m = gx.Molecule([1, 1], [[0, 0, 0], [0, 0, 1.4]])
+    h = m.h
+    p = ctypes.c_void_p(ctypes.addressof(h))
+    L.gauxc_object_delete(st.ptr, ctypes.byref(p))
+    st.raise_if()
+    assert not h.ptr
+    m.h = None
+    objs = [gx.Functional("PBE"), gx.RuntimeEnvironment(), gx.DeviceRuntimeEnvironment(0.3)]
+    arr = (ctypes.c_void_p * 4)(*[ctypes.addressof(o.h) for o in objs], None)
+    L.gauxc_objects_delete(st.ptr, arr, 4)
+    st.raise_if()
+    assert not objs[0].h.ptr and not objs[1].h.ptr and not objs[2].h.device_ptr
+    for o in objs:
+        o.h = None
+    bad = N.GauXCHandle(99, None)
+    p = ctypes.c_void_p(ctypes.addressof(bad))
+    L.gauxc_object_delete(st.ptr, ctypes.byref(p))
+    with pytest.raises(gx.GauXCError, match="Unknown object type"):
+        st.raise_if()
