@@ -3,7 +3,7 @@
 Workload (one "step" = one full EXC+VXC evaluation over every task):
   150-water cluster in a 17 A box, O 5s4p2d1f / H 3s2p even-tempered
   (beta 2.5) -> 7800 basis functions, 99 x 590 Mura-Knowles/Lebedev grid per
-  atom (26.3M points, 172,543 tasks), SSF weights, 512-point batches, basis
+  atom (26.3M points, 133,238 tasks), SSF weights, 512-point batches, basis
   screening 1e-10, P = 2 C C^T with C of width 750: the configuration the
   mixed-vs-FP64 target is quoted on.  --config c1 runs the 24-water cluster
   (10 A box, 1248 functions, C width 120).  Inputs (tasks, weights, P,

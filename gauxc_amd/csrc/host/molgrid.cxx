@@ -1,4 +1,4 @@
-// Atomic quadratures, octree micro-batching and molecular grid defaults.
+// Atomic quadratures, spherical micro-batching and molecular grid defaults.
 // Reference behaviour: src/grid_factory.cxx:35-69 (unpruned product grids),
 // src/molgrid_defaults.cxx:52-71 (Mura-Knowles scale 7.0 for alkali and
 // alkaline-earth metals, else 5.0) and :144-202 (standard grid sizes).
@@ -48,44 +48,93 @@ void expand_orbit(const LebedevOrbit& o, std::vector<std::array<double, 3>>& pts
   if ((int)seen.size() != o.n) GAUXC_GENERIC_EXCEPTION("Lebedev orbit expansion mismatch");
 }
 
-void octree_partition(const std::vector<std::array<double, 3>>& pts, std::vector<size_t>& idx,
-                      size_t begin, size_t end, std::array<double, 3> lo,
-                      std::array<double, 3> up, size_t max_batch, std::vector<size_t>& offs) {
-  const size_t n = end - begin;
-  if (n == 0) return;
-  if (n <= max_batch) {
+// Tight bounding box of the points idx[begin, end).
+void tight_box(const std::vector<std::array<double, 3>>& pts, const std::vector<size_t>& idx,
+               size_t begin, size_t end, std::array<double, 3>& lo, std::array<double, 3>& up) {
+  lo = up = pts[idx[begin]];
+  for (size_t k = begin + 1; k < end; ++k)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], pts[idx[k]][d]);
+      up[d] = std::max(up[d], pts[idx[k]][d]);
+    }
+}
+
+// Regroups idx[begin, end) by a cell label in [0, ncell) (stable within a
+// cell) and returns the cell boundaries (ncell + 1 offsets).
+template <class Label>
+std::vector<size_t> group_by_cell(std::vector<size_t>& idx, size_t begin, size_t end, int ncell,
+                                  Label label) {
+  std::vector<size_t> count(ncell + 1, 0);
+  std::vector<int> lab(end - begin);
+  for (size_t k = begin; k < end; ++k) {
+    lab[k - begin] = label(idx[k]);
+    ++count[lab[k - begin] + 1];
+  }
+  for (int c = 0; c < ncell; ++c) count[c + 1] += count[c];
+  std::vector<size_t> tmp(idx.begin() + begin, idx.begin() + end), pos(count.begin(), count.end() - 1);
+  for (size_t k = 0; k < tmp.size(); ++k) idx[begin + pos[lab[k]]++] = tmp[k];
+  for (auto& c : count) c += begin;
+  return count;
+}
+
+// Octree over tight boxes: a cell with more than max_batch points is split at
+// the midpoint of its points' bounding box; a coordinate equal to the midpoint
+// goes to the lower half.
+void octree_cells(const std::vector<std::array<double, 3>>& pts, std::vector<size_t>& idx,
+                  size_t begin, size_t end, size_t max_batch, std::vector<size_t>& offs) {
+  if (end == begin) return;
+  if (end - begin <= max_batch) {
     offs.push_back(end);
     return;
   }
-  std::array<double, 3> mid;
+  std::array<double, 3> lo, up, mid;
+  tight_box(pts, idx, begin, end, lo, up);
   for (int d = 0; d < 3; ++d) mid[d] = 0.5 * (lo[d] + up[d]);
-  // Stable three-level split: x, then y, then z; child c has bit d set when
-  // coordinate d lies at or above the midpoint.
-  std::vector<size_t> tmp(idx.begin() + begin, idx.begin() + end);
-  size_t pos = begin;
+  const auto cut = group_by_cell(idx, begin, end, 8, [&](size_t k) {
+    return 4 * (pts[k][0] > mid[0]) + 2 * (pts[k][1] > mid[1]) + (pts[k][2] > mid[2]);
+  });
   for (int c = 0; c < 8; ++c) {
-    const size_t cbeg = pos;
-    for (size_t k : tmp) {
-      bool in = true;
-      for (int d = 0; d < 3 && in; ++d) {
-        const bool hi = pts[k][d] >= mid[d];
-        in = (((c >> d) & 1) != 0) == hi;
-      }
-      if (in) idx[pos++] = k;
+    if (cut[c + 1] - cut[c] == end - begin) {  // no split possible (coincident points)
+      offs.push_back(end);
+      return;
     }
-    std::array<double, 3> clo = lo, cup = up;
-    for (int d = 0; d < 3; ++d) {
-      if ((c >> d) & 1) clo[d] = mid[d];
-      else cup[d] = mid[d];
+    octree_cells(pts, idx, cut[c], cut[c + 1], max_batch, offs);
+  }
+}
+
+// Spherical partition of an atom-centred quadrature: a cell with more than
+// max_batch points is cut into 3 x 3 x 3 sub-cells at the thirds of its
+// points' bounding box; the central sub-cell (around the nucleus, where the
+// radial grid is densest) is partitioned the same way, the 26 outer ones by
+// the octree above.  This reproduces GauXC's batches (IntegratorXX's
+// SphericalMicroBatcher, grid_impl.cxx:32-37) point set for point set: the
+// 664 load-balancer tasks of benzene on the ultra-fine grid and the point
+// sets of the fine-grid SSF fixture tasks.
+void spherical_cells(const std::vector<std::array<double, 3>>& pts, std::vector<size_t>& idx,
+                     size_t begin, size_t end, size_t max_batch, std::vector<size_t>& offs) {
+  if (end == begin) return;
+  if (end - begin <= max_batch) {
+    offs.push_back(end);
+    return;
+  }
+  std::array<double, 3> lo, up, e1, e2;
+  tight_box(pts, idx, begin, end, lo, up);
+  for (int d = 0; d < 3; ++d) {
+    e1[d] = lo[d] + (up[d] - lo[d]) / 3.;
+    e2[d] = lo[d] + 2. * (up[d] - lo[d]) / 3.;
+  }
+  auto third = [&](size_t k, int d) { return (pts[k][d] >= e1[d]) + (pts[k][d] >= e2[d]); };
+  const auto cut = group_by_cell(idx, begin, end, 27, [&](size_t k) {
+    return 9 * third(k, 0) + 3 * third(k, 1) + third(k, 2);
+  });
+  constexpr int kCentre = 13;  // (1, 1, 1)
+  for (int c = 0; c < 27; ++c) {
+    if (cut[c + 1] - cut[c] == end - begin) {
+      offs.push_back(end);
+      return;
     }
-    if (pos > cbeg) {
-      if (pos - cbeg == n) {
-        // All points coincide in one child (degenerate box): stop splitting.
-        offs.push_back(pos);
-      } else {
-        octree_partition(pts, idx, cbeg, pos, clo, cup, max_batch, offs);
-      }
-    }
+    if (c == kCentre) spherical_cells(pts, idx, cut[c], cut[c + 1], max_batch, offs);
+    else octree_cells(pts, idx, cut[c], cut[c + 1], max_batch, offs);
   }
 }
 
@@ -99,13 +148,7 @@ MicroBatcher::MicroBatcher(size_t max_batch, std::shared_ptr<const AtomicQuadrat
   std::iota(order_.begin(), order_.end(), size_t{0});
   offsets_.assign(1, 0);
   if (pts.empty()) return;
-  std::array<double, 3> lo = pts[0], up = pts[0];
-  for (const auto& p : pts)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = std::min(lo[d], p[d]);
-      up[d] = std::max(up[d], p[d]);
-    }
-  octree_partition(pts, order_, 0, pts.size(), lo, up, max_batch_, offsets_);
+  spherical_cells(pts, order_, 0, pts.size(), max_batch_, offsets_);
 }
 
 GridBatch MicroBatcher::at(size_t ib, const std::array<double, 3>& c) const {

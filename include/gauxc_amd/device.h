@@ -33,10 +33,14 @@ enum xcq_functional { XCQ_FUNC_SVWN5 = 1, XCQ_FUNC_PBE = 2, XCQ_FUNC_PBE0 = 3 };
 
 /* Arithmetic of the per-task contractions.
  *   XCQ_PREC_FP64  : everything in FP64.
- *   XCQ_PREC_MIXED : collocation in FP32; X = B*P_sub and B^T*Z on BF16 MFMA
- *                    with split operands and FP32 accumulation; SSF weights,
- *                    the functional at rho < 1e-6 and the EXC/VXC
- *                    accumulation in FP64. */
+ *   XCQ_PREC_MIXED : basis functions evaluated in FP64 and consumed in FP32
+ *                    (the value plane with a bfloat16 residual); X = B*P_sub
+ *                    on FP32 MFMA (FP32 sums within 32-deep K blocks, FP64
+ *                    across them); B^T*Z + Z^T*B on BF16 MFMA over exact
+ *                    bfloat16 slices of the FP32 operands; SSF weights, the
+ *                    functional at every point, EXC/NEL and the VXC
+ *                    accumulation in FP64, with EXC corrected to first order
+ *                    for the FP32 rounding of P. */
 enum xcq_precision { XCQ_PREC_FP64 = 0, XCQ_PREC_MIXED = 1 };
 
 /* Contracted shell with normalised coefficients (GauXC Shell<double>,

@@ -23,8 +23,9 @@ Fixtures converted (reference paths relative to the GauXC tree):
   (``tests/collocation_host.hpp:111-180``).
 * ``tests/basis/old/cc-pvdz.g94`` (H, C, O records) -> raw, unnormalised
   shells stored inside ``water_collocation.npz`` and
-  ``benzene_tasks_ufg.npz`` so the product's own ``Shell`` normalisation is
-  exercised (``tests/basis/parse_basis.cxx:191-231``: l <= 1 cartesian,
+  ``benzene_tasks_ufg.npz``, and ``tests/basis/old/6-31g-star.g94`` (H, C)
+  inside ``benzene_weights_ssf.npz``, so the product's own ``Shell``
+  normalisation is exercised (``tests/basis/parse_basis.cxx:191-231``: l <= 1 cartesian,
   l >= 2 spherical).
 """
 import ctypes
@@ -258,11 +259,16 @@ def main():
     unm = read_tasks(f, "tasks_unm", True)
     mod = read_tasks(f, "tasks_mod", False)
     f.close()
+    # The fixture's tasks were made with 6-31G(d) at shell tolerance 1e-6
+    # (tests/weights.cxx:40-44, standards.cxx:1411-1415): its raw shells are
+    # stored so the load balancer's task point sets can be checked too.
+    g631 = parse_g94(os.path.join(REF, "basis/old/6-31g-star.g94"), {"H", "C"})
     np.savez_compressed(os.path.join(OUT, "benzene_weights_ssf.npz"), Z=Z, xyz=xyz,
                         iParent=unm["iParent"], npts=unm["npts"],
                         dist_nearest=unm["dist_nearest"], points=unm["points"],
                         weights_unm=unm["weights"], weights_mod=mod["weights"],
-                        iParent_mod=mod["iParent"], npts_mod=mod["npts"])
+                        iParent_mod=mod["iParent"], npts_mod=mod["npts"],
+                        **raw_basis_for(Z, xyz, g631))
 
     f = H5File(os.path.join(REF, "ref_data", "benzene_cc-pvdz_ufg_tasks_1mpi_rank0_pv1.hdf5"))
     t = read_tasks(f, "tasks", False)

@@ -94,7 +94,11 @@ def test_device_context_caller_arena_chunks(gx, system):
     ref.upload_density(P)
     ref.exc_vxc("PBE", "fp64", True)
     e0, n0, v0 = ref.results()
-    nbytes = 8 << 20
+    # Scratch of one task: five FP64 planes of nbe x npts (npts rounded up to
+    # 64).  An arena of twice the largest task holds a few tasks at a time.
+    task_bytes = 5 * 8 * t["nbe"].astype(np.int64) * ((t["npts"].astype(np.int64) + 63) // 64 * 64)
+    nbytes = int(2 * task_bytes.max() + 255) // 256 * 256
+    assert task_bytes.sum() > 4 * nbytes
     buf = _device_buffer(nbytes)
     try:
         lib = ref.lib
@@ -106,7 +110,7 @@ def test_device_context_caller_arena_chunks(gx, system):
         ctx.upload_tasks(t)
         ctx.upload_density(P)
         for prec in ("fp64", "mixed"):
-            _flush(f"caller arena 8 MiB {prec}")
+            _flush(f"caller arena {nbytes >> 20} MiB {prec}")
             ctx.exc_vxc("PBE", prec, True)
             e, n, v = ctx.results()
             tol = 1e-12 if prec == "fp64" else 1e-6

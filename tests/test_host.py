@@ -8,7 +8,9 @@
   (benzene_weights_ssf tasks_unm), and the UltraFineGrid integrates a
   Gaussian exactly;
 * load-balancer invariants (point conservation, sorted unique merges,
-  consistent nbe, screening necessary condition);
+  consistent nbe, screening necessary condition), GauXC's 664-task list of
+  benzene on the ultra-fine grid reproduced exactly, and the point sets of
+  the SSF fixture's fine-grid tasks;
 * error paths surface through GauXCStatus without a device.
 """
 import os
@@ -154,9 +156,53 @@ def test_load_balancer_invariants(gx):
             c = np.array(shells[s].origin[:3])
             gap = np.maximum(0, np.maximum(lo - c, c - up))
             assert np.dot(gap, gap) <= rad[s] ** 2 * (1 + 1e-12)
-    # GauXC's IntegratorXX batcher yields 664 merged tasks here; this
-    # octree batcher yields a different partition of the same points.
-    assert 300 < len(t["npts"]) < 2000
+
+
+def test_load_balancer_tasks_equal_gauxc_fixture(gx):
+    """The 664 merged tasks of benzene / cc-pVDZ (shell tolerance eps) on the
+    ultra-fine grid, in order: parent atom, point count, screened function
+    count, shell list and nearest-neighbour distance
+    (GauXC tests/load_balancer_test.cxx:40-82, 90-112)."""
+    d = load_golden("benzene_svwn5.npz")
+    ref = load_golden("benzene_tasks_ufg.npz")
+    _, t = _grid_tasks(gx, d, "UltraFineGrid")
+    assert len(t["npts"]) == len(ref["npts"]) == 664
+    np.testing.assert_array_equal(t["iparent"], ref["iParent"])
+    np.testing.assert_array_equal(t["npts"], ref["npts"])
+    np.testing.assert_array_equal(t["nbe"], ref["nbe"])
+    np.testing.assert_array_equal(t["nshells"], np.diff(ref["shell_off"]))
+    np.testing.assert_array_equal(t["shell_list"], ref["shell_list"])
+    np.testing.assert_allclose(t["dist_nearest"], ref["dist_nearest"], rtol=1e-14)
+
+
+def test_load_balancer_point_sets_equal_gauxc_ssf_fixture(gx):
+    """The 15 fine-grid tasks of GauXC's SSF fixture (6-31G(d), shell
+    tolerance 1e-6; tests/weights_generate.hpp:28-64) are tasks of this load
+    balancer: same parent atom, same point count and the same set of points."""
+    w = load_golden("benzene_weights_ssf.npz")
+    mol = gx.Molecule(w["Z"], w["xyz"])
+    bs = gx.BasisSet.from_arrays(w["raw_nprim"], w["raw_l"], w["raw_pure"], w["raw_alpha"],
+                                 w["raw_coeff"], w["raw_origin"], normalize=True,
+                                 shell_tolerance=1e-6)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles",
+                                                  "FineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    t = lb.get_tasks()
+
+    def canon(p):
+        p = np.round(p, 11)
+        return p[np.lexsort(p.T[::-1])]
+
+    mine = {}
+    for i in range(len(t["npts"])):
+        a, n = t["point_offset"][i], t["npts"][i]
+        mine.setdefault((int(t["iparent"][i]), int(n)), []).append(canon(t["points"][a:a + n]))
+    off = np.concatenate([[0], np.cumsum(w["npts"])])
+    for j in range(len(w["npts"])):
+        key = (int(w["iParent"][j]), int(w["npts"][j]))
+        ref = canon(w["points"][off[j]:off[j + 1]])
+        assert any(np.abs(m - ref).max() < 1e-10 for m in mine.get(key, [])), j
 
 
 def test_errors_without_device(gx):
