@@ -107,6 +107,7 @@ HOST_SYMBOLS = [
     "gauxc_molecular_weights_delete", "gauxc_functional_from_string", "gauxc_functional_delete",
     "gauxc_integrator_new", "gauxc_integrator_delete", "gauxc_integrator_integrate_den",
     "gauxc_integrator_eval_exc_rks", "gauxc_integrator_eval_exc_vxc_rks",
+    "gauxc_integrator_eval_exc_uks", "gauxc_integrator_eval_exc_vxc_uks",
     "gauxc_integrator_device_context",
 ]
 DEVICE_SYMBOLS = [
@@ -114,6 +115,7 @@ DEVICE_SYMBOLS = [
     "xcq_context_destroy",
     "xcq_synchronize", "xcq_upload_basis", "xcq_upload_tasks", "xcq_upload_density",
     "xcq_ssf_weights", "xcq_ssf_weights_tasks",
+    "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
@@ -160,6 +162,9 @@ def device_lib():
             "xcq_ssf_weights": (ctypes.c_int, [ctypes.c_int, c_i32, c_vp, c_i64, c_vp, c_vp,
                                                c_vp]),
             "xcq_ssf_weights_tasks": (ctypes.c_int, [H, c_i32, c_vp]),
+            "xcq_upload_density_z": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
+            "xcq_exc_vxc_uks": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+            "xcq_download_results_uks": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_vp, c_i64]),
             "xcq_download_weights": (ctypes.c_int, [H, c_vp]),
             "xcq_exc_vxc": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "xcq_integrate_den": (ctypes.c_int, [H, c_vp]),
@@ -258,6 +263,10 @@ def host_lib():
             "gauxc_integrator_eval_exc_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp]),
             "gauxc_integrator_eval_exc_vxc_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp,
                                                          c_vp, c_i64]),
+            "gauxc_integrator_eval_exc_uks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp,
+                                                     c_i64, c_vp]),
+            "gauxc_integrator_eval_exc_vxc_uks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64, c_vp,
+                                                         c_i64, c_vp, c_vp, c_i64, c_vp, c_i64]),
             "gauxc_integrator_device_context": (c_vp, [S, Hd]),
         }
         for name, (res, args) in sig.items():

@@ -284,25 +284,41 @@ class XCIntegrator(_Handle):
             raise N.GauXCError("P must be a matrix")
         return P
 
-    def eval_exc_vxc(self, P):
+    def eval_exc_vxc(self, P, Pz=None):
+        """RKS: (EXC, VXC).  UKS (Pz given; P is then P_s = P_a + P_b and
+        Pz = P_a - P_b): (EXC, VXC_s, VXC_z), as GauXC's eval_exc_vxc(Ps, Pz)."""
         P = self._mat(P)
         m, n = P.shape
-        vxc = np.zeros((m, n), order="F")
         exc = ctypes.c_double(0.)
         st = N.Status()
+        if Pz is not None:
+            Pz = self._mat(Pz)
+            vs, vz = np.zeros((m, n), order="F"), np.zeros((m, n), order="F")
+            N.host_lib().gauxc_integrator_eval_exc_vxc_uks(
+                st.ptr, self.h, m, n, P.ctypes.data, m, Pz.ctypes.data, Pz.shape[0],
+                ctypes.addressof(exc), vs.ctypes.data, m, vz.ctypes.data, m)
+            st.raise_if()
+            return exc.value, vs, vz
+        vxc = np.zeros((m, n), order="F")
         N.host_lib().gauxc_integrator_eval_exc_vxc_rks(st.ptr, self.h, m, n, P.ctypes.data, m,
                                                        ctypes.addressof(exc), vxc.ctypes.data,
                                                        m)
         st.raise_if()
         return exc.value, vxc
 
-    def eval_exc(self, P):
+    def eval_exc(self, P, Pz=None):
         P = self._mat(P)
         m, n = P.shape
         exc = ctypes.c_double(0.)
         st = N.Status()
-        N.host_lib().gauxc_integrator_eval_exc_rks(st.ptr, self.h, m, n, P.ctypes.data, m,
-                                                   ctypes.addressof(exc))
+        if Pz is not None:
+            Pz = self._mat(Pz)
+            N.host_lib().gauxc_integrator_eval_exc_uks(st.ptr, self.h, m, n, P.ctypes.data, m,
+                                                       Pz.ctypes.data, Pz.shape[0],
+                                                       ctypes.addressof(exc))
+        else:
+            N.host_lib().gauxc_integrator_eval_exc_rks(st.ptr, self.h, m, n, P.ctypes.data, m,
+                                                       ctypes.addressof(exc))
         st.raise_if()
         return exc.value
 
@@ -408,6 +424,29 @@ class DeviceContext:
             self.h, ctypes.addressof(exc), ctypes.addressof(nel),
             vxc.ctypes.data if want_vxc else None, self.nbf), "xcq_download_results")
         return exc.value, nel.value, vxc
+
+    def upload_density_z(self, Pz):
+        """UKS spin density P_z = P_a - P_b (P_s = P_a + P_b goes through
+        upload_density)."""
+        Pz = np.asfortranarray(Pz, dtype=np.float64)
+        N.xcq_check(self.lib.xcq_upload_density_z(self.h, Pz.shape[0], Pz.ctypes.data,
+                                                  Pz.shape[0]), "xcq_upload_density_z")
+
+    def exc_vxc_uks(self, func="PBE", want_vxc=True):
+        N.xcq_check(self.lib.xcq_exc_vxc_uks(self.h, N.XCQ_FUNC[func], int(bool(want_vxc))),
+                    "xcq_exc_vxc_uks")
+
+    def results_uks(self, want_vxc=True):
+        """(EXC, NEL, VXC_s, VXC_z) of the last UKS evaluation."""
+        exc = ctypes.c_double(0.)
+        nel = ctypes.c_double(0.)
+        vs = np.zeros((self.nbf, self.nbf), order="F") if want_vxc else None
+        vz = np.zeros((self.nbf, self.nbf), order="F") if want_vxc else None
+        N.xcq_check(self.lib.xcq_download_results_uks(
+            self.h, ctypes.addressof(exc), ctypes.addressof(nel),
+            vs.ctypes.data if want_vxc else None, vz.ctypes.data if want_vxc else None,
+            self.nbf), "xcq_download_results_uks")
+        return exc.value, nel.value, vs, vz
 
     def integrate_den(self):
         nel = ctypes.c_double(0.)

@@ -77,29 +77,42 @@ void xcd_interleave(std::vector<WorkItem>& v, size_t b, size_t e) {
   }
 }
 
-// Longest-processing-time-first packing of items [b, e) of v (costs c) into
-// nq queues.  Rewrites v[b, e) queue after queue (each queue in decreasing
-// cost order) and appends the nq + 1 queue offsets (relative to b) to qoff.
-void lpt_pack(std::vector<WorkItem>& v, size_t b, size_t e, const std::vector<double>& c,
-              int nq, std::vector<int32_t>& qoff) {
-  const size_t n = e - b;
-  std::vector<size_t> order(n);
-  std::iota(order.begin(), order.end(), size_t(0));
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return c[x] > c[y]; });
-  using QE = std::pair<double, int>;  // (load, queue)
+// Dynamic-queue layout of items [b, e) of v (item costs c): nx sub-lists
+// (one per XCD, so a task's point blocks or tiles share that XCD's L2),
+// filled task by task, largest predicted task cost first, each task going
+// to the sub-list with the least cost so far (longest-processing-time-first
+// across XCDs); inside a sub-list tasks keep that decreasing-cost order and
+// a task's items stay consecutive.  Rewrites v[b, e) sub-list after
+// sub-list and appends the nx + 1 sub-list offsets (relative to b) to qoff.
+void xcd_queue_layout(std::vector<WorkItem>& v, size_t b, size_t e, const std::vector<double>& c,
+                      int nx, std::vector<int32_t>& qoff) {
+  struct Group {
+    size_t first, count;
+    double cost;
+  };
+  std::vector<Group> groups;
+  for (size_t i = b; i < e; ++i) {
+    if (groups.empty() || v[i].task != v[groups.back().first].task)
+      groups.push_back(Group{i, 0, 0.0});
+    groups.back().count++;
+    groups.back().cost += c[i - b];
+  }
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const Group& x, const Group& y) { return x.cost > y.cost; });
+  using QE = std::pair<double, int>;  // (load, sub-list)
   std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
-  for (int q = 0; q < nq; ++q) heap.push({0.0, q});
-  std::vector<std::vector<WorkItem>> qs(nq);
-  for (size_t k : order) {
+  for (int q = 0; q < nx; ++q) heap.push({0.0, q});
+  std::vector<std::vector<WorkItem>> qs(nx);
+  for (const auto& g : groups) {
     QE top = heap.top();
     heap.pop();
-    qs[top.second].push_back(v[b + k]);
-    top.first += c[k];
+    for (size_t k = 0; k < g.count; ++k) qs[top.second].push_back(v[g.first + k]);
+    top.first += g.cost;
     heap.push(top);
   }
   size_t pos = b;
   qoff.push_back(0);
-  for (int q = 0; q < nq; ++q) {
+  for (int q = 0; q < nx; ++q) {
     for (const auto& w : qs[q]) v[pos++] = w;
     qoff.push_back((int32_t)(pos - b));
   }
@@ -150,6 +163,8 @@ struct xcq_context_s {
   std::vector<int32_t> h_qoff;
   WorkItem* d_work = nullptr;
   int32_t* d_qoff = nullptr;
+  unsigned long long* d_counters = nullptr;  // ticket counters of the dynamic queues
+  int64_t ncounters = 0;
   int32_t dens32_max_nbe = 48, syr2k32_max_nbe = 96;  // tile policy
   int32_t dens_hiocc_max_nbe = 0;  // 64-wide density tiles at three workgroups per CU
   int32_t dens128_min_nbe = 1 << 30;  // 128-wide (mixed) density tiles from this nbe on
@@ -169,6 +184,11 @@ struct xcq_context_s {
   bool pf32_valid = false;
   double* d_vxc = nullptr;
   double* d_red = nullptr;
+  double* d_uks = nullptr;  // UKS: [4][pblocks][256] density sums (s, z), Z coefficients (s, z)
+  double* d_Pz = nullptr;   // UKS: P_z = P_a - P_b (single molecule)
+  double* d_vxcz = nullptr; // UKS: VXC_z
+  bool have_pz = false;
+  bool last_uks = false;    // the last evaluation was a UKS one
   double* d_zc = nullptr;   // Z coefficients per point block (in-stage mode 3), lazily
   int64_t pblocks = 0;      // point blocks of the uploaded tasks
   double* d_out2 = nullptr;
@@ -210,8 +230,10 @@ struct xcq_context_s {
     dfree(d_lsh);
     dfree(d_work);
     dfree(d_qoff);
+    dfree(d_counters);
     dfree(d_red);
     dfree(d_zc);
+    dfree(d_uks);
     pblocks = 0;
     dfree(d_tmax);
     tmax_total = 0;
@@ -241,6 +263,8 @@ struct xcq_context_s {
     dfree(d_P);
     dfree(d_Pf32);
     dfree(d_vxc);
+    dfree(d_Pz);
+    dfree(d_vxcz);
     dfree(d_out2);
     dfree(d_corr);
     dfree(d_rpart);
@@ -253,7 +277,7 @@ namespace {
 CtxView view_of(xcq_context_s* c) {
   return CtxView{c->stream,  c->d_tasks, c->d_shells, c->d_shell_list, c->d_bfmap,
                  c->d_lsh,   c->d_px,    c->d_py,     c->d_pz,         c->d_w,
-                 c->d_work,  c->d_qoff,  c->timing ? c->d_tstamp : nullptr,
+                 c->d_work,  c->d_qoff,  c->d_counters, c->timing ? c->d_tstamp : nullptr,
                  c->d_P,     c->d_Pf32,  c->d_vxc,   c->d_red,        c->d_scratch,
                  c->vxc_drop, reinterpret_cast<unsigned*>(c->d_tmax),
                  c->max_l <= kInStageMaxL ? c->dens_in_stage : 0, c->d_shc, c->d_zc};
@@ -274,6 +298,7 @@ int build_work_lists(xcq_context_s* ctx) {
   auto& Q = ctx->h_qoff;
   W.clear();
   Q.clear();
+  int64_t ncounters = 0;
   const bool have_cost = ctx->cost_d.size() == (size_t)ctx->ntasks &&
                          ctx->cost_s.size() == (size_t)ctx->ntasks;
   for (auto& d : ctx->h_tasks) {
@@ -290,7 +315,12 @@ int build_work_lists(xcq_context_s* ctx) {
     if (r.n == 0) return;
     if (ctx->nqueues > 0) {
       r.nq = (int32_t)std::min<int64_t>(ctx->nqueues, r.n);
-      lpt_pack(W, (size_t)r.off, W.size(), cost, r.nq, Q);
+      // Workgroup b runs on XCD b % 8 (round-robin dispatch): eight
+      // sub-lists when every XCD gets a workgroup, else one shared list.
+      r.nx = r.nq >= (int32_t)kXcds ? (int32_t)kXcds : 1;
+      r.cnt_off = ncounters;
+      ncounters += r.nx;
+      xcd_queue_layout(W, (size_t)r.off, W.size(), cost, r.nx, Q);
     } else {
       xcd_interleave(W, (size_t)r.off, W.size());
     }
@@ -336,9 +366,12 @@ int build_work_lists(xcq_context_s* ctx) {
   dfree(ctx->d_work);
   dfree(ctx->d_qoff);
   dfree(ctx->d_tstamp);
+  dfree(ctx->d_counters);
   int rc;
   if ((rc = dalloc(&ctx->d_work, W.size()))) return rc;
   if ((rc = dalloc(&ctx->d_qoff, Q.size()))) return rc;
+  ctx->ncounters = ncounters;
+  if (ncounters && (rc = dalloc(&ctx->d_counters, (size_t)ncounters))) return rc;
   if (ctx->timing && (rc = dalloc(&ctx->d_tstamp, 2 * W.size()))) return rc;
   if (!W.empty())
     XCQ_HIP(hipMemcpy(ctx->d_work, W.data(), W.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
@@ -368,6 +401,12 @@ int enqueue_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc, 
   int rc;
   if (want_vxc && (rc = launch_fill(s, ctx->d_vxc, ctx->p_total))) return rc;
   if (want_vxc && (rc = launch_fill(s, ctx->d_tmax, ctx->tmax_total))) return rc;
+  // Ticket counters of the dynamic queues start at zero (a fill kernel, not
+  // a memset node, so graph replays reset them too; zero has the same bits
+  // as 0.0).
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
   int64_t launches = want_vxc ? 1 : 0, items = 0;
   auto rec = [&](hipEvent_t e) { return events ? hipEventRecord(e, s) : hipSuccess; };
   for (size_t ci = 0; ci < ctx->chunks.size(); ++ci) {
@@ -465,6 +504,7 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
     ctx->pf32_valid = true;
   }
   ctx->last_want_vxc = want_vxc;
+  ctx->last_uks = false;
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
   if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
@@ -658,6 +698,9 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
     dfree(ctx->d_P);
     dfree(ctx->d_Pf32);
     dfree(ctx->d_vxc);
+    dfree(ctx->d_Pz);
+    dfree(ctx->d_vxcz);
+    ctx->have_pz = false;
     dfree(ctx->d_out2);
     dfree(ctx->d_corr);
     dfree(ctx->d_rpart);
@@ -1158,7 +1201,9 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
     if ((rc = xcq_set_density_occupancy(ctx, best.h))) return rc;
     if ((rc = xcq_set_density_wide(ctx, best.w))) return rc;
   }
-  // Persistent queues (flop-heuristic packing) against one workgroup per item.
+  // Persistent workgroups on dynamic queues (at least as many as any stage
+  // kernel keeps resident per CU; surplus ones find the queues empty)
+  // against one workgroup per item.
   int dev_cus = 0;
   if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
       hipSuccess) {
@@ -1168,7 +1213,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
   const std::vector<double> keep_d = ctx->cost_d, keep_s = ctx->cost_s;
   int32_t best_q = 0;
   double bt = 1e300;
-  for (int32_t q : {0, 2 * dev_cus, 4 * dev_cus}) {
+  for (int32_t q : {0, 4 * dev_cus}) {
     if ((rc = xcq_set_schedule(ctx, q, keep_d.empty() ? nullptr : keep_d.data(),
                                keep_s.empty() ? nullptr : keep_s.data())))
       return rc;
@@ -1219,6 +1264,119 @@ int xcq_upload_density(xcq_context ctx, int32_t nbf, const double* P, int64_t ld
 int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc) {
   XCQ_REQUIRE(functional != 0, XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
   return run_pipeline(ctx, functional, precision, want_vxc ? 1 : 0);
+}
+
+// ---- UKS (GauXC's eval_exc_vxc(Ps, Pz)) --------------------------------------
+namespace {
+
+// One UKS evaluation on the context's stream (FP64, stored basis planes):
+// per chunk, collocation with gradient planes; the density stage twice (P_s,
+// then P_z; factor 1) leaving raw per-point sums; the polarised functional
+// and Z coefficients per point; Z_s into the chunk's Z plane and its
+// B^T Z + Z^T B into VXC_s; then Z_z into VXC_z.  The VXC drop screening is
+// off (every contribution is added, as GauXC does).
+int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  CtxView v = view_of(ctx);
+  v.in_stage = 0;
+  v.tmax = nullptr;
+  v.vxc_drop = 0.0;
+  hipStream_t s = ctx->stream;
+  const size_t blk = (size_t)256 * ctx->pblocks;
+  double* ds = ctx->d_uks;
+  double* dz = ds + blk;
+  double* zs = dz + blk;
+  double* zz = zs + blk;
+  int rc;
+  if (want_vxc && ((rc = launch_fill(s, ctx->d_vxc, ctx->p_total)) ||
+                   (rc = launch_fill(s, ctx->d_vxcz, ctx->p_total))))
+    return rc;
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
+  for (const Chunk& c : ctx->chunks) {
+    const WorkItem* cw = v.work + c.coll.off;
+    if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
+                                      ctx->max_l)))
+      return rc;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < 4; ++k)
+        if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
+                                 1.0, 0, pass == 0 ? ctx->d_P : ctx->d_Pz,
+                                 pass == 0 ? ds : dz)))
+          return rc;
+    if ((rc = launch_uks_points(v, c.coll, func, gga, ds, dz, zs, zz))) return rc;
+    if (!want_vxc) continue;
+    for (int comp = 0; comp < 2; ++comp) {
+      CtxView vc = v;
+      vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
+      if ((rc = launch_z_from_coeffs(vc, c.coll, gga, comp == 0 ? zs : zz))) return rc;
+      for (int k = 0; k < 2; ++k)
+        if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+    }
+  }
+  if (want_vxc) {
+    for (double* m : {ctx->d_vxc, ctx->d_vxcz})
+      if ((rc = launch_symmetrize(s, 1, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off, m)))
+        return rc;
+  }
+  return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
+                            ctx->d_out2);
+}
+
+}  // namespace
+
+int xcq_upload_density_z(xcq_context ctx, int32_t nbf, const double* Pz, int64_t ldp) {
+  XCQ_REQUIRE(ctx && Pz, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE, "UKS needs a single-molecule context");
+  const MolInfo& M = ctx->mols[0];
+  XCQ_REQUIRE(nbf == M.nbf && nbf > 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "density dimension does not match the basis");
+  XCQ_REQUIRE(ldp >= nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldp < nbf");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_Pz && (rc = dalloc(&ctx->d_Pz, (size_t)ctx->p_total))) return rc;
+  XCQ_HIP(hipMemcpy2DAsync(ctx->d_Pz, (size_t)nbf * 8, Pz, (size_t)ldp * 8, (size_t)nbf * 8, nbf,
+                           hipMemcpyHostToDevice, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_pz = true;
+  return XCQ_SUCCESS;
+}
+
+int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(functional == XCQ_FUNC_SVWN5 || functional == XCQ_FUNC_PBE ||
+                  functional == XCQ_FUNC_PBE0,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE, "UKS needs a single-molecule context");
+  XCQ_REQUIRE(ctx->mols[0].have_density && ctx->have_pz, XCQ_ERROR_STATE,
+              "UKS needs P_s (xcq_upload_density) and P_z (xcq_upload_density_z)");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
+    return rc;
+  if (!ctx->d_vxcz && (rc = dalloc(&ctx->d_vxcz, (size_t)ctx->p_total))) return rc;
+  ctx->last_want_vxc = want_vxc ? 1 : 0;
+  ctx->last_vxc_eval = want_vxc ? 1 : 0;
+  ctx->last_graph = false;
+  ctx->last_uks = true;
+  return enqueue_uks(ctx, functional, want_vxc ? 1 : 0);
+}
+
+int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* vxc_s,
+                             double* vxc_z, int64_t ldvxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->last_uks, XCQ_ERROR_STATE, "the last evaluation was not a UKS one");
+  XCQ_REQUIRE(!vxc_z || vxc_s, XCQ_ERROR_INVALID_ARGUMENT, "vxc_z needs vxc_s");
+  int rc = xcq_download_results_mol(ctx, 0, exc, nel, vxc_s, ldvxc);
+  if (rc || !vxc_z) return rc;
+  const MolInfo& M = ctx->mols[0];
+  XCQ_HIP(hipMemcpy2DAsync(vxc_z, (size_t)ldvxc * 8, ctx->d_vxcz, (size_t)M.nbf * 8,
+                           (size_t)M.nbf * 8, M.nbf, hipMemcpyDeviceToHost, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  return XCQ_SUCCESS;
 }
 
 int xcq_integrate_den(xcq_context ctx, double* nel) {
@@ -1368,7 +1526,8 @@ int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t 
     cleanup();
     return XCQ_ERROR_HIP;
   }
-  rc = launch_ssf(nullptr, natoms, d_xyz, d_rab, d_dn, npts, d_x, d_y, d_z, d_par, d_w);
+  rc = launch_ssf(nullptr, natoms, d_xyz, d_rab, d_dn, npts, d_x, d_y, d_z, 1, d_par, d_w,
+                  nullptr);
   if (rc == XCQ_SUCCESS) {
     e = hipMemcpy(weights, d_w, 8 * (size_t)npts, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
@@ -1419,7 +1578,7 @@ int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xy
     e = hipMemcpy(d_par, parent.data(), 4 * parent.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     rc = launch_ssf(ctx->stream, natoms, d_xyz, d_rab, d_dn, ctx->npts_total, ctx->d_px,
-                    ctx->d_py, ctx->d_pz, d_par, ctx->d_w);
+                    ctx->d_py, ctx->d_pz, 1, d_par, ctx->d_w, nullptr);
     if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
   }
   dfree(d_xyz);

@@ -609,7 +609,7 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa,
-    double* __restrict__ zbuf) {
+    double* __restrict__ zbuf, double* __restrict__ dens_out) {
   constexpr int BI = NT / 32;
   constexpr int kPtD = 2 * 16 * (NT + 4), kStageD = kPtD + 2 * 16 * kPad64;
   static_assert(INSTAGE == 0 || NT * kXLd <= kStageD, "X^T tile must fit the stages");
@@ -739,13 +739,29 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
       for (int c = 0; c < 4; ++c) dens[wave][lane][c] = part[c];
       __syncthreads();
     }
-    finish_points<INSTAGE != 0 ? 4 : 2>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
-    if (want_z) {
-      __syncthreads();
-      if constexpr (INSTAGE != 0)
-        zbuf[(t.red_off + wi.a) * 256 + tid] = (&zc[0][0])[tid];
-      else
-        write_z<double, GGA>(t, m0, tid, zc, Bm, smax, tmax ? tmax + 2 * t.tmax_off : nullptr);
+    if (dens_out) {
+      // One spin component of a UKS evaluation: the block's raw sums
+      // (sum B X, sum dB/dx X, ...) per point, [block][4][64]; the
+      // polarised functional runs once both components are in.
+      constexpr int ND = INSTAGE != 0 ? 4 : 2;
+      if (tid < 64) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = 0.;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) v += dens[d][tid][c];
+          dens_out[(t.red_off + wi.a) * 256 + c * 64 + tid] = v;
+        }
+      }
+    } else {
+      finish_points<INSTAGE != 0 ? 4 : 2>(t, wi.a, tid, dens, zc, weights, func, GGA, red);
+      if (want_z) {
+        __syncthreads();
+        if constexpr (INSTAGE != 0)
+          zbuf[(t.red_off + wi.a) * 256 + tid] = (&zc[0][0])[tid];
+        else
+          write_z<double, GGA>(t, m0, tid, zc, Bm, smax, tmax ? tmax + 2 * t.tmax_off : nullptr);
+      }
     }
     stamp_item(qa, it, t0);
     __syncthreads();
@@ -1024,8 +1040,133 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
   }
 }
 
+// ---------------------------------------------------------------------------
+// UKS tail (GauXC's generic host driver with spin_dim 2,
+// reference_replicated_xc_host_integrator_exc_vxc.hpp:387-547, and the
+// device UKS kernels it mirrors, cuda/kernels/zmat_vxc.cu:62-96,197-274).
+// Inputs per point block: the raw sums of the P_s and P_z density passes
+// (sum B X, sum dB X; X = B P with factor 1).  One thread per point:
+//   rho_+- = (rho_s +- rho_z) / 2, grad n = 2 sum dB X_s, grad m = 2 sum dB X_z,
+//   sigma_{aa, ab, bb} = (|gn|^2 + |gm|^2)/4 +- gn.gm/2, (|gn|^2 - |gm|^2)/4
+//   (eval_uvvar_gga_uks, reference_local_host_work_driver.cxx:270-330);
+// the polarised functional; weight folding; the block's (w n eps, w n) sums;
+// and the Z coefficients of eval_zmat_{lda,gga}_vxc_uks (:607-632, 715-770):
+//   Z_s = (v_a + v_b)/4 B + c_s . grad B,  Z_z = (v_a - v_b)/4 B + c_z . grad B,
+//   c_s = g1 gn + g2 gm, c_z = g3 gm + g2 gn, with g1 = (v_aa + v_ab + v_bb)/2,
+//   g2 = (v_aa - v_bb)/2, g3 = (v_aa - v_ab + v_bb)/2 (weights folded in).
+__global__ __launch_bounds__(64) void k_uks_points(const WorkItem* __restrict__ items,
+                                                   const DevTask* __restrict__ tasks,
+                                                   const double* __restrict__ weights, int func,
+                                                   int gga, const double* __restrict__ ds,
+                                                   const double* __restrict__ dz,
+                                                   double* __restrict__ red,
+                                                   double* __restrict__ zs,
+                                                   double* __restrict__ zz) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  const int64_t b = t.red_off + wi.a;
+  const int p = wi.a * kPtBlock + tid;
+  double sv[4], zv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sv[c] = ds[b * 256 + c * 64 + tid];
+    zv[c] = dz[b * 256 + c * 64 + tid];
+  }
+  const double ra = 0.5 * (sv[0] + zv[0]), rb = 0.5 * (sv[0] - zv[0]);
+  double gn[3], gm[3], nn = 0., mm = 0., nm = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    gn[d] = 2.0 * sv[1 + d];
+    gm[d] = 2.0 * zv[1 + d];
+    nn += gn[d] * gn[d];
+    mm += gm[d] * gm[d];
+    nm += gn[d] * gm[d];
+  }
+  double e = 0., n = 0., as = 0., az = 0., cs[3] = {0., 0., 0.}, cz[3] = {0., 0., 0.};
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    n = w * (ra + rb);
+    if (func != 0) {
+      const double saa = gga ? 0.25 * (nn + mm) + 0.5 * nm : 0.;
+      const double sab = gga ? 0.25 * (nn - mm) : 0.;
+      const double sbb = gga ? 0.25 * (nn + mm) - 0.5 * nm : 0.;
+      const XCOutPol o = dev_eval_functional_pol(func, ra, rb, saa, sab, sbb);
+      e = w * o.e;
+      const double fp = 0.5 * w * o.vra, fm = 0.5 * w * o.vrb;
+      as = 0.5 * (fp + fm);
+      az = 0.5 * (fp - fm);
+      if (gga) {
+        const double vpp = w * o.vsaa, vpm = w * o.vsab, vmm = w * o.vsbb;
+        const double g1 = 0.5 * (vpp + vpm + vmm), g2 = 0.5 * (vpp - vmm),
+                     g3 = 0.5 * (vpp - vpm + vmm);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          cs[d] = g1 * gn[d] + g2 * gm[d];
+          cz[d] = g3 * gm[d] + g2 * gn[d];
+        }
+      }
+    }
+  }
+  zs[b * 256 + tid] = as;
+  zz[b * 256 + tid] = az;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    zs[b * 256 + (1 + d) * 64 + tid] = cs[d];
+    zz[b * 256 + (1 + d) * 64 + tid] = cz[d];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (tid == 0) {
+    red[2 * b] = e;
+    red[2 * b + 1] = n;
+  }
+}
+
+// Z = a B + c . grad B of one point block from stored coefficients
+// ([block][4][64]) and the stored FP64 planes (no screening bounds).
+template <bool GGA>
+__global__ __launch_bounds__(256) void k_z_from_coeffs(const WorkItem* __restrict__ items,
+                                                       const DevTask* __restrict__ tasks,
+                                                       double* __restrict__ scratch,
+                                                       const double* __restrict__ coef) {
+  __shared__ double zc[4][64];
+  __shared__ unsigned smax[kGranLds][2];
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  (&zc[0][0])[tid] = coef[(t.red_off + wi.a) * 256 + tid];
+  __syncthreads();
+  write_z<double, GGA>(t, wi.a * kPtBlock, tid, zc, scratch + t.b_off, smax, nullptr);
+}
+
+int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* ds,
+                      const double* dz, double* zs, double* zz) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_uks_points, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
+                     v.tasks, v.w, func, gga, ds, dz, v.red, zs, zz);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  double* scr = static_cast<double*>(v.scratch);
+  if (gga)
+    hipLaunchKernelGGL(k_z_from_coeffs<true>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
+                       v.work + r.off, v.tasks, scr, coef);
+  else
+    hipLaunchKernelGGL(k_z_from_coeffs<false>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
+                       v.work + r.off, v.tasks, scr, coef);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
 int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
-                   int func, int gga, double fac, int want_z) {
+                   int func, int gga, double fac, int want_z, const double* P_override,
+                   double* dens_out) {
   if (r.n == 0) return XCQ_SUCCESS;
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
@@ -1035,9 +1176,10 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
   const ShellArgs sa{v.shells, v.shc, v.shell_list, v.lsh, v.px, v.py, v.pz};
   if (precision == XCQ_PREC_FP64) {
     double* scr = static_cast<double*>(v.scratch);
+    const double* P = P_override ? P_override : v.P;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, v.P, scr, v.w,
-                         func, fac, want_z, v.red, tm, sa, v.zc);
+      hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, P, scr, v.w,
+                         func, fac, want_z, v.red, tm, sa, v.zc, dens_out);
     };
     // Any in-stage mode runs the FP64 kernel's one (mode 3).
     if (v.in_stage != 0) {

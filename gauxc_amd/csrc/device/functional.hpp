@@ -1,4 +1,5 @@
-// Device evaluation of the unpolarised functionals on the quadrature path,
+// Device evaluation of the functionals on the quadrature path (unpolarised,
+// and spin-polarised for UKS below),
 // in FP64.  Outputs follow the ExchCXX/libxc convention GauXC consumes
 // (host_integrator_exc_vxc.hpp:445-450): energy per particle eps,
 // vrho = d(rho eps)/d rho, vsigma = d(rho eps)/d sigma.
@@ -134,6 +135,182 @@ __device__ __forceinline__ XCOut dev_eval_functional(int func, double rho, doubl
     o.e += e;
     o.vr += vr;
     o.vs += vs;
+  }
+  return o;
+}
+
+// ---- spin-polarised forms (UKS) -------------------------------------------
+// Inputs rho_a, rho_b and sigma_aa, sigma_ab, sigma_bb; outputs e = n eps and
+// the five derivatives (ExchCXX's polarised layout: vrho (a, b), vsigma
+// (aa, ab, bb)).
+//  * Exchange by spin scaling (Oliver-Perdew 1979):
+//    e_x[ra, rb] = (e_x[2 ra, 4 saa] + e_x[2 rb, 4 sbb]) / 2, so
+//    vrho_a = vrho(2 ra, 4 saa), vsigma_aa = 2 vsigma(2 ra, 4 saa), vsigma_ab = 0.
+//  * Correlation in (n, zeta, sigma_tot = saa + 2 sab + sbb), differentiated
+//    by forward-mode dual numbers in those three variables, then
+//    d/d rho_a = d/dn + (1 - zeta)/n d/dzeta, d/d rho_b = d/dn - (1 + zeta)/n d/dzeta,
+//    d/d sigma_aa = d/d sigma_bb = d/d sigma_tot, d/d sigma_ab = 2 d/d sigma_tot.
+//    VWN(RPA): eps = eps_P + (eps_F - eps_P) f(zeta) with the paramagnetic and
+//    ferromagnetic RPA fits (ferromagnetic A=0.01554535, b=20.1231,
+//    c=101.578, x0=-0.743294), the interpolation of libxc's LDA_C_VWN_RPA.
+//    PBE: PW92 with spin stiffness (modified parameters of libxc's GGA_C_PBE:
+//    ferromagnetic a=0.01554535, alpha1=0.20548, beta=14.1189, 6.1977,
+//    3.3662, 0.62517; -alpha_c a=0.0168869, alpha1=0.11125, beta=10.357,
+//    3.6231, 0.88026, 0.49671; f''(0)=1.709920934161365617563962776245) and
+//    phi(zeta) = ((1+zeta)^(2/3) + (1-zeta)^(2/3)) / 2 in H.
+// Thresholds: no exchange from a channel with 2 rho_s <= 1e-15; no
+// correlation for n <= 1e-15 (SVWN5) or 1e-12 (PBE).
+struct XCOutPol {
+  double e, vra, vrb, vsaa, vsab, vsbb;
+};
+
+struct D3 {
+  double v, d0, d1, d2;
+};
+__device__ __forceinline__ D3 d3c(double v) { return D3{v, 0., 0., 0.}; }
+__device__ __forceinline__ D3 d3ch(const D3& a, double f, double df) {
+  return D3{f, df * a.d0, df * a.d1, df * a.d2};
+}
+__device__ __forceinline__ D3 operator+(const D3& a, const D3& b) {
+  return D3{a.v + b.v, a.d0 + b.d0, a.d1 + b.d1, a.d2 + b.d2};
+}
+__device__ __forceinline__ D3 operator-(const D3& a, const D3& b) {
+  return D3{a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2};
+}
+__device__ __forceinline__ D3 operator*(const D3& a, const D3& b) {
+  return D3{a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1,
+            a.d2 * b.v + a.v * b.d2};
+}
+__device__ __forceinline__ D3 operator/(const D3& a, const D3& b) {
+  const double i = 1.0 / b.v, q = a.v * i;
+  return D3{q, (a.d0 - q * b.d0) * i, (a.d1 - q * b.d1) * i, (a.d2 - q * b.d2) * i};
+}
+__device__ __forceinline__ D3 operator+(const D3& a, double b) { return D3{a.v + b, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator+(double a, const D3& b) { return b + a; }
+__device__ __forceinline__ D3 operator-(const D3& a, double b) { return D3{a.v - b, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator-(double a, const D3& b) {
+  return D3{a - b.v, -b.d0, -b.d1, -b.d2};
+}
+__device__ __forceinline__ D3 operator*(const D3& a, double b) {
+  return D3{a.v * b, a.d0 * b, a.d1 * b, a.d2 * b};
+}
+__device__ __forceinline__ D3 operator*(double a, const D3& b) { return b * a; }
+__device__ __forceinline__ D3 operator/(const D3& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ D3 operator/(double a, const D3& b) { return d3c(a) / b; }
+__device__ __forceinline__ D3 d3sqrt(const D3& a) {
+  const double s = sqrt(a.v);
+  return d3ch(a, s, 0.5 / s);
+}
+__device__ __forceinline__ D3 d3cbrt(const D3& a) {
+  const double c = cbrt(a.v);
+  return d3ch(a, c, 1.0 / (3.0 * c * c));
+}
+__device__ __forceinline__ D3 d3pow(const D3& a, double p) {
+  const double x = pow(a.v, p);
+  return d3ch(a, x, p * pow(a.v, p - 1.0));
+}
+__device__ __forceinline__ D3 d3log(const D3& a) { return d3ch(a, log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ D3 d3log1p(const D3& a) { return d3ch(a, log1p(a.v), 1.0 / (1.0 + a.v)); }
+__device__ __forceinline__ D3 d3exp(const D3& a) {
+  const double x = exp(a.v);
+  return d3ch(a, x, x);
+}
+__device__ __forceinline__ D3 d3atan(const D3& a) { return d3ch(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+
+__device__ __forceinline__ D3 d3_vwn_fit(const D3& x, double A, double b, double c, double x0) {
+  const D3 X = x * x + b * x + c;
+  const double X0 = x0 * x0 + b * x0 + c, Q = sqrt(4.0 * c - b * b);
+  const D3 at = d3atan(Q / (2.0 * x + b));
+  return A * (d3log(x * x / X) + (2.0 * b / Q) * at -
+              (b * x0 / X0) * (d3log((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
+}
+__device__ __forceinline__ D3 d3_pw92(const D3& rs, double a, double a1, double b1, double b2,
+                                      double b3, double b4) {
+  const D3 sr = d3sqrt(rs);
+  const D3 q = 2.0 * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
+  return -2.0 * a * (1.0 + a1 * rs) * d3log1p(1.0 / q);
+}
+__device__ __forceinline__ D3 d3_fzeta(const D3& z) {
+  return (d3pow(1.0 + z, 4.0 / 3.0) + d3pow(1.0 - z, 4.0 / 3.0) - 2.0) *
+         (1.0 / 0.51984209978974632953);  // 2^(4/3) - 2
+}
+
+// Correlation energy density n eps_c(n, zeta, sigma_tot) (sigma_tot unused
+// for SVWN5).
+__device__ __forceinline__ D3 d3_corr(int func, const D3& n, const D3& z, const D3& st) {
+  const double pi = 3.14159265358979323846;
+  const D3 rs = d3cbrt(3.0 / (4.0 * pi * n));
+  if (func == XCQ_FUNC_SVWN5) {
+    const D3 x = d3sqrt(rs);
+    const D3 ep = d3_vwn_fit(x, 0.0310907, 13.0720, 42.7198, -0.409286);
+    const D3 ef = d3_vwn_fit(x, 0.01554535, 20.1231, 101.578, -0.743294);
+    return n * (ep + (ef - ep) * d3_fzeta(z));
+  }
+  const D3 ep = d3_pw92(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const D3 ef = d3_pw92(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const D3 mac = d3_pw92(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const double fz20 = 1.709920934161365617563962776245;
+  const D3 f = d3_fzeta(z), z2 = z * z, z4 = z2 * z2;
+  const D3 ec = ep - mac * f * (1.0 / fz20) * (1.0 - z4) + (ef - ep) * f * z4;
+  const double beta = 0.06672455060314922, gamma = 0.0310906908696549;
+  const D3 phi = 0.5 * (d3pow(1.0 + z, 2.0 / 3.0) + d3pow(1.0 - z, 2.0 / 3.0));
+  const D3 phi3 = phi * phi * phi;
+  const double D = 0.0634682060977037;  // pi / (16 (3 pi^2)^(1/3))
+  const D3 t2 = D * st / (phi * phi * n * n * d3cbrt(n));
+  const D3 A = (beta / gamma) / (d3exp(-1.0 * ec / (gamma * phi3)) - 1.0);
+  const D3 At2 = A * t2;
+  const D3 H = gamma * phi3 * d3log1p((beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return n * (ec + H);
+}
+
+__device__ __forceinline__ XCOutPol dev_eval_functional_pol(int func, double ra, double rb,
+                                                            double saa, double sab,
+                                                            double sbb) {
+  XCOutPol o{0., 0., 0., 0., 0., 0.};
+  ra = fmax(ra, 0.0);
+  rb = fmax(rb, 0.0);
+  const bool gga = func != XCQ_FUNC_SVWN5;
+  if (gga) {
+    const double stol2 = 2.1544346900318838e-43 * 2.1544346900318838e-43;
+    saa = fmax(saa, stol2);
+    sbb = fmax(sbb, stol2);
+    const double hs = 0.5 * (saa + sbb);
+    sab = fmax(fmin(sab, hs), -hs);
+  }
+  const double xs = func == XCQ_FUNC_PBE0 ? 0.75 : 1.0;
+  double e, vr, vs;
+  if (2.0 * ra > 1e-15) {
+    if (gga) {
+      dev_pbe_x(2.0 * ra, 4.0 * saa, e, vr, vs);
+      o.vsaa += 2.0 * xs * vs;
+    } else {
+      dev_slater(2.0 * ra, e, vr);
+    }
+    o.e += 0.5 * xs * e;
+    o.vra += xs * vr;
+  }
+  if (2.0 * rb > 1e-15) {
+    if (gga) {
+      dev_pbe_x(2.0 * rb, 4.0 * sbb, e, vr, vs);
+      o.vsbb += 2.0 * xs * vs;
+    } else {
+      dev_slater(2.0 * rb, e, vr);
+    }
+    o.e += 0.5 * xs * e;
+    o.vrb += xs * vr;
+  }
+  const double nt = ra + rb;
+  if (nt <= (gga ? 1e-12 : 1e-15)) return o;
+  const double zt = (ra - rb) / nt;
+  const D3 c = d3_corr(func, D3{nt, 1., 0., 0.}, D3{zt, 0., 1., 0.},
+                       D3{saa + 2.0 * sab + sbb, 0., 0., 1.});
+  o.e += c.v;
+  o.vra += c.d0 + (1.0 - zt) / nt * c.d1;
+  o.vrb += c.d0 - (1.0 + zt) / nt * c.d1;
+  if (gga) {
+    o.vsaa += c.d2;
+    o.vsab += 2.0 * c.d2;
+    o.vsbb += c.d2;
   }
   return o;
 }

@@ -91,20 +91,27 @@ struct WorkItem {  // (task, a, b): a/b are point-block or tile indices
 };
 
 // A launchable work list: items [off, off + n) of the context's work array.
-// nq > 0 runs it as a persistent launch of nq workgroups, workgroup q taking
-// items [qoff[q], qoff[q+1]) of the list in order (queue table at qoff_off
-// of the context's queue array, nq + 1 entries).
+// nq > 0 runs it as a persistent launch of nq workgroups drawing items from
+// dynamic queues: the list is split into nx sub-lists (nx = 8, one per XCD,
+// when nq >= 8; else 1), sub-list x = items [qoff[x], qoff[x+1]) (queue
+// table at qoff_off of the context's queue array, nx + 1 entries), and a
+// workgroup b takes the next item of sub-list b % nx from that sub-list's
+// ticket counter (counters at cnt_off of the context's counter array, zeroed
+// at the start of every evaluation) until the sub-list is exhausted.
 struct WorkRange {
   int64_t off = 0, n = 0;
   int64_t qoff_off = 0;
-  int32_t nq = 0, pad_ = 0;
+  int32_t nq = 0, nx = 0;
+  int64_t cnt_off = 0;
 };
 
 // Launch arguments common to every queued stage kernel.
 struct QueueArgs {
   const WorkItem* items;  // first item of the list
-  const int32_t* qoff;    // queue table (nullptr: one workgroup per item)
+  const int32_t* qoff;    // sub-list table (nullptr: one workgroup per item)
   int64_t* tstamp;        // per-item (start, end) wall clock, or nullptr
+  unsigned long long* counters;  // ticket counter per sub-list
+  int32_t nx;             // sub-lists
 };
 
 struct Chunk {
@@ -125,6 +132,7 @@ struct CtxView {
   const double *px, *py, *pz, *w;
   const WorkItem* work;
   const int32_t* qoff;
+  unsigned long long* counters;  // ticket counters of the dynamic queues
   int64_t* tstamp;  // nullptr unless task timing is on
   const double* P;
   const float* Pf32;  // FP32 copy of P (mixed-precision density stage)
@@ -155,16 +163,38 @@ constexpr int kGranLds = 64;   // granules a density workgroup reduces in LDS (n
 
 inline QueueArgs queue_args(const CtxView& v, const WorkRange& r) {
   return QueueArgs{v.work + r.off, r.nq ? v.qoff + r.qoff_off : nullptr,
-                   v.tstamp ? v.tstamp + 2 * r.off : nullptr};
+                   v.tstamp ? v.tstamp + 2 * r.off : nullptr,
+                   r.nq ? v.counters + r.cnt_off : nullptr, r.nx};
 }
 inline unsigned grid_of(const WorkRange& r) { return (unsigned)(r.nq ? r.nq : r.n); }
 
-// Item loop of a queued stage kernel: one item per workgroup, or the
-// workgroup's queue.  The body must end with all threads done with LDS.
-#define XCQ_FOR_EACH_ITEM(qa, it_var)                                              \
-  for (int64_t it_var = (qa).qoff ? (qa).qoff[blockIdx.x] : (int64_t)blockIdx.x,  \
-               it_end_ = (qa).qoff ? (qa).qoff[blockIdx.x + 1] : it_var + 1;       \
-       it_var < it_end_; ++it_var)
+constexpr int64_t kNoItem = INT64_MAX;
+
+// Next item of this workgroup's dynamic queue, or kNoItem.  Every thread of
+// the workgroup calls it (uniform control flow): one lane takes a ticket
+// with a vector atomic and the item index is broadcast through LDS.  The
+// leading barrier keeps the previous item's readers of the slot ahead of
+// the write.
+__device__ __forceinline__ int64_t next_ticket(const QueueArgs& qa) {
+  __shared__ int64_t slot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int x = (int)(blockIdx.x % (unsigned)qa.nx);
+    const unsigned long long t = atomicAdd(qa.counters + x, 1ull);
+    const int64_t n = qa.qoff[x + 1] - qa.qoff[x];
+    slot = (int64_t)t < n ? qa.qoff[x] + (int64_t)t : kNoItem;
+  }
+  __syncthreads();
+  return slot;
+}
+
+// Item loop of a queued stage kernel: one item per workgroup, or items from
+// the workgroup's dynamic queue.  The body must end with all threads done
+// with LDS.
+#define XCQ_FOR_EACH_ITEM(qa, it_var)                                                  \
+  for (int64_t it_var = (qa).qoff ? next_ticket(qa) : (int64_t)blockIdx.x,            \
+               it_end_ = (qa).qoff ? kNoItem : it_var + 1;                             \
+       it_var < it_end_; it_var = (qa).qoff ? next_ticket(qa) : it_var + 1)
 
 // True when every element of the B^T Z + Z^T B tile pair (rows i0.., columns
 // j0.., nt wide) of task t is bounded below `drop` by the granule maxima the
@@ -199,8 +229,11 @@ __device__ __forceinline__ void stamp_item(const QueueArgs& qa, int64_t it, int6
 // ---- kernel launchers (defined in the .hip files) ---------------------------
 int upload_sph_table(hipStream_t s);
 int launch_ssf(hipStream_t s, int32_t natoms, const double* atom_xyz, const double* rab,
-               const double* dist_nearest, int64_t npts, const double* px, const double* py,
-               const double* pz, const int32_t* parent, double* w);
+               const double* dnear, int64_t npts, const double* px, const double* py,
+               const double* pz, int64_t stride, const int32_t* parent, double* w,
+               int32_t* bad_parent);
+int launch_molmeta(hipStream_t s, int32_t natoms, const double* xyz, double* rab, double* dnear,
+                   int32_t* bad_parent);
 int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,
                             const DevTask* tasks, const DevShell* shells,
                             const int32_t* shell_list, const int32_t* lsh_first, const double* px,
@@ -223,7 +256,13 @@ int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const Dev
 // whose tasks use `nt`-function tiles (hiocc: the three-workgroups-per-CU
 // configuration of the mixed kernel).
 int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
-                   int func, int gga, double fac, int want_z);
+                   int func, int gga, double fac, int want_z,
+                   const double* P_override = nullptr, double* dens_out = nullptr);
+// UKS: polarised functional and Z coefficients per point block from the two
+// density passes' raw sums; Z planes from stored coefficients.
+int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* ds,
+                      const double* dz, double* zs, double* zz);
+int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);
 int launch_P_to_f32(hipStream_t s, int64_t n, const double* P, float* out);

@@ -36,13 +36,19 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
                                              const double* __restrict__ px,
                                              const double* __restrict__ py,
                                              const double* __restrict__ pz,
+                                             int64_t stride,
                                              const int32_t* __restrict__ parent,
-                                             double* __restrict__ w) {
+                                             double* __restrict__ w,
+                                             int32_t* __restrict__ bad_parent) {
   const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= npts) return;
   constexpr double a = 0.64;
-  const double x = px[ip], y = py[ip], z = pz[ip];
+  const double x = px[ip * stride], y = py[ip * stride], z = pz[ip * stride];
   const int par = parent[ip];
+  if (par < 0 || par >= natoms) {  // callers that cannot validate on the host
+    if (bad_parent) *bad_parent = 1;
+    return;
+  }
   auto dist = [&](int A) {
     const double dx = x - xyz[3 * A], dy = y - xyz[3 * A + 1], dz = z - xyz[3 * A + 2];
     return sqrt(dx * dx + dy * dy + dz * dz);
@@ -81,11 +87,45 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
 
 int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* rab,
                const double* dnear, int64_t npts, const double* px, const double* py,
-               const double* pz, const int32_t* parent, double* w) {
+               const double* pz, int64_t stride, const int32_t* parent, double* w,
+               int32_t* bad_parent) {
   if (npts == 0) return XCQ_SUCCESS;
   const int64_t nb = (npts + 255) / 256;
   hipLaunchKernelGGL(k_ssf, dim3((unsigned)nb), dim3(256), 0, s, natoms, xyz, rab, dnear, npts,
-                     px, py, pz, parent, w);
+                     px, py, pz, stride, parent, w, bad_parent);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+// MolMeta on the device (GauXC src/molmeta.cxx:28-60): R_AB (column-major,
+// leading dimension natoms) and each atom's nearest-neighbour distance, one
+// thread per atom.  Products and sums are rounded separately (no fused
+// multiply-add), as the host computes them.  Thread 0 also clears the
+// bad-parent flag of the SSF launch that follows on the same stream.
+__global__ __launch_bounds__(256) void k_molmeta(int32_t natoms, const double* __restrict__ xyz,
+                                                 double* __restrict__ rab,
+                                                 double* __restrict__ dnear,
+                                                 int32_t* __restrict__ bad_parent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && bad_parent) *bad_parent = 0;
+  if (i >= natoms) return;
+  double dmin = INFINITY;
+  for (int j = 0; j < natoms; ++j) {
+    const double dx = xyz[3 * i] - xyz[3 * j];
+    const double dy = xyz[3 * i + 1] - xyz[3 * j + 1];
+    const double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                    __dmul_rn(dz, dz)));
+    rab[(size_t)i + (size_t)j * natoms] = r;
+    if (j != i) dmin = fmin(dmin, r);
+  }
+  dnear[i] = dmin;
+}
+
+int launch_molmeta(hipStream_t s, int32_t natoms, const double* xyz, double* rab, double* dnear,
+                   int32_t* bad_parent) {
+  hipLaunchKernelGGL(k_molmeta, dim3((unsigned)((natoms + 255) / 256)), dim3(256), 0, s, natoms,
+                     xyz, rab, dnear, bad_parent);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -471,6 +471,29 @@ void gauxc_integrator_eval_exc_vxc_rks(GauXCStatus* st, const GauXCIntegrator in
   });
 }
 
+void gauxc_integrator_eval_exc_uks(GauXCStatus* st, const GauXCIntegrator in, const int64_t m,
+                                   const int64_t n, const double* Ps, const int64_t ldps,
+                                   const double* Pz, const int64_t ldpz, double* exc) {
+  guarded_void(st, [&] {
+    if (!Ps || !Pz || !exc) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_exc_uks(m, n, Ps, ldps, Pz, ldpz, exc, IntegratorSettingsXC{});
+  });
+}
+
+void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* st, const GauXCIntegrator in,
+                                       const int64_t m, const int64_t n, const double* Ps,
+                                       const int64_t ldps, const double* Pz, const int64_t ldpz,
+                                       double* exc, double* vxc_s, const int64_t ldvxc_s,
+                                       double* vxc_z, const int64_t ldvxc_z) {
+  guarded_void(st, [&] {
+    if (!Ps || !Pz || !exc || !vxc_s || !vxc_z) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_exc_vxc_uks(m, n, Ps, ldps, Pz, ldpz, vxc_s, ldvxc_s, vxc_z, ldvxc_z,
+                                     exc, IntegratorSettingsXC{});
+  });
+}
+
 void* gauxc_integrator_device_context(GauXCStatus* st, const GauXCIntegrator in) {
   return guarded(st, [&]() -> void* {
     auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");

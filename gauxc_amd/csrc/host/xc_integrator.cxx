@@ -150,8 +150,21 @@ struct DeviceXCIntegratorImpl::State {
     if (have_vxc && ldvxc < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDVXC");
     if (!lb->state().modified_weights_are_stored)
       GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
+  }
+  void validate_rks(int64_t m, int64_t n, int64_t ldp, int64_t ldvxc, bool have_vxc) {
+    validate(m, n, ldp, ldvxc, have_vxc);
     if (func->is_polarized())
-      GAUXC_GENERIC_EXCEPTION("Only restricted (RKS) evaluation is provided");
+      GAUXC_GENERIC_EXCEPTION("RKS evaluation needs an unpolarized functional");
+  }
+  void validate_uks(int64_t m, int64_t n, int64_t ldps, int64_t ldpz, int64_t ldvs, int64_t ldvz,
+                    bool have_vxc) {
+    validate(m, n, ldps, ldvs, have_vxc);
+    if (ldpz < lb->basis().nbf()) GAUXC_GENERIC_EXCEPTION("Invalid LDPZ");
+    if (have_vxc && ldvz < lb->basis().nbf()) GAUXC_GENERIC_EXCEPTION("Invalid LDVXCZ");
+    if (!func->is_polarized())
+      GAUXC_GENERIC_EXCEPTION("UKS evaluation needs a polarized functional");
+    if (prec != DevicePrecision::FP64)
+      GAUXC_GENERIC_EXCEPTION("UKS evaluation is provided in FP64 (local work driver Default)");
   }
 };
 
@@ -183,7 +196,7 @@ void DeviceXCIntegratorImpl::eval_exc_vxc(int64_t m, int64_t n, const double* P,
                                           double* VXC, int64_t ldvxc, double* EXC,
                                           const IntegratorSettingsXC&) {
   auto& s = *s_;
-  s.validate(m, n, ldp, ldvxc, true);
+  s.validate_rks(m, n, ldp, ldvxc, true);
   if (!P || !VXC || !EXC) GAUXC_GENERIC_EXCEPTION("null pointer argument");
   const auto t0 = std::chrono::steady_clock::now();
   s.ensure_static_data();
@@ -203,12 +216,59 @@ void DeviceXCIntegratorImpl::eval_exc_vxc(int64_t m, int64_t n, const double* P,
 void DeviceXCIntegratorImpl::eval_exc(int64_t m, int64_t n, const double* P, int64_t ldp,
                                       double* EXC, const IntegratorSettingsXC&) {
   auto& s = *s_;
-  s.validate(m, n, ldp, 0, false);
+  s.validate_rks(m, n, ldp, 0, false);
   if (!P || !EXC) GAUXC_GENERIC_EXCEPTION("null pointer argument");
   s.ensure_static_data();
   check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), P, ldp), "xcq_upload_density");
   check(xcq_exc_vxc(s.ctx, xcq_func_id(*s.func), static_cast<int>(s.prec), 0), "xcq_exc_vxc");
   check(xcq_download_results(s.ctx, EXC, &s.last_nel, nullptr, 0), "xcq_download_results");
+}
+
+void DeviceXCIntegratorImpl::eval_exc_vxc_uks(int64_t m, int64_t n, const double* Ps,
+                                              int64_t ldps, const double* Pz, int64_t ldpz,
+                                              double* VXCs, int64_t ldvxcs, double* VXCz,
+                                              int64_t ldvxcz, double* EXC,
+                                              const IntegratorSettingsXC&) {
+  auto& s = *s_;
+  s.validate_uks(m, n, ldps, ldpz, ldvxcs, ldvxcz, true);
+  if (!Ps || !Pz || !VXCs || !VXCz || !EXC) GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  if (ldvxcs != ldvxcz) {
+    // The device download takes one leading dimension for both potentials.
+    std::vector<double> vz(size_t(m) * size_t(m));
+    eval_exc_vxc_uks(m, n, Ps, ldps, Pz, ldpz, VXCs, ldvxcs, vz.data(), m, EXC, {});
+    for (int64_t j = 0; j < m; ++j)
+      for (int64_t i = 0; i < m; ++i) VXCz[i + j * ldvxcz] = vz[size_t(i) + size_t(j) * m];
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  s.ensure_static_data();
+  check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), Ps, ldps), "xcq_upload_density");
+  check(xcq_upload_density_z(s.ctx, static_cast<int32_t>(m), Pz, ldpz), "xcq_upload_density_z");
+  const auto t1 = std::chrono::steady_clock::now();
+  check(xcq_exc_vxc_uks(s.ctx, xcq_func_id(*s.func), 1), "xcq_exc_vxc_uks");
+  check(xcq_synchronize(s.ctx), "xcq_synchronize");
+  const auto t2 = std::chrono::steady_clock::now();
+  check(xcq_download_results_uks(s.ctx, EXC, &s.last_nel, VXCs, VXCz, ldvxcs),
+        "xcq_download_results_uks");
+  const auto t3 = std::chrono::steady_clock::now();
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  s.timer.set("XCIntegrator.HostToDevice_EXC_VXC", ms(t0, t1));
+  s.timer.set("XCIntegrator.LocalWork_EXC_VXC", ms(t1, t2));
+  s.timer.set("XCIntegrator.DeviceToHostCopy_EXC_VXC", ms(t2, t3));
+}
+
+void DeviceXCIntegratorImpl::eval_exc_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps,
+                                          const double* Pz, int64_t ldpz, double* EXC,
+                                          const IntegratorSettingsXC&) {
+  auto& s = *s_;
+  s.validate_uks(m, n, ldps, ldpz, 0, 0, false);
+  if (!Ps || !Pz || !EXC) GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  s.ensure_static_data();
+  check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), Ps, ldps), "xcq_upload_density");
+  check(xcq_upload_density_z(s.ctx, static_cast<int32_t>(m), Pz, ldpz), "xcq_upload_density_z");
+  check(xcq_exc_vxc_uks(s.ctx, xcq_func_id(*s.func), 0), "xcq_exc_vxc_uks");
+  check(xcq_download_results_uks(s.ctx, EXC, &s.last_nel, nullptr, nullptr, m),
+        "xcq_download_results_uks");
 }
 
 void DeviceXCIntegratorImpl::integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp,

@@ -217,6 +217,20 @@ void gauxc_integrator_eval_exc_vxc_rks(GauXCStatus* status, const GauXCIntegrato
                                        const int64_t m, const int64_t n,
                                        const double* density_matrix, const int64_t ldp,
                                        double* exc, double* vxc_matrix, const int64_t ldvxc);
+/* UKS (GauXC include/gauxc/c/xc_integrator.h:113-133, 193-222): total density
+ * P_s = P_a + P_b and spin density P_z = P_a - P_b; needs a polarized
+ * functional (FP64 local work driver). */
+void gauxc_integrator_eval_exc_uks(GauXCStatus* status, const GauXCIntegrator integrator,
+                                   const int64_t m, const int64_t n,
+                                   const double* density_matrix_s, const int64_t ldp_s,
+                                   const double* density_matrix_z, const int64_t ldp_z,
+                                   double* exc);
+void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* status, const GauXCIntegrator integrator,
+                                       const int64_t m, const int64_t n,
+                                       const double* density_matrix_s, const int64_t ldp_s,
+                                       const double* density_matrix_z, const int64_t ldp_z,
+                                       double* exc, double* vxc_matrix_s, const int64_t vxc_ld_s,
+                                       double* vxc_matrix_z, const int64_t vxc_ld_z);
 /* Extension: the integrator's device context (an xcq_context of
  * include/gauxc_amd/device.h) for stage timings; NULL before first use. */
 void* gauxc_integrator_device_context(GauXCStatus* status, const GauXCIntegrator integrator);

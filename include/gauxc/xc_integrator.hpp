@@ -86,6 +86,12 @@ class DeviceXCIntegratorImpl {
                     int64_t ldvxc, double* EXC, const IntegratorSettingsXC& s);
   void eval_exc(int64_t m, int64_t n, const double* P, int64_t ldp, double* EXC,
                 const IntegratorSettingsXC& s);
+  /// UKS (P_s = P_a + P_b, P_z = P_a - P_b), FP64; a polarized functional.
+  void eval_exc_vxc_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz,
+                        int64_t ldpz, double* VXCs, int64_t ldvxcs, double* VXCz,
+                        int64_t ldvxcz, double* EXC, const IntegratorSettingsXC& s);
+  void eval_exc_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz,
+                    int64_t ldpz, double* EXC, const IntegratorSettingsXC& s);
   void integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp, double* N_EL);
   const util::Timer& get_timings() const;
   const LoadBalancer& load_balancer() const;
@@ -111,6 +117,7 @@ class XCIntegrator {
   using matrix_type = MatrixType;
   using value_type = typename MatrixType::value_type;
   using exc_vxc_type_rks = std::tuple<value_type, matrix_type>;
+  using exc_vxc_type_uks = std::tuple<value_type, matrix_type, matrix_type>;
 
   XCIntegrator() = default;
   explicit XCIntegrator(std::shared_ptr<detail::DeviceXCIntegratorImpl> p) : pimpl_(std::move(p)) {}
@@ -131,6 +138,20 @@ class XCIntegrator {
     value_type e = 0.;
     impl().eval_exc_vxc(P.rows(), P.cols(), P.data(), P.rows(), VXC.data(), VXC.rows(), &e, s);
     return std::make_tuple(e, std::move(VXC));
+  }
+  value_type eval_exc(const MatrixType& Ps, const MatrixType& Pz,
+                      const IntegratorSettingsXC& s = IntegratorSettingsXC{}) {
+    value_type e = 0.;
+    impl().eval_exc_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(), Pz.rows(), &e, s);
+    return e;
+  }
+  exc_vxc_type_uks eval_exc_vxc(const MatrixType& Ps, const MatrixType& Pz,
+                                const IntegratorSettingsXC& s = IntegratorSettingsXC{}) {
+    matrix_type VXCs(Ps.rows(), Ps.cols()), VXCz(Ps.rows(), Ps.cols());
+    value_type e = 0.;
+    impl().eval_exc_vxc_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(), Pz.rows(),
+                            VXCs.data(), VXCs.rows(), VXCz.data(), VXCz.rows(), &e, s);
+    return std::make_tuple(e, std::move(VXCs), std::move(VXCz));
   }
   const util::Timer& get_timings() const { return impl().get_timings(); }
   const LoadBalancer& load_balancer() const { return impl().load_balancer(); }

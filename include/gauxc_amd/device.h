@@ -150,10 +150,14 @@ int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol);
 int xcq_get_tile_policy(xcq_context ctx, int32_t* out5);
 /* nqueues = 0: one workgroup per work item (large-first upload order,
  * XCD-interleaved).  nqueues > 0: persistent launches of nqueues workgroups
- * per stage list, each draining its own queue; work items are packed
- * longest-first into the queues by per-task predicted costs (dens_cost,
- * syr2k_cost, ntasks entries each, split evenly over a task's items), or by
- * the tile-flop heuristic when both are NULL. */
+ * per stage list drawing items from dynamic queues: the list is split into
+ * one sub-list per XCD (tasks assigned largest predicted cost first to the
+ * least-loaded sub-list, a task's items kept together in its sub-list, tasks
+ * in decreasing cost order), and every workgroup takes the next item of its
+ * XCD's sub-list from a ticket counter until the sub-list is empty.  Costs
+ * are per-task predictions (dens_cost, syr2k_cost, ntasks entries each,
+ * split evenly over a task's items), or the tile-flop heuristic when both
+ * are NULL. */
 int xcq_set_schedule(xcq_context ctx, int32_t nqueues, const double* dens_cost,
                      const double* syr2k_cost);
 /* Per-item wall-clock stamps in the density and B^T Z kernels; after a run,
@@ -198,6 +202,17 @@ int xcq_download_weights(xcq_context ctx, double* weights);
  *      results stay in HBM until xcq_download_results.  want_vxc = 0 gives
  *      the EXC-only path (eval_exc). ---- */
 int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc);
+/* ---- UKS EXC/VXC (GauXC eval_exc_vxc(Ps, Pz): the generic driver with
+ *      spin_dim 2, reference_replicated_xc_host_integrator_exc_vxc.hpp:106-601;
+ *      device kernels cuda/kernels/zmat_vxc.cu:62-96,197-274).  P_s = P_a + P_b
+ *      goes through xcq_upload_density, P_z = P_a - P_b through
+ *      xcq_upload_density_z (single-molecule contexts).  FP64, stored basis
+ *      planes, every VXC contribution added.  Synchronous results through
+ *      xcq_download_results_uks (vxc_s / vxc_z may be NULL). ---- */
+int xcq_upload_density_z(xcq_context ctx, int32_t nbf, const double* Pz, int64_t ldp);
+int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc);
+int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* vxc_s,
+                             double* vxc_z, int64_t ldvxc);
 /* Integrated density only (eval_integrate_den). */
 int xcq_integrate_den(xcq_context ctx, double* nel);
 int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc);

@@ -54,6 +54,12 @@ def lib():
                                      ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64,
                                      ctypes.c_int]
         L.oracle_exc_vxc.restype = ctypes.c_int
+        L.oracle_functional_pol.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp, vp, vp]
+        L.oracle_functional_pol.restype = ctypes.c_int
+        L.oracle_exc_vxc_uks.argtypes = [ctypes.c_int, ctypes.c_int32, vp, vp, ctypes.c_int64,
+                                         vp, ctypes.c_int64, ctypes.c_int32, vp, vp, vp, vp,
+                                         vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
+        L.oracle_exc_vxc_uks.restype = ctypes.c_int
         L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.oracle_use_blas.restype = ctypes.c_int
         _lib = L
@@ -143,6 +149,40 @@ def exc_vxc(func, nbf, shells, P, t, weights=None, nthreads=0):
                               ctypes.addressof(nel), vxc.ctypes.data, nbf, nthreads)
     assert rc == 0
     return exc.value, nel.value, vxc
+
+
+def functional_pol(func, rho, sigma=None):
+    """Spin-polarised functional: rho (n, 2) = (rho_a, rho_b), sigma (n, 3) =
+    (aa, ab, bb); returns eps (n), vrho (n, 2), vsigma (n, 3) or None."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    n = len(rho)
+    eps, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+    sig = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+    rc = lib().oracle_functional_pol(FUNC[func], n, rho.ctypes.data,
+                                     None if sig is None else sig.ctypes.data, eps.ctypes.data,
+                                     vr.ctypes.data, None if sig is None else vs.ctypes.data)
+    assert rc == 0
+    return eps, vr, (vs if sig is not None else None)
+
+
+def exc_vxc_uks(func, nbf, shells, Ps, Pz, t, weights=None, nthreads=0):
+    """(EXC, NEL, VXC_s, VXC_z) of the restated UKS host integrator."""
+    Ps = np.asfortranarray(Ps, dtype=np.float64)
+    Pz = np.asfortranarray(Pz, dtype=np.float64)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    vs, vz = np.zeros((nbf, nbf), order="F"), np.zeros((nbf, nbf), order="F")
+    exc = ctypes.c_double(0.)
+    nel = ctypes.c_double(0.)
+    tasks = _task_array(t)
+    rc = lib().oracle_exc_vxc_uks(FUNC[func], nbf, ctypes.addressof(shells), Ps.ctypes.data,
+                                  nbf, Pz.ctypes.data, nbf, len(t["npts"]), tasks.ctypes.data,
+                                  pts.ctypes.data, w.ctypes.data, sl.ctypes.data,
+                                  ctypes.addressof(exc), ctypes.addressof(nel), vs.ctypes.data,
+                                  vz.ctypes.data, nbf, nthreads)
+    assert rc == 0
+    return exc.value, nel.value, vs, vz
 
 
 # Host BLAS libraries tried, in order, for the CPU baseline (MKL's single

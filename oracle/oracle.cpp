@@ -257,6 +257,187 @@ inline FuncOut eval_func(int func, double rho, double sigma) {
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Spin-polarised functionals (UKS): energy density per volume as a function
+// of (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb), differentiated exactly by
+// forward-mode dual numbers (five directions), so vrho / vsigma need no
+// hand-derived formulas.  Published forms:
+//   exchange by spin scaling, E_x[ra, rb] = (E_x[2 ra, 4 saa] + E_x[2 rb, 4 sbb]) / 2
+//   (Oliver-Perdew 1979), for Slater and PBE exchange;
+//   VWN RPA correlation with the simple spin interpolation
+//   eps = eps_P + (eps_F - eps_P) f(zeta) of libxc's LDA_C_VWN_RPA
+//   (paramagnetic and ferromagnetic RPA fits of Vosko-Wilk-Nusair 1980);
+//   PBE correlation (Perdew-Burke-Ernzerhof 1996, phi(zeta) spin scaling)
+//   over PW92 with its spin stiffness, "modified" parameters as libxc's
+//   GGA_C_PBE (f''(0) = 1.709920934161365617563962776245).
+// Thresholds: a spin channel with 2 rho_s <= 1e-15 gives no exchange; no
+// correlation when rho_a + rho_b <= 1e-15 (LDA) or 1e-12 (PBE).  GauXC's UKS
+// path calls these through ExchCXX (reference_replicated_xc_host_integrator_
+// exc_vxc.hpp:445-450 with spin_dim 2).
+// ---------------------------------------------------------------------------
+struct Dual {
+  double v, d[5];
+  Dual(double x = 0.) : v(x) { for (double& e : d) e = 0.; }
+  static Dual var(double x, int k) {
+    Dual r(x);
+    r.d[k] = 1.;
+    return r;
+  }
+};
+inline Dual chain(const Dual& a, double f, double df) {
+  Dual r(f);
+  for (int k = 0; k < 5; ++k) r.d[k] = df * a.d[k];
+  return r;
+}
+inline Dual operator+(const Dual& a, const Dual& b) {
+  Dual r(a.v + b.v);
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+inline Dual operator-(const Dual& a, const Dual& b) {
+  Dual r(a.v - b.v);
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+inline Dual operator-(const Dual& a) { return Dual(0.) - a; }
+inline Dual operator*(const Dual& a, const Dual& b) {
+  Dual r(a.v * b.v);
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+inline Dual operator/(const Dual& a, const Dual& b) {
+  Dual r(a.v / b.v);
+  for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+  return r;
+}
+inline Dual operator+(const Dual& a, double b) { return a + Dual(b); }
+inline Dual operator+(double a, const Dual& b) { return Dual(a) + b; }
+inline Dual operator-(const Dual& a, double b) { return a - Dual(b); }
+inline Dual operator-(double a, const Dual& b) { return Dual(a) - b; }
+inline Dual operator*(const Dual& a, double b) { return chain(a, a.v * b, b); }
+inline Dual operator*(double a, const Dual& b) { return chain(b, a * b.v, a); }
+inline Dual operator/(const Dual& a, double b) { return chain(a, a.v / b, 1. / b); }
+inline Dual operator/(double a, const Dual& b) { return Dual(a) / b; }
+inline Dual dsqrt(const Dual& a) { const double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
+inline Dual dcbrt(const Dual& a) {
+  const double c = std::cbrt(a.v);
+  return chain(a, c, 1. / (3. * c * c));
+}
+inline Dual dpow(const Dual& a, double e) {
+  const double p = std::pow(a.v, e);
+  return chain(a, p, e * std::pow(a.v, e - 1.));
+}
+inline Dual dlog(const Dual& a) { return chain(a, std::log(a.v), 1. / a.v); }
+inline Dual dlog1p(const Dual& a) { return chain(a, std::log1p(a.v), 1. / (1. + a.v)); }
+inline Dual dexp(const Dual& a) { const double e = std::exp(a.v); return chain(a, e, e); }
+inline Dual datan(const Dual& a) { return chain(a, std::atan(a.v), 1. / (1. + a.v * a.v)); }
+
+// Unpolarised exchange energy densities of total density n (gradient s2 = |grad n|^2).
+inline Dual slater_e(const Dual& n) { return -0.75 * std::cbrt(3. / kPi) * n * dcbrt(n); }
+inline Dual pbe_x_e(const Dual& n, const Dual& s2) {
+  const double kappa = 0.804, mu = 0.2195149727645171;
+  const double C = 1. / (4. * std::pow(3. * kPi * kPi, 2. / 3.));
+  const Dual n43 = n * dcbrt(n);
+  const Dual ss = C * s2 / (n43 * n43);
+  const Dual F = 1. + kappa - kappa / (1. + mu * ss / kappa);
+  return -0.75 * std::cbrt(3. / kPi) * n43 * F;
+}
+// VWN fit of one spin state (x = sqrt(rs)).
+inline Dual vwn_fit(const Dual& x, double A, double b, double c, double x0) {
+  const Dual X = x * x + b * x + c;
+  const double X0 = x0 * x0 + b * x0 + c, Q = std::sqrt(4. * c - b * b);
+  const Dual at = datan(Q / (2. * x + b));
+  return A * (dlog(x * x / X) + 2. * b / Q * at -
+              b * x0 / X0 * (dlog((x - x0) * (x - x0) / X) + 2. * (b + 2. * x0) / Q * at));
+}
+// PW92 G(rs) of one parameter set.
+inline Dual pw92_g(const Dual& rs, double a, double a1, double b1, double b2, double b3,
+                   double b4) {
+  const Dual sr = dsqrt(rs);
+  const Dual q = 2. * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
+  return -2. * a * (1. + a1 * rs) * dlog1p(1. / q);
+}
+inline Dual fzeta(const Dual& z) {
+  return (dpow(1. + z, 4. / 3.) + dpow(1. - z, 4. / 3.) - 2.) / (std::pow(2., 4. / 3.) - 2.);
+}
+
+// Energy density of the polarised functional; all five inputs are duals.
+Dual pol_energy(int func, const Dual& ra, const Dual& rb, const Dual& saa, const Dual& sab,
+                const Dual& sbb) {
+  const double dens_tol = 1e-15;
+  Dual e(0.);
+  const double xs = func == ORACLE_PBE0 ? 0.75 : 1.0;
+  // Exchange by spin scaling.
+  if (2. * ra.v > dens_tol)
+    e = e + 0.5 * xs * (func == ORACLE_SVWN5 ? slater_e(2. * ra) : pbe_x_e(2. * ra, 4. * saa));
+  if (2. * rb.v > dens_tol)
+    e = e + 0.5 * xs * (func == ORACLE_SVWN5 ? slater_e(2. * rb) : pbe_x_e(2. * rb, 4. * sbb));
+  const Dual n = ra + rb;
+  if (n.v <= (func == ORACLE_SVWN5 ? 1e-15 : 1e-12)) return e;
+  const Dual zeta = (ra - rb) / n;
+  const Dual rs = dcbrt(3. / (4. * kPi * n));
+  if (func == ORACLE_SVWN5) {
+    const Dual x = dsqrt(rs);
+    const Dual ep = vwn_fit(x, 0.0310907, 13.0720, 42.7198, -0.409286);
+    const Dual ef = vwn_fit(x, 0.01554535, 20.1231, 101.578, -0.743294);
+    return e + n * (ep + (ef - ep) * fzeta(zeta));
+  }
+  const Dual ep = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const Dual ef = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const Dual mac = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const double fz20 = 1.709920934161365617563962776245;
+  const Dual f = fzeta(zeta), z4 = zeta * zeta * zeta * zeta;
+  const Dual ec = ep - mac * f / fz20 * (1. - z4) + (ef - ep) * f * z4;
+  const double beta = 0.06672455060314922, gamma = (1. - std::log(2.)) / (kPi * kPi);
+  const Dual phi = 0.5 * (dpow(1. + zeta, 2. / 3.) + dpow(1. - zeta, 2. / 3.));
+  const Dual phi3 = phi * phi * phi;
+  const double D = kPi / (16. * std::cbrt(3. * kPi * kPi));
+  const Dual stot = saa + 2. * sab + sbb;
+  const Dual t2 = D * stot / (phi * phi * dpow(n, 7. / 3.));
+  const Dual A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.);
+  const Dual At2 = A * t2;
+  const Dual H = gamma * phi3 *
+                 dlog1p((beta / gamma) * t2 * (1. + At2) / (1. + At2 + At2 * At2));
+  return e + n * (ec + H);
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_functional_pol(int func, int64_t n, const double* rho, const double* sigma,
+                          double* eps, double* vrho, double* vsigma) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  const double sig_tol2 = 2.1544346900318838e-43 * 2.1544346900318838e-43;
+  for (int64_t i = 0; i < n; ++i) {
+    const double ra = std::max(rho[2 * i], 0.), rb = std::max(rho[2 * i + 1], 0.);
+    double saa = 0., sab = 0., sbb = 0.;
+    if (gga) {
+      saa = std::max(sigma[3 * i], sig_tol2);
+      sbb = std::max(sigma[3 * i + 2], sig_tol2);
+      // |sigma_ab| <= (sigma_aa + sigma_bb) / 2 keeps sigma_tot >= 0.
+      sab = std::max(std::min(sigma[3 * i + 1], 0.5 * (saa + sbb)), -0.5 * (saa + sbb));
+    }
+    const Dual e = pol_energy(func, Dual::var(ra, 0), Dual::var(rb, 1), Dual::var(saa, 2),
+                              Dual::var(sab, 3), Dual::var(sbb, 4));
+    const double nt = ra + rb;
+    eps[i] = nt > 0. ? e.v / nt : 0.;
+    vrho[2 * i] = e.d[0];
+    vrho[2 * i + 1] = e.d[1];
+    if (gga) {
+      vsigma[3 * i] = e.d[2];
+      vsigma[3 * i + 1] = e.d[3];
+      vsigma[3 * i + 2] = e.d[4];
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
 }  // namespace
 
 extern "C" {
@@ -573,6 +754,178 @@ int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const doub
   // Symmetrise lower -> upper (:577-599).
   for (int32_t j = 0; j < nbf; ++j)
     for (int32_t i = j + 1; i < nbf; ++i) vxc[j + (size_t)i * ldvxc] = vxc[i + (size_t)j * ldvxc];
+  *exc = EXC;
+  *nel = NEL;
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// UKS EXC/VXC local work: GauXC's generic host driver with spin_dim 2
+// (reference_replicated_xc_host_integrator_exc_vxc.hpp:106-601, is_uks):
+// X_s = B P_s and X_z = B P_z (factor 1, :387-396); rho_+- = (rho_s +- rho_z)/2
+// and the three sigmas of eval_uvvar_{lda,gga}_uks
+// (reference_local_host_work_driver.cxx:166-187, 270-330); the polarised
+// functional, weight folding (:452-470); Z_s, Z_z of eval_zmat_{lda,gga}_vxc_uks
+// (:607-632, 715-770); VXC_s = B^T Z_s + Z_s^T B and VXC_z likewise (inc_vxc),
+// symmetrised.
+int oracle_exc_vxc_uks(int func, int32_t nbf, const oracle_shell* shells, const double* Ps,
+                       int64_t ldps, const double* Pz, int64_t ldpz, int32_t ntasks,
+                       const oracle_task* tasks, const double* points, const double* weights,
+                       const int32_t* shell_lists, double* exc, double* nel, double* vxc_s,
+                       double* vxc_z, int64_t ldvxc, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = 0; i < nbf; ++i) vxc_s[i + (size_t)j * ldvxc] = vxc_z[i + (size_t)j * ldvxc] = 0.;
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+  }
+  double EXC = 0., NEL = 0.;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> B, Bx, By, Bz, X[2], Z[2], Psub[2], V, rho, sig, eps, vr, vs;
+    std::vector<double> g[2][3];
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic) reduction(+ : EXC, NEL)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      B.assign((size_t)npts * nbe, 0.);
+      if (gga) {
+        Bx.assign(B.size(), 0.);
+        By.assign(B.size(), 0.);
+        Bz.assign(B.size(), 0.);
+      }
+      oracle_collocation(shells, t.nshells, sl, npts, pts, B.data(), gga ? Bx.data() : nullptr,
+                         gga ? By.data() : nullptr, gga ? Bz.data() : nullptr);
+      const double* Pm[2] = {Ps, Pz};
+      const int64_t ldm[2] = {ldps, ldpz};
+      for (int c = 0; c < 2; ++c) {
+        Psub[c].resize((size_t)nbe * nbe);
+        for (int32_t j = 0; j < nbe; ++j)
+          for (int32_t i = 0; i < nbe; ++i)
+            Psub[c][(size_t)i + (size_t)j * nbe] = Pm[c][bf[i] + (size_t)bf[j] * ldm[c]];
+        X[c].assign((size_t)npts * nbe, 0.);
+        for (int32_t p = 0; p < npts; ++p) {
+          double* xr = X[c].data() + (size_t)p * nbe;
+          const double* br = B.data() + (size_t)p * nbe;
+          for (int32_t k = 0; k < nbe; ++k) {
+            const double bk = br[k];
+            const double* pc = Psub[c].data() + (size_t)k * nbe;
+            for (int32_t j = 0; j < nbe; ++j) xr[j] += bk * pc[j];
+          }
+        }
+      }
+      rho.assign(2 * (size_t)npts, 0.);
+      sig.assign(3 * (size_t)npts, 0.);
+      for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 3; ++d) g[c][d].assign(npts, 0.);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        double r[2] = {0., 0.}, gr[2][3] = {{0., 0., 0.}, {0., 0., 0.}};
+        for (int c = 0; c < 2; ++c)
+          for (int32_t k = 0; k < nbe; ++k) {
+            const double x = X[c][o + k];
+            r[c] += B[o + k] * x;
+            if (gga) {
+              gr[c][0] += Bx[o + k] * x;
+              gr[c][1] += By[o + k] * x;
+              gr[c][2] += Bz[o + k] * x;
+            }
+          }
+        rho[2 * p] = 0.5 * (r[0] + r[1]);
+        rho[2 * p + 1] = 0.5 * (r[0] - r[1]);
+        if (gga) {
+          double nn = 0., mm = 0., nm = 0.;
+          for (int d = 0; d < 3; ++d) {
+            g[0][d][p] = 2. * gr[0][d];  // grad n
+            g[1][d][p] = 2. * gr[1][d];  // grad m_z
+            nn += g[0][d][p] * g[0][d][p];
+            mm += g[1][d][p] * g[1][d][p];
+            nm += g[0][d][p] * g[1][d][p];
+          }
+          sig[3 * p] = 0.25 * (nn + mm) + 0.5 * nm;
+          sig[3 * p + 1] = 0.25 * (nn - mm);
+          sig[3 * p + 2] = 0.25 * (nn + mm) - 0.5 * nm;
+        }
+      }
+      eps.resize(npts);
+      vr.resize(2 * (size_t)npts);
+      vs.assign(3 * (size_t)npts, 0.);
+      oracle_functional_pol(func, npts, rho.data(), gga ? sig.data() : nullptr, eps.data(),
+                            vr.data(), gga ? vs.data() : nullptr);
+      for (int32_t p = 0; p < npts; ++p) {
+        const double n = rho[2 * p] + rho[2 * p + 1];
+        NEL += w[p] * n;
+        EXC += w[p] * eps[p] * n;
+      }
+      for (int c = 0; c < 2; ++c) Z[c].resize((size_t)npts * nbe);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        const double fp = 0.5 * w[p] * vr[2 * p], fm = 0.5 * w[p] * vr[2 * p + 1];
+        const double as = 0.5 * (fp + fm), az = 0.5 * (fp - fm);
+        double cs[3] = {0., 0., 0.}, cz[3] = {0., 0., 0.};
+        if (gga) {
+          const double vpp = w[p] * vs[3 * p], vpm = w[p] * vs[3 * p + 1],
+                       vmm = w[p] * vs[3 * p + 2];
+          const double g1 = 0.5 * (vpp + vpm + vmm), g2 = 0.5 * (vpp - vmm),
+                       g3 = 0.5 * (vpp - vpm + vmm);
+          for (int d = 0; d < 3; ++d) {
+            cs[d] = g1 * g[0][d][p] + g2 * g[1][d][p];
+            cz[d] = g3 * g[1][d][p] + g2 * g[0][d][p];
+          }
+        }
+        for (int32_t k = 0; k < nbe; ++k) {
+          double zs = as * B[o + k], zz = az * B[o + k];
+          if (gga) {
+            zs += cs[0] * Bx[o + k] + cs[1] * By[o + k] + cs[2] * Bz[o + k];
+            zz += cz[0] * Bx[o + k] + cz[1] * By[o + k] + cz[2] * Bz[o + k];
+          }
+          Z[0][o + k] = zs;
+          Z[1][o + k] = zz;
+        }
+      }
+      double* out[2] = {vxc_s, vxc_z};
+      for (int c = 0; c < 2; ++c) {
+        V.assign((size_t)nbe * nbe, 0.);
+        for (int32_t p = 0; p < npts; ++p) {
+          const size_t o = (size_t)p * nbe;
+          for (int32_t i = 0; i < nbe; ++i) {
+            const double bi = B[o + i], zi = Z[c][o + i];
+            double* vrow = V.data() + (size_t)i * nbe;
+            for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[c][o + j] + zi * B[o + j];
+          }
+        }
+        for (int32_t i = 0; i < nbe; ++i)
+          for (int32_t j = 0; j <= i; ++j) {
+#pragma omp atomic
+            out[c][bf[i] + (size_t)bf[j] * ldvxc] += V[(size_t)i * nbe + j];
+          }
+      }
+    }
+  }
+  for (double* v : {vxc_s, vxc_z})
+    for (int32_t j = 0; j < nbf; ++j)
+      for (int32_t i = j + 1; i < nbf; ++i) v[j + (size_t)i * ldvxc] = v[i + (size_t)j * ldvxc];
   *exc = EXC;
   *nel = NEL;
   return 0;

@@ -65,6 +65,20 @@ int oracle_exc_vxc(int func, int32_t nbf, const oracle_shell* shells, const doub
                    const double* weights, const int32_t* shell_lists, double* exc, double* nel,
                    double* vxc, int64_t ldvxc, int nthreads);
 
+/* Spin-polarised functional on n points: rho = (rho_a, rho_b) pairs, sigma =
+ * (aa, ab, bb) triples (GGA only); eps per particle, vrho pairs, vsigma
+ * triples. */
+int oracle_functional_pol(int func, int64_t n, const double* rho, const double* sigma,
+                          double* eps, double* vrho, double* vsigma);
+
+/* UKS EXC/VXC local work: P_s = P_a + P_b and P_z = P_a - P_b in, VXC_s and
+ * VXC_z out (symmetrised), GauXC's generic host driver with spin_dim 2. */
+int oracle_exc_vxc_uks(int func, int32_t nbf, const oracle_shell* shells, const double* Ps,
+                       int64_t ldps, const double* Pz, int64_t ldpz, int32_t ntasks,
+                       const oracle_task* tasks, const double* points, const double* weights,
+                       const int32_t* shell_lists, double* exc, double* nel, double* vxc_s,
+                       double* vxc_z, int64_t ldvxc, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,9 @@ Fixtures converted (reference paths relative to the GauXC tree):
 * ``tests/ref_data/benzene_cc-pvdz_ufg_tasks_1mpi_rank0_pv1.hdf5`` ->
   ``benzene_tasks_ufg.npz``: the 664 load-balancer tasks checked by
   ``tests/load_balancer_test.cxx:40-82``.
+* ``tests/ref_data/li_svwn5_sto3g_uks.bin`` (HDF5) -> ``li_svwn5_uks.npz``:
+  the UKS check of ``tests/xc_integrator.cxx:218-237,448-453`` (scalar and
+  z densities and potentials, EXC).
 * ``tests/ref_data/water_cc-pVDZ_collocation.hdf5`` ->
   ``water_collocation.npz``: gau2grid collocation + gradient golden values
   (``tests/collocation_host.hpp:111-180``).
@@ -311,6 +314,19 @@ def main():
              pts_off=np.array(offs_p, np.int64), mask_off=np.array(offs_m, np.int64),
              eval_off=np.array(offs_e, np.int64))
     np.savez_compressed(os.path.join(OUT, "water_collocation.npz"), **d)
+
+    # UKS (tests/xc_integrator.cxx:218-237, 448-453): Li / SVWN5 / STO-3G on
+    # the unpruned ultra-fine grid; scalar (Pa + Pb) and z (Pa - Pb) densities
+    # and potentials.
+    f = H5File(os.path.join(REF, "ref_data", "li_svwn5_sto3g_uks.bin"))
+    Z, xyz = read_mol(f)
+    d = dict(Z=Z, xyz=xyz, **read_basis(f))
+    for src, dst in (("/DENSITY_SCALAR", "density_s"), ("/DENSITY_Z", "density_z"),
+                     ("/VXC_SCALAR", "vxc_s"), ("/VXC_Z", "vxc_z")):
+        d[dst] = f.read(src).T.copy()
+    d["exc"] = f.read("/EXC")
+    f.close()
+    np.savez_compressed(os.path.join(OUT, "li_svwn5_uks.npz"), **d)
     print("fixtures written to", OUT)
 
 

@@ -8,7 +8,10 @@ Each check is chosen so that a plausible mistake fails it:
   * SSF weights of 15 tasks (tests/weights_host.hpp:22-57);
   * gau2grid collocation + gradient (tests/collocation_host.hpp:111-180),
     and GauXC's gau2grid itself compiled under oracle/_ref for l = 0..4;
-  * functional derivatives by central differences, and closed-form limits.
+  * functional derivatives by central differences, and closed-form limits;
+  * UKS: GauXC's Li / SVWN5 UKS fixture; the polarised functionals' unpolarised
+    limit and finite-difference derivatives; UKS with equal spins equals RKS;
+    VXC_s / VXC_z are the finite-difference derivatives of the UKS EXC.
 """
 import ctypes
 
@@ -191,6 +194,125 @@ def test_oracle_functional_limits(orc):
     ecv = e_v[0] + 0.75 * (3 / np.pi) ** (1 / 3) * rs1 ** (1 / 3)
     assert -0.0800 < ecv < -0.0780
     assert np.all(v_lda < 0) and np.all(v_pbe < 0)
+
+
+def test_oracle_uks_matches_gauxc_li_fixture(gx, orc):
+    """UKS: Li / SVWN5 / STO-3G on the unpruned ultra-fine grid (GauXC
+    tests/xc_integrator.cxx:218-237, 448-453): pins the spin-resolved
+    densities, the polarised Slater + VWN(RPA) with its spin interpolation,
+    the weight fold, Z_s / Z_z and both potentials."""
+    d = load_golden("li_svwn5_uks.npz")
+    mol = gx.Molecule(d["Z"], d["xyz"])
+    bs = gx.BasisSet.from_arrays(d["sh_nprim"], d["sh_l"], d["sh_pure"], d["sh_alpha"],
+                                 d["sh_coeff"], d["sh_origin"], normalize=False,
+                                 shell_tolerance=EPS)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles",
+                                                  "UltraFineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    t = lb.get_tasks()
+    exc, nel, vs, vz = orc.exc_vxc_uks("SVWN5", bs.nbf(), orc.shells_from(bs.xcq_shells()),
+                                       d["density_s"], d["density_z"], t)
+    assert abs(exc - d["exc"][0]) <= 1e-10 * abs(d["exc"][0])
+    assert np.linalg.norm(vs - d["vxc_s"]) / bs.nbf() < 1e-11
+    assert np.linalg.norm(vz - d["vxc_z"]) / bs.nbf() < 1e-11
+    assert abs(nel - 3.0) < 1e-9
+
+
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
+def test_oracle_polarised_functional_unpolarised_limit(orc, func):
+    """rho_a = rho_b = rho/2, every sigma = sigma/4: the polarised functional
+    is the unpolarised one (eps, vrho_a = vrho_b = vrho, and the sigma
+    derivatives sum to 4 vsigma along that path)."""
+    rng = np.random.default_rng(11)
+    rho = 10 ** rng.uniform(-4, 1.5, 300)
+    sigma = None if func == "SVWN5" else rho ** (8 / 3) * 10 ** rng.uniform(-3, 1.5, 300)
+    e_u, v_u, s_u = orc.functional(func, rho, sigma)
+    rp = np.stack([rho / 2, rho / 2], axis=1)
+    sp = None if sigma is None else np.stack([sigma / 4] * 3, axis=1)
+    e_p, v_p, s_p = orc.functional_pol(func, rp, sp)
+    np.testing.assert_allclose(e_p, e_u, rtol=1e-12)
+    np.testing.assert_allclose(v_p[:, 0], v_u, rtol=1e-11)
+    np.testing.assert_allclose(v_p[:, 1], v_u, rtol=1e-11)
+    if sigma is not None:
+        np.testing.assert_allclose(s_p.sum(axis=1) / 4, s_u, rtol=1e-10,
+                                   atol=1e-12 * np.abs(s_u).max())
+
+
+def test_oracle_polarised_pbe_derivatives_by_finite_difference(orc):
+    """Every spin derivative of the polarised PBE energy density against
+    central differences, at strongly polarised points (zeta up to +-0.9)."""
+    rng = np.random.default_rng(12)
+    n = 200
+    rt = 10 ** rng.uniform(-3, 1, n)
+    zeta = rng.uniform(-0.9, 0.9, n)
+    rho = np.stack([rt * (1 + zeta) / 2, rt * (1 - zeta) / 2], axis=1)
+    saa = rho[:, 0] ** (8 / 3) * 10 ** rng.uniform(-2, 1, n)
+    sbb = rho[:, 1] ** (8 / 3) * 10 ** rng.uniform(-2, 1, n)
+    sab = np.sqrt(saa * sbb) * rng.uniform(-0.9, 0.9, n)
+    sig = np.stack([saa, sab, sbb], axis=1)
+    eps, vr, vs = orc.functional_pol("PBE", rho, sig)
+
+    def energy(r, s):
+        e, _, _ = orc.functional_pol("PBE", r, s)
+        return e * r.sum(axis=1)
+
+    for k in range(2):
+        h = 1e-6 * rho[:, k]
+        dr = np.zeros_like(rho)
+        dr[:, k] = h
+        fd = (energy(rho + dr, sig) - energy(rho - dr, sig)) / (2 * h)
+        np.testing.assert_allclose(vr[:, k], fd, rtol=2e-6, atol=1e-9)
+    for k in range(3):
+        # Step on each sigma's own scale (sigma_ab's is sqrt(saa sbb)).
+        h = 1e-3 * (saa, np.sqrt(saa * sbb), sbb)[k]
+        ds = np.zeros_like(sig)
+        ds[:, k] = h
+        fd = (energy(rho, sig + ds) - energy(rho, sig - ds)) / (2 * h)
+        np.testing.assert_allclose(vs[:, k], fd, rtol=2e-5, atol=1e-9)
+
+
+def _uks_system(gx):
+    Z, xyz = gx.workloads.water_cluster(2, 5.0, 4)
+    l, pure, alpha, coeff, origin = gx.workloads.even_tempered_basis(Z, xyz,
+                                                                     gx.workloads.BASIS_C0)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=1e-10)
+    lb = gx.workloads.build_tasks(gx.Molecule(Z, xyz), bs, grid="GM3")
+    nbf = bs.nbf()
+    pa = gx.workloads.density(nbf, 11, 21) / 2
+    pb = gx.workloads.density(nbf, 9, 22) / 2
+    return xyz, bs, lb.get_tasks(), pa, pb
+
+
+def test_oracle_uks_reduces_to_rks_and_is_its_own_derivative(gx, orc):
+    """UKS with P_a = P_b = P equals RKS with P (PBE: EXC, VXC_s = VXC,
+    VXC_z = 0); and for a spin-polarised density VXC_s / VXC_z are the
+    derivatives of the UKS EXC itself (d EXC / d P_s,ij = 2 VXC_s,ij for a
+    symmetric perturbation of an off-diagonal pair), which pins the Z_s / Z_z
+    and sigma combinations of the GGA spin terms."""
+    xyz, bs, t, pa, pb = _uks_system(gx)
+    nbf = bs.nbf()
+    sh = orc.shells_from(bs.xcq_shells())
+    w = orc.ssf_weights(xyz, t)
+    e_r, n_r, v_r = orc.exc_vxc("PBE", nbf, sh, pa, t, weights=w)
+    e_u, n_u, vs, vz = orc.exc_vxc_uks("PBE", nbf, sh, 2 * pa, np.zeros_like(pa), t, weights=w)
+    assert abs(e_u - e_r) <= 1e-12 * abs(e_r) and abs(n_u - n_r) <= 1e-12 * n_r
+    np.testing.assert_allclose(vs, v_r, rtol=0, atol=1e-13 * np.abs(v_r).max())
+    assert np.abs(vz).max() <= 1e-13 * np.abs(v_r).max()
+    ps, pz = pa + pb, pa - pb
+    e0, _, vs, vz = orc.exc_vxc_uks("PBE", nbf, sh, ps, pz, t, weights=w)
+    for (i, j) in ((3, 7), (0, nbf - 1), (12, 13)):
+        for which, vref in ((0, vs), (1, vz)):
+            h = 1e-5
+            d = np.zeros((nbf, nbf))
+            d[i, j] = d[j, i] = h
+            args = [ps, pz]
+            args[which] = args[which] + d
+            ep = orc.exc_vxc_uks("PBE", nbf, sh, *args, t, weights=w)[0]
+            args[which] = args[which] - 2 * d
+            em = orc.exc_vxc_uks("PBE", nbf, sh, *args, t, weights=w)[0]
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - 2 * vref[i, j]) <= 1e-7 * max(1.0, abs(fd)), (which, i, j)
 
 
 def test_oracle_mlp_gradient_matches_finite_differences():
