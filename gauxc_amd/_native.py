@@ -114,7 +114,7 @@ DEVICE_SYMBOLS = [
     "xcq_device_count", "xcq_last_error", "xcq_context_create", "xcq_context_create_mem",
     "xcq_context_destroy",
     "xcq_synchronize", "xcq_upload_basis", "xcq_upload_tasks", "xcq_upload_density",
-    "xcq_ssf_weights", "xcq_ssf_weights_tasks",
+    "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_ssf_weights_noalloc",
     "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
@@ -165,6 +165,8 @@ def device_lib():
             "xcq_upload_density_z": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
             "xcq_exc_vxc_uks": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
             "xcq_download_results_uks": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_vp, c_i64]),
+            "xcq_ssf_weights_noalloc": (ctypes.c_int, [ctypes.c_int, c_i32, c_vp, c_i64, c_vp,
+                                                       c_vp, c_vp]),
             "xcq_download_weights": (ctypes.c_int, [H, c_vp]),
             "xcq_exc_vxc": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "xcq_integrate_den": (ctypes.c_int, [H, c_vp]),

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <string>
@@ -1537,6 +1538,82 @@ int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t 
   }
   cleanup();
   return rc;
+}
+
+// ---- SSF weights without device allocation ----
+// Scratch of xcq_ssf_weights_noalloc: atom coordinates, nearest-neighbour
+// distances, a bad-parent flag and the inter-atomic distance matrix of at
+// most XCQ_SSF_NOALLOC_MAX_ATOMS atoms, 8.4 MB per device, held by every
+// device the library runs on until the process ends.  In host code the name
+// `scratch` is only a stand-in: its device address is looked up once per
+// device (hipGetSymbolAddress, with that device current) and only that
+// address is passed to kernels and copies.
+namespace {
+constexpr size_t kNoAllocMaxAtoms = XCQ_SSF_NOALLOC_MAX_ATOMS;
+constexpr size_t kNoAllocXyz = 0;                                   // 3 * max doubles
+constexpr size_t kNoAllocDnear = 3 * kNoAllocMaxAtoms;              // max doubles
+constexpr size_t kNoAllocFlag = 4 * kNoAllocMaxAtoms;               // one int32 (32 doubles reserved)
+constexpr size_t kNoAllocRab = 4 * kNoAllocMaxAtoms + 32;           // max * max doubles
+constexpr size_t kNoAllocDoubles = kNoAllocRab + kNoAllocMaxAtoms * kNoAllocMaxAtoms;
+}  // namespace
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up on first use.
+int noalloc_scratch(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return XCQ_ERROR_HIP;
+    }
+    g_scratch_addr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_addr[device];
+  return XCQ_SUCCESS;
+}
+}  // namespace
+
+int xcq_ssf_weights_noalloc(int device, int32_t natoms, const double* atom_xyz, int64_t npts,
+                            const double* points, const int32_t* parent, double* weights) {
+  XCQ_REQUIRE(natoms > 0 && atom_xyz, XCQ_ERROR_INVALID_ARGUMENT, "empty molecule");
+  XCQ_REQUIRE((size_t)natoms <= kNoAllocMaxAtoms, XCQ_ERROR_INVALID_ARGUMENT,
+              "xcq_ssf_weights_noalloc: natoms exceeds XCQ_SSF_NOALLOC_MAX_ATOMS (" +
+                  std::to_string(kNoAllocMaxAtoms) + ")");
+  XCQ_REQUIRE(npts >= 0 && (npts == 0 || (points && parent && weights)),
+              XCQ_ERROR_INVALID_ARGUMENT, "bad point arrays");
+  if (npts == 0) return XCQ_SUCCESS;
+  int n = 0;
+  if (xcq_device_count(&n) != XCQ_SUCCESS || device < 0 || device >= n ||
+      device >= kMaxDevices) {
+    set_error("no HIP device available");
+    return XCQ_ERROR_NO_DEVICE;
+  }
+  XCQ_HIP(hipSetDevice(device));
+  double* base = nullptr;
+  int rc = noalloc_scratch(device, &base);
+  if (rc) return rc;
+  double* d_xyz = base + kNoAllocXyz;
+  double* d_dn = base + kNoAllocDnear;
+  int32_t* d_flag = reinterpret_cast<int32_t*>(base + kNoAllocFlag);
+  double* d_rab = base + kNoAllocRab;
+  XCQ_HIP(hipMemcpy(d_xyz, atom_xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));
+  if ((rc = launch_molmeta(nullptr, natoms, d_xyz, d_rab, d_dn, d_flag))) return rc;
+  if ((rc = launch_ssf(nullptr, natoms, d_xyz, d_rab, d_dn, npts, points, points + 1, points + 2,
+                       3, parent, weights, d_flag)))
+    return rc;
+  int32_t bad = 0;
+  XCQ_HIP(hipMemcpy(&bad, d_flag, sizeof(bad), hipMemcpyDeviceToHost));
+  XCQ_REQUIRE(bad == 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "parent atom out of range (the weights of those points are unchanged)");
+  return XCQ_SUCCESS;
 }
 
 int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xyz) {

@@ -189,6 +189,18 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
  * weights modified in place. */
 int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t npts,
                     const double* points, const int32_t* parent, double* weights);
+/* Allocation-free variant of xcq_ssf_weights, for callers that cannot
+ * allocate device memory (code under HIP graph capture is not a target: the
+ * call synchronises).  No counterpart in GauXC.  points (x,y,z interleaved,
+ * 3*npts), parent and weights are DEVICE buffers owned by the caller;
+ * atom_xyz is a host array.  The inter-atomic distances live in a static
+ * device array of XCQ_SSF_NOALLOC_MAX_ATOMS atoms (8.4 MB per device, held
+ * until the process ends); larger molecules are rejected.  No hipMalloc or
+ * hipFree is called.  Calls that overlap on one device race on that array.
+ * A parent index out of range is reported as an error after the launch. */
+#define XCQ_SSF_NOALLOC_MAX_ATOMS 1024
+int xcq_ssf_weights_noalloc(int device, int32_t natoms, const double* atom_xyz, int64_t npts,
+                            const double* points, const int32_t* parent, double* weights);
 /* Same, on the weights of the uploaded task set (device resident). */
 int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xyz);
 /* Copy the (possibly SSF-modified) resident weights back to the host. */
