@@ -121,20 +121,27 @@ def algorithmic(stage, prec, ts, mode=0):
 
 
 def measured_traffic(config, stage, prec, mode=0):
-    """HBM bytes of the stage's launches from the committed rocprofv3 PMC
-    summary of the workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled
-    by the copy calibration factors it records), or None."""
+    """HBM bytes of one evaluation's launches of the stage, from the committed
+    rocprofv3 PMC summary of the workload: FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by the copy calibration factors it records, times the
+    kernel's dispatches per evaluation (its dispatch count over that of
+    k_reduce_final, which runs once per evaluation); None without a summary."""
     try:
         with open(PMC_SUMMARY[config]) as f:
             pmc = json.load(f)
     except (KeyError, OSError, ValueError):
         return None
+    kern = pmc.get("kernels", {})
+    evals = kern.get("k_reduce_final", {}).get("dispatches", 0)
+    if not evals:
+        return None
     total = 0.0
     found = False
     for k in stage_kernels(stage, prec, mode):
-        e = pmc.get("kernels", {}).get(k)
+        e = kern.get(k)
         if e:
-            total += (e["fetch_kib"] * pmc["fetch_scale"] + e["write_kib"] * pmc["write_scale"]) * 1024
+            per = (e["fetch_kib"] * pmc["fetch_scale"] + e["write_kib"] * pmc["write_scale"]) * 1024
+            total += per * e["dispatches"] / evals
             found = True
     return total if found else None
 

@@ -260,6 +260,7 @@ def test_bench_roofline_model_follows_basis_eval_mode():
         assert len(bench.stage_kernels("zmat", prec, 3)) == 1
     assert bench.measured_traffic("c1", "density", "mixed", 3) is not None
     assert bench.measured_traffic("c1", "syr2k_scatter", "mixed", 3) is not None
+    assert bench.measured_traffic("c4", "density", "mixed", 3) > 1e11
     assert bench.measured_traffic("c3", "density", "mixed", 3) is None
 
 
