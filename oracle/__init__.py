@@ -60,6 +60,13 @@ def lib():
                                          vp, ctypes.c_int64, ctypes.c_int32, vp, vp, vp, vp,
                                          vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
         L.oracle_exc_vxc_uks.restype = ctypes.c_int
+        L.oracle_functional_fxc.argtypes = [ctypes.c_int, ctypes.c_int64] + [vp] * 8
+        L.oracle_functional_fxc.restype = ctypes.c_int
+        L.oracle_fxc_contraction.argtypes = [ctypes.c_int, ctypes.c_int32, vp, vp,
+                                             ctypes.c_int64, vp, ctypes.c_int64,
+                                             ctypes.c_int32, vp, vp, vp, vp, vp, vp,
+                                             ctypes.c_int64, ctypes.c_int]
+        L.oracle_fxc_contraction.restype = ctypes.c_int
         L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.oracle_use_blas.restype = ctypes.c_int
         _lib = L
@@ -183,6 +190,43 @@ def exc_vxc_uks(func, nbf, shells, Ps, Pz, t, weights=None, nthreads=0):
                                   vz.ctypes.data, nbf, nthreads)
     assert rc == 0
     return exc.value, nel.value, vs, vz
+
+
+def functional_fxc(func, rho, sigma, drho, dsigma):
+    """Unpolarised functional with directional second derivatives: returns
+    vrho, vsigma, a = v2rho2 drho + v2rhosigma dsigma and
+    b = v2rhosigma drho + v2sigma2 dsigma (vsigma, b None for LDA)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    drho = np.ascontiguousarray(drho, dtype=np.float64)
+    n = len(rho)
+    gga = func != "SVWN5"
+    sig = np.ascontiguousarray(sigma if gga else np.zeros(n), dtype=np.float64)
+    dsig = np.ascontiguousarray(dsigma if gga else np.zeros(n), dtype=np.float64)
+    vr, vs, a, b = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib().oracle_functional_fxc(FUNC[func], n, rho.ctypes.data, sig.ctypes.data,
+                                     drho.ctypes.data, dsig.ctypes.data, vr.ctypes.data,
+                                     vs.ctypes.data, a.ctypes.data, b.ctypes.data)
+    assert rc == 0
+    return vr, (vs if gga else None), a, (b if gga else None)
+
+
+def fxc_contraction(func, nbf, shells, P, tP, t, weights=None, nthreads=0):
+    """(NEL, FXC) of the restated RKS FXC contraction over task dict t."""
+    P = np.asfortranarray(P, dtype=np.float64)
+    tP = np.asfortranarray(tP, dtype=np.float64)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    fxc = np.zeros((nbf, nbf), order="F")
+    nel = ctypes.c_double(0.)
+    tasks = _task_array(t)
+    rc = lib().oracle_fxc_contraction(FUNC[func], nbf, ctypes.addressof(shells), P.ctypes.data,
+                                      nbf, tP.ctypes.data, nbf, len(t["npts"]),
+                                      tasks.ctypes.data, pts.ctypes.data, w.ctypes.data,
+                                      sl.ctypes.data, ctypes.addressof(nel), fxc.ctypes.data,
+                                      nbf, nthreads)
+    assert rc == 0
+    return nel.value, fxc
 
 
 # Host BLAS libraries tried, in order, for the CPU baseline (MKL's single

@@ -275,128 +275,187 @@ inline FuncOut eval_func(int func, double rho, double sigma) {
 // path calls these through ExchCXX (reference_replicated_xc_host_integrator_
 // exc_vxc.hpp:445-450 with spin_dim 2).
 // ---------------------------------------------------------------------------
-struct Dual {
-  double v, d[5];
-  Dual(double x = 0.) : v(x) { for (double& e : d) e = 0.; }
-  static Dual var(double x, int k) {
-    Dual r(x);
-    r.d[k] = 1.;
+// Dual numbers over a scalar S with five derivative slots.  S = double
+// gives first derivatives; S = DualT<double> nests them, so the outer
+// slots carry derivatives of the inner ones (second derivatives, used by
+// the FXC contraction).
+template <class S>
+struct DualT {
+  S v;
+  S d[5];
+  DualT(double x = 0.) : v(x) {
+    for (S& e : d) e = S(0.);
+  }
+  static DualT var(double x, int k) {
+    DualT r(x);
+    r.d[k] = S(1.);
     return r;
   }
 };
-inline Dual chain(const Dual& a, double f, double df) {
-  Dual r(f);
+using Dual = DualT<double>;
+
+inline double val(double x) { return x; }
+template <class S>
+double val(const DualT<S>& a) { return val(a.v); }
+
+template <class S>
+DualT<S> chain(const DualT<S>& a, const S& f, const S& df) {
+  DualT<S> r;
+  r.v = f;
   for (int k = 0; k < 5; ++k) r.d[k] = df * a.d[k];
   return r;
 }
-inline Dual operator+(const Dual& a, const Dual& b) {
-  Dual r(a.v + b.v);
+template <class S>
+DualT<S> operator+(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r;
+  r.v = a.v + b.v;
   for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
   return r;
 }
-inline Dual operator-(const Dual& a, const Dual& b) {
-  Dual r(a.v - b.v);
+template <class S>
+DualT<S> operator-(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r;
+  r.v = a.v - b.v;
   for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] - b.d[k];
   return r;
 }
-inline Dual operator-(const Dual& a) { return Dual(0.) - a; }
-inline Dual operator*(const Dual& a, const Dual& b) {
-  Dual r(a.v * b.v);
+template <class S>
+DualT<S> operator-(const DualT<S>& a) { return DualT<S>(0.) - a; }
+template <class S>
+DualT<S> operator*(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r;
+  r.v = a.v * b.v;
   for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
   return r;
 }
-inline Dual operator/(const Dual& a, const Dual& b) {
-  Dual r(a.v / b.v);
+template <class S>
+DualT<S> operator/(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r;
+  r.v = a.v / b.v;
   for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
   return r;
 }
-inline Dual operator+(const Dual& a, double b) { return a + Dual(b); }
-inline Dual operator+(double a, const Dual& b) { return Dual(a) + b; }
-inline Dual operator-(const Dual& a, double b) { return a - Dual(b); }
-inline Dual operator-(double a, const Dual& b) { return Dual(a) - b; }
-inline Dual operator*(const Dual& a, double b) { return chain(a, a.v * b, b); }
-inline Dual operator*(double a, const Dual& b) { return chain(b, a * b.v, a); }
-inline Dual operator/(const Dual& a, double b) { return chain(a, a.v / b, 1. / b); }
-inline Dual operator/(double a, const Dual& b) { return Dual(a) / b; }
-inline Dual dsqrt(const Dual& a) { const double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
-inline Dual dcbrt(const Dual& a) {
-  const double c = std::cbrt(a.v);
+template <class S>
+DualT<S> operator+(const DualT<S>& a, double b) { return a + DualT<S>(b); }
+template <class S>
+DualT<S> operator+(double a, const DualT<S>& b) { return DualT<S>(a) + b; }
+template <class S>
+DualT<S> operator-(const DualT<S>& a, double b) { return a - DualT<S>(b); }
+template <class S>
+DualT<S> operator-(double a, const DualT<S>& b) { return DualT<S>(a) - b; }
+template <class S>
+DualT<S> operator*(const DualT<S>& a, double b) { return chain(a, a.v * b, S(b)); }
+template <class S>
+DualT<S> operator*(double a, const DualT<S>& b) { return chain(b, a * b.v, S(a)); }
+template <class S>
+DualT<S> operator/(const DualT<S>& a, double b) { return chain(a, a.v / b, S(1. / b)); }
+template <class S>
+DualT<S> operator/(double a, const DualT<S>& b) { return DualT<S>(a) / b; }
+
+// Elementary functions: the double overloads end the recursion.
+inline double dsqrt(double x) { return std::sqrt(x); }
+inline double dcbrt(double x) { return std::cbrt(x); }
+inline double dpow(double x, double e) { return std::pow(x, e); }
+inline double dlog(double x) { return std::log(x); }
+inline double dlog1p(double x) { return std::log1p(x); }
+inline double dexp(double x) { return std::exp(x); }
+inline double datan(double x) { return std::atan(x); }
+template <class S>
+DualT<S> dsqrt(const DualT<S>& a) {
+  const S s = dsqrt(a.v);
+  return chain(a, s, 0.5 / s);
+}
+template <class S>
+DualT<S> dcbrt(const DualT<S>& a) {
+  const S c = dcbrt(a.v);
   return chain(a, c, 1. / (3. * c * c));
 }
-inline Dual dpow(const Dual& a, double e) {
-  const double p = std::pow(a.v, e);
-  return chain(a, p, e * std::pow(a.v, e - 1.));
+template <class S>
+DualT<S> dpow(const DualT<S>& a, double e) {
+  return chain(a, dpow(a.v, e), e * dpow(a.v, e - 1.));
 }
-inline Dual dlog(const Dual& a) { return chain(a, std::log(a.v), 1. / a.v); }
-inline Dual dlog1p(const Dual& a) { return chain(a, std::log1p(a.v), 1. / (1. + a.v)); }
-inline Dual dexp(const Dual& a) { const double e = std::exp(a.v); return chain(a, e, e); }
-inline Dual datan(const Dual& a) { return chain(a, std::atan(a.v), 1. / (1. + a.v * a.v)); }
+template <class S>
+DualT<S> dlog(const DualT<S>& a) { return chain(a, dlog(a.v), 1. / a.v); }
+template <class S>
+DualT<S> dlog1p(const DualT<S>& a) { return chain(a, dlog1p(a.v), 1. / (1. + a.v)); }
+template <class S>
+DualT<S> dexp(const DualT<S>& a) {
+  const S e = dexp(a.v);
+  return chain(a, e, e);
+}
+template <class S>
+DualT<S> datan(const DualT<S>& a) { return chain(a, datan(a.v), 1. / (1. + a.v * a.v)); }
 
 // Unpolarised exchange energy densities of total density n (gradient s2 = |grad n|^2).
-inline Dual slater_e(const Dual& n) { return -0.75 * std::cbrt(3. / kPi) * n * dcbrt(n); }
-inline Dual pbe_x_e(const Dual& n, const Dual& s2) {
+template <class D>
+D slater_e(const D& n) { return -0.75 * std::cbrt(3. / kPi) * n * dcbrt(n); }
+template <class D>
+D pbe_x_e(const D& n, const D& s2) {
   const double kappa = 0.804, mu = 0.2195149727645171;
   const double C = 1. / (4. * std::pow(3. * kPi * kPi, 2. / 3.));
-  const Dual n43 = n * dcbrt(n);
-  const Dual ss = C * s2 / (n43 * n43);
-  const Dual F = 1. + kappa - kappa / (1. + mu * ss / kappa);
+  const D n43 = n * dcbrt(n);
+  const D ss = C * s2 / (n43 * n43);
+  const D F = 1. + kappa - kappa / (1. + mu * ss / kappa);
   return -0.75 * std::cbrt(3. / kPi) * n43 * F;
 }
 // VWN fit of one spin state (x = sqrt(rs)).
-inline Dual vwn_fit(const Dual& x, double A, double b, double c, double x0) {
-  const Dual X = x * x + b * x + c;
+template <class D>
+D vwn_fit(const D& x, double A, double b, double c, double x0) {
+  const D X = x * x + b * x + c;
   const double X0 = x0 * x0 + b * x0 + c, Q = std::sqrt(4. * c - b * b);
-  const Dual at = datan(Q / (2. * x + b));
+  const D at = datan(Q / (2. * x + b));
   return A * (dlog(x * x / X) + 2. * b / Q * at -
               b * x0 / X0 * (dlog((x - x0) * (x - x0) / X) + 2. * (b + 2. * x0) / Q * at));
 }
 // PW92 G(rs) of one parameter set.
-inline Dual pw92_g(const Dual& rs, double a, double a1, double b1, double b2, double b3,
+template <class D>
+D pw92_g(const D& rs, double a, double a1, double b1, double b2, double b3,
                    double b4) {
-  const Dual sr = dsqrt(rs);
-  const Dual q = 2. * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
+  const D sr = dsqrt(rs);
+  const D q = 2. * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
   return -2. * a * (1. + a1 * rs) * dlog1p(1. / q);
 }
-inline Dual fzeta(const Dual& z) {
+template <class D>
+D fzeta(const D& z) {
   return (dpow(1. + z, 4. / 3.) + dpow(1. - z, 4. / 3.) - 2.) / (std::pow(2., 4. / 3.) - 2.);
 }
 
 // Energy density of the polarised functional; all five inputs are duals.
-Dual pol_energy(int func, const Dual& ra, const Dual& rb, const Dual& saa, const Dual& sab,
-                const Dual& sbb) {
+template <class D>
+D pol_energy(int func, const D& ra, const D& rb, const D& saa, const D& sab, const D& sbb) {
   const double dens_tol = 1e-15;
-  Dual e(0.);
+  D e(0.);
   const double xs = func == ORACLE_PBE0 ? 0.75 : 1.0;
   // Exchange by spin scaling.
-  if (2. * ra.v > dens_tol)
+  if (2. * val(ra) > dens_tol)
     e = e + 0.5 * xs * (func == ORACLE_SVWN5 ? slater_e(2. * ra) : pbe_x_e(2. * ra, 4. * saa));
-  if (2. * rb.v > dens_tol)
+  if (2. * val(rb) > dens_tol)
     e = e + 0.5 * xs * (func == ORACLE_SVWN5 ? slater_e(2. * rb) : pbe_x_e(2. * rb, 4. * sbb));
-  const Dual n = ra + rb;
-  if (n.v <= (func == ORACLE_SVWN5 ? 1e-15 : 1e-12)) return e;
-  const Dual zeta = (ra - rb) / n;
-  const Dual rs = dcbrt(3. / (4. * kPi * n));
+  const D n = ra + rb;
+  if (val(n) <= (func == ORACLE_SVWN5 ? 1e-15 : 1e-12)) return e;
+  const D zeta = (ra - rb) / n;
+  const D rs = dcbrt(3. / (4. * kPi * n));
   if (func == ORACLE_SVWN5) {
-    const Dual x = dsqrt(rs);
-    const Dual ep = vwn_fit(x, 0.0310907, 13.0720, 42.7198, -0.409286);
-    const Dual ef = vwn_fit(x, 0.01554535, 20.1231, 101.578, -0.743294);
+    const D x = dsqrt(rs);
+    const D ep = vwn_fit(x, 0.0310907, 13.0720, 42.7198, -0.409286);
+    const D ef = vwn_fit(x, 0.01554535, 20.1231, 101.578, -0.743294);
     return e + n * (ep + (ef - ep) * fzeta(zeta));
   }
-  const Dual ep = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
-  const Dual ef = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
-  const Dual mac = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const D ep = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const D ef = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const D mac = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
   const double fz20 = 1.709920934161365617563962776245;
-  const Dual f = fzeta(zeta), z4 = zeta * zeta * zeta * zeta;
-  const Dual ec = ep - mac * f / fz20 * (1. - z4) + (ef - ep) * f * z4;
+  const D f = fzeta(zeta), z4 = zeta * zeta * zeta * zeta;
+  const D ec = ep - mac * f / fz20 * (1. - z4) + (ef - ep) * f * z4;
   const double beta = 0.06672455060314922, gamma = (1. - std::log(2.)) / (kPi * kPi);
-  const Dual phi = 0.5 * (dpow(1. + zeta, 2. / 3.) + dpow(1. - zeta, 2. / 3.));
-  const Dual phi3 = phi * phi * phi;
-  const double D = kPi / (16. * std::cbrt(3. * kPi * kPi));
-  const Dual stot = saa + 2. * sab + sbb;
-  const Dual t2 = D * stot / (phi * phi * dpow(n, 7. / 3.));
-  const Dual A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.);
-  const Dual At2 = A * t2;
-  const Dual H = gamma * phi3 *
+  const D phi = 0.5 * (dpow(1. + zeta, 2. / 3.) + dpow(1. - zeta, 2. / 3.));
+  const D phi3 = phi * phi * phi;
+  const double kD = kPi / (16. * std::cbrt(3. * kPi * kPi));
+  const D stot = saa + 2. * sab + sbb;
+  const D t2 = kD * stot / (phi * phi * dpow(n, 7. / 3.));
+  const D A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.);
+  const D At2 = A * t2;
+  const D H = gamma * phi3 *
                  dlog1p((beta / gamma) * t2 * (1. + At2) / (1. + At2 + At2 * At2));
   return e + n * (ec + H);
 }
@@ -928,6 +987,207 @@ int oracle_exc_vxc_uks(int func, int32_t nbf, const oracle_shell* shells, const 
       for (int32_t i = j + 1; i < nbf; ++i) v[j + (size_t)i * ldvxc] = v[i + (size_t)j * ldvxc];
   *exc = EXC;
   *nel = NEL;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FXC contraction (RKS): GauXC's host driver
+// reference_replicated_xc_host_integrator_fxc_contraction.hpp:94-600 with
+// spin_dim 1.  Per task: X = 2 P_sub B and tX = 2 tP_sub B (:393-396,
+// :441-444); rho, grad rho from X and trho, grad trho from tX
+// (eval_uvvar_{lda,gga}_rks); the functional's first and second derivatives
+// at rho (:433-439); the trial kernels of eval_tmat_{lda,gga}_vxc_rks
+// (reference_local_host_work_driver.cxx:1114-1143):
+//   tgamma = grad trho . grad rho,
+//   A = v2rho2 trho + 2 v2rhogamma tgamma,
+//   B = 2 (v2rhogamma trho + 2 v2gamma2 tgamma) grad rho + 2 vgamma grad trho,
+// weight folding (:497-517); Z = A/2 B + B . grad B (eval_zmat_gga_vxc_rks_ts,
+// :1382-1408); FXC = B^T Z + Z^T B (inc_vxc), symmetrised (:568-576).
+// ---------------------------------------------------------------------------
+namespace {
+
+// vrho, vsigma and the directional second derivatives
+//   A = v2rho2 drho + v2rhosigma dsig,  Bc = v2rhosigma drho + v2sigma2 dsig
+// of the unpolarised functional at (rho, sigma), by nested duals over the
+// spin-polarised energy at zeta = 0 (rho_a = rho_b = rho / 2, sigma_aa =
+// sigma_ab = sigma_bb = sigma / 4): inner slots d/drho (0) and d/dsigma (1),
+// outer slot 0 the direction (drho, dsig).
+void fxc_point(int func, double rho, double sigma, double drho, double dsig, double& vr,
+               double& vs, double& A, double& Bc) {
+  using D2 = DualT<Dual>;
+  const bool gga = func != ORACLE_SVWN5;
+  const double sig_tol2 = 2.1544346900318838e-43 * 2.1544346900318838e-43;
+  D2 R, S;
+  R.v = Dual::var(std::max(rho, 0.), 0);
+  R.d[0] = Dual(drho);
+  S.v = Dual::var(gga ? std::max(sigma, sig_tol2) : 0., 1);
+  S.d[0] = Dual(gga ? dsig : 0.);
+  const D2 e = pol_energy(func, 0.5 * R, 0.5 * R, 0.25 * S, 0.25 * S, 0.25 * S);
+  vr = e.v.d[0];
+  vs = e.v.d[1];
+  A = e.d[0].d[0];
+  Bc = e.d[0].d[1];
+}
+
+// out[p][j] = fac * sum_k B[p][k] M_sub[k][j] (M_sub symmetric), the
+// eval_xmat product of reference_local_host_work_driver.cxx:123-146.
+void xmat(const std::vector<double>& B, const double* M, int64_t ldm,
+          const std::vector<int32_t>& bf, int32_t npts, double fac, std::vector<double>& sub,
+          std::vector<double>& out) {
+  const int32_t nbe = (int32_t)bf.size();
+  sub.resize((size_t)nbe * nbe);
+  for (int32_t j = 0; j < nbe; ++j)
+    for (int32_t i = 0; i < nbe; ++i) sub[(size_t)i + (size_t)j * nbe] = M[bf[i] + (size_t)bf[j] * ldm];
+  out.assign((size_t)npts * nbe, 0.);
+  if (g_dgemm) {
+    const double zero = 0.0;
+    g_dgemm("N", "N", &nbe, &npts, &nbe, &fac, sub.data(), &nbe, B.data(), &nbe, &zero,
+            out.data(), &nbe);
+    return;
+  }
+  for (int32_t p = 0; p < npts; ++p) {
+    double* xr = out.data() + (size_t)p * nbe;
+    const double* br = B.data() + (size_t)p * nbe;
+    for (int32_t k = 0; k < nbe; ++k) {
+      const double bk = fac * br[k];
+      const double* pc = sub.data() + (size_t)k * nbe;
+      for (int32_t j = 0; j < nbe; ++j) xr[j] += bk * pc[j];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_functional_fxc(int func, int64_t n, const double* rho, const double* sigma,
+                          const double* drho, const double* dsigma, double* vrho,
+                          double* vsigma, double* a, double* b) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int64_t i = 0; i < n; ++i) {
+    double vr, vs, A, Bc;
+    fxc_point(func, rho[i], gga ? sigma[i] : 0., drho[i], gga ? dsigma[i] : 0., vr, vs, A, Bc);
+    vrho[i] = vr;
+    a[i] = A;
+    if (gga) {
+      vsigma[i] = vs;
+      b[i] = Bc;
+    }
+  }
+  return 0;
+}
+
+int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, const double* P,
+                           int64_t ldp, const double* tP, int64_t ldtp, int32_t ntasks,
+                           const oracle_task* tasks, const double* points,
+                           const double* weights, const int32_t* shell_lists, double* nel,
+                           double* fxc, int64_t ldfxc, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = 0; i < nbf; ++i) fxc[i + (size_t)j * ldfxc] = 0.;
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+  }
+  double NEL = 0.;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> B, Bx, By, Bz, X, tX, Z, sub, V;
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic) reduction(+ : NEL)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      B.assign((size_t)npts * nbe, 0.);
+      if (gga) {
+        Bx.assign(B.size(), 0.);
+        By.assign(B.size(), 0.);
+        Bz.assign(B.size(), 0.);
+      }
+      oracle_collocation(shells, t.nshells, sl, npts, pts, B.data(), gga ? Bx.data() : nullptr,
+                         gga ? By.data() : nullptr, gga ? Bz.data() : nullptr);
+      xmat(B, P, ldp, bf, npts, 2.0, sub, X);
+      xmat(B, tP, ldtp, bf, npts, 2.0, sub, tX);
+      Z.resize((size_t)npts * nbe);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        double r = 0., tr = 0., g[3] = {0., 0., 0.}, tg[3] = {0., 0., 0.};
+        for (int32_t k = 0; k < nbe; ++k) {
+          r += B[o + k] * X[o + k];
+          tr += B[o + k] * tX[o + k];
+          if (gga) {
+            g[0] += Bx[o + k] * X[o + k];
+            g[1] += By[o + k] * X[o + k];
+            g[2] += Bz[o + k] * X[o + k];
+            tg[0] += Bx[o + k] * tX[o + k];
+            tg[1] += By[o + k] * tX[o + k];
+            tg[2] += Bz[o + k] * tX[o + k];
+          }
+        }
+        double sig = 0., tgam = 0.;
+        for (int d = 0; d < 3; ++d) {
+          g[d] *= 2.;
+          tg[d] *= 2.;
+          sig += g[d] * g[d];
+          tgam += g[d] * tg[d];
+        }
+        double vr, vs, A, Bc;
+        fxc_point(func, r, sig, tr, 2. * tgam, vr, vs, A, Bc);
+        NEL += w[p] * r;
+        const double fa = 0.5 * w[p] * A;
+        double c[3] = {0., 0., 0.};
+        if (gga)
+          for (int d = 0; d < 3; ++d) c[d] = w[p] * (2. * Bc * g[d] + 2. * vs * tg[d]);
+        for (int32_t k = 0; k < nbe; ++k) {
+          double zv = fa * B[o + k];
+          if (gga) zv += c[0] * Bx[o + k] + c[1] * By[o + k] + c[2] * Bz[o + k];
+          Z[o + k] = zv;
+        }
+      }
+      V.assign((size_t)nbe * nbe, 0.);
+      if (g_dsyr2k) {
+        const double one = 1.0, zero = 0.0;
+        g_dsyr2k("U", "N", &nbe, &npts, &one, B.data(), &nbe, Z.data(), &nbe, &zero, V.data(),
+                 &nbe);
+      } else {
+        for (int32_t p = 0; p < npts; ++p) {
+          const size_t o = (size_t)p * nbe;
+          for (int32_t i = 0; i < nbe; ++i) {
+            const double bi = B[o + i], zi = Z[o + i];
+            double* vrow = V.data() + (size_t)i * nbe;
+            for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[o + j] + zi * B[o + j];
+          }
+        }
+      }
+      for (int32_t i = 0; i < nbe; ++i)
+        for (int32_t j = 0; j <= i; ++j) {
+#pragma omp atomic
+          fxc[bf[i] + (size_t)bf[j] * ldfxc] += V[(size_t)i * nbe + j];
+        }
+    }
+  }
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = j + 1; i < nbf; ++i) fxc[j + (size_t)i * ldfxc] = fxc[i + (size_t)j * ldfxc];
+  if (nel) *nel = NEL;
   return 0;
 }
 

@@ -79,6 +79,22 @@ int oracle_exc_vxc_uks(int func, int32_t nbf, const oracle_shell* shells, const 
                        const int32_t* shell_lists, double* exc, double* nel, double* vxc_s,
                        double* vxc_z, int64_t ldvxc, int nthreads);
 
+/* Unpolarised functional with directional second derivatives (FXC
+ * contraction): vrho, vsigma (GGA) and
+ *   a = v2rho2 drho + v2rhosigma dsigma,  b = v2rhosigma drho + v2sigma2 dsigma
+ * per point (b, vsigma, sigma and dsigma GGA only). */
+int oracle_functional_fxc(int func, int64_t n, const double* rho, const double* sigma,
+                          const double* drho, const double* dsigma, double* vrho,
+                          double* vsigma, double* a, double* b);
+
+/* RKS FXC contraction: FXC = d VXC[P + e tP] / de at e = 0, GauXC's host
+ * driver (nbf x nbf column-major, symmetrised); nel = integrated density. */
+int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, const double* P,
+                           int64_t ldp, const double* tP, int64_t ldtp, int32_t ntasks,
+                           const oracle_task* tasks, const double* points,
+                           const double* weights, const int32_t* shell_lists, double* nel,
+                           double* fxc, int64_t ldfxc, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,9 @@ Fixtures converted (reference paths relative to the GauXC tree):
   ``benzene_pbe0_cc-pvdz_ufg_ssf.hdf5`` -> ``benzene_{svwn5,pbe0}.npz``:
   MOLECULE, BASIS (stored already normalised, read back with
   ``normalize=false`` at ``src/external/hdf5_read.cxx:47-48``), DENSITY, VXC,
-  EXC.  Checked by ``tests/xc_integrator.cxx:199-212``.
+  EXC (checked by ``tests/xc_integrator.cxx:199-212``), TRIAL_DENSITY and FXC
+  (``tests/2nd_derivative_test.cxx:121-126``) and EXC_GRAD with weight
+  derivatives (``tests/xc_integrator.cxx:117-137,278-289``).
 * ``tests/ref_data/benzene_weights_ssf.hdf5`` -> ``benzene_weights_ssf.npz``:
   15 tasks with raw (``tasks_unm``) points/weights and SSF-modified weights
   (``tasks_mod``), checked by ``tests/weights_host.hpp:22-57``.
@@ -254,6 +256,12 @@ def main():
         d["density"] = f.read("/DENSITY").T.copy()
         d["vxc"] = f.read("/VXC").T.copy()
         d["exc"] = f.read("/EXC")
+        # FXC contraction check (tests/2nd_derivative_test.cxx:73-92,121-126)
+        # and the full nuclear gradient (/EXC_GRAD carries the attribute
+        # includes_weight_derivatives = 1; tests/xc_integrator.cxx:117-137).
+        d["trial_density"] = f.read("/TRIAL_DENSITY").T.copy()
+        d["fxc"] = f.read("/FXC").T.copy()
+        d["exc_grad"] = f.read("/EXC_GRAD")
         f.close()
         np.savez_compressed(os.path.join(OUT, f"benzene_{tag}.npz"), **d)
 

@@ -11,7 +11,10 @@ Each check is chosen so that a plausible mistake fails it:
   * functional derivatives by central differences, and closed-form limits;
   * UKS: GauXC's Li / SVWN5 UKS fixture; the polarised functionals' unpolarised
     limit and finite-difference derivatives; UKS with equal spins equals RKS;
-    VXC_s / VXC_z are the finite-difference derivatives of the UKS EXC.
+    VXC_s / VXC_z are the finite-difference derivatives of the UKS EXC;
+  * FXC contraction: GauXC's benzene SVWN5 / PBE0 FXC fixtures
+    (tests/2nd_derivative_test.cxx:121-126); the second derivatives by
+    differences of the first; FXC as the derivative of VXC along tP.
 """
 import ctypes
 
@@ -313,6 +316,67 @@ def test_oracle_uks_reduces_to_rks_and_is_its_own_derivative(gx, orc):
             em = orc.exc_vxc_uks("PBE", nbf, sh, *args, t, weights=w)[0]
             fd = (ep - em) / (2 * h)
             assert abs(fd - 2 * vref[i, j]) <= 1e-7 * max(1.0, abs(fd)), (which, i, j)
+
+
+@pytest.mark.parametrize("tag,func", [("svwn5", "SVWN5"), ("pbe0", "PBE0")])
+def test_oracle_fxc_matches_gauxc_benzene_fixture(orc, tag, func):
+    """GauXC's FXC contraction check (tests/2nd_derivative_test.cxx:121-126):
+    ||FXC - FXC_ref||_F / nbf < 1e-10 for the fixture's trial density."""
+    d, mol, bs, lb = benzene_setup(tag)
+    t = lb.get_tasks()
+    w = orc.ssf_weights(d["xyz"], t)
+    nel, fxc = orc.fxc_contraction(func, bs.nbf(), orc.shells_from(bs.xcq_shells()),
+                                   d["density"], d["trial_density"], t, weights=w)
+    assert np.linalg.norm(fxc - d["fxc"]) / bs.nbf() < 1e-12
+    assert abs(nel - 42.0) < 1e-4
+    assert np.array_equal(fxc, fxc.T)
+
+
+@pytest.mark.parametrize("func", ["SVWN5", "PBE", "PBE0"])
+def test_oracle_fxc_functional_by_finite_difference(orc, func):
+    """a and b are the derivatives of vrho and vsigma along (drho, dsigma):
+    central differences of the first-derivative functional (itself pinned by
+    the EXC/VXC fixtures and by test_oracle_functional_derivatives_by_finite_difference)."""
+    rng = np.random.default_rng(11)
+    n = 300
+    rho = 10 ** rng.uniform(-3, 1.0, n)
+    gga = func != "SVWN5"
+    sigma = rho ** (8 / 3) * 10 ** rng.uniform(-2, 1.0, n) if gga else None
+    drho = rho * rng.uniform(-1, 1, n)
+    dsig = sigma * rng.uniform(-1, 1, n) if gga else None
+    vr, vs, a, b = orc.functional_fxc(func, rho, sigma, drho, dsig)
+    eps0, vr0, vs0 = orc.functional(func, rho, sigma)
+    # Same derivatives by two routes (closed forms / duals); the PBE
+    # correlation's cancellations at low density leave ~1e-10 relative.
+    assert np.allclose(vr, vr0, rtol=1e-11, atol=0)
+    if gga:
+        assert np.allclose(vs, vs0, rtol=1e-9, atol=0)
+    h = 1e-4
+    _, vrp, vsp = orc.functional(func, rho + h * drho, None if not gga else sigma + h * dsig)
+    _, vrm, vsm = orc.functional(func, rho - h * drho, None if not gga else sigma - h * dsig)
+    fa = (vrp - vrm) / (2 * h)
+    # truncation ~ h^2 |third derivative| (relative 1e-7 here), round-off ~ u |vrho| / h
+    assert np.all(np.abs(a - fa) <= 1e-6 * np.abs(fa) + 1e-11 * np.abs(vr0) / h)
+    if gga:
+        fb = (vsp - vsm) / (2 * h)
+        assert np.all(np.abs(b - fb) <= 1e-6 * np.abs(fb) + 1e-11 * np.abs(vs0) / h)
+
+
+def test_oracle_fxc_is_the_derivative_of_vxc(orc):
+    """FXC = d VXC[P + e tP] / de at e = 0 (the contraction's definition):
+    central differences of the oracle's VXC along the fixture's trial density
+    (PBE0, the GGA terms included)."""
+    d, mol, bs, lb = benzene_setup("pbe0")
+    t = lb.get_tasks()
+    w = orc.ssf_weights(d["xyz"], t)
+    sh = orc.shells_from(bs.xcq_shells())
+    P, tP = d["density"], d["trial_density"]
+    _, fxc = orc.fxc_contraction("PBE0", bs.nbf(), sh, P, tP, t, weights=w)
+    h = 1e-4
+    _, _, vp = orc.exc_vxc("PBE0", bs.nbf(), sh, P + h * tP, t, weights=w)
+    _, _, vm = orc.exc_vxc("PBE0", bs.nbf(), sh, P - h * tP, t, weights=w)
+    fd = (vp - vm) / (2 * h)
+    assert np.abs(fxc - fd).max() <= 1e-6 * np.abs(fxc).max()
 
 
 def test_oracle_mlp_gradient_matches_finite_differences():
