@@ -322,6 +322,19 @@ class XCIntegrator(_Handle):
         st.raise_if()
         return exc.value
 
+    def eval_fxc_contraction(self, P, tP):
+        """RKS FXC contraction d VXC[P + e tP] / de, as GauXC's
+        eval_fxc_contraction(P, tP)."""
+        P, tP = self._mat(P), self._mat(tP)
+        m, n = P.shape
+        fxc = np.zeros((m, n), order="F")
+        st = N.Status()
+        N.host_lib().gauxc_integrator_eval_fxc_contraction_rks(
+            st.ptr, self.h, m, n, P.ctypes.data, m, tP.ctypes.data, tP.shape[0],
+            fxc.ctypes.data, m)
+        st.raise_if()
+        return fxc
+
     def integrate_den(self, P):
         P = self._mat(P)
         m, n = P.shape
@@ -447,6 +460,24 @@ class DeviceContext:
             vs.ctypes.data if want_vxc else None, vz.ctypes.data if want_vxc else None,
             self.nbf), "xcq_download_results_uks")
         return exc.value, nel.value, vs, vz
+
+    def upload_trial_density(self, tP):
+        """Trial density of the FXC contraction."""
+        tP = np.asfortranarray(tP, dtype=np.float64)
+        N.xcq_check(self.lib.xcq_upload_trial_density(self.h, tP.shape[0], tP.ctypes.data,
+                                                      tP.shape[0]), "xcq_upload_trial_density")
+
+    def fxc_contraction(self, func="PBE"):
+        N.xcq_check(self.lib.xcq_fxc_contraction(self.h, N.XCQ_FUNC[func]),
+                    "xcq_fxc_contraction")
+
+    def results_fxc(self):
+        """(NEL, FXC) of the last FXC contraction."""
+        nel = ctypes.c_double(0.)
+        fxc = np.zeros((self.nbf, self.nbf), order="F")
+        N.xcq_check(self.lib.xcq_download_fxc(self.h, ctypes.addressof(nel), fxc.ctypes.data,
+                                              self.nbf), "xcq_download_fxc")
+        return nel.value, fxc
 
     def integrate_den(self):
         nel = ctypes.c_double(0.)

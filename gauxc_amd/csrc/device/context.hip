@@ -190,6 +190,9 @@ struct xcq_context_s {
   double* d_vxcz = nullptr; // UKS: VXC_z
   bool have_pz = false;
   bool last_uks = false;    // the last evaluation was a UKS one
+  double* d_tP = nullptr;   // FXC contraction: trial density (single molecule)
+  bool have_tp = false;
+  bool last_fxc = false;    // the last evaluation was an FXC contraction (in d_vxc)
   double* d_zc = nullptr;   // Z coefficients per point block (in-stage mode 3), lazily
   int64_t pblocks = 0;      // point blocks of the uploaded tasks
   double* d_out2 = nullptr;
@@ -266,6 +269,7 @@ struct xcq_context_s {
     dfree(d_vxc);
     dfree(d_Pz);
     dfree(d_vxcz);
+    dfree(d_tP);
     dfree(d_out2);
     dfree(d_corr);
     dfree(d_rpart);
@@ -506,6 +510,7 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
   }
   ctx->last_want_vxc = want_vxc;
   ctx->last_uks = false;
+  ctx->last_fxc = false;
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
   if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
@@ -702,6 +707,8 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
     dfree(ctx->d_Pz);
     dfree(ctx->d_vxcz);
     ctx->have_pz = false;
+    dfree(ctx->d_tP);
+    ctx->have_tp = false;
     dfree(ctx->d_out2);
     dfree(ctx->d_corr);
     dfree(ctx->d_rpart);
@@ -1363,6 +1370,7 @@ int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc) {
   ctx->last_vxc_eval = want_vxc ? 1 : 0;
   ctx->last_graph = false;
   ctx->last_uks = true;
+  ctx->last_fxc = false;
   return enqueue_uks(ctx, functional, want_vxc ? 1 : 0);
 }
 
@@ -1377,6 +1385,114 @@ int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* 
   XCQ_HIP(hipMemcpy2DAsync(vxc_z, (size_t)ldvxc * 8, ctx->d_vxcz, (size_t)M.nbf * 8,
                            (size_t)M.nbf * 8, M.nbf, hipMemcpyDeviceToHost, ctx->stream));
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  return XCQ_SUCCESS;
+}
+
+// ---- FXC contraction (GauXC's eval_fxc_contraction(P, tP)) ------------------
+namespace {
+
+// One RKS FXC contraction on the context's stream (FP64, stored basis
+// planes): per chunk, collocation with gradient planes; the density stage
+// twice (P, then the trial density; factor 2) leaving raw per-point sums;
+// the second-derivative kernel and Z coefficients per point; Z into the
+// chunk's Z plane and its B^T Z + Z^T B into d_vxc; symmetrised.  Every
+// contribution is added (no drop screening), as GauXC does.
+int enqueue_fxc(xcq_context_s* ctx, int func) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  CtxView v = view_of(ctx);
+  v.in_stage = 0;
+  v.tmax = nullptr;
+  v.vxc_drop = 0.0;
+  hipStream_t s = ctx->stream;
+  const size_t blk = (size_t)256 * ctx->pblocks;
+  double* dp = ctx->d_uks;
+  double* dt = dp + blk;
+  double* zc = dt + blk;
+  int rc;
+  if ((rc = launch_fill(s, ctx->d_vxc, ctx->p_total))) return rc;
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
+  for (const Chunk& c : ctx->chunks) {
+    const WorkItem* cw = v.work + c.coll.off;
+    if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
+                                      ctx->max_l)))
+      return rc;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < 4; ++k)
+        if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
+                                 2.0, 0, pass == 0 ? ctx->d_P : ctx->d_tP,
+                                 pass == 0 ? dp : dt)))
+          return rc;
+    if ((rc = launch_fxc_points(v, c.coll, func, gga, dp, dt, zc))) return rc;
+    if ((rc = launch_z_from_coeffs(v, c.coll, gga, zc))) return rc;
+    for (int k = 0; k < 2; ++k)
+      if ((rc = launch_syr2k(v, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+  }
+  if ((rc = launch_symmetrize(s, 1, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off, ctx->d_vxc)))
+    return rc;
+  return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
+                            ctx->d_out2);
+}
+
+}  // namespace
+
+int xcq_upload_trial_density(xcq_context ctx, int32_t nbf, const double* tP, int64_t ldp) {
+  XCQ_REQUIRE(ctx && tP, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "the FXC contraction needs a single-molecule context");
+  const MolInfo& M = ctx->mols[0];
+  XCQ_REQUIRE(nbf == M.nbf && nbf > 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "density dimension does not match the basis");
+  XCQ_REQUIRE(ldp >= nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldp < nbf");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_tP && (rc = dalloc(&ctx->d_tP, (size_t)ctx->p_total))) return rc;
+  XCQ_HIP(hipMemcpy2DAsync(ctx->d_tP, (size_t)nbf * 8, tP, (size_t)ldp * 8, (size_t)nbf * 8, nbf,
+                           hipMemcpyHostToDevice, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_tp = true;
+  return XCQ_SUCCESS;
+}
+
+int xcq_fxc_contraction(xcq_context ctx, int functional) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(functional == XCQ_FUNC_SVWN5 || functional == XCQ_FUNC_PBE ||
+                  functional == XCQ_FUNC_PBE0,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "the FXC contraction needs a single-molecule context");
+  XCQ_REQUIRE(ctx->mols[0].have_density && ctx->have_tp, XCQ_ERROR_STATE,
+              "the FXC contraction needs P (xcq_upload_density) and the trial density "
+              "(xcq_upload_trial_density)");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
+    return rc;
+  ctx->last_want_vxc = 0;
+  ctx->last_vxc_eval = 0;
+  ctx->last_graph = false;
+  ctx->last_uks = false;
+  ctx->last_fxc = true;
+  return enqueue_fxc(ctx, functional);
+}
+
+int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->last_fxc, XCQ_ERROR_STATE, "the last evaluation was not an FXC contraction");
+  const MolInfo& M = ctx->mols[0];
+  XCQ_HIP(hipSetDevice(ctx->device));
+  double out[2];
+  XCQ_HIP(hipMemcpyAsync(out, ctx->d_out2, 16, hipMemcpyDeviceToHost, ctx->stream));
+  if (fxc) {
+    XCQ_REQUIRE(ldfxc >= M.nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldfxc < nbf");
+    XCQ_HIP(hipMemcpy2DAsync(fxc, (size_t)ldfxc * 8, ctx->d_vxc, (size_t)M.nbf * 8,
+                             (size_t)M.nbf * 8, M.nbf, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  if (nel) *nel = out[1];
   return XCQ_SUCCESS;
 }
 

@@ -1125,6 +1125,71 @@ __global__ __launch_bounds__(64) void k_uks_points(const WorkItem* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// FXC contraction tail, RKS (reference_replicated_xc_host_integrator_
+// fxc_contraction.hpp:393-560 with spin_dim 1).  Inputs per point block: the
+// raw sums of the P and trial-density passes with factor 2 (rho = sum B X,
+// grad rho = 2 sum dB X; likewise trho, grad trho).  One thread per point:
+//   tgamma = grad rho . grad trho,
+//   A = v2rho2 trho + 2 v2rhogamma tgamma,
+//   B = 2 (v2rhogamma trho + 2 v2gamma2 tgamma) grad rho + 2 vgamma grad trho
+// (eval_tmat_{lda,gga}_vxc_rks, reference_local_host_work_driver.cxx:
+// 1114-1143), weights folded in, and the Z coefficients of
+// eval_zmat_gga_vxc_rks_ts (:1382-1408): a = A / 2, c = B.  The block's
+// (w rho eps, w rho) sums of the ground-state density go to red.
+__global__ __launch_bounds__(64) void k_fxc_points(const WorkItem* __restrict__ items,
+                                                   const DevTask* __restrict__ tasks,
+                                                   const double* __restrict__ weights, int func,
+                                                   int gga, const double* __restrict__ dp,
+                                                   const double* __restrict__ dt,
+                                                   double* __restrict__ red,
+                                                   double* __restrict__ zc) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  const int64_t b = t.red_off + wi.a;
+  const int p = wi.a * kPtBlock + tid;
+  double sv[4], tv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sv[c] = dp[b * 256 + c * 64 + tid];
+    tv[c] = dt[b * 256 + c * 64 + tid];
+  }
+  double g[3], tg[3], sig = 0., tgam = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g[d] = 2.0 * sv[1 + d];
+    tg[d] = 2.0 * tv[1 + d];
+    sig += g[d] * g[d];
+    tgam += g[d] * tg[d];
+  }
+  double e = 0., n = 0., a = 0., cv[3] = {0., 0., 0.};
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    n = w * sv[0];
+    const XCOut o = dev_eval_functional(func, sv[0], sig);
+    e = w * o.e;
+    double vs, A, Bc;
+    dev_fxc_point(func, sv[0], sig, tv[0], 2.0 * tgam, vs, A, Bc);
+    a = 0.5 * w * A;
+    if (gga) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cv[d] = w * (2.0 * Bc * g[d] + 2.0 * vs * tg[d]);
+    }
+  }
+  zc[b * 256 + tid] = a;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) zc[b * 256 + (1 + d) * 64 + tid] = cv[d];
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (tid == 0) {
+    red[2 * b] = e;
+    red[2 * b + 1] = n;
+  }
+}
+
 // Z = a B + c . grad B of one point block from stored coefficients
 // ([block][4][64]) and the stored FP64 planes (no screening bounds).
 template <bool GGA>
@@ -1147,6 +1212,15 @@ int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, c
   if (r.n == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_uks_points, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
                      v.tasks, v.w, func, gga, ds, dz, v.red, zs, zz);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
+                      const double* dt, double* zc) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_fxc_points, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
+                     v.tasks, v.w, func, gga, dp, dt, v.red, zc);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -315,4 +315,145 @@ __device__ __forceinline__ XCOutPol dev_eval_functional_pol(int func, double ra,
   return o;
 }
 
+// ---- second derivatives (FXC contraction) --------------------------------
+// The trial kernels of the FXC contraction need, at each point,
+//   A  = v2rho2 drho + v2rhosigma dsigma,
+//   Bc = v2rhosigma drho + v2sigma2 dsigma
+// for one direction (drho, dsigma) (reference_local_host_work_driver.cxx:
+// 1114-1143 with drho = trho, dsigma = 2 grad rho . grad trho).  Both are
+// derivatives along that direction of vrho and vsigma, so the unpolarised
+// energy density is evaluated once in a second-order forward-mode number:
+// value, partials in rho (r) and sigma (s), the directional derivative (t)
+// and its partials (rt = A, st = Bc).
+struct HD {
+  double v, r, s, t, rt, st;
+};
+__device__ __forceinline__ HD hdc(double v) { return HD{v, 0., 0., 0., 0., 0.}; }
+// f(a) from f, f' and f'' at a.v.
+__device__ __forceinline__ HD hd_chain(const HD& a, double f0, double f1, double f2) {
+  return HD{f0, f1 * a.r, f1 * a.s, f1 * a.t, f2 * a.r * a.t + f1 * a.rt,
+            f2 * a.s * a.t + f1 * a.st};
+}
+__device__ __forceinline__ HD operator+(const HD& a, const HD& b) {
+  return HD{a.v + b.v, a.r + b.r, a.s + b.s, a.t + b.t, a.rt + b.rt, a.st + b.st};
+}
+__device__ __forceinline__ HD operator-(const HD& a, const HD& b) {
+  return HD{a.v - b.v, a.r - b.r, a.s - b.s, a.t - b.t, a.rt - b.rt, a.st - b.st};
+}
+__device__ __forceinline__ HD operator*(const HD& a, const HD& b) {
+  return HD{a.v * b.v,
+            a.r * b.v + a.v * b.r,
+            a.s * b.v + a.v * b.s,
+            a.t * b.v + a.v * b.t,
+            a.rt * b.v + a.r * b.t + a.t * b.r + a.v * b.rt,
+            a.st * b.v + a.s * b.t + a.t * b.s + a.v * b.st};
+}
+__device__ __forceinline__ HD hd_inv(const HD& a) {
+  const double i = 1.0 / a.v;
+  return hd_chain(a, i, -i * i, 2.0 * i * i * i);
+}
+__device__ __forceinline__ HD operator/(const HD& a, const HD& b) { return a * hd_inv(b); }
+__device__ __forceinline__ HD operator+(const HD& a, double b) {
+  return HD{a.v + b, a.r, a.s, a.t, a.rt, a.st};
+}
+__device__ __forceinline__ HD operator+(double a, const HD& b) { return b + a; }
+__device__ __forceinline__ HD operator-(const HD& a, double b) { return a + (-b); }
+__device__ __forceinline__ HD operator-(double a, const HD& b) {
+  return HD{a - b.v, -b.r, -b.s, -b.t, -b.rt, -b.st};
+}
+__device__ __forceinline__ HD operator*(const HD& a, double b) {
+  return HD{a.v * b, a.r * b, a.s * b, a.t * b, a.rt * b, a.st * b};
+}
+__device__ __forceinline__ HD operator*(double a, const HD& b) { return b * a; }
+__device__ __forceinline__ HD operator/(const HD& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ HD operator/(double a, const HD& b) { return a * hd_inv(b); }
+__device__ __forceinline__ HD hd_sqrt(const HD& a) {
+  const double q = sqrt(a.v);
+  return hd_chain(a, q, 0.5 / q, -0.25 / (q * a.v));
+}
+__device__ __forceinline__ HD hd_cbrt(const HD& a) {
+  const double c = cbrt(a.v), c2 = c * c;
+  return hd_chain(a, c, 1.0 / (3.0 * c2), -2.0 / (9.0 * c2 * a.v));
+}
+__device__ __forceinline__ HD hd_log(const HD& a) {
+  const double i = 1.0 / a.v;
+  return hd_chain(a, log(a.v), i, -i * i);
+}
+__device__ __forceinline__ HD hd_log1p(const HD& a) {
+  const double i = 1.0 / (1.0 + a.v);
+  return hd_chain(a, log1p(a.v), i, -i * i);
+}
+__device__ __forceinline__ HD hd_exp(const HD& a) {
+  const double e = exp(a.v);
+  return hd_chain(a, e, e, e);
+}
+__device__ __forceinline__ HD hd_atan(const HD& a) {
+  const double i = 1.0 / (1.0 + a.v * a.v);
+  return hd_chain(a, atan(a.v), i, -2.0 * a.v * i * i);
+}
+
+// The unpolarised energy densities of the forms at the top of this file.
+__device__ __forceinline__ HD hd_vwn_fit(const HD& x, double A, double b, double c, double x0) {
+  const HD X = x * x + b * x + c;
+  const double X0 = x0 * x0 + b * x0 + c, Q = sqrt(4.0 * c - b * b);
+  const HD at = hd_atan(Q / (2.0 * x + b));
+  return A * (hd_log(x * x / X) + (2.0 * b / Q) * at -
+              (b * x0 / X0) * (hd_log((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
+}
+__device__ __forceinline__ HD hd_pw92(const HD& rs, double a, double a1, double b1, double b2,
+                                      double b3, double b4) {
+  const HD sr = hd_sqrt(rs);
+  const HD q = 2.0 * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
+  return -2.0 * a * (1.0 + a1 * rs) * hd_log1p(1.0 / q);
+}
+__device__ __forceinline__ HD hd_energy(int func, const HD& rho, const HD& sigma) {
+  const double pi = 3.14159265358979323846;
+  const double ax = -0.73855876638202240587;  // -(3/4)(3/pi)^(1/3)
+  HD e = hdc(0.);
+  if (func == XCQ_FUNC_SVWN5) {
+    if (rho.v > 1e-15) {
+      e = ax * rho * hd_cbrt(rho);
+      const HD rs = hd_cbrt((3.0 / (4.0 * pi)) / rho);
+      e = e + rho * hd_vwn_fit(hd_sqrt(rs), 0.0310907, 13.0720, 42.7198, -0.409286);
+    }
+    return e;
+  }
+  const double xs = (func == XCQ_FUNC_PBE0) ? 0.75 : 1.0;
+  if (rho.v > 1e-15) {
+    const double kappa = 0.804, mu = 0.2195149727645171;
+    const double C = 0.026121172985233605;  // 1 / (4 (3 pi^2)^(2/3))
+    const HD r43 = rho * hd_cbrt(rho);
+    const HD s2 = C * sigma / (r43 * r43);
+    const HD F = (1.0 + kappa) - kappa / (1.0 + (mu / kappa) * s2);
+    e = (xs * ax) * r43 * F;
+  }
+  if (rho.v > 1e-12) {
+    const double beta = 0.06672455060314922, gamma = 0.0310906908696549;
+    const double y = beta / gamma;
+    const double D = 0.0634682060977037;  // pi / (16 (3 pi^2)^(1/3))
+    const HD rs = hd_cbrt((3.0 / (4.0 * pi)) / rho);
+    const HD ec = hd_pw92(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+    const HD t2 = D * sigma / (rho * rho * hd_cbrt(rho));
+    const HD Aa = y / (hd_exp((-1.0 / gamma) * ec) - 1.0);
+    const HD At2 = Aa * t2;
+    const HD H = gamma * hd_log1p(y * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+    e = e + rho * (ec + H);
+  }
+  return e;
+}
+
+// vsigma, A and Bc (above) at (rho, sigma) along (drho, dsigma); LDA leaves
+// vs = Bc = 0.  Thresholds and the sigma floor as dev_eval_functional.
+__device__ __forceinline__ void dev_fxc_point(int func, double rho, double sigma, double drho,
+                                              double dsigma, double& vs, double& A, double& Bc) {
+  const bool gga = func != XCQ_FUNC_SVWN5;
+  const double stol = 2.1544346900318838e-43;
+  const HD R{rho, 1., 0., drho, 0., 0.};
+  const HD S{gga ? fmax(sigma, stol * stol) : 0., 0., 1., gga ? dsigma : 0., 0., 0.};
+  const HD e = hd_energy(func, R, S);
+  vs = gga ? e.s : 0.;
+  A = e.rt;
+  Bc = gga ? e.st : 0.;
+}
+
 }  // namespace xcq

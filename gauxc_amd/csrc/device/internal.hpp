@@ -262,6 +262,9 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
 // density passes' raw sums; Z planes from stored coefficients.
 int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* ds,
                       const double* dz, double* zs, double* zz);
+// FXC contraction (RKS): Z coefficients from the P and trial-density passes.
+int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
+                      const double* dt, double* zc);
 int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);

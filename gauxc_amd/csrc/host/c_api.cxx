@@ -494,6 +494,19 @@ void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* st, const GauXCIntegrator in
   });
 }
 
+void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* st, const GauXCIntegrator in,
+                                               const int64_t m, const int64_t n,
+                                               const double* P, const int64_t ldp,
+                                               const double* tP, const int64_t ldtp,
+                                               double* fxc, const int64_t ldfxc) {
+  guarded_void(st, [&] {
+    if (!P || !tP || !fxc) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_fxc_contraction(m, n, P, ldp, tP, ldtp, fxc, ldfxc,
+                                         IntegratorSettingsXC{});
+  });
+}
+
 void* gauxc_integrator_device_context(GauXCStatus* st, const GauXCIntegrator in) {
   return guarded(st, [&]() -> void* {
     auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");

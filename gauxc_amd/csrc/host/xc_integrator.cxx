@@ -271,6 +271,43 @@ void DeviceXCIntegratorImpl::eval_exc_uks(int64_t m, int64_t n, const double* Ps
         "xcq_download_results_uks");
 }
 
+void DeviceXCIntegratorImpl::eval_fxc_contraction(int64_t m, int64_t n, const double* P,
+                                                  int64_t ldp, const double* tP, int64_t ldtp,
+                                                  double* FXC, int64_t ldfxc,
+                                                  const IntegratorSettingsXC&) {
+  // Argument checks of reference_replicated_xc_host_integrator_fxc_contraction.hpp:39-57.
+  auto& s = *s_;
+  const int64_t nbf = s.lb->basis().nbf();
+  if (m != n) GAUXC_GENERIC_EXCEPTION("P/FXC Must Be Square");
+  if (m != nbf) GAUXC_GENERIC_EXCEPTION("P/FXC Must Have Same Dimension as Basis");
+  if (ldp < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDPS");
+  if (ldtp < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDTPS");
+  if (ldfxc < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDFXCS");
+  if (!s.lb->state().modified_weights_are_stored)
+    GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
+  if (s.func->is_polarized())
+    GAUXC_GENERIC_EXCEPTION("RKS evaluation needs an unpolarized functional");
+  if (s.prec != DevicePrecision::FP64)
+    GAUXC_GENERIC_EXCEPTION(
+        "The FXC contraction is provided in FP64 (local work driver Default)");
+  if (!P || !tP || !FXC) GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  s.ensure_static_data();
+  check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), P, ldp), "xcq_upload_density");
+  check(xcq_upload_trial_density(s.ctx, static_cast<int32_t>(m), tP, ldtp),
+        "xcq_upload_trial_density");
+  const auto t1 = std::chrono::steady_clock::now();
+  check(xcq_fxc_contraction(s.ctx, xcq_func_id(*s.func)), "xcq_fxc_contraction");
+  check(xcq_synchronize(s.ctx), "xcq_synchronize");
+  const auto t2 = std::chrono::steady_clock::now();
+  check(xcq_download_fxc(s.ctx, &s.last_nel, FXC, ldfxc), "xcq_download_fxc");
+  const auto t3 = std::chrono::steady_clock::now();
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  s.timer.set("XCIntegrator.HostToDevice_FXC", ms(t0, t1));
+  s.timer.set("XCIntegrator.LocalWork_FXC", ms(t1, t2));
+  s.timer.set("XCIntegrator.DeviceToHostCopy_FXC", ms(t2, t3));
+}
+
 void DeviceXCIntegratorImpl::integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp,
                                            double* N_EL) {
   auto& s = *s_;

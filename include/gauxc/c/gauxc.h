@@ -231,6 +231,16 @@ void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* status, const GauXCIntegrato
                                        const double* density_matrix_z, const int64_t ldp_z,
                                        double* exc, double* vxc_matrix_s, const int64_t vxc_ld_s,
                                        double* vxc_matrix_z, const int64_t vxc_ld_z);
+/* RKS FXC contraction (GauXC include/gauxc/c/xc_integrator.h:351-362):
+ * fxc = d VXC[P + e tP] / de for the trial density tP; FP64 local work
+ * driver, unpolarized functional. */
+void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* status,
+                                               const GauXCIntegrator integrator,
+                                               const int64_t m, const int64_t n,
+                                               const double* density_matrix, const int64_t ldp,
+                                               const double* t_density_matrix,
+                                               const int64_t ldtp, double* fxc,
+                                               const int64_t ldfxc);
 /* Extension: the integrator's device context (an xcq_context of
  * include/gauxc_amd/device.h) for stage timings; NULL before first use. */
 void* gauxc_integrator_device_context(GauXCStatus* status, const GauXCIntegrator integrator);

@@ -92,6 +92,10 @@ class DeviceXCIntegratorImpl {
                         int64_t ldvxcz, double* EXC, const IntegratorSettingsXC& s);
   void eval_exc_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz,
                     int64_t ldpz, double* EXC, const IntegratorSettingsXC& s);
+  /// RKS FXC contraction d VXC[P + e tP] / de (FP64; unpolarized functional).
+  void eval_fxc_contraction(int64_t m, int64_t n, const double* P, int64_t ldp, const double* tP,
+                            int64_t ldtp, double* FXC, int64_t ldfxc,
+                            const IntegratorSettingsXC& s);
   void integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp, double* N_EL);
   const util::Timer& get_timings() const;
   const LoadBalancer& load_balancer() const;
@@ -152,6 +156,15 @@ class XCIntegrator {
     impl().eval_exc_vxc_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(), Pz.rows(),
                             VXCs.data(), VXCs.rows(), VXCz.data(), VXCz.rows(), &e, s);
     return std::make_tuple(e, std::move(VXCs), std::move(VXCz));
+  }
+  /// GauXC XCIntegrator::eval_fxc_contraction(P, tP)
+  /// (include/gauxc/xc_integrator/impl.hpp:104-109), RKS.
+  matrix_type eval_fxc_contraction(const MatrixType& P, const MatrixType& tP,
+                                   const IntegratorSettingsXC& s = IntegratorSettingsXC{}) {
+    matrix_type FXC(P.rows(), P.cols());
+    impl().eval_fxc_contraction(P.rows(), P.cols(), P.data(), P.rows(), tP.data(), tP.rows(),
+                                FXC.data(), FXC.rows(), s);
+    return FXC;
   }
   const util::Timer& get_timings() const { return impl().get_timings(); }
   const LoadBalancer& load_balancer() const { return impl().load_balancer(); }

@@ -225,6 +225,18 @@ int xcq_upload_density_z(xcq_context ctx, int32_t nbf, const double* Pz, int64_t
 int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc);
 int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* vxc_s,
                              double* vxc_z, int64_t ldvxc);
+/* ---- RKS FXC contraction (GauXC eval_fxc_contraction(P, tP):
+ *      reference_replicated_xc_host_integrator_fxc_contraction.hpp:94-600,
+ *      device zmat_fxc.cu; C API gauxc_integrator_eval_fxc_contraction_rks,
+ *      include/gauxc/c/xc_integrator.h:351-362).  P through
+ *      xcq_upload_density, the trial density tP through
+ *      xcq_upload_trial_density (single-molecule contexts).  FXC =
+ *      d VXC[P + e tP] / de is left in the context (FP64, stored basis planes,
+ *      every contribution added) and read back with xcq_download_fxc, which
+ *      also returns the integrated density of P. ---- */
+int xcq_upload_trial_density(xcq_context ctx, int32_t nbf, const double* tP, int64_t ldp);
+int xcq_fxc_contraction(xcq_context ctx, int functional);
+int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc);
 /* Integrated density only (eval_integrate_den). */
 int xcq_integrate_den(xcq_context ctx, double* nel);
 int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc);
