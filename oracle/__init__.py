@@ -67,6 +67,12 @@ def lib():
                                              ctypes.c_int32, vp, vp, vp, vp, vp, vp,
                                              ctypes.c_int64, ctypes.c_int]
         L.oracle_fxc_contraction.restype = ctypes.c_int
+        L.oracle_collocation_hessian.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp]
+        L.oracle_collocation_hessian.restype = None
+        L.oracle_exc_grad.argtypes = [ctypes.c_int, ctypes.c_int32, vp, ctypes.c_int32, vp, vp,
+                                      ctypes.c_int64, ctypes.c_int32, vp, vp, vp, vp, vp,
+                                      ctypes.c_int]
+        L.oracle_exc_grad.restype = ctypes.c_int
         L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.oracle_use_blas.restype = ctypes.c_int
         _lib = L
@@ -229,6 +235,39 @@ def fxc_contraction(func, nbf, shells, P, tP, t, weights=None, nthreads=0):
     return nel.value, fxc
 
 
+def collocation_hessian(shells, shell_list, points):
+    """(10, npts, nbe): B, its gradient (x, y, z) and Hessian (xx, xy, xz,
+    yy, yz, zz) of the shell list at the points."""
+    sl = np.ascontiguousarray(shell_list, dtype=np.int32)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    nbe = 0
+    for s in sl:
+        sh = shells[int(s)]
+        nbe += 2 * sh.l + 1 if sh.pure else (sh.l + 1) * (sh.l + 2) // 2
+    out = np.zeros((10, len(pts), nbe))
+    ptrs = (ctypes.c_void_p * 10)(*[out[q].ctypes.data for q in range(10)])
+    lib().oracle_collocation_hessian(ctypes.addressof(shells), len(sl), sl.ctypes.data, len(pts),
+                                     pts.ctypes.data, ptrs)
+    return out
+
+
+def exc_grad(func, nbf, shells, xyz, P, t, weights=None, nthreads=0):
+    """RKS EXC gradient (natoms, 3) with the grid-weight derivatives."""
+    P = np.asfortranarray(P, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    g = np.zeros((len(xyz), 3))
+    tasks = _task_array(t)
+    rc = lib().oracle_exc_grad(FUNC[func], nbf, ctypes.addressof(shells), len(xyz),
+                               xyz.ctypes.data, P.ctypes.data, nbf, len(t["npts"]),
+                               tasks.ctypes.data, pts.ctypes.data, w.ctypes.data, sl.ctypes.data,
+                               g.ctypes.data, nthreads)
+    assert rc == 0
+    return g
+
+
 # Host BLAS libraries tried, in order, for the CPU baseline (MKL's single
 # dynamic library first, then OpenBLAS and the reference BLAS of the system).
 BLAS_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1",
@@ -260,6 +299,9 @@ def gg_ref():
     L.gg_collocation_deriv1.argtypes = [ctypes.c_int, ctypes.c_ulong, vp, ctypes.c_ulong,
                                         ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
     L.gg_collocation_deriv1.restype = None
+    L.gg_collocation_deriv2.argtypes = [ctypes.c_int, ctypes.c_ulong, vp, ctypes.c_ulong,
+                                        ctypes.c_int, vp, vp, vp, ctypes.c_int] + [vp] * 10
+    L.gg_collocation_deriv2.restype = None
     return L
 
 

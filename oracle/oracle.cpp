@@ -1192,3 +1192,308 @@ int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, co
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// EXC nuclear gradient (RKS) with the grid-weight derivatives: GauXC's host
+// driver reference_replicated_xc_host_integrator_exc_grad.hpp:106-600
+// (include_weight_derivatives = true) and the SSF weight-derivative
+// contraction of weights.cxx:806-990.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Monomial x^a y^b z^c times the radial part, with gradient and Hessian.
+// With R(r^2), R1 = dR/d(r^2) * 2 = -2 sum a c e, R2 = 4 sum a^2 c e:
+//   d_i f = d_i m R + m x_i R1,
+//   d_i d_j f = d_i d_j m R + (d_i m x_j + d_j m x_i) R1 + m (delta_ij R1 + x_i x_j R2).
+// out: v, gx, gy, gz, hxx, hxy, hxz, hyy, hyz, hzz.
+void monomial_hess(int a, int b, int c, double x, double y, double z, double R, double R1,
+                   double R2, double* out) {
+  const int e[3] = {a, b, c};
+  const double r[3] = {x, y, z};
+  auto p = [](double t, int n) { return n < 0 ? 0. : ipow(t, n); };
+  // d^k/dt^k t^n = n (n-1) .. t^(n-k)
+  auto d1 = [&](int d) {
+    double v = 1.;
+    for (int k = 0; k < 3; ++k) v *= (k == d) ? (e[k] ? e[k] * p(r[k], e[k] - 1) : 0.) : p(r[k], e[k]);
+    return v;
+  };
+  auto d2 = [&](int d, int f) {
+    double v = 1.;
+    for (int k = 0; k < 3; ++k) {
+      const int n = e[k] - (k == d) - (k == f);
+      double coef = 1.;
+      for (int q = 0; q < (k == d) + (k == f); ++q) coef *= (e[k] - q);
+      v *= n < 0 ? 0. : coef * p(r[k], n);
+    }
+    return v;
+  };
+  const double m = p(x, a) * p(y, b) * p(z, c);
+  out[0] = m * R;
+  double g[3];
+  for (int d = 0; d < 3; ++d) {
+    g[d] = d1(d);
+    out[1 + d] = g[d] * R + m * r[d] * R1;
+  }
+  int k = 4;
+  for (int d = 0; d < 3; ++d)
+    for (int f = d; f < 3; ++f, ++k)
+      out[k] = d2(d, f) * R + (g[d] * r[f] + g[f] * r[d]) * R1 +
+               m * ((d == f ? R1 : 0.) + r[d] * r[f] * R2);
+}
+
+// SSF weight-derivative contraction of one task (weights.cxx:806-990):
+// d/dR_B of sum_i w_i f_i with w_i = w0_i P_A(r_i) / sum_C P_C(r_i), points
+// moving with their parent atom A (its row follows by translational
+// invariance).  wf_i = w_i f_i.  rab = atom distance matrix.
+void ssf_grad_contract(int32_t natoms, const double* xyz, const double* rab, const oracle_task& t,
+                       const double* pts, const double* wf, double* grad) {
+  const double bound = kSSFa - 1e-4;
+  // d ln s(mu) / d mu for s(mu) = (1 - g(mu / a)) / 2, g the Frisch
+  // polynomial: 1 - g(v) = (1 - v)^4 (5 v^3 + 20 v^2 + 29 v + 16) / 16 and
+  // g'(v) = 35 (1 - v^2)^3 / 16, so t = 35 (1 + v)^3 / ((mu - a)(5 v^3 + 20 v^2 + 29 v + 16)).
+  auto tfr = [](double mu) {
+    const double v = mu / kSSFa;
+    return 35. * (1. + v) * (1. + v) * (1. + v) /
+           ((mu - kSSFa) * (((5. * v + 20.) * v + 29.) * v + 16.));
+  };
+  std::vector<double> part(natoms), dist(natoms);
+  const int32_t A = t.iparent;
+  auto X = [&](int at, int d) { return xyz[3 * at + d]; };
+  for (int32_t i = 0; i < t.npts; ++i) {
+    const double f = wf[i];
+    if (std::fabs(f) < 1e-12) continue;
+    const double* r = pts + 3 * i;
+    for (int at = 0; at < natoms; ++at) {
+      const double dx = r[0] - X(at, 0), dy = r[1] - X(at, 1), dz = r[2] - X(at, 2);
+      dist[at] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    if (dist[A] < 0.5 * (1. - kSSFa) * t.dist_nearest) continue;
+    std::fill(part.begin(), part.end(), 1.);
+    for (int a = 0; a < natoms; ++a)
+      for (int b = 0; b < a; ++b) {
+        if (!(part[a] > kSSFTol || part[b] > kSSFTol)) continue;
+        const double mu = (dist[a] - dist[b]) / rab[(size_t)b + (size_t)a * natoms];
+        if (mu <= -kSSFa) {
+          part[b] = 0.;
+        } else if (mu >= kSSFa) {
+          part[a] = 0.;
+        } else {
+          const double g = 0.5 * (1. - frisch_g(mu));
+          part[a] *= g;
+          part[b] *= 1. - g;
+        }
+      }
+    double sum = 0.;
+    for (int a = 0; a < natoms; ++a) sum += part[a];
+    for (int B = 0; B < natoms; ++B) {
+      if (B == A) continue;
+      double gb[3] = {0., 0., 0.};
+      const double rAB = rab[(size_t)B + (size_t)A * natoms];
+      const double muAB = (dist[A] - dist[B]) / rAB;
+      if (std::fabs(muAB) < bound) {
+        const double c1 = tfr(muAB) / rAB * (part[A] - sum) / sum * f / dist[B];
+        for (int d = 0; d < 3; ++d)
+          gb[d] = c1 * ((X(B, d) - r[d]) + muAB * (X(B, d) - X(A, d)) / rAB * dist[B]);
+      }
+      if (part[B] > kSSFTol) {
+        for (int C = 0; C < natoms; ++C) {
+          if (C == B) continue;
+          const double rBC = rab[(size_t)C + (size_t)B * natoms];
+          const double muBC = (dist[B] - dist[C]) / rBC;
+          if (std::fabs(muBC) >= bound) continue;
+          const double c = part[B] * tfr(muBC) / rBC / sum * f;
+          for (int d = 0; d < 3; ++d)
+            gb[d] -= c * ((X(B, d) - r[d]) / dist[B] - muBC * (X(B, d) - X(C, d)) / rBC);
+          if (C != A)
+            for (int d = 0; d < 3; ++d) {
+              const double gc = c * ((X(C, d) - r[d]) / dist[C] + muBC * (X(C, d) - X(B, d)) / rBC);
+              grad[3 * C + d] += gc;
+              grad[3 * A + d] -= gc;
+            }
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        grad[3 * B + d] += gb[d];
+        grad[3 * A + d] -= gb[d];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_collocation_hessian(const oracle_shell* shells, int32_t nlist,
+                                const int32_t* shell_list, int32_t npts, const double* points,
+                                double* const* out) {
+  int32_t nbe = 0;
+  for (int32_t i = 0; i < nlist; ++i) {
+    const auto& sh = shells[shell_list[i]];
+    nbe += sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2;
+  }
+  const auto& sph = sph_tables();
+  double mv[10];
+  for (int32_t ip = 0; ip < npts; ++ip) {
+    int32_t off = 0;
+    for (int32_t i = 0; i < nlist; ++i) {
+      const auto& sh = shells[shell_list[i]];
+      const double x = points[3 * ip] - sh.origin[0];
+      const double y = points[3 * ip + 1] - sh.origin[1];
+      const double z = points[3 * ip + 2] - sh.origin[2];
+      const double r2 = x * x + y * y + z * z;
+      double R = 0., R1 = 0., R2 = 0.;
+      for (int k = 0; k < sh.nprim; ++k) {
+        const double e = sh.coeff[k] * std::exp(-sh.alpha[k] * r2);
+        R += e;
+        R1 += -2. * sh.alpha[k] * e;
+        R2 += 4. * sh.alpha[k] * sh.alpha[k] * e;
+      }
+      const int l = sh.l;
+      const size_t row = (size_t)ip * nbe;
+      if (!sh.pure) {
+        int ic = 0;
+        for (int a = l; a >= 0; --a)
+          for (int bb = l - a; bb >= 0; --bb, ++ic) {
+            monomial_hess(a, bb, l - a - bb, x, y, z, R, R1, R2, mv);
+            for (int q = 0; q < 10; ++q) out[q][row + off + ic] = mv[q];
+          }
+        off += ic;
+      } else {
+        for (int m = 0; m < 2 * l + 1; ++m) {
+          double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+          for (const auto& tm : sph.t[l][m]) {
+            monomial_hess(tm.a, tm.b, tm.c, x, y, z, R, R1, R2, mv);
+            for (int q = 0; q < 10; ++q) acc[q] += tm.coef * mv[q];
+          }
+          for (int q = 0; q < 10; ++q) out[q][row + off + m] = acc[q];
+        }
+        off += 2 * l + 1;
+      }
+    }
+  }
+}
+
+int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
+                    const double* xyz, const double* P, int64_t ldp, int32_t ntasks,
+                    const oracle_task* tasks, const double* points, const double* weights,
+                    const int32_t* shell_lists, double* grad, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int k = 0; k < 3 * natoms; ++k) grad[k] = 0.;
+  std::vector<double> rab((size_t)natoms * natoms);
+  for (int a = 0; a < natoms; ++a)
+    for (int b = 0; b < natoms; ++b) {
+      const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                   dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+      rab[(size_t)a + (size_t)b * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0), centre(nsh_total, -1);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+    // BasisSetMap: the atom at exactly the shell's origin (basisset_map.hpp:69-77).
+    for (int a = 0; a < natoms && centre[s] < 0; ++a)
+      if (xyz[3 * a] == sh.origin[0] && xyz[3 * a + 1] == sh.origin[1] &&
+          xyz[3 * a + 2] == sh.origin[2])
+        centre[s] = a;
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> pl[10], X[4], sub, wf, vr, vs, rho, sig, eps, gr[3];
+    std::vector<double> gloc(3 * (size_t)natoms, 0.);
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      double* o[10];
+      for (int q = 0; q < 10; ++q) {
+        pl[q].assign((size_t)npts * nbe, 0.);
+        o[q] = pl[q].data();
+      }
+      oracle_collocation_hessian(shells, t.nshells, sl, npts, pts, o);
+      // X = 2 P_sub B and, for GGA, X_j = 2 P_sub dB/dj (:356-365).
+      const int nx = gga ? 4 : 1;
+      for (int q = 0; q < nx; ++q) xmat(pl[q], P, ldp, bf, npts, 2.0, sub, X[q]);
+      rho.assign(npts, 0.);
+      sig.assign(npts, 0.);
+      for (int d = 0; d < 3; ++d) gr[d].assign(npts, 0.);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t off = (size_t)p * nbe;
+        double r = 0., g[3] = {0., 0., 0.};
+        for (int32_t k = 0; k < nbe; ++k) {
+          r += pl[0][off + k] * X[0][off + k];
+          for (int d = 0; d < 3; ++d) g[d] += pl[1 + d][off + k] * X[0][off + k];
+        }
+        rho[p] = r;
+        for (int d = 0; d < 3; ++d) gr[d][p] = 2. * g[d];
+        sig[p] = gr[0][p] * gr[0][p] + gr[1][p] * gr[1][p] + gr[2][p] * gr[2][p];
+      }
+      eps.resize(npts);
+      vr.resize(npts);
+      vs.assign(npts, 0.);
+      oracle_functional(func, npts, rho.data(), gga ? sig.data() : nullptr, eps.data(), vr.data(),
+                        gga ? vs.data() : nullptr);
+      // Grid-weight part with w_i eps_i rho_i (:418-425).
+      wf.resize(npts);
+      for (int32_t p = 0; p < npts; ++p) wf[p] = eps[p] * rho[p] * w[p];
+      ssf_grad_contract(natoms, xyz, rab.data(), t, pts, wf.data(), gloc.data());
+      // Basis-function part, shell by shell (:427-590); shells on the parent
+      // atom move with the grid and drop out.
+      int32_t boff = 0;
+      for (int32_t k = 0; k < t.nshells; ++k) {
+        const int32_t s = sl[k];
+        const int32_t n = first_ao[s + 1] - first_ao[s];
+        const int32_t at = centre[s];
+        if (at == t.iparent || at < 0) {
+          boff += n;
+          continue;
+        }
+        double g[3] = {0., 0., 0.};
+        for (int32_t mu = boff; mu < boff + n; ++mu)
+          for (int32_t p = 0; p < npts; ++p) {
+            const size_t i = (size_t)p * nbe + mu;
+            const double wv = w[p] * vr[p], xn = X[0][i];
+            for (int d = 0; d < 3; ++d) g[d] += wv * xn * pl[1 + d][i];
+            if (gga) {
+              const double wg = w[p] * vs[p];
+              const double dn[3] = {gr[0][p], gr[1][p], gr[2][p]};
+              const double h[3][3] = {{pl[4][i], pl[5][i], pl[6][i]},
+                                      {pl[5][i], pl[7][i], pl[8][i]},
+                                      {pl[6][i], pl[8][i], pl[9][i]}};
+              const double d11 = dn[0] * X[1][i] + dn[1] * X[2][i] + dn[2] * X[3][i];
+              for (int d = 0; d < 3; ++d) {
+                const double hd = h[d][0] * dn[0] + h[d][1] * dn[1] + h[d][2] * dn[2];
+                g[d] += 2. * wg * (xn * hd + pl[1 + d][i] * d11);
+              }
+            }
+          }
+        for (int d = 0; d < 3; ++d) {
+          gloc[3 * at + d] += -2. * g[d];
+          gloc[3 * t.iparent + d] -= -2. * g[d];
+        }
+        boff += n;
+      }
+    }
+#pragma omp critical
+    for (int k = 0; k < 3 * natoms; ++k) grad[k] += gloc[k];
+  }
+  return 0;
+}
+
+}  // extern "C"

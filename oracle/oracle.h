@@ -95,6 +95,22 @@ int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, co
                            const double* weights, const int32_t* shell_lists, double* nel,
                            double* fxc, int64_t ldfxc, int nthreads);
 
+/* Collocation with gradient and Hessian of one shell list: out[0..9] =
+ * B, dB/dx, dB/dy, dB/dz, d2B/dxx, dxy, dxz, dyy, dyz, dzz, each [npts][nbe]
+ * (gau2grid's deriv2 semantics). */
+void oracle_collocation_hessian(const oracle_shell* shells, int32_t nlist,
+                                const int32_t* shell_list, int32_t npts, const double* points,
+                                double* const* out);
+
+/* RKS EXC nuclear gradient including the SSF grid-weight derivatives
+ * (GauXC eval_exc_grad with include_weight_derivatives): grad[3 * natoms],
+ * atom-major (x, y, z).  Shells are assigned to the atom at exactly their
+ * origin. */
+int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
+                    const double* xyz, const double* P, int64_t ldp, int32_t ntasks,
+                    const oracle_task* tasks, const double* points, const double* weights,
+                    const int32_t* shell_lists, double* grad, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,10 @@ Each check is chosen so that a plausible mistake fails it:
     VXC_s / VXC_z are the finite-difference derivatives of the UKS EXC;
   * FXC contraction: GauXC's benzene SVWN5 / PBE0 FXC fixtures
     (tests/2nd_derivative_test.cxx:121-126); the second derivatives by
-    differences of the first; FXC as the derivative of VXC along tP.
+    differences of the first; FXC as the derivative of VXC along tP;
+  * EXC gradient: Hessian collocation against gau2grid's deriv2 and central
+    differences; GauXC's benzene EXC_GRAD fixtures (with grid-weight
+    derivatives); translational invariance; dEXC/dR by central differences.
 """
 import ctypes
 
@@ -377,6 +380,97 @@ def test_oracle_fxc_is_the_derivative_of_vxc(orc):
     _, _, vm = orc.exc_vxc("PBE0", bs.nbf(), sh, P - h * tP, t, weights=w)
     fd = (vp - vm) / (2 * h)
     assert np.abs(fxc - fd).max() <= 1e-6 * np.abs(fxc).max()
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_oracle_collocation_hessian(orc, pure):
+    """Hessian collocation: against GauXC's own gau2grid (deriv2, compiled
+    under oracle/_ref) where built, and always against central differences
+    of the pinned gradient collocation; value and gradient equal the
+    gradient collocation's."""
+    rng = np.random.default_rng(8)
+    shells = _random_shells(orc, rng, 4, pure)
+    pts = rng.uniform(-2.0, 2.0, (41, 3))
+    gg = orc.gg_ref()
+    h = 1e-5
+    for l in range(5):
+        out = orc.collocation_hessian(shells, [l], pts)
+        base = orc.collocation(shells, [l], pts, deriv=True)
+        for k in range(4):
+            np.testing.assert_allclose(out[k], base[k], rtol=1e-13, atol=1e-15)
+        # d/dx_j of the gradient: column (i, j) of the Hessian.
+        comp = {(0, 0): 4, (0, 1): 5, (0, 2): 6, (1, 1): 7, (1, 2): 8, (2, 2): 9}
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = h
+            gp = orc.collocation(shells, [l], pts + e, deriv=True)
+            gm = orc.collocation(shells, [l], pts - e, deriv=True)
+            for i in range(3):
+                fd = (gp[1 + i] - gm[1 + i]) / (2 * h)
+                hij = out[comp[(min(i, j), max(i, j))]]
+                assert np.abs(hij - fd).max() <= 1e-7 * max(1.0, np.abs(hij).max())
+        if gg is not None:
+            s = shells[l]
+            nc = 2 * l + 1 if pure else (l + 1) * (l + 2) // 2
+            ref = np.zeros((10, nc, len(pts)))
+            coeff = np.array(s.coeff[:3])
+            alpha = np.array(s.alpha[:3])
+            center = np.array(s.origin[:3])
+            P = ctypes.c_void_p
+            gg.gg_collocation_deriv2(l, len(pts), np.ascontiguousarray(pts).ctypes.data, 3, 3,
+                                     coeff.ctypes.data, alpha.ctypes.data, center.ctypes.data,
+                                     orc.GG_SPHERICAL_CCA if pure else orc.GG_CARTESIAN_CCA,
+                                     *[P(ref[k].ctypes.data) for k in range(10)])
+            for k in range(10):
+                assert np.abs(out[k] - ref[k].T).max() <= 1e-12 * max(1.0, np.abs(ref[k]).max())
+
+
+@pytest.mark.parametrize("tag,func", [("svwn5", "SVWN5"), ("pbe0", "PBE0")])
+def test_oracle_exc_grad_matches_gauxc_benzene_fixture(orc, tag, func):
+    """GauXC's gradient check (tests/xc_integrator.cxx:278-289): the full
+    gradient with grid-weight derivatives (the fixture's EXC_GRAD carries
+    includes_weight_derivatives = 1), ||dG||_F / sqrt(3 natoms) < 1e-8."""
+    d, mol, bs, lb = benzene_setup(tag)
+    t = lb.get_tasks()
+    w = orc.ssf_weights(d["xyz"], t)
+    g = orc.exc_grad(func, bs.nbf(), orc.shells_from(bs.xcq_shells()), d["xyz"], d["density"],
+                     t, weights=w)
+    ref = d["exc_grad"].reshape(-1, 3)
+    assert np.linalg.norm(g - ref) / np.sqrt(g.size) < 1e-12
+    # Translational invariance: the rows sum to zero.
+    assert np.abs(g.sum(axis=0)).max() < 1e-12
+
+
+def _dimer_system(gx, orc, xyz, Z):
+    l, pure, alpha, coeff, origin = gx.workloads.even_tempered_basis(Z, xyz,
+                                                                     gx.workloads.BASIS_C0)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=EPS)
+    lb = gx.workloads.build_tasks(gx.Molecule(Z, xyz), bs, grid="GM3")
+    t = lb.get_tasks()
+    return bs, t, orc.ssf_weights(xyz, t)
+
+
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
+def test_oracle_exc_grad_is_the_derivative_of_exc(gx, orc, func):
+    """dEXC/dR_A by central differences of the whole evaluation (basis, grid
+    and SSF weights moving with the atoms) on a water dimer."""
+    Z, xyz = gx.workloads.water_cluster(2, 5.0, 4)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    bs, t, w = _dimer_system(gx, orc, xyz, Z)
+    P = gx.workloads.density(bs.nbf(), 10, 21)
+    g = orc.exc_grad(func, bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P, t, weights=w)
+    assert np.abs(g.sum(axis=0)).max() < 1e-10
+    h = 1e-4
+    for atom, comp in ((0, 0), (4, 2)):
+        e = []
+        for sgn in (1, -1):
+            x2 = xyz.copy()
+            x2[atom, comp] += sgn * h
+            b2, t2, w2 = _dimer_system(gx, orc, x2, Z)
+            e.append(orc.exc_vxc(func, b2.nbf(), orc.shells_from(b2.xcq_shells()), P, t2,
+                                 weights=w2)[0])
+        fd = (e[0] - e[1]) / (2 * h)
+        assert abs(g[atom, comp] - fd) <= 1e-6 * max(abs(fd), 1e-3), (atom, comp, g[atom, comp], fd)
 
 
 def test_oracle_mlp_gradient_matches_finite_differences():
