@@ -322,6 +322,20 @@ class XCIntegrator(_Handle):
         st.raise_if()
         return exc.value
 
+    def eval_exc_grad(self, P, include_weight_derivatives=True):
+        """RKS EXC nuclear gradient, (natoms, 3), as GauXC's eval_exc_grad(P)
+        (with the grid-weight derivatives unless include_weight_derivatives
+        is False)."""
+        P = self._mat(P)
+        m, n = P.shape
+        g = np.zeros(3 * self.natoms)
+        st = N.Status()
+        fn = (N.host_lib().gauxc_integrator_eval_exc_grad_rks if include_weight_derivatives
+              else N.host_lib().gauxc_integrator_eval_exc_grad_hf_rks)
+        fn(st.ptr, self.h, m, n, P.ctypes.data, m, g.ctypes.data)
+        st.raise_if()
+        return g.reshape(-1, 3)
+
     def eval_fxc_contraction(self, P, tP):
         """RKS FXC contraction d VXC[P + e tP] / de, as GauXC's
         eval_fxc_contraction(P, tP)."""
@@ -365,6 +379,7 @@ class XCIntegratorFactory:
         st.raise_if()
         integ = XCIntegrator(h)
         integ._keep = (func, lb)
+        integ.natoms = lb._keep[0].natoms()
         return integ
 
 
@@ -460,6 +475,24 @@ class DeviceContext:
             vs.ctypes.data if want_vxc else None, vz.ctypes.data if want_vxc else None,
             self.nbf), "xcq_download_results_uks")
         return exc.value, nel.value, vs, vz
+
+    def exc_grad(self, func, xyz, weight_derivs=True):
+        """RKS EXC gradient of the uploaded density; xyz (natoms, 3) in the
+        order of the tasks' parent indices."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._grad_natoms = len(xyz)
+        N.xcq_check(self.lib.xcq_exc_grad(self.h, N.XCQ_FUNC[func], len(xyz), xyz.ctypes.data,
+                                          int(bool(weight_derivs))), "xcq_exc_grad")
+
+    def results_grad(self):
+        """(EXC, NEL, gradient (natoms, 3)) of the last EXC gradient."""
+        exc = ctypes.c_double(0.)
+        nel = ctypes.c_double(0.)
+        g = np.zeros((getattr(self, "_grad_natoms", 0), 3))
+        N.xcq_check(self.lib.xcq_download_exc_grad(self.h, ctypes.addressof(exc),
+                                                   ctypes.addressof(nel), g.ctypes.data),
+                    "xcq_download_exc_grad")
+        return exc.value, nel.value, g
 
     def upload_trial_density(self, tP):
         """Trial density of the FXC contraction."""

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -193,6 +194,14 @@ struct xcq_context_s {
   double* d_tP = nullptr;   // FXC contraction: trial density (single molecule)
   bool have_tp = false;
   bool last_fxc = false;    // the last evaluation was an FXC contraction (in d_vxc)
+  std::vector<double> sh_origin;  // shell centres (3 per shell), host copy
+  // EXC gradient: atoms, shell -> atom map, nearest-neighbour distances, result.
+  int32_t grad_natoms = 0;
+  double* d_gxyz = nullptr;
+  double* d_dnear = nullptr;
+  double* d_grad = nullptr;
+  int32_t* d_shell_atom = nullptr;
+  bool last_grad = false;   // the last evaluation was an EXC gradient
   double* d_zc = nullptr;   // Z coefficients per point block (in-stage mode 3), lazily
   int64_t pblocks = 0;      // point blocks of the uploaded tasks
   double* d_out2 = nullptr;
@@ -270,6 +279,10 @@ struct xcq_context_s {
     dfree(d_Pz);
     dfree(d_vxcz);
     dfree(d_tP);
+    dfree(d_gxyz);
+    dfree(d_dnear);
+    dfree(d_grad);
+    dfree(d_shell_atom);
     dfree(d_out2);
     dfree(d_corr);
     dfree(d_rpart);
@@ -511,6 +524,7 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
   ctx->last_want_vxc = want_vxc;
   ctx->last_uks = false;
   ctx->last_fxc = false;
+  ctx->last_grad = false;
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
   if (precision == XCQ_PREC_MIXED && func != 0) want_vxc = 1;
@@ -644,6 +658,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
   ctx->sh_size.assign(nshells, 0);
   ctx->sh_first.assign(nshells, 0);
   ctx->sh_mol.assign(nshells, 0);
+  ctx->sh_origin.assign(3 * (size_t)nshells, 0.);
   int32_t i = 0;
   int64_t p_total = 0;
   int32_t max_nbf = 0, max_l = 0;
@@ -663,7 +678,7 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
         d.alpha[q] = s.alpha[q];
         d.coeff[q] = s.coeff[q];
       }
-      for (int q = 0; q < 3; ++q) d.O[q] = s.origin[q];
+      for (int q = 0; q < 3; ++q) d.O[q] = ctx->sh_origin[3 * (size_t)i + q] = s.origin[q];
       d.nprim = s.nprim;
       d.l = s.l;
       d.pure = s.pure;
@@ -1371,6 +1386,7 @@ int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc) {
   ctx->last_graph = false;
   ctx->last_uks = true;
   ctx->last_fxc = false;
+  ctx->last_grad = false;
   return enqueue_uks(ctx, functional, want_vxc ? 1 : 0);
 }
 
@@ -1476,6 +1492,7 @@ int xcq_fxc_contraction(xcq_context ctx, int functional) {
   ctx->last_graph = false;
   ctx->last_uks = false;
   ctx->last_fxc = true;
+  ctx->last_grad = false;
   return enqueue_fxc(ctx, functional);
 }
 
@@ -1492,6 +1509,136 @@ int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc) {
                              (size_t)M.nbf * 8, M.nbf, hipMemcpyDeviceToHost, ctx->stream));
   }
   XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  if (nel) *nel = out[1];
+  return XCQ_SUCCESS;
+}
+
+// ---- EXC gradient (GauXC eval_exc_grad(P)) ----------------------------------
+namespace {
+
+// One RKS EXC gradient on the context's stream (FP64, stored basis planes):
+// per chunk, collocation (with gradient planes for GGAs); the density stage
+// (factor 2) leaving raw per-point sums; per-point Za coefficients and
+// w eps rho; Za into plane 4; for GGAs X = 2 P_sub B into plane 1; Y =
+// 2 P_sub Za into plane 2; the basis-function part and the grid-weight part
+// into d_grad (the latter only with weight_derivs).
+int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  CtxView v = view_of(ctx);
+  v.in_stage = 0;
+  v.tmax = nullptr;
+  v.vxc_drop = 0.0;
+  hipStream_t s = ctx->stream;
+  const size_t blk = (size_t)256 * ctx->pblocks;
+  double* dp = ctx->d_uks;
+  double* coef = dp + blk;
+  double* wf = coef + blk;
+  int rc;
+  if ((rc = launch_fill(s, ctx->d_grad, 3 * (int64_t)ctx->grad_natoms))) return rc;
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
+  for (const Chunk& c : ctx->chunks) {
+    const WorkItem* cw = v.work + c.coll.off;
+    if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
+                                      ctx->max_l)))
+      return rc;
+    for (int k = 0; k < 4; ++k)
+      if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
+                               2.0, 0, ctx->d_P, dp)))
+        return rc;
+    if ((rc = launch_grad_points(v, c.coll, func, gga, dp, coef, wf))) return rc;
+    if ((rc = launch_z_from_coeffs(v, c.coll, gga, coef))) return rc;
+    if (gga && (rc = launch_plane_product(v, c.coll, 0, 1))) return rc;
+    if ((rc = launch_plane_product(v, c.coll, 4, 2))) return rc;
+    if ((rc = launch_grad_contract(v, c.coll, gga, ctx->d_shell_atom, coef, weight_derivs,
+                                   ctx->d_grad)))
+      return rc;
+    if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
+                                               ctx->d_dnear, wf, ctx->d_grad)))
+      return rc;
+  }
+  return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
+                            ctx->d_out2);
+}
+
+}  // namespace
+
+int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
+                 int weight_derivs) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(functional == XCQ_FUNC_SVWN5 || functional == XCQ_FUNC_PBE ||
+                  functional == XCQ_FUNC_PBE0,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
+  XCQ_REQUIRE(natoms > 0 && xyz, XCQ_ERROR_INVALID_ARGUMENT, "no atoms");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "the EXC gradient needs a single-molecule context");
+  XCQ_REQUIRE(ctx->mols[0].have_density, XCQ_ERROR_STATE,
+              "the EXC gradient needs P (xcq_upload_density)");
+  for (const DevTask& d : ctx->h_tasks)
+    XCQ_REQUIRE(d.iparent >= 0 && d.iparent < natoms, XCQ_ERROR_INVALID_ARGUMENT,
+                "a task's parent atom is outside the atom list");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  // Shell -> atom: the atom at exactly the shell's centre, -1 if none
+  // (BasisSetMap, basisset_map.hpp:69-77).
+  const size_t nsh = ctx->sh_size.size();
+  std::vector<int32_t> sa(nsh, -1);
+  for (size_t i = 0; i < nsh; ++i)
+    for (int32_t a = 0; a < natoms && sa[i] < 0; ++a)
+      if (xyz[3 * a] == ctx->sh_origin[3 * i] && xyz[3 * a + 1] == ctx->sh_origin[3 * i + 1] &&
+          xyz[3 * a + 2] == ctx->sh_origin[3 * i + 2])
+        sa[i] = a;
+  // Nearest-neighbour distance of each atom (MolMeta::dist_nearest); a
+  // single atom has none (its points all keep their weights).
+  std::vector<double> dn(natoms, std::numeric_limits<double>::infinity());
+  for (int32_t a = 0; a < natoms; ++a)
+    for (int32_t b = 0; b < natoms; ++b)
+      if (a != b) {
+        const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                     dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+        dn[a] = std::min(dn[a], std::sqrt(dx * dx + dy * dy + dz * dz));
+      }
+  int rc;
+  if (natoms != ctx->grad_natoms) {
+    dfree(ctx->d_gxyz);
+    dfree(ctx->d_dnear);
+    dfree(ctx->d_grad);
+    ctx->grad_natoms = 0;
+    if ((rc = dalloc(&ctx->d_gxyz, 3 * (size_t)natoms)) ||
+        (rc = dalloc(&ctx->d_dnear, (size_t)natoms)) ||
+        (rc = dalloc(&ctx->d_grad, 3 * (size_t)natoms)))
+      return rc;
+    ctx->grad_natoms = natoms;
+  }
+  dfree(ctx->d_shell_atom);
+  if ((rc = dalloc(&ctx->d_shell_atom, std::max<size_t>(nsh, 1)))) return rc;
+  XCQ_HIP(hipMemcpy(ctx->d_gxyz, xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));
+  XCQ_HIP(hipMemcpy(ctx->d_dnear, dn.data(), 8 * (size_t)natoms, hipMemcpyHostToDevice));
+  XCQ_HIP(hipMemcpy(ctx->d_shell_atom, sa.data(), 4 * nsh, hipMemcpyHostToDevice));
+  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
+    return rc;
+  ctx->last_want_vxc = 0;
+  ctx->last_vxc_eval = 0;
+  ctx->last_graph = false;
+  ctx->last_uks = false;
+  ctx->last_fxc = false;
+  ctx->last_grad = true;
+  return enqueue_grad(ctx, functional, weight_derivs ? 1 : 0);
+}
+
+int xcq_download_exc_grad(xcq_context ctx, double* exc, double* nel, double* grad) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->last_grad, XCQ_ERROR_STATE, "the last evaluation was not an EXC gradient");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  double out[2];
+  XCQ_HIP(hipMemcpyAsync(out, ctx->d_out2, 16, hipMemcpyDeviceToHost, ctx->stream));
+  if (grad)
+    XCQ_HIP(hipMemcpyAsync(grad, ctx->d_grad, 24 * (size_t)ctx->grad_natoms,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  if (exc) *exc = out[0];
   if (nel) *nel = out[1];
   return XCQ_SUCCESS;
 }

@@ -1,0 +1,479 @@
+// EXC nuclear gradient, RKS, FP64 (GauXC eval_exc_grad with
+// include_weight_derivatives: reference_replicated_xc_host_integrator_
+// exc_grad.hpp:106-600, weights.cxx:806-990; device counterparts
+// increment_exc_grad.cu and cuda_ssf_1d.cu).
+//
+// GauXC contracts, per task, point and function mu on an atom other than the
+// task's parent,
+//   g_mu = w vrho X_mu grad B_mu
+//        + 2 w vgamma [ X_mu (H_mu grad rho) + grad B_mu (grad rho . Xg_mu) ]
+// with X = 2 P_sub B, Xg_j = 2 P_sub dB/dj and H_mu the Hessian of B_mu,
+// and adds -2 g to the function's atom and +2 g to the parent (the grid
+// moves with the parent).  Here the first and third terms are one product:
+//   w vrho X + 2 w vgamma grad rho . Xg = 2 P_sub Za,
+//   Za = w vrho B + 2 w vgamma grad rho . grad B,
+// so a task needs two plane products (X = 2 P_sub B for the Hessian term of
+// GGAs, Y = 2 P_sub Za) instead of four, and
+//   g_mu = grad B_mu . Y_mu + X_mu (H_mu c),  c = 2 w vgamma grad rho.
+// grad B and H c are evaluated where they are used (k_grad_contract), so the
+// chunk scratch's five planes suffice: B, grad B (collocation), Za (plane 4),
+// then X over plane 1 and Y over plane 2 once Za is formed.
+// The grid-weight part contracts the SSF weight derivatives with w eps rho
+// per point (k_ssf_grad).
+#include "internal.hpp"
+#include "functional.hpp"
+#include "shell_eval.hpp"
+
+namespace xcq {
+
+// ---- per-point coefficients ------------------------------------------------
+// From the raw sums of the P pass (factor 2: rho = sum B X, grad rho =
+// 2 sum dB X): coef[block][4][64] = (w vrho, 2 w vsigma grad rho) -- the Za
+// coefficients, whose gradient part is also the Hessian term's c -- and
+// wf[block][64] = w eps rho; the block's (w eps rho, w rho) sums to red.
+__global__ __launch_bounds__(64) void k_grad_points(const WorkItem* __restrict__ items,
+                                                    const DevTask* __restrict__ tasks,
+                                                    const double* __restrict__ weights,
+                                                    int func, int gga,
+                                                    const double* __restrict__ dp,
+                                                    double* __restrict__ red,
+                                                    double* __restrict__ coef,
+                                                    double* __restrict__ wf) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  const int64_t b = t.red_off + wi.a;
+  const int p = wi.a * kPtBlock + tid;
+  double s[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) s[c] = dp[b * 256 + c * 64 + tid];
+  double g[3], sig = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g[d] = 2.0 * s[1 + d];
+    sig += g[d] * g[d];
+  }
+  double e = 0., n = 0., a = 0., cv[3] = {0., 0., 0.};
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    const XCOut o = dev_eval_functional(func, s[0], sig);
+    e = w * o.e;
+    n = w * s[0];
+    a = w * o.vr;
+    if (gga) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cv[d] = 2.0 * w * o.vs * g[d];
+    }
+  }
+  coef[b * 256 + tid] = a;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) coef[b * 256 + (1 + d) * 64 + tid] = cv[d];
+  wf[b * 64 + tid] = e;
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (tid == 0) {
+    red[2 * b] = e;
+    red[2 * b + 1] = n;
+  }
+}
+
+// ---- plane products ----------------------------------------------------------
+// out[f][p] = 2 sum_g P_sub[f][g] in[g][p] for one (task, 64-point block),
+// every 64-function tile of the task: 256 threads, each a 4 x 4 patch
+// (functions 4 ty.., points 4 tx..), K steps of 16 staged in LDS.  P_sub is
+// gathered through the basis map (symmetric, so the storage order of P does
+// not matter).  Rows past nbe are neither read nor written.
+__global__ __launch_bounds__(256) void k_plane_product(const WorkItem* __restrict__ items,
+                                                       const DevTask* __restrict__ tasks,
+                                                       const int32_t* __restrict__ bfmap,
+                                                       const double* __restrict__ P,
+                                                       double* __restrict__ scratch, int in_pl,
+                                                       int out_pl) {
+  __shared__ double Ps[16][65];
+  __shared__ double Is[16][64];
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  const int64_t m0 = (int64_t)wi.a * kPtBlock;
+  const double* in = scratch + t.b_off + in_pl * plane + m0;
+  double* out = scratch + t.b_off + out_pl * plane + m0;
+  const int32_t* bf = bfmap + t.bf_off;
+  const double* Pm = P + t.p_off;
+  for (int f0 = 0; f0 < t.nbe; f0 += 64) {
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0.;
+    for (int g0 = 0; g0 < t.nbe; g0 += 16) {
+      __syncthreads();  // the previous step's reads of Ps / Is are done
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int idx = tid + 256 * q, i = idx & 63, k = idx >> 6;
+        const int f = f0 + i, g = g0 + k;
+        Ps[k][i] = (f < t.nbe && g < t.nbe) ? Pm[(int64_t)bf[f] + (int64_t)bf[g] * t.ldp] : 0.;
+        Is[k][i] = g < t.nbe ? in[(int64_t)g * ld + i] : 0.;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        double pv[4], iv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pv[i] = Ps[k][4 * ty + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) iv[j] = Is[k][4 * tx + j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = fma(pv[i], iv[j], acc[i][j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = f0 + 4 * ty + i;
+      if (f < t.nbe) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[(int64_t)f * ld + 4 * tx + j] = 2.0 * acc[i][j];
+      }
+    }
+  }
+}
+
+// ---- gradient and Hessian-vector product of a shell's functions ------------
+// For B = R(r^2) m(r) with R1 = -2 sum a c e^{-a r^2} and R2 = 4 sum a^2 c
+// e^{-a r^2} (so d_i R = x_i R1, d_i d_j R = delta_ij R1 + x_i x_j R2):
+//   d_i B = d_i m R + m x_i R1,
+//   (H c)_i = R d_i(c . grad m) + R1 (d_i m (r . c) + x_i (c . grad m))
+//           + m (c_i R1 + x_i (r . c) R2).
+// Calls sink(f, gx, gy, gz, hx, hy, hz) per function (CCA order; spherical
+// outputs one at a time from their nonzero Cartesian terms).
+template <int L, typename Sink>
+__device__ __forceinline__ void shell_grad_hc(double x, double y, double z, double R, double R1,
+                                              double R2, double cx, double cy, double cz,
+                                              bool pure, Sink&& sink) {
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.;
+#pragma unroll
+  for (int k = 1; k <= L; ++k) {
+    xp[k] = xp[k - 1] * x;
+    yp[k] = yp[k - 1] * y;
+    zp[k] = zp[k - 1] * z;
+  }
+  const double rc = x * cx + y * cy + z * cz;
+  // x^n and its first and second derivatives (0 where the power is negative).
+  auto pw = [](const double* t, int n) { return n < 0 ? 0. : t[n < 0 ? 0 : n]; };
+  auto term = [&](int a, int b, double* o) {
+    const int c = L - a - b;
+    const double m = xp[a] * yp[b] * zp[c];
+    const double mx = a * pw(xp, a - 1) * yp[b] * zp[c];
+    const double my = b * xp[a] * pw(yp, b - 1) * zp[c];
+    const double mz = c * xp[a] * yp[b] * pw(zp, c - 1);
+    const double mxx = a * (a - 1) * pw(xp, a - 2) * yp[b] * zp[c];
+    const double myy = b * (b - 1) * xp[a] * pw(yp, b - 2) * zp[c];
+    const double mzz = c * (c - 1) * xp[a] * yp[b] * pw(zp, c - 2);
+    const double mxy = a * b * pw(xp, a - 1) * pw(yp, b - 1) * zp[c];
+    const double mxz = a * c * pw(xp, a - 1) * yp[b] * pw(zp, c - 1);
+    const double myz = b * c * xp[a] * pw(yp, b - 1) * pw(zp, c - 1);
+    const double cm = cx * mx + cy * my + cz * mz;
+    o[0] = mx * R + m * x * R1;
+    o[1] = my * R + m * y * R1;
+    o[2] = mz * R + m * z * R1;
+    o[3] = R * (cx * mxx + cy * mxy + cz * mxz) + R1 * (mx * rc + x * cm) + m * (cx * R1 + x * rc * R2);
+    o[4] = R * (cx * mxy + cy * myy + cz * myz) + R1 * (my * rc + y * cm) + m * (cy * R1 + y * rc * R2);
+    o[5] = R * (cx * mxz + cy * myz + cz * mzz) + R1 * (mz * rc + z * cm) + m * (cz * R1 + z * rc * R2);
+  };
+  if (pure) {
+#pragma unroll
+    for (int mo = 0; mo < 2 * L + 1; ++mo) {
+      double acc[6] = {0., 0., 0., 0., 0., 0.};
+      int ic = 0;
+#pragma unroll
+      for (int a = L; a >= 0; --a) {
+#pragma unroll
+        for (int b = L - a; b >= 0; --b) {
+          const double cf = kSph.c[L][mo][ic];
+          if (cf != 0.) {
+            double o[6];
+            term(a, b, o);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[q] += cf * o[q];
+          }
+          ++ic;
+        }
+      }
+      sink(mo, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]);
+    }
+  } else {
+    int ic = 0;
+#pragma unroll
+    for (int a = L; a >= 0; --a) {
+#pragma unroll
+      for (int b = L - a; b >= 0; --b) {
+        double o[6];
+        term(a, b, o);
+        sink(ic, o[0], o[1], o[2], o[3], o[4], o[5]);
+        ++ic;
+      }
+    }
+  }
+}
+
+// ---- basis-function part of the gradient -----------------------------------
+// One (task, 64-point block) per 256-thread workgroup; lane = point, wave w
+// takes the task's shells w, w + 4, ...  Per shell (with the weight
+// derivatives: on an atom other than the parent): sum over its functions and the block's points of
+// grad B . Y + X (H c) (X only for GGAs), reduced over the wave; runs of
+// shells on one atom are added to that atom with one set of atomics, the
+// parent's share (weight derivatives only) once per wave at the end.
+template <bool GGA>
+__global__ __launch_bounds__(256) void k_grad_contract(
+    const WorkItem* __restrict__ items, const DevTask* __restrict__ tasks,
+    const DevShell* __restrict__ shells, const int32_t* __restrict__ shell_list,
+    const int32_t* __restrict__ lsh_first, const int32_t* __restrict__ shell_atom,
+    const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const double* __restrict__ scratch,
+    const double* __restrict__ coef, int weight_derivs, double* __restrict__ grad) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p = wi.a * kPtBlock + lane;
+  const bool valid = p < t.npts;
+  const int pc = min(p, t.npts - 1);
+  const double gx = px[t.pt_off + pc], gy = py[t.pt_off + pc], gz = pz[t.pt_off + pc];
+  const int64_t b = t.red_off + wi.a;
+  double cx = 0., cy = 0., cz = 0.;
+  if (GGA && valid) {
+    cx = coef[b * 256 + 64 + lane];
+    cy = coef[b * 256 + 128 + lane];
+    cz = coef[b * 256 + 192 + lane];
+  }
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  const int64_t m0 = (int64_t)wi.a * kPtBlock + lane;
+  const double* Xp = scratch + t.b_off + plane + m0;
+  const double* Yp = scratch + t.b_off + 2 * plane + m0;
+  int cur = -1;                            // atom of the running sum (uniform)
+  double run[3] = {0., 0., 0.}, par[3] = {0., 0., 0.};
+  auto flush = [&]() {
+    if (cur >= 0 && lane == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * cur + d, -2.0 * run[d]);
+    }
+  };
+  for (int k = wave; k < t.nshells; k += 4) {
+    const int ish = __builtin_amdgcn_readfirstlane(shell_list[t.sh_off + k]);
+    const int at = __builtin_amdgcn_readfirstlane(shell_atom[ish]);
+    if (at < 0 || (weight_derivs && at == t.iparent)) continue;
+    const int bf0 = __builtin_amdgcn_readfirstlane(lsh_first[t.lsh_off + k]);
+    const DevShell* sh = shells + ish;
+    const int l = sh->l, nprim = sh->nprim;
+    const bool pure = sh->pure != 0;
+    const double x = gx - sh->O[0], y = gy - sh->O[1], z = gz - sh->O[2];
+    const double r2 = x * x + y * y + z * z;
+    double R = 0., R1 = 0., R2 = 0.;
+    for (int q = 0; q < nprim; ++q) {
+      const double a = sh->alpha[q], e = sh->coeff[q] * exp(-a * r2);
+      R += e;
+      R1 -= 2. * a * e;
+      R2 += 4. * a * a * e;
+    }
+    double s[3] = {0., 0., 0.};
+    auto put = [&](int f, double bx, double by, double bz, double hx, double hy, double hz) {
+      const int64_t r = (int64_t)(bf0 + f) * ld;
+      const double Y = valid ? Yp[r] : 0.;
+      s[0] = fma(bx, Y, s[0]);
+      s[1] = fma(by, Y, s[1]);
+      s[2] = fma(bz, Y, s[2]);
+      if constexpr (GGA) {
+        const double X = valid ? Xp[r] : 0.;
+        s[0] = fma(X, hx, s[0]);
+        s[1] = fma(X, hy, s[1]);
+        s[2] = fma(X, hz, s[2]);
+      }
+    };
+    if (l == 0) shell_grad_hc<0>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+    else if (l == 1) shell_grad_hc<1>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+    else if (l == 2) shell_grad_hc<2>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+    else if (l == 3) shell_grad_hc<3>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+    else shell_grad_hc<4>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      for (int off = 32; off > 0; off >>= 1) s[d] += __shfl_xor(s[d], off);
+    if (at != cur) {
+      flush();
+      cur = at;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) run[d] = 0.;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      run[d] += s[d];
+      par[d] += s[d];
+    }
+  }
+  flush();
+  if (weight_derivs && lane == 0 && cur >= 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * t.iparent + d, 2.0 * par[d]);
+  }
+}
+
+// ---- grid-weight part ---------------------------------------------------------
+// Derivatives of the SSF weights w_i = w0_i P_A(r_i) / sum_C P_C(r_i)
+// contracted with f_i = w_i eps_i rho_i, the points moving with their parent
+// atom A (weights.cxx:806-990).  P_B = prod_{C != B} s(mu_BC), s(mu) = 1
+// (mu <= -a), 0 (mu >= a), (1 - g(mu / a)) / 2 otherwise (g the Frisch
+// polynomial, a = 0.64), formed here as that product; its logarithmic
+// derivative t(mu) = s'(mu) / s(mu) in the factored form
+//   1 - g(v) = (1 - v)^4 (5 v^3 + 20 v^2 + 29 v + 16) / 16,  g'(v) = 35 (1 - v^2)^3 / 16,
+//   t(mu) = 35 (1 + v)^3 / ((mu - a)(5 v^3 + 20 v^2 + 29 v + 16)),  v = mu / a,
+// which stays accurate up to the |mu| < a - 1e-4 bound the contraction uses.
+// One thread per point (lane of a 64-point block); distances are recomputed
+// where needed, O(natoms^2) per point.
+constexpr double kSsfA = 0.64;
+constexpr double kSsfTol = 1e-13;
+
+__device__ __forceinline__ double ssf_s(double mu) {
+  if (mu <= -kSsfA) return 1.;
+  if (mu >= kSsfA) return 0.;
+  const double v = mu / kSsfA, v2 = v * v;
+  const double g = v * (35. + v2 * (-35. + v2 * (21. - 5. * v2))) / 16.;
+  return 0.5 * (1. - g);
+}
+__device__ __forceinline__ double ssf_t(double mu) {
+  const double v = mu / kSsfA;
+  return 35. * (1. + v) * (1. + v) * (1. + v) /
+         ((mu - kSsfA) * (((5. * v + 20.) * v + 29.) * v + 16.));
+}
+
+__global__ __launch_bounds__(64) void k_ssf_grad(const WorkItem* __restrict__ items,
+                                                 const DevTask* __restrict__ tasks,
+                                                 const double* __restrict__ px,
+                                                 const double* __restrict__ py,
+                                                 const double* __restrict__ pz, int natoms,
+                                                 const double* __restrict__ xyz,
+                                                 const double* __restrict__ dnear,
+                                                 const double* __restrict__ wf,
+                                                 double* __restrict__ grad) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int lane = threadIdx.x;
+  const int p = wi.a * kPtBlock + lane;
+  if (p >= t.npts) return;
+  const double f = wf[(t.red_off + wi.a) * 64 + lane];
+  if (fabs(f) < 1e-12) return;
+  const double r[3] = {px[t.pt_off + p], py[t.pt_off + p], pz[t.pt_off + p]};
+  auto X = [&](int at, int d) { return xyz[3 * at + d]; };
+  auto dist = [&](int at) {
+    const double dx = r[0] - X(at, 0), dy = r[1] - X(at, 1), dz = r[2] - X(at, 2);
+    return sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  auto rab = [&](int a, int b) {
+    const double dx = X(a, 0) - X(b, 0), dy = X(a, 1) - X(b, 1), dz = X(a, 2) - X(b, 2);
+    return sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  const int A = t.iparent;
+  const double dA = dist(A);
+  if (dA < 0.5 * (1. - kSsfA) * dnear[A]) return;
+  auto part = [&](int B, double dB) {
+    double v = 1.;
+    for (int C = 0; C < natoms && v > 0.; ++C)
+      if (C != B) v *= ssf_s((dB - dist(C)) / rab(B, C));
+    return v;
+  };
+  double sum = 0.;
+  for (int B = 0; B < natoms; ++B) sum += part(B, B == A ? dA : dist(B));
+  const double pA = part(A, dA);
+  const double bound = kSsfA - 1e-4;
+  double gA[3] = {0., 0., 0.};
+  for (int B = 0; B < natoms; ++B) {
+    if (B == A) continue;
+    const double dB = dist(B);
+    double gb[3] = {0., 0., 0.};
+    const double rAB = rab(A, B);
+    const double muAB = (dA - dB) / rAB;
+    if (fabs(muAB) < bound) {
+      const double c1 = ssf_t(muAB) / rAB * (pA - sum) / sum * f / dB;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        gb[d] = c1 * ((X(B, d) - r[d]) + muAB * (X(B, d) - X(A, d)) / rAB * dB);
+    }
+    const double pB = part(B, dB);
+    if (pB > kSsfTol) {
+      for (int C = 0; C < natoms; ++C) {
+        if (C == B) continue;
+        const double rBC = rab(B, C), dC = dist(C);
+        const double muBC = (dB - dC) / rBC;
+        if (fabs(muBC) >= bound) continue;
+        const double c = pB * ssf_t(muBC) / rBC / sum * f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          gb[d] -= c * ((X(B, d) - r[d]) / dB - muBC * (X(B, d) - X(C, d)) / rBC);
+        if (C != A) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double gc = c * ((X(C, d) - r[d]) / dC + muBC * (X(C, d) - X(B, d)) / rBC);
+            atomicAdd(grad + 3 * C + d, gc);
+            gA[d] -= gc;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (gb[d] != 0.) atomicAdd(grad + 3 * B + d, gb[d]);
+      gA[d] -= gb[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * A + d, gA[d]);
+}
+
+// ---- launchers ------------------------------------------------------------------
+int launch_grad_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
+                       double* coef, double* wf) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_grad_points, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
+                     v.tasks, v.w, func, gga, dp, v.red, coef, wf);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_plane_product, dim3((unsigned)r.n), dim3(256), 0, v.stream,
+                     v.work + r.off, v.tasks, v.bfmap, v.P, static_cast<double*>(v.scratch),
+                     in_pl, out_pl);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
+                         const int32_t* shell_atom, const double* coef, int weight_derivs,
+                         double* grad) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)r.n), dim3(256), 0, v.stream, v.work + r.off,
+                       v.tasks, v.shells, v.shell_list, v.lsh, shell_atom, v.px, v.py, v.pz,
+                       static_cast<const double*>(v.scratch), coef, weight_derivs, grad);
+  };
+  gga ? go(k_grad_contract<true>) : go(k_grad_contract<false>);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
+                    const double* dnear, const double* wf, double* grad) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_ssf_grad, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
+                     v.tasks, v.px, v.py, v.pz, natoms, xyz, dnear, wf, grad);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+}  // namespace xcq

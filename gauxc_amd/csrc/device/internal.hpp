@@ -265,6 +265,16 @@ int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, c
 // FXC contraction (RKS): Z coefficients from the P and trial-density passes.
 int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
                       const double* dt, double* zc);
+// EXC gradient (RKS): per-point coefficients, plane products out = 2 P_sub in,
+// the basis-function part and the grid-weight part (gradient.hip).
+int launch_grad_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
+                       double* coef, double* wf);
+int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl);
+int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
+                         const int32_t* shell_atom, const double* coef, int weight_derivs,
+                         double* grad);
+int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
+                    const double* dnear, const double* wf, double* grad);
 int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);

@@ -494,6 +494,28 @@ void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* st, const GauXCIntegrator in
   });
 }
 
+void gauxc_integrator_eval_exc_grad_rks(GauXCStatus* st, const GauXCIntegrator in,
+                                        const int64_t m, const int64_t n, const double* P,
+                                        const int64_t ldp, double* exc_grad) {
+  guarded_void(st, [&] {
+    if (!P || !exc_grad) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_exc_grad(m, n, P, ldp, exc_grad, IntegratorSettingsEXC_GRAD{});
+  });
+}
+
+void gauxc_integrator_eval_exc_grad_hf_rks(GauXCStatus* st, const GauXCIntegrator in,
+                                           const int64_t m, const int64_t n, const double* P,
+                                           const int64_t ldp, double* exc_grad) {
+  guarded_void(st, [&] {
+    if (!P || !exc_grad) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    IntegratorSettingsEXC_GRAD gs;
+    gs.include_weight_derivatives = false;
+    I.device_impl().eval_exc_grad(m, n, P, ldp, exc_grad, gs);
+  });
+}
+
 void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* st, const GauXCIntegrator in,
                                                const int64_t m, const int64_t n,
                                                const double* P, const int64_t ldp,

@@ -271,6 +271,49 @@ void DeviceXCIntegratorImpl::eval_exc_uks(int64_t m, int64_t n, const double* Ps
         "xcq_download_results_uks");
 }
 
+void DeviceXCIntegratorImpl::eval_exc_grad(int64_t m, int64_t n, const double* P, int64_t ldp,
+                                           double* grad, const IntegratorSettingsXC& settings) {
+  // Argument checks of reference_replicated_xc_host_integrator_exc_grad.hpp:29-37.
+  auto& s = *s_;
+  const int64_t nbf = s.lb->basis().nbf();
+  if (m != n) GAUXC_GENERIC_EXCEPTION("P Must Be Square");
+  if (m != nbf) GAUXC_GENERIC_EXCEPTION("P Must Have Same Dimension as Basis");
+  if (ldp < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDP");
+  if (!s.lb->state().modified_weights_are_stored)
+    GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
+  if (s.func->is_polarized())
+    GAUXC_GENERIC_EXCEPTION("RKS evaluation needs an unpolarized functional");
+  if (s.prec != DevicePrecision::FP64)
+    GAUXC_GENERIC_EXCEPTION("The EXC gradient is provided in FP64 (local work driver Default)");
+  if (!P || !grad) GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  bool weight_derivs = true;
+  if (const auto* gs = dynamic_cast<const IntegratorSettingsEXC_GRAD*>(&settings))
+    weight_derivs = gs->include_weight_derivatives;
+  const Molecule& mol = s.lb->molecule();
+  std::vector<double> xyz(3 * mol.size());
+  for (size_t a = 0; a < mol.size(); ++a) {
+    xyz[3 * a] = mol[a].x;
+    xyz[3 * a + 1] = mol[a].y;
+    xyz[3 * a + 2] = mol[a].z;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  s.ensure_static_data();
+  check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), P, ldp), "xcq_upload_density");
+  const auto t1 = std::chrono::steady_clock::now();
+  check(xcq_exc_grad(s.ctx, xcq_func_id(*s.func), static_cast<int32_t>(mol.size()), xyz.data(),
+                     weight_derivs ? 1 : 0),
+        "xcq_exc_grad");
+  check(xcq_synchronize(s.ctx), "xcq_synchronize");
+  const auto t2 = std::chrono::steady_clock::now();
+  double exc = 0.;
+  check(xcq_download_exc_grad(s.ctx, &exc, &s.last_nel, grad), "xcq_download_exc_grad");
+  const auto t3 = std::chrono::steady_clock::now();
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  s.timer.set("XCIntegrator.HostToDevice_EXC_GRAD", ms(t0, t1));
+  s.timer.set("XCIntegrator.LocalWork_EXC_GRAD", ms(t1, t2));
+  s.timer.set("XCIntegrator.DeviceToHostCopy_EXC_GRAD", ms(t2, t3));
+}
+
 void DeviceXCIntegratorImpl::eval_fxc_contraction(int64_t m, int64_t n, const double* P,
                                                   int64_t ldp, const double* tP, int64_t ldtp,
                                                   double* FXC, int64_t ldfxc,

@@ -231,6 +231,20 @@ void gauxc_integrator_eval_exc_vxc_uks(GauXCStatus* status, const GauXCIntegrato
                                        const double* density_matrix_z, const int64_t ldp_z,
                                        double* exc, double* vxc_matrix_s, const int64_t vxc_ld_s,
                                        double* vxc_matrix_z, const int64_t vxc_ld_z);
+/* RKS EXC nuclear gradient with the grid-weight derivatives (GauXC
+ * include/gauxc/c/xc_integrator.h:282-290): exc_grad[3 * natoms], atom-major;
+ * FP64 local work driver, unpolarized functional. */
+void gauxc_integrator_eval_exc_grad_rks(GauXCStatus* status, const GauXCIntegrator integrator,
+                                        const int64_t m, const int64_t n,
+                                        const double* density_matrix, const int64_t ldp,
+                                        double* exc_grad);
+/* Extension: the Hellmann-Feynman gradient (no grid-weight derivatives;
+ * GauXC's IntegratorSettingsEXC_GRAD::include_weight_derivatives = false,
+ * which its C API does not expose). */
+void gauxc_integrator_eval_exc_grad_hf_rks(GauXCStatus* status,
+                                           const GauXCIntegrator integrator, const int64_t m,
+                                           const int64_t n, const double* density_matrix,
+                                           const int64_t ldp, double* exc_grad);
 /* RKS FXC contraction (GauXC include/gauxc/c/xc_integrator.h:351-362):
  * fxc = d VXC[P + e tP] / de for the trial density tP; FP64 local work
  * driver, unpolarized functional. */

@@ -34,6 +34,12 @@ struct IntegratorSettingsXC {
 struct IntegratorSettingsKS : public IntegratorSettingsXC {
   double gks_dtol = 1e-12;
 };
+/// GauXC include/gauxc/xc_integrator_settings.hpp:28-30: with the grid-weight
+/// derivatives (and translational invariance for the parent atoms), or the
+/// Hellmann-Feynman gradient alone.
+struct IntegratorSettingsEXC_GRAD : public IntegratorSettingsKS {
+  bool include_weight_derivatives = true;
+};
 
 struct LocalWorkSettings {};
 
@@ -92,6 +98,10 @@ class DeviceXCIntegratorImpl {
                         int64_t ldvxcz, double* EXC, const IntegratorSettingsXC& s);
   void eval_exc_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz,
                     int64_t ldpz, double* EXC, const IntegratorSettingsXC& s);
+  /// RKS EXC nuclear gradient, grad[3 * natoms] atom-major (FP64; unpolarized
+  /// functional); settings of type IntegratorSettingsEXC_GRAD select the form.
+  void eval_exc_grad(int64_t m, int64_t n, const double* P, int64_t ldp, double* grad,
+                     const IntegratorSettingsXC& s);
   /// RKS FXC contraction d VXC[P + e tP] / de (FP64; unpolarized functional).
   void eval_fxc_contraction(int64_t m, int64_t n, const double* P, int64_t ldp, const double* tP,
                             int64_t ldtp, double* FXC, int64_t ldfxc,
@@ -156,6 +166,15 @@ class XCIntegrator {
     impl().eval_exc_vxc_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(), Pz.rows(),
                             VXCs.data(), VXCs.rows(), VXCz.data(), VXCz.rows(), &e, s);
     return std::make_tuple(e, std::move(VXCs), std::move(VXCz));
+  }
+  using exc_grad_type = std::vector<value_type>;
+  /// GauXC XCIntegrator::eval_exc_grad(P, settings)
+  /// (include/gauxc/xc_integrator/impl.hpp:82-86), RKS.
+  exc_grad_type eval_exc_grad(const MatrixType& P,
+                              const IntegratorSettingsXC& s = IntegratorSettingsEXC_GRAD{}) {
+    exc_grad_type g(3 * impl().load_balancer().molecule().size());
+    impl().eval_exc_grad(P.rows(), P.cols(), P.data(), P.rows(), g.data(), s);
+    return g;
   }
   /// GauXC XCIntegrator::eval_fxc_contraction(P, tP)
   /// (include/gauxc/xc_integrator/impl.hpp:104-109), RKS.

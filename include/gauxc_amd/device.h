@@ -237,6 +237,21 @@ int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* 
 int xcq_upload_trial_density(xcq_context ctx, int32_t nbf, const double* tP, int64_t ldp);
 int xcq_fxc_contraction(xcq_context ctx, int functional);
 int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc);
+/* ---- RKS EXC nuclear gradient with the SSF grid-weight derivatives (GauXC
+ *      eval_exc_grad(P) with include_weight_derivatives:
+ *      reference_replicated_xc_host_integrator_exc_grad.hpp:106-600,
+ *      weights.cxx:806-990; device increment_exc_grad.cu, cuda_ssf_1d.cu; C
+ *      API gauxc_integrator_eval_exc_grad_rks, include/gauxc/c/xc_integrator.h:
+ *      282-290).  P through xcq_upload_density (single-molecule contexts);
+ *      xyz = the natoms atom positions (x, y, z), in the order of the tasks'
+ *      parent indices; a shell belongs to the atom at exactly its centre.
+ *      FP64, stored basis planes.  weight_derivs = 0 gives the Hellmann-
+ *      Feynman form (IntegratorSettingsEXC_GRAD::include_weight_derivatives =
+ *      false).  xcq_download_exc_grad returns EXC, NEL and grad[3 * natoms]
+ *      (atom-major). ---- */
+int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
+                 int weight_derivs);
+int xcq_download_exc_grad(xcq_context ctx, double* exc, double* nel, double* grad);
 /* Integrated density only (eval_integrate_den). */
 int xcq_integrate_den(xcq_context ctx, double* nel);
 int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc);

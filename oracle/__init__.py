@@ -70,8 +70,8 @@ def lib():
         L.oracle_collocation_hessian.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp]
         L.oracle_collocation_hessian.restype = None
         L.oracle_exc_grad.argtypes = [ctypes.c_int, ctypes.c_int32, vp, ctypes.c_int32, vp, vp,
-                                      ctypes.c_int64, ctypes.c_int32, vp, vp, vp, vp, vp,
-                                      ctypes.c_int]
+                                      ctypes.c_int64, ctypes.c_int32, vp, vp, vp, vp,
+                                      ctypes.c_int, vp, ctypes.c_int]
         L.oracle_exc_grad.restype = ctypes.c_int
         L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.oracle_use_blas.restype = ctypes.c_int
@@ -251,8 +251,9 @@ def collocation_hessian(shells, shell_list, points):
     return out
 
 
-def exc_grad(func, nbf, shells, xyz, P, t, weights=None, nthreads=0):
-    """RKS EXC gradient (natoms, 3) with the grid-weight derivatives."""
+def exc_grad(func, nbf, shells, xyz, P, t, weights=None, nthreads=0, weight_derivs=True):
+    """RKS EXC gradient (natoms, 3), with the grid-weight derivatives unless
+    weight_derivs is False (Hellmann-Feynman form)."""
     P = np.asfortranarray(P, dtype=np.float64)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     pts = np.ascontiguousarray(t["points"], dtype=np.float64)
@@ -263,7 +264,7 @@ def exc_grad(func, nbf, shells, xyz, P, t, weights=None, nthreads=0):
     rc = lib().oracle_exc_grad(FUNC[func], nbf, ctypes.addressof(shells), len(xyz),
                                xyz.ctypes.data, P.ctypes.data, nbf, len(t["npts"]),
                                tasks.ctypes.data, pts.ctypes.data, w.ctypes.data, sl.ctypes.data,
-                               g.ctypes.data, nthreads)
+                               int(bool(weight_derivs)), g.ctypes.data, nthreads)
     assert rc == 0
     return g
 

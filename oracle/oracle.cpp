@@ -1377,7 +1377,7 @@ void oracle_collocation_hessian(const oracle_shell* shells, int32_t nlist,
 int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
                     const double* xyz, const double* P, int64_t ldp, int32_t ntasks,
                     const oracle_task* tasks, const double* points, const double* weights,
-                    const int32_t* shell_lists, double* grad, int nthreads) {
+                    const int32_t* shell_lists, int weight_derivs, double* grad, int nthreads) {
   if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
   const bool gga = func != ORACLE_SVWN5;
   for (int k = 0; k < 3 * natoms; ++k) grad[k] = 0.;
@@ -1451,16 +1451,18 @@ int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t n
                         gga ? vs.data() : nullptr);
       // Grid-weight part with w_i eps_i rho_i (:418-425).
       wf.resize(npts);
-      for (int32_t p = 0; p < npts; ++p) wf[p] = eps[p] * rho[p] * w[p];
-      ssf_grad_contract(natoms, xyz, rab.data(), t, pts, wf.data(), gloc.data());
-      // Basis-function part, shell by shell (:427-590); shells on the parent
-      // atom move with the grid and drop out.
+      if (weight_derivs) {
+        for (int32_t p = 0; p < npts; ++p) wf[p] = eps[p] * rho[p] * w[p];
+        ssf_grad_contract(natoms, xyz, rab.data(), t, pts, wf.data(), gloc.data());
+      }
+      // Basis-function part, shell by shell (:427-590); with the weight
+      // derivatives, shells on the parent atom move with the grid and drop out.
       int32_t boff = 0;
       for (int32_t k = 0; k < t.nshells; ++k) {
         const int32_t s = sl[k];
         const int32_t n = first_ao[s + 1] - first_ao[s];
         const int32_t at = centre[s];
-        if (at == t.iparent || at < 0) {
+        if (at < 0 || (weight_derivs && at == t.iparent)) {
           boff += n;
           continue;
         }
@@ -1485,7 +1487,7 @@ int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t n
           }
         for (int d = 0; d < 3; ++d) {
           gloc[3 * at + d] += -2. * g[d];
-          gloc[3 * t.iparent + d] -= -2. * g[d];
+          if (weight_derivs) gloc[3 * t.iparent + d] -= -2. * g[d];
         }
         boff += n;
       }

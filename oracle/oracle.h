@@ -102,14 +102,16 @@ void oracle_collocation_hessian(const oracle_shell* shells, int32_t nlist,
                                 const int32_t* shell_list, int32_t npts, const double* points,
                                 double* const* out);
 
-/* RKS EXC nuclear gradient including the SSF grid-weight derivatives
- * (GauXC eval_exc_grad with include_weight_derivatives): grad[3 * natoms],
+/* RKS EXC nuclear gradient, with the SSF grid-weight derivatives
+ * (weight_derivs = 1, GauXC's default) or the Hellmann-Feynman form without
+ * them (IntegratorSettingsEXC_GRAD::include_weight_derivatives): grad[3 * natoms],
  * atom-major (x, y, z).  Shells are assigned to the atom at exactly their
  * origin. */
 int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
                     const double* xyz, const double* P, int64_t ldp, int32_t ntasks,
                     const oracle_task* tasks, const double* points, const double* weights,
-                    const int32_t* shell_lists, double* grad, int nthreads);
+                    const int32_t* shell_lists, int weight_derivs, double* grad,
+                    int nthreads);
 
 #ifdef __cplusplus
 }
