@@ -197,12 +197,20 @@ def main():
         return run_batched(args)
     t0 = time.time()
     mol, bs, P, cfg = workloads.make_config(args.config, seed=0, n_water=args.n_water)
-    lb = workloads.build_tasks(mol, bs, grid=cfg["grid"])
+    # Task generation with the batch screening on the device (GauXC's device
+    # load balancer), then the SSF weights on the device.
+    lb = workloads.build_tasks(mol, bs, grid=cfg["grid"], ex="Device")
+    t1 = time.time()
+    lb.task_counts()
+    t2 = time.time()
     gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    t3 = time.time()
     t = lb.get_tasks()
     npts_total = int(t["npts"].sum())
+    setup = {"task_build_s": round(t2 - t1, 3), "ssf_weights_s": round(t3 - t2, 3)}
     log(f"setup: nbf={cfg['nbf']} atoms={len(cfg['Z'])} tasks={len(t['npts'])} "
-        f"points={npts_total} ({time.time() - t0:.1f}s)")
+        f"points={npts_total} ({time.time() - t0:.1f}s; tasks {setup['task_build_s']} s with "
+        f"device screening, SSF {setup['ssf_weights_s']} s)")
 
     ctx = gx.DeviceContext(0)
     ctx.upload_basis(bs.xcq_shells())
@@ -327,6 +335,7 @@ def main():
         "stage_ms": dict(zip(STAGES, [float(x) for x in r["stage"][:6]])),
         "exc": r["exc"],
         "mixed_vs_fp64": accuracy,
+        "setup": setup,
     }
     if learned is not None:
         line["learned_schedule"] = learned

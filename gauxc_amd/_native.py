@@ -118,7 +118,7 @@ DEVICE_SYMBOLS = [
     "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_ssf_weights_noalloc",
     "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_upload_trial_density", "xcq_fxc_contraction", "xcq_download_fxc",
-    "xcq_exc_grad", "xcq_download_exc_grad",
+    "xcq_exc_grad", "xcq_download_exc_grad", "xcq_collision_detection",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
@@ -173,6 +173,8 @@ def device_lib():
             "xcq_download_fxc": (ctypes.c_int, [H, c_vp, c_vp, c_i64]),
             "xcq_exc_grad": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
             "xcq_download_exc_grad": (ctypes.c_int, [H, c_vp, c_vp, c_vp]),
+            "xcq_collision_detection": (ctypes.c_int, [ctypes.c_int, c_i64, c_vp, c_vp, c_i32,
+                                                       c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             "xcq_ssf_weights_noalloc": (ctypes.c_int, [ctypes.c_int, c_i32, c_vp, c_i64, c_vp,
                                                        c_vp, c_vp]),
             "xcq_download_weights": (ctypes.c_int, [H, c_vp]),

@@ -1,8 +1,11 @@
 // Replicated task generation.  Behaviour follows
 // src/load_balancer/host/replicated_host_load_balancer.cxx:22-193 and
-// petite_replicated_load_balancer.cxx:31-65 (see load_balancer.hpp).
+// petite_replicated_load_balancer.cxx:31-65 (see load_balancer.hpp); the
+// Device execution space screens the batches with xcq_collision_detection
+// (replicated_hip_load_balancer.cxx:72-220).
 #include <gauxc/load_balancer.hpp>
 #include <gauxc/exceptions.hpp>
+#include <gauxc_amd/device.h>
 
 #include <algorithm>
 #include <cctype>
@@ -26,55 +29,107 @@ bool cube_sphere_intersect(const double* lo, const double* up, const double* c, 
 }  // namespace geometry
 
 LoadBalancer::LoadBalancer(const Molecule& mol, const MolGrid& mg,
-                           const BasisSet<double>& basis, const RuntimeEnvironment& rt)
+                           const BasisSet<double>& basis, const RuntimeEnvironment& rt,
+                           ExecutionSpace ex)
     : mol_(std::make_shared<Molecule>(mol)),
       mg_(std::make_shared<MolGrid>(mg)),
       basis_(std::make_shared<BasisSet<double>>(basis)),
       meta_(std::make_shared<MolMeta>(mol)),
-      rt_(copy_runtime(rt)) {}
+      rt_(copy_runtime(rt)),
+      ex_(ex) {}
 
 LoadBalancer::LoadBalancer(const LoadBalancer& o)
-    : mol_(o.mol_), mg_(o.mg_), basis_(o.basis_), meta_(o.meta_), rt_(o.rt_), tasks_(o.tasks_),
-      built_(o.built_), state_(o.state_), revision_(o.revision_) {}
+    : mol_(o.mol_), mg_(o.mg_), basis_(o.basis_), meta_(o.meta_), rt_(o.rt_), ex_(o.ex_),
+      tasks_(o.tasks_), built_(o.built_), state_(o.state_), revision_(o.revision_) {}
+
+void LoadBalancer::screen_batches(const std::vector<GridBatch>& batches,
+                                  std::vector<std::vector<int32_t>>& lists,
+                                  std::vector<int32_t>& nbe) const {
+  const auto& basis = *basis_;
+  const int32_t nsh = static_cast<int32_t>(basis.nshells());
+  const size_t nb = batches.size();
+  lists.assign(nb, {});
+  nbe.assign(nb, 0);
+  if (ex_ != ExecutionSpace::Device) {
+#pragma omp parallel for schedule(dynamic)
+    for (size_t ib = 0; ib < nb; ++ib) {
+      const GridBatch& b = batches[ib];
+      for (int32_t s = 0; s < nsh; ++s) {
+        const auto& sh = basis[s];
+        if (geometry::cube_sphere_intersect(b.lo.data(), b.up.data(), sh.O_data(),
+                                            sh.cutoff_radius())) {
+          lists[ib].push_back(s);
+          nbe[ib] += sh.size();
+        }
+      }
+    }
+    return;
+  }
+  // Device: one bit row per batch from xcq_collision_detection.
+  std::vector<double> lo(3 * nb), up(3 * nb), cen(3 * (size_t)nsh), rad(nsh);
+  std::vector<int32_t> sz(nsh), cnt(nb);
+  for (size_t ib = 0; ib < nb; ++ib)
+    for (int d = 0; d < 3; ++d) {
+      lo[3 * ib + d] = batches[ib].lo[d];
+      up[3 * ib + d] = batches[ib].up[d];
+    }
+  for (int32_t s = 0; s < nsh; ++s) {
+    for (int d = 0; d < 3; ++d) cen[3 * (size_t)s + d] = basis[s].O_data()[d];
+    rad[s] = basis[s].cutoff_radius();
+    sz[s] = basis[s].size();
+  }
+  const int32_t ld32 = (nsh + 31) / 32;
+  std::vector<uint32_t> bits((size_t)ld32 * nb);
+  int device = 0;
+  if (const auto* drt = as_device_runtime_ptr(*rt_)) device = drt->device_id();
+  if (xcq_collision_detection(device, static_cast<int64_t>(nb), lo.data(), up.data(), nsh,
+                              cen.data(), rad.data(), sz.data(), bits.data(), cnt.data(),
+                              nbe.data()) != XCQ_SUCCESS)
+    GAUXC_GENERIC_EXCEPTION(std::string("device load balancer: ") + xcq_last_error());
+#pragma omp parallel for schedule(dynamic)
+  for (size_t ib = 0; ib < nb; ++ib) {
+    auto& l = lists[ib];
+    l.reserve(cnt[ib]);
+    const uint32_t* row = bits.data() + ib * ld32;
+    for (int32_t w = 0; w < ld32; ++w)
+      for (uint32_t m = row[w]; m; m &= m - 1) l.push_back(32 * w + __builtin_ctz(m));
+  }
+}
 
 void LoadBalancer::build_tasks() const {
   const auto& mol = *mol_;
-  const auto& basis = *basis_;
   const size_t natoms = mol.natoms();
-  std::vector<XCTask> local;
+  // Every atom's batches (GauXC's host and device load balancers batch on
+  // the host), then one screening pass over all of them.
+  std::vector<GridBatch> batches;
+  std::vector<int32_t> parent;
   for (size_t ia = 0; ia < natoms; ++ia) {
     const auto& atom = mol[ia];
     const std::array<double, 3> center = {atom.x, atom.y, atom.z};
     const auto& batcher = mg_->get_grid(atom.Z).batcher();
     const size_t nb = batcher.nbatches();
-    std::vector<XCTask> atom_tasks(nb);
-    std::vector<char> keep(nb, 0);
+    const size_t b0 = batches.size();
+    batches.resize(b0 + nb);
 #pragma omp parallel for schedule(dynamic)
-    for (size_t ib = 0; ib < nb; ++ib) {
-      GridBatch b = batcher.at(ib, center);
-      if (b.points.empty()) continue;
-      XCTask t;
-      int32_t nbe = 0;
-      for (int32_t s = 0; s < basis.nshells(); ++s) {
-        const auto& sh = basis[s];
-        if (geometry::cube_sphere_intersect(b.lo.data(), b.up.data(), sh.O_data(),
-                                            sh.cutoff_radius())) {
-          t.bfn_screening.shell_list.push_back(s);
-          nbe += sh.size();
-        }
-      }
-      if (t.bfn_screening.shell_list.empty()) continue;
-      t.iParent = static_cast<int32_t>(ia);
-      t.npts = static_cast<int32_t>(b.points.size());
-      t.points = std::move(b.points);
-      t.weights = std::move(b.weights);
-      t.bfn_screening.nbe = nbe;
-      t.dist_nearest = meta_->dist_nearest()[ia];
-      atom_tasks[ib] = std::move(t);
-      keep[ib] = 1;
-    }
-    for (size_t ib = 0; ib < nb; ++ib)
-      if (keep[ib]) local.push_back(std::move(atom_tasks[ib]));
+    for (size_t ib = 0; ib < nb; ++ib) batches[b0 + ib] = batcher.at(ib, center);
+    parent.resize(b0 + nb, static_cast<int32_t>(ia));
+  }
+  std::vector<std::vector<int32_t>> lists;
+  std::vector<int32_t> nbe;
+  screen_batches(batches, lists, nbe);
+  std::vector<XCTask> local;
+  for (size_t ib = 0; ib < batches.size(); ++ib) {
+    GridBatch& b = batches[ib];
+    if (b.points.empty() || lists[ib].empty()) continue;
+    XCTask t;
+    t.iParent = parent[ib];
+    t.npts = static_cast<int32_t>(b.points.size());
+    t.points = std::move(b.points);
+    t.weights = std::move(b.weights);
+    t.bfn_screening.shell_list = std::move(lists[ib]);
+    t.bfn_screening.nbe = nbe[ib];
+    t.dist_nearest = meta_->dist_nearest()[parent[ib]];
+    local.push_back(std::move(t));
   }
   std::stable_sort(local.begin(), local.end(), [](const XCTask& a, const XCTask& b) {
     if (a.iParent != b.iParent) return a.iParent < b.iParent;
@@ -125,7 +180,7 @@ LoadBalancerFactory::LoadBalancerFactory(ExecutionSpace ex, std::string kernel_n
 LoadBalancer LoadBalancerFactory::get_instance(const RuntimeEnvironment& rt, const Molecule& mol,
                                                const MolGrid& mg,
                                                const BasisSet<double>& basis) {
-  return LoadBalancer(mol, mg, basis, rt);
+  return LoadBalancer(mol, mg, basis, rt, ex_);
 }
 
 std::shared_ptr<LoadBalancer> LoadBalancerFactory::get_shared_instance(

@@ -107,11 +107,12 @@ def make_config(name, seed=0, n_water=None, grid=None):
     return mol, basis, P, cfg
 
 
-def build_tasks(mol, basis, grid="UltraFineGrid", batch_size=512):
-    """Load balancer with SSF weights applied on the device."""
+def build_tasks(mol, basis, grid="UltraFineGrid", batch_size=512, ex="Host"):
+    """Load balancer (batch screening on the host, or on the device with
+    ex="Device"); its tasks are built on first use."""
     from . import api
     mg = api.MolGridFactory.create_default_molgrid(mol, "Unpruned", batch_size, "MuraKnowles",
                                                    grid)
-    lb = api.LoadBalancerFactory("Host", "Default").get_instance(api.RuntimeEnvironment(), mol,
-                                                                  mg, basis)
+    lb = api.LoadBalancerFactory(ex, "Default").get_instance(api.RuntimeEnvironment(), mol,
+                                                              mg, basis)
     return lb

@@ -9,7 +9,11 @@
 //  * tasks are ordered by (parent atom, shell list) and consecutive tasks
 //    with equal parent and shell list are merged (points concatenated).
 // With a single process every task is local, so GauXC's greedy rank
-// assignment reduces to keeping all of them.
+// assignment reduces to keeping all of them.  ExecutionSpace::Device runs
+// the screening on the GPU (GauXC's device load balancer,
+// src/load_balancer/device/hip/replicated_hip_load_balancer.cxx:72-220);
+// the batching, sort and merge stay on the host, so both spaces give the
+// same task list.
 #pragma once
 #include <gauxc/basisset.hpp>
 #include <gauxc/enums.hpp>
@@ -34,16 +38,23 @@ class LoadBalancer {
   std::shared_ptr<const BasisSet<double>> basis_;
   std::shared_ptr<const MolMeta> meta_;
   std::shared_ptr<const RuntimeEnvironment> rt_;
+  ExecutionSpace ex_ = ExecutionSpace::Host;  // where the batch screening runs
   mutable std::vector<XCTask> tasks_;
   mutable bool built_ = false;
   LoadBalancerState state_;
   uint64_t revision_ = 0;  // bumped whenever tasks may have changed
 
   void build_tasks() const;
+  /// Screening of every (batch, shell) pair: host loops, or the device
+  /// kernel of xcq_collision_detection (GauXC's device load balancer).
+  void screen_batches(const std::vector<GridBatch>& batches,
+                      std::vector<std::vector<int32_t>>& lists,
+                      std::vector<int32_t>& nbe) const;
 
  public:
   LoadBalancer(const Molecule& mol, const MolGrid& mg, const BasisSet<double>& basis,
-               const RuntimeEnvironment& rt = RuntimeEnvironment());
+               const RuntimeEnvironment& rt = RuntimeEnvironment(),
+               ExecutionSpace ex = ExecutionSpace::Host);
   LoadBalancer(const LoadBalancer&);
   LoadBalancer(LoadBalancer&&) noexcept = default;
   ~LoadBalancer() = default;
@@ -61,6 +72,8 @@ class LoadBalancer {
   const LoadBalancerState& state() const { return state_; }
   size_t max_npts() const;
   size_t total_npts() const;
+  /// Where the batch screening runs (the factory's execution space).
+  ExecutionSpace execution_space() const { return ex_; }
   /// Monotone counter that changes whenever the task list or its weights change.
   uint64_t revision() const { return revision_; }
   void mark_modified() { ++revision_; }

@@ -256,6 +256,20 @@ int xcq_download_exc_grad(xcq_context ctx, double* exc, double* nel, double* gra
 int xcq_integrate_den(xcq_context ctx, double* nel);
 int xcq_download_results(xcq_context ctx, double* exc, double* nel, double* vxc, int64_t ldvxc);
 
+/* ---- load-balancer screening on the device (GauXC's device load balancer:
+ *      src/load_balancer/device/hip/replicated_hip_load_balancer.cxx:72-220,
+ *      hip_collision_detection.hip:75-220).  Host buffers in and out: for
+ *      batch b with box [lo[3b..], up[3b..]] and shell s with cutoff sphere
+ *      (centers[3s..], radii[s]), bit s % 32 of bits[b * ld32 + s / 32] is set
+ *      when the sphere meets the box (ld32 = ceil(nshells / 32)); counts[b]
+ *      is the number of such shells and nbe[b] the sum of their
+ *      shell_sizes.  The test is geometry::cube_sphere_intersect's, operation
+ *      for operation. ---- */
+int xcq_collision_detection(int device, int64_t nbatches, const double* lo, const double* up,
+                            int32_t nshells, const double* centers, const double* radii,
+                            const int32_t* shell_sizes, uint32_t* bits, int32_t* counts,
+                            int32_t* nbe);
+
 /* ---- kernel-level entry points used by parity tests (host buffers) ---- */
 /* Collocation (+gradient if deriv = 1) of uploaded task `task`, written as
  * [comp][nbe][npts] (comp = value, d/dx, d/dy, d/dz) in FP64.  precision

@@ -61,11 +61,15 @@ CALLER_BIN = os.path.join(ROOT, "tests", "cxx", "_build", "caller")
 
 def build_caller():
     """The C++ caller program (tests/cxx/caller.cxx) against include/ and
-    libgauxc_amd.so; rebuilt when the source is newer than the binary."""
+    libgauxc_amd.so; rebuilt when the source or a header is newer than the
+    binary (the classes' layout is part of the ABI it uses)."""
     import subprocess
     lib = os.path.join(ROOT, "gauxc_amd", "lib")
+    import glob
+    deps = [CALLER_SRC] + glob.glob(os.path.join(ROOT, "include", "gauxc", "**", "*.h*"),
+                                    recursive=True)
     if (not os.path.exists(CALLER_BIN)
-            or os.path.getmtime(CALLER_BIN) < os.path.getmtime(CALLER_SRC)):
+            or os.path.getmtime(CALLER_BIN) < max(os.path.getmtime(d) for d in deps)):
         os.makedirs(os.path.dirname(CALLER_BIN), exist_ok=True)
         subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
                         CALLER_SRC, "-o", CALLER_BIN, "-L" + lib, "-lgauxc_amd",
