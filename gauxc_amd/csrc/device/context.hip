@@ -1724,6 +1724,83 @@ int xcq_work_counters(xcq_context ctx, int64_t* out, int n) {
   return m;
 }
 
+namespace {
+
+// Largest molecule whose neighbour lists (12 bytes per atom pair) the SSF
+// launches build; above it they loop over every atom.
+constexpr int32_t kSsfNbrMaxAtoms = 8192;
+
+// Per atom A, every atom in order of increasing R_AB with A itself first
+// (row-major natoms x natoms), for k_ssf's bounded loops.
+void ssf_neighbours(int32_t natoms, const std::vector<double>& rab, std::vector<int32_t>& idx,
+                    std::vector<double>& r) {
+  idx.resize((size_t)natoms * natoms);
+  r.resize(idx.size());
+  std::vector<int32_t> ord(natoms);
+  for (int32_t A = 0; A < natoms; ++A) {
+    const double* col = rab.data() + (size_t)A * natoms;  // R_AB = rab[B + A * natoms]
+    std::iota(ord.begin(), ord.end(), 0);
+    std::swap(ord[0], ord[A]);
+    std::stable_sort(ord.begin() + 1, ord.end(),
+                     [&](int32_t u, int32_t v) { return col[u] < col[v]; });
+    for (int32_t k = 0; k < natoms; ++k) {
+      idx[(size_t)A * natoms + k] = ord[k];
+      r[(size_t)A * natoms + k] = k == 0 ? 0.0 : col[ord[k]];
+    }
+  }
+}
+
+// R_AB (column-major) and nearest-neighbour distances of MolMeta.
+void ssf_molmeta(int32_t natoms, const double* atom_xyz, std::vector<double>& rab,
+                 std::vector<double>& dn) {
+  rab.assign((size_t)natoms * natoms, 0.);
+  dn.assign(natoms, INFINITY);
+  for (int i = 0; i < natoms; ++i)
+    for (int j = 0; j < natoms; ++j) {
+      const double dx = atom_xyz[3 * i] - atom_xyz[3 * j];
+      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1];
+      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
+      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (i != j) dn[i] = std::min(dn[i], rab[(size_t)i + (size_t)j * natoms]);
+    }
+}
+
+// Device copies of the SSF geometry: atom coordinates, R_AB, nearest
+// distances and (up to kSsfNbrMaxAtoms atoms) the neighbour lists.
+struct SsfGeometry {
+  double *xyz = nullptr, *rab = nullptr, *dn = nullptr, *nr = nullptr;
+  int32_t* ni = nullptr;
+  ~SsfGeometry() {
+    dfree(xyz);
+    dfree(rab);
+    dfree(dn);
+    dfree(nr);
+    dfree(ni);
+  }
+  int upload(int32_t natoms, const double* atom_xyz) {
+    std::vector<double> hr, hd, hnr;
+    std::vector<int32_t> hni;
+    ssf_molmeta(natoms, atom_xyz, hr, hd);
+    const bool lists = natoms <= kSsfNbrMaxAtoms;
+    if (lists) ssf_neighbours(natoms, hr, hni, hnr);
+    int rc;
+    if ((rc = dalloc(&xyz, 3 * (size_t)natoms)) || (rc = dalloc(&rab, hr.size())) ||
+        (rc = dalloc(&dn, (size_t)natoms)))
+      return rc;
+    if (lists && ((rc = dalloc(&ni, hni.size())) || (rc = dalloc(&nr, hnr.size())))) return rc;
+    XCQ_HIP(hipMemcpy(xyz, atom_xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));
+    XCQ_HIP(hipMemcpy(rab, hr.data(), 8 * hr.size(), hipMemcpyHostToDevice));
+    XCQ_HIP(hipMemcpy(dn, hd.data(), 8 * hd.size(), hipMemcpyHostToDevice));
+    if (lists) {
+      XCQ_HIP(hipMemcpy(ni, hni.data(), 4 * hni.size(), hipMemcpyHostToDevice));
+      XCQ_HIP(hipMemcpy(nr, hnr.data(), 8 * hnr.size(), hipMemcpyHostToDevice));
+    }
+    return XCQ_SUCCESS;
+  }
+};
+
+}  // namespace
+
 int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t npts,
                     const double* points, const int32_t* parent, double* weights) {
   XCQ_REQUIRE(natoms > 0 && atom_xyz, XCQ_ERROR_INVALID_ARGUMENT, "empty molecule");
@@ -1739,50 +1816,23 @@ int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t 
     return XCQ_ERROR_NO_DEVICE;
   }
   XCQ_HIP(hipSetDevice(device));
-  // Inter-atomic distances and nearest-neighbour distances (MolMeta).
-  std::vector<double> rab((size_t)natoms * natoms, 0.), dn(natoms, INFINITY);
-  for (int i = 0; i < natoms; ++i)
-    for (int j = 0; j < natoms; ++j) {
-      const double dx = atom_xyz[3 * i] - atom_xyz[3 * j];
-      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1];
-      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
-      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
-      if (i != j) dn[i] = std::min(dn[i], rab[(size_t)i + (size_t)j * natoms]);
-    }
-  std::vector<double> hx(npts), hy(npts), hz(npts);
-  for (int64_t k = 0; k < npts; ++k) {
-    hx[k] = points[3 * k];
-    hy[k] = points[3 * k + 1];
-    hz[k] = points[3 * k + 2];
-  }
-  double *d_xyz = nullptr, *d_rab = nullptr, *d_dn = nullptr, *d_x = nullptr, *d_y = nullptr,
-         *d_z = nullptr, *d_w = nullptr;
+  SsfGeometry g;
+  int rc = g.upload(natoms, atom_xyz);
+  if (rc) return rc;
+  // Points stay interleaved (x, y, z): the kernel reads them with stride 3.
+  double *d_pts = nullptr, *d_w = nullptr;
   int32_t* d_par = nullptr;
-  int rc = XCQ_SUCCESS;
   auto cleanup = [&] {
-    dfree(d_xyz);
-    dfree(d_rab);
-    dfree(d_dn);
-    dfree(d_x);
-    dfree(d_y);
-    dfree(d_z);
+    dfree(d_pts);
     dfree(d_w);
     dfree(d_par);
   };
-  if ((rc = dalloc(&d_xyz, 3 * (size_t)natoms)) || (rc = dalloc(&d_rab, rab.size())) ||
-      (rc = dalloc(&d_dn, (size_t)natoms)) || (rc = dalloc(&d_x, (size_t)npts)) ||
-      (rc = dalloc(&d_y, (size_t)npts)) || (rc = dalloc(&d_z, (size_t)npts)) ||
-      (rc = dalloc(&d_w, (size_t)npts)) || (rc = dalloc(&d_par, (size_t)npts))) {
+  if ((rc = dalloc(&d_pts, 3 * (size_t)npts)) || (rc = dalloc(&d_w, (size_t)npts)) ||
+      (rc = dalloc(&d_par, (size_t)npts))) {
     cleanup();
     return rc;
   }
-  hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = hipMemcpy(d_xyz, atom_xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_rab, rab.data(), 8 * rab.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_dn, dn.data(), 8 * (size_t)natoms, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_x, hx.data(), 8 * (size_t)npts, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_y, hy.data(), 8 * (size_t)npts, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_z, hz.data(), 8 * (size_t)npts, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(d_pts, points, 24 * (size_t)npts, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d_w, weights, 8 * (size_t)npts, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d_par, parent, 4 * (size_t)npts, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1790,8 +1840,8 @@ int xcq_ssf_weights(int device, int32_t natoms, const double* atom_xyz, int64_t 
     cleanup();
     return XCQ_ERROR_HIP;
   }
-  rc = launch_ssf(nullptr, natoms, d_xyz, d_rab, d_dn, npts, d_x, d_y, d_z, 1, d_par, d_w,
-                  nullptr);
+  rc = launch_ssf(nullptr, natoms, g.xyz, g.rab, g.dn, npts, d_pts, d_pts + 1, d_pts + 2, 3,
+                  d_par, d_w, nullptr, g.ni, g.nr);
   if (rc == XCQ_SUCCESS) {
     e = hipMemcpy(weights, d_w, 8 * (size_t)npts, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
@@ -1891,39 +1941,17 @@ int xcq_ssf_weights_tasks(xcq_context ctx, int32_t natoms, const double* atom_xy
                 "task parent atom out of range");
     for (int32_t i = 0; i < d.npts; ++i) parent[d.pt_off + i] = d.iparent;
   }
-  std::vector<double> rab((size_t)natoms * natoms, 0.), dn(natoms, INFINITY);
-  for (int i = 0; i < natoms; ++i)
-    for (int j = 0; j < natoms; ++j) {
-      const double dx = atom_xyz[3 * i] - atom_xyz[3 * j];
-      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1];
-      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
-      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
-      if (i != j) dn[i] = std::min(dn[i], rab[(size_t)i + (size_t)j * natoms]);
-    }
-  double *d_xyz = nullptr, *d_rab = nullptr, *d_dn = nullptr;
+  SsfGeometry g;
+  int rc = g.upload(natoms, atom_xyz);
+  if (rc) return rc;
   int32_t* d_par = nullptr;
-  int rc;
-  if ((rc = dalloc(&d_xyz, 3 * (size_t)natoms)) || (rc = dalloc(&d_rab, rab.size())) ||
-      (rc = dalloc(&d_dn, (size_t)natoms)) || (rc = dalloc(&d_par, parent.size()))) {
-    dfree(d_xyz);
-    dfree(d_rab);
-    dfree(d_dn);
-    dfree(d_par);
-    return rc;
-  }
-  hipError_t e = hipMemcpy(d_xyz, atom_xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_rab, rab.data(), 8 * rab.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_dn, dn.data(), 8 * (size_t)natoms, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(d_par, parent.data(), 4 * parent.size(), hipMemcpyHostToDevice);
+  if ((rc = dalloc(&d_par, parent.size()))) return rc;
+  hipError_t e = hipMemcpy(d_par, parent.data(), 4 * parent.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    rc = launch_ssf(ctx->stream, natoms, d_xyz, d_rab, d_dn, ctx->npts_total, ctx->d_px,
-                    ctx->d_py, ctx->d_pz, 1, d_par, ctx->d_w, nullptr);
+    rc = launch_ssf(ctx->stream, natoms, g.xyz, g.rab, g.dn, ctx->npts_total, ctx->d_px,
+                    ctx->d_py, ctx->d_pz, 1, d_par, ctx->d_w, nullptr, g.ni, g.nr);
     if (rc == XCQ_SUCCESS) e = hipStreamSynchronize(ctx->stream);
   }
-  dfree(d_xyz);
-  dfree(d_rab);
-  dfree(d_dn);
   dfree(d_par);
   if (e != hipSuccess) {
     set_error(std::string("SSF on resident tasks: ") + hipGetErrorString(e));

@@ -231,7 +231,8 @@ int upload_sph_table(hipStream_t s);
 int launch_ssf(hipStream_t s, int32_t natoms, const double* atom_xyz, const double* rab,
                const double* dnear, int64_t npts, const double* px, const double* py,
                const double* pz, int64_t stride, const int32_t* parent, double* w,
-               int32_t* bad_parent);
+               int32_t* bad_parent, const int32_t* nbr_idx = nullptr,
+               const double* nbr_r = nullptr);
 int launch_molmeta(hipStream_t s, int32_t natoms, const double* xyz, double* rab, double* dnear,
                    int32_t* bad_parent);
 int launch_collocation_fp64(hipStream_t s, const WorkItem* items, int64_t n,

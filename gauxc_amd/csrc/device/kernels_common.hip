@@ -30,6 +30,17 @@ __device__ __forceinline__ double ssf_s(double mu) {
   return 0.5 * (1.0 - f);
 }
 
+// With neighbour lists (nbr_idx / nbr_r: row A holds every atom in order of
+// increasing R_AB, A itself first) each loop stops at a distance the
+// triangle inequality fixes, instead of running over every atom:
+//  * the nearest atom B* has d_B* <= d_P (P the parent), so R_PB* <= 2 d_P;
+//  * a candidate A has d_A <= bound, so R_PA <= d_P + bound;
+//  * a factor s(mu_AB) differs from 1 only if mu_AB > -a, i.e.
+//    d_B < d_A + a R_AB, and d_B >= R_AB - d_A, so only R_AB < 2 d_A / (1 - a)
+//    can contribute (the factors skipped are exactly 1).
+// Each limit is widened by a relative 1e-10, far above rounding, so the
+// result differs from the all-atom loops only in the order of the product's
+// factors.  Without lists (nbr_idx null) the loops run over all atoms.
 __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __restrict__ xyz,
                                              const double* __restrict__ rab,
                                              const double* __restrict__ dnear, int64_t npts,
@@ -39,7 +50,9 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
                                              int64_t stride,
                                              const int32_t* __restrict__ parent,
                                              double* __restrict__ w,
-                                             int32_t* __restrict__ bad_parent) {
+                                             int32_t* __restrict__ bad_parent,
+                                             const int32_t* __restrict__ nbr_idx,
+                                             const double* __restrict__ nbr_r) {
   const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= npts) return;
   constexpr double a = 0.64;
@@ -55,6 +68,42 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
   };
   const double dpar = dist(par);
   if (dpar < 0.5 * (1.0 - a) * dnear[par]) return;
+  constexpr double kWiden = 1.0 + 1e-10;
+  if (nbr_idx) {
+    const int32_t* np = nbr_idx + (size_t)par * natoms;
+    const double* nr = nbr_r + (size_t)par * natoms;
+    double dmin = dpar;
+    int inear = par;
+    const double rnear = 2.0 * dpar * kWiden;
+    for (int k = 1; k < natoms && nr[k] <= rnear; ++k) {
+      const int B = np[k];
+      const double d = dist(B);
+      if (d < dmin) {
+        dmin = d;
+        inear = B;
+      }
+    }
+    const double bound = (1.0 + a) / (1.0 - a) * dmin * kWiden + 1e-12;
+    const double rcand = (dpar + bound) * kWiden;
+    double sum = 0.0, ppar = 0.0;
+    for (int k = 0; k < natoms && nr[k] <= rcand; ++k) {
+      const int A = np[k];
+      const double dA = A == par ? dpar : dist(A);
+      if (dA > bound) continue;
+      if (A != inear && ssf_s((dA - dmin) / rab[(size_t)A + (size_t)inear * natoms]) == 0.0)
+        continue;
+      const int32_t* ap = nbr_idx + (size_t)A * natoms;
+      const double* ar = nbr_r + (size_t)A * natoms;
+      const double rfac = 2.0 * dA / (1.0 - a) * kWiden;
+      double PA = 1.0;
+      for (int q = 1; q < natoms && ar[q] < rfac && PA > 0.0; ++q)
+        PA *= ssf_s((dA - dist(ap[q])) / ar[q]);
+      sum += PA;
+      if (A == par) ppar = PA;
+    }
+    w[ip] *= ppar / sum;
+    return;
+  }
   double dmin = dpar;
   int inear = par;
   for (int A = 0; A < natoms; ++A) {
@@ -88,11 +137,11 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
 int launch_ssf(hipStream_t s, int32_t natoms, const double* xyz, const double* rab,
                const double* dnear, int64_t npts, const double* px, const double* py,
                const double* pz, int64_t stride, const int32_t* parent, double* w,
-               int32_t* bad_parent) {
+               int32_t* bad_parent, const int32_t* nbr_idx, const double* nbr_r) {
   if (npts == 0) return XCQ_SUCCESS;
   const int64_t nb = (npts + 255) / 256;
   hipLaunchKernelGGL(k_ssf, dim3((unsigned)nb), dim3(256), 0, s, natoms, xyz, rab, dnear, npts,
-                     px, py, pz, stride, parent, w, bad_parent);
+                     px, py, pz, stride, parent, w, bad_parent, nbr_idx, nbr_r);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -32,19 +32,23 @@ void MolecularWeights::modify_weights(LoadBalancer& lb) const {
 
   auto& tasks = lb.get_tasks();
   const auto& mol = lb.molecule();
-  size_t npts = 0;
-  for (const auto& t : tasks) npts += t.points.size();
+  const size_t ntasks = tasks.size();
+  std::vector<size_t> off(ntasks + 1, 0);
+  for (size_t it = 0; it < ntasks; ++it) off[it + 1] = off[it] + tasks[it].points.size();
+  const size_t npts = off[ntasks];
   std::vector<double> pts(3 * npts), w(npts);
   std::vector<int32_t> parent(npts);
-  size_t k = 0;
-  for (const auto& t : tasks)
-    for (size_t i = 0; i < t.points.size(); ++i, ++k) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t it = 0; it < ntasks; ++it) {
+    const auto& t = tasks[it];
+    for (size_t i = 0, k = off[it]; i < t.points.size(); ++i, ++k) {
       pts[3 * k] = t.points[i][0];
       pts[3 * k + 1] = t.points[i][1];
       pts[3 * k + 2] = t.points[i][2];
       w[k] = t.weights[i];
       parent[k] = t.iParent;
     }
+  }
   std::vector<double> xyz(3 * mol.natoms());
   for (size_t a = 0; a < mol.natoms(); ++a) {
     xyz[3 * a] = mol[a].x;
@@ -59,9 +63,11 @@ void MolecularWeights::modify_weights(LoadBalancer& lb) const {
                       static_cast<int64_t>(npts), pts.data(), parent.data(), w.data()) !=
       XCQ_SUCCESS)
     GAUXC_GENERIC_EXCEPTION(std::string("SSF weights failed: ") + xcq_last_error());
-  k = 0;
-  for (auto& t : tasks)
-    for (size_t i = 0; i < t.weights.size(); ++i, ++k) t.weights[i] = w[k];
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t it = 0; it < ntasks; ++it) {
+    auto& t = tasks[it];
+    for (size_t i = 0; i < t.weights.size(); ++i) t.weights[i] = w[off[it] + i];
+  }
   lb.state().modified_weights_are_stored = true;
   lb.mark_modified();
 }
