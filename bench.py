@@ -466,7 +466,6 @@ def run_batched(args):
     full = context(range(len(mols)))
     feats = np.concatenate([task_features(t, bs.xcq_shells()) for bs, _, t in mols])
     pred = mlp.predict(feats)
-    nq = 4 * 256
     out = {}
     full.set_density_occupancy(mixed_extra["hiocc"])
     full.set_density_wide(mixed_extra["wide"])
@@ -476,7 +475,14 @@ def run_batched(args):
         full.use_graph(False)
         full.set_schedule(0)
         direct = time_eval(full, prec, reps=2)
-        full.set_schedule(nq, pred[:, 0], pred[:, 1])
+        # Persistent workgroups per CU: the stage kernels keep different
+        # numbers resident (more for FP64), so 4 and 8 per CU are both tried.
+        qt = {}
+        for q in (4 * 256, 8 * 256):
+            full.set_schedule(q, pred[:, 0], pred[:, 1])
+            qt[q] = time_eval(full, prec, reps=2)
+        nq_p = min(qt, key=qt.get)
+        full.set_schedule(nq_p, pred[:, 0], pred[:, 1])
         full.use_graph(True)
         for _ in range(args.warmup):
             full.exc_vxc("PBE", prec, True)
@@ -487,9 +493,9 @@ def run_batched(args):
         full.synchronize()
         ms = (time.perf_counter() - tic) * 1e3 / args.steps
         res = [full.results(i, want_vxc=True) for i in range(len(mols))]
-        out[prec] = dict(ms=ms, direct_ms=direct, res=res)
-        log(f"c3 {prec}: {ms:.1f} ms/step (queues {nq}, learned costs, graph); "
-            f"direct dispatch {direct:.1f} ms")
+        out[prec] = dict(ms=ms, direct_ms=direct, res=res, queues=nq_p)
+        log(f"c3 {prec}: {ms:.1f} ms/step (queues {nq_p}, learned costs, graph; eager "
+            f"{ {k: round(v, 1) for k, v in qt.items()} }); direct dispatch {direct:.1f} ms")
         full.use_graph(False)
     de = max(abs(a[0] - b[0]) for a, b in zip(out["mixed"]["res"], out["fp64"]["res"]))
     dv = max(float(np.abs(a[2] - b[2]).max()) for a, b in zip(out["mixed"]["res"],
@@ -502,7 +508,8 @@ def run_batched(args):
             "config": {"workload": f"{len(mols)} water clusters of {int(nws.min())}-"
                                    f"{int(nws.max())} waters, PBE, 99x590 grid, "
                                    f"{ntasks} tasks, {npts_total} points",
-                       "molecules": len(mols), "queues": nq, "graph": True,
+                       "molecules": len(mols),
+                       "queues": {p: out[p]["queues"] for p in out}, "graph": True,
                        "autotuned_on_subset": {k: list(v) for k, v in tuned.items()}},
             "fp64_ms_per_step": out["fp64"]["ms"],
             "direct_dispatch_ms": {p: out[p]["direct_ms"] for p in out},

@@ -1236,7 +1236,7 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
   const std::vector<double> keep_d = ctx->cost_d, keep_s = ctx->cost_s;
   int32_t best_q = 0;
   double bt = 1e300;
-  for (int32_t q : {0, 4 * dev_cus}) {
+  for (int32_t q : {0, 4 * dev_cus, 8 * dev_cus}) {
     if ((rc = xcq_set_schedule(ctx, q, keep_d.empty() ? nullptr : keep_d.data(),
                                keep_s.empty() ? nullptr : keep_s.data())))
       return rc;
