@@ -67,6 +67,10 @@ def lib():
                                              ctypes.c_int32, vp, vp, vp, vp, vp, vp,
                                              ctypes.c_int64, ctypes.c_int]
         L.oracle_fxc_contraction.restype = ctypes.c_int
+        L.oracle_fxc_contraction_uks.argtypes = [ctypes.c_int, ctypes.c_int32, vp, vp, vp, vp,
+                                                 vp, ctypes.c_int64, ctypes.c_int32, vp, vp,
+                                                 vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
+        L.oracle_fxc_contraction_uks.restype = ctypes.c_int
         L.oracle_collocation_hessian.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp]
         L.oracle_collocation_hessian.restype = None
         L.oracle_exc_grad.argtypes = [ctypes.c_int, ctypes.c_int32, vp, ctypes.c_int32, vp, vp,
@@ -233,6 +237,23 @@ def fxc_contraction(func, nbf, shells, P, tP, t, weights=None, nthreads=0):
                                       nbf, nthreads)
     assert rc == 0
     return nel.value, fxc
+
+
+def fxc_contraction_uks(func, nbf, shells, Ps, Pz, tPs, tPz, t, weights=None, nthreads=0):
+    """(FXC_s, FXC_z) of the restated UKS FXC contraction over task dict t."""
+    mats = [np.asfortranarray(m, dtype=np.float64) for m in (Ps, Pz, tPs, tPz)]
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    fs, fz = np.zeros((nbf, nbf), order="F"), np.zeros((nbf, nbf), order="F")
+    tasks = _task_array(t)
+    rc = lib().oracle_fxc_contraction_uks(FUNC[func], nbf, ctypes.addressof(shells),
+                                          *[m.ctypes.data for m in mats], nbf, len(t["npts"]),
+                                          tasks.ctypes.data, pts.ctypes.data, w.ctypes.data,
+                                          sl.ctypes.data, fs.ctypes.data, fz.ctypes.data, nbf,
+                                          nthreads)
+    assert rc == 0
+    return fs, fz
 
 
 def collocation_hessian(shells, shell_list, points):

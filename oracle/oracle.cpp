@@ -1080,6 +1080,25 @@ int oracle_functional_fxc(int func, int64_t n, const double* rho, const double* 
   return 0;
 }
 
+// Spin-polarised point kernel of the UKS FXC contraction: the derivatives
+// of the polarised vrho (a, b) and vsigma (aa, ab, bb) along the direction
+// (dra, drb, dsaa, dsab, dsbb), by nested duals over pol_energy (inner slots
+// the five variables, outer slot the direction).  out: vr[2], vs[3],
+// dvr[2], dvs[3].
+void fxc_point_pol(int func, const double* x, const double* dx, double* out) {
+  using D2 = DualT<Dual>;
+  D2 v[5];
+  for (int k = 0; k < 5; ++k) {
+    v[k].v = Dual::var(x[k], k);
+    v[k].d[0] = Dual(dx[k]);
+  }
+  const D2 e = pol_energy(func, v[0], v[1], v[2], v[3], v[4]);
+  for (int k = 0; k < 5; ++k) {
+    out[k] = e.v.d[k];
+    out[5 + k] = e.d[0].d[k];
+  }
+}
+
 int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, const double* P,
                            int64_t ldp, const double* tP, int64_t ldtp, int32_t ntasks,
                            const oracle_task* tasks, const double* points,
@@ -1495,6 +1514,155 @@ int oracle_exc_grad(int func, int32_t nbf, const oracle_shell* shells, int32_t n
 #pragma omp critical
     for (int k = 0; k < 3 * natoms; ++k) grad[k] += gloc[k];
   }
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// UKS FXC contraction (GauXC's host driver with spin_dim 2,
+// reference_replicated_xc_host_integrator_fxc_contraction.hpp:94-600):
+// FXC_a,b = d VXC_a,b[P + e tP] / de through eval_tmat_{lda,gga}_vxc_uks and
+// eval_zmat_{lda,gga}_vxc_uks_ts (reference_local_host_work_driver.cxx:
+// 1118-1124, 1149-1215, 1354-1367, 1441-1481), returned as FXC_s = (FXC_a +
+// FXC_b) / 2 and FXC_z = (FXC_a - FXC_b) / 2 (:577-600).  Written here as
+// the derivative of the UKS Z coefficients of oracle_exc_vxc_uks along the
+// trial direction, which is the same quantity.
+int oracle_fxc_contraction_uks(int func, int32_t nbf, const oracle_shell* shells,
+                               const double* Ps, const double* Pz, const double* tPs,
+                               const double* tPz, int64_t ldp, int32_t ntasks,
+                               const oracle_task* tasks, const double* points,
+                               const double* weights, const int32_t* shell_lists,
+                               double* fxc_s, double* fxc_z, int64_t ldfxc, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int32_t j = 0; j < nbf; ++j)
+    for (int32_t i = 0; i < nbf; ++i) fxc_s[i + (size_t)j * ldfxc] = fxc_z[i + (size_t)j * ldfxc] = 0.;
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+  }
+  const double sig_tol2 = 2.1544346900318838e-43 * 2.1544346900318838e-43;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> B, Bx, By, Bz, X[4], Z[2], sub, V;
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      B.assign((size_t)npts * nbe, 0.);
+      if (gga) {
+        Bx.assign(B.size(), 0.);
+        By.assign(B.size(), 0.);
+        Bz.assign(B.size(), 0.);
+      }
+      oracle_collocation(shells, t.nshells, sl, npts, pts, B.data(), gga ? Bx.data() : nullptr,
+                         gga ? By.data() : nullptr, gga ? Bz.data() : nullptr);
+      const double* Pm[4] = {Ps, Pz, tPs, tPz};
+      for (int c = 0; c < 4; ++c) xmat(B, Pm[c], ldp, bf, npts, 1.0, sub, X[c]);
+      for (int c = 0; c < 2; ++c) Z[c].resize((size_t)npts * nbe);
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t o = (size_t)p * nbe;
+        double r[4] = {0., 0., 0., 0.}, g[4][3] = {};
+        for (int c = 0; c < 4; ++c)
+          for (int32_t k = 0; k < nbe; ++k) {
+            const double xv = X[c][o + k];
+            r[c] += B[o + k] * xv;
+            if (gga) {
+              g[c][0] += Bx[o + k] * xv;
+              g[c][1] += By[o + k] * xv;
+              g[c][2] += Bz[o + k] * xv;
+            }
+          }
+        // n, m = P_s, P_z densities; tn, tm their trial counterparts.
+        double gn[3], gm[3], tgn[3], tgm[3];
+        for (int d = 0; d < 3; ++d) {
+          gn[d] = 2. * g[0][d];
+          gm[d] = 2. * g[1][d];
+          tgn[d] = 2. * g[2][d];
+          tgm[d] = 2. * g[3][d];
+        }
+        auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        const double nn = dot(gn, gn), mm = dot(gm, gm), nm = dot(gn, gm);
+        const double dnn = 2. * dot(gn, tgn), dmm = 2. * dot(gm, tgm),
+                     dnm = dot(tgn, gm) + dot(gn, tgm);
+        double x[5] = {std::max(0.5 * (r[0] + r[1]), 0.), std::max(0.5 * (r[0] - r[1]), 0.), 0., 0., 0.};
+        double dx[5] = {0.5 * (r[2] + r[3]), 0.5 * (r[2] - r[3]), 0., 0., 0.};
+        if (gga) {
+          x[2] = std::max(0.25 * (nn + mm) + 0.5 * nm, sig_tol2);
+          x[4] = std::max(0.25 * (nn + mm) - 0.5 * nm, sig_tol2);
+          const double hs = 0.5 * (x[2] + x[4]);
+          x[3] = std::max(std::min(0.25 * (nn - mm), hs), -hs);
+          dx[2] = 0.25 * (dnn + dmm) + 0.5 * dnm;
+          dx[3] = 0.25 * (dnn - dmm);
+          dx[4] = 0.25 * (dnn + dmm) - 0.5 * dnm;
+        }
+        double out[10];
+        fxc_point_pol(func, x, dx, out);
+        // d/de of the UKS Z coefficients of oracle_exc_vxc_uks.
+        const double wp = w[p];
+        const double das = 0.25 * wp * (out[5] + out[6]), daz = 0.25 * wp * (out[5] - out[6]);
+        double dcs[3] = {0., 0., 0.}, dcz[3] = {0., 0., 0.};
+        if (gga) {
+          const double g1 = 0.5 * wp * (out[2] + out[3] + out[4]), g2 = 0.5 * wp * (out[2] - out[4]),
+                       g3 = 0.5 * wp * (out[2] - out[3] + out[4]);
+          const double dg1 = 0.5 * wp * (out[7] + out[8] + out[9]),
+                       dg2 = 0.5 * wp * (out[7] - out[9]),
+                       dg3 = 0.5 * wp * (out[7] - out[8] + out[9]);
+          for (int d = 0; d < 3; ++d) {
+            dcs[d] = dg1 * gn[d] + g1 * tgn[d] + dg2 * gm[d] + g2 * tgm[d];
+            dcz[d] = dg3 * gm[d] + g3 * tgm[d] + dg2 * gn[d] + g2 * tgn[d];
+          }
+        }
+        for (int32_t k = 0; k < nbe; ++k) {
+          double zs = das * B[o + k], zz = daz * B[o + k];
+          if (gga) {
+            zs += dcs[0] * Bx[o + k] + dcs[1] * By[o + k] + dcs[2] * Bz[o + k];
+            zz += dcz[0] * Bx[o + k] + dcz[1] * By[o + k] + dcz[2] * Bz[o + k];
+          }
+          Z[0][o + k] = zs;
+          Z[1][o + k] = zz;
+        }
+      }
+      double* outm[2] = {fxc_s, fxc_z};
+      for (int c = 0; c < 2; ++c) {
+        V.assign((size_t)nbe * nbe, 0.);
+        for (int32_t p = 0; p < npts; ++p) {
+          const size_t o = (size_t)p * nbe;
+          for (int32_t i = 0; i < nbe; ++i) {
+            const double bi = B[o + i], zi = Z[c][o + i];
+            double* vrow = V.data() + (size_t)i * nbe;
+            for (int32_t j = 0; j <= i; ++j) vrow[j] += bi * Z[c][o + j] + zi * B[o + j];
+          }
+        }
+        for (int32_t i = 0; i < nbe; ++i)
+          for (int32_t j = 0; j <= i; ++j) {
+#pragma omp atomic
+            outm[c][bf[i] + (size_t)bf[j] * ldfxc] += V[(size_t)i * nbe + j];
+          }
+      }
+    }
+  }
+  for (double* v : {fxc_s, fxc_z})
+    for (int32_t j = 0; j < nbf; ++j)
+      for (int32_t i = j + 1; i < nbf; ++i) v[j + (size_t)i * ldfxc] = v[i + (size_t)j * ldfxc];
   return 0;
 }
 

@@ -95,6 +95,15 @@ int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, co
                            const double* weights, const int32_t* shell_lists, double* nel,
                            double* fxc, int64_t ldfxc, int nthreads);
 
+/* UKS FXC contraction: FXC_s, FXC_z = d (VXC_s, VXC_z)[P + e tP] / de for
+ * P = (P_s, P_z), tP = (tP_s, tP_z) (all with leading dimension ldp). */
+int oracle_fxc_contraction_uks(int func, int32_t nbf, const oracle_shell* shells,
+                               const double* Ps, const double* Pz, const double* tPs,
+                               const double* tPz, int64_t ldp, int32_t ntasks,
+                               const oracle_task* tasks, const double* points,
+                               const double* weights, const int32_t* shell_lists,
+                               double* fxc_s, double* fxc_z, int64_t ldfxc, int nthreads);
+
 /* Collocation with gradient and Hessian of one shell list: out[0..9] =
  * B, dB/dx, dB/dy, dB/dz, d2B/dxx, dxy, dxz, dyy, dyz, dzz, each [npts][nbe]
  * (gau2grid's deriv2 semantics). */

@@ -22,7 +22,8 @@ Fixtures converted (reference paths relative to the GauXC tree):
   ``tests/load_balancer_test.cxx:40-82``.
 * ``tests/ref_data/li_svwn5_sto3g_uks.bin`` (HDF5) -> ``li_svwn5_uks.npz``:
   the UKS check of ``tests/xc_integrator.cxx:218-237,448-453`` (scalar and
-  z densities and potentials, EXC).
+  z densities and potentials, EXC) and the UKS FXC check of
+  ``tests/2nd_derivative_test.cxx:128-134`` (trial densities, FXC_s, FXC_z).
 * ``tests/ref_data/water_cc-pVDZ_collocation.hdf5`` ->
   ``water_collocation.npz``: gau2grid collocation + gradient golden values
   (``tests/collocation_host.hpp:111-180``).
@@ -330,7 +331,10 @@ def main():
     Z, xyz = read_mol(f)
     d = dict(Z=Z, xyz=xyz, **read_basis(f))
     for src, dst in (("/DENSITY_SCALAR", "density_s"), ("/DENSITY_Z", "density_z"),
-                     ("/VXC_SCALAR", "vxc_s"), ("/VXC_Z", "vxc_z")):
+                     ("/VXC_SCALAR", "vxc_s"), ("/VXC_Z", "vxc_z"),
+                     ("/TRIAL_DENSITY_SCALAR", "trial_density_s"),
+                     ("/TRIAL_DENSITY_Z", "trial_density_z"),
+                     ("/FXC_SCALAR", "fxc_s"), ("/FXC_Z", "fxc_z")):
         d[dst] = f.read(src).T.copy()
     d["exc"] = f.read("/EXC")
     f.close()

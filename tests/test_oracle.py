@@ -13,8 +13,9 @@ Each check is chosen so that a plausible mistake fails it:
     limit and finite-difference derivatives; UKS with equal spins equals RKS;
     VXC_s / VXC_z are the finite-difference derivatives of the UKS EXC;
   * FXC contraction: GauXC's benzene SVWN5 / PBE0 FXC fixtures
-    (tests/2nd_derivative_test.cxx:121-126); the second derivatives by
-    differences of the first; FXC as the derivative of VXC along tP;
+    (tests/2nd_derivative_test.cxx:121-126) and the Li UKS FXC fixture;
+    the second derivatives by differences of the first; FXC as the
+    derivative of VXC along tP (RKS and UKS); UKS with P_z = 0 equals RKS;
   * EXC gradient: Hessian collocation against gau2grid's deriv2 and central
     differences; GauXC's benzene EXC_GRAD fixtures (with grid-weight
     derivatives); translational invariance; dEXC/dR by central differences.
@@ -380,6 +381,52 @@ def test_oracle_fxc_is_the_derivative_of_vxc(orc):
     _, _, vm = orc.exc_vxc("PBE0", bs.nbf(), sh, P - h * tP, t, weights=w)
     fd = (vp - vm) / (2 * h)
     assert np.abs(fxc - fd).max() <= 1e-6 * np.abs(fxc).max()
+
+
+def test_oracle_uks_fxc_matches_gauxc_li_fixture(gx, orc):
+    """UKS FXC: GauXC's Li / SVWN5 check (tests/2nd_derivative_test.cxx:128-134):
+    ||FXC_s - ref|| / nbf and ||FXC_z - ref|| / nbf < 1e-10."""
+    d = load_golden("li_svwn5_uks.npz")
+    mol = gx.Molecule(d["Z"], d["xyz"])
+    bs = gx.BasisSet.from_arrays(d["sh_nprim"], d["sh_l"], d["sh_pure"], d["sh_alpha"],
+                                 d["sh_coeff"], d["sh_origin"], normalize=False,
+                                 shell_tolerance=EPS)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles",
+                                                  "UltraFineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    t = lb.get_tasks()
+    fs, fz = orc.fxc_contraction_uks("SVWN5", bs.nbf(), orc.shells_from(bs.xcq_shells()),
+                                     d["density_s"], d["density_z"], d["trial_density_s"],
+                                     d["trial_density_z"], t)
+    assert np.linalg.norm(fs - d["fxc_s"]) / bs.nbf() < 1e-13
+    assert np.linalg.norm(fz - d["fxc_z"]) / bs.nbf() < 1e-13
+
+
+def test_oracle_uks_fxc_is_the_derivative_of_vxc_and_reduces_to_rks(gx, orc):
+    """UKS FXC (PBE, spin-polarised water dimer) equals central differences
+    of the UKS VXC_s / VXC_z along the trial direction; with P_z = tP_z = 0
+    and P_s = 2P, tP_s = 2tP it equals the RKS FXC of (P, tP)."""
+    xyz, bs, t, pa, pb = _uks_system(gx)
+    w = orc.ssf_weights(xyz, t)
+    sh = orc.shells_from(bs.xcq_shells())
+    nbf = bs.nbf()
+    rng = np.random.default_rng(5)
+    ta = rng.standard_normal((nbf, nbf)) * 0.01
+    tb = rng.standard_normal((nbf, nbf)) * 0.01
+    ta, tb = ta + ta.T, tb + tb.T
+    Ps, Pz, tPs, tPz = pa + pb, pa - pb, ta + tb, ta - tb
+    fs, fz = orc.fxc_contraction_uks("PBE", nbf, sh, Ps, Pz, tPs, tPz, t, weights=w)
+    h = 1e-4
+    _, _, vsp, vzp = orc.exc_vxc_uks("PBE", nbf, sh, Ps + h * tPs, Pz + h * tPz, t, weights=w)
+    _, _, vsm, vzm = orc.exc_vxc_uks("PBE", nbf, sh, Ps - h * tPs, Pz - h * tPz, t, weights=w)
+    assert np.abs(fs - (vsp - vsm) / (2 * h)).max() <= 1e-6 * np.abs(fs).max()
+    assert np.abs(fz - (vzp - vzm) / (2 * h)).max() <= 1e-6 * np.abs(fz).max()
+    _, fr = orc.fxc_contraction("PBE", nbf, sh, pa, ta, t, weights=w)
+    fs0, fz0 = orc.fxc_contraction_uks("PBE", nbf, sh, 2 * pa, 0 * pa, 2 * ta, 0 * ta, t,
+                                       weights=w)
+    assert np.abs(fs0 - fr).max() <= 1e-11 * np.abs(fr).max()
+    assert np.abs(fz0).max() <= 1e-14 * np.abs(fr).max()
 
 
 @pytest.mark.parametrize("pure", [True, False])
