@@ -109,7 +109,8 @@ HOST_SYMBOLS = [
     "gauxc_integrator_eval_exc_rks", "gauxc_integrator_eval_exc_vxc_rks",
     "gauxc_integrator_eval_exc_uks", "gauxc_integrator_eval_exc_vxc_uks",
     "gauxc_integrator_eval_fxc_contraction_rks", "gauxc_integrator_eval_exc_grad_rks",
-    "gauxc_integrator_eval_exc_grad_hf_rks", "gauxc_integrator_device_context",
+    "gauxc_integrator_eval_exc_grad_hf_rks", "gauxc_integrator_eval_fxc_contraction_uks",
+    "gauxc_integrator_device_context",
 ]
 DEVICE_SYMBOLS = [
     "xcq_device_count", "xcq_last_error", "xcq_context_create", "xcq_context_create_mem",
@@ -119,6 +120,7 @@ DEVICE_SYMBOLS = [
     "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_upload_trial_density", "xcq_fxc_contraction", "xcq_download_fxc",
     "xcq_exc_grad", "xcq_download_exc_grad", "xcq_collision_detection",
+    "xcq_upload_trial_density_z", "xcq_fxc_contraction_uks", "xcq_download_fxc_uks",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
     "xcq_eval_functional", "xcq_stage_times", "xcq_work_counters",
@@ -172,6 +174,9 @@ def device_lib():
             "xcq_fxc_contraction": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_download_fxc": (ctypes.c_int, [H, c_vp, c_vp, c_i64]),
             "xcq_exc_grad": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
+            "xcq_upload_trial_density_z": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
+            "xcq_fxc_contraction_uks": (ctypes.c_int, [H, ctypes.c_int]),
+            "xcq_download_fxc_uks": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_i64]),
             "xcq_download_exc_grad": (ctypes.c_int, [H, c_vp, c_vp, c_vp]),
             "xcq_collision_detection": (ctypes.c_int, [ctypes.c_int, c_i64, c_vp, c_vp, c_i32,
                                                        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
@@ -286,6 +291,8 @@ def host_lib():
                                                           c_vp]),
             "gauxc_integrator_eval_exc_grad_hf_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64,
                                                              c_vp]),
+            "gauxc_integrator_eval_fxc_contraction_uks": (None, [S, Hd, c_i64, c_i64] +
+                                                          [c_vp, c_i64] * 6),
             "gauxc_integrator_device_context": (c_vp, [S, Hd]),
         }
         for name, (res, args) in sig.items():

@@ -336,9 +336,22 @@ class XCIntegrator(_Handle):
         st.raise_if()
         return g.reshape(-1, 3)
 
-    def eval_fxc_contraction(self, P, tP):
+    def eval_fxc_contraction(self, P, tP, Pz=None, tPz=None):
         """RKS FXC contraction d VXC[P + e tP] / de, as GauXC's
-        eval_fxc_contraction(P, tP)."""
+        eval_fxc_contraction(P, tP); with Pz and tPz (P, tP then the total
+        densities) the UKS form, returning (FXC_s, FXC_z)."""
+        if Pz is not None or tPz is not None:
+            mats = [self._mat(x) for x in (P, Pz, tP, tPz)]
+            m, n = mats[0].shape
+            fs, fz = np.zeros((m, n), order="F"), np.zeros((m, n), order="F")
+            st = N.Status()
+            args = []
+            for x in mats:
+                args += [x.ctypes.data, x.shape[0]]
+            N.host_lib().gauxc_integrator_eval_fxc_contraction_uks(
+                st.ptr, self.h, m, n, *args, fs.ctypes.data, m, fz.ctypes.data, m)
+            st.raise_if()
+            return fs, fz
         P, tP = self._mat(P), self._mat(tP)
         m, n = P.shape
         fxc = np.zeros((m, n), order="F")
@@ -503,6 +516,27 @@ class DeviceContext:
     def fxc_contraction(self, func="PBE"):
         N.xcq_check(self.lib.xcq_fxc_contraction(self.h, N.XCQ_FUNC[func]),
                     "xcq_fxc_contraction")
+
+    def upload_trial_density_z(self, tPz):
+        """Trial spin density of the UKS FXC contraction."""
+        tPz = np.asfortranarray(tPz, dtype=np.float64)
+        N.xcq_check(self.lib.xcq_upload_trial_density_z(self.h, tPz.shape[0], tPz.ctypes.data,
+                                                        tPz.shape[0]),
+                    "xcq_upload_trial_density_z")
+
+    def fxc_contraction_uks(self, func="PBE"):
+        N.xcq_check(self.lib.xcq_fxc_contraction_uks(self.h, N.XCQ_FUNC[func]),
+                    "xcq_fxc_contraction_uks")
+
+    def results_fxc_uks(self):
+        """(NEL, FXC_s, FXC_z) of the last UKS FXC contraction."""
+        nel = ctypes.c_double(0.)
+        fs = np.zeros((self.nbf, self.nbf), order="F")
+        fz = np.zeros((self.nbf, self.nbf), order="F")
+        N.xcq_check(self.lib.xcq_download_fxc_uks(self.h, ctypes.addressof(nel),
+                                                  fs.ctypes.data, fz.ctypes.data, self.nbf),
+                    "xcq_download_fxc_uks")
+        return nel.value, fs, fz
 
     def results_fxc(self):
         """(NEL, FXC) of the last FXC contraction."""

@@ -193,6 +193,10 @@ struct xcq_context_s {
   bool last_uks = false;    // the last evaluation was a UKS one
   double* d_tP = nullptr;   // FXC contraction: trial density (single molecule)
   bool have_tp = false;
+  double* d_tPz = nullptr;  // UKS FXC contraction: trial tP_z
+  bool have_tpz = false;
+  double* d_fxcu = nullptr; // UKS FXC: [6][pblocks][256] density sums and Z coefficients
+  bool last_fxc_uks = false;  // the last evaluation was a UKS FXC contraction
   bool last_fxc = false;    // the last evaluation was an FXC contraction (in d_vxc)
   std::vector<double> sh_origin;  // shell centres (3 per shell), host copy
   // EXC gradient: atoms, shell -> atom map, nearest-neighbour distances, result.
@@ -247,6 +251,7 @@ struct xcq_context_s {
     dfree(d_red);
     dfree(d_zc);
     dfree(d_uks);
+    dfree(d_fxcu);
     pblocks = 0;
     dfree(d_tmax);
     tmax_total = 0;
@@ -279,6 +284,7 @@ struct xcq_context_s {
     dfree(d_Pz);
     dfree(d_vxcz);
     dfree(d_tP);
+    dfree(d_tPz);
     dfree(d_gxyz);
     dfree(d_dnear);
     dfree(d_grad);
@@ -524,6 +530,7 @@ int run_pipeline(xcq_context_s* ctx, int func, int precision, int want_vxc) {
   ctx->last_want_vxc = want_vxc;
   ctx->last_uks = false;
   ctx->last_fxc = false;
+  ctx->last_fxc_uks = false;
   ctx->last_grad = false;
   // The mixed path's EXC correction needs VXC, so EXC-only requests in that
   // precision evaluate it too (and do not return it).
@@ -724,6 +731,8 @@ int xcq_upload_molecules(xcq_context ctx, int32_t nmol, const int32_t* mol_nshel
     ctx->have_pz = false;
     dfree(ctx->d_tP);
     ctx->have_tp = false;
+    dfree(ctx->d_tPz);
+    ctx->have_tpz = false;
     dfree(ctx->d_out2);
     dfree(ctx->d_corr);
     dfree(ctx->d_rpart);
@@ -1386,6 +1395,7 @@ int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc) {
   ctx->last_graph = false;
   ctx->last_uks = true;
   ctx->last_fxc = false;
+  ctx->last_fxc_uks = false;
   ctx->last_grad = false;
   return enqueue_uks(ctx, functional, want_vxc ? 1 : 0);
 }
@@ -1492,8 +1502,134 @@ int xcq_fxc_contraction(xcq_context ctx, int functional) {
   ctx->last_graph = false;
   ctx->last_uks = false;
   ctx->last_fxc = true;
+  ctx->last_fxc_uks = false;
   ctx->last_grad = false;
   return enqueue_fxc(ctx, functional);
+}
+
+namespace {
+
+// One UKS FXC contraction on the context's stream (FP64, stored basis
+// planes): per chunk, collocation with gradient planes; the density stage
+// four times (P_s, P_z, tP_s, tP_z; factor 1) leaving raw per-point sums;
+// the derivatives of the UKS Z coefficients along the trial direction; Z_s
+// and its B^T Z + Z^T B into d_vxc (FXC_s), Z_z into d_vxcz (FXC_z);
+// symmetrised.  Every contribution is added.
+int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  CtxView v = view_of(ctx);
+  v.in_stage = 0;
+  v.tmax = nullptr;
+  v.vxc_drop = 0.0;
+  hipStream_t s = ctx->stream;
+  const size_t blk = (size_t)256 * ctx->pblocks;
+  double* dd[4] = {ctx->d_fxcu, ctx->d_fxcu + blk, ctx->d_fxcu + 2 * blk, ctx->d_fxcu + 3 * blk};
+  double* zs = ctx->d_fxcu + 4 * blk;
+  double* zz = ctx->d_fxcu + 5 * blk;
+  const double* mats[4] = {ctx->d_P, ctx->d_Pz, ctx->d_tP, ctx->d_tPz};
+  int rc;
+  if ((rc = launch_fill(s, ctx->d_vxc, ctx->p_total)) ||
+      (rc = launch_fill(s, ctx->d_vxcz, ctx->p_total)))
+    return rc;
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
+  for (const Chunk& c : ctx->chunks) {
+    const WorkItem* cw = v.work + c.coll.off;
+    if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
+                                      ctx->max_l)))
+      return rc;
+    for (int pass = 0; pass < 4; ++pass)
+      for (int k = 0; k < 4; ++k)
+        if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
+                                 1.0, 0, mats[pass], dd[pass])))
+          return rc;
+    if ((rc = launch_fxc_points_uks(v, c.coll, func, gga, dd[0], dd[1], dd[2], dd[3], zs, zz)))
+      return rc;
+    for (int comp = 0; comp < 2; ++comp) {
+      CtxView vc = v;
+      vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
+      if ((rc = launch_z_from_coeffs(vc, c.coll, gga, comp == 0 ? zs : zz))) return rc;
+      for (int k = 0; k < 2; ++k)
+        if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+    }
+  }
+  for (double* m : {ctx->d_vxc, ctx->d_vxcz})
+    if ((rc = launch_symmetrize(s, 1, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off, m))) return rc;
+  return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
+                            ctx->d_out2);
+}
+
+}  // namespace
+
+int xcq_upload_trial_density_z(xcq_context ctx, int32_t nbf, const double* tPz, int64_t ldp) {
+  XCQ_REQUIRE(ctx && tPz, XCQ_ERROR_INVALID_ARGUMENT, "null argument");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "the FXC contraction needs a single-molecule context");
+  const MolInfo& M = ctx->mols[0];
+  XCQ_REQUIRE(nbf == M.nbf && nbf > 0, XCQ_ERROR_INVALID_ARGUMENT,
+              "density dimension does not match the basis");
+  XCQ_REQUIRE(ldp >= nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldp < nbf");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_tPz && (rc = dalloc(&ctx->d_tPz, (size_t)ctx->p_total))) return rc;
+  XCQ_HIP(hipMemcpy2DAsync(ctx->d_tPz, (size_t)nbf * 8, tPz, (size_t)ldp * 8, (size_t)nbf * 8,
+                           nbf, hipMemcpyHostToDevice, ctx->stream));
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_tpz = true;
+  return XCQ_SUCCESS;
+}
+
+int xcq_fxc_contraction_uks(xcq_context ctx, int functional) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(functional == XCQ_FUNC_SVWN5 || functional == XCQ_FUNC_PBE ||
+                  functional == XCQ_FUNC_PBE0,
+              XCQ_ERROR_INVALID_ARGUMENT, "unknown functional");
+  XCQ_REQUIRE(ctx->ntasks > 0, XCQ_ERROR_STATE, "no tasks uploaded");
+  XCQ_REQUIRE(ctx->mols.size() == 1, XCQ_ERROR_STATE,
+              "the FXC contraction needs a single-molecule context");
+  XCQ_REQUIRE(ctx->mols[0].have_density && ctx->have_pz && ctx->have_tp && ctx->have_tpz,
+              XCQ_ERROR_STATE,
+              "the UKS FXC contraction needs P_s, P_z (xcq_upload_density, "
+              "xcq_upload_density_z) and tP_s, tP_z (xcq_upload_trial_density, "
+              "xcq_upload_trial_density_z)");
+  XCQ_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (!ctx->d_fxcu &&
+      (rc = dalloc(&ctx->d_fxcu, (size_t)6 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
+    return rc;
+  if (!ctx->d_vxcz && (rc = dalloc(&ctx->d_vxcz, (size_t)ctx->p_total))) return rc;
+  ctx->last_want_vxc = 0;
+  ctx->last_vxc_eval = 0;
+  ctx->last_graph = false;
+  ctx->last_uks = false;
+  ctx->last_fxc = false;
+  ctx->last_grad = false;
+  ctx->last_fxc_uks = true;
+  return enqueue_fxc_uks(ctx, functional);
+}
+
+int xcq_download_fxc_uks(xcq_context ctx, double* nel, double* fxc_s, double* fxc_z,
+                         int64_t ldfxc) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->last_fxc_uks, XCQ_ERROR_STATE,
+              "the last evaluation was not a UKS FXC contraction");
+  const MolInfo& M = ctx->mols[0];
+  XCQ_HIP(hipSetDevice(ctx->device));
+  double out[2];
+  XCQ_HIP(hipMemcpyAsync(out, ctx->d_out2, 16, hipMemcpyDeviceToHost, ctx->stream));
+  for (int c = 0; c < 2; ++c) {
+    double* dst = c == 0 ? fxc_s : fxc_z;
+    if (!dst) continue;
+    XCQ_REQUIRE(ldfxc >= M.nbf, XCQ_ERROR_INVALID_ARGUMENT, "ldfxc < nbf");
+    XCQ_HIP(hipMemcpy2DAsync(dst, (size_t)ldfxc * 8, c == 0 ? ctx->d_vxc : ctx->d_vxcz,
+                             (size_t)M.nbf * 8, (size_t)M.nbf * 8, M.nbf,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  }
+  XCQ_HIP(hipStreamSynchronize(ctx->stream));
+  if (nel) *nel = out[1];
+  return XCQ_SUCCESS;
 }
 
 int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc) {
@@ -1624,6 +1760,7 @@ int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* 
   ctx->last_graph = false;
   ctx->last_uks = false;
   ctx->last_fxc = false;
+  ctx->last_fxc_uks = false;
   ctx->last_grad = true;
   return enqueue_grad(ctx, functional, weight_derivs ? 1 : 0);
 }

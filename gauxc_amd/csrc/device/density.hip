@@ -1190,6 +1190,101 @@ __global__ __launch_bounds__(64) void k_fxc_points(const WorkItem* __restrict__ 
   }
 }
 
+// UKS FXC contraction tail (reference_replicated_xc_host_integrator_
+// fxc_contraction.hpp:94-600 with spin_dim 2; the trial kernels
+// eval_tmat_{lda,gga}_vxc_uks and eval_zmat_*_uks_ts,
+// reference_local_host_work_driver.cxx:1118-1124, 1149-1215, 1354-1367,
+// 1441-1481).  FXC_s and FXC_z are the derivatives of VXC_s and VXC_z along
+// the trial density, so this kernel writes the derivatives of k_uks_points'
+// Z coefficients along (tP_s, tP_z): inputs are the raw sums of the four
+// density passes (P_s, P_z, tP_s, tP_z; factor 1), the second derivatives
+// come from dev_fxc_point_pol.  Padded points give zero coefficients.
+__global__ __launch_bounds__(64) void k_fxc_points_uks(
+    const WorkItem* __restrict__ items, const DevTask* __restrict__ tasks,
+    const double* __restrict__ weights, int func, int gga, const double* __restrict__ ds,
+    const double* __restrict__ dz, const double* __restrict__ dts,
+    const double* __restrict__ dtz, double* __restrict__ red, double* __restrict__ zs,
+    double* __restrict__ zz) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  const int64_t b = t.red_off + wi.a;
+  const int p = wi.a * kPtBlock + tid;
+  double sv[4], zv[4], tsv[4], tzv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sv[c] = ds[b * 256 + c * 64 + tid];
+    zv[c] = dz[b * 256 + c * 64 + tid];
+    tsv[c] = dts[b * 256 + c * 64 + tid];
+    tzv[c] = dtz[b * 256 + c * 64 + tid];
+  }
+  double gn[3], gm[3], tgn[3], tgm[3];
+  double nn = 0., mm = 0., nm = 0., dnn = 0., dmm = 0., dnm = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    gn[d] = 2.0 * sv[1 + d];
+    gm[d] = 2.0 * zv[1 + d];
+    tgn[d] = 2.0 * tsv[1 + d];
+    tgm[d] = 2.0 * tzv[1 + d];
+    nn += gn[d] * gn[d];
+    mm += gm[d] * gm[d];
+    nm += gn[d] * gm[d];
+    dnn += 2.0 * gn[d] * tgn[d];
+    dmm += 2.0 * gm[d] * tgm[d];
+    dnm += tgn[d] * gm[d] + gn[d] * tgm[d];
+  }
+  double e = 0., n = 0., das = 0., daz = 0., dcs[3] = {0., 0., 0.}, dcz[3] = {0., 0., 0.};
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    double x[5] = {fmax(0.5 * (sv[0] + zv[0]), 0.0), fmax(0.5 * (sv[0] - zv[0]), 0.0), 0., 0.,
+                   0.};
+    double dx[5] = {0.5 * (tsv[0] + tzv[0]), 0.5 * (tsv[0] - tzv[0]), 0., 0., 0.};
+    if (gga) {
+      const double stol2 = 2.1544346900318838e-43 * 2.1544346900318838e-43;
+      x[2] = fmax(0.25 * (nn + mm) + 0.5 * nm, stol2);
+      x[4] = fmax(0.25 * (nn + mm) - 0.5 * nm, stol2);
+      const double hs = 0.5 * (x[2] + x[4]);
+      x[3] = fmax(fmin(0.25 * (nn - mm), hs), -hs);
+      dx[2] = 0.25 * (dnn + dmm) + 0.5 * dnm;
+      dx[3] = 0.25 * (dnn - dmm);
+      dx[4] = 0.25 * (dnn + dmm) - 0.5 * dnm;
+    }
+    n = w * sv[0];
+    const XCOutPol o = dev_eval_functional_pol(func, x[0], x[1], x[2], x[3], x[4]);
+    e = w * o.e;
+    double out[10];
+    dev_fxc_point_pol(func, x, dx, out);
+    das = 0.25 * w * (out[5] + out[6]);
+    daz = 0.25 * w * (out[5] - out[6]);
+    if (gga) {
+      const double g1 = 0.5 * w * (out[2] + out[3] + out[4]), g2 = 0.5 * w * (out[2] - out[4]),
+                   g3 = 0.5 * w * (out[2] - out[3] + out[4]);
+      const double dg1 = 0.5 * w * (out[7] + out[8] + out[9]), dg2 = 0.5 * w * (out[7] - out[9]),
+                   dg3 = 0.5 * w * (out[7] - out[8] + out[9]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        dcs[d] = dg1 * gn[d] + g1 * tgn[d] + dg2 * gm[d] + g2 * tgm[d];
+        dcz[d] = dg3 * gm[d] + g3 * tgm[d] + dg2 * gn[d] + g2 * tgn[d];
+      }
+    }
+  }
+  zs[b * 256 + tid] = das;
+  zz[b * 256 + tid] = daz;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    zs[b * 256 + (1 + d) * 64 + tid] = dcs[d];
+    zz[b * 256 + (1 + d) * 64 + tid] = dcz[d];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (tid == 0) {
+    red[2 * b] = e;
+    red[2 * b + 1] = n;
+  }
+}
+
 // Z = a B + c . grad B of one point block from stored coefficients
 // ([block][4][64]) and the stored FP64 planes (no screening bounds).
 template <bool GGA>
@@ -1221,6 +1316,16 @@ int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, c
   if (r.n == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_fxc_points, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
                      v.tasks, v.w, func, gga, dp, dt, v.red, zc);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_fxc_points_uks(const CtxView& v, const WorkRange& r, int func, int gga,
+                          const double* ds, const double* dz, const double* dts,
+                          const double* dtz, double* zs, double* zz) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_fxc_points_uks, dim3((unsigned)r.n), dim3(64), 0, v.stream,
+                     v.work + r.off, v.tasks, v.w, func, gga, ds, dz, dts, dtz, v.red, zs, zz);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -456,4 +456,226 @@ __device__ __forceinline__ void dev_fxc_point(int func, double rho, double sigma
   Bc = gga ? e.st : 0.;
 }
 
+// ---- generic forward-mode duals (UKS FXC contraction) -----------------------
+// GD<S, N>: value and N derivative slots over the scalar S.  S = double gives
+// first derivatives; GD<GD<double, 5>, 1> carries the five partials of the
+// polarised variables (inner) and their derivatives along one direction
+// (outer), i.e. the second derivatives the UKS trial kernels contract.
+template <class S, int N>
+struct GD {
+  S v;
+  S d[N];
+};
+template <class S>
+struct GCst;
+template <>
+struct GCst<double> {
+  __device__ __forceinline__ static double of(double x) { return x; }
+};
+template <class S, int N>
+struct GCst<GD<S, N>> {
+  __device__ __forceinline__ static GD<S, N> of(double x) {
+    GD<S, N> r;
+    r.v = GCst<S>::of(x);
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = GCst<S>::of(0.);
+    return r;
+  }
+};
+__device__ __forceinline__ double gval(double x) { return x; }
+template <class S, int N>
+__device__ __forceinline__ double gval(const GD<S, N>& a) { return gval(a.v); }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gchain(const GD<S, N>& a, const S& f, const S& df) {
+  GD<S, N> r;
+  r.v = f;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = df * a.d[k];
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator+(const GD<S, N>& a, const GD<S, N>& b) {
+  GD<S, N> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator-(const GD<S, N>& a, const GD<S, N>& b) {
+  GD<S, N> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator*(const GD<S, N>& a, const GD<S, N>& b) {
+  GD<S, N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> ginv(const GD<S, N>& b) {
+  const S i = 1.0 / b.v;
+  return gchain(b, i, -1.0 * (i * i));
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator/(const GD<S, N>& a, const GD<S, N>& b) {
+  return a * ginv(b);
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator+(const GD<S, N>& a, double b) {
+  GD<S, N> r = a;
+  r.v = a.v + b;
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator+(double a, const GD<S, N>& b) { return b + a; }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator-(const GD<S, N>& a, double b) { return a + (-b); }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator-(double a, const GD<S, N>& b) {
+  return GCst<GD<S, N>>::of(a) - b;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator*(const GD<S, N>& a, double b) {
+  GD<S, N> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b;
+  return r;
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator*(double a, const GD<S, N>& b) { return b * a; }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator/(const GD<S, N>& a, double b) { return a * (1.0 / b); }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> operator/(double a, const GD<S, N>& b) { return a * ginv(b); }
+
+__device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double gcbrt(double x) { return cbrt(x); }
+__device__ __forceinline__ double gpow(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ double glog(double x) { return log(x); }
+__device__ __forceinline__ double glog1p(double x) { return log1p(x); }
+__device__ __forceinline__ double gexp(double x) { return exp(x); }
+__device__ __forceinline__ double gatan(double x) { return atan(x); }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gsqrt(const GD<S, N>& a) {
+  const S q = gsqrt(a.v);
+  return gchain(a, q, 0.5 / q);
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gcbrt(const GD<S, N>& a) {
+  const S c = gcbrt(a.v);
+  return gchain(a, c, 1.0 / (3.0 * (c * c)));
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gpow(const GD<S, N>& a, double e) {
+  return gchain(a, gpow(a.v, e), e * gpow(a.v, e - 1.0));
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> glog(const GD<S, N>& a) { return gchain(a, glog(a.v), 1.0 / a.v); }
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> glog1p(const GD<S, N>& a) {
+  return gchain(a, glog1p(a.v), 1.0 / (1.0 + a.v));
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gexp(const GD<S, N>& a) {
+  const S e = gexp(a.v);
+  return gchain(a, e, e);
+}
+template <class S, int N>
+__device__ __forceinline__ GD<S, N> gatan(const GD<S, N>& a) {
+  return gchain(a, gatan(a.v), 1.0 / (1.0 + a.v * a.v));
+}
+
+// The spin-polarised energy density n eps of the forms above (exchange by
+// spin scaling, the spin-interpolated VWN(RPA) and PBE correlations), for
+// any dual type T; thresholds as dev_eval_functional_pol.
+template <class T>
+__device__ __forceinline__ T gpol_energy(int func, const T& ra, const T& rb, const T& saa,
+                                         const T& sab, const T& sbb) {
+  const double pi = 3.14159265358979323846;
+  const double ax = -0.73855876638202240587;  // -(3/4)(3/pi)^(1/3)
+  const bool gga = func != XCQ_FUNC_SVWN5;
+  const double xs = func == XCQ_FUNC_PBE0 ? 0.75 : 1.0;
+  T e = GCst<T>::of(0.);
+  auto ex = [&](const T& n, const T& s) {  // unpolarised exchange of (n, s)
+    const T n43 = n * gcbrt(n);
+    if (!gga) return ax * n43;
+    const double kappa = 0.804, mu = 0.2195149727645171;
+    const double C = 0.026121172985233605;  // 1 / (4 (3 pi^2)^(2/3))
+    const T F = (1.0 + kappa) - kappa / (1.0 + (mu / kappa) * (C * s / (n43 * n43)));
+    return ax * n43 * F;
+  };
+  if (2.0 * gval(ra) > 1e-15) e = e + (0.5 * xs) * ex(2.0 * ra, 4.0 * saa);
+  if (2.0 * gval(rb) > 1e-15) e = e + (0.5 * xs) * ex(2.0 * rb, 4.0 * sbb);
+  const T n = ra + rb;
+  if (gval(n) <= (gga ? 1e-12 : 1e-15)) return e;
+  const T z = (ra - rb) / n;
+  const T rs = gcbrt((3.0 / (4.0 * pi)) / n);
+  auto fz = [&](const T& zz) {
+    return (gpow(1.0 + zz, 4.0 / 3.0) + gpow(1.0 - zz, 4.0 / 3.0) - 2.0) *
+           (1.0 / 0.51984209978974632953);
+  };
+  auto vwn = [&](const T& x, double A, double b, double c, double x0) {
+    const T X = x * x + b * x + c;
+    const double X0 = x0 * x0 + b * x0 + c, Q = sqrt(4.0 * c - b * b);
+    const T at = gatan(Q / (2.0 * x + b));
+    return A * (glog(x * x / X) + (2.0 * b / Q) * at -
+                (b * x0 / X0) * (glog((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
+  };
+  auto pw92 = [&](double a, double a1, double b1, double b2, double b3, double b4) {
+    const T sr = gsqrt(rs);
+    const T q = 2.0 * a * (b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs);
+    return -2.0 * a * (1.0 + a1 * rs) * glog1p(1.0 / q);
+  };
+  if (!gga) {
+    const T x = gsqrt(rs);
+    const T ep = vwn(x, 0.0310907, 13.0720, 42.7198, -0.409286);
+    const T ef = vwn(x, 0.01554535, 20.1231, 101.578, -0.743294);
+    return e + n * (ep + (ef - ep) * fz(z));
+  }
+  const T ep = pw92(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const T ef = pw92(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const T mac = pw92(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const double fz20 = 1.709920934161365617563962776245;
+  const T f = fz(z), z2 = z * z, z4 = z2 * z2;
+  const T ec = ep - mac * f * (1.0 / fz20) * (1.0 - z4) + (ef - ep) * f * z4;
+  const double beta = 0.06672455060314922, gamma = 0.0310906908696549;
+  const T phi = 0.5 * (gpow(1.0 + z, 2.0 / 3.0) + gpow(1.0 - z, 2.0 / 3.0));
+  const T phi3 = phi * phi * phi;
+  const double D = 0.0634682060977037;  // pi / (16 (3 pi^2)^(1/3))
+  const T t2 = D * (saa + 2.0 * sab + sbb) / (phi * phi * n * n * gcbrt(n));
+  const T A = (beta / gamma) / (gexp((-1.0 / gamma) * ec / phi3) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * phi3 * glog1p((beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return e + n * (ec + H);
+}
+
+// Polarised first derivatives at x = (ra, rb, saa, sab, sbb) and their
+// derivatives along dx: out[0..4] = (vra, vrb, vsaa, vsab, vsbb),
+// out[5..9] the same differentiated along dx.
+__device__ __forceinline__ void dev_fxc_point_pol(int func, const double* x, const double* dx,
+                                                  double* out) {
+  using G1 = GD<double, 5>;
+  using G2 = GD<G1, 1>;
+  G2 v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    v[k].v = GCst<G1>::of(x[k]);
+    v[k].v.d[k] = 1.0;
+    v[k].d[0] = GCst<G1>::of(dx[k]);
+  }
+  const G2 e = gpol_energy(func, v[0], v[1], v[2], v[3], v[4]);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    out[k] = e.v.d[k];
+    out[5 + k] = e.d[0].d[k];
+  }
+}
+
 }  // namespace xcq

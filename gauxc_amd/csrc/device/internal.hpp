@@ -266,6 +266,11 @@ int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, c
 // FXC contraction (RKS): Z coefficients from the P and trial-density passes.
 int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
                       const double* dt, double* zc);
+// UKS FXC contraction: derivatives of the UKS Z coefficients along the trial
+// density from the four density passes.
+int launch_fxc_points_uks(const CtxView& v, const WorkRange& r, int func, int gga,
+                          const double* ds, const double* dz, const double* dts,
+                          const double* dtz, double* zs, double* zz);
 // EXC gradient (RKS): per-point coefficients, plane products out = 2 P_sub in,
 // the basis-function part and the grid-weight part (gradient.hip).
 int launch_grad_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,

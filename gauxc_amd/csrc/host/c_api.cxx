@@ -529,6 +529,21 @@ void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* st, const GauXCInteg
   });
 }
 
+void gauxc_integrator_eval_fxc_contraction_uks(
+    GauXCStatus* st, const GauXCIntegrator in, const int64_t m, const int64_t n,
+    const double* Ps, const int64_t ldps, const double* Pz, const int64_t ldpz,
+    const double* tPs, const int64_t ldtps, const double* tPz, const int64_t ldtpz,
+    double* fxc_s, const int64_t ldfxc_s, double* fxc_z, const int64_t ldfxc_z) {
+  guarded_void(st, [&] {
+    if (!Ps || !Pz || !tPs || !tPz || !fxc_s || !fxc_z)
+      throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_fxc_contraction_uks(m, n, Ps, ldps, Pz, ldpz, tPs, ldtps, tPz, ldtpz,
+                                             fxc_s, ldfxc_s, fxc_z, ldfxc_z,
+                                             IntegratorSettingsXC{});
+  });
+}
+
 void* gauxc_integrator_device_context(GauXCStatus* st, const GauXCIntegrator in) {
   return guarded(st, [&]() -> void* {
     auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");

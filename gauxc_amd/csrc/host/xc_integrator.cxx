@@ -351,6 +351,45 @@ void DeviceXCIntegratorImpl::eval_fxc_contraction(int64_t m, int64_t n, const do
   s.timer.set("XCIntegrator.DeviceToHostCopy_FXC", ms(t2, t3));
 }
 
+void DeviceXCIntegratorImpl::eval_fxc_contraction_uks(
+    int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz, int64_t ldpz,
+    const double* tPs, int64_t ldtps, const double* tPz, int64_t ldtpz, double* FXCs,
+    int64_t ldfxcs, double* FXCz, int64_t ldfxcz, const IntegratorSettingsXC&) {
+  // Argument checks of reference_replicated_xc_host_integrator_fxc_contraction.hpp:39-57.
+  auto& s = *s_;
+  const int64_t nbf = s.lb->basis().nbf();
+  if (m != n) GAUXC_GENERIC_EXCEPTION("P/FXC Must Be Square");
+  if (m != nbf) GAUXC_GENERIC_EXCEPTION("P/FXC Must Have Same Dimension as Basis");
+  if (ldps < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDPS");
+  if (ldpz < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDPZ");
+  if (ldtps < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDTPS");
+  if (ldtpz < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDTZP");
+  if (ldfxcs < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDFXCS");
+  if (ldfxcz < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDFXCZ");
+  if (!s.lb->state().modified_weights_are_stored)
+    GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
+  if (!s.func->is_polarized())
+    GAUXC_GENERIC_EXCEPTION("UKS evaluation needs a polarized functional");
+  if (s.prec != DevicePrecision::FP64)
+    GAUXC_GENERIC_EXCEPTION(
+        "The FXC contraction is provided in FP64 (local work driver Default)");
+  if (!Ps || !Pz || !tPs || !tPz || !FXCs || !FXCz)
+    GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  s.ensure_static_data();
+  const auto m32 = static_cast<int32_t>(m);
+  check(xcq_upload_density(s.ctx, m32, Ps, ldps), "xcq_upload_density");
+  check(xcq_upload_density_z(s.ctx, m32, Pz, ldpz), "xcq_upload_density_z");
+  check(xcq_upload_trial_density(s.ctx, m32, tPs, ldtps), "xcq_upload_trial_density");
+  check(xcq_upload_trial_density_z(s.ctx, m32, tPz, ldtpz), "xcq_upload_trial_density_z");
+  check(xcq_fxc_contraction_uks(s.ctx, xcq_func_id(*s.func)), "xcq_fxc_contraction_uks");
+  if (ldfxcs == ldfxcz) {
+    check(xcq_download_fxc_uks(s.ctx, &s.last_nel, FXCs, FXCz, ldfxcs), "xcq_download_fxc_uks");
+    return;
+  }
+  check(xcq_download_fxc_uks(s.ctx, &s.last_nel, FXCs, nullptr, ldfxcs), "xcq_download_fxc_uks");
+  check(xcq_download_fxc_uks(s.ctx, nullptr, nullptr, FXCz, ldfxcz), "xcq_download_fxc_uks");
+}
+
 void DeviceXCIntegratorImpl::integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp,
                                            double* N_EL) {
   auto& s = *s_;

@@ -255,6 +255,15 @@ void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* status,
                                                const double* t_density_matrix,
                                                const int64_t ldtp, double* fxc,
                                                const int64_t ldfxc);
+/* UKS FXC contraction (GauXC include/gauxc/c/xc_integrator.h:383-400): total
+ * and spin densities and trial densities in, FXC_s and FXC_z out; FP64 local
+ * work driver, polarized functional. */
+void gauxc_integrator_eval_fxc_contraction_uks(
+    GauXCStatus* status, const GauXCIntegrator integrator, const int64_t m, const int64_t n,
+    const double* density_matrix_s, const int64_t ldp_s, const double* density_matrix_z,
+    const int64_t ldp_z, const double* t_density_matrix_s, const int64_t ldtp_s,
+    const double* t_density_matrix_z, const int64_t ldtp_z, double* fxc_s,
+    const int64_t ldfxc_s, double* fxc_z, const int64_t ldfxc_z);
 /* Extension: the integrator's device context (an xcq_context of
  * include/gauxc_amd/device.h) for stage timings; NULL before first use. */
 void* gauxc_integrator_device_context(GauXCStatus* status, const GauXCIntegrator integrator);

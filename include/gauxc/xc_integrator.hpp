@@ -106,6 +106,12 @@ class DeviceXCIntegratorImpl {
   void eval_fxc_contraction(int64_t m, int64_t n, const double* P, int64_t ldp, const double* tP,
                             int64_t ldtp, double* FXC, int64_t ldfxc,
                             const IntegratorSettingsXC& s);
+  /// UKS FXC contraction (FP64; polarized functional): FXC_s, FXC_z.
+  void eval_fxc_contraction_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps,
+                                const double* Pz, int64_t ldpz, const double* tPs,
+                                int64_t ldtps, const double* tPz, int64_t ldtpz, double* FXCs,
+                                int64_t ldfxcs, double* FXCz, int64_t ldfxcz,
+                                const IntegratorSettingsXC& s);
   void integrate_den(int64_t m, int64_t n, const double* P, int64_t ldp, double* N_EL);
   const util::Timer& get_timings() const;
   const LoadBalancer& load_balancer() const;
@@ -184,6 +190,19 @@ class XCIntegrator {
     impl().eval_fxc_contraction(P.rows(), P.cols(), P.data(), P.rows(), tP.data(), tP.rows(),
                                 FXC.data(), FXC.rows(), s);
     return FXC;
+  }
+  using fxc_contraction_type_uks = std::tuple<matrix_type, matrix_type>;
+  /// GauXC XCIntegrator::eval_fxc_contraction(Ps, Pz, tPs, tPz)
+  /// (include/gauxc/xc_integrator/impl.hpp:112-117), UKS.
+  fxc_contraction_type_uks eval_fxc_contraction(const MatrixType& Ps, const MatrixType& Pz,
+                                                const MatrixType& tPs, const MatrixType& tPz,
+                                                const IntegratorSettingsXC& s =
+                                                    IntegratorSettingsXC{}) {
+    matrix_type Fs(Ps.rows(), Ps.cols()), Fz(Ps.rows(), Ps.cols());
+    impl().eval_fxc_contraction_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(),
+                                    Pz.rows(), tPs.data(), tPs.rows(), tPz.data(), tPz.rows(),
+                                    Fs.data(), Fs.rows(), Fz.data(), Fz.rows(), s);
+    return std::make_tuple(std::move(Fs), std::move(Fz));
   }
   const util::Timer& get_timings() const { return impl().get_timings(); }
   const LoadBalancer& load_balancer() const { return impl().load_balancer(); }

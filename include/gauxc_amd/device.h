@@ -237,6 +237,17 @@ int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* 
 int xcq_upload_trial_density(xcq_context ctx, int32_t nbf, const double* tP, int64_t ldp);
 int xcq_fxc_contraction(xcq_context ctx, int functional);
 int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc);
+/* UKS FXC contraction (GauXC eval_fxc_contraction(Ps, Pz, tPs, tPz), the same
+ * driver with spin_dim 2; C API gauxc_integrator_eval_fxc_contraction_uks,
+ * include/gauxc/c/xc_integrator.h:383-400): P_s / P_z through
+ * xcq_upload_density / xcq_upload_density_z, tP_s / tP_z through
+ * xcq_upload_trial_density / xcq_upload_trial_density_z.  FXC_s and FXC_z
+ * (GauXC's (FXC_a + FXC_b) / 2 and (FXC_a - FXC_b) / 2) come back with
+ * xcq_download_fxc_uks (either may be NULL). */
+int xcq_upload_trial_density_z(xcq_context ctx, int32_t nbf, const double* tPz, int64_t ldp);
+int xcq_fxc_contraction_uks(xcq_context ctx, int functional);
+int xcq_download_fxc_uks(xcq_context ctx, double* nel, double* fxc_s, double* fxc_z,
+                         int64_t ldfxc);
 /* ---- RKS EXC nuclear gradient with the SSF grid-weight derivatives (GauXC
  *      eval_exc_grad(P) with include_weight_derivatives:
  *      reference_replicated_xc_host_integrator_exc_grad.hpp:106-600,

@@ -1,12 +1,13 @@
-"""RKS FXC contraction on the device (GauXC eval_fxc_contraction(P, tP)):
+"""FXC contraction on the device (GauXC eval_fxc_contraction): RKS against
 GauXC's benzene SVWN5 / PBE0 FXC fixtures through the GauXC API at GauXC's
 bar (tests/2nd_derivative_test.cxx:121-126) and tighter, a water cluster
 against the oracle at the FP64 VXC bar, linearity in the trial density, and
-the boundary's argument and state checks."""
+the boundary's argument and state checks; UKS against GauXC's Li fixture
+(:128-134) and a spin-polarised water dimer against the oracle."""
 import numpy as np
 import pytest
 
-from conftest import benzene_setup
+from conftest import EPS, benzene_setup, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +101,66 @@ def test_fxc_boundary_errors(gx, gpu):
     integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional("SVWN5"), lb)
     with pytest.raises(gx.GauXCError, match="Same Dimension"):
         integ.eval_fxc_contraction(P[:-1, :-1], tP[:-1, :-1])
+
+
+def _li(gx):
+    d = load_golden("li_svwn5_uks.npz")
+    mol = gx.Molecule(d["Z"], d["xyz"])
+    bs = gx.BasisSet.from_arrays(d["sh_nprim"], d["sh_l"], d["sh_pure"], d["sh_alpha"],
+                                 d["sh_coeff"], d["sh_origin"], normalize=False,
+                                 shell_tolerance=EPS)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles",
+                                                  "UltraFineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    return d, bs, lb
+
+
+def test_fxc_uks_li_matches_gauxc_fixture(gx, gpu):
+    """GauXC's UKS FXC check (tests/2nd_derivative_test.cxx:128-134): Li /
+    SVWN5 / STO-3G, ||FXC_s - ref|| / nbf and ||FXC_z - ref|| / nbf < 1e-10."""
+    d, bs, lb = _li(gx)
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional("SVWN5", polarized=True), lb)
+    _flush("UKS FXC Li SVWN5 through the GauXC API")
+    fs, fz = integ.eval_fxc_contraction(d["density_s"], d["trial_density_s"], d["density_z"],
+                                        d["trial_density_z"])
+    es = np.linalg.norm(fs - d["fxc_s"]) / bs.nbf()
+    ez = np.linalg.norm(fz - d["fxc_z"]) / bs.nbf()
+    _flush(f"|dFXC_s|/nbf={es:.3e} |dFXC_z|/nbf={ez:.3e}")
+    assert es < 1e-12 and ez < 1e-12  # GauXC's bar is 1e-10
+
+
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
+def test_fxc_uks_dimer_matches_oracle(gx, orc, gpu, func):
+    Z, xyz = gx.workloads.water_cluster(2, 5.0, 4)
+    l, pure, alpha, coeff, origin = gx.workloads.even_tempered_basis(Z, xyz,
+                                                                     gx.workloads.BASIS_C0)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=1e-10)
+    mol = gx.Molecule(Z, xyz)
+    lb = gx.workloads.build_tasks(mol, bs, grid="GM3")
+    nbf = bs.nbf()
+    pa = gx.workloads.density(nbf, 11, 21) / 2
+    pb = gx.workloads.density(nbf, 9, 22) / 2
+    rng = np.random.default_rng(6)
+    ta, tb = rng.standard_normal((2, nbf, nbf)) * 0.01
+    ta, tb = ta + ta.T, tb + tb.T
+    Ps, Pz, tPs, tPz = pa + pb, pa - pb, ta + tb, ta - tb
+    t = lb.get_tasks()
+    w = orc.ssf_weights(xyz, t)
+    fs_o, fz_o = orc.fxc_contraction_uks(func, nbf, orc.shells_from(bs.xcq_shells()), Ps, Pz,
+                                         tPs, tPz, t, weights=w)
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional(func, polarized=True), lb)
+    _flush(f"UKS FXC water dimer {func}")
+    fs, fz = integ.eval_fxc_contraction(Ps, tPs, Pz, tPz)
+    _flush(f"max|dFXC_s|={np.abs(fs - fs_o).max():.3e} max|dFXC_z|={np.abs(fz - fz_o).max():.3e}")
+    assert np.abs(fs - fs_o).max() <= FP64_VXC
+    assert np.abs(fz - fz_o).max() <= FP64_VXC
+    assert np.array_equal(fs, fs.T) and np.array_equal(fz, fz.T)
+    # An unpolarized functional is refused for the UKS form.
+    rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional(func), lb)
+    with pytest.raises(gx.GauXCError, match="polarized"):
+        rks.eval_fxc_contraction(Ps, tPs, Pz, tPz)
