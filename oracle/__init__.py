@@ -71,6 +71,10 @@ def lib():
                                                  vp, ctypes.c_int64, ctypes.c_int32, vp, vp,
                                                  vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
         L.oracle_fxc_contraction_uks.restype = ctypes.c_int
+        L.oracle_exc_grad_uks.argtypes = [ctypes.c_int, ctypes.c_int32, vp, ctypes.c_int32, vp,
+                                          vp, vp, ctypes.c_int64, ctypes.c_int32, vp, vp, vp,
+                                          vp, ctypes.c_int, vp, ctypes.c_int]
+        L.oracle_exc_grad_uks.restype = ctypes.c_int
         L.oracle_collocation_hessian.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp]
         L.oracle_collocation_hessian.restype = None
         L.oracle_exc_grad.argtypes = [ctypes.c_int, ctypes.c_int32, vp, ctypes.c_int32, vp, vp,
@@ -237,6 +241,26 @@ def fxc_contraction(func, nbf, shells, P, tP, t, weights=None, nthreads=0):
                                       nbf, nthreads)
     assert rc == 0
     return nel.value, fxc
+
+
+def exc_grad_uks(func, nbf, shells, xyz, Ps, Pz, t, weights=None, nthreads=0,
+                 weight_derivs=True):
+    """UKS EXC gradient (natoms, 3) of the restated host integrator."""
+    Ps = np.asfortranarray(Ps, dtype=np.float64)
+    Pz = np.asfortranarray(Pz, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    pts = np.ascontiguousarray(t["points"], dtype=np.float64)
+    w = np.ascontiguousarray(t["weights"] if weights is None else weights, dtype=np.float64)
+    sl = np.ascontiguousarray(t["shell_list"], dtype=np.int32)
+    g = np.zeros((len(xyz), 3))
+    tasks = _task_array(t)
+    rc = lib().oracle_exc_grad_uks(FUNC[func], nbf, ctypes.addressof(shells), len(xyz),
+                                   xyz.ctypes.data, Ps.ctypes.data, Pz.ctypes.data, nbf,
+                                   len(t["npts"]), tasks.ctypes.data, pts.ctypes.data,
+                                   w.ctypes.data, sl.ctypes.data, int(bool(weight_derivs)),
+                                   g.ctypes.data, nthreads)
+    assert rc == 0
+    return g
 
 
 def fxc_contraction_uks(func, nbf, shells, Ps, Pz, tPs, tPz, t, weights=None, nthreads=0):

@@ -1667,3 +1667,171 @@ int oracle_fxc_contraction_uks(int func, int32_t nbf, const oracle_shell* shells
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// UKS EXC nuclear gradient: the same driver with spin_dim 2
+// (reference_replicated_xc_host_integrator_exc_grad.hpp:106-600, is_uks
+// branches): X_N = B P_s and X_Z = B P_z (factor 1) with their gradient
+// products for GGAs, the polarised functional, the grid-weight part with
+// w eps (rho_a + rho_b), and per shell
+//   0.5 (vrho_n X_N + vrho_z X_Z) grad B
+//   + GGA: 0.5 (v_pp + v_pm + v_mm)(H grad n X_N + grad B (grad n . Xg_N))
+//        + 0.5 (v_pp - v_mm)(H grad m X_N + grad B (grad m . Xg_N))
+//        + 0.5 (v_pp - v_mm)(H grad n X_Z + grad B (grad n . Xg_Z))
+//        + 0.5 (v_pp - v_pm + v_mm)(H grad m X_Z + grad B (grad m . Xg_Z))
+// (vrho_n,z = w (vrho_a +- vrho_b), v_pp/pm/mm = w vsigma_aa/ab/bb).
+int oracle_exc_grad_uks(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
+                        const double* xyz, const double* Ps, const double* Pz, int64_t ldp,
+                        int32_t ntasks, const oracle_task* tasks, const double* points,
+                        const double* weights, const int32_t* shell_lists, int weight_derivs,
+                        double* grad, int nthreads) {
+  if (func != ORACLE_SVWN5 && func != ORACLE_PBE && func != ORACLE_PBE0) return 1;
+  const bool gga = func != ORACLE_SVWN5;
+  for (int k = 0; k < 3 * natoms; ++k) grad[k] = 0.;
+  std::vector<double> rab((size_t)natoms * natoms);
+  for (int a = 0; a < natoms; ++a)
+    for (int b = 0; b < natoms; ++b) {
+      const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                   dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+      rab[(size_t)a + (size_t)b * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  int32_t nsh_total = 0;
+  for (int32_t it = 0; it < ntasks; ++it)
+    for (int32_t k = 0; k < tasks[it].nshells; ++k)
+      nsh_total = std::max(nsh_total, shell_lists[tasks[it].shell_offset + k] + 1);
+  std::vector<int32_t> first_ao(nsh_total + 1, 0), centre(nsh_total, -1);
+  for (int32_t s = 0; s < nsh_total; ++s) {
+    const auto& sh = shells[s];
+    first_ao[s + 1] = first_ao[s] + (sh.pure ? 2 * sh.l + 1 : (sh.l + 1) * (sh.l + 2) / 2);
+    for (int a = 0; a < natoms && centre[s] < 0; ++a)
+      if (xyz[3 * a] == sh.origin[0] && xyz[3 * a + 1] == sh.origin[1] &&
+          xyz[3 * a + 2] == sh.origin[2])
+        centre[s] = a;
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> pl[10], XN[4], XZ[4], sub, wf, rho, sig, eps, vr, vs, gn[3], gm[3];
+    std::vector<double> gloc(3 * (size_t)natoms, 0.);
+    std::vector<int32_t> bf;
+#pragma omp for schedule(dynamic)
+    for (int32_t it = 0; it < ntasks; ++it) {
+      const oracle_task& t = tasks[it];
+      const int32_t npts = t.npts;
+      const int32_t* sl = shell_lists + t.shell_offset;
+      bf.clear();
+      for (int32_t k = 0; k < t.nshells; ++k)
+        for (int32_t f = first_ao[sl[k]]; f < first_ao[sl[k] + 1]; ++f) bf.push_back(f);
+      const int32_t nbe = (int32_t)bf.size();
+      const double* pts = points + 3 * t.point_offset;
+      const double* w = weights + t.point_offset;
+      double* o[10];
+      for (int q = 0; q < 10; ++q) {
+        pl[q].assign((size_t)npts * nbe, 0.);
+        o[q] = pl[q].data();
+      }
+      oracle_collocation_hessian(shells, t.nshells, sl, npts, pts, o);
+      const int nx = gga ? 4 : 1;
+      for (int q = 0; q < nx; ++q) {
+        xmat(pl[q], Ps, ldp, bf, npts, 1.0, sub, XN[q]);
+        xmat(pl[q], Pz, ldp, bf, npts, 1.0, sub, XZ[q]);
+      }
+      rho.assign(2 * (size_t)npts, 0.);
+      sig.assign(3 * (size_t)npts, 0.);
+      for (int d = 0; d < 3; ++d) {
+        gn[d].assign(npts, 0.);
+        gm[d].assign(npts, 0.);
+      }
+      for (int32_t p = 0; p < npts; ++p) {
+        const size_t off = (size_t)p * nbe;
+        double rs = 0., rz = 0., an[3] = {0., 0., 0.}, am[3] = {0., 0., 0.};
+        for (int32_t k = 0; k < nbe; ++k) {
+          rs += pl[0][off + k] * XN[0][off + k];
+          rz += pl[0][off + k] * XZ[0][off + k];
+          for (int d = 0; d < 3; ++d) {
+            an[d] += pl[1 + d][off + k] * XN[0][off + k];
+            am[d] += pl[1 + d][off + k] * XZ[0][off + k];
+          }
+        }
+        rho[2 * p] = 0.5 * (rs + rz);
+        rho[2 * p + 1] = 0.5 * (rs - rz);
+        double nn = 0., mm = 0., nm = 0.;
+        for (int d = 0; d < 3; ++d) {
+          gn[d][p] = 2. * an[d];
+          gm[d][p] = 2. * am[d];
+          nn += gn[d][p] * gn[d][p];
+          mm += gm[d][p] * gm[d][p];
+          nm += gn[d][p] * gm[d][p];
+        }
+        sig[3 * p] = 0.25 * (nn + mm) + 0.5 * nm;
+        sig[3 * p + 1] = 0.25 * (nn - mm);
+        sig[3 * p + 2] = 0.25 * (nn + mm) - 0.5 * nm;
+      }
+      eps.resize(npts);
+      vr.resize(2 * (size_t)npts);
+      vs.assign(3 * (size_t)npts, 0.);
+      oracle_functional_pol(func, npts, rho.data(), gga ? sig.data() : nullptr, eps.data(),
+                            vr.data(), gga ? vs.data() : nullptr);
+      if (weight_derivs) {
+        wf.resize(npts);
+        for (int32_t p = 0; p < npts; ++p)
+          wf[p] = eps[p] * (rho[2 * p] + rho[2 * p + 1]) * w[p];
+        ssf_grad_contract(natoms, xyz, rab.data(), t, pts, wf.data(), gloc.data());
+      }
+      int32_t boff = 0;
+      for (int32_t k = 0; k < t.nshells; ++k) {
+        const int32_t s = sl[k];
+        const int32_t n = first_ao[s + 1] - first_ao[s];
+        const int32_t at = centre[s];
+        if (at < 0 || (weight_derivs && at == t.iparent)) {
+          boff += n;
+          continue;
+        }
+        double g[3] = {0., 0., 0.};
+        for (int32_t mu = boff; mu < boff + n; ++mu)
+          for (int32_t p = 0; p < npts; ++p) {
+            const size_t i = (size_t)p * nbe + mu;
+            const double vn = w[p] * (vr[2 * p] + vr[2 * p + 1]),
+                         vz = w[p] * (vr[2 * p] - vr[2 * p + 1]);
+            const double xn = XN[0][i], xz = XZ[0][i];
+            for (int d = 0; d < 3; ++d) g[d] += 0.5 * (vn * xn + vz * xz) * pl[1 + d][i];
+            if (gga) {
+              const double vpp = w[p] * vs[3 * p], vpm = w[p] * vs[3 * p + 1],
+                           vmm = w[p] * vs[3 * p + 2];
+              const double dn[3] = {gn[0][p], gn[1][p], gn[2][p]};
+              const double dm[3] = {gm[0][p], gm[1][p], gm[2][p]};
+              const double h[3][3] = {{pl[4][i], pl[5][i], pl[6][i]},
+                                      {pl[5][i], pl[7][i], pl[8][i]},
+                                      {pl[6][i], pl[8][i], pl[9][i]}};
+              const double nN = dn[0] * XN[1][i] + dn[1] * XN[2][i] + dn[2] * XN[3][i];
+              const double mN = dm[0] * XN[1][i] + dm[1] * XN[2][i] + dm[2] * XN[3][i];
+              const double nZ = dn[0] * XZ[1][i] + dn[1] * XZ[2][i] + dn[2] * XZ[3][i];
+              const double mZ = dm[0] * XZ[1][i] + dm[1] * XZ[2][i] + dm[2] * XZ[3][i];
+              const double c1 = 0.5 * (vpp + vpm + vmm), c2 = 0.5 * (vpp - vmm),
+                           c3 = 0.5 * (vpp - vpm + vmm);
+              for (int d = 0; d < 3; ++d) {
+                const double hn = h[d][0] * dn[0] + h[d][1] * dn[1] + h[d][2] * dn[2];
+                const double hm = h[d][0] * dm[0] + h[d][1] * dm[1] + h[d][2] * dm[2];
+                const double db = pl[1 + d][i];
+                g[d] += c1 * (hn * xn + nN * db) + c2 * (hm * xn + mN * db) +
+                        c2 * (hn * xz + nZ * db) + c3 * (hm * xz + mZ * db);
+              }
+            }
+          }
+        for (int d = 0; d < 3; ++d) {
+          gloc[3 * at + d] += -2. * g[d];
+          if (weight_derivs) gloc[3 * t.iparent + d] -= -2. * g[d];
+        }
+        boff += n;
+      }
+    }
+#pragma omp critical
+    for (int k = 0; k < 3 * natoms; ++k) grad[k] += gloc[k];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -95,6 +95,14 @@ int oracle_fxc_contraction(int func, int32_t nbf, const oracle_shell* shells, co
                            const double* weights, const int32_t* shell_lists, double* nel,
                            double* fxc, int64_t ldfxc, int nthreads);
 
+/* UKS EXC nuclear gradient (P_s, P_z; polarised functional), forms and
+ * layout as oracle_exc_grad. */
+int oracle_exc_grad_uks(int func, int32_t nbf, const oracle_shell* shells, int32_t natoms,
+                        const double* xyz, const double* Ps, const double* Pz, int64_t ldp,
+                        int32_t ntasks, const oracle_task* tasks, const double* points,
+                        const double* weights, const int32_t* shell_lists, int weight_derivs,
+                        double* grad, int nthreads);
+
 /* UKS FXC contraction: FXC_s, FXC_z = d (VXC_s, VXC_z)[P + e tP] / de for
  * P = (P_s, P_z), tP = (tP_s, tP_z) (all with leading dimension ldp). */
 int oracle_fxc_contraction_uks(int func, int32_t nbf, const oracle_shell* shells,

@@ -18,7 +18,8 @@ Each check is chosen so that a plausible mistake fails it:
     derivative of VXC along tP (RKS and UKS); UKS with P_z = 0 equals RKS;
   * EXC gradient: Hessian collocation against gau2grid's deriv2 and central
     differences; GauXC's benzene EXC_GRAD fixtures (with grid-weight
-    derivatives); translational invariance; dEXC/dR by central differences.
+    derivatives); translational invariance; dEXC/dR by central differences
+    (RKS and UKS); UKS with P_z = 0 equals RKS.
 """
 import ctypes
 
@@ -518,6 +519,41 @@ def test_oracle_exc_grad_is_the_derivative_of_exc(gx, orc, func):
                                  weights=w2)[0])
         fd = (e[0] - e[1]) / (2 * h)
         assert abs(g[atom, comp] - fd) <= 1e-6 * max(abs(fd), 1e-3), (atom, comp, g[atom, comp], fd)
+
+
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
+def test_oracle_uks_exc_grad(gx, orc, func):
+    """UKS gradient: dEXC/dR by central differences of the whole UKS
+    evaluation on a spin-polarised water dimer, translational invariance,
+    and the RKS limit (P_s = 2P, P_z = 0 gives the RKS gradient of P)."""
+    Z, xyz = gx.workloads.water_cluster(2, 5.0, 4)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    bs, t, w = _dimer_system(gx, orc, xyz, Z)
+    nbf = bs.nbf()
+    pa = gx.workloads.density(nbf, 11, 21) / 2
+    pb = gx.workloads.density(nbf, 9, 22) / 2
+    Ps, Pz = pa + pb, pa - pb
+    sh = orc.shells_from(bs.xcq_shells())
+    g = orc.exc_grad_uks(func, nbf, sh, xyz, Ps, Pz, t, weights=w)
+    assert np.abs(g.sum(axis=0)).max() < 1e-10
+    h = 1e-4
+    for atom, comp in ((1, 1), (3, 0)):
+        e = []
+        for sgn in (1, -1):
+            x2 = xyz.copy()
+            x2[atom, comp] += sgn * h
+            b2, t2, w2 = _dimer_system(gx, orc, x2, Z)
+            e.append(orc.exc_vxc_uks(func, b2.nbf(), orc.shells_from(b2.xcq_shells()), Ps, Pz,
+                                     t2, weights=w2)[0])
+        fd = (e[0] - e[1]) / (2 * h)
+        assert abs(g[atom, comp] - fd) <= 1e-6 * max(abs(fd), 1e-3), (atom, comp, g[atom, comp], fd)
+    gr = orc.exc_grad(func, nbf, sh, xyz, pa, t, weights=w)
+    g0 = orc.exc_grad_uks(func, nbf, sh, xyz, 2 * pa, 0 * pa, t, weights=w)
+    assert np.abs(g0 - gr).max() <= 1e-11 * np.abs(gr).max()
+    for wd in (False,):
+        gh = orc.exc_grad_uks(func, nbf, sh, xyz, 2 * pa, 0 * pa, t, weights=w, weight_derivs=wd)
+        ghr = orc.exc_grad(func, nbf, sh, xyz, pa, t, weights=w, weight_derivs=wd)
+        assert np.abs(gh - ghr).max() <= 1e-11 * np.abs(ghr).max()
 
 
 def test_oracle_mlp_gradient_matches_finite_differences():
