@@ -84,6 +84,7 @@ def pack_tasks(t, point_base=0, shell_base=0):
 RADIAL_MURA_KNOWLES = 1
 GRID = {"FineGrid": 0, "UltraFineGrid": 1, "SuperFineGrid": 2, "GM3": 3, "GM5": 4}
 PRUNING_UNPRUNED = 0
+PRUNING = {"Unpruned": 0, "Robust": 1, "Treutler": 2}
 WEIGHT_SSF = 2
 EXEC_HOST, EXEC_DEVICE = 0, 1
 XCQ_FUNC = {"SVWN5": 1, "PBE": 2, "PBE0": 3}
@@ -110,6 +111,7 @@ HOST_SYMBOLS = [
     "gauxc_integrator_eval_exc_uks", "gauxc_integrator_eval_exc_vxc_uks",
     "gauxc_integrator_eval_fxc_contraction_rks", "gauxc_integrator_eval_exc_grad_rks",
     "gauxc_integrator_eval_exc_grad_hf_rks", "gauxc_integrator_eval_fxc_contraction_uks",
+    "gauxc_integrator_eval_exc_grad_uks", "gauxc_integrator_eval_exc_grad_hf_uks",
     "gauxc_integrator_device_context",
 ]
 DEVICE_SYMBOLS = [
@@ -119,7 +121,7 @@ DEVICE_SYMBOLS = [
     "xcq_ssf_weights", "xcq_ssf_weights_tasks", "xcq_ssf_weights_noalloc",
     "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_upload_trial_density", "xcq_fxc_contraction", "xcq_download_fxc",
-    "xcq_exc_grad", "xcq_download_exc_grad", "xcq_collision_detection",
+    "xcq_exc_grad", "xcq_exc_grad_uks", "xcq_download_exc_grad", "xcq_collision_detection",
     "xcq_upload_trial_density_z", "xcq_fxc_contraction_uks", "xcq_download_fxc_uks",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
@@ -174,6 +176,7 @@ def device_lib():
             "xcq_fxc_contraction": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_download_fxc": (ctypes.c_int, [H, c_vp, c_vp, c_i64]),
             "xcq_exc_grad": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
+            "xcq_exc_grad_uks": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
             "xcq_upload_trial_density_z": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
             "xcq_fxc_contraction_uks": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_download_fxc_uks": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_i64]),
@@ -291,6 +294,10 @@ def host_lib():
                                                           c_vp]),
             "gauxc_integrator_eval_exc_grad_hf_rks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64,
                                                              c_vp]),
+            "gauxc_integrator_eval_exc_grad_uks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64,
+                                                          c_vp, c_i64, c_vp]),
+            "gauxc_integrator_eval_exc_grad_hf_uks": (None, [S, Hd, c_i64, c_i64, c_vp, c_i64,
+                                                             c_vp, c_i64, c_vp]),
             "gauxc_integrator_eval_fxc_contraction_uks": (None, [S, Hd, c_i64, c_i64] +
                                                           [c_vp, c_i64] * 6),
             "gauxc_integrator_device_context": (c_vp, [S, Hd]),

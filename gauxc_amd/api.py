@@ -132,11 +132,11 @@ class MolGridFactory:
     @staticmethod
     def create_default_molgrid(mol, pruning="Unpruned", batch_size=512, radial="MuraKnowles",
                                grid="UltraFineGrid"):
-        if pruning != "Unpruned":
-            raise N.GauXCError("Only unpruned atomic grids are provided")
+        if pruning not in N.PRUNING:
+            raise N.GauXCError(f"Unknown pruning scheme {pruning!r}")
         rq = {"Becke": 0, "MuraKnowles": 1, "MurrayHandyLaming": 2, "TreutlerAhlrichs": 3}[radial]
         st = N.Status()
-        h = N.host_lib().gauxc_molgrid_new_default(st.ptr, mol.h, N.PRUNING_UNPRUNED,
+        h = N.host_lib().gauxc_molgrid_new_default(st.ptr, mol.h, N.PRUNING[pruning],
                                                    int(batch_size), rq, N.GRID[grid])
         st.raise_if()
         return MolGrid(h)
@@ -322,17 +322,26 @@ class XCIntegrator(_Handle):
         st.raise_if()
         return exc.value
 
-    def eval_exc_grad(self, P, include_weight_derivatives=True):
-        """RKS EXC nuclear gradient, (natoms, 3), as GauXC's eval_exc_grad(P)
-        (with the grid-weight derivatives unless include_weight_derivatives
-        is False)."""
+    def eval_exc_grad(self, P, Pz=None, include_weight_derivatives=True):
+        """EXC nuclear gradient, (natoms, 3), as GauXC's eval_exc_grad(P)
+        (RKS) or eval_exc_grad(Ps, Pz) (UKS, when Pz is given), with the
+        grid-weight derivatives unless include_weight_derivatives is False."""
         P = self._mat(P)
         m, n = P.shape
         g = np.zeros(3 * self.natoms)
         st = N.Status()
-        fn = (N.host_lib().gauxc_integrator_eval_exc_grad_rks if include_weight_derivatives
-              else N.host_lib().gauxc_integrator_eval_exc_grad_hf_rks)
-        fn(st.ptr, self.h, m, n, P.ctypes.data, m, g.ctypes.data)
+        L = N.host_lib()
+        if Pz is None:
+            fn = (L.gauxc_integrator_eval_exc_grad_rks if include_weight_derivatives
+                  else L.gauxc_integrator_eval_exc_grad_hf_rks)
+            fn(st.ptr, self.h, m, n, P.ctypes.data, m, g.ctypes.data)
+        else:
+            Pz = self._mat(Pz)
+            if Pz.shape != P.shape:
+                raise N.GauXCError("P_s and P_z differ in shape")
+            fn = (L.gauxc_integrator_eval_exc_grad_uks if include_weight_derivatives
+                  else L.gauxc_integrator_eval_exc_grad_hf_uks)
+            fn(st.ptr, self.h, m, n, P.ctypes.data, m, Pz.ctypes.data, m, g.ctypes.data)
         st.raise_if()
         return g.reshape(-1, 3)
 
@@ -489,13 +498,15 @@ class DeviceContext:
             self.nbf), "xcq_download_results_uks")
         return exc.value, nel.value, vs, vz
 
-    def exc_grad(self, func, xyz, weight_derivs=True):
-        """RKS EXC gradient of the uploaded density; xyz (natoms, 3) in the
-        order of the tasks' parent indices."""
+    def exc_grad(self, func, xyz, weight_derivs=True, uks=False):
+        """EXC gradient of the uploaded density (RKS; UKS with P_z uploaded
+        and uks=True); xyz (natoms, 3) in the order of the tasks' parent
+        indices."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         self._grad_natoms = len(xyz)
-        N.xcq_check(self.lib.xcq_exc_grad(self.h, N.XCQ_FUNC[func], len(xyz), xyz.ctypes.data,
-                                          int(bool(weight_derivs))), "xcq_exc_grad")
+        fn = self.lib.xcq_exc_grad_uks if uks else self.lib.xcq_exc_grad
+        N.xcq_check(fn(self.h, N.XCQ_FUNC[func], len(xyz), xyz.ctypes.data,
+                       int(bool(weight_derivs))), "xcq_exc_grad")
 
     def results_grad(self):
         """(EXC, NEL, gradient (natoms, 3)) of the last EXC gradient."""

@@ -186,7 +186,9 @@ struct xcq_context_s {
   bool pf32_valid = false;
   double* d_vxc = nullptr;
   double* d_red = nullptr;
-  double* d_uks = nullptr;  // UKS: [4][pblocks][256] density sums (s, z), Z coefficients (s, z)
+  // UKS and gradient per-point buffers, [5][pblocks][256]: density sums
+  // (s, z), Z coefficients (s, z), w eps rho (gradient).
+  double* d_uks = nullptr;
   double* d_Pz = nullptr;   // UKS: P_z = P_a - P_b (single molecule)
   double* d_vxcz = nullptr; // UKS: VXC_z
   bool have_pz = false;
@@ -1301,6 +1303,13 @@ int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc) {
 // ---- UKS (GauXC's eval_exc_vxc(Ps, Pz)) --------------------------------------
 namespace {
 
+// The per-point buffers of the UKS, FXC and gradient paths (freed with the
+// task set, whose point-block count sizes them).
+int ensure_point_buffers(xcq_context_s* ctx) {
+  if (ctx->d_uks) return XCQ_SUCCESS;
+  return dalloc(&ctx->d_uks, (size_t)5 * 256 * std::max<int64_t>(ctx->pblocks, 1));
+}
+
 // One UKS evaluation on the context's stream (FP64, stored basis planes):
 // per chunk, collocation with gradient planes; the density stage twice (P_s,
 // then P_z; factor 1) leaving raw per-point sums; the polarised functional
@@ -1387,8 +1396,7 @@ int xcq_exc_vxc_uks(xcq_context ctx, int functional, int want_vxc) {
               "UKS needs P_s (xcq_upload_density) and P_z (xcq_upload_density_z)");
   XCQ_HIP(hipSetDevice(ctx->device));
   int rc;
-  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
-    return rc;
+  if ((rc = ensure_point_buffers(ctx))) return rc;
   if (!ctx->d_vxcz && (rc = dalloc(&ctx->d_vxcz, (size_t)ctx->p_total))) return rc;
   ctx->last_want_vxc = want_vxc ? 1 : 0;
   ctx->last_vxc_eval = want_vxc ? 1 : 0;
@@ -1495,8 +1503,7 @@ int xcq_fxc_contraction(xcq_context ctx, int functional) {
               "(xcq_upload_trial_density)");
   XCQ_HIP(hipSetDevice(ctx->device));
   int rc;
-  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
-    return rc;
+  if ((rc = ensure_point_buffers(ctx))) return rc;
   ctx->last_want_vxc = 0;
   ctx->last_vxc_eval = 0;
   ctx->last_graph = false;
@@ -1699,10 +1706,60 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
                             ctx->d_out2);
 }
 
-}  // namespace
+// One UKS EXC gradient: per chunk, collocation; the density stage twice (P_s,
+// then P_z; factor 1); the polarised functional's coefficients written over
+// the sums, w eps rho; Za_s into plane 4 and Za_z into plane 3; Y = P_s Za_s
+// + P_z Za_z into plane 1; for GGAs X_N = P_s B into plane 2 and X_Z = P_z B
+// into plane 3; the basis-function and grid-weight parts.
+int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
+  const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  CtxView v = view_of(ctx);
+  v.in_stage = 0;
+  v.tmax = nullptr;
+  v.vxc_drop = 0.0;
+  hipStream_t s = ctx->stream;
+  const size_t blk = (size_t)256 * ctx->pblocks;
+  double* ds = ctx->d_uks;
+  double* dz = ds + blk;
+  double* wf = dz + blk;
+  int rc;
+  if ((rc = launch_fill(s, ctx->d_grad, 3 * (int64_t)ctx->grad_natoms))) return rc;
+  if (ctx->ncounters &&
+      (rc = launch_fill(s, reinterpret_cast<double*>(ctx->d_counters), ctx->ncounters)))
+    return rc;
+  for (const Chunk& c : ctx->chunks) {
+    const WorkItem* cw = v.work + c.coll.off;
+    if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
+                                      ctx->max_l)))
+      return rc;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < 4; ++k)
+        if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
+                                 1.0, 0, pass == 0 ? ctx->d_P : ctx->d_Pz,
+                                 pass == 0 ? ds : dz)))
+          return rc;
+    if ((rc = launch_grad_points_uks(v, c.coll, func, gga, ds, dz, wf))) return rc;
+    if ((rc = launch_grad_zpair(v, c.coll, gga, ds, dz))) return rc;
+    if ((rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_P, 1.0, 0)) ||
+        (rc = launch_plane_product(v, c.coll, 3, 1, ctx->d_Pz, 1.0, 1)))
+      return rc;
+    if (gga && ((rc = launch_plane_product(v, c.coll, 0, 2, ctx->d_P, 1.0, 0)) ||
+                (rc = launch_plane_product(v, c.coll, 0, 3, ctx->d_Pz, 1.0, 0))))
+      return rc;
+    if ((rc = launch_grad_contract_uks(v, c.coll, gga, ctx->d_shell_atom, ds, dz, weight_derivs,
+                                       ctx->d_grad)))
+      return rc;
+    if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
+                                               ctx->d_dnear, wf, ctx->d_grad)))
+      return rc;
+  }
+  return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
+                            ctx->d_out2);
+}
 
-int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
-                 int weight_derivs) {
+// Checks and geometry upload shared by the RKS and UKS gradients.
+int grad_prepare(xcq_context_s* ctx, int functional, int32_t natoms, const double* xyz) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_REQUIRE(functional == XCQ_FUNC_SVWN5 || functional == XCQ_FUNC_PBE ||
                   functional == XCQ_FUNC_PBE0,
@@ -1753,8 +1810,7 @@ int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* 
   XCQ_HIP(hipMemcpy(ctx->d_gxyz, xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));
   XCQ_HIP(hipMemcpy(ctx->d_dnear, dn.data(), 8 * (size_t)natoms, hipMemcpyHostToDevice));
   XCQ_HIP(hipMemcpy(ctx->d_shell_atom, sa.data(), 4 * nsh, hipMemcpyHostToDevice));
-  if (!ctx->d_uks && (rc = dalloc(&ctx->d_uks, (size_t)4 * 256 * std::max<int64_t>(ctx->pblocks, 1))))
-    return rc;
+  if ((rc = ensure_point_buffers(ctx))) return rc;
   ctx->last_want_vxc = 0;
   ctx->last_vxc_eval = 0;
   ctx->last_graph = false;
@@ -1762,7 +1818,24 @@ int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* 
   ctx->last_fxc = false;
   ctx->last_fxc_uks = false;
   ctx->last_grad = true;
-  return enqueue_grad(ctx, functional, weight_derivs ? 1 : 0);
+  return XCQ_SUCCESS;
+}
+
+}  // namespace
+
+int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
+                 int weight_derivs) {
+  const int rc = grad_prepare(ctx, functional, natoms, xyz);
+  return rc ? rc : enqueue_grad(ctx, functional, weight_derivs ? 1 : 0);
+}
+
+int xcq_exc_grad_uks(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
+                     int weight_derivs) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  XCQ_REQUIRE(ctx->have_pz, XCQ_ERROR_STATE,
+              "the UKS EXC gradient needs P_z (xcq_upload_density_z)");
+  const int rc = grad_prepare(ctx, functional, natoms, xyz);
+  return rc ? rc : enqueue_grad_uks(ctx, functional, weight_derivs ? 1 : 0);
 }
 
 int xcq_download_exc_grad(xcq_context ctx, double* exc, double* nel, double* grad) {

@@ -1,4 +1,4 @@
-// EXC nuclear gradient, RKS, FP64 (GauXC eval_exc_grad with
+// EXC nuclear gradient, RKS and UKS, FP64 (GauXC eval_exc_grad with
 // include_weight_derivatives: reference_replicated_xc_host_integrator_
 // exc_grad.hpp:106-600, weights.cxx:806-990; device counterparts
 // increment_exc_grad.cu and cuda_ssf_1d.cu).
@@ -20,6 +20,19 @@
 // then X over plane 1 and Y over plane 2 once Za is formed.
 // The grid-weight part contracts the SSF weight derivatives with w eps rho
 // per point (k_ssf_grad).
+//
+// UKS (GauXC eval_exc_grad(Ps, Pz), the same driver with spin_dim 2): with
+// X_N = P_s B, X_Z = P_z B (factor 1), rho_+- = (n +- m)/2 and the polarised
+// functional,
+//   g_mu = grad B_mu . Y_mu + X_N,mu (H_mu c_s) + X_Z,mu (H_mu c_z),
+//   Y = P_s Za_s + P_z Za_z,
+//   Za_s = w (vrho_a + vrho_b)/2 B + c_s . grad B,
+//   Za_z = w (vrho_a - vrho_b)/2 B + c_z . grad B,
+//   c_s = g1 grad n + g2 grad m,  c_z = g2 grad n + g3 grad m,
+// g1 = (v_aa + v_ab + v_bb)/2, g2 = (v_aa - v_bb)/2, g3 = (v_aa - v_ab + v_bb)/2
+// (w vsigma folded in), and f = w eps (rho_a + rho_b) for the weight part.
+// Planes: Za_s -> 4 and Za_z -> 3 (k_grad_zpair), Y -> 1, then X_N -> 2 and
+// X_Z -> 3.
 #include "internal.hpp"
 #include "functional.hpp"
 #include "shell_eval.hpp"
@@ -79,8 +92,117 @@ __global__ __launch_bounds__(64) void k_grad_points(const WorkItem* __restrict__
   }
 }
 
+// UKS: from the raw sums of the P_s and P_z passes (factor 1) at block b,
+// coef_s[b] = (w (vrho_a + vrho_b)/2, c_s), coef_z[b] = (w (vrho_a -
+// vrho_b)/2, c_z) and wf[b] = w eps (rho_a + rho_b).  coef_s and coef_z may
+// be the sum buffers themselves: each thread reads its own entries first.
+__global__ __launch_bounds__(64) void k_grad_points_uks(const WorkItem* __restrict__ items,
+                                                        const DevTask* __restrict__ tasks,
+                                                        const double* __restrict__ weights,
+                                                        int func, int gga, double* ds,
+                                                        double* dz, double* __restrict__ red,
+                                                        double* __restrict__ wf) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x;
+  const int64_t b = t.red_off + wi.a;
+  const int p = wi.a * kPtBlock + tid;
+  double sv[4], zv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sv[c] = ds[b * 256 + c * 64 + tid];
+    zv[c] = dz[b * 256 + c * 64 + tid];
+  }
+  const double ra = 0.5 * (sv[0] + zv[0]), rb = 0.5 * (sv[0] - zv[0]);
+  double gn[3], gm[3], nn = 0., mm = 0., nm = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    gn[d] = 2.0 * sv[1 + d];
+    gm[d] = 2.0 * zv[1 + d];
+    nn += gn[d] * gn[d];
+    mm += gm[d] * gm[d];
+    nm += gn[d] * gm[d];
+  }
+  double e = 0., n = 0., as = 0., az = 0., cs[3] = {0., 0., 0.}, cz[3] = {0., 0., 0.};
+  if (p < t.npts) {
+    const double w = weights[t.pt_off + p];
+    const double saa = gga ? 0.25 * (nn + mm) + 0.5 * nm : 0.;
+    const double sab = gga ? 0.25 * (nn - mm) : 0.;
+    const double sbb = gga ? 0.25 * (nn + mm) - 0.5 * nm : 0.;
+    const XCOutPol o = dev_eval_functional_pol(func, ra, rb, saa, sab, sbb);
+    e = w * o.e;
+    n = w * (ra + rb);
+    as = 0.5 * w * (o.vra + o.vrb);
+    az = 0.5 * w * (o.vra - o.vrb);
+    if (gga) {
+      const double vpp = w * o.vsaa, vpm = w * o.vsab, vmm = w * o.vsbb;
+      const double g1 = 0.5 * (vpp + vpm + vmm), g2 = 0.5 * (vpp - vmm),
+                   g3 = 0.5 * (vpp - vpm + vmm);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cs[d] = g1 * gn[d] + g2 * gm[d];
+        cz[d] = g2 * gn[d] + g3 * gm[d];
+      }
+    }
+  }
+  ds[b * 256 + tid] = as;
+  dz[b * 256 + tid] = az;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ds[b * 256 + (1 + d) * 64 + tid] = cs[d];
+    dz[b * 256 + (1 + d) * 64 + tid] = cz[d];
+  }
+  wf[b * 64 + tid] = e;
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    n += __shfl_xor(n, off);
+  }
+  if (tid == 0) {
+    red[2 * b] = e;
+    red[2 * b + 1] = n;
+  }
+}
+
+// Za_s = a_s B + c_s . grad B into plane 4 and Za_z = a_z B + c_z . grad B
+// into plane 3 (over grad_z B, which each thread reads before it writes) for
+// one (task, 64-point block): lane = point, the four waves take rows w,
+// w + 4, ...
+template <bool GGA>
+__global__ __launch_bounds__(256) void k_grad_zpair(const WorkItem* __restrict__ items,
+                                                    const DevTask* __restrict__ tasks,
+                                                    double* __restrict__ scratch,
+                                                    const double* __restrict__ coef_s,
+                                                    const double* __restrict__ coef_z) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t b = t.red_off + wi.a;
+  double cs[4], cz[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    cs[c] = coef_s[b * 256 + c * 64 + lane];
+    cz[c] = coef_z[b * 256 + c * 64 + lane];
+  }
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  double* B = scratch + t.b_off + (int64_t)wi.a * kPtBlock + lane;
+  for (int j = wave; j < t.nbe; j += 4) {
+    const int64_t o = (int64_t)j * ld;
+    const double bv = B[o];
+    double zs = cs[0] * bv, zz = cz[0] * bv;
+    if constexpr (GGA) {
+      const double x = B[plane + o], y = B[2 * plane + o], z = B[3 * plane + o];
+      zs += cs[1] * x + cs[2] * y + cs[3] * z;
+      zz += cz[1] * x + cz[2] * y + cz[3] * z;
+    }
+    B[4 * plane + o] = zs;
+    B[3 * plane + o] = zz;
+  }
+}
+
 // ---- plane products ----------------------------------------------------------
-// out[f][p] = 2 sum_g P_sub[f][g] in[g][p] for one (task, 64-point block),
+// out[f][p] = fac sum_g P_sub[f][g] in[g][p] (+ out[f][p] with accumulate)
+// for one (task, 64-point block),
 // every 64-function tile of the task: 256 threads, each a 4 x 4 patch
 // (functions 4 ty.., points 4 tx..), K steps of 16 staged in LDS.  P_sub is
 // gathered through the basis map (symmetric, so the storage order of P does
@@ -90,7 +212,7 @@ __global__ __launch_bounds__(256) void k_plane_product(const WorkItem* __restric
                                                        const int32_t* __restrict__ bfmap,
                                                        const double* __restrict__ P,
                                                        double* __restrict__ scratch, int in_pl,
-                                                       int out_pl) {
+                                                       int out_pl, double fac, int accumulate) {
   __shared__ double Ps[16][65];
   __shared__ double Is[16][64];
   const WorkItem wi = items[blockIdx.x];
@@ -137,7 +259,10 @@ __global__ __launch_bounds__(256) void k_plane_product(const WorkItem* __restric
       const int f = f0 + 4 * ty + i;
       if (f < t.nbe) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) out[(int64_t)f * ld + 4 * tx + j] = 2.0 * acc[i][j];
+        for (int j = 0; j < 4; ++j) {
+          double* o = out + (int64_t)f * ld + 4 * tx + j;
+          *o = accumulate ? fma(fac, acc[i][j], *o) : fac * acc[i][j];
+        }
       }
     }
   }
@@ -216,6 +341,79 @@ __device__ __forceinline__ void shell_grad_hc(double x, double y, double z, doub
         double o[6];
         term(a, b, o);
         sink(ic, o[0], o[1], o[2], o[3], o[4], o[5]);
+        ++ic;
+      }
+    }
+  }
+}
+
+// Gradient and full Hessian of a shell's functions:
+//   d_i d_j B = R d_i d_j m + R1 (d_i m x_j + d_j m x_i) + m (delta_ij R1 + x_i x_j R2).
+// Calls sink(f, gx, gy, gz, h) with h = (xx, xy, xz, yy, yz, zz).
+template <int L, typename Sink>
+__device__ __forceinline__ void shell_grad_hess(double x, double y, double z, double R,
+                                                double R1, double R2, bool pure, Sink&& sink) {
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.;
+#pragma unroll
+  for (int k = 1; k <= L; ++k) {
+    xp[k] = xp[k - 1] * x;
+    yp[k] = yp[k - 1] * y;
+    zp[k] = zp[k - 1] * z;
+  }
+  auto pw = [](const double* t, int n) { return n < 0 ? 0. : t[n < 0 ? 0 : n]; };
+  auto term = [&](int a, int b, double* o) {
+    const int c = L - a - b;
+    const double m = xp[a] * yp[b] * zp[c];
+    const double mx = a * pw(xp, a - 1) * yp[b] * zp[c];
+    const double my = b * xp[a] * pw(yp, b - 1) * zp[c];
+    const double mz = c * xp[a] * yp[b] * pw(zp, c - 1);
+    const double mxx = a * (a - 1) * pw(xp, a - 2) * yp[b] * zp[c];
+    const double myy = b * (b - 1) * xp[a] * pw(yp, b - 2) * zp[c];
+    const double mzz = c * (c - 1) * xp[a] * yp[b] * pw(zp, c - 2);
+    const double mxy = a * b * pw(xp, a - 1) * pw(yp, b - 1) * zp[c];
+    const double mxz = a * c * pw(xp, a - 1) * yp[b] * pw(zp, c - 1);
+    const double myz = b * c * xp[a] * pw(yp, b - 1) * pw(zp, c - 1);
+    o[0] = mx * R + m * x * R1;
+    o[1] = my * R + m * y * R1;
+    o[2] = mz * R + m * z * R1;
+    o[3] = R * mxx + 2. * R1 * mx * x + m * (R1 + x * x * R2);
+    o[4] = R * mxy + R1 * (mx * y + my * x) + m * x * y * R2;
+    o[5] = R * mxz + R1 * (mx * z + mz * x) + m * x * z * R2;
+    o[6] = R * myy + 2. * R1 * my * y + m * (R1 + y * y * R2);
+    o[7] = R * myz + R1 * (my * z + mz * y) + m * y * z * R2;
+    o[8] = R * mzz + 2. * R1 * mz * z + m * (R1 + z * z * R2);
+  };
+  if (pure) {
+#pragma unroll
+    for (int mo = 0; mo < 2 * L + 1; ++mo) {
+      double acc[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+      int ic = 0;
+#pragma unroll
+      for (int a = L; a >= 0; --a) {
+#pragma unroll
+        for (int b = L - a; b >= 0; --b) {
+          const double cf = kSph.c[L][mo][ic];
+          if (cf != 0.) {
+            double o[9];
+            term(a, b, o);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) acc[q] += cf * o[q];
+          }
+          ++ic;
+        }
+      }
+      sink(mo, acc[0], acc[1], acc[2], acc + 3);
+    }
+  } else {
+    int ic = 0;
+#pragma unroll
+    for (int a = L; a >= 0; --a) {
+#pragma unroll
+      for (int b = L - a; b >= 0; --b) {
+        double o[9];
+        term(a, b, o);
+        sink(ic, o[0], o[1], o[2], o + 3);
         ++ic;
       }
     }
@@ -301,6 +499,122 @@ __global__ __launch_bounds__(256) void k_grad_contract(
     else if (l == 2) shell_grad_hc<2>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
     else if (l == 3) shell_grad_hc<3>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
     else shell_grad_hc<4>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      for (int off = 32; off > 0; off >>= 1) s[d] += __shfl_xor(s[d], off);
+    if (at != cur) {
+      flush();
+      cur = at;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) run[d] = 0.;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      run[d] += s[d];
+      par[d] += s[d];
+    }
+  }
+  flush();
+  if (weight_derivs && lane == 0 && cur >= 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * t.iparent + d, 2.0 * par[d]);
+  }
+}
+
+// UKS basis-function part: as k_grad_contract with Y in plane 1, X_N in
+// plane 2 and X_Z in plane 3; per function grad B . Y + H (X_N c_s + X_Z c_z)
+// (the Hessian term for GGAs only).
+template <bool GGA>
+__global__ __launch_bounds__(256) void k_grad_contract_uks(
+    const WorkItem* __restrict__ items, const DevTask* __restrict__ tasks,
+    const DevShell* __restrict__ shells, const int32_t* __restrict__ shell_list,
+    const int32_t* __restrict__ lsh_first, const int32_t* __restrict__ shell_atom,
+    const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const double* __restrict__ scratch,
+    const double* __restrict__ coef_s, const double* __restrict__ coef_z, int weight_derivs,
+    double* __restrict__ grad) {
+  const WorkItem wi = items[blockIdx.x];
+  const DevTask t = tasks[wi.task];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p = wi.a * kPtBlock + lane;
+  const bool valid = p < t.npts;
+  const int pc = min(p, t.npts - 1);
+  const double gx = px[t.pt_off + pc], gy = py[t.pt_off + pc], gz = pz[t.pt_off + pc];
+  const int64_t b = t.red_off + wi.a;
+  double cs[3] = {0., 0., 0.}, cz[3] = {0., 0., 0.};
+  if (GGA && valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      cs[d] = coef_s[b * 256 + (1 + d) * 64 + lane];
+      cz[d] = coef_z[b * 256 + (1 + d) * 64 + lane];
+    }
+  }
+  const int64_t ld = t.npts_pad;
+  const int64_t plane = (int64_t)t.nbe * ld;
+  const int64_t m0 = (int64_t)wi.a * kPtBlock + lane;
+  const double* Yp = scratch + t.b_off + plane + m0;
+  const double* XNp = scratch + t.b_off + 2 * plane + m0;
+  const double* XZp = scratch + t.b_off + 3 * plane + m0;
+  int cur = -1;
+  double run[3] = {0., 0., 0.}, par[3] = {0., 0., 0.};
+  auto flush = [&]() {
+    if (cur >= 0 && lane == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * cur + d, -2.0 * run[d]);
+    }
+  };
+  for (int k = wave; k < t.nshells; k += 4) {
+    const int ish = __builtin_amdgcn_readfirstlane(shell_list[t.sh_off + k]);
+    const int at = __builtin_amdgcn_readfirstlane(shell_atom[ish]);
+    if (at < 0 || (weight_derivs && at == t.iparent)) continue;
+    const int bf0 = __builtin_amdgcn_readfirstlane(lsh_first[t.lsh_off + k]);
+    const DevShell* sh = shells + ish;
+    const int l = sh->l, nprim = sh->nprim;
+    const bool pure = sh->pure != 0;
+    const double x = gx - sh->O[0], y = gy - sh->O[1], z = gz - sh->O[2];
+    const double r2 = x * x + y * y + z * z;
+    double R = 0., R1 = 0., R2 = 0.;
+    for (int q = 0; q < nprim; ++q) {
+      const double a = sh->alpha[q], e = sh->coeff[q] * exp(-a * r2);
+      R += e;
+      R1 -= 2. * a * e;
+      R2 += 4. * a * a * e;
+    }
+    double s[3] = {0., 0., 0.};
+    auto put = [&](int f, double bx, double by, double bz, const double* h) {
+      const int64_t r = (int64_t)(bf0 + f) * ld;
+      const double Y = valid ? Yp[r] : 0.;
+      s[0] = fma(bx, Y, s[0]);
+      s[1] = fma(by, Y, s[1]);
+      s[2] = fma(bz, Y, s[2]);
+      if constexpr (GGA) {
+        const double xn = valid ? XNp[r] : 0., xz = valid ? XZp[r] : 0.;
+        const double vx = xn * cs[0] + xz * cz[0], vy = xn * cs[1] + xz * cz[1],
+                     vz = xn * cs[2] + xz * cz[2];
+        s[0] += h[0] * vx + h[1] * vy + h[2] * vz;
+        s[1] += h[1] * vx + h[3] * vy + h[4] * vz;
+        s[2] += h[2] * vx + h[4] * vy + h[5] * vz;
+      }
+    };
+    if constexpr (GGA) {
+      if (l == 0) shell_grad_hess<0>(x, y, z, R, R1, R2, pure, put);
+      else if (l == 1) shell_grad_hess<1>(x, y, z, R, R1, R2, pure, put);
+      else if (l == 2) shell_grad_hess<2>(x, y, z, R, R1, R2, pure, put);
+      else if (l == 3) shell_grad_hess<3>(x, y, z, R, R1, R2, pure, put);
+      else shell_grad_hess<4>(x, y, z, R, R1, R2, pure, put);
+    } else {
+      // LDA needs only grad B: the Hessian-vector form with c = 0.
+      auto put3 = [&](int f, double bx, double by, double bz, double, double, double) {
+        const double h0[6] = {0., 0., 0., 0., 0., 0.};
+        put(f, bx, by, bz, h0);
+      };
+      if (l == 0) shell_grad_hc<0>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+      else if (l == 1) shell_grad_hc<1>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+      else if (l == 2) shell_grad_hc<2>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+      else if (l == 3) shell_grad_hc<3>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+      else shell_grad_hc<4>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       for (int off = 32; off > 0; off >>= 1) s[d] += __shfl_xor(s[d], off);
@@ -444,11 +758,35 @@ int launch_grad_points(const CtxView& v, const WorkRange& r, int func, int gga, 
   return XCQ_SUCCESS;
 }
 
-int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl) {
+int launch_grad_points_uks(const CtxView& v, const WorkRange& r, int func, int gga, double* ds,
+                           double* dz, double* wf) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  hipLaunchKernelGGL(k_grad_points_uks, dim3((unsigned)r.n), dim3(64), 0, v.stream,
+                     v.work + r.off, v.tasks, v.w, func, gga, ds, dz, v.red, wf);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_grad_zpair(const CtxView& v, const WorkRange& r, int gga, const double* coef_s,
+                      const double* coef_z) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  double* scr = static_cast<double*>(v.scratch);
+  if (gga)
+    hipLaunchKernelGGL(k_grad_zpair<true>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
+                       v.work + r.off, v.tasks, scr, coef_s, coef_z);
+  else
+    hipLaunchKernelGGL(k_grad_zpair<false>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
+                       v.work + r.off, v.tasks, scr, coef_s, coef_z);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl,
+                         const double* P, double fac, int accumulate) {
   if (r.n == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_plane_product, dim3((unsigned)r.n), dim3(256), 0, v.stream,
-                     v.work + r.off, v.tasks, v.bfmap, v.P, static_cast<double*>(v.scratch),
-                     in_pl, out_pl);
+                     v.work + r.off, v.tasks, v.bfmap, P ? P : v.P,
+                     static_cast<double*>(v.scratch), in_pl, out_pl, fac, accumulate);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
@@ -463,6 +801,21 @@ int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                        static_cast<const double*>(v.scratch), coef, weight_derivs, grad);
   };
   gga ? go(k_grad_contract<true>) : go(k_grad_contract<false>);
+  XCQ_HIP(hipGetLastError());
+  return XCQ_SUCCESS;
+}
+
+int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
+                             const int32_t* shell_atom, const double* coef_s,
+                             const double* coef_z, int weight_derivs, double* grad) {
+  if (r.n == 0) return XCQ_SUCCESS;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)r.n), dim3(256), 0, v.stream, v.work + r.off,
+                       v.tasks, v.shells, v.shell_list, v.lsh, shell_atom, v.px, v.py, v.pz,
+                       static_cast<const double*>(v.scratch), coef_s, coef_z, weight_derivs,
+                       grad);
+  };
+  gga ? go(k_grad_contract_uks<true>) : go(k_grad_contract_uks<false>);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -271,14 +271,25 @@ int launch_fxc_points(const CtxView& v, const WorkRange& r, int func, int gga, c
 int launch_fxc_points_uks(const CtxView& v, const WorkRange& r, int func, int gga,
                           const double* ds, const double* dz, const double* dts,
                           const double* dtz, double* zs, double* zz);
-// EXC gradient (RKS): per-point coefficients, plane products out = 2 P_sub in,
-// the basis-function part and the grid-weight part (gradient.hip).
+// EXC gradient: per-point coefficients, plane products out = fac P_sub in
+// (+ out with accumulate; P defaults to the context's P), the
+// basis-function part and the grid-weight part (gradient.hip).  RKS:
 int launch_grad_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* dp,
                        double* coef, double* wf);
-int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl);
+int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int out_pl,
+                         const double* P = nullptr, double fac = 2.0, int accumulate = 0);
 int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                          const int32_t* shell_atom, const double* coef, int weight_derivs,
                          double* grad);
+// UKS: coefficients written over the P_s / P_z sums; Za_s -> plane 4,
+// Za_z -> plane 3; contraction with Y (plane 1), X_N (2), X_Z (3).
+int launch_grad_points_uks(const CtxView& v, const WorkRange& r, int func, int gga, double* ds,
+                           double* dz, double* wf);
+int launch_grad_zpair(const CtxView& v, const WorkRange& r, int gga, const double* coef_s,
+                      const double* coef_z);
+int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
+                             const int32_t* shell_atom, const double* coef_s,
+                             const double* coef_z, int weight_derivs, double* grad);
 int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
                     const double* dnear, const double* wf, double* grad);
 int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef);

@@ -516,6 +516,31 @@ void gauxc_integrator_eval_exc_grad_hf_rks(GauXCStatus* st, const GauXCIntegrato
   });
 }
 
+void gauxc_integrator_eval_exc_grad_uks(GauXCStatus* st, const GauXCIntegrator in,
+                                        const int64_t m, const int64_t n, const double* Ps,
+                                        const int64_t ldps, const double* Pz,
+                                        const int64_t ldpz, double* exc_grad) {
+  guarded_void(st, [&] {
+    if (!Ps || !Pz || !exc_grad) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    I.device_impl().eval_exc_grad_uks(m, n, Ps, ldps, Pz, ldpz, exc_grad,
+                                      IntegratorSettingsEXC_GRAD{});
+  });
+}
+
+void gauxc_integrator_eval_exc_grad_hf_uks(GauXCStatus* st, const GauXCIntegrator in,
+                                           const int64_t m, const int64_t n, const double* Ps,
+                                           const int64_t ldps, const double* Pz,
+                                           const int64_t ldpz, double* exc_grad) {
+  guarded_void(st, [&] {
+    if (!Ps || !Pz || !exc_grad) throw std::runtime_error("null pointer argument");
+    auto& I = deref<IntegratorT>(in, GauXC_Type_Integrator, "integrator");
+    IntegratorSettingsEXC_GRAD gs;
+    gs.include_weight_derivatives = false;
+    I.device_impl().eval_exc_grad_uks(m, n, Ps, ldps, Pz, ldpz, exc_grad, gs);
+  });
+}
+
 void gauxc_integrator_eval_fxc_contraction_rks(GauXCStatus* st, const GauXCIntegrator in,
                                                const int64_t m, const int64_t n,
                                                const double* P, const int64_t ldp,

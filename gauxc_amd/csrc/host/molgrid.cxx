@@ -1,5 +1,6 @@
 // Atomic quadratures, spherical micro-batching and molecular grid defaults.
 // Reference behaviour: src/grid_factory.cxx:35-69 (unpruned product grids),
+// :83-148 and :155-250 (pruned grids and the pruning schemes),
 // src/molgrid_defaults.cxx:52-71 (Mura-Knowles scale 7.0 for alkali and
 // alkaline-earth metals, else 5.0) and :144-202 (standard grid sizes).
 #include <gauxc/molgrid.hpp>
@@ -228,6 +229,107 @@ Grid AtomicGridFactory::generate_unpruned_grid(RadialQuad rq, RadialSize nrad, A
   return Grid(generate_unpruned_quadrature(rq, nrad, nang, rscal), bsz);
 }
 
+std::shared_ptr<AtomicQuadrature> AtomicGridFactory::generate_pruned_quadrature(
+    RadialQuad rq, RadialSize nrad, const std::vector<PruningRegion>& regions,
+    RadialScale rscal) {
+  if (rq != RadialQuad::MuraKnowles)
+    GAUXC_GENERIC_EXCEPTION("Only the Mura-Knowles radial quadrature is provided");
+  const auto rad = mura_knowles(nrad, rscal);
+  auto q = std::make_shared<AtomicQuadrature>();
+  size_t next = 0;
+  for (const auto& reg : regions) {
+    if (reg.idx_st != next || reg.idx_en < reg.idx_st || reg.idx_en > rad.npts())
+      GAUXC_GENERIC_EXCEPTION("Pruning regions must tile the radial nodes in order");
+    const auto ang = lebedev_laikov(reg.angular_size);
+    for (size_t ir = reg.idx_st; ir < reg.idx_en; ++ir) {
+      const double r = rad.points[ir][0];
+      for (size_t ia = 0; ia < ang.npts(); ++ia) {
+        const auto& u = ang.points[ia];
+        q->points.push_back({r * u[0], r * u[1], r * u[2]});
+        q->weights.push_back(rad.weights[ir] * ang.weights[ia]);
+      }
+    }
+    next = reg.idx_en;
+  }
+  if (next != rad.npts())
+    GAUXC_GENERIC_EXCEPTION("Pruning regions must tile the radial nodes in order");
+  return q;
+}
+
+Grid AtomicGridFactory::generate_pruned_grid(RadialQuad rq, RadialSize nrad,
+                                             const std::vector<PruningRegion>& regions,
+                                             RadialScale rscal, BatchSize bsz) {
+  return Grid(generate_pruned_quadrature(rq, nrad, regions, rscal), bsz);
+}
+
+Grid AtomicGridFactory::generate_grid(const atomic_grid_variant& spec, BatchSize bsz) {
+  if (const auto* u = std::get_if<UnprunedAtomicGridSpecification>(&spec))
+    return generate_unpruned_grid(u->radial_quad, u->radial_size, u->angular_size,
+                                  u->radial_scale, bsz);
+  const auto& p = std::get<PrunedAtomicGridSpecification>(spec);
+  return generate_pruned_grid(p.radial_quad, p.radial_size, p.pruning_regions, p.radial_scale,
+                              bsz);
+}
+
+namespace {
+
+// Algebraic order of a tabulated Lebedev rule of npts points (-1 if none),
+// and the smallest tabulated order >= order.
+int lebedev_order(int64_t npts) {
+  for (const auto& rule : kLebedevRules)
+    if (rule.npts == npts) return rule.order;
+  return -1;
+}
+int lebedev_next_order(int order) {
+  int best = -1;
+  for (const auto& rule : kLebedevRules)
+    if (rule.order >= order && (best < 0 || rule.order < best)) best = rule.order;
+  if (best < 0) GAUXC_GENERIC_EXCEPTION("No Lebedev rule of order >= " + std::to_string(order));
+  return best;
+}
+AngularSize lebedev_size(int order) {
+  for (const auto& rule : kLebedevRules)
+    if (rule.order == order) return AngularSize(rule.npts);
+  GAUXC_GENERIC_EXCEPTION("No Lebedev rule of order " + std::to_string(order));
+}
+
+PrunedAtomicGridSpecification three_regions(const UnprunedAtomicGridSpecification& unp,
+                                            size_t first_end, AngularSize first,
+                                            AngularSize second) {
+  const size_t n = static_cast<size_t>(unp.radial_size.get());
+  const size_t mid_end = n / 2 + 1;
+  return PrunedAtomicGridSpecification{
+      unp.radial_quad, unp.radial_size, unp.radial_scale,
+      {{0, first_end, first}, {first_end, mid_end, second}, {mid_end, n, unp.angular_size}}};
+}
+
+}  // namespace
+
+PrunedAtomicGridSpecification robust_psi4_pruning_scheme(UnprunedAtomicGridSpecification unp) {
+  const int base = lebedev_order(unp.angular_size.get());
+  if (base < 0) GAUXC_GENERIC_EXCEPTION("Invalid Base Grid");
+  const int med = lebedev_next_order(base > 6 ? base - 6 : base);
+  const size_t n = static_cast<size_t>(unp.radial_size.get());
+  return three_regions(unp, n / 4 + 1, lebedev_size(7), lebedev_size(med));
+}
+
+PrunedAtomicGridSpecification treutler_pruning_scheme(UnprunedAtomicGridSpecification unp) {
+  const size_t n = static_cast<size_t>(unp.radial_size.get());
+  return three_regions(unp, n / 3 + 1, lebedev_size(7), lebedev_size(11));
+}
+
+PrunedAtomicGridSpecification create_pruned_spec(PruningScheme scheme,
+                                                 UnprunedAtomicGridSpecification unp) {
+  switch (scheme) {
+    case PruningScheme::Robust: return robust_psi4_pruning_scheme(unp);
+    case PruningScheme::Treutler: return treutler_pruning_scheme(unp);
+    default:
+      return PrunedAtomicGridSpecification{
+          unp.radial_quad, unp.radial_size, unp.radial_scale,
+          {{0, static_cast<size_t>(unp.radial_size.get()), unp.angular_size}}};
+  }
+}
+
 RadialScale default_mk_radial_scaling_factor(AtomicNumber Z) {
   switch (Z.get()) {
     case 3: case 4: case 11: case 12: case 19: case 20:
@@ -266,19 +368,20 @@ UnprunedAtomicGridSpecification MolGridFactory::create_default_unpruned_grid_spe
 atomic_grid_spec_map MolGridFactory::create_default_gridmap(const Molecule& mol,
                                                             PruningScheme ps, RadialQuad rq,
                                                             AtomicGridSizeDefault s) {
-  if (ps != PruningScheme::Unpruned)
-    GAUXC_GENERIC_EXCEPTION("Only unpruned atomic grids are provided");
+  // GauXC builds every element's specification through create_pruned_spec
+  // (include/gauxc/molgrid/defaults.hpp:43-64); Unpruned stays one region.
   atomic_grid_spec_map m;
   for (const auto& a : mol)
-    if (!m.count(a.Z)) m.emplace(a.Z, create_default_unpruned_grid_spec(a.Z, rq, s));
+    if (!m.count(a.Z)) {
+      auto unp = create_default_unpruned_grid_spec(a.Z, rq, s);
+      if (ps == PruningScheme::Unpruned) m.emplace(a.Z, unp);
+      else m.emplace(a.Z, create_pruned_spec(ps, unp));
+    }
   return m;
 }
 
 MolGrid::MolGrid(const atomic_grid_spec_map& specs, BatchSize bsz) {
-  for (const auto& [Z, sp] : specs)
-    grids_.emplace(Z, AtomicGridFactory::generate_unpruned_grid(sp.radial_quad, sp.radial_size,
-                                                                sp.angular_size,
-                                                                sp.radial_scale, bsz));
+  for (const auto& [Z, sp] : specs) grids_.emplace(Z, AtomicGridFactory::generate_grid(sp, bsz));
 }
 
 MolGrid MolGridFactory::create_default_molgrid(const Molecule& mol, PruningScheme ps,

@@ -279,10 +279,34 @@ void DeviceXCIntegratorImpl::eval_exc_grad(int64_t m, int64_t n, const double* P
   if (m != n) GAUXC_GENERIC_EXCEPTION("P Must Be Square");
   if (m != nbf) GAUXC_GENERIC_EXCEPTION("P Must Have Same Dimension as Basis");
   if (ldp < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDP");
-  if (!s.lb->state().modified_weights_are_stored)
-    GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
   if (s.func->is_polarized())
     GAUXC_GENERIC_EXCEPTION("RKS evaluation needs an unpolarized functional");
+  exc_grad_impl(m, P, ldp, nullptr, 0, grad, settings);
+}
+
+void DeviceXCIntegratorImpl::eval_exc_grad_uks(int64_t m, int64_t n, const double* Ps,
+                                               int64_t ldps, const double* Pz, int64_t ldpz,
+                                               double* grad,
+                                               const IntegratorSettingsXC& settings) {
+  // Argument checks of reference_replicated_xc_host_integrator_exc_grad.hpp:70-79.
+  auto& s = *s_;
+  const int64_t nbf = s.lb->basis().nbf();
+  if (m != n) GAUXC_GENERIC_EXCEPTION("P Must Be Square");
+  if (m != nbf) GAUXC_GENERIC_EXCEPTION("P Must Have Same Dimension as Basis");
+  if (ldps < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDPS");
+  if (ldpz < nbf) GAUXC_GENERIC_EXCEPTION("Invalid LDPZ");
+  if (!s.func->is_polarized())
+    GAUXC_GENERIC_EXCEPTION("UKS evaluation needs a polarized functional");
+  if (!Pz) GAUXC_GENERIC_EXCEPTION("null pointer argument");
+  exc_grad_impl(m, Ps, ldps, Pz, ldpz, grad, settings);
+}
+
+void DeviceXCIntegratorImpl::exc_grad_impl(int64_t m, const double* P, int64_t ldp,
+                                           const double* Pz, int64_t ldpz, double* grad,
+                                           const IntegratorSettingsXC& settings) {
+  auto& s = *s_;
+  if (!s.lb->state().modified_weights_are_stored)
+    GAUXC_GENERIC_EXCEPTION("Weights Have Not Been Modified");
   if (s.prec != DevicePrecision::FP64)
     GAUXC_GENERIC_EXCEPTION("The EXC gradient is provided in FP64 (local work driver Default)");
   if (!P || !grad) GAUXC_GENERIC_EXCEPTION("null pointer argument");
@@ -299,10 +323,17 @@ void DeviceXCIntegratorImpl::eval_exc_grad(int64_t m, int64_t n, const double* P
   const auto t0 = std::chrono::steady_clock::now();
   s.ensure_static_data();
   check(xcq_upload_density(s.ctx, static_cast<int32_t>(m), P, ldp), "xcq_upload_density");
+  if (Pz)
+    check(xcq_upload_density_z(s.ctx, static_cast<int32_t>(m), Pz, ldpz), "xcq_upload_density_z");
   const auto t1 = std::chrono::steady_clock::now();
-  check(xcq_exc_grad(s.ctx, xcq_func_id(*s.func), static_cast<int32_t>(mol.size()), xyz.data(),
-                     weight_derivs ? 1 : 0),
-        "xcq_exc_grad");
+  if (Pz)
+    check(xcq_exc_grad_uks(s.ctx, xcq_func_id(*s.func), static_cast<int32_t>(mol.size()),
+                           xyz.data(), weight_derivs ? 1 : 0),
+          "xcq_exc_grad_uks");
+  else
+    check(xcq_exc_grad(s.ctx, xcq_func_id(*s.func), static_cast<int32_t>(mol.size()),
+                       xyz.data(), weight_derivs ? 1 : 0),
+          "xcq_exc_grad");
   check(xcq_synchronize(s.ctx), "xcq_synchronize");
   const auto t2 = std::chrono::steady_clock::now();
   double exc = 0.;

@@ -238,6 +238,13 @@ void gauxc_integrator_eval_exc_grad_rks(GauXCStatus* status, const GauXCIntegrat
                                         const int64_t m, const int64_t n,
                                         const double* density_matrix, const int64_t ldp,
                                         double* exc_grad);
+/* UKS EXC nuclear gradient (GauXC include/gauxc/c/xc_integrator.h:304-313):
+ * P_s and P_z; FP64 local work driver, polarized functional. */
+void gauxc_integrator_eval_exc_grad_uks(GauXCStatus* status, const GauXCIntegrator integrator,
+                                        const int64_t m, const int64_t n,
+                                        const double* density_matrix_s, const int64_t ldp_s,
+                                        const double* density_matrix_z, const int64_t ldp_z,
+                                        double* exc_grad);
 /* Extension: the Hellmann-Feynman gradient (no grid-weight derivatives;
  * GauXC's IntegratorSettingsEXC_GRAD::include_weight_derivatives = false,
  * which its C API does not expose). */
@@ -245,6 +252,11 @@ void gauxc_integrator_eval_exc_grad_hf_rks(GauXCStatus* status,
                                            const GauXCIntegrator integrator, const int64_t m,
                                            const int64_t n, const double* density_matrix,
                                            const int64_t ldp, double* exc_grad);
+void gauxc_integrator_eval_exc_grad_hf_uks(GauXCStatus* status,
+                                           const GauXCIntegrator integrator, const int64_t m,
+                                           const int64_t n, const double* density_matrix_s,
+                                           const int64_t ldp_s, const double* density_matrix_z,
+                                           const int64_t ldp_z, double* exc_grad);
 /* RKS FXC contraction (GauXC include/gauxc/c/xc_integrator.h:351-362):
  * fxc = d VXC[P + e tP] / de for the trial density tP; FP64 local work
  * driver, unpolarized functional. */

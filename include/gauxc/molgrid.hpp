@@ -9,7 +9,10 @@
 //  * Mura-Knowles radial rule: x_i = (i+1)/(n+1), r_i = -s ln(1 - x_i^3),
 //    w_i = 3 s x_i^2 / ((1 - x_i^3)(n+1)) * r_i^2  (s = radial scale).
 //  * Lebedev-Laikov angular rules with weights summing to 4 pi.
-//  * Product grid: every radial node carries a full angular shell.
+//  * Product grid: every radial node carries a full angular shell; a pruned
+//    grid gives the radial nodes of each pruning region the region's
+//    angular rule (src/grid_factory.cxx:83-148, the Robust and Treutler
+//    schemes of :155-240).
 //  * Batching: recursive octree bisection of the atomic grid's bounding
 //    box until a cell holds at most BatchSize points; a batch's screening
 //    box is the tight bounding box of its points.
@@ -21,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <tuple>
+#include <variant>
 #include <vector>
 
 namespace GauXC {
@@ -72,7 +76,38 @@ struct UnprunedAtomicGridSpecification {
   AngularSize angular_size;
 };
 
-using atomic_grid_spec_map = std::map<AtomicNumber, UnprunedAtomicGridSpecification>;
+/// Radial nodes [idx_st, idx_en) carry the angular rule of angular_size
+/// (GauXC include/gauxc/grid_factory.hpp:32-45).
+struct PruningRegion {
+  size_t idx_st;
+  size_t idx_en;
+  AngularSize angular_size;
+};
+
+struct PrunedAtomicGridSpecification {
+  RadialQuad radial_quad;
+  RadialSize radial_size;
+  RadialScale radial_scale;
+  std::vector<PruningRegion> pruning_regions;
+};
+
+using atomic_grid_variant =
+    std::variant<UnprunedAtomicGridSpecification, PrunedAtomicGridSpecification>;
+using atomic_grid_spec_map = std::map<AtomicNumber, atomic_grid_variant>;
+
+/// Psi4's "Robust" pruning of an unpruned Lebedev specification: the first
+/// quarter of the radial nodes (n/4 + 1) on the order-7 rule, up to n/2 + 1
+/// on the rule of order (base - 6) (rounded up to a tabulated order; the
+/// base order when it is 6 or less), the rest on the base rule
+/// (src/grid_factory.cxx:155-187).
+PrunedAtomicGridSpecification robust_psi4_pruning_scheme(UnprunedAtomicGridSpecification unp);
+/// Treutler's pruning: n/3 + 1 nodes on order 7, up to n/2 + 1 on order 11,
+/// the rest on the base rule (src/grid_factory.cxx:191-220).
+PrunedAtomicGridSpecification treutler_pruning_scheme(UnprunedAtomicGridSpecification unp);
+/// The scheme's pruned specification; Unpruned gives a single region
+/// (src/grid_factory.cxx:225-250).
+PrunedAtomicGridSpecification create_pruned_spec(PruningScheme scheme,
+                                                 UnprunedAtomicGridSpecification unp);
 
 class MolGrid {
   std::map<AtomicNumber, Grid> grids_;
@@ -100,6 +135,13 @@ struct AtomicGridFactory {
                                                                         RadialScale rscal);
   static Grid generate_unpruned_grid(RadialQuad rq, RadialSize nrad, AngularSize nang,
                                      RadialScale rscal, BatchSize bsz);
+  static std::shared_ptr<AtomicQuadrature> generate_pruned_quadrature(
+      RadialQuad rq, RadialSize nrad, const std::vector<PruningRegion>& regions,
+      RadialScale rscal);
+  static Grid generate_pruned_grid(RadialQuad rq, RadialSize nrad,
+                                   const std::vector<PruningRegion>& regions, RadialScale rscal,
+                                   BatchSize bsz);
+  static Grid generate_grid(const atomic_grid_variant& spec, BatchSize bsz);
 };
 
 struct MolGridFactory {

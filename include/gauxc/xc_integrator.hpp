@@ -81,6 +81,9 @@ namespace detail {
 class DeviceXCIntegratorImpl {
   struct State;
   std::unique_ptr<State> s_;
+  // The EXC gradient after the spin-specific checks (Pz null for RKS).
+  void exc_grad_impl(int64_t m, const double* P, int64_t ldp, const double* Pz, int64_t ldpz,
+                     double* grad, const IntegratorSettingsXC& s);
 
  public:
   /// The device, the fill fraction or the caller's device arena come from
@@ -102,6 +105,9 @@ class DeviceXCIntegratorImpl {
   /// functional); settings of type IntegratorSettingsEXC_GRAD select the form.
   void eval_exc_grad(int64_t m, int64_t n, const double* P, int64_t ldp, double* grad,
                      const IntegratorSettingsXC& s);
+  /// UKS EXC nuclear gradient (FP64; polarized functional).
+  void eval_exc_grad_uks(int64_t m, int64_t n, const double* Ps, int64_t ldps, const double* Pz,
+                         int64_t ldpz, double* grad, const IntegratorSettingsXC& s);
   /// RKS FXC contraction d VXC[P + e tP] / de (FP64; unpolarized functional).
   void eval_fxc_contraction(int64_t m, int64_t n, const double* P, int64_t ldp, const double* tP,
                             int64_t ldtp, double* FXC, int64_t ldfxc,
@@ -180,6 +186,15 @@ class XCIntegrator {
                               const IntegratorSettingsXC& s = IntegratorSettingsEXC_GRAD{}) {
     exc_grad_type g(3 * impl().load_balancer().molecule().size());
     impl().eval_exc_grad(P.rows(), P.cols(), P.data(), P.rows(), g.data(), s);
+    return g;
+  }
+  /// GauXC XCIntegrator::eval_exc_grad(Ps, Pz, settings)
+  /// (include/gauxc/xc_integrator/impl.hpp:89-93), UKS.
+  exc_grad_type eval_exc_grad(const MatrixType& Ps, const MatrixType& Pz,
+                              const IntegratorSettingsXC& s = IntegratorSettingsEXC_GRAD{}) {
+    exc_grad_type g(3 * impl().load_balancer().molecule().size());
+    impl().eval_exc_grad_uks(Ps.rows(), Ps.cols(), Ps.data(), Ps.rows(), Pz.data(), Pz.rows(),
+                             g.data(), s);
     return g;
   }
   /// GauXC XCIntegrator::eval_fxc_contraction(P, tP)

@@ -262,6 +262,14 @@ int xcq_download_fxc_uks(xcq_context ctx, double* nel, double* fxc_s, double* fx
  *      (atom-major). ---- */
 int xcq_exc_grad(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
                  int weight_derivs);
+/* UKS EXC nuclear gradient (GauXC eval_exc_grad(Ps, Pz): the same driver's
+ * spin_dim 2 branches, reference_replicated_xc_host_integrator_exc_grad.hpp:
+ * 64-100,106-600; C API gauxc_integrator_eval_exc_grad_uks, include/gauxc/c/
+ * xc_integrator.h:304-313).  P_s through xcq_upload_density, P_z through
+ * xcq_upload_density_z; otherwise as xcq_exc_grad; results through
+ * xcq_download_exc_grad. */
+int xcq_exc_grad_uks(xcq_context ctx, int functional, int32_t natoms, const double* xyz,
+                     int weight_derivs);
 int xcq_download_exc_grad(xcq_context ctx, double* exc, double* nel, double* grad);
 /* Integrated density only (eval_integrate_den). */
 int xcq_integrate_den(xcq_context ctx, double* nel);

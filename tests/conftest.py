@@ -55,6 +55,34 @@ def benzene_setup(tag, grid="UltraFineGrid"):
     return d, mol, bs, lb
 
 
+def unpack_sym(packed, n):
+    """Symmetric n x n matrix from its packed upper triangle (row-major
+    triu order)."""
+    m = np.zeros((n, n))
+    m[np.triu_indices(n)] = packed
+    return m + np.triu(m, 1).T
+
+
+def cytosine_uks_setup():
+    """Doublet cytosine / SVWN5 / cc-pVDZ on the Robust-pruned ultra-fine grid
+    (tests/xc_integrator.cxx:455-459): fixture dict with unpacked densities
+    and potentials, molecule, basis, load balancer."""
+    import gauxc_amd as gx
+    d = dict(load_golden("cytosine_svwn5_uks.npz"))
+    n = int(d["nbf"][0])
+    for k in ("density_s", "density_z", "vxc_s", "vxc_z"):
+        d[k] = unpack_sym(d[k + "_packed"], n)
+    mol = gx.Molecule(d["Z"], d["xyz"])
+    bs = gx.BasisSet.from_arrays(d["sh_nprim"], d["sh_l"], d["sh_pure"], d["sh_alpha"],
+                                 d["sh_coeff"], d["sh_origin"], normalize=False,
+                                 shell_tolerance=EPS)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, "Robust", 512, "MuraKnowles",
+                                                  "UltraFineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    return d, mol, bs, lb
+
+
 CALLER_SRC = os.path.join(ROOT, "tests", "cxx", "caller.cxx")
 CALLER_BIN = os.path.join(ROOT, "tests", "cxx", "_build", "caller")
 

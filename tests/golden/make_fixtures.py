@@ -12,8 +12,14 @@ Fixtures converted (reference paths relative to the GauXC tree):
   MOLECULE, BASIS (stored already normalised, read back with
   ``normalize=false`` at ``src/external/hdf5_read.cxx:47-48``), DENSITY, VXC,
   EXC (checked by ``tests/xc_integrator.cxx:199-212``), TRIAL_DENSITY and FXC
-  (``tests/2nd_derivative_test.cxx:121-126``) and EXC_GRAD with weight
-  derivatives (``tests/xc_integrator.cxx:117-137,278-289``).
+  (``tests/2nd_derivative_test.cxx:121-126``), EXC_GRAD with weight
+  derivatives and EXC_GRAD_HELLFEY without (``tests/xc_integrator.cxx:
+  117-145,278-300``).
+* ``tests/ref_data/benzene_svwn5_cc-pvdz_ufg_ssf_{robust,treutler}_prune.hdf5``
+  -> ``benzene_svwn5_pruned.npz``: EXC, VXC, FXC and both gradients on the
+  Robust and Treutler pruned grids (``tests/xc_integrator.cxx:410-420``,
+  ``tests/2nd_derivative_test.cxx:198-208``); inputs are those of the
+  unpruned SVWN5 file.
 * ``tests/ref_data/benzene_weights_ssf.hdf5`` -> ``benzene_weights_ssf.npz``:
   15 tasks with raw (``tasks_unm``) points/weights and SSF-modified weights
   (``tasks_mod``), checked by ``tests/weights_host.hpp:22-57``.
@@ -24,6 +30,12 @@ Fixtures converted (reference paths relative to the GauXC tree):
   the UKS check of ``tests/xc_integrator.cxx:218-237,448-453`` (scalar and
   z densities and potentials, EXC) and the UKS FXC check of
   ``tests/2nd_derivative_test.cxx:128-134`` (trial densities, FXC_s, FXC_z).
+* ``tests/ref_data/cytosine_svwn5_cc-pvdz_ufg_ssf_robust_uks.hdf5`` ->
+  ``cytosine_svwn5_uks.npz``: the UKS check of ``tests/xc_integrator.cxx:
+  218-237,455-459`` on the Robust-pruned ultra-fine grid (EXC; scalar and z
+  densities and potentials, stored as packed upper triangles since all four
+  are symmetric) and the UKS gradient checks of :278-300 (EXC_GRAD with
+  weight derivatives, EXC_GRAD_HELLFEY without).
 * ``tests/ref_data/water_cc-pVDZ_collocation.hdf5`` ->
   ``water_collocation.npz``: gau2grid collocation + gradient golden values
   (``tests/collocation_host.hpp:111-180``).
@@ -263,8 +275,32 @@ def main():
         d["trial_density"] = f.read("/TRIAL_DENSITY").T.copy()
         d["fxc"] = f.read("/FXC").T.copy()
         d["exc_grad"] = f.read("/EXC_GRAD")
+        # Hellmann-Feynman gradient (tests/xc_integrator.cxx:138-145,291-300).
+        d["exc_grad_hf"] = f.read("/EXC_GRAD_HELLFEY")
         f.close()
         np.savez_compressed(os.path.join(OUT, f"benzene_{tag}.npz"), **d)
+
+    # Pruned grids (tests/xc_integrator.cxx:410-420, 2nd_derivative_test.cxx:
+    # 198-208): the same molecule, basis, density and trial density as the
+    # unpruned SVWN5 file (checked here), on the Robust and Treutler pruned
+    # ultra-fine grids; EXC, VXC, FXC (packed upper triangles) and both
+    # gradients.
+    base = dict(np.load(os.path.join(OUT, "benzene_svwn5.npz")))
+    d = {}
+    for scheme in ("robust", "treutler"):
+        f = H5File(os.path.join(REF, "ref_data", f"benzene_svwn5_cc-pvdz_ufg_ssf_{scheme}_prune.hdf5"))
+        Z, xyz = read_mol(f)
+        assert np.array_equal(Z, base["Z"]) and np.array_equal(xyz, base["xyz"])
+        assert np.array_equal(f.read("/DENSITY").T, base["density"])
+        assert np.array_equal(f.read("/TRIAL_DENSITY").T, base["trial_density"])
+        iu = np.triu_indices(base["density"].shape[0])
+        d[f"{scheme}_exc"] = f.read("/EXC")
+        d[f"{scheme}_vxc_packed"] = f.read("/VXC").T[iu].copy()
+        d[f"{scheme}_fxc_packed"] = f.read("/FXC").T[iu].copy()
+        d[f"{scheme}_exc_grad"] = f.read("/EXC_GRAD")
+        d[f"{scheme}_exc_grad_hf"] = f.read("/EXC_GRAD_HELLFEY")
+        f.close()
+    np.savez_compressed(os.path.join(OUT, "benzene_svwn5_pruned.npz"), **d)
 
     f = H5File(os.path.join(REF, "ref_data", "benzene_weights_ssf.hdf5"))
     Z, xyz = read_mol(f)
@@ -339,6 +375,24 @@ def main():
     d["exc"] = f.read("/EXC")
     f.close()
     np.savez_compressed(os.path.join(OUT, "li_svwn5_uks.npz"), **d)
+
+    # UKS with gradients (tests/xc_integrator.cxx:455-459, 278-300): doublet
+    # cytosine / SVWN5 / cc-pVDZ, Robust pruning.  /EXC_GRAD carries
+    # includes_weight_derivatives = 1 (read with h5dump -A).
+    f = H5File(os.path.join(REF, "ref_data", "cytosine_svwn5_cc-pvdz_ufg_ssf_robust_uks.hdf5"))
+    Z, xyz = read_mol(f)
+    d = dict(Z=Z, xyz=xyz, **read_basis(f))
+    for src, dst in (("/DENSITY_SCALAR", "density_s"), ("/DENSITY_Z", "density_z"),
+                     ("/VXC_SCALAR", "vxc_s"), ("/VXC_Z", "vxc_z")):
+        m = f.read(src).T
+        assert np.abs(m - m.T).max() < 1e-13
+        d[dst + "_packed"] = m[np.triu_indices(m.shape[0])].copy()
+    d["nbf"] = np.array([m.shape[0]])
+    d["exc"] = f.read("/EXC")
+    d["exc_grad"] = f.read("/EXC_GRAD")
+    d["exc_grad_hf"] = f.read("/EXC_GRAD_HELLFEY")
+    f.close()
+    np.savez_compressed(os.path.join(OUT, "cytosine_svwn5_uks.npz"), **d)
     print("fixtures written to", OUT)
 
 

@@ -1,9 +1,11 @@
-"""RKS EXC nuclear gradient on the device (GauXC eval_exc_grad(P)): GauXC's
-benzene SVWN5 / PBE0 EXC_GRAD fixtures (with grid-weight derivatives)
-through the GauXC API at GauXC's bar (tests/xc_integrator.cxx:278-289) and
-tighter; a water cluster and a two-atom basis with g functions against the
-oracle, with and without the weight derivatives; translational invariance;
-the boundary's checks."""
+"""EXC nuclear gradient on the device (GauXC eval_exc_grad(P) and
+eval_exc_grad(Ps, Pz)): GauXC's benzene SVWN5 / PBE0 EXC_GRAD and
+EXC_GRAD_HELLFEY fixtures through the GauXC API at GauXC's bar
+(tests/xc_integrator.cxx:278-300) and tighter; water clusters and a
+two-atom basis with g functions against the oracle, RKS and UKS, with and
+without the weight derivatives; translational invariance; UKS with P_z = 0
+against RKS; the boundary's checks.  (The cytosine UKS fixture is checked in
+the UKS tests.)"""
 import numpy as np
 import pytest
 
@@ -30,6 +32,10 @@ def test_exc_grad_benzene_matches_gauxc_fixture(gx, gpu, tag, func):
     _flush(f"|dG|/sqrt(3N)={err:.3e} max|dG|={np.abs(g - ref).max():.3e}")
     assert err < 1e-11  # GauXC's bar is 1e-8
     assert np.abs(g.sum(axis=0)).max() < 1e-11
+    gh = integ.eval_exc_grad(d["density"], include_weight_derivatives=False)
+    err_h = np.linalg.norm(gh - d["exc_grad_hf"].reshape(-1, 3)) / np.sqrt(g.size)
+    _flush(f"Hellmann-Feynman |dG|/sqrt(3N)={err_h:.3e}")
+    assert err_h < 1e-11
     # An EXC/VXC evaluation afterwards is unaffected.
     exc, vxc = integ.eval_exc_vxc(d["density"])
     assert abs(exc - d["exc"][0]) <= 1e-10 * abs(d["exc"][0])
@@ -78,6 +84,49 @@ def test_exc_grad_water_cluster_matches_oracle(gx, orc, gpu, func):
         ctx.results_grad()
 
 
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
+def test_uks_exc_grad_water_cluster_matches_oracle(gx, orc, gpu, func):
+    xyz, mol, bs, lb, P = _cluster(gx)
+    nbf = bs.nbf()
+    pa = P
+    pb = gx.workloads.density(nbf, 5 * 4 - 1, 32)
+    Ps, Pz = pa + pb, pa - pb
+    t = lb.get_tasks()
+    w = orc.ssf_weights(xyz, t)
+    sh = orc.shells_from(bs.xcq_shells())
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional(func, polarized=True), lb)
+    for wd in (True, False):
+        g_o = orc.exc_grad_uks(func, nbf, sh, xyz, Ps, Pz, t, weights=w, weight_derivs=wd)
+        _flush(f"UKS EXC gradient 4 waters {func} weight derivatives {wd}")
+        g = integ.eval_exc_grad(Ps, Pz, include_weight_derivatives=wd)
+        _flush(f"max|dG|={np.abs(g - g_o).max():.3e} (max|G|={np.abs(g_o).max():.3e})")
+        assert np.abs(g - g_o).max() <= GRAD_BAR
+        if wd:
+            assert np.abs(g.sum(axis=0)).max() < 1e-11
+    # P_z = 0 gives the RKS gradient of P_s / 2.
+    rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional(func), lb)
+    g_r = rks.eval_exc_grad(pa)
+    g_u = integ.eval_exc_grad(2 * pa, np.zeros_like(pa))
+    assert np.abs(g_u - g_r).max() <= 1e-12 * np.abs(g_r).max()
+    # The device C ABI directly.
+    ctx = gx.DeviceContext(0)
+    ctx.upload_basis(bs.xcq_shells())
+    t2 = dict(t)
+    t2["weights"] = w
+    ctx.upload_tasks(t2)
+    ctx.upload_density(Ps)
+    with pytest.raises(gx.GauXCError, match="needs P_z"):
+        ctx.exc_grad(func, xyz, uks=True)
+    ctx.upload_density_z(Pz)
+    ctx.exc_grad(func, xyz, weight_derivs=True, uks=True)
+    exc, nel, g3 = ctx.results_grad()
+    exc_o, nel_o = orc.exc_vxc_uks(func, nbf, sh, Ps, Pz, t, weights=w)[:2]
+    assert abs(exc - exc_o) <= 1e-10 and abs(nel - nel_o) <= 1e-10
+    assert np.abs(g3 - orc.exc_grad_uks(func, nbf, sh, xyz, Ps, Pz, t, weights=w)).max() <= GRAD_BAR
+
+
 @pytest.mark.parametrize("lmax", [3, 4])
 def test_exc_grad_high_l_matches_oracle(gx, orc, gpu, lmax):
     """Basis with f and g functions (spherical and Cartesian) and a
@@ -111,6 +160,16 @@ def test_exc_grad_high_l_matches_oracle(gx, orc, gpu, lmax):
     g = integ.eval_exc_grad(P)
     _flush(f"max|dG|={np.abs(g - g_o).max():.3e} (max|G|={np.abs(g_o).max():.3e})")
     assert np.abs(g - g_o).max() <= GRAD_BAR
+    # UKS: the full Hessian per function (two coefficient vectors).
+    Pb = gx.workloads.density(bs.nbf(), 3, 8)
+    gu_o = orc.exc_grad_uks("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P + Pb,
+                            P - Pb, t, weights=w)
+    integ_u = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional("PBE", polarized=True), lb)
+    _flush(f"UKS EXC gradient lmax={lmax}")
+    gu = integ_u.eval_exc_grad(P + Pb, P - Pb)
+    _flush(f"max|dG|={np.abs(gu - gu_o).max():.3e}")
+    assert np.abs(gu - gu_o).max() <= GRAD_BAR
 
 
 def test_exc_grad_boundary_errors(gx, gpu):
@@ -128,3 +187,9 @@ def test_exc_grad_boundary_errors(gx, gpu):
     integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional("SVWN5"), lb)
     with pytest.raises(gx.GauXCError, match="Same Dimension"):
         integ.eval_exc_grad(P[:-1, :-1])
+    with pytest.raises(gx.GauXCError, match="polarized"):
+        integ.eval_exc_grad(P, 0 * P)
+    mixed_u = gx.XCIntegratorFactory("Device", "Replicated", "Default", "Scheme1-Mixed").get_instance(
+        gx.Functional("SVWN5", polarized=True), lb)
+    with pytest.raises(gx.GauXCError, match="FP64"):
+        mixed_u.eval_exc_grad(P, 0 * P)

@@ -114,3 +114,48 @@ def test_uks_boundary_checks(gx, gpu):
     with pytest.raises(gx.GauXCError, match="FP64"):
         gx.XCIntegratorFactory("Device", "Replicated", "Default", "Scheme1-Mixed",
                                "Default").get_instance(pol, lb).eval_exc_vxc(pa + pb, pa - pb)
+
+
+def test_uks_cytosine_matches_gauxc_fixture_and_oracle(gx, orc, gpu):
+    """Doublet cytosine / SVWN5 / cc-pVDZ on the Robust-pruned ultra-fine grid
+    (tests/xc_integrator.cxx:218-237,278-300,455-459) through the GauXC API:
+    EXC / VXC against GauXC's values at GauXC's bars and against the oracle
+    at the FP64 bars; the full UKS gradient against the fixture (1e-12) and
+    the oracle; the Hellmann-Feynman gradient against the oracle (the
+    fixture's EXC, VXC_s and Hellmann-Feynman gradient sit 2.6e-11
+    relative, 6.5e-12 and 2.8e-9 from the oracle, which its own test
+    records)."""
+    from conftest import cytosine_uks_setup
+    d, mol, bs, lb = cytosine_uks_setup()
+    t = lb.get_tasks()
+    xyz = np.asarray(d["xyz"], dtype=np.float64)
+    n = bs.nbf()
+    w = orc.ssf_weights(xyz, t)
+    sh = orc.shells_from(bs.xcq_shells())
+    Ps, Pz = d["density_s"], d["density_z"]
+    exc_o, _, vs_o, vz_o = orc.exc_vxc_uks("SVWN5", n, sh, Ps, Pz, t, weights=w)[:4]
+    g_o = orc.exc_grad_uks("SVWN5", n, sh, xyz, Ps, Pz, t, weights=w)
+    gh_o = orc.exc_grad_uks("SVWN5", n, sh, xyz, Ps, Pz, t, weights=w, weight_derivs=False)
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional("SVWN5", polarized=True), lb)
+    _flush("UKS cytosine SVWN5 (Robust grid) through the GauXC API")
+    exc, vs, vz = integ.eval_exc_vxc(Ps, Pz)
+    _flush(f"dEXC(fixture)={exc - d['exc'][0]:.3e} dEXC(oracle)={exc - exc_o:.3e} "
+           f"max|dVs|={np.abs(vs - vs_o).max():.3e} max|dVz|={np.abs(vz - vz_o).max():.3e}")
+    assert abs(exc - d["exc"][0]) <= 1e-10 * abs(exc)
+    assert np.linalg.norm(vs - d["vxc_s"]) / n < 1e-10
+    assert np.linalg.norm(vz - d["vxc_z"]) / n < 1e-10
+    assert abs(exc - exc_o) <= FP64_EXC
+    assert np.abs(vs - vs_o).max() <= FP64_VXC and np.abs(vz - vz_o).max() <= FP64_VXC
+    _flush("UKS cytosine EXC gradient")
+    g = integ.eval_exc_grad(Ps, Pz)
+    gh = integ.eval_exc_grad(Ps, Pz, include_weight_derivatives=False)
+    _flush(f"max|dG|(fixture)={np.abs(g.ravel() - d['exc_grad']).max():.3e} "
+           f"max|dG|(oracle)={np.abs(g - g_o).max():.3e} "
+           f"max|dG_HF|(oracle)={np.abs(gh - gh_o).max():.3e}")
+    assert np.abs(g.ravel() - d["exc_grad"]).max() < 1e-12
+    assert np.abs(g - g_o).max() < 1e-12
+    assert np.abs(g.sum(axis=0)).max() < 1e-12
+    assert np.abs(gh - gh_o).max() < 1e-12
+    assert np.linalg.norm(gh.ravel() - d["exc_grad_hf"]) / np.sqrt(gh.size) < 1e-8

@@ -17,9 +17,12 @@ Each check is chosen so that a plausible mistake fails it:
     the second derivatives by differences of the first; FXC as the
     derivative of VXC along tP (RKS and UKS); UKS with P_z = 0 equals RKS;
   * EXC gradient: Hessian collocation against gau2grid's deriv2 and central
-    differences; GauXC's benzene EXC_GRAD fixtures (with grid-weight
-    derivatives); translational invariance; dEXC/dR by central differences
-    (RKS and UKS); UKS with P_z = 0 equals RKS.
+    differences; GauXC's benzene EXC_GRAD (with grid-weight derivatives) and
+    EXC_GRAD_HELLFEY fixtures; translational invariance; dEXC/dR by central
+    differences (RKS and UKS); UKS with P_z = 0 equals RKS; GauXC's doublet
+    cytosine UKS fixture (EXC, VXC_s, VXC_z, both gradients);
+  * pruned grids: GauXC's benzene Robust and Treutler fixtures (point
+    counts, EXC, VXC, FXC, both gradients).
 """
 import ctypes
 
@@ -487,6 +490,75 @@ def test_oracle_exc_grad_matches_gauxc_benzene_fixture(orc, tag, func):
     assert np.linalg.norm(g - ref) / np.sqrt(g.size) < 1e-12
     # Translational invariance: the rows sum to zero.
     assert np.abs(g.sum(axis=0)).max() < 1e-12
+    # The Hellmann-Feynman form (EXC_GRAD_HELLFEY, tests/xc_integrator.cxx:291-300).
+    gh = orc.exc_grad(func, bs.nbf(), orc.shells_from(bs.xcq_shells()), d["xyz"], d["density"],
+                      t, weights=w, weight_derivs=False)
+    assert np.linalg.norm(gh - d["exc_grad_hf"].reshape(-1, 3)) / np.sqrt(g.size) < 1e-11
+
+
+@pytest.mark.parametrize("scheme", ["Robust", "Treutler"])
+def test_oracle_pruned_grid_matches_gauxc_benzene_fixture(gx, orc, scheme):
+    """Benzene / SVWN5 on the pruned ultra-fine grids (tests/xc_integrator.cxx:
+    410-420, 2nd_derivative_test.cxx:198-208): the point count follows the
+    scheme's regions (C and H: 99 radial nodes, base rule 590 of order 41;
+    Robust: 25 nodes on 26 points, 25 on 434 (order 35), 49 on 590;
+    Treutler: 34 on 26, 16 on 50, 49 on 590), and EXC, VXC, FXC and both
+    gradients match GauXC's values."""
+    from conftest import unpack_sym
+    d, mol, bs, _ = benzene_setup("svwn5")
+    pr = load_golden("benzene_svwn5_pruned.npz")
+    key = scheme.lower()
+    mg = gx.MolGridFactory.create_default_molgrid(mol, scheme, 512, "MuraKnowles",
+                                                  "UltraFineGrid")
+    lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
+                                                                 mg, bs)
+    t = lb.get_tasks()
+    per_atom = 25 * 26 + 25 * 434 + 49 * 590 if scheme == "Robust" else 34 * 26 + 16 * 50 + 49 * 590
+    assert int(np.sum(t["npts"])) == 12 * per_atom
+    n = bs.nbf()
+    w = orc.ssf_weights(d["xyz"], t)
+    sh = orc.shells_from(bs.xcq_shells())
+    exc, nel, vxc = orc.exc_vxc("SVWN5", n, sh, d["density"], t, weights=w)
+    assert abs(exc - pr[f"{key}_exc"][0]) <= 1e-12 * abs(exc)
+    assert np.abs(vxc - unpack_sym(pr[f"{key}_vxc_packed"], n)).max() < 1e-11
+    fxc = orc.fxc_contraction("SVWN5", n, sh, d["density"], d["trial_density"], t, weights=w)[-1]
+    assert np.abs(fxc - unpack_sym(pr[f"{key}_fxc_packed"], n)).max() < 1e-11
+    for wd, ref in ((True, pr[f"{key}_exc_grad"]), (False, pr[f"{key}_exc_grad_hf"])):
+        g = orc.exc_grad("SVWN5", n, sh, d["xyz"], d["density"], t, weights=w, weight_derivs=wd)
+        assert np.abs(g.ravel() - ref).max() < 1e-11, wd
+
+
+def test_oracle_uks_matches_gauxc_cytosine_fixture(orc):
+    """Doublet cytosine / SVWN5 / cc-pVDZ, Robust-pruned ultra-fine grid
+    (tests/xc_integrator.cxx:218-237,278-300,455-459).  The full gradient
+    (EXC_GRAD, with weight derivatives) matches to 1e-12.  EXC, VXC_s and
+    the Hellmann-Feynman gradient match at GauXC's own bars (EXC relative
+    Catch epsilon, ||dVXC||_F / nbf < 1e-10, ||dG|| / sqrt(3 natoms) < 1e-8)
+    but not tighter: measured 2.6e-11 relative, 6.5e-12 and 2.8e-9, with
+    VXC_z at 1e-14.  The same restatement reproduces every benzene fixture
+    (unpruned, Robust, Treutler; EXC, VXC, FXC, both gradients) at 1e-12 and
+    ties UKS to RKS at 1e-11, and the fixture's own full gradient agrees at
+    1e-13, so the residual is taken to be in how those three datasets were
+    produced; the bounds below are GauXC's, with the measured values noted."""
+    from conftest import cytosine_uks_setup
+    d, mol, bs, lb = cytosine_uks_setup()
+    t = lb.get_tasks()
+    xyz = np.asarray(d["xyz"], dtype=np.float64)
+    n = bs.nbf()
+    w = orc.ssf_weights(xyz, t)
+    sh = orc.shells_from(bs.xcq_shells())
+    exc, nel, vs, vz = orc.exc_vxc_uks("SVWN5", n, sh, d["density_s"], d["density_z"], t,
+                                       weights=w)[:4]
+    assert abs(exc - d["exc"][0]) <= 1e-10 * abs(exc)          # measured 2.6e-11
+    assert np.linalg.norm(vs - d["vxc_s"]) / n < 1e-10         # measured 6.5e-12
+    assert np.linalg.norm(vz - d["vxc_z"]) / n < 1e-13         # measured 1.0e-14
+    assert abs(nel - 57.0) < 1e-4                               # 29 + 28 electrons
+    g = orc.exc_grad_uks("SVWN5", n, sh, xyz, d["density_s"], d["density_z"], t, weights=w)
+    assert np.abs(g.ravel() - d["exc_grad"]).max() < 1e-12     # measured 1.3e-13
+    assert np.abs(g.sum(axis=0)).max() < 1e-12
+    gh = orc.exc_grad_uks("SVWN5", n, sh, xyz, d["density_s"], d["density_z"], t, weights=w,
+                          weight_derivs=False)
+    assert np.linalg.norm(gh.ravel() - d["exc_grad_hf"]) / np.sqrt(gh.size) < 1e-8
 
 
 def _dimer_system(gx, orc, xyz, Z):
