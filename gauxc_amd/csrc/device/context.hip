@@ -205,6 +205,8 @@ struct xcq_context_s {
   int32_t grad_natoms = 0;
   double* d_gxyz = nullptr;
   double* d_dnear = nullptr;
+  int32_t* d_gnbr_idx = nullptr;  // neighbour lists of the gradient's weight part
+  double* d_gnbr_r = nullptr;
   double* d_grad = nullptr;
   int32_t* d_shell_atom = nullptr;
   bool last_grad = false;   // the last evaluation was an EXC gradient
@@ -289,6 +291,8 @@ struct xcq_context_s {
     dfree(d_tPz);
     dfree(d_gxyz);
     dfree(d_dnear);
+    dfree(d_gnbr_idx);
+    dfree(d_gnbr_r);
     dfree(d_grad);
     dfree(d_shell_atom);
     dfree(d_out2);
@@ -1303,6 +1307,46 @@ int xcq_exc_vxc(xcq_context ctx, int functional, int precision, int want_vxc) {
 // ---- UKS (GauXC's eval_exc_vxc(Ps, Pz)) --------------------------------------
 namespace {
 
+// Largest molecule whose neighbour lists (12 bytes per atom pair) the SSF
+// launches build; above it they loop over every atom.
+constexpr int32_t kSsfNbrMaxAtoms = 8192;
+
+// Per atom A, every atom in order of increasing R_AB with A itself first
+// (row-major natoms x natoms), for k_ssf's bounded loops.
+void ssf_neighbours(int32_t natoms, const std::vector<double>& rab, std::vector<int32_t>& idx,
+                    std::vector<double>& r) {
+  idx.resize((size_t)natoms * natoms);
+  r.resize(idx.size());
+  std::vector<int32_t> ord(natoms);
+  for (int32_t A = 0; A < natoms; ++A) {
+    const double* col = rab.data() + (size_t)A * natoms;  // R_AB = rab[B + A * natoms]
+    std::iota(ord.begin(), ord.end(), 0);
+    std::swap(ord[0], ord[A]);
+    std::stable_sort(ord.begin() + 1, ord.end(),
+                     [&](int32_t u, int32_t v) { return col[u] < col[v]; });
+    for (int32_t k = 0; k < natoms; ++k) {
+      idx[(size_t)A * natoms + k] = ord[k];
+      r[(size_t)A * natoms + k] = k == 0 ? 0.0 : col[ord[k]];
+    }
+  }
+}
+
+// R_AB (column-major) and nearest-neighbour distances of MolMeta.
+void ssf_molmeta(int32_t natoms, const double* atom_xyz, std::vector<double>& rab,
+                 std::vector<double>& dn) {
+  rab.assign((size_t)natoms * natoms, 0.);
+  dn.assign(natoms, INFINITY);
+  for (int i = 0; i < natoms; ++i)
+    for (int j = 0; j < natoms; ++j) {
+      const double dx = atom_xyz[3 * i] - atom_xyz[3 * j];
+      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1];
+      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
+      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (i != j) dn[i] = std::min(dn[i], rab[(size_t)i + (size_t)j * natoms]);
+    }
+}
+
+
 // The per-point buffers of the UKS, FXC and gradient paths (freed with the
 // task set, whose point-block count sizes them).
 int ensure_point_buffers(xcq_context_s* ctx) {
@@ -1699,7 +1743,8 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
                                    ctx->d_grad)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
-                                               ctx->d_dnear, wf, ctx->d_grad)))
+                                               ctx->d_dnear, wf, ctx->d_grad,
+                                               ctx->d_gnbr_idx, ctx->d_gnbr_r)))
       return rc;
   }
   return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
@@ -1751,7 +1796,8 @@ int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
                                        ctx->d_grad)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
-                                               ctx->d_dnear, wf, ctx->d_grad)))
+                                               ctx->d_dnear, wf, ctx->d_grad,
+                                               ctx->d_gnbr_idx, ctx->d_gnbr_r)))
       return rc;
   }
   return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
@@ -1783,25 +1829,29 @@ int grad_prepare(xcq_context_s* ctx, int functional, int32_t natoms, const doubl
       if (xyz[3 * a] == ctx->sh_origin[3 * i] && xyz[3 * a + 1] == ctx->sh_origin[3 * i + 1] &&
           xyz[3 * a + 2] == ctx->sh_origin[3 * i + 2])
         sa[i] = a;
-  // Nearest-neighbour distance of each atom (MolMeta::dist_nearest); a
-  // single atom has none (its points all keep their weights).
-  std::vector<double> dn(natoms, std::numeric_limits<double>::infinity());
-  for (int32_t a = 0; a < natoms; ++a)
-    for (int32_t b = 0; b < natoms; ++b)
-      if (a != b) {
-        const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
-                     dz = xyz[3 * a + 2] - xyz[3 * b + 2];
-        dn[a] = std::min(dn[a], std::sqrt(dx * dx + dy * dy + dz * dz));
-      }
+  // R_AB and the nearest-neighbour distance of each atom
+  // (MolMeta::dist_nearest; a single atom has none, its points all keep
+  // their weights), and the neighbour lists of the weight part's bounded
+  // loops (up to kSsfNbrMaxAtoms atoms).
+  std::vector<double> rab, dn, hnr;
+  std::vector<int32_t> hni;
+  ssf_molmeta(natoms, xyz, rab, dn);
+  const bool lists = natoms <= kSsfNbrMaxAtoms;
+  if (lists) ssf_neighbours(natoms, rab, hni, hnr);
   int rc;
   if (natoms != ctx->grad_natoms) {
     dfree(ctx->d_gxyz);
     dfree(ctx->d_dnear);
     dfree(ctx->d_grad);
+    dfree(ctx->d_gnbr_idx);
+    dfree(ctx->d_gnbr_r);
     ctx->grad_natoms = 0;
     if ((rc = dalloc(&ctx->d_gxyz, 3 * (size_t)natoms)) ||
         (rc = dalloc(&ctx->d_dnear, (size_t)natoms)) ||
         (rc = dalloc(&ctx->d_grad, 3 * (size_t)natoms)))
+      return rc;
+    if (lists && ((rc = dalloc(&ctx->d_gnbr_idx, hni.size())) ||
+                  (rc = dalloc(&ctx->d_gnbr_r, hnr.size()))))
       return rc;
     ctx->grad_natoms = natoms;
   }
@@ -1810,6 +1860,10 @@ int grad_prepare(xcq_context_s* ctx, int functional, int32_t natoms, const doubl
   XCQ_HIP(hipMemcpy(ctx->d_gxyz, xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));
   XCQ_HIP(hipMemcpy(ctx->d_dnear, dn.data(), 8 * (size_t)natoms, hipMemcpyHostToDevice));
   XCQ_HIP(hipMemcpy(ctx->d_shell_atom, sa.data(), 4 * nsh, hipMemcpyHostToDevice));
+  if (lists) {
+    XCQ_HIP(hipMemcpy(ctx->d_gnbr_idx, hni.data(), 4 * hni.size(), hipMemcpyHostToDevice));
+    XCQ_HIP(hipMemcpy(ctx->d_gnbr_r, hnr.data(), 8 * hnr.size(), hipMemcpyHostToDevice));
+  }
   if ((rc = ensure_point_buffers(ctx))) return rc;
   ctx->last_want_vxc = 0;
   ctx->last_vxc_eval = 0;
@@ -1935,45 +1989,6 @@ int xcq_work_counters(xcq_context ctx, int64_t* out, int n) {
 }
 
 namespace {
-
-// Largest molecule whose neighbour lists (12 bytes per atom pair) the SSF
-// launches build; above it they loop over every atom.
-constexpr int32_t kSsfNbrMaxAtoms = 8192;
-
-// Per atom A, every atom in order of increasing R_AB with A itself first
-// (row-major natoms x natoms), for k_ssf's bounded loops.
-void ssf_neighbours(int32_t natoms, const std::vector<double>& rab, std::vector<int32_t>& idx,
-                    std::vector<double>& r) {
-  idx.resize((size_t)natoms * natoms);
-  r.resize(idx.size());
-  std::vector<int32_t> ord(natoms);
-  for (int32_t A = 0; A < natoms; ++A) {
-    const double* col = rab.data() + (size_t)A * natoms;  // R_AB = rab[B + A * natoms]
-    std::iota(ord.begin(), ord.end(), 0);
-    std::swap(ord[0], ord[A]);
-    std::stable_sort(ord.begin() + 1, ord.end(),
-                     [&](int32_t u, int32_t v) { return col[u] < col[v]; });
-    for (int32_t k = 0; k < natoms; ++k) {
-      idx[(size_t)A * natoms + k] = ord[k];
-      r[(size_t)A * natoms + k] = k == 0 ? 0.0 : col[ord[k]];
-    }
-  }
-}
-
-// R_AB (column-major) and nearest-neighbour distances of MolMeta.
-void ssf_molmeta(int32_t natoms, const double* atom_xyz, std::vector<double>& rab,
-                 std::vector<double>& dn) {
-  rab.assign((size_t)natoms * natoms, 0.);
-  dn.assign(natoms, INFINITY);
-  for (int i = 0; i < natoms; ++i)
-    for (int j = 0; j < natoms; ++j) {
-      const double dx = atom_xyz[3 * i] - atom_xyz[3 * j];
-      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1];
-      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
-      rab[(size_t)i + (size_t)j * natoms] = std::sqrt(dx * dx + dy * dy + dz * dz);
-      if (i != j) dn[i] = std::min(dn[i], rab[(size_t)i + (size_t)j * natoms]);
-    }
-}
 
 // Device copies of the SSF geometry: atom coordinates, R_AB, nearest
 // distances and (up to kSsfNbrMaxAtoms atoms) the neighbour lists.

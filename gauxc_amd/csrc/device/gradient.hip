@@ -648,7 +648,15 @@ __global__ __launch_bounds__(256) void k_grad_contract_uks(
 //   t(mu) = 35 (1 + v)^3 / ((mu - a)(5 v^3 + 20 v^2 + 29 v + 16)),  v = mu / a,
 // which stays accurate up to the |mu| < a - 1e-4 bound the contraction uses.
 // One thread per point (lane of a 64-point block); distances are recomputed
-// where needed, O(natoms^2) per point.
+// where needed.  Without neighbour lists every loop runs over all atoms,
+// O(natoms^2) per point.  With them (nbr_idx / nbr_r: row A holds every atom
+// in order of increasing R_AB, A itself first) the loops stop where the
+// triangle inequality makes every remaining term vanish, as in the weights
+// kernel: a factor s(mu_BC) differs from 1, and |mu_BC| can fall below a,
+// only for R_BC < 2 d_B / (1 - a); P_B > 0 only for d_B <= (1 + a)/(1 - a)
+// d_min; so the parent's list is read up to max(2 d_A / (1 - a),
+// d_A + that bound) and each B's list up to 2 d_B / (1 - a), all widened by
+// a relative 1e-10.  The terms skipped are exactly zero (factors exactly 1).
 constexpr double kSsfA = 0.64;
 constexpr double kSsfTol = 1e-13;
 
@@ -673,7 +681,9 @@ __global__ __launch_bounds__(64) void k_ssf_grad(const WorkItem* __restrict__ it
                                                  const double* __restrict__ xyz,
                                                  const double* __restrict__ dnear,
                                                  const double* __restrict__ wf,
-                                                 double* __restrict__ grad) {
+                                                 double* __restrict__ grad,
+                                                 const int32_t* __restrict__ nbr_idx,
+                                                 const double* __restrict__ nbr_r) {
   const WorkItem wi = items[blockIdx.x];
   const DevTask t = tasks[wi.task];
   const int lane = threadIdx.x;
@@ -694,6 +704,80 @@ __global__ __launch_bounds__(64) void k_ssf_grad(const WorkItem* __restrict__ it
   const int A = t.iparent;
   const double dA = dist(A);
   if (dA < 0.5 * (1. - kSsfA) * dnear[A]) return;
+  const double bound = kSsfA - 1e-4;
+  if (nbr_idx) {
+    constexpr double kWiden = 1.0 + 1e-10;
+    const int32_t* an = nbr_idx + (size_t)A * natoms;
+    const double* ar = nbr_r + (size_t)A * natoms;
+    // Nearest atom (R_AB* <= 2 d_A) and the candidate bound.
+    double dmin = dA;
+    for (int k = 1; k < natoms && ar[k] <= 2.0 * dA * kWiden; ++k) dmin = fmin(dmin, dist(an[k]));
+    const double cbound = (1.0 + kSsfA) / (1.0 - kSsfA) * dmin * kWiden + 1e-12;
+    const double rmax = fmax(2.0 * dA / (1.0 - kSsfA), dA + cbound) * kWiden;
+    // P_B over B's list (factors beyond 2 d_B / (1 - a) are exactly 1).
+    auto part_l = [&](int B, double dB) {
+      if (dB > cbound) return 0.;
+      const int32_t* bn = nbr_idx + (size_t)B * natoms;
+      const double* br = nbr_r + (size_t)B * natoms;
+      const double rf = 2.0 * dB / (1.0 - kSsfA) * kWiden;
+      double v = 1.;
+      for (int q = 1; q < natoms && br[q] < rf && v > 0.; ++q)
+        v *= ssf_s((dB - dist(bn[q])) / br[q]);
+      return v;
+    };
+    double sum = 0.;
+    for (int k = 0; k < natoms && ar[k] <= rmax; ++k) {
+      const int B = an[k];
+      sum += part_l(B, k == 0 ? dA : dist(B));
+    }
+    const double pA = part_l(A, dA);
+    double gA[3] = {0., 0., 0.};
+    for (int k = 1; k < natoms && ar[k] <= rmax; ++k) {
+      const int B = an[k];
+      const double dB = dist(B);
+      double gb[3] = {0., 0., 0.};
+      const double rAB = ar[k];
+      const double muAB = (dA - dB) / rAB;
+      if (fabs(muAB) < bound) {
+        const double c1 = ssf_t(muAB) / rAB * (pA - sum) / sum * f / dB;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          gb[d] = c1 * ((X(B, d) - r[d]) + muAB * (X(B, d) - X(A, d)) / rAB * dB);
+      }
+      const double pB = part_l(B, dB);
+      if (pB > kSsfTol) {
+        const int32_t* bn = nbr_idx + (size_t)B * natoms;
+        const double* br = nbr_r + (size_t)B * natoms;
+        const double rf = 2.0 * dB / (1.0 - kSsfA) * kWiden;
+        for (int q = 1; q < natoms && br[q] < rf; ++q) {
+          const int C = bn[q];
+          const double rBC = br[q], dC = dist(C);
+          const double muBC = (dB - dC) / rBC;
+          if (fabs(muBC) >= bound) continue;
+          const double c = pB * ssf_t(muBC) / rBC / sum * f;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            gb[d] -= c * ((X(B, d) - r[d]) / dB - muBC * (X(B, d) - X(C, d)) / rBC);
+          if (C != A) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const double gc = c * ((X(C, d) - r[d]) / dC + muBC * (X(C, d) - X(B, d)) / rBC);
+              atomicAdd(grad + 3 * C + d, gc);
+              gA[d] -= gc;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (gb[d] != 0.) atomicAdd(grad + 3 * B + d, gb[d]);
+        gA[d] -= gb[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * A + d, gA[d]);
+    return;
+  }
   auto part = [&](int B, double dB) {
     double v = 1.;
     for (int C = 0; C < natoms && v > 0.; ++C)
@@ -703,7 +787,6 @@ __global__ __launch_bounds__(64) void k_ssf_grad(const WorkItem* __restrict__ it
   double sum = 0.;
   for (int B = 0; B < natoms; ++B) sum += part(B, B == A ? dA : dist(B));
   const double pA = part(A, dA);
-  const double bound = kSsfA - 1e-4;
   double gA[3] = {0., 0., 0.};
   for (int B = 0; B < natoms; ++B) {
     if (B == A) continue;
@@ -821,10 +904,11 @@ int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
 }
 
 int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
-                    const double* dnear, const double* wf, double* grad) {
+                    const double* dnear, const double* wf, double* grad,
+                    const int32_t* nbr_idx, const double* nbr_r) {
   if (r.n == 0) return XCQ_SUCCESS;
   hipLaunchKernelGGL(k_ssf_grad, dim3((unsigned)r.n), dim3(64), 0, v.stream, v.work + r.off,
-                     v.tasks, v.px, v.py, v.pz, natoms, xyz, dnear, wf, grad);
+                     v.tasks, v.px, v.py, v.pz, natoms, xyz, dnear, wf, grad, nbr_idx, nbr_r);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

@@ -291,7 +291,8 @@ int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
                              const int32_t* shell_atom, const double* coef_s,
                              const double* coef_z, int weight_derivs, double* grad);
 int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
-                    const double* dnear, const double* wf, double* grad);
+                    const double* dnear, const double* wf, double* grad,
+                    const int32_t* nbr_idx = nullptr, const double* nbr_r = nullptr);
 int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const double* coef);
 // B^T Z + Z^T B with the VXC scatter-add, one work range of `nt`-wide tiles.
 int launch_syr2k(const CtxView& v, const WorkRange& r, int nt, int precision);
