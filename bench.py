@@ -165,6 +165,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--derivatives", action="store_true",
+                    help="also time the FP64 EXC gradient (with and without the grid-weight "
+                         "derivatives) and the RKS FXC contraction on the same workload")
     ap.add_argument("--tile-policy", default=None,
                     help="D,S,Q[,H[,W[,M]]]: fixed density / SYR2K 32-wide thresholds, "
                          "queue count, (mixed) density high-occupancy threshold, 128-wide "
@@ -271,6 +274,7 @@ def main():
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
             f"stages(ms)={np.round(stage, 3).tolist()}")
     ctx.use_graph(False)
+    derivs = time_derivatives(ctx, cfg, args.steps) if args.derivatives else None
     learned = None
     if args.learned:
         learned = learned_schedule(ctx, bs, ts)
@@ -339,7 +343,36 @@ def main():
     }
     if learned is not None:
         line["learned_schedule"] = learned
+    if derivs is not None:
+        line["derivatives"] = derivs
     print(json.dumps(line), flush=True)
+
+
+def time_derivatives(ctx, cfg, steps):
+    """Wall ms per call (device work, results downloaded) of the FP64 EXC
+    gradient with and without the grid-weight derivatives and of the RKS FXC
+    contraction, on the uploaded workload (PBE)."""
+    from gauxc_amd import workloads
+    xyz = np.asarray(cfg["xyz"], dtype=np.float64)
+    out = {}
+    for key, wd in (("exc_grad_ms", True), ("exc_grad_hf_ms", False)):
+        ctx.exc_grad("PBE", xyz, weight_derivs=wd)
+        ctx.results_grad()
+        tic = time.perf_counter()
+        for _ in range(steps):
+            ctx.exc_grad("PBE", xyz, weight_derivs=wd)
+            ctx.results_grad()
+        out[key] = (time.perf_counter() - tic) * 1e3 / steps
+    ctx.upload_trial_density(workloads.density(cfg["nbf"], 8, 7))
+    ctx.fxc_contraction("PBE")
+    ctx.results_fxc()
+    tic = time.perf_counter()
+    for _ in range(steps):
+        ctx.fxc_contraction("PBE")
+        ctx.results_fxc()
+    out["fxc_contraction_ms"] = (time.perf_counter() - tic) * 1e3 / steps
+    log(f"derivatives (FP64, ms per call): {out}")
+    return out
 
 
 def time_eval(ctx, prec, reps=3):

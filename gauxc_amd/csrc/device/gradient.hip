@@ -202,22 +202,31 @@ __global__ __launch_bounds__(256) void k_grad_zpair(const WorkItem* __restrict__
 
 // ---- plane products ----------------------------------------------------------
 // out[f][p] = fac sum_g P_sub[f][g] in[g][p] (+ out[f][p] with accumulate)
-// for one (task, 64-point block),
-// every 64-function tile of the task: 256 threads, each a 4 x 4 patch
-// (functions 4 ty.., points 4 tx..), K steps of 16 staged in LDS.  P_sub is
-// gathered through the basis map (symmetric, so the storage order of P does
-// not matter).  Rows past nbe are neither read nor written.
+// for one (task, 64-point block), every 64-function tile of the task, on the
+// FP64 matrix cores (v_mfma_f64_16x16x4f64; lane l holds A[l%16][l/16] and
+// B[l/16][l%16], accumulator register r of lane l is D[4r + l/16][l%16]).
+// 256 threads = four waves in a 2 x 2 arrangement, each a 32 x 32 sub-tile
+// (2 x 2 MFMA blocks); K steps of 16 staged in LDS.  P_sub is gathered
+// through the basis map (symmetric, so the storage order of P does not
+// matter).  Rows past nbe are neither read nor written.  The LDS stage is
+// double-buffered: the next K step's operands are loaded into registers
+// before the current step's matrix instructions and stored after them, one
+// barrier per step.
+using v4d = double __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_plane_product(const WorkItem* __restrict__ items,
                                                        const DevTask* __restrict__ tasks,
                                                        const int32_t* __restrict__ bfmap,
                                                        const double* __restrict__ P,
                                                        double* __restrict__ scratch, int in_pl,
                                                        int out_pl, double fac, int accumulate) {
-  __shared__ double Ps[16][65];
-  __shared__ double Is[16][64];
+  __shared__ double Ps[2][16][68];  // [buffer][g][f]
+  __shared__ double Is[2][16][68];  // [buffer][g][p]
   const WorkItem wi = items[blockIdx.x];
   const DevTask t = tasks[wi.task];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
   const int64_t m0 = (int64_t)wi.a * kPtBlock;
@@ -225,46 +234,71 @@ __global__ __launch_bounds__(256) void k_plane_product(const WorkItem* __restric
   double* out = scratch + t.b_off + out_pl * plane + m0;
   const int32_t* bf = bfmap + t.bf_off;
   const double* Pm = P + t.p_off;
+  const int lr = lane & 15, lk = lane >> 4;
+  // Thread tid moves elements (k, i) = ((tid + 256 q) >> 6, (tid + 256 q) & 63).
+  const int ci = tid & 63, ck = tid >> 6;
   for (int f0 = 0; f0 < t.nbe; f0 += 64) {
-    double acc[4][4];
+    v4d acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = 0.;
-    for (int g0 = 0; g0 < t.nbe; g0 += 16) {
-      __syncthreads();  // the previous step's reads of Ps / Is are done
+      for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0., 0., 0., 0.};
+    const bool fok = f0 + ci < t.nbe;
+    const int64_t pcol = fok ? bf[f0 + ci] : 0;
+    double rp[4], ri[4];
+    auto fetch = [&](int g0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int idx = tid + 256 * q, i = idx & 63, k = idx >> 6;
-        const int f = f0 + i, g = g0 + k;
-        Ps[k][i] = (f < t.nbe && g < t.nbe) ? Pm[(int64_t)bf[f] + (int64_t)bf[g] * t.ldp] : 0.;
-        Is[k][i] = g < t.nbe ? in[(int64_t)g * ld + i] : 0.;
+        const int g = g0 + ck + 4 * q;
+        rp[q] = (fok && g < t.nbe) ? Pm[pcol + (int64_t)bf[g] * t.ldp] : 0.;
+        ri[q] = g < t.nbe ? in[(int64_t)g * ld + ci] : 0.;
       }
-      __syncthreads();
+    };
+    auto put = [&](int buf) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        double pv[4], iv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) pv[i] = Ps[k][4 * ty + i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) iv[j] = Is[k][4 * tx + j];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = fma(pv[i], iv[j], acc[i][j]);
+      for (int q = 0; q < 4; ++q) {
+        Ps[buf][ck + 4 * q][ci] = rp[q];
+        Is[buf][ck + 4 * q][ci] = ri[q];
       }
+    };
+    __syncthreads();  // the previous tile's reads of both buffers are done
+    fetch(0);
+    put(0);
+    int buf = 0;
+    for (int g0 = 0; g0 < t.nbe; g0 += 16, buf ^= 1) {
+      __syncthreads();  // stage buf is written; stage buf ^ 1 is no longer read
+      const bool more = g0 + 16 < t.nbe;
+      if (more) fetch(g0 + 16);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = 4 * kk + lk;
+        double a[2], b[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          a[q] = Ps[buf][k][wm * 32 + q * 16 + lr];
+          b[q] = Is[buf][k][wn * 32 + q * 16 + lr];
+        }
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+      }
+      if (more) put(buf ^ 1);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int f = f0 + 4 * ty + i;
-      if (f < t.nbe) {
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          double* o = out + (int64_t)f * ld + 4 * tx + j;
-          *o = accumulate ? fma(fac, acc[i][j], *o) : fac * acc[i][j];
+      for (int r = 0; r < 4; ++r) {
+        const int f = f0 + wm * 32 + bi * 16 + 4 * r + lk;
+        if (f < t.nbe) {
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj) {
+            double* o = out + (int64_t)f * ld + wn * 32 + bj * 16 + lr;
+            *o = accumulate ? fma(fac, acc[bi][bj][r], *o) : fac * acc[bi][bj][r];
+          }
         }
       }
-    }
   }
 }
 

@@ -41,13 +41,15 @@ __device__ __forceinline__ double ssf_s(double mu) {
 // Each limit is widened by a relative 1e-10, far above rounding, so the
 // result differs from the all-atom loops only in the order of the product's
 // factors.  Without lists (nbr_idx null) the loops run over all atoms.
-// The parent's product comes first (row P of the lists starts with P): when
-// it is exactly 0 the weight is w * 0 whatever the sum (the nearest atom's
-// product is positive, so the sum is), and the point is done.  Every other
-// candidate's product stops once it falls below 2^-70 times the running sum
-// and is then left out: factors lie in [0, 1], so each omitted term is below
-// 2^-70 of the final sum and all of them together (at most 8,192) below
-// 2^-57, under the rounding of the sum itself.
+// The parent's product comes first: when it is exactly 0 the weight is
+// w * 0 whatever the sum (the nearest atom's product is positive, so the
+// sum is), and the point is done.  The nearest atom's product (usually the
+// largest) comes next.  Every other candidate's product stops once it falls
+// below 2^-70 times the running sum and is then left out, and a candidate
+// whose factor against the nearest atom is already below that is not
+// formed: factors lie in [0, 1], so each omitted term is below 2^-70 of the
+// final sum and all of them together (at most 8,192) below 2^-57, under the
+// rounding of the sum itself.
 __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __restrict__ xyz,
                                              const double* __restrict__ rab,
                                              const double* __restrict__ dnear, int64_t npts,
@@ -93,36 +95,31 @@ __global__ __launch_bounds__(256) void k_ssf(int32_t natoms, const double* __res
     const double bound = (1.0 + a) / (1.0 - a) * dmin * kWiden + 1e-12;
     const double rcand = (dpar + bound) * kWiden;
     constexpr double kNegligible = 0x1p-70;
-    double sum = 0.0, ppar = 0.0;
-    for (int k = 0; k < natoms && nr[k] <= rcand; ++k) {
-      const int A = np[k];
-      const bool is_par = k == 0;  // np[0] == par
-      const double dA = is_par ? dpar : dist(A);
-      if (dA > bound || (A != inear && ssf_s((dA - dmin) /
-                                             rab[(size_t)A + (size_t)inear * natoms]) == 0.0)) {
-        if (is_par) {  // the parent's product is 0
-          w[ip] *= 0.0;
-          return;
-        }
-        continue;
-      }
+    // Product of candidate A, or 0 where it is 0 or below `floor_`.
+    auto product = [&](int A, double dA, double floor_) {
+      if (dA > bound) return 0.0;
+      if (A != inear &&
+          ssf_s((dA - dmin) / rab[(size_t)A + (size_t)inear * natoms]) <= floor_)
+        return 0.0;
       const int32_t* ap = nbr_idx + (size_t)A * natoms;
       const double* ar = nbr_r + (size_t)A * natoms;
       const double rfac = 2.0 * dA / (1.0 - a) * kWiden;
-      const double floor_ = is_par ? 0.0 : kNegligible * sum;
       double PA = 1.0;
       for (int q = 1; q < natoms && ar[q] < rfac && PA > floor_; ++q)
         PA *= ssf_s((dA - dist(ap[q])) / ar[q]);
-      if (is_par) {
-        if (PA == 0.0) {
-          w[ip] *= 0.0;
-          return;
-        }
-        ppar = PA;
-      } else if (PA <= floor_) {
-        continue;
-      }
-      sum += PA;
+      return PA > floor_ ? PA : 0.0;
+    };
+    const double ppar = product(par, dpar, 0.0);
+    if (ppar == 0.0) {
+      w[ip] *= 0.0;
+      return;
+    }
+    double sum = ppar;
+    if (inear != par) sum += product(inear, dmin, kNegligible * sum);
+    for (int k = 1; k < natoms && nr[k] <= rcand; ++k) {
+      const int A = np[k];
+      if (A == inear) continue;
+      sum += product(A, dist(A), kNegligible * sum);
     }
     w[ip] *= ppar / sum;
     return;
