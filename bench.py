@@ -274,7 +274,11 @@ def main():
         log(f"{prec}: {ms:.2f} ms/step, EXC={exc:.10f} NEL={nel:.6f} "
             f"stages(ms)={np.round(stage, 3).tolist()}")
     ctx.use_graph(False)
-    derivs = time_derivatives(ctx, cfg, args.steps) if args.derivatives else None
+    derivs = None
+    if args.derivatives:
+        derivs = time_derivatives(ctx, cfg, args.steps)
+        if not args.no_cpu:
+            derivs["cpu"] = cpu_derivatives(bs, P, ts, cfg, args.cpu_seconds / 3)
     learned = None
     if args.learned:
         learned = learned_schedule(ctx, bs, ts)
@@ -348,11 +352,42 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def cpu_derivatives(bs, P, ts, cfg, budget_s):
+    """The oracle's EXC gradient (with weight derivatives) and RKS FXC
+    contraction on bounded samples of the same tasks (points/s; its X
+    products through the host BLAS when one loads)."""
+    import oracle
+    nthreads = cpu_threads()
+    blas = oracle.use_blas("auto")
+    sh = oracle.shells_from(bs.xcq_shells())
+    xyz = np.asarray(cfg["xyz"], dtype=np.float64)
+    tP = workloads_density(cfg["nbf"])
+    out = {"cores": nthreads, "blas": blas or "none"}
+    for key, fn in (("exc_grad_pts_per_s",
+                     lambda sub: oracle.exc_grad("PBE", cfg["nbf"], sh, xyz, P, sub,
+                                                 nthreads=nthreads)),
+                    ("fxc_contraction_pts_per_s",
+                     lambda sub: oracle.fxc_contraction("PBE", cfg["nbf"], sh, P, tP, sub,
+                                                        nthreads=nthreads))):
+        idx, pts, elapsed = sample_rate(ts, fn, budget_s, nthreads)
+        out[key] = pts / elapsed
+        out[key.replace("_pts_per_s", "_sample")] = (f"{len(idx)} tasks, {pts} points, "
+                                                     f"{elapsed:.2f} s")
+    oracle.use_blas(None)
+    log(f"cpu derivatives: {out}")
+    return out
+
+
+def workloads_density(nbf):
+    """The trial density the derivative timings use."""
+    from gauxc_amd import workloads
+    return workloads.density(nbf, 8, 7)
+
+
 def time_derivatives(ctx, cfg, steps):
     """Wall ms per call (device work, results downloaded) of the FP64 EXC
     gradient with and without the grid-weight derivatives and of the RKS FXC
     contraction, on the uploaded workload (PBE)."""
-    from gauxc_amd import workloads
     xyz = np.asarray(cfg["xyz"], dtype=np.float64)
     out = {}
     for key, wd in (("exc_grad_ms", True), ("exc_grad_hf_ms", False)):
@@ -363,7 +398,7 @@ def time_derivatives(ctx, cfg, steps):
             ctx.exc_grad("PBE", xyz, weight_derivs=wd)
             ctx.results_grad()
         out[key] = (time.perf_counter() - tic) * 1e3 / steps
-    ctx.upload_trial_density(workloads.density(cfg["nbf"], 8, 7))
+    ctx.upload_trial_density(workloads_density(cfg["nbf"]))
     ctx.fxc_contraction("PBE")
     ctx.results_fxc()
     tic = time.perf_counter()
@@ -598,6 +633,27 @@ def cpu_threads():
     return max(1, n)
 
 
+def sample_rate(ts, fn, budget_s, nthreads):
+    """fn on evenly spaced tasks of ts, the sample grown until one run takes
+    at least a third of budget_s (the time scales with the sample): (task
+    indices, points, seconds) of the last run."""
+    ntask = len(ts["npts"])
+
+    def run(count):
+        idx = np.unique(np.linspace(0, ntask - 1, max(1, min(count, ntask))).astype(np.int64))
+        sub = reorder_tasks(ts, idx)
+        tic = time.perf_counter()
+        fn(sub)
+        return idx, int(sub["npts"].sum()), time.perf_counter() - tic
+
+    run(nthreads)  # first-call costs (library initialisation, thread pool)
+    idx, pts, elapsed = run(4 * nthreads)
+    while elapsed < budget_s / 3 and len(idx) < ntask:
+        grow = min(20.0, max(2.0, budget_s / max(elapsed, 1e-3)))
+        idx, pts, elapsed = run(int(len(idx) * grow))
+    return idx, pts, elapsed
+
+
 def cpu_baseline(bs, P, ts, cfg, budget_s):
     """Oracle (restated GauXC host integrator) on a bounded sample of tasks:
     OpenMP over tasks as GauXC's host integrator, the per-task DGEMM and
@@ -606,22 +662,10 @@ def cpu_baseline(bs, P, ts, cfg, budget_s):
     nthreads = cpu_threads()
     blas = oracle.use_blas("auto")
     sh = oracle.shells_from(bs.xcq_shells())
-    # Sample: evenly spaced tasks, grown until one run takes at least a third
-    # of budget_s (the time scales with the sample).
     ntask = len(ts["npts"])
-
-    def run(count):
-        idx = np.unique(np.linspace(0, ntask - 1, max(1, min(count, ntask))).astype(np.int64))
-        sub = reorder_tasks(ts, idx)
-        tic = time.perf_counter()
-        oracle.exc_vxc("PBE", cfg["nbf"], sh, P, sub, nthreads=nthreads)
-        return idx, int(sub["npts"].sum()), time.perf_counter() - tic
-
-    run(nthreads)  # first-call costs (library initialisation, thread pool)
-    idx, pts, elapsed = run(4 * nthreads)
-    while elapsed < budget_s / 3 and len(idx) < ntask:
-        grow = min(20.0, max(2.0, budget_s / max(elapsed, 1e-3)))
-        idx, pts, elapsed = run(int(len(idx) * grow))
+    idx, pts, elapsed = sample_rate(
+        ts, lambda sub: oracle.exc_vxc("PBE", cfg["nbf"], sh, P, sub, nthreads=nthreads),
+        budget_s, nthreads)
     oracle.use_blas(None)
     visible = os.cpu_count()
     log(f"cpu baseline: {len(idx)} tasks, {pts} points in {elapsed:.2f}s on {nthreads} threads "
