@@ -90,12 +90,12 @@ def test_screening_radius_definition(gx, tol):
         assert f(r) <= tol < f(r - 0.01)
 
 
-def _grid_tasks(gx, d, grid):
+def _grid_tasks(gx, d, grid, pruning="Unpruned"):
     mol = gx.Molecule(d["Z"], d["xyz"])
     bs = gx.BasisSet.from_arrays(d["sh_nprim"], d["sh_l"], d["sh_pure"], d["sh_alpha"],
                                  d["sh_coeff"], d["sh_origin"], normalize=False,
                                  shell_tolerance=EPS)
-    mg = gx.MolGridFactory.create_default_molgrid(mol, "Unpruned", 512, "MuraKnowles", grid)
+    mg = gx.MolGridFactory.create_default_molgrid(mol, pruning, 512, "MuraKnowles", grid)
     lb = gx.LoadBalancerFactory("Host", "Default").get_instance(gx.RuntimeEnvironment(), mol,
                                                                  mg, bs)
     return bs, lb.get_tasks()
@@ -127,6 +127,40 @@ def test_ultrafine_grid_integrates_gaussian(gx):
     for a in (0.5, 2.0, 10.0):
         val = np.dot(t["weights"], np.exp(-a * r2))
         assert abs(val / (np.pi / a) ** 1.5 - 1) < 1e-10
+
+
+@pytest.mark.parametrize("grid,nrad,base", [("GM3", 35, (110, 17)), ("FineGrid", 75, (302, 29)),
+                                           ("UltraFineGrid", 99, (590, 41))])
+def test_pruned_grids_follow_the_schemes(gx, grid, nrad, base):
+    """One carbon atom on the Robust and Treutler pruned grids
+    (src/grid_factory.cxx:155-220): Robust puts radial nodes [0, n/4 + 1) on
+    the order-7 rule (26 points), [n/4 + 1, n/2 + 1) on the rule of order
+    base - 6 (base >= 7; the next tabulated order), the rest on the base rule;
+    Treutler [0, n/3 + 1) on 26 points and [n/3 + 1, n/2 + 1) on order 11
+    (50 points).  Both still integrate Gaussians to 1e-8 or better."""
+    sizes = {11: 50, 13: 74, 17: 110, 23: 194, 29: 302, 35: 434, 41: 590}
+    d = load_golden("benzene_svwn5.npz")
+    d1 = {k: d[k] for k in d.files}
+    d1["Z"], d1["xyz"] = d["Z"][:1], d["xyz"][:1]
+    keep = np.all(d["sh_origin"] == d["xyz"][0], axis=1)
+    for k in ("sh_nprim", "sh_l", "sh_pure", "sh_alpha", "sh_coeff", "sh_origin"):
+        d1[k] = d[k][keep]
+    assert d1["Z"][0] == 6
+    npts_base, order = base
+    q, h = nrad // 4 + 1, nrad // 2 + 1
+    robust = q * 26 + (h - q) * sizes[order - 6] + (nrad - h) * npts_base
+    t3 = nrad // 3 + 1
+    treutler = t3 * 26 + (h - t3) * 50 + (nrad - h) * npts_base
+    for scheme, want in (("Robust", robust), ("Treutler", treutler)):
+        _, t = _grid_tasks(gx, d1, grid, scheme)
+        assert int(t["npts"].sum()) == want, scheme
+        r2 = np.sum((t["points"] - d["xyz"][0]) ** 2, axis=1)
+        for a in (0.5, 2.0):
+            val = np.dot(t["weights"], np.exp(-a * r2))
+            assert abs(val / (np.pi / a) ** 1.5 - 1) < 1e-8, (scheme, a)
+    with pytest.raises(gx.GauXCError, match="Unknown pruning"):
+        gx.MolGridFactory.create_default_molgrid(gx.Molecule(d1["Z"], d1["xyz"]), "Psi4", 512,
+                                                 "MuraKnowles", grid)
 
 
 def test_load_balancer_invariants(gx):
