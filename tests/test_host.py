@@ -6,7 +6,8 @@
 * screening radii satisfy their defining inequality;
 * the FineGrid quadrature contains GauXC's raw grid points and weights
   (benzene_weights_ssf tasks_unm), and the UltraFineGrid integrates a
-  Gaussian exactly;
+  Gaussian exactly; the Robust and Treutler pruned grids have the schemes'
+  point counts and still integrate Gaussians;
 * load-balancer invariants (point conservation, sorted unique merges,
   consistent nbe, screening necessary condition), GauXC's 664-task list of
   benzene on the ultra-fine grid reproduced exactly, and the point sets of
