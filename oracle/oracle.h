@@ -10,6 +10,22 @@
  *
  * Layouts: basis values are [npts][nbe] (nbe fastest, as GauXC's host
  * driver), matrices are column-major with leading dimension ld.
+ *
+ * How each function is pinned (tests/test_oracle.py): GauXC's fixtures --
+ * benzene SVWN5 / PBE0 EXC, VXC, FXC, EXC_GRAD and EXC_GRAD_HELLFEY on the
+ * unpruned, Robust and Treutler ultra-fine grids; Li / SVWN5 UKS EXC, VXC
+ * and FXC; doublet cytosine / SVWN5 UKS EXC, VXC and both gradients; the
+ * SSF weights of 15 tasks; water collocation and GauXC's own gau2grid
+ * compiled under oracle/_ref -- and mathematics: central differences of
+ * every functional derivative, of VXC along a trial density (FXC) and of
+ * EXC under nuclear displacement (gradients, RKS and UKS), closed-form
+ * limits, the unpolarised limit of the polarised functionals, UKS with
+ * equal spins against RKS, translational invariance, Gaussian quadrature.
+ * Parity unpinned: per-point values of plain PBE (1.0 PBE_X) beyond the
+ * PBE0 fixture's 0.75 PBE_X + PBE_C (exchange enters linearly, so only its
+ * scale is left open, and the scale is the published 1.0), and polarised
+ * PBE at spin polarisation != 0 against GauXC itself (pinned by the
+ * mathematical checks above only).
  */
 #ifndef GAUXC_ORACLE_H
 #define GAUXC_ORACLE_H
