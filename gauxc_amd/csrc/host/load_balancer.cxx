@@ -102,45 +102,58 @@ void LoadBalancer::build_tasks() const {
   // Every atom's batches (GauXC's host and device load balancers batch on
   // the host), then one screening pass over all of them.
   std::vector<GridBatch> batches;
-  std::vector<int32_t> parent;
+  std::vector<size_t> first(natoms + 1, 0);  // atom ia owns batches [first[ia], first[ia+1])
+  for (size_t ia = 0; ia < natoms; ++ia)
+    first[ia + 1] = first[ia] + mg_->get_grid(mol[ia].Z).batcher().nbatches();
+  batches.resize(first[natoms]);
+#pragma omp parallel for schedule(dynamic)
   for (size_t ia = 0; ia < natoms; ++ia) {
     const auto& atom = mol[ia];
     const std::array<double, 3> center = {atom.x, atom.y, atom.z};
     const auto& batcher = mg_->get_grid(atom.Z).batcher();
-    const size_t nb = batcher.nbatches();
-    const size_t b0 = batches.size();
-    batches.resize(b0 + nb);
-#pragma omp parallel for schedule(dynamic)
-    for (size_t ib = 0; ib < nb; ++ib) batches[b0 + ib] = batcher.at(ib, center);
-    parent.resize(b0 + nb, static_cast<int32_t>(ia));
+    for (size_t ib = 0; ib < first[ia + 1] - first[ia]; ++ib)
+      batches[first[ia] + ib] = batcher.at(ib, center);
   }
   std::vector<std::vector<int32_t>> lists;
   std::vector<int32_t> nbe;
   screen_batches(batches, lists, nbe);
-  std::vector<XCTask> local;
-  for (size_t ib = 0; ib < batches.size(); ++ib) {
-    GridBatch& b = batches[ib];
-    if (b.points.empty() || lists[ib].empty()) continue;
-    XCTask t;
-    t.iParent = parent[ib];
-    t.npts = static_cast<int32_t>(b.points.size());
-    t.points = std::move(b.points);
-    t.weights = std::move(b.weights);
-    t.bfn_screening.shell_list = std::move(lists[ib]);
-    t.bfn_screening.nbe = nbe[ib];
-    t.dist_nearest = meta_->dist_nearest()[parent[ib]];
-    local.push_back(std::move(t));
+  // Per parent atom (tasks are ordered by parent first): the tasks of its
+  // non-empty batches, stably sorted by shell list, equivalent neighbours
+  // merged -- the same list as one stable sort by (parent, shell list) and
+  // one merge pass over all tasks, formed for every atom in parallel.
+  std::vector<std::vector<XCTask>> per(natoms);
+#pragma omp parallel for schedule(dynamic)
+  for (size_t ia = 0; ia < natoms; ++ia) {
+    std::vector<XCTask> local;
+    for (size_t ib = first[ia]; ib < first[ia + 1]; ++ib) {
+      GridBatch& b = batches[ib];
+      if (b.points.empty() || lists[ib].empty()) continue;
+      XCTask t;
+      t.iParent = static_cast<int32_t>(ia);
+      t.npts = static_cast<int32_t>(b.points.size());
+      t.points = std::move(b.points);
+      t.weights = std::move(b.weights);
+      t.bfn_screening.shell_list = std::move(lists[ib]);
+      t.bfn_screening.nbe = nbe[ib];
+      t.dist_nearest = meta_->dist_nearest()[ia];
+      local.push_back(std::move(t));
+    }
+    std::stable_sort(local.begin(), local.end(), [](const XCTask& a, const XCTask& b) {
+      return a.bfn_screening.shell_list < b.bfn_screening.shell_list;
+    });
+    auto& merged = per[ia];
+    for (auto& t : local) {
+      if (!merged.empty() && merged.back().equiv_with(t)) merged.back().merge_with(t);
+      else merged.push_back(std::move(t));
+    }
   }
-  std::stable_sort(local.begin(), local.end(), [](const XCTask& a, const XCTask& b) {
-    if (a.iParent != b.iParent) return a.iParent < b.iParent;
-    return a.bfn_screening.shell_list < b.bfn_screening.shell_list;
-  });
-  std::vector<XCTask> merged;
-  for (auto& t : local) {
-    if (!merged.empty() && merged.back().equiv_with(t)) merged.back().merge_with(t);
-    else merged.push_back(std::move(t));
-  }
-  tasks_ = std::move(merged);
+  size_t ntask = 0;
+  for (const auto& v : per) ntask += v.size();
+  std::vector<XCTask> all;
+  all.reserve(ntask);
+  for (auto& v : per)
+    for (auto& t : v) all.push_back(std::move(t));
+  tasks_ = std::move(all);
   built_ = true;
 }
 
