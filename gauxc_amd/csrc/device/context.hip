@@ -1705,10 +1705,10 @@ namespace {
 
 // One RKS EXC gradient on the context's stream (FP64, stored basis planes):
 // per chunk, collocation (with gradient planes for GGAs); the density stage
-// (factor 2) leaving raw per-point sums; per-point Za coefficients and
-// w eps rho; Za into plane 4; for GGAs X = 2 P_sub B into plane 1; Y =
-// 2 P_sub Za into plane 2; the basis-function part and the grid-weight part
-// into d_grad (the latter only with weight_derivs).
+// (factor 2) leaving raw per-point sums and, for GGAs, its X = 2 P_sub B in
+// plane 4; per-point Za coefficients and w eps rho; Za over grad_z B in
+// plane 3; Y = 2 P_sub Za into plane 2; the basis-function part and the
+// grid-weight part into d_grad (the latter only with weight_derivs).
 int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
   CtxView v = view_of(ctx);
@@ -1733,14 +1733,13 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
       return rc;
     for (int k = 0; k < 4; ++k)
       if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
-                               2.0, 0, ctx->d_P, dp)))
+                               2.0, 0, ctx->d_P, dp, gga ? 4 : -1)))
         return rc;
     if ((rc = launch_grad_points(v, c.coll, func, gga, dp, coef, wf))) return rc;
-    if ((rc = launch_z_from_coeffs(v, c.coll, gga, coef))) return rc;
-    if (gga && (rc = launch_plane_product(v, c.coll, 0, 1))) return rc;
-    if ((rc = launch_plane_product(v, c.coll, 4, 2))) return rc;
+    if ((rc = launch_grad_zpair(v, c.coll, gga, coef, nullptr, 3, -1))) return rc;
+    if ((rc = launch_plane_product(v, c.coll, 3, 2))) return rc;
     if ((rc = launch_grad_contract(v, c.coll, gga, ctx->d_shell_atom, coef, weight_derivs,
-                                   ctx->d_grad)))
+                                   ctx->d_grad, 4, 2)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
                                                ctx->d_dnear, wf, ctx->d_grad,
@@ -1785,7 +1784,7 @@ int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
                                  pass == 0 ? ds : dz)))
           return rc;
     if ((rc = launch_grad_points_uks(v, c.coll, func, gga, ds, dz, wf))) return rc;
-    if ((rc = launch_grad_zpair(v, c.coll, gga, ds, dz))) return rc;
+    if ((rc = launch_grad_zpair(v, c.coll, gga, ds, dz, 4, 3))) return rc;
     if ((rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_P, 1.0, 0)) ||
         (rc = launch_plane_product(v, c.coll, 3, 1, ctx->d_Pz, 1.0, 1)))
       return rc;

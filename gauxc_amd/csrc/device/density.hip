@@ -609,7 +609,7 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
     const double* __restrict__ Pall, double* __restrict__ scratch,
     const double* __restrict__ weights, int func, double fac, int want_z,
     double* __restrict__ red, unsigned* __restrict__ tmax, ShellArgs sa,
-    double* __restrict__ zbuf, double* __restrict__ dens_out) {
+    double* __restrict__ zbuf, double* __restrict__ dens_out, int x_plane) {
   constexpr int BI = NT / 32;
   constexpr int kPtD = 2 * 16 * (NT + 4), kStageD = kPtD + 2 * 16 * kPad64;
   static_assert(INSTAGE == 0 || NT * kXLd <= kStageD, "X^T tile must fit the stages");
@@ -704,6 +704,24 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
             acc[bi][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b1, acc[bi][1], 0, 0, 0);
           }
         }
+      }
+      if (INSTAGE == 0 && x_plane >= 0) {
+        // The X tile itself into plane x_plane (row = function, column =
+        // point), for callers that contract it again (the EXC gradient;
+        // stored-plane mode only, so the in-stage kernels keep their
+        // register budget).
+        double* xo = Bm + (int64_t)x_plane * plane + m0;
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = n0 + wm * 16 * BI + bi * 16 + frow(r);
+            if (row < nbe) {
+#pragma unroll
+              for (int bj = 0; bj < 2; ++bj)
+                xo[(int64_t)row * ld + wn * 32 + bj * 16 + (lane & 15)] = acc[bi][bj][r];
+            }
+          }
       }
       if constexpr (INSTAGE != 0) {
         double* xt = stage;
@@ -1345,8 +1363,10 @@ int launch_z_from_coeffs(const CtxView& v, const WorkRange& r, int gga, const do
 
 int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
                    int func, int gga, double fac, int want_z, const double* P_override,
-                   double* dens_out) {
+                   double* dens_out, int x_plane) {
   if (r.n == 0) return XCQ_SUCCESS;
+  XCQ_REQUIRE(x_plane < 0 || (precision == XCQ_PREC_FP64 && v.in_stage == 0 && x_plane < 5),
+              XCQ_ERROR_INVALID_ARGUMENT, "the X plane is stored by the FP64 stored-plane mode only");
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
   // The bound table is filled only when VXC is computed and contributions
@@ -1358,7 +1378,7 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
     const double* P = P_override ? P_override : v.P;
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, grid, block, 0, v.stream, qa, v.tasks, v.bfmap, P, scr, v.w,
-                         func, fac, want_z, v.red, tm, sa, v.zc, dens_out);
+                         func, fac, want_z, v.red, tm, sa, v.zc, dens_out, x_plane);
     };
     // Any in-stage mode runs the FP64 kernel's one (mode 3).
     if (v.in_stage != 0) {

@@ -16,8 +16,9 @@
 // GGAs, Y = 2 P_sub Za) instead of four, and
 //   g_mu = grad B_mu . Y_mu + X_mu (H_mu c),  c = 2 w vgamma grad rho.
 // grad B and H c are evaluated where they are used (k_grad_contract), so the
-// chunk scratch's five planes suffice: B, grad B (collocation), Za (plane 4),
-// then X over plane 1 and Y over plane 2 once Za is formed.
+// chunk scratch's five planes suffice: B and grad B (collocation), X stored
+// by the density stage itself into plane 4 (GGA), Za over grad_z B in
+// plane 3 once the coefficients exist, then Y = 2 P_sub Za into plane 2.
 // The grid-weight part contracts the SSF weight derivatives with w eps rho
 // per point (k_ssf_grad).
 //
@@ -163,25 +164,28 @@ __global__ __launch_bounds__(64) void k_grad_points_uks(const WorkItem* __restri
   }
 }
 
-// Za_s = a_s B + c_s . grad B into plane 4 and Za_z = a_z B + c_z . grad B
-// into plane 3 (over grad_z B, which each thread reads before it writes) for
-// one (task, 64-point block): lane = point, the four waves take rows w,
-// w + 4, ...
+// Za_s = a_s B + c_s . grad B into plane pl_s and, when coef_z is given,
+// Za_z = a_z B + c_z . grad B into plane pl_z (planes 3 or 4: each thread
+// reads its element's B and grad B before it writes, so plane 3 may be
+// overwritten in place) for one (task, 64-point block): lane = point, the
+// four waves take rows w, w + 4, ...
 template <bool GGA>
 __global__ __launch_bounds__(256) void k_grad_zpair(const WorkItem* __restrict__ items,
                                                     const DevTask* __restrict__ tasks,
                                                     double* __restrict__ scratch,
                                                     const double* __restrict__ coef_s,
-                                                    const double* __restrict__ coef_z) {
+                                                    const double* __restrict__ coef_z, int pl_s,
+                                                    int pl_z) {
   const WorkItem wi = items[blockIdx.x];
   const DevTask t = tasks[wi.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t b = t.red_off + wi.a;
+  const bool two = coef_z != nullptr;
   double cs[4], cz[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     cs[c] = coef_s[b * 256 + c * 64 + lane];
-    cz[c] = coef_z[b * 256 + c * 64 + lane];
+    cz[c] = two ? coef_z[b * 256 + c * 64 + lane] : 0.;
   }
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
@@ -195,8 +199,8 @@ __global__ __launch_bounds__(256) void k_grad_zpair(const WorkItem* __restrict__
       zs += cs[1] * x + cs[2] * y + cs[3] * z;
       zz += cz[1] * x + cz[2] * y + cz[3] * z;
     }
-    B[4 * plane + o] = zs;
-    B[3 * plane + o] = zz;
+    B[pl_s * plane + o] = zs;
+    if (two) B[pl_z * plane + o] = zz;
   }
 }
 
@@ -468,7 +472,8 @@ __global__ __launch_bounds__(256) void k_grad_contract(
     const int32_t* __restrict__ lsh_first, const int32_t* __restrict__ shell_atom,
     const double* __restrict__ px, const double* __restrict__ py,
     const double* __restrict__ pz, const double* __restrict__ scratch,
-    const double* __restrict__ coef, int weight_derivs, double* __restrict__ grad) {
+    const double* __restrict__ coef, int weight_derivs, double* __restrict__ grad, int x_pl,
+    int y_pl) {
   const WorkItem wi = items[blockIdx.x];
   const DevTask t = tasks[wi.task];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -487,8 +492,8 @@ __global__ __launch_bounds__(256) void k_grad_contract(
   const int64_t ld = t.npts_pad;
   const int64_t plane = (int64_t)t.nbe * ld;
   const int64_t m0 = (int64_t)wi.a * kPtBlock + lane;
-  const double* Xp = scratch + t.b_off + plane + m0;
-  const double* Yp = scratch + t.b_off + 2 * plane + m0;
+  const double* Xp = scratch + t.b_off + x_pl * plane + m0;
+  const double* Yp = scratch + t.b_off + y_pl * plane + m0;
   int cur = -1;                            // atom of the running sum (uniform)
   double run[3] = {0., 0., 0.}, par[3] = {0., 0., 0.};
   auto flush = [&]() {
@@ -885,15 +890,17 @@ int launch_grad_points_uks(const CtxView& v, const WorkRange& r, int func, int g
 }
 
 int launch_grad_zpair(const CtxView& v, const WorkRange& r, int gga, const double* coef_s,
-                      const double* coef_z) {
+                      const double* coef_z, int pl_s, int pl_z) {
   if (r.n == 0) return XCQ_SUCCESS;
+  XCQ_REQUIRE((pl_s == 3 || pl_s == 4) && (!coef_z || ((pl_z == 3 || pl_z == 4) && pl_z != pl_s)),
+              XCQ_ERROR_INVALID_ARGUMENT, "Za planes must be 3 or 4");
   double* scr = static_cast<double*>(v.scratch);
   if (gga)
     hipLaunchKernelGGL(k_grad_zpair<true>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
-                       v.work + r.off, v.tasks, scr, coef_s, coef_z);
+                       v.work + r.off, v.tasks, scr, coef_s, coef_z, pl_s, pl_z);
   else
     hipLaunchKernelGGL(k_grad_zpair<false>, dim3((unsigned)r.n), dim3(256), 0, v.stream,
-                       v.work + r.off, v.tasks, scr, coef_s, coef_z);
+                       v.work + r.off, v.tasks, scr, coef_s, coef_z, pl_s, pl_z);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
@@ -910,12 +917,13 @@ int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int ou
 
 int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                          const int32_t* shell_atom, const double* coef, int weight_derivs,
-                         double* grad) {
+                         double* grad, int x_pl, int y_pl) {
   if (r.n == 0) return XCQ_SUCCESS;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)r.n), dim3(256), 0, v.stream, v.work + r.off,
                        v.tasks, v.shells, v.shell_list, v.lsh, shell_atom, v.px, v.py, v.pz,
-                       static_cast<const double*>(v.scratch), coef, weight_derivs, grad);
+                       static_cast<const double*>(v.scratch), coef, weight_derivs, grad, x_pl,
+                       y_pl);
   };
   gga ? go(k_grad_contract<true>) : go(k_grad_contract<false>);
   XCQ_HIP(hipGetLastError());

@@ -255,10 +255,13 @@ int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const Dev
                       unsigned* tmax, int gga, int lmax);
 // Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
 // whose tasks use `nt`-function tiles (hiocc: the three-workgroups-per-CU
-// configuration of the mixed kernel).
+// configuration of the mixed kernel).  FP64 stored-plane mode only
+// (in_stage 0): x_plane >= 0 also stores X = fac P_sub B into that scratch
+// plane.
 int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
                    int func, int gga, double fac, int want_z,
-                   const double* P_override = nullptr, double* dens_out = nullptr);
+                   const double* P_override = nullptr, double* dens_out = nullptr,
+                   int x_plane = -1);
 // UKS: polarised functional and Z coefficients per point block from the two
 // density passes' raw sums; Z planes from stored coefficients.
 int launch_uks_points(const CtxView& v, const WorkRange& r, int func, int gga, const double* ds,
@@ -280,13 +283,15 @@ int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int ou
                          const double* P = nullptr, double fac = 2.0, int accumulate = 0);
 int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                          const int32_t* shell_atom, const double* coef, int weight_derivs,
-                         double* grad);
-// UKS: coefficients written over the P_s / P_z sums; Za_s -> plane 4,
-// Za_z -> plane 3; contraction with Y (plane 1), X_N (2), X_Z (3).
+                         double* grad, int x_pl, int y_pl);
+// Za = a B + c . grad B from [block][4][64] coefficients into plane pl_s
+// (and, with coef_z, a second one into pl_z; planes 3 and 4 only).  UKS:
+// coefficients written over the P_s / P_z sums; contraction with Y
+// (plane 1), X_N (2), X_Z (3).
 int launch_grad_points_uks(const CtxView& v, const WorkRange& r, int func, int gga, double* ds,
                            double* dz, double* wf);
 int launch_grad_zpair(const CtxView& v, const WorkRange& r, int gga, const double* coef_s,
-                      const double* coef_z);
+                      const double* coef_z, int pl_s, int pl_z);
 int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
                              const int32_t* shell_atom, const double* coef_s,
                              const double* coef_z, int weight_derivs, double* grad);
