@@ -1354,6 +1354,21 @@ int ensure_point_buffers(xcq_context_s* ctx) {
   return dalloc(&ctx->d_uks, (size_t)5 * 256 * std::max<int64_t>(ctx->pblocks, 1));
 }
 
+// Basis-evaluation mode of the FP64 UKS and FXC paths: in-stage (3: the
+// density stage evaluates B and grad B itself and Z is formed by
+// k_z_in_stage from the per-point coefficients, so collocation stores the
+// value plane only) up to kInStageMaxL, stored planes (0) above.
+int derivative_mode(const xcq_context_s* ctx) { return ctx->max_l <= kInStageMaxL ? 3 : 0; }
+
+// Z of a chunk from per-point coefficients ([block][4][64]) into plane 4.
+int z_from_point_coeffs(xcq_context_s* ctx, const CtxView& v, const Chunk& c, int mode, int gga,
+                        const double* coef) {
+  if (mode == 0) return launch_z_from_coeffs(v, c.coll, gga, coef);
+  return launch_z_in_stage(ctx->stream, v.work + c.coll.off, c.coll.n, v.tasks, v.shells,
+                           v.shell_list, v.lsh, v.px, v.py, v.pz, coef, v.scratch,
+                           XCQ_PREC_FP64, nullptr, gga, ctx->max_l);
+}
+
 // One UKS evaluation on the context's stream (FP64, stored basis planes):
 // per chunk, collocation with gradient planes; the density stage twice (P_s,
 // then P_z; factor 1) leaving raw per-point sums; the polarised functional
@@ -1362,8 +1377,9 @@ int ensure_point_buffers(xcq_context_s* ctx) {
 // off (every contribution is added, as GauXC does).
 int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const int mode = derivative_mode(ctx);
   CtxView v = view_of(ctx);
-  v.in_stage = 0;
+  v.in_stage = mode;
   v.tmax = nullptr;
   v.vxc_drop = 0.0;
   hipStream_t s = ctx->stream;
@@ -1382,8 +1398,8 @@ int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
   for (const Chunk& c : ctx->chunks) {
     const WorkItem* cw = v.work + c.coll.off;
     if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
-                                      ctx->max_l)))
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch),
+                                      gga && mode == 0, ctx->max_l)))
       return rc;
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < 4; ++k)
@@ -1396,7 +1412,7 @@ int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
     for (int comp = 0; comp < 2; ++comp) {
       CtxView vc = v;
       vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
-      if ((rc = launch_z_from_coeffs(vc, c.coll, gga, comp == 0 ? zs : zz))) return rc;
+      if ((rc = z_from_point_coeffs(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
       for (int k = 0; k < 2; ++k)
         if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
     }
@@ -1477,8 +1493,9 @@ namespace {
 // contribution is added (no drop screening), as GauXC does.
 int enqueue_fxc(xcq_context_s* ctx, int func) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const int mode = derivative_mode(ctx);
   CtxView v = view_of(ctx);
-  v.in_stage = 0;
+  v.in_stage = mode;
   v.tmax = nullptr;
   v.vxc_drop = 0.0;
   hipStream_t s = ctx->stream;
@@ -1494,8 +1511,8 @@ int enqueue_fxc(xcq_context_s* ctx, int func) {
   for (const Chunk& c : ctx->chunks) {
     const WorkItem* cw = v.work + c.coll.off;
     if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
-                                      ctx->max_l)))
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch),
+                                      gga && mode == 0, ctx->max_l)))
       return rc;
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < 4; ++k)
@@ -1504,7 +1521,7 @@ int enqueue_fxc(xcq_context_s* ctx, int func) {
                                  pass == 0 ? dp : dt)))
           return rc;
     if ((rc = launch_fxc_points(v, c.coll, func, gga, dp, dt, zc))) return rc;
-    if ((rc = launch_z_from_coeffs(v, c.coll, gga, zc))) return rc;
+    if ((rc = z_from_point_coeffs(ctx, v, c, mode, gga, zc))) return rc;
     for (int k = 0; k < 2; ++k)
       if ((rc = launch_syr2k(v, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
   }
@@ -1568,8 +1585,9 @@ namespace {
 // symmetrised.  Every contribution is added.
 int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const int mode = derivative_mode(ctx);
   CtxView v = view_of(ctx);
-  v.in_stage = 0;
+  v.in_stage = mode;
   v.tmax = nullptr;
   v.vxc_drop = 0.0;
   hipStream_t s = ctx->stream;
@@ -1588,8 +1606,8 @@ int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
   for (const Chunk& c : ctx->chunks) {
     const WorkItem* cw = v.work + c.coll.off;
     if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
-                                      ctx->max_l)))
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch),
+                                      gga && mode == 0, ctx->max_l)))
       return rc;
     for (int pass = 0; pass < 4; ++pass)
       for (int k = 0; k < 4; ++k)
@@ -1601,7 +1619,7 @@ int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
     for (int comp = 0; comp < 2; ++comp) {
       CtxView vc = v;
       vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
-      if ((rc = launch_z_from_coeffs(vc, c.coll, gga, comp == 0 ? zs : zz))) return rc;
+      if ((rc = z_from_point_coeffs(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
       for (int k = 0; k < 2; ++k)
         if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
     }

@@ -3,7 +3,9 @@ GauXC's benzene SVWN5 / PBE0 FXC fixtures through the GauXC API at GauXC's
 bar (tests/2nd_derivative_test.cxx:121-126) and tighter, a water cluster
 against the oracle at the FP64 VXC bar, linearity in the trial density, and
 the boundary's argument and state checks; UKS against GauXC's Li fixture
-(:128-134) and a spin-polarised water dimer against the oracle."""
+(:128-134) and a spin-polarised water dimer against the oracle; both
+contractions and the UKS EXC/VXC with f and g shells (in-stage and
+stored-plane basis evaluation) against the oracle."""
 import numpy as np
 import pytest
 
@@ -164,3 +166,62 @@ def test_fxc_uks_dimer_matches_oracle(gx, orc, gpu, func):
     rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional(func), lb)
     with pytest.raises(gx.GauXCError, match="polarized"):
         rks.eval_fxc_contraction(Ps, tPs, Pz, tPz)
+
+
+@pytest.mark.parametrize("lmax", [3, 4])
+def test_derivative_paths_high_l_match_oracle(gx, orc, gpu, lmax):
+    """RKS and UKS FXC contractions and the UKS EXC/VXC on a two-atom basis
+    with f (and g) shells, spherical and Cartesian, and a six-primitive s
+    shell: up to f the density stage evaluates the basis itself and Z comes
+    from the per-point coefficients (in-stage mode); with g shells both read
+    the stored collocation planes."""
+    Z = [8, 1]
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 1.8]])
+    l, pure, alpha, coeff, origin = [], [], [], [], []
+    for r in xyz:
+        for am in range(lmax + 1):
+            for p in ([False] if am < 2 else [True, False]):
+                l.append(am)
+                pure.append(p)
+                alpha.append([1.7 / (am + 1), 0.45])
+                coeff.append([0.6, 0.8])
+                origin.append(r)
+        l.append(0)
+        pure.append(False)
+        alpha.append([30.0, 9.0, 3.0, 1.0, 0.35, 0.12])
+        coeff.append([0.05, 0.15, 0.3, 0.4, 0.3, 0.1])
+        origin.append(r)
+    mol = gx.Molecule(Z, xyz)
+    bs = gx.BasisSet(l, pure, alpha, coeff, origin, normalize=True, shell_tolerance=1e-10)
+    nbf = bs.nbf()
+    pa = gx.workloads.density(nbf, 4, 7)
+    pb = gx.workloads.density(nbf, 3, 8)
+    rng = np.random.default_rng(lmax)
+    ta, tb = rng.standard_normal((2, nbf, nbf)) * 0.01
+    ta, tb = ta + ta.T, tb + tb.T
+    lb = gx.workloads.build_tasks(mol, bs, grid="GM3")
+    t = lb.get_tasks()
+    w = orc.ssf_weights(xyz, t)
+    sh = orc.shells_from(bs.xcq_shells())
+    gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
+    rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional("PBE"), lb)
+    uks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
+        gx.Functional("PBE", polarized=True), lb)
+    _flush(f"FXC lmax={lmax}")
+    f = rks.eval_fxc_contraction(pa, ta)
+    f_o = orc.fxc_contraction("PBE", nbf, sh, pa, ta, t, weights=w)[1]
+    _flush(f"max|dFXC|={np.abs(f - f_o).max():.3e}")
+    assert np.abs(f - f_o).max() <= FP64_VXC
+    _flush(f"UKS FXC lmax={lmax}")
+    fs, fz = uks.eval_fxc_contraction(pa + pb, ta + tb, pa - pb, ta - tb)
+    fs_o, fz_o = orc.fxc_contraction_uks("PBE", nbf, sh, pa + pb, pa - pb, ta + tb, ta - tb, t,
+                                         weights=w)
+    _flush(f"max|dFXC_s|={np.abs(fs - fs_o).max():.3e} max|dFXC_z|={np.abs(fz - fz_o).max():.3e}")
+    assert np.abs(fs - fs_o).max() <= FP64_VXC and np.abs(fz - fz_o).max() <= FP64_VXC
+    _flush(f"UKS EXC/VXC lmax={lmax}")
+    exc, vs, vz = uks.eval_exc_vxc(pa + pb, pa - pb)
+    exc_o, _, vs_o, vz_o = orc.exc_vxc_uks("PBE", nbf, sh, pa + pb, pa - pb, t, weights=w)[:4]
+    _flush(f"dEXC={exc - exc_o:.3e} max|dVs|={np.abs(vs - vs_o).max():.3e} "
+           f"max|dVz|={np.abs(vz - vz_o).max():.3e}")
+    assert abs(exc - exc_o) <= 1e-10
+    assert np.abs(vs - vs_o).max() <= FP64_VXC and np.abs(vz - vz_o).max() <= FP64_VXC
