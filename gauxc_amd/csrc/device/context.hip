@@ -1721,16 +1721,20 @@ int xcq_download_fxc(xcq_context ctx, double* nel, double* fxc, int64_t ldfxc) {
 // ---- EXC gradient (GauXC eval_exc_grad(P)) ----------------------------------
 namespace {
 
-// One RKS EXC gradient on the context's stream (FP64, stored basis planes):
-// per chunk, collocation (with gradient planes for GGAs); the density stage
-// (factor 2) leaving raw per-point sums and, for GGAs, its X = 2 P_sub B in
-// plane 4; per-point Za coefficients and w eps rho; Za over grad_z B in
-// plane 3; Y = 2 P_sub Za into plane 2; the basis-function part and the
-// grid-weight part into d_grad (the latter only with weight_derivs).
+// One RKS EXC gradient on the context's stream (FP64): per chunk,
+// collocation; the density stage (factor 2) leaving raw per-point sums and,
+// for GGAs, its X = 2 P_sub B in a plane; per-point Za coefficients and
+// w eps rho; Za; Y = 2 P_sub Za into plane 2; the basis-function part and
+// the grid-weight part into d_grad (the latter only with weight_derivs).
+// Up to f functions the basis is evaluated in-stage (collocation stores the
+// value plane only; X in plane 1, Za by k_z_in_stage into plane 4); with g
+// functions on stored planes (B and grad B; X in plane 4, Za over grad_z B
+// in plane 3).
 int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const int mode = derivative_mode(ctx);
   CtxView v = view_of(ctx);
-  v.in_stage = 0;
+  v.in_stage = mode;
   v.tmax = nullptr;
   v.vxc_drop = 0.0;
   hipStream_t s = ctx->stream;
@@ -1746,18 +1750,24 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
   for (const Chunk& c : ctx->chunks) {
     const WorkItem* cw = v.work + c.coll.off;
     if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
-                                      ctx->max_l)))
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch),
+                                      gga && mode == 0, ctx->max_l)))
       return rc;
+    // X: plane 1 in the in-stage mode (no gradient planes are stored),
+    // plane 4 on stored planes.
+    const int x_pl = mode == 3 ? 1 : 4;
     for (int k = 0; k < 4; ++k)
       if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
-                               2.0, 0, ctx->d_P, dp, gga ? 4 : -1)))
+                               2.0, 0, ctx->d_P, dp, gga ? x_pl : -1)))
         return rc;
     if ((rc = launch_grad_points(v, c.coll, func, gga, dp, coef, wf))) return rc;
-    if ((rc = launch_grad_zpair(v, c.coll, gga, coef, nullptr, 3, -1))) return rc;
-    if ((rc = launch_plane_product(v, c.coll, 3, 2))) return rc;
+    // Za: plane 4 (in-stage evaluation) or over grad_z B in plane 3.
+    const int za_pl = mode == 3 ? 4 : 3;
+    if (mode == 3) rc = z_from_point_coeffs(ctx, v, c, mode, gga, coef);
+    else rc = launch_grad_zpair(v, c.coll, gga, coef, nullptr, 3, -1);
+    if (rc || (rc = launch_plane_product(v, c.coll, za_pl, 2))) return rc;
     if ((rc = launch_grad_contract(v, c.coll, gga, ctx->d_shell_atom, coef, weight_derivs,
-                                   ctx->d_grad, 4, 2)))
+                                   ctx->d_grad, x_pl, 2)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
                                                ctx->d_dnear, wf, ctx->d_grad,
@@ -1770,13 +1780,17 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
 
 // One UKS EXC gradient: per chunk, collocation; the density stage twice (P_s,
 // then P_z; factor 1); the polarised functional's coefficients written over
-// the sums, w eps rho; Za_s into plane 4 and Za_z into plane 3; Y = P_s Za_s
-// + P_z Za_z into plane 1; for GGAs X_N = P_s B into plane 2 and X_Z = P_z B
-// into plane 3; the basis-function and grid-weight parts.
+// the sums, w eps rho; Y = P_s Za_s + P_z Za_z into plane 1, X_N = P_s B
+// into plane 2 and X_Z = P_z B into plane 3 (GGA); the basis-function and
+// grid-weight parts.  In-stage mode (up to f functions): X_N and X_Z come
+// from the density passes, Za_s and Za_z in turn from k_z_in_stage through
+// plane 4.  Stored planes (g functions): Za_s into plane 4 and Za_z over
+// grad_z B in plane 3, X_N and X_Z by plane products after Y.
 int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
+  const int mode = derivative_mode(ctx);
   CtxView v = view_of(ctx);
-  v.in_stage = 0;
+  v.in_stage = mode;
   v.tmax = nullptr;
   v.vxc_drop = 0.0;
   hipStream_t s = ctx->stream;
@@ -1792,23 +1806,35 @@ int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
   for (const Chunk& c : ctx->chunks) {
     const WorkItem* cw = v.work + c.coll.off;
     if ((rc = launch_collocation_fp64(s, cw, c.coll.n, v.tasks, v.shells, v.shell_list, v.lsh,
-                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch), gga,
-                                      ctx->max_l)))
+                                      v.px, v.py, v.pz, static_cast<double*>(v.scratch),
+                                      gga && mode == 0, ctx->max_l)))
       return rc;
+    // In-stage mode: the density passes leave X_N in plane 2 and X_Z in
+    // plane 3 (GGA), which no gradient plane occupies.
     for (int pass = 0; pass < 2; ++pass)
       for (int k = 0; k < 4; ++k)
         if ((rc = launch_density(v, c.dens[k], k == 0 ? 32 : 64, false, XCQ_PREC_FP64, func, gga,
                                  1.0, 0, pass == 0 ? ctx->d_P : ctx->d_Pz,
-                                 pass == 0 ? ds : dz)))
+                                 pass == 0 ? ds : dz,
+                                 (gga && mode == 3) ? 2 + pass : -1)))
           return rc;
     if ((rc = launch_grad_points_uks(v, c.coll, func, gga, ds, dz, wf))) return rc;
-    if ((rc = launch_grad_zpair(v, c.coll, gga, ds, dz, 4, 3))) return rc;
-    if ((rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_P, 1.0, 0)) ||
-        (rc = launch_plane_product(v, c.coll, 3, 1, ctx->d_Pz, 1.0, 1)))
-      return rc;
-    if (gga && ((rc = launch_plane_product(v, c.coll, 0, 2, ctx->d_P, 1.0, 0)) ||
-                (rc = launch_plane_product(v, c.coll, 0, 3, ctx->d_Pz, 1.0, 0))))
-      return rc;
+    if (mode == 3) {
+      // Za_s, then Za_z, through plane 4: Y = P_s Za_s + P_z Za_z in plane 1.
+      if ((rc = z_from_point_coeffs(ctx, v, c, mode, gga, ds)) ||
+          (rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_P, 1.0, 0)) ||
+          (rc = z_from_point_coeffs(ctx, v, c, mode, gga, dz)) ||
+          (rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_Pz, 1.0, 1)))
+        return rc;
+    } else {
+      if ((rc = launch_grad_zpair(v, c.coll, gga, ds, dz, 4, 3))) return rc;
+      if ((rc = launch_plane_product(v, c.coll, 4, 1, ctx->d_P, 1.0, 0)) ||
+          (rc = launch_plane_product(v, c.coll, 3, 1, ctx->d_Pz, 1.0, 1)))
+        return rc;
+      if (gga && ((rc = launch_plane_product(v, c.coll, 0, 2, ctx->d_P, 1.0, 0)) ||
+                  (rc = launch_plane_product(v, c.coll, 0, 3, ctx->d_Pz, 1.0, 0))))
+        return rc;
+    }
     if ((rc = launch_grad_contract_uks(v, c.coll, gga, ctx->d_shell_atom, ds, dz, weight_derivs,
                                        ctx->d_grad)))
       return rc;

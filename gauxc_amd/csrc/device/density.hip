@@ -705,11 +705,11 @@ __global__ __launch_bounds__(256, HIOCC ? 3 : 2) void k_density_f64(
           }
         }
       }
-      if (INSTAGE == 0 && x_plane >= 0) {
+      if ((INSTAGE == 0 || !HIOCC) && x_plane >= 0) {
         // The X tile itself into plane x_plane (row = function, column =
-        // point), for callers that contract it again (the EXC gradient;
-        // stored-plane mode only, so the in-stage kernels keep their
-        // register budget).
+        // point), for callers that contract it again (the EXC gradient; not
+        // in the three-workgroups-per-CU configuration, whose register
+        // budget it would overrun).
         double* xo = Bm + (int64_t)x_plane * plane + m0;
 #pragma unroll
         for (int bi = 0; bi < BI; ++bi)
@@ -1365,8 +1365,9 @@ int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int
                    int func, int gga, double fac, int want_z, const double* P_override,
                    double* dens_out, int x_plane) {
   if (r.n == 0) return XCQ_SUCCESS;
-  XCQ_REQUIRE(x_plane < 0 || (precision == XCQ_PREC_FP64 && v.in_stage == 0 && x_plane < 5),
-              XCQ_ERROR_INVALID_ARGUMENT, "the X plane is stored by the FP64 stored-plane mode only");
+  XCQ_REQUIRE(x_plane < 0 || (precision == XCQ_PREC_FP64 && !hiocc && x_plane < 5),
+              XCQ_ERROR_INVALID_ARGUMENT,
+              "the X plane is stored by the FP64 kernels in their two-workgroup configuration");
   const QueueArgs qa = queue_args(v, r);
   const dim3 grid(grid_of(r)), block(256);
   // The bound table is filled only when VXC is computed and contributions

@@ -255,9 +255,8 @@ int launch_z_in_stage(hipStream_t s, const WorkItem* items, int64_t n, const Dev
                       unsigned* tmax, int gga, int lmax);
 // Density stage (X, rho/grad rho, functional, EXC/NEL, Z) of one work range
 // whose tasks use `nt`-function tiles (hiocc: the three-workgroups-per-CU
-// configuration of the mixed kernel).  FP64 stored-plane mode only
-// (in_stage 0): x_plane >= 0 also stores X = fac P_sub B into that scratch
-// plane.
+// configuration of the mixed kernel).  FP64 only, without hiocc:
+// x_plane >= 0 also stores X = fac P_sub B into that scratch plane.
 int launch_density(const CtxView& v, const WorkRange& r, int nt, bool hiocc, int precision,
                    int func, int gga, double fac, int want_z,
                    const double* P_override = nullptr, double* dens_out = nullptr,
