@@ -1767,7 +1767,7 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
     else rc = launch_grad_zpair(v, c.coll, gga, coef, nullptr, 3, -1);
     if (rc || (rc = launch_plane_product(v, c.coll, za_pl, 2))) return rc;
     if ((rc = launch_grad_contract(v, c.coll, gga, ctx->d_shell_atom, coef, weight_derivs,
-                                   ctx->d_grad, x_pl, 2)))
+                                   ctx->d_grad, x_pl, 2, ctx->max_l)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
                                                ctx->d_dnear, wf, ctx->d_grad,
@@ -1836,7 +1836,7 @@ int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
         return rc;
     }
     if ((rc = launch_grad_contract_uks(v, c.coll, gga, ctx->d_shell_atom, ds, dz, weight_derivs,
-                                       ctx->d_grad)))
+                                       ctx->d_grad, ctx->max_l)))
       return rc;
     if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
                                                ctx->d_dnear, wf, ctx->d_grad,

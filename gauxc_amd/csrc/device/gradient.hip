@@ -458,6 +458,12 @@ __device__ __forceinline__ void shell_grad_hess(double x, double y, double z, do
   }
 }
 
+// The highest angular momentum a contraction kernel instantiates: the
+// kernels are compiled for LMAX = 3 (the register budget of f functions)
+// and LMAX = 4, and the launchers pick by the basis.
+template <int LMAX>
+constexpr int kTopL = LMAX > 3 ? 4 : 3;
+
 // ---- basis-function part of the gradient -----------------------------------
 // One (task, 64-point block) per 256-thread workgroup; lane = point, wave w
 // takes the task's shells w, w + 4, ...  Per shell (with the weight
@@ -465,8 +471,8 @@ __device__ __forceinline__ void shell_grad_hess(double x, double y, double z, do
 // grad B . Y + X (H c) (X only for GGAs), reduced over the wave; runs of
 // shells on one atom are added to that atom with one set of atomics, the
 // parent's share (weight derivatives only) once per wave at the end.
-template <bool GGA>
-__global__ __launch_bounds__(256) void k_grad_contract(
+template <bool GGA, int LMAX>
+__global__ __launch_bounds__(256, 2) void k_grad_contract(
     const WorkItem* __restrict__ items, const DevTask* __restrict__ tasks,
     const DevShell* __restrict__ shells, const int32_t* __restrict__ shell_list,
     const int32_t* __restrict__ lsh_first, const int32_t* __restrict__ shell_atom,
@@ -537,7 +543,7 @@ __global__ __launch_bounds__(256) void k_grad_contract(
     else if (l == 1) shell_grad_hc<1>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
     else if (l == 2) shell_grad_hc<2>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
     else if (l == 3) shell_grad_hc<3>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
-    else shell_grad_hc<4>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
+    else shell_grad_hc<kTopL<LMAX>>(x, y, z, R, R1, R2, cx, cy, cz, pure, put);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       for (int off = 32; off > 0; off >>= 1) s[d] += __shfl_xor(s[d], off);
@@ -563,8 +569,8 @@ __global__ __launch_bounds__(256) void k_grad_contract(
 // UKS basis-function part: as k_grad_contract with Y in plane 1, X_N in
 // plane 2 and X_Z in plane 3; per function grad B . Y + H (X_N c_s + X_Z c_z)
 // (the Hessian term for GGAs only).
-template <bool GGA>
-__global__ __launch_bounds__(256) void k_grad_contract_uks(
+template <bool GGA, int LMAX>
+__global__ __launch_bounds__(256, 2) void k_grad_contract_uks(
     const WorkItem* __restrict__ items, const DevTask* __restrict__ tasks,
     const DevShell* __restrict__ shells, const int32_t* __restrict__ shell_list,
     const int32_t* __restrict__ lsh_first, const int32_t* __restrict__ shell_atom,
@@ -641,7 +647,7 @@ __global__ __launch_bounds__(256) void k_grad_contract_uks(
       else if (l == 1) shell_grad_hess<1>(x, y, z, R, R1, R2, pure, put);
       else if (l == 2) shell_grad_hess<2>(x, y, z, R, R1, R2, pure, put);
       else if (l == 3) shell_grad_hess<3>(x, y, z, R, R1, R2, pure, put);
-      else shell_grad_hess<4>(x, y, z, R, R1, R2, pure, put);
+      else shell_grad_hess<kTopL<LMAX>>(x, y, z, R, R1, R2, pure, put);
     } else {
       // LDA needs only grad B: the Hessian-vector form with c = 0.
       auto put3 = [&](int f, double bx, double by, double bz, double, double, double) {
@@ -652,7 +658,7 @@ __global__ __launch_bounds__(256) void k_grad_contract_uks(
       else if (l == 1) shell_grad_hc<1>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
       else if (l == 2) shell_grad_hc<2>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
       else if (l == 3) shell_grad_hc<3>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
-      else shell_grad_hc<4>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
+      else shell_grad_hc<kTopL<LMAX>>(x, y, z, R, R1, R2, 0., 0., 0., pure, put3);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -917,7 +923,7 @@ int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int ou
 
 int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                          const int32_t* shell_atom, const double* coef, int weight_derivs,
-                         double* grad, int x_pl, int y_pl) {
+                         double* grad, int x_pl, int y_pl, int lmax) {
   if (r.n == 0) return XCQ_SUCCESS;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)r.n), dim3(256), 0, v.stream, v.work + r.off,
@@ -925,14 +931,15 @@ int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                        static_cast<const double*>(v.scratch), coef, weight_derivs, grad, x_pl,
                        y_pl);
   };
-  gga ? go(k_grad_contract<true>) : go(k_grad_contract<false>);
+  if (lmax <= 3) gga ? go(k_grad_contract<true, 3>) : go(k_grad_contract<false, 3>);
+  else gga ? go(k_grad_contract<true, 4>) : go(k_grad_contract<false, 4>);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }
 
 int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
                              const int32_t* shell_atom, const double* coef_s,
-                             const double* coef_z, int weight_derivs, double* grad) {
+                             const double* coef_z, int weight_derivs, double* grad, int lmax) {
   if (r.n == 0) return XCQ_SUCCESS;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)r.n), dim3(256), 0, v.stream, v.work + r.off,
@@ -940,7 +947,8 @@ int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
                        static_cast<const double*>(v.scratch), coef_s, coef_z, weight_derivs,
                        grad);
   };
-  gga ? go(k_grad_contract_uks<true>) : go(k_grad_contract_uks<false>);
+  if (lmax <= 3) gga ? go(k_grad_contract_uks<true, 3>) : go(k_grad_contract_uks<false, 3>);
+  else gga ? go(k_grad_contract_uks<true, 4>) : go(k_grad_contract_uks<false, 4>);
   XCQ_HIP(hipGetLastError());
   return XCQ_SUCCESS;
 }

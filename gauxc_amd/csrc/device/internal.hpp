@@ -282,7 +282,7 @@ int launch_plane_product(const CtxView& v, const WorkRange& r, int in_pl, int ou
                          const double* P = nullptr, double fac = 2.0, int accumulate = 0);
 int launch_grad_contract(const CtxView& v, const WorkRange& r, int gga,
                          const int32_t* shell_atom, const double* coef, int weight_derivs,
-                         double* grad, int x_pl, int y_pl);
+                         double* grad, int x_pl, int y_pl, int lmax);
 // Za = a B + c . grad B from [block][4][64] coefficients into plane pl_s
 // (and, with coef_z, a second one into pl_z; planes 3 and 4 only).  UKS:
 // coefficients written over the P_s / P_z sums; contraction with Y
@@ -293,7 +293,7 @@ int launch_grad_zpair(const CtxView& v, const WorkRange& r, int gga, const doubl
                       const double* coef_z, int pl_s, int pl_z);
 int launch_grad_contract_uks(const CtxView& v, const WorkRange& r, int gga,
                              const int32_t* shell_atom, const double* coef_s,
-                             const double* coef_z, int weight_derivs, double* grad);
+                             const double* coef_z, int weight_derivs, double* grad, int lmax);
 int launch_ssf_grad(const CtxView& v, const WorkRange& r, int natoms, const double* xyz,
                     const double* dnear, const double* wf, double* grad,
                     const int32_t* nbr_idx = nullptr, const double* nbr_r = nullptr);
