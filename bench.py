@@ -167,7 +167,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--derivatives", action="store_true",
                     help="also time the FP64 EXC gradient (with and without the grid-weight "
-                         "derivatives) and the RKS FXC contraction on the same workload")
+                         "derivatives), the RKS FXC contraction and the UKS EXC/VXC on the "
+                         "same workload")
     ap.add_argument("--tile-policy", default=None,
                     help="D,S,Q[,H[,W[,M]]]: fixed density / SYR2K 32-wide thresholds, "
                          "queue count, (mixed) density high-occupancy threshold, 128-wide "
@@ -355,9 +356,9 @@ def main():
 
 
 def cpu_derivatives(bs, P, ts, cfg, budget_s):
-    """The oracle's EXC gradient (with weight derivatives) and RKS FXC
-    contraction on bounded samples of the same tasks (points/s; its X
-    products through the host BLAS when one loads)."""
+    """The oracle's EXC gradient (with weight derivatives), RKS FXC
+    contraction and UKS EXC/VXC on bounded samples of the same tasks
+    (points/s; its X products through the host BLAS when one loads)."""
     import oracle
     nthreads = cpu_threads()
     blas = oracle.use_blas("auto")
@@ -370,7 +371,10 @@ def cpu_derivatives(bs, P, ts, cfg, budget_s):
                                                  nthreads=nthreads)),
                     ("fxc_contraction_pts_per_s",
                      lambda sub: oracle.fxc_contraction("PBE", cfg["nbf"], sh, P, tP, sub,
-                                                        nthreads=nthreads))):
+                                                        nthreads=nthreads)),
+                    ("uks_exc_vxc_pts_per_s",
+                     lambda sub: oracle.exc_vxc_uks("PBE", cfg["nbf"], sh, P, 0.05 * tP, sub,
+                                                    nthreads=nthreads))):
         idx, pts, elapsed = sample_rate(ts, fn, budget_s, nthreads)
         out[key] = pts / elapsed
         out[key.replace("_pts_per_s", "_sample")] = (f"{len(idx)} tasks, {pts} points, "
@@ -388,8 +392,8 @@ def workloads_density(nbf):
 
 def time_derivatives(ctx, cfg, steps):
     """Wall ms per call (device work, results downloaded) of the FP64 EXC
-    gradient with and without the grid-weight derivatives and of the RKS FXC
-    contraction, on the uploaded workload (PBE)."""
+    gradient with and without the grid-weight derivatives, of the RKS FXC
+    contraction and of the UKS EXC/VXC, on the uploaded workload (PBE)."""
     xyz = np.asarray(cfg["xyz"], dtype=np.float64)
     out = {}
     for key, wd in (("exc_grad_ms", True), ("exc_grad_hf_ms", False)):
@@ -408,6 +412,15 @@ def time_derivatives(ctx, cfg, steps):
         ctx.fxc_contraction("PBE")
         ctx.results_fxc()
     out["fxc_contraction_ms"] = (time.perf_counter() - tic) * 1e3 / steps
+    # UKS EXC/VXC with P_s = P and a small spin density P_z.
+    ctx.upload_density_z(0.05 * workloads_density(cfg["nbf"]))
+    ctx.exc_vxc_uks("PBE", True)
+    ctx.results_uks()
+    tic = time.perf_counter()
+    for _ in range(steps):
+        ctx.exc_vxc_uks("PBE", True)
+        ctx.results_uks()
+    out["uks_exc_vxc_ms"] = (time.perf_counter() - tic) * 1e3 / steps
     log(f"derivatives (FP64, ms per call): {out}")
     return out
 

@@ -1369,12 +1369,39 @@ int z_from_point_coeffs(xcq_context_s* ctx, const CtxView& v, const Chunk& c, in
                            XCQ_PREC_FP64, nullptr, gga, ctx->max_l);
 }
 
-// One UKS evaluation on the context's stream (FP64, stored basis planes):
-// per chunk, collocation with gradient planes; the density stage twice (P_s,
-// then P_z; factor 1) leaving raw per-point sums; the polarised functional
-// and Z coefficients per point; Z_s into the chunk's Z plane and its
-// B^T Z + Z^T B into VXC_s; then Z_z into VXC_z.  The VXC drop screening is
-// off (every contribution is added, as GauXC does).
+// Z from per-point coefficients, then its B^T Z + Z^T B scattered into
+// v.vxc.  In the in-stage mode with a drop tolerance the Z launch also
+// records the granule maxima of |B| and |Z| (the bound table, cleared
+// first), and the contraction skips negligible tile pairs and drops
+// negligible elements as the energy path does; on stored planes only the
+// element drop applies.
+int z_and_vxc(xcq_context_s* ctx, CtxView v, const Chunk& c, int mode, int gga,
+              const double* coef) {
+  int rc;
+  unsigned* tm = reinterpret_cast<unsigned*>(ctx->d_tmax);
+  v.vxc_drop = ctx->vxc_drop;
+  if (mode == 3 && v.vxc_drop > 0.0 && tm) {
+    if ((rc = launch_fill(ctx->stream, ctx->d_tmax, ctx->tmax_total)) ||
+        (rc = launch_z_in_stage(ctx->stream, v.work + c.coll.off, c.coll.n, v.tasks, v.shells,
+                                v.shell_list, v.lsh, v.px, v.py, v.pz, coef, v.scratch,
+                                XCQ_PREC_FP64, tm, gga, ctx->max_l)))
+      return rc;
+    v.tmax = tm;
+  } else {
+    if ((rc = z_from_point_coeffs(ctx, v, c, mode, gga, coef))) return rc;
+    v.tmax = nullptr;
+  }
+  for (int k = 0; k < 2; ++k)
+    if ((rc = launch_syr2k(v, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+  return XCQ_SUCCESS;
+}
+
+// One UKS evaluation on the context's stream (FP64; in-stage basis
+// evaluation up to f functions, stored planes above): per chunk,
+// collocation; the density stage twice (P_s, then P_z; factor 1) leaving
+// raw per-point sums; the polarised functional and Z coefficients per
+// point; Z_s into the chunk's Z plane and its B^T Z + Z^T B into VXC_s; then
+// Z_z into VXC_z, both with the energy path's drop screening (z_and_vxc).
 int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
   const int mode = derivative_mode(ctx);
@@ -1412,9 +1439,7 @@ int enqueue_uks(xcq_context_s* ctx, int func, int want_vxc) {
     for (int comp = 0; comp < 2; ++comp) {
       CtxView vc = v;
       vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
-      if ((rc = z_from_point_coeffs(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
-      for (int k = 0; k < 2; ++k)
-        if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+      if ((rc = z_and_vxc(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
     }
   }
   if (want_vxc) {
@@ -1485,12 +1510,13 @@ int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* 
 // ---- FXC contraction (GauXC's eval_fxc_contraction(P, tP)) ------------------
 namespace {
 
-// One RKS FXC contraction on the context's stream (FP64, stored basis
-// planes): per chunk, collocation with gradient planes; the density stage
-// twice (P, then the trial density; factor 2) leaving raw per-point sums;
-// the second-derivative kernel and Z coefficients per point; Z into the
-// chunk's Z plane and its B^T Z + Z^T B into d_vxc; symmetrised.  Every
-// contribution is added (no drop screening), as GauXC does.
+// One RKS FXC contraction on the context's stream (FP64; in-stage basis
+// evaluation up to f functions, stored planes above): per chunk,
+// collocation; the density stage twice (P, then the trial density; factor
+// 2) leaving raw per-point sums; the second-derivative kernel and Z
+// coefficients per point; Z into the chunk's Z plane and its B^T Z + Z^T B
+// into d_vxc with the energy path's drop screening (z_and_vxc);
+// symmetrised.
 int enqueue_fxc(xcq_context_s* ctx, int func) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
   const int mode = derivative_mode(ctx);
@@ -1521,9 +1547,7 @@ int enqueue_fxc(xcq_context_s* ctx, int func) {
                                  pass == 0 ? dp : dt)))
           return rc;
     if ((rc = launch_fxc_points(v, c.coll, func, gga, dp, dt, zc))) return rc;
-    if ((rc = z_from_point_coeffs(ctx, v, c, mode, gga, zc))) return rc;
-    for (int k = 0; k < 2; ++k)
-      if ((rc = launch_syr2k(v, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+    if ((rc = z_and_vxc(ctx, v, c, mode, gga, zc))) return rc;
   }
   if ((rc = launch_symmetrize(s, 1, ctx->max_nbf, ctx->d_mol_n, ctx->d_mol_off, ctx->d_vxc)))
     return rc;
@@ -1577,12 +1601,12 @@ int xcq_fxc_contraction(xcq_context ctx, int functional) {
 
 namespace {
 
-// One UKS FXC contraction on the context's stream (FP64, stored basis
-// planes): per chunk, collocation with gradient planes; the density stage
-// four times (P_s, P_z, tP_s, tP_z; factor 1) leaving raw per-point sums;
-// the derivatives of the UKS Z coefficients along the trial direction; Z_s
-// and its B^T Z + Z^T B into d_vxc (FXC_s), Z_z into d_vxcz (FXC_z);
-// symmetrised.  Every contribution is added.
+// One UKS FXC contraction on the context's stream (FP64; basis mode as
+// the RKS one): per chunk, collocation; the density stage four times (P_s,
+// P_z, tP_s, tP_z; factor 1) leaving raw per-point sums; the derivatives of
+// the UKS Z coefficients along the trial direction; Z_s and its B^T Z +
+// Z^T B into d_vxc (FXC_s), Z_z into d_vxcz (FXC_z), with the drop
+// screening (z_and_vxc); symmetrised.
 int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
   const int gga = (func == XCQ_FUNC_PBE || func == XCQ_FUNC_PBE0) ? 1 : 0;
   const int mode = derivative_mode(ctx);
@@ -1619,9 +1643,7 @@ int enqueue_fxc_uks(xcq_context_s* ctx, int func) {
     for (int comp = 0; comp < 2; ++comp) {
       CtxView vc = v;
       vc.vxc = comp == 0 ? ctx->d_vxc : ctx->d_vxcz;
-      if ((rc = z_from_point_coeffs(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
-      for (int k = 0; k < 2; ++k)
-        if ((rc = launch_syr2k(vc, c.syr2k[k], k == 0 ? 32 : 64, XCQ_PREC_FP64))) return rc;
+      if ((rc = z_and_vxc(ctx, vc, c, mode, gga, comp == 0 ? zs : zz))) return rc;
     }
   }
   for (double* m : {ctx->d_vxc, ctx->d_vxcz})
