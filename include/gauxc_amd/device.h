@@ -143,6 +143,7 @@ int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode);
 /* B^T Z contributions with |v| < tol are not scattered into VXC (each
  * scatter-add is an FP64 atomic; on large diffuse systems most are of
  * negligible function pairs).  Default 1e-18; 0 adds every contribution.
+ * Applies to the RKS and UKS EXC/VXC and to both FXC contractions.
  * Allowed range [0, 1e-10].  No GauXC counterpart (GauXC adds all). */
 int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol);
 /* Current settings: out5 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
@@ -231,8 +232,8 @@ int xcq_download_results_uks(xcq_context ctx, double* exc, double* nel, double* 
  *      include/gauxc/c/xc_integrator.h:351-362).  P through
  *      xcq_upload_density, the trial density tP through
  *      xcq_upload_trial_density (single-molecule contexts).  FXC =
- *      d VXC[P + e tP] / de is left in the context (FP64, stored basis planes,
- *      every contribution added) and read back with xcq_download_fxc, which
+ *      d VXC[P + e tP] / de is left in the context (FP64, with the VXC drop
+ *      tolerance of the energy path) and read back with xcq_download_fxc, which
  *      also returns the integrated density of P. ---- */
 int xcq_upload_trial_density(xcq_context ctx, int32_t nbf, const double* tP, int64_t ldp);
 int xcq_fxc_contraction(xcq_context ctx, int functional);
