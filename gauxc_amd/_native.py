@@ -122,6 +122,7 @@ DEVICE_SYMBOLS = [
     "xcq_upload_density_z", "xcq_exc_vxc_uks", "xcq_download_results_uks",
     "xcq_upload_trial_density", "xcq_fxc_contraction", "xcq_download_fxc",
     "xcq_exc_grad", "xcq_exc_grad_uks", "xcq_download_exc_grad", "xcq_collision_detection",
+    "xcq_set_ssf_neighbour_lists",
     "xcq_upload_trial_density_z", "xcq_fxc_contraction_uks", "xcq_download_fxc_uks",
     "xcq_download_weights", "xcq_exc_vxc",
     "xcq_integrate_den", "xcq_download_results", "xcq_task_collocation",
@@ -177,6 +178,7 @@ def device_lib():
             "xcq_download_fxc": (ctypes.c_int, [H, c_vp, c_vp, c_i64]),
             "xcq_exc_grad": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
             "xcq_exc_grad_uks": (ctypes.c_int, [H, ctypes.c_int, c_i32, c_vp, ctypes.c_int]),
+            "xcq_set_ssf_neighbour_lists": (ctypes.c_int, [H, c_i32]),
             "xcq_upload_trial_density_z": (ctypes.c_int, [H, c_i32, c_vp, c_i64]),
             "xcq_fxc_contraction_uks": (ctypes.c_int, [H, ctypes.c_int]),
             "xcq_download_fxc_uks": (ctypes.c_int, [H, c_vp, c_vp, c_vp, c_i64]),

@@ -508,6 +508,11 @@ class DeviceContext:
         N.xcq_check(fn(self.h, N.XCQ_FUNC[func], len(xyz), xyz.ctypes.data,
                        int(bool(weight_derivs))), "xcq_exc_grad")
 
+    def set_ssf_neighbour_lists(self, enable=True):
+        """Bounded (True) or all-atom loops in the gradient's weight part."""
+        N.xcq_check(self.lib.xcq_set_ssf_neighbour_lists(self.h, int(bool(enable))),
+                    "xcq_set_ssf_neighbour_lists")
+
     def results_grad(self):
         """(EXC, NEL, gradient (natoms, 3)) of the last EXC gradient."""
         exc = ctypes.c_double(0.)

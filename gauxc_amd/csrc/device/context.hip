@@ -207,6 +207,8 @@ struct xcq_context_s {
   double* d_dnear = nullptr;
   int32_t* d_gnbr_idx = nullptr;  // neighbour lists of the gradient's weight part
   double* d_gnbr_r = nullptr;
+  bool ssf_lists = true;          // xcq_set_ssf_neighbour_lists
+  bool grad_lists = false;        // the last gradient's weight part uses the lists
   double* d_grad = nullptr;
   int32_t* d_shell_atom = nullptr;
   bool last_grad = false;   // the last evaluation was an EXC gradient
@@ -1007,6 +1009,12 @@ int xcq_get_tile_policy(xcq_context ctx, int32_t* out5) {
   return XCQ_SUCCESS;
 }
 
+int xcq_set_ssf_neighbour_lists(xcq_context ctx, int32_t enable) {
+  XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
+  ctx->ssf_lists = enable != 0;
+  return XCQ_SUCCESS;
+}
+
 int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol) {
   XCQ_REQUIRE(ctx, XCQ_ERROR_INVALID_ARGUMENT, "null context");
   XCQ_REQUIRE(tol >= 0.0 && tol <= 1e-10, XCQ_ERROR_INVALID_ARGUMENT,
@@ -1791,9 +1799,10 @@ int enqueue_grad(xcq_context_s* ctx, int func, int weight_derivs) {
     if ((rc = launch_grad_contract(v, c.coll, gga, ctx->d_shell_atom, coef, weight_derivs,
                                    ctx->d_grad, x_pl, 2, ctx->max_l)))
       return rc;
-    if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
-                                               ctx->d_dnear, wf, ctx->d_grad,
-                                               ctx->d_gnbr_idx, ctx->d_gnbr_r)))
+    if (weight_derivs &&
+        (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz, ctx->d_dnear, wf,
+                              ctx->d_grad, ctx->grad_lists ? ctx->d_gnbr_idx : nullptr,
+                              ctx->grad_lists ? ctx->d_gnbr_r : nullptr)))
       return rc;
   }
   return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
@@ -1860,9 +1869,10 @@ int enqueue_grad_uks(xcq_context_s* ctx, int func, int weight_derivs) {
     if ((rc = launch_grad_contract_uks(v, c.coll, gga, ctx->d_shell_atom, ds, dz, weight_derivs,
                                        ctx->d_grad, ctx->max_l)))
       return rc;
-    if (weight_derivs && (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz,
-                                               ctx->d_dnear, wf, ctx->d_grad,
-                                               ctx->d_gnbr_idx, ctx->d_gnbr_r)))
+    if (weight_derivs &&
+        (rc = launch_ssf_grad(v, c.coll, ctx->grad_natoms, ctx->d_gxyz, ctx->d_dnear, wf,
+                              ctx->d_grad, ctx->grad_lists ? ctx->d_gnbr_idx : nullptr,
+                              ctx->grad_lists ? ctx->d_gnbr_r : nullptr)))
       return rc;
   }
   return launch_reduce_sums(s, 1, ctx->d_mol_blk, ctx->d_red, ctx->d_rpart, nullptr, 1.0,
@@ -1901,7 +1911,7 @@ int grad_prepare(xcq_context_s* ctx, int functional, int32_t natoms, const doubl
   std::vector<double> rab, dn, hnr;
   std::vector<int32_t> hni;
   ssf_molmeta(natoms, xyz, rab, dn);
-  const bool lists = natoms <= kSsfNbrMaxAtoms;
+  const bool lists = ctx->ssf_lists && natoms <= kSsfNbrMaxAtoms;
   if (lists) ssf_neighbours(natoms, rab, hni, hnr);
   int rc;
   if (natoms != ctx->grad_natoms) {
@@ -1915,11 +1925,12 @@ int grad_prepare(xcq_context_s* ctx, int functional, int32_t natoms, const doubl
         (rc = dalloc(&ctx->d_dnear, (size_t)natoms)) ||
         (rc = dalloc(&ctx->d_grad, 3 * (size_t)natoms)))
       return rc;
-    if (lists && ((rc = dalloc(&ctx->d_gnbr_idx, hni.size())) ||
-                  (rc = dalloc(&ctx->d_gnbr_r, hnr.size()))))
-      return rc;
     ctx->grad_natoms = natoms;
   }
+  if (lists && !ctx->d_gnbr_idx &&
+      ((rc = dalloc(&ctx->d_gnbr_idx, hni.size())) || (rc = dalloc(&ctx->d_gnbr_r, hnr.size()))))
+    return rc;
+  ctx->grad_lists = lists;
   dfree(ctx->d_shell_atom);
   if ((rc = dalloc(&ctx->d_shell_atom, std::max<size_t>(nsh, 1)))) return rc;
   XCQ_HIP(hipMemcpy(ctx->d_gxyz, xyz, 24 * (size_t)natoms, hipMemcpyHostToDevice));

@@ -146,6 +146,11 @@ int xcq_get_density_basis_eval(xcq_context ctx, int32_t* mode);
  * Applies to the RKS and UKS EXC/VXC and to both FXC contractions.
  * Allowed range [0, 1e-10].  No GauXC counterpart (GauXC adds all). */
 int xcq_set_vxc_drop_tolerance(xcq_context ctx, double tol);
+/* Whether the EXC gradient's grid-weight part bounds its loops with
+ * R_AB-sorted neighbour lists (default 1; they are built up to 8,192
+ * atoms) or loops over every atom (0).  Both give the same gradient to
+ * rounding; the switch exists to exercise both.  No GauXC counterpart. */
+int xcq_set_ssf_neighbour_lists(xcq_context ctx, int32_t enable);
 /* Current settings: out5 = {dens32_max_nbe, syr2k32_max_nbe, hiocc_max_nbe,
  * nqueues, wide_min_nbe}. */
 int xcq_get_tile_policy(xcq_context ctx, int32_t* out5);

@@ -79,6 +79,16 @@ def test_exc_grad_water_cluster_matches_oracle(gx, orc, gpu, func):
     exc, nel, g3 = ctx.results_grad()
     assert abs(exc - exc_o) <= 1e-10 and abs(nel - nel_o) <= 1e-10
     assert np.abs(g3 - g).max() <= GRAD_BAR  # g: the Hellmann-Feynman form above
+    # The weight part's all-atom loops (used above 8,192 atoms) agree with
+    # the bounded ones.
+    ctx.exc_grad(func, xyz, weight_derivs=True)
+    g_l = ctx.results_grad()[2]
+    ctx.set_ssf_neighbour_lists(False)
+    ctx.exc_grad(func, xyz, weight_derivs=True)
+    g_a = ctx.results_grad()[2]
+    ctx.set_ssf_neighbour_lists(True)
+    _flush(f"lists vs all atoms max|dG|={np.abs(g_l - g_a).max():.3e}")
+    assert np.abs(g_l - g_a).max() <= 1e-13
     ctx.exc_vxc(func, "fp64", True)
     with pytest.raises(gx.GauXCError, match="not an EXC gradient"):
         ctx.results_grad()
