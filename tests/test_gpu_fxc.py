@@ -168,8 +168,9 @@ def test_fxc_uks_dimer_matches_oracle(gx, orc, gpu, func):
         rks.eval_fxc_contraction(Ps, tPs, Pz, tPz)
 
 
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
 @pytest.mark.parametrize("lmax", [3, 4])
-def test_derivative_paths_high_l_match_oracle(gx, orc, gpu, lmax):
+def test_derivative_paths_high_l_match_oracle(gx, orc, gpu, lmax, func):
     """RKS and UKS FXC contractions and the UKS EXC/VXC on a two-atom basis
     with f (and g) shells, spherical and Cartesian, and a six-primitive s
     shell: up to f the density stage evaluates the basis itself and Z comes
@@ -204,23 +205,23 @@ def test_derivative_paths_high_l_match_oracle(gx, orc, gpu, lmax):
     w = orc.ssf_weights(xyz, t)
     sh = orc.shells_from(bs.xcq_shells())
     gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
-    rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional("PBE"), lb)
+    rks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional(func), lb)
     uks = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
-        gx.Functional("PBE", polarized=True), lb)
+        gx.Functional(func, polarized=True), lb)
     _flush(f"FXC lmax={lmax}")
     f = rks.eval_fxc_contraction(pa, ta)
-    f_o = orc.fxc_contraction("PBE", nbf, sh, pa, ta, t, weights=w)[1]
+    f_o = orc.fxc_contraction(func, nbf, sh, pa, ta, t, weights=w)[1]
     _flush(f"max|dFXC|={np.abs(f - f_o).max():.3e}")
     assert np.abs(f - f_o).max() <= FP64_VXC
     _flush(f"UKS FXC lmax={lmax}")
     fs, fz = uks.eval_fxc_contraction(pa + pb, ta + tb, pa - pb, ta - tb)
-    fs_o, fz_o = orc.fxc_contraction_uks("PBE", nbf, sh, pa + pb, pa - pb, ta + tb, ta - tb, t,
+    fs_o, fz_o = orc.fxc_contraction_uks(func, nbf, sh, pa + pb, pa - pb, ta + tb, ta - tb, t,
                                          weights=w)
     _flush(f"max|dFXC_s|={np.abs(fs - fs_o).max():.3e} max|dFXC_z|={np.abs(fz - fz_o).max():.3e}")
     assert np.abs(fs - fs_o).max() <= FP64_VXC and np.abs(fz - fz_o).max() <= FP64_VXC
     _flush(f"UKS EXC/VXC lmax={lmax}")
     exc, vs, vz = uks.eval_exc_vxc(pa + pb, pa - pb)
-    exc_o, _, vs_o, vz_o = orc.exc_vxc_uks("PBE", nbf, sh, pa + pb, pa - pb, t, weights=w)[:4]
+    exc_o, _, vs_o, vz_o = orc.exc_vxc_uks(func, nbf, sh, pa + pb, pa - pb, t, weights=w)[:4]
     _flush(f"dEXC={exc - exc_o:.3e} max|dVs|={np.abs(vs - vs_o).max():.3e} "
            f"max|dVz|={np.abs(vz - vz_o).max():.3e}")
     assert abs(exc - exc_o) <= 1e-10

@@ -137,8 +137,9 @@ def test_uks_exc_grad_water_cluster_matches_oracle(gx, orc, gpu, func):
     assert np.abs(g3 - orc.exc_grad_uks(func, nbf, sh, xyz, Ps, Pz, t, weights=w)).max() <= GRAD_BAR
 
 
+@pytest.mark.parametrize("func", ["SVWN5", "PBE"])
 @pytest.mark.parametrize("lmax", [3, 4])
-def test_exc_grad_high_l_matches_oracle(gx, orc, gpu, lmax):
+def test_exc_grad_high_l_matches_oracle(gx, orc, gpu, lmax, func):
     """Basis with f and g functions (spherical and Cartesian) and a
     six-primitive shell: the Hessian-vector products of every l."""
     Z = [8, 1]
@@ -163,19 +164,19 @@ def test_exc_grad_high_l_matches_oracle(gx, orc, gpu, lmax):
     lb = gx.workloads.build_tasks(mol, bs, grid="GM3")
     t = lb.get_tasks()
     w = orc.ssf_weights(xyz, t)
-    g_o = orc.exc_grad("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P, t, weights=w)
+    g_o = orc.exc_grad(func, bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P, t, weights=w)
     gx.MolecularWeightsFactory("Device", "Default").get_instance().modify_weights(lb)
-    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional("PBE"), lb)
-    _flush(f"EXC gradient lmax={lmax}")
+    integ = gx.XCIntegratorFactory("Device", "Replicated").get_instance(gx.Functional(func), lb)
+    _flush(f"EXC gradient lmax={lmax} {func}")
     g = integ.eval_exc_grad(P)
     _flush(f"max|dG|={np.abs(g - g_o).max():.3e} (max|G|={np.abs(g_o).max():.3e})")
     assert np.abs(g - g_o).max() <= GRAD_BAR
     # UKS: the full Hessian per function (two coefficient vectors).
     Pb = gx.workloads.density(bs.nbf(), 3, 8)
-    gu_o = orc.exc_grad_uks("PBE", bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P + Pb,
+    gu_o = orc.exc_grad_uks(func, bs.nbf(), orc.shells_from(bs.xcq_shells()), xyz, P + Pb,
                             P - Pb, t, weights=w)
     integ_u = gx.XCIntegratorFactory("Device", "Replicated").get_instance(
-        gx.Functional("PBE", polarized=True), lb)
+        gx.Functional(func, polarized=True), lb)
     _flush(f"UKS EXC gradient lmax={lmax}")
     gu = integ_u.eval_exc_grad(P + Pb, P - Pb)
     _flush(f"max|dG|={np.abs(gu - gu_o).max():.3e}")
