@@ -108,7 +108,11 @@ __device__ __forceinline__ void fold_tile(const double (&x)[BI][2][4], FRow frow
         // clamped B values they read contribute nothing.
         const double xv = x[bi][bj][r];
         double b = (double)v0[r];
-        if constexpr (sizeof(T) == 4) b += bf16_val(l0[r]);
+        // FP32 storage: X was formed from the FP32 part B32 of B alone, so
+        // rho = B^T 2P B misses res^T X (res = B - B32, P symmetric), a
+        // first-order term that does not average out (it biased EXC by
+        // about -3e-10 relative): fold B32 + 2 res.
+        if constexpr (sizeof(T) == 4) b += 2.0 * bf16_val(l0[r]);
         s0 += b * xv;
         if constexpr (GGA) {
           s1 += (double)v1[r] * xv;
@@ -188,8 +192,9 @@ __device__ __forceinline__ void fold_tile_lds(const double (&x)[BI][2][4], doubl
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const unsigned w = e < 2 ? l.x : l.y;
-      const double bv = (double)bb[e] +
-                        (double)__builtin_bit_cast(float, (e & 1) ? (w & 0xffff0000u) : (w << 16));
+      // B32 + 2 res: X lacks the residual (see fold_tile).
+      const double bv = (double)bb[e] + 2.0 * (double)__builtin_bit_cast(
+                                                  float, (e & 1) ? (w & 0xffff0000u) : (w << 16));
       const double xv = xr[e];
       s[e][0] += bv * xv;
       if constexpr (GGA) {
@@ -477,7 +482,10 @@ __device__ __forceinline__ int for_range_shells(const DevTask& t, const ShellArg
 // R sy to rho and R sg + dR r sy to grad rho / 2, where sy = sum Y_f X_f
 // and sg = sum grad Y_f X_f over its functions in the range: the functions
 // are formed from their angular factors alone.
-template <bool GGA>
+// RES32: X was formed from the FP32-rounded value plane (mixed precision);
+// rho then also takes sum (B - fl32(B)) X, the first-order term the X
+// operand misses (see fold_tile).
+template <bool GGA, bool RES32 = false>
 __device__ __forceinline__ int fold_in_stage(const DevTask& t, const ShellArgs& sa, int s_first,
                                              int nlo, int nhi, int wave, int lane,
                                              const double* xt, double rx, double ry, double rz,
@@ -487,13 +495,17 @@ __device__ __forceinline__ int fold_in_stage(const DevTask& t, const ShellArgs& 
     const bool pure = (u.info >> 8) & 1;
     double x, y, z, R, dR;
     shell_radial(sa, u, rx, ry, rz, x, y, z, R, dR);
-    double sy = 0., sgx = 0., sgy = 0., sgz = 0.;
+    double sy = 0., sgx = 0., sgy = 0., sgz = 0., sres = 0.;
     shell_functions_l<kInStageMaxL, GGA, true, true>(
         l, x, y, z, R, dR, pure, [&](int f, double v, double gx, double gy, double gz) {
           const int n = u.bf0 + f;  // uniform
           if (n >= nlo && n < nhi) {
             const double xv = xt[(n - nlo) * kXLd + lane];
             sy += v * xv;
+            if constexpr (RES32) {
+              const double b = R * v;
+              sres += (b - (double)(float)b) * xv;
+            }
             if constexpr (GGA) {
               sgx += gx * xv;
               sgy += gy * xv;
@@ -502,6 +514,7 @@ __device__ __forceinline__ int fold_in_stage(const DevTask& t, const ShellArgs& 
           }
         });
     r[0] += R * sy;
+    if constexpr (RES32) r[0] += sres;
     if constexpr (GGA) {
       const double d = dR * sy;
       r[1] += R * sgx + d * x;
@@ -1022,7 +1035,7 @@ __global__ __launch_bounds__(256, density_waves(NT, HIOCC)) void k_density_mx(
         for (int h = 0; h < (NT + 63) / 64; ++h) {
           const int nlo = n0 + 64 * h;
           if (nlo >= nbe) break;
-          s_cur = fold_in_stage<GGA>(t, sa, s_cur, nlo, min(nlo + min(NT, 64), nbe), wave,
+          s_cur = fold_in_stage<GGA, true>(t, sa, s_cur, nlo, min(nlo + min(NT, 64), nbe), wave,
                                      lane, xt + 64 * h * kXLd, rx, ry, rz, part);
         }
       } else if constexpr (NT <= 64) {
