@@ -41,16 +41,20 @@ __device__ __forceinline__ unsigned short bf_bits(__bf16 v) {
   return __builtin_bit_cast(unsigned short, v);
 }
 
-__device__ __forceinline__ void split3f(float x, unsigned short& a, unsigned short& b,
-                                        unsigned short& c) {
-  const __bf16 h1 = (__bf16)x;
-  const float r1 = x - (float)h1;
-  const __bf16 h2 = (__bf16)r1;
-  const float r2 = r1 - (float)h2;
-  const __bf16 h3 = (__bf16)r2;
-  a = bf_bits(h1);
-  b = bf_bits(h2);
-  c = bf_bits(h3);
+// Splits two FP32 values into three bfloat16 slices each (round to nearest
+// even at every step, so x = a + b + c exactly); the slices of the pair come
+// packed, one v_cvt_pk_bf16_f32 per slice.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split3f_pair(f2v x, unsigned& a, unsigned& b, unsigned& c) {
+  const bf2v h1 = __builtin_convertvector(x, bf2v);
+  const f2v r1 = x - __builtin_convertvector(h1, f2v);
+  const bf2v h2 = __builtin_convertvector(r1, bf2v);
+  const f2v r2 = r1 - __builtin_convertvector(h2, f2v);
+  const bf2v h3 = __builtin_convertvector(r2, bf2v);
+  a = __builtin_bit_cast(unsigned, h1);
+  b = __builtin_bit_cast(unsigned, h2);
+  c = __builtin_bit_cast(unsigned, h3);
 }
 
 __device__ __forceinline__ bf16x8 ldfrag(const unsigned short* p) {
@@ -167,18 +171,16 @@ __global__ __launch_bounds__(256, NT == 32 ? 4 : 2) void k_syr2k_mx(QueueArgs qa
     };
     auto stage = [&](const float4 (&v)[2], bool ok, unsigned short (*dst)[NT][kRow]) {
       const float* pv = reinterpret_cast<const float*>(&v[0]);  // 8 consecutive points
-      us8 s1, s2, s3;
+      uint4 s1, s2, s3;
+      unsigned* w1 = reinterpret_cast<unsigned*>(&s1);
+      unsigned* w2 = reinterpret_cast<unsigned*>(&s2);
+      unsigned* w3 = reinterpret_cast<unsigned*>(&s3);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        unsigned short a, b, c;
-        split3f(ok ? pv[e] : 0.f, a, b, c);
-        s1[e] = a;
-        s2[e] = b;
-        s3[e] = c;
-      }
-      *reinterpret_cast<us8*>(&dst[0][frow][pg * 8]) = s1;
-      *reinterpret_cast<us8*>(&dst[1][frow][pg * 8]) = s2;
-      *reinterpret_cast<us8*>(&dst[2][frow][pg * 8]) = s3;
+      for (int e = 0; e < 4; ++e)
+        split3f_pair(f2v{ok ? pv[2 * e] : 0.f, ok ? pv[2 * e + 1] : 0.f}, w1[e], w2[e], w3[e]);
+      *reinterpret_cast<uint4*>(&dst[0][frow][pg * 8]) = s1;
+      *reinterpret_cast<uint4*>(&dst[1][frow][pg * 8]) = s2;
+      *reinterpret_cast<uint4*>(&dst[2][frow][pg * 8]) = s3;
     };
 
     auto stage_lo = [&](const uint4& l, bool ok, unsigned short (*dst)[NT][kRow]) {
