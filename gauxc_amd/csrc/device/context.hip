@@ -1241,11 +1241,37 @@ int xcq_autotune(xcq_context ctx, int functional, int precision, int32_t* dens32
       }
       if (p.ms < best.ms) best = p;
     }
-    best_d = best.d;
+    // A second pass over the 32-wide and the occupancy thresholds in the
+    // winning mode, the other settings at their chosen values: the first
+    // pass picked the 32-wide threshold with every 64-wide task at two
+    // workgroups per CU, and the two thresholds interact.
     if ((rc = xcq_set_density_basis_eval(ctx, best.mode))) return rc;
-    if ((rc = xcq_set_tile_policy(ctx, best.d, best_s))) return rc;
     if ((rc = xcq_set_density_occupancy(ctx, best.h))) return rc;
     if ((rc = xcq_set_density_wide(ctx, best.w))) return rc;
+    for (int32_t c : cand) {
+      if ((rc = xcq_set_tile_policy(ctx, c, best_s))) return rc;
+      double dm, sm;
+      if ((rc = measure(&dm, &sm, &tot))) return rc;
+      if (dm < best.ms) {
+        best.ms = dm;
+        best.d = c;
+      }
+    }
+    if ((rc = xcq_set_tile_policy(ctx, best.d, best_s))) return rc;
+    if ((mixed && best.mode != 1) || (!mixed && best.mode == 3)) {
+      for (int32_t c : cand) {
+        if ((rc = xcq_set_density_occupancy(ctx, c))) return rc;
+        double dm, sm;
+        if ((rc = measure(&dm, &sm, &tot))) return rc;
+        if (dm < best.ms) {
+          best.ms = dm;
+          best.h = c;
+        }
+      }
+    }
+    best_d = best.d;
+    if ((rc = xcq_set_tile_policy(ctx, best.d, best_s))) return rc;
+    if ((rc = xcq_set_density_occupancy(ctx, best.h))) return rc;
   }
   // Persistent workgroups on dynamic queues (at least as many as any stage
   // kernel keeps resident per CU; surplus ones find the queues empty)
